@@ -1,0 +1,268 @@
+"""ctypes binding of the C oracle (oracle/crt_oracle.c).  Test infrastructure only."""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB = HERE / "liboracle.so"
+
+_i = np.ctypeslib.ndpointer(np.int32, flags="C")
+_d = np.ctypeslib.ndpointer(np.float64, flags="C")
+_z = np.ctypeslib.ndpointer(np.complex128, flags="C")
+
+
+class OrcMesh(C.Structure):
+    _fields_ = [("sanz", C.c_int), ("typanz", C.c_int), ("mb", C.c_int), ("eanz", C.c_int),
+                ("typ", C.c_void_p), ("nelanz", C.c_void_p), ("selanz", C.c_void_p),
+                ("nrel", C.c_void_p), ("rnr", C.c_void_p), ("snr", C.c_void_p),
+                ("sx", C.c_void_p), ("sy", C.c_void_p), ("enr", C.c_void_p),
+                ("lsytop", C.c_int), ("lsink", C.c_int), ("nsink", C.c_int)]
+
+
+def build(force=False):
+    if force or not LIB.exists() or LIB.stat().st_mtime < (HERE / "crt_oracle.c").stat().st_mtime:
+        subprocess.run(["make", "-C", str(HERE), "liboracle.so"], check=True,
+                       stdout=subprocess.DEVNULL)
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not LIB.exists():
+            build()
+        L = C.CDLL(str(LIB))
+        P = C.POINTER(OrcMesh)
+        L.orc_rwaven.argtypes = [P, C.c_int, _i, _i, C.POINTER(C.c_int), _d, _d,
+                                 C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        L.orc_precal.argtypes = [P, C.c_int, _d, _d, _d, _d, C.POINTER(C.c_int),
+                                 C.POINTER(C.c_int), C.POINTER(C.c_double)]
+        L.orc_assemble.argtypes = [P, C.c_int, _d, _d, _d, C.c_int, _z, C.c_int, C.c_int,
+                                   _z, _z, _d]
+        L.orc_chol.argtypes = [C.c_int, C.c_int, _z]
+        L.orc_vre.argtypes = [C.c_int, C.c_int, _z, _z, _z]
+        L.orc_vre.restype = None
+        L.orc_forward.argtypes = [P, C.c_int, _d, C.c_int, _d, _d, _d, C.c_int, C.c_int, _z,
+                                  _z, _z, C.c_int, C.c_int, C.c_int]
+        L.orc_volt.argtypes = [P, C.c_int, _i, _i, _z, C.c_int, C.c_void_p, _z, _z]
+        L.orc_sens.argtypes = [P, C.c_int, _i, _i, C.c_int, _d, C.c_int, _d, _z, C.c_int, _z,
+                               _z, _z, C.c_int]
+        L.orc_update_rhs.argtypes = [C.c_int, C.c_int, _z, _d, _i, _z, _z, C.c_double, C.c_int,
+                                     _d, C.c_void_p, _d, _z]
+        L.orc_update_rhs.restype = None
+        L.orc_cjggra.argtypes = [C.c_int, C.c_int, _z, _d, _i, _d, _z, C.c_double, C.c_int,
+                                 C.c_int, _d, C.c_int, C.c_void_p, C.c_int, C.c_int, C.c_double,
+                                 _z, _d]
+        L.orc_bsmatmreg.argtypes = [P, C.c_int, C.c_int, C.c_double, C.c_double, _d]
+        L.orc_bsmatmreg.restype = None
+        L.orc_rm_reg.argtypes = [C.c_int, C.c_int, _d, _z, _z]
+        L.orc_rm_reg.restype = None
+        L.orc_adjoint_grad.argtypes = [C.c_int, C.c_int, _z, _d, _i, _z, _z]
+        L.orc_adjoint_grad.restype = None
+        _lib = L
+    return _lib
+
+
+class OracleError(RuntimeError):
+    def __init__(self, where, code):
+        super().__init__(f"{where}: reference error code {code}")
+        self.code = code
+
+
+class Oracle:
+    """Oracle state for one mesh: mirrors the reference's module arrays."""
+
+    def __init__(self, mesh, lsytop=False, lsink=False, nsink=0):
+        self.mesh = mesh
+        self._keep = dict(
+            typ=np.ascontiguousarray(mesh.typ, np.int32),
+            nelanz=np.ascontiguousarray(mesh.nelanz, np.int32),
+            selanz=np.ascontiguousarray(mesh.selanz, np.int32),
+            nrel=np.asfortranarray(mesh.nrel, np.int32).ravel(order="F").copy(),
+            rnr=np.ascontiguousarray(mesh.rnr if len(mesh.rnr) else np.zeros(1), np.int32),
+            snr=np.ascontiguousarray(mesh.snr, np.int32),
+            sx=np.ascontiguousarray(mesh.sx, np.float64),
+            sy=np.ascontiguousarray(mesh.sy, np.float64),
+            enr=np.ascontiguousarray(mesh.enr, np.int32),
+        )
+        k = self._keep
+        self.cm = OrcMesh(mesh.sanz, mesh.typanz, mesh.mb, mesh.eanz,
+                          *(k[n].ctypes.data for n in ("typ", "nelanz", "selanz", "nrel", "rnr",
+                                                       "snr", "sx", "sy", "enr")),
+                          int(lsytop), int(lsink), int(nsink))
+        self.elanz = mesh.elanz
+        self.relanz = mesh.relanz
+        self.nimn = mesh.maxnodes * (mesh.maxnodes + 1) // 2
+        self.swrtr = 1
+
+    # --- A1 -----------------------------------------------------------------
+    def rwaven(self, strnr, vnr):
+        K = C.c_int()
+        kwn = np.zeros(30)
+        kwnwi = np.zeros(30)
+        amin = C.c_double()
+        amax = C.c_double()
+        e = lib().orc_rwaven(C.byref(self.cm), len(strnr), np.ascontiguousarray(strnr, np.int32),
+                             np.ascontiguousarray(vnr, np.int32), C.byref(K), kwn, kwnwi,
+                             C.byref(amin), C.byref(amax))
+        if e:
+            raise OracleError("rwaven", e)
+        self.K = K.value
+        self.kwn = kwn[:self.K].copy()
+        self.kwnwi = kwnwi[:self.K].copy()
+        self.amin, self.amax = amin.value, amax.value
+        return self.kwn, self.kwnwi
+
+    def set_waven(self, kwn, kwnwi, swrtr=1):
+        self.kwn = np.ascontiguousarray(kwn, np.float64)
+        self.kwnwi = np.ascontiguousarray(kwnwi, np.float64)
+        self.K = len(self.kwn)
+        self.swrtr = swrtr
+
+    # --- A2/A3 --------------------------------------------------------------
+    def precal(self):
+        K = self.K
+        self.elbg = np.zeros(self.elanz * self.nimn * K)
+        self.relbg = np.zeros(max(self.relanz, 1) * self.nimn)
+        self.kg = np.zeros(max(self.relanz, 1) * self.mesh.eanz * K)
+        lb, lr, st = C.c_int(), C.c_int(), C.c_double()
+        e = lib().orc_precal(C.byref(self.cm), K, self.kwn, self.elbg, self.relbg, self.kg,
+                             C.byref(lb), C.byref(lr), C.byref(st))
+        if e:
+            raise OracleError("precal", e)
+        self.lbeta, self.lrandb, self.sytop = lb.value, lr.value, st.value
+        return self.elbg, self.relbg, self.kg
+
+    def assemble(self, sigma, l, k):
+        n, mb = self.mesh.sanz, self.mesh.mb
+        a = np.zeros((mb + 1) * n, np.complex128)
+        b = np.zeros(n, np.complex128)
+        fak = np.zeros(n)
+        e = lib().orc_assemble(C.byref(self.cm), self.K, self.elbg, self.relbg, self.kg,
+                               self.lrandb, np.ascontiguousarray(sigma, np.complex128), l, k, a,
+                               b, fak)
+        if e:
+            raise OracleError("assemble", e)
+        return a, b, fak
+
+    def chol(self, a):
+        a = a.copy()
+        e = lib().orc_chol(self.mesh.sanz, self.mesh.mb, a)
+        if e:
+            raise OracleError("chol", e)
+        return a
+
+    def vre(self, a, b):
+        pot = np.zeros(self.mesh.sanz, np.complex128)
+        lib().orc_vre(self.mesh.sanz, self.mesh.mb, a, np.ascontiguousarray(b), pot)
+        return pot
+
+    # --- A4..A11 ------------------------------------------------------------
+    def forward(self, sigma, nthreads=0, sys_range=None):
+        n, E, K = self.mesh.sanz, self.mesh.eanz, self.K
+        kpot = np.zeros(n * E * K, np.complex128)
+        hpot = np.zeros(n * E, np.complex128)
+        b, e_ = (0, 0) if sys_range is None else sys_range
+        e = lib().orc_forward(C.byref(self.cm), K, self.kwnwi, self.swrtr, self.elbg, self.relbg,
+                              self.kg, self.lbeta, self.lrandb,
+                              np.ascontiguousarray(sigma, np.complex128), kpot, hpot, nthreads,
+                              b, e_)
+        if e:
+            raise OracleError("forward", e)
+        self.kpot = kpot.reshape((n, E, K), order="F")
+        self.hpot = hpot.reshape((n, E), order="F")
+        self.sigma = np.ascontiguousarray(sigma, np.complex128)
+        return self.kpot, self.hpot
+
+    # --- A12 ----------------------------------------------------------------
+    def volt(self, strnr, vnr, mode=0):
+        N = len(strnr)
+        volt = np.zeros(N, np.complex128)
+        sigmaa = np.zeros(N, np.complex128)
+        hp = np.asfortranarray(self.hpot).ravel(order="F")
+        e = lib().orc_volt(C.byref(self.cm), N, np.ascontiguousarray(strnr, np.int32),
+                           np.ascontiguousarray(vnr, np.int32), hp, mode, None, volt, sigmaa)
+        if e:
+            raise OracleError("volt", e)
+        self.strnr = np.ascontiguousarray(strnr, np.int32)
+        self.vnr = np.ascontiguousarray(vnr, np.int32)
+        self.voltv = volt
+        return volt, sigmaa
+
+    # --- A14 ----------------------------------------------------------------
+    def sens(self, mode=0, nthreads=0):
+        N = len(self.strnr)
+        out = np.zeros(N * self.elanz, np.complex128)
+        kp = np.asfortranarray(self.kpot).ravel(order="F")
+        e = lib().orc_sens(C.byref(self.cm), N, self.strnr, self.vnr, self.K, self.kwnwi,
+                           self.swrtr, self.elbg, kp, mode, self.sigma, self.voltv, out, nthreads)
+        if e:
+            raise OracleError("sens", e)
+        return out.reshape((N, self.elanz), order="F")
+
+
+def bsmatmreg(orc: Oracle, nx, nz, alfx, alfz):
+    sm = np.zeros(orc.elanz * 3)
+    lib().orc_bsmatmreg(C.byref(orc.cm), nx, nz, alfx, alfz, sm)
+    return sm.reshape((orc.elanz, 3), order="F")
+
+
+def rm_reg(smatm, nx, par):
+    manz = smatm.shape[0]
+    out = np.zeros(manz, np.complex128)
+    lib().orc_rm_reg(manz, nx, np.asfortranarray(smatm).ravel(order="F"),
+                     np.ascontiguousarray(par, np.complex128), out)
+    return out
+
+
+def update_rhs(sens, wmatd, wdfak, dat, sigmaa, lam, ltri, rdiag, rm=None):
+    N, M = sens.shape
+    cgfac = np.zeros(M)
+    bvec = np.zeros(M, np.complex128)
+    rm_arr = None if rm is None else np.ascontiguousarray(rm, np.complex128)
+    lib().orc_update_rhs(N, M, np.asfortranarray(sens).ravel(order="F"),
+                         np.ascontiguousarray(wmatd, np.float64),
+                         np.ascontiguousarray(wdfak, np.int32),
+                         np.ascontiguousarray(dat, np.complex128),
+                         np.ascontiguousarray(sigmaa, np.complex128), lam, ltri,
+                         np.ascontiguousarray(rdiag, np.float64),
+                         None if rm_arr is None else rm_arr.ctypes.data, cgfac, bvec)
+    return cgfac, bvec
+
+
+def cjggra(sens, wmatd, wdfak, cgfac, bvec, lam, ltri, nx, smatm, ncgmax, eps, nachbar=None,
+           maxnodes=0):
+    N, M = sens.shape
+    dpar = np.zeros(M, np.complex128)
+    cgres = np.zeros(ncgmax + 2)
+    sm = np.asfortranarray(smatm)
+    nb = None if nachbar is None else np.asfortranarray(nachbar, np.int32)
+    ncg = lib().orc_cjggra(N, M, np.asfortranarray(sens).ravel(order="F"),
+                           np.ascontiguousarray(wmatd, np.float64),
+                           np.ascontiguousarray(wdfak, np.int32),
+                           np.ascontiguousarray(cgfac, np.float64),
+                           np.ascontiguousarray(bvec, np.complex128), lam, ltri, nx,
+                           sm.ravel(order="F"), sm.shape[0], None if nb is None else nb.ctypes.data,
+                           maxnodes, ncgmax, eps, dpar, cgres)
+    return dpar, cgres, ncg
+
+
+def adjoint_grad(sens, wmatd, wdfak, resid):
+    N, M = sens.shape
+    g = np.zeros(M, np.complex128)
+    lib().orc_adjoint_grad(N, M, np.asfortranarray(sens).ravel(order="F"),
+                           np.ascontiguousarray(wmatd, np.float64),
+                           np.ascontiguousarray(wdfak, np.int32),
+                           np.ascontiguousarray(resid, np.complex128), g)
+    return g
+
+
+def nthreads_default():
+    return int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))

@@ -1,0 +1,155 @@
+"""ctypes binding of the reference harness (oracle/_ref/libcrtref.so).
+
+TEST INFRASTRUCTURE ONLY.  The library holds the reference's own Fortran routines
+compiled from its source by oracle/Makefile plus oracle/ref_driver.f90.  It is
+module-global state (like the reference program), so one mesh at a time.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB = HERE / "_ref" / "libcrtref.so"
+
+_i = np.ctypeslib.ndpointer(np.int32, flags="C")
+_d = np.ctypeslib.ndpointer(np.float64, flags="C")
+_z = np.ctypeslib.ndpointer(np.complex128, flags="C")
+
+_lib = None
+
+
+def available() -> bool:
+    return LIB.exists()
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        L = C.CDLL(str(LIB))
+        L.ref_load_mesh.argtypes = [C.c_int, C.c_int, C.c_int, _i, _i, _i, _i, _i, _i, _d, _d,
+                                    C.c_int, _i, C.c_int, C.c_int, C.c_int]
+        L.ref_load_data.argtypes = [C.c_int, _i, _i]
+        L.ref_rwaven.argtypes = [C.POINTER(C.c_int), _d, _d, C.POINTER(C.c_double),
+                                 C.POINTER(C.c_double)]
+        L.ref_set_waven.argtypes = [C.c_int, _d, _d, C.c_int]
+        L.ref_precal.argtypes = [C.POINTER(C.c_int), C.POINTER(C.c_int), _d, _d, _d,
+                                 C.POINTER(C.c_double)]
+        L.ref_assemble.argtypes = [C.c_int, C.c_int, _z, _z, _z, _d]
+        L.ref_chol.argtypes = [_z]
+        L.ref_vre.argtypes = [_z, _z, _z]
+        L.ref_forward.argtypes = [_z, _z, _z, C.c_int]
+        L.ref_volt.argtypes = [C.c_int, _z, _z]
+        L.ref_sens.argtypes = [C.c_int, _z]
+        _lib = L
+    return _lib
+
+
+class RefError(RuntimeError):
+    def __init__(self, where, code):
+        super().__init__(f"reference {where}: errnr {code}")
+        self.code = code
+
+
+class Reference:
+    """Drive the reference routines for one mesh (module-global state)."""
+
+    def __init__(self, mesh, lsytop=False, lsink=False, nsink=0):
+        self.mesh = mesh
+        nrel = np.asfortranarray(mesh.nrel, np.int32).ravel(order="F").copy()
+        rnr = np.ascontiguousarray(mesh.rnr if len(mesh.rnr) else np.zeros(1), np.int32)
+        e = lib().ref_load_mesh(mesh.sanz, mesh.typanz, mesh.mb,
+                                np.ascontiguousarray(mesh.typ, np.int32),
+                                np.ascontiguousarray(mesh.nelanz, np.int32),
+                                np.ascontiguousarray(mesh.selanz, np.int32), nrel, rnr,
+                                np.ascontiguousarray(mesh.snr, np.int32),
+                                np.ascontiguousarray(mesh.sx, np.float64),
+                                np.ascontiguousarray(mesh.sy, np.float64), mesh.eanz,
+                                np.ascontiguousarray(mesh.enr, np.int32), int(lsytop),
+                                int(lsink), int(nsink))
+        if e:
+            raise RefError("load_mesh", e)
+        self.elanz, self.relanz = mesh.elanz, mesh.relanz
+        self.nimn = mesh.maxnodes * (mesh.maxnodes + 1) // 2
+
+    def load_data(self, strnr, vnr):
+        self.nanz = len(strnr)
+        lib().ref_load_data(self.nanz, np.ascontiguousarray(strnr, np.int32),
+                            np.ascontiguousarray(vnr, np.int32))
+
+    def rwaven(self):
+        K = C.c_int()
+        kwn = np.zeros(30)
+        kwnwi = np.zeros(30)
+        amin, amax = C.c_double(), C.c_double()
+        e = lib().ref_rwaven(C.byref(K), kwn, kwnwi, C.byref(amin), C.byref(amax))
+        if e:
+            raise RefError("rwaven", e)
+        self.K = K.value
+        self.kwn, self.kwnwi = kwn[:self.K].copy(), kwnwi[:self.K].copy()
+        return self.kwn, self.kwnwi
+
+    def set_waven(self, kwn, kwnwi, swrtr=1):
+        self.K = len(kwn)
+        lib().ref_set_waven(self.K, np.ascontiguousarray(kwn, np.float64),
+                            np.ascontiguousarray(kwnwi, np.float64), swrtr)
+
+    def precal(self):
+        K = self.K
+        elbg = np.zeros(self.elanz * self.nimn * K)
+        relbg = np.zeros(max(self.relanz, 1) * self.nimn)
+        kg = np.zeros(max(self.relanz, 1) * self.mesh.eanz * K)
+        lb, lr, st = C.c_int(), C.c_int(), C.c_double()
+        e = lib().ref_precal(C.byref(lb), C.byref(lr), elbg, relbg, kg, C.byref(st))
+        if e:
+            raise RefError("precal", e)
+        self.lbeta, self.lrandb, self.sytop = lb.value, lr.value, st.value
+        return elbg, relbg, kg
+
+    def assemble(self, sigma, l, k):
+        n, mb = self.mesh.sanz, self.mesh.mb
+        a = np.zeros((mb + 1) * n, np.complex128)
+        b = np.zeros(n, np.complex128)
+        fak = np.zeros(n)
+        e = lib().ref_assemble(l + 1, k + 1, np.ascontiguousarray(sigma, np.complex128), a, b, fak)
+        if e:
+            raise RefError("assemble", e)
+        return a, b, fak
+
+    def chol(self, a):
+        a = a.copy()
+        e = lib().ref_chol(a)
+        if e:
+            raise RefError("chol", e)
+        return a
+
+    def vre(self, a, b):
+        pot = np.zeros(self.mesh.sanz, np.complex128)
+        lib().ref_vre(a, np.ascontiguousarray(b), pot)
+        return pot
+
+    def forward(self, sigma, nthreads=0):
+        n, E, K = self.mesh.sanz, self.mesh.eanz, self.K
+        kpot = np.zeros(n * E * K, np.complex128)
+        hpot = np.zeros(n * E, np.complex128)
+        e = lib().ref_forward(np.ascontiguousarray(sigma, np.complex128), kpot, hpot, nthreads)
+        if e:
+            raise RefError("forward", e)
+        return kpot.reshape((n, E, K), order="F"), hpot.reshape((n, E), order="F")
+
+    def volt(self, mode=0):
+        volt = np.zeros(self.nanz, np.complex128)
+        sigmaa = np.zeros(self.nanz, np.complex128)
+        e = lib().ref_volt(mode, volt, sigmaa)
+        if e:
+            raise RefError("volt", e)
+        return volt, sigmaa
+
+    def sens(self, mode=0):
+        out = np.zeros(self.nanz * self.elanz, np.complex128)
+        e = lib().ref_sens(mode, out)
+        if e:
+            raise RefError("sens", e)
+        return out.reshape((self.nanz, self.elanz), order="F")

@@ -1,0 +1,295 @@
+!> C-callable harness around the reference CRMod/CRTomo hot-path routines.
+!>
+!> TEST INFRASTRUCTURE ONLY.  This file is compiled together with the
+!> reference's own, unmodified sources (kompab.f90, chol.f90, vre.f90, ...),
+!> which stay where they lie in the reference tree; the recipe is
+!> oracle/Makefile and the output goes to oracle/_ref/ (never committed).
+!> It replaces the reference's main program (src/fem.f90) with entry points
+!> that load the module state from caller arrays and run one stage each, so
+!> that tests can read the reference's full-precision intermediate results.
+!>
+!> The forward loop below follows the order of src/fem.f90:328-397
+!> (kompab -> randb -> randb2 -> scalab -> chol, or kompb when the factor is
+!> reused, then vre and kpot = pot*fak), with the same OpenMP split over
+!> wavenumbers.
+module crt_ref_driver
+  use iso_c_binding
+  use alloci
+  use femmod
+  use sigmamod
+  use electrmod
+  use elemmod
+  use wavenmod
+  use datmod
+  use modelmod
+  use errmod
+  use konvmod, only: lverb, lsytop
+  use randbmod
+  implicit none
+
+contains
+
+  !> Load mesh + electrodes (replaces relem/relectr, src/relem.f90:62-151,
+  !> src/relectr.f90:46-68).  nrel is (elanz+relanz) x maxnodes, column major,
+  !> 1-based node numbers, exactly the layout of elemmod::nrel.
+  integer(c_int) function ref_load_mesh(c_sanz, c_typanz, c_mb, c_typ, c_nelanz, &
+       c_selanz, c_nrel, c_rnr, c_snr, c_sx, c_sy, c_eanz, c_enr, c_lsytop, &
+       c_lsink, c_nsink) bind(C, name='ref_load_mesh')
+    integer(c_int), value :: c_sanz, c_typanz, c_mb, c_eanz, c_lsytop, c_lsink, c_nsink
+    integer(c_int), intent(in) :: c_typ(c_typanz), c_nelanz(c_typanz), c_selanz(c_typanz)
+    integer(c_int), intent(in) :: c_nrel(*), c_rnr(*), c_snr(c_sanz), c_enr(c_eanz)
+    real(c_double), intent(in) :: c_sx(c_sanz), c_sy(c_sanz)
+    integer :: i, j, nel
+
+    call ref_free_mesh()
+    sanz = c_sanz; typanz = c_typanz; mb = c_mb
+    allocate(typ(typanz), nelanz(typanz), selanz(typanz))
+    typ = c_typ; nelanz = c_nelanz; selanz = c_selanz
+    max_nr_element_nodes = maxval(selanz)
+    elanz = 0; relanz = 0
+    do i = 1, typanz
+       if (typ(i) > 10) then
+          relanz = relanz + nelanz(i)
+       else
+          elanz = elanz + nelanz(i)
+       end if
+    end do
+    nel = elanz + relanz
+    allocate(sx(sanz), sy(sanz), snr(sanz))
+    sx = c_sx; sy = c_sy; snr = c_snr
+    allocate(nrel(nel, max_nr_element_nodes), rnr(max(relanz,1)))
+    nrel = 0
+    do j = 1, max_nr_element_nodes
+       do i = 1, nel
+          nrel(i, j) = c_nrel(i + (j-1)*nel)
+       end do
+    end do
+    if (relanz > 0) rnr(1:relanz) = c_rnr(1:relanz)
+    eanz = c_eanz
+    allocate(enr(eanz))
+    enr = c_enr
+    lsytop = (c_lsytop /= 0)
+    lsink = (c_lsink /= 0)
+    nsink = c_nsink
+    lrandb2 = .false.
+    rwdanz = 0
+    nr_elec_capacitances = 0
+    lsr = .false.
+    ldc = .false.
+    lverb = .false.
+    manz = elanz
+    allocate(mnr(elanz))
+    do i = 1, elanz
+       mnr(i) = i
+    end do
+    ref_load_mesh = 0
+  end function ref_load_mesh
+
+  subroutine ref_free_mesh()
+    if (allocated(typ)) deallocate(typ, nelanz, selanz)
+    if (allocated(sx)) deallocate(sx, sy, snr)
+    if (allocated(nrel)) deallocate(nrel)
+    if (allocated(rnr)) deallocate(rnr)
+    if (allocated(enr)) deallocate(enr)
+    if (allocated(mnr)) deallocate(mnr)
+    if (allocated(elbg)) deallocate(elbg)
+    if (allocated(relbg)) deallocate(relbg)
+    if (allocated(kg)) deallocate(kg)
+    if (allocated(kpot)) deallocate(kpot)
+    if (allocated(hpot)) deallocate(hpot)
+  end subroutine ref_free_mesh
+
+  !> Measurement configurations (replaces rdatm, src/rdatm.f90:54-115).
+  integer(c_int) function ref_load_data(c_nanz, c_strnr, c_vnr) bind(C, name='ref_load_data')
+    integer(c_int), value :: c_nanz
+    integer(c_int), intent(in) :: c_strnr(c_nanz), c_vnr(c_nanz)
+    if (allocated(strnr)) deallocate(strnr, strom, volt, sigmaa, kfak, vnr)
+    nanz = c_nanz
+    allocate(strnr(nanz), strom(nanz), volt(nanz), sigmaa(nanz), kfak(nanz), vnr(nanz))
+    strnr = c_strnr; vnr = c_vnr
+    strom = 1d0
+    kfak = 1d0
+    ref_load_data = 0
+  end function ref_load_data
+
+  !> Run the reference rwaven (src/rwaven.f90:22-171); arrays sized >= 30.
+  integer(c_int) function ref_rwaven(c_kwnanz, c_kwn, c_kwnwi, c_amin, c_amax) bind(C, name='ref_rwaven')
+    integer(c_int), intent(out) :: c_kwnanz
+    real(c_double), intent(out) :: c_kwn(30), c_kwnwi(30), c_amin, c_amax
+    if (allocated(kwn)) deallocate(kwn, kwnwi)
+    swrtr = 1
+    call rwaven()
+    ref_rwaven = errnr
+    if (errnr /= 0) return
+    c_kwnanz = kwnanz
+    c_kwn(1:kwnanz) = kwn
+    c_kwnwi(1:kwnanz) = kwnwi
+    c_amin = amin
+    c_amax = amax
+  end function ref_rwaven
+
+  !> Set wavenumbers directly (2D mode: c_swrtr=0, one wavenumber 0).
+  integer(c_int) function ref_set_waven(c_kwnanz, c_kwn, c_kwnwi, c_swrtr) bind(C, name='ref_set_waven')
+    integer(c_int), value :: c_kwnanz, c_swrtr
+    real(c_double), intent(in) :: c_kwn(c_kwnanz), c_kwnwi(c_kwnanz)
+    if (allocated(kwn)) deallocate(kwn, kwnwi)
+    kwnanz = c_kwnanz
+    swrtr = c_swrtr
+    allocate(kwn(kwnanz), kwnwi(kwnanz))
+    kwn = c_kwn; kwnwi = c_kwnwi
+    ref_set_waven = 0
+  end function ref_set_waven
+
+  !> Run the reference precal (src/precal.f90:13-222) and copy elbg/relbg/kg out.
+  !> elbg_out: elanz x (maxnodes*(maxnodes+1)/2) x kwnanz; relbg_out: relanz x same;
+  !> kg_out: relanz x eanz x kwnanz (all column major).
+  integer(c_int) function ref_precal(c_lbeta, c_lrandb, c_elbg, c_relbg, c_kg, c_sytop) bind(C, name='ref_precal')
+    integer(c_int), intent(out) :: c_lbeta, c_lrandb
+    real(c_double), intent(out) :: c_elbg(*), c_relbg(*), c_kg(*), c_sytop
+    integer :: nimn, ntot
+    if (allocated(elbg)) deallocate(elbg)
+    if (allocated(relbg)) deallocate(relbg)
+    if (allocated(kg)) deallocate(kg)
+    call precal()
+    ref_precal = errnr
+    if (errnr /= 0) return
+    c_lbeta = merge(1, 0, lbeta)
+    c_lrandb = merge(1, 0, lrandb)
+    c_sytop = sytop
+    nimn = (max_nr_element_nodes*(max_nr_element_nodes+1))/2
+    ntot = elanz*nimn*kwnanz
+    c_elbg(1:ntot) = reshape(elbg, (/ntot/))
+    if (relanz > 0) then
+       c_relbg(1:relanz*nimn) = reshape(relbg, (/relanz*nimn/))
+       c_kg(1:relanz*eanz*kwnanz) = reshape(kg, (/relanz*eanz*kwnanz/))
+    end if
+  end function ref_precal
+
+  !> One (k,l) system exactly as assembled by the reference: kompab, then the
+  !> optional Dirichlet routines, then scalab.  Returns the scaled band matrix
+  !> ((mb+1)*sanz), the scaled RHS and the scaling factors.
+  integer(c_int) function ref_assemble(c_l, c_k, c_sigma, c_a, c_b, c_fak) bind(C, name='ref_assemble')
+    integer(c_int), value :: c_l, c_k
+    complex(c_double_complex), intent(in) :: c_sigma(*)
+    complex(c_double_complex), intent(out) :: c_a(*), c_b(*)
+    real(c_double), intent(out) :: c_fak(*)
+    complex(kind(0d0)), allocatable :: my_a(:), my_b(:)
+    real(kind(0d0)), allocatable :: my_fak(:)
+    call set_sigma(c_sigma)
+    allocate(my_a((mb+1)*sanz), my_b(sanz), my_fak(sanz))
+    call kompab(c_l, c_k, my_a, my_b)
+    if (errnr /= 0) then
+       ref_assemble = errnr
+       return
+    end if
+    if (lrandb) call randb(my_a, my_b)
+    call scalab(my_a, my_b, my_fak)
+    ref_assemble = errnr
+    c_a(1:(mb+1)*sanz) = my_a
+    c_b(1:sanz) = my_b
+    c_fak(1:sanz) = my_fak
+  end function ref_assemble
+
+  !> In-place reference chol (src/chol.f90:7-88) on a caller band matrix.
+  integer(c_int) function ref_chol(c_a) bind(C, name='ref_chol')
+    complex(c_double_complex), intent(inout) :: c_a(*)
+    call chol(c_a)
+    ref_chol = errnr
+  end function ref_chol
+
+  !> Reference vre (src/vre.f90:5-71) with a factored band matrix.
+  integer(c_int) function ref_vre(c_a, c_b, c_pot) bind(C, name='ref_vre')
+    complex(c_double_complex), intent(in) :: c_a(*), c_b(*)
+    complex(c_double_complex), intent(out) :: c_pot(*)
+    call vre(c_a, c_b, c_pot)
+    ref_vre = 0
+  end function ref_vre
+
+  subroutine set_sigma(c_sigma)
+    complex(c_double_complex), intent(in) :: c_sigma(*)
+    if (allocated(sigma)) deallocate(sigma)
+    allocate(sigma(elanz))
+    sigma = c_sigma(1:elanz)
+  end subroutine set_sigma
+
+  !> Forward potentials for all (k,l), following src/fem.f90:328-397, then
+  !> rtrafo (src/rtrafo.f90) when swrtr=1.  kpot_out: sanz x eanz x kwnanz,
+  !> hpot_out: sanz x eanz.  Requires ref_precal to have run.
+  integer(c_int) function ref_forward(c_sigma, c_kpot, c_hpot, c_nthreads) bind(C, name='ref_forward')
+    use omp_lib
+    complex(c_double_complex), intent(in) :: c_sigma(*)
+    complex(c_double_complex), intent(out) :: c_kpot(*), c_hpot(*)
+    integer(c_int), value :: c_nthreads
+    complex(kind(0d0)), allocatable :: my_a(:), my_b(:), my_pot(:)
+    real(kind(0d0)), allocatable :: my_fak(:)
+    integer :: k, l, j, nerr
+    call set_sigma(c_sigma)
+    if (allocated(kpot)) deallocate(kpot)
+    if (allocated(hpot)) deallocate(hpot)
+    allocate(kpot(sanz, eanz, kwnanz), hpot(sanz, eanz))
+    if (c_nthreads > 0) call omp_set_num_threads(c_nthreads)
+    nerr = 0
+    !$OMP PARALLEL DEFAULT(shared) PRIVATE(my_a, my_b, my_pot, my_fak, k, l, j)
+    allocate(my_a((mb+1)*sanz), my_b(sanz), my_pot(sanz), my_fak(sanz))
+    !$OMP DO
+    do k = 1, kwnanz
+       do l = 1, eanz
+          if (lbeta .or. l == 1) then
+             call kompab(l, k, my_a, my_b)
+             if (lrandb) call randb(my_a, my_b)
+             if (lrandb2) call randb2(my_a, my_b)
+             call scalab(my_a, my_b, my_fak)
+             call chol(my_a)
+          else
+             call kompb(l, my_b, my_fak)
+          end if
+          call vre(my_a, my_b, my_pot)
+          do j = 1, sanz
+             kpot(j, l, k) = my_pot(j) * dcmplx(my_fak(j))
+             if (swrtr == 0) hpot(j, l) = kpot(j, l, k)
+          end do
+       end do
+    end do
+    !$OMP END DO
+    deallocate(my_a, my_b, my_pot, my_fak)
+    !$OMP END PARALLEL
+    if (swrtr == 1) call rtrafo()
+    c_kpot(1:sanz*eanz*kwnanz) = reshape(kpot, (/sanz*eanz*kwnanz/))
+    c_hpot(1:sanz*eanz) = reshape(hpot, (/sanz*eanz/))
+    ref_forward = errnr
+  end function ref_forward
+
+  !> Reference bvolt (CRMod, mode 0) or bvolti (CRTomo, mode 1).
+  integer(c_int) function ref_volt(c_mode, c_volt, c_sigmaa) bind(C, name='ref_volt')
+    integer(c_int), value :: c_mode
+    complex(c_double_complex), intent(out) :: c_volt(*), c_sigmaa(*)
+    kfak = 1d0
+    if (c_mode == 0) then
+       call bvolt()
+    else
+       call bvolti()
+    end if
+    ref_volt = errnr
+    c_volt(1:nanz) = volt
+    c_sigmaa(1:nanz) = sigmaa
+  end function ref_volt
+
+  !> Reference bsens (mode 0, dV/dsigma) or bsensi (mode 1, d(-lnV)/d(ln sigma)).
+  !> sens_out: nanz x elanz column major.  Requires ref_forward and ref_volt.
+  integer(c_int) function ref_sens(c_mode, c_sens) bind(C, name='ref_sens')
+    integer(c_int), value :: c_mode
+    complex(c_double_complex), intent(out) :: c_sens(*)
+    real(kind(0d0)) :: c1
+    if (allocated(sens)) deallocate(sens)
+    allocate(sens(nanz, manz))
+    if (c_mode == 0) then
+       call bsens()
+    else
+       call bsensi(.false.)
+    end if
+    c_sens(1:nanz*manz) = reshape(sens, (/nanz*manz/))
+    ref_sens = errnr
+    c1 = 0d0
+  end function ref_sens
+
+end module crt_ref_driver
