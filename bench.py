@@ -1,0 +1,211 @@
+#!/usr/bin/env python3
+"""Benchmark of the 2.5D FE forward + sensitivity path on one MI355X.
+
+Metric: modelled quadrupoles/s (forward: assembly -> factor -> substitution ->
+back-transform -> voltages) on the 240x80-element / 19,521-node / 96-electrode /
+3,090-quadrupole workload, complex FP64; plus the adjoint-gradient time versus
+the explicit-Jacobian path on the same model.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
+
+Prints one JSON line.  A step is one full forward over all 14 x 96 (k, l)
+systems with the conductivities already resident in device memory.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+PEAK_FP64_TFLOPS = 78.6   # MI355X dense FP64 (vector and matrix)
+PEAK_HBM_GBS = 8000.0
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(w, nsys, K, sample=64):
+    """Time the reference (oracle/_ref: the reference's Fortran routines built
+    from its source) on a bounded sample of (k, l) systems; fall back to the
+    C restatement when the reference build is absent."""
+    import numpy as np
+    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    cores = max(1, min(cores, K))   # the reference runs at most kwnanz threads
+    sample = min(sample, nsys)
+    stride = max(1, nsys // sample)
+    try:
+        from oracle import ref
+        if not ref.available():
+            raise ImportError
+        r = ref.Reference(w.mesh)
+        r.load_data(w.strnr, w.vnr)
+        r.rwaven()
+        r.precal()
+        first = 1
+        t0 = time.perf_counter()
+        ref.forward_sample(w.sigma, first, first + sample - 1, cores)
+        dt = time.perf_counter() - t0
+        kind = "reference"
+    except ImportError:
+        from oracle import cpu
+        o = cpu.Oracle(w.mesh)
+        o.rwaven(w.strnr, w.vnr)
+        o.precal()
+        t0 = time.perf_counter()
+        o.forward(w.sigma, nthreads=cores, sys_range=(0, sample))
+        dt = time.perf_counter() - t0
+        kind = "port"
+    del stride, np
+    t_full = dt * nsys / sample
+    return {
+        "value": w.nanz / t_full,
+        "unit": "quadrupoles/s",
+        "cores": cores,
+        "kind": kind,
+        "sample": f"{sample} of {nsys} (k,l) systems (assemble+scale+chol+vre), "
+                  f"{dt:.2f} s wall, extrapolated linearly to the full forward",
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-sample", type=int, default=64)
+    ap.add_argument("--skip-adjoint", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1",
+                          "n_gpus": args.gpus}))
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    from titatest_amd import crt, meshgen
+
+    if not torch.cuda.is_available():
+        sys.exit("bench.py needs a HIP device")
+    dev = torch.device("cuda:0")
+
+    w = meshgen.large_grf()
+    m = w.mesh
+    g = crt.CRT(m)
+    g.rwaven(w.strnr, w.vnr)
+    g.precal()
+    n, mb, E, K, N = m.sanz, m.mb, m.eanz, g.K, w.nanz
+    nsys = K * E
+    log(f"workload large_grf: n={n} mb={mb} E={E} K={K} N={N} systems={nsys}")
+    sigma_dev = torch.tensor(w.sigma, dtype=torch.complex128, device=dev)
+    st = np.ascontiguousarray(w.strnr, np.int32)
+    vn = np.ascontiguousarray(w.vnr, np.int32)
+    volt = torch.empty(N, dtype=torch.complex128, device=dev)
+    sigmaa = torch.empty(N, dtype=torch.complex128, device=dev)
+
+    def step():
+        rc = g.lib.crt_forward_dev(g.h, sigma_dev.data_ptr())
+        if rc:
+            raise crt.CRTError("forward_dev", rc)
+        rc = g.lib.crt_voltages_dev(g.h, N, st.ctypes.data, vn.ctypes.data, 0, volt.data_ptr(),
+                                    sigmaa.data_ptr())
+        if rc:
+            raise crt.CRTError("voltages_dev", rc)
+
+    torch.cuda.synchronize()
+    for i in range(args.warmup):
+        step()
+        log(f"warmup {i + 1}/{args.warmup} done")
+    g.set_timing(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        step()
+        log(f"step {i + 1}/{args.steps} done")
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    times = g.stage_times()
+    g.set_timing(False)
+    ms_per_step = (t1 - t0) * 1e3 / args.steps
+    value = N / (ms_per_step / 1e3)
+
+    fac_ms, fac_n = times.get("factor_solve", (float("nan"), 1))
+    fac_avg_s = fac_ms / fac_n / 1e3
+    flops = nsys * (4.0 * n * mb * mb + 16.0 * n * mb)
+    achieved = flops / fac_avg_s / 1e12
+    lbytes = nsys * 2.0 * 16.0 * n * (mb + 1)
+
+    result = {
+        "metric": "modelled quadrupoles/s and adjoint-gradient ms vs explicit J, FP64, 1 GPU",
+        "value": value,
+        "unit": "quadrupoles/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "c128",
+        "data": "synthetic (seeded Gaussian random-field model, generated mesh)",
+        "config": {"workload": "240x80 quads, 19521 nodes, mb 82, 96 electrodes, "
+                               "3090 dipole-dipole quadrupoles, 14 wavenumbers x 96 sources",
+                   "systems": nsys, "nodes": n, "bandwidth": mb, "electrodes": E,
+                   "quadrupoles": N, "wavenumbers": K},
+        "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP64_TFLOPS,
+                     "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None,
+                     "kernel": "crt_factor_kernel", "avg_launch_ms": fac_ms / fac_n,
+                     "algorithmic_flops": flops, "algorithmic_bytes": lbytes},
+        "stage_ms_per_step": {k: v[0] / v[1] for k, v in times.items()},
+    }
+
+    if not args.skip_adjoint:
+        # config with data: the adjoint gradient of the weighted misfit vs the
+        # explicit-J data term, both after the same forward
+        rng = np.random.default_rng(1004)
+        g.forward(w.sigma)
+        v1, sa1 = g.bvolti(st, vn)
+        dat = sa1 + 0.02 * rng.standard_normal(N)
+        wgt = 1.0 / (0.02 + 1e-4 / np.abs(v1)) ** 2
+        r = dat - sa1
+        g.set_timing(True)
+        ga = None
+        for _ in range(3):
+            ga = g.adjoint_grad(wgt, r, mode=1)
+        t_adj = g.stage_times()
+        g.set_timing(True)
+        gj = None
+        for _ in range(3):
+            g.sensitivities(1, to_host=False)
+            gj = g.jhw_r(wgt, r)
+        t_exp = g.stage_times()
+        g.set_timing(False)
+        adj_ms = sum(v[0] for v in t_adj.values()) / 3
+        exp_ms = sum(v[0] for v in t_exp.values()) / 3
+        result["adjoint"] = {
+            "adjoint_gradient_ms": adj_ms,
+            "explicit_j_ms": exp_ms,
+            "explicit_over_adjoint": exp_ms / adj_ms,
+            "forward_ms": ms_per_step,
+            "rel_diff": float(np.max(np.abs(ga - gj)) / np.max(np.abs(gj))),
+            "kernels_ms": {**{k: v[0] / v[1] for k, v in t_adj.items()},
+                           **{k: v[0] / v[1] for k, v in t_exp.items()}},
+        }
+        log(f"adjoint {adj_ms:.3f} ms, explicit J {exp_ms:.3f} ms")
+
+    if not args.no_cpu_baseline:
+        log("cpu baseline sample")
+        result["cpu_baseline"] = cpu_baseline(w, nsys, K, args.cpu_sample)
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,139 @@
+/* C-ABI of the MI355X (gfx950) CRMod/CRTomo forward + sensitivity path.
+ *
+ * The entry points replace, stage for stage, the serial-region calls of the
+ * reference programs (CRMod src/fem.f90, CRTomo src/inv.f90 and update.f90).
+ * Arrays keep the layout of the reference's Fortran module arrays: column
+ * major, with 1-based node / element / electrode numbers as VALUES inside
+ * integer arrays.  Complex arrays are interleaved (re, im) doubles, i.e.
+ * COMPLEX(KIND(0D0)) / C double _Complex, passed as double*.
+ *
+ * Every function returns the reference's error number semantics (errnr):
+ * 0 = success, otherwise the code from the reference's error table (e.g. 19
+ * band width too small, 26 bad element geometry, 27 zero diagonal, 28 zero
+ * Cholesky pivot, 35 beta geometry, 73 duplicate electrodes, 82 zero voltage,
+ * 101 mixed and Dirichlet boundaries together).  Codes >= 1000 are errors of
+ * this library (1000 invalid argument, 1001 device/HIP failure, 1002 stage
+ * called out of order, 1003 band width above the supported limit).
+ * Calls are synchronous on return; a handle is not re-entrant, distinct handles
+ * may be used from different threads.
+ */
+#ifndef CRT_HIP_H
+#define CRT_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct crt_handle crt_handle;
+
+/* Mesh + electrodes as read by relem/relectr (src/relem.f90:62-151,
+ * src/relectr.f90:46-68); the elemmod/electrmod module arrays. */
+typedef struct crt_mesh {
+  int32_t sanz;           /* number of nodes */
+  int32_t typanz;         /* number of element types */
+  int32_t mb;             /* band width */
+  int32_t eanz;           /* number of electrodes */
+  const int32_t *typ;     /* typanz: element type (3, 5, 8 area; 11, 12, 13 boundary) */
+  const int32_t *nelanz;  /* typanz: elements of the type */
+  const int32_t *selanz;  /* typanz: nodes per element of the type */
+  const int32_t *nrel;    /* (elanz+relanz) x maxnodes, column major, 1-based nodes */
+  const int32_t *rnr;     /* relanz: area element adjacent to each boundary element */
+  const int32_t *snr;     /* sanz: coordinate pointer */
+  const double *sx;       /* sanz */
+  const double *sy;       /* sanz */
+  const int32_t *enr;     /* eanz: electrode nodes */
+  int32_t lsytop;         /* mirror sources about the mean no-flow height (bsytop) */
+  int32_t lsink;          /* fictitious sink (crmod.cfg) */
+  int32_t nsink;          /* sink node (1-based) */
+} crt_mesh;
+
+/* Create a handle for one mesh: copies the arrays, builds the device-side
+ * row-assembly plan.  Replaces the mesh set-up of src/fem.f90:208-215.
+ * Returns 19 when an element spans more than mb nodes (kompab's check). */
+int crt_create(crt_handle **h, const crt_mesh *mesh);
+int crt_destroy(crt_handle *h);
+
+/* Wavenumbers and weights, src/rwaven.f90:22-171 (host, bit-identical to the
+ * reference).  kwn/kwnwi must hold 30 values; the set is also kept in h. */
+int crt_rwaven(crt_handle *h, int32_t nanz, const int32_t *strnr, const int32_t *vnr,
+               int32_t *kwnanz, double *kwn, double *kwnwi);
+/* Use a given set instead (swrtr = 0: 2-D, one wavenumber 0, src/fem.f90:240-251). */
+int crt_set_wavenumbers(crt_handle *h, int32_t kwnanz, const double *kwn,
+                        const double *kwnwi, int32_t swrtr);
+
+/* Element and mixed-boundary pre-computation on the device, src/precal.f90:13-222
+ * (elem1/3/5/8, beta with the reference's Bessel polynomials, bsytop). */
+int crt_precal(crt_handle *h, int32_t *lbeta, int32_t *lrandb);
+/* Copy the precomputed arrays back in the reference layout (for checking):
+ * elbg elanz x nimn x K, relbg relanz x nimn, kg relanz x eanz x K (column major,
+ * nimn = maxnodes(maxnodes+1)/2).  Any pointer may be NULL. */
+int crt_get_precal(crt_handle *h, double *elbg, double *relbg, double *kg);
+
+/* Forward potentials for all (wavenumber, source electrode) systems: assembly
+ * (kompab, randb), scaling (scalab), band Cholesky (chol) and substitution (vre),
+ * src/fem.f90:328-397, then the back-transform (rtrafo, src/rtrafo.f90).
+ * sigma: elanz complex conductivities.  kpot (sanz x eanz x K) and hpot
+ * (sanz x eanz) are optional host outputs (NULL to keep them on the device only). */
+int crt_forward(crt_handle *h, const double *sigma, double *kpot, double *hpot);
+/* Same with sigma already in device memory (elanz complex). */
+int crt_forward_dev(crt_handle *h, const double *sigma_dev);
+
+/* Voltages and apparent values, src/bvolt.f90 (mode 0: sigmaa = kfak*V) and
+ * src/bvolti.f90 (mode 1: sigmaa = -ln V, error 82 on V = 0).  kfak may be NULL
+ * (1).  The configuration is kept in h for the sensitivity/gradient stages. */
+int crt_voltages(crt_handle *h, int32_t nanz, const int32_t *strnr, const int32_t *vnr,
+                 int32_t mode, const double *kfak, double *volt, double *sigmaa);
+/* Device-pointer variant of the outputs (volt_dev / sigmaa_dev: nanz complex). */
+int crt_voltages_dev(crt_handle *h, int32_t nanz, const int32_t *strnr, const int32_t *vnr,
+                     int32_t mode, double *volt_dev, double *sigmaa_dev);
+
+/* Explicit Jacobian, src/bsens.f90 (mode 0: dV/dsigma) and src/bsensi.f90
+ * (mode 1: d(-ln V)/d(ln sigma)); sens: nanz x elanz column major complex, or NULL
+ * to keep it on the device for crt_cg_solve. */
+int crt_sensitivities(crt_handle *h, int32_t mode, double *sens);
+
+/* Data-misfit gradient without forming J: grad_e = sum_i conj(J_ie) w_i r_i with
+ * J as crt_sensitivities(mode) defines it (the data term of src/update.f90:235-289).
+ * w: nanz weights (wmatd*wdfak), resid: nanz complex (d - f). */
+int crt_adjoint_gradient(crt_handle *h, int32_t mode, const double *w, const double *resid,
+                         double *grad);
+int crt_adjoint_gradient_dev(crt_handle *h, int32_t mode, const double *w_dev,
+                             const double *resid_dev, double *grad_dev);
+
+/* The same data term from the explicit Jacobian of the last crt_sensitivities
+ * call: grad_j = sum_i conj(J_ij) w_i r_i (src/update.f90:263-266, before the
+ * cgfac scaling).  The explicit-J path the adjoint gradient is compared with. */
+int crt_jhw_r(crt_handle *h, const double *w, const double *resid, double *grad);
+
+/* One model update's linear solve, src/update.f90:179-307 with cjggra
+ * (src/cg_mod.f90:426-518): preconditioner cgfac, scaled RHS, then Fletcher-Reeves
+ * CG on (J^H W J + lam R) with products bap/bp/bptri/bplma/bb, using the J of the
+ * last crt_sensitivities call.  smatm: manz x (3 for ltri=0 | 1 for ltri=3,4 |
+ * maxnodes+1 for triangular), nachbar: manz x maxnodes (ltri 1,2,5..14, else NULL),
+ * rdiag: the diagonal of R the preconditioner adds, rm: R m (manz complex, NULL for
+ * m = 0).  Outputs dpar (manz complex, scaled by cgfac as update does) and cgres
+ * (ncgmax+2: cgres[0] = iterations, cgres[k] = relative residual). */
+int crt_cg_solve(crt_handle *h, const double *wmatd_wdfak, const double *dat,
+                 const double *sigmaa, double lam, int32_t ltri, int32_t nx,
+                 const double *smatm, int32_t smatm_cols, const int32_t *nachbar,
+                 int32_t maxnodes, const double *rdiag, const double *rm, int32_t ncgmax,
+                 double eps, double *dpar, double *cgfac, double *cgres);
+
+/* First failing system of the last call: error code, wavenumber and electrode
+ * (1-based) where known. */
+int crt_error_info(crt_handle *h, int32_t *code, int32_t *k, int32_t *l);
+
+/* Per-kernel device time of the calls since the last reset (HIP events on the
+ * handle's stream).  names: n_max slots of 48 chars; returns the count. */
+int crt_set_timing(crt_handle *h, int32_t enable);
+int crt_stage_times(crt_handle *h, int32_t n_max, char *names, double *ms, int32_t *launches);
+/* Device pointer of the last forward's potentials kpot (sanz x eanz x K complex). */
+int crt_kpot_device(crt_handle *h, const double **kpot_dev);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -153,3 +153,12 @@ class Reference:
         if e:
             raise RefError("sens", e)
         return out.reshape((self.nanz, self.elanz), order="F")
+
+
+def forward_sample(sigma, first, last, nthreads=0):
+    """Time-only sample of the reference forward loop (systems first..last, 1-based)."""
+    L = lib()
+    L.ref_forward_sample.argtypes = [_z, C.c_int, C.c_int, C.c_int]
+    e = L.ref_forward_sample(np.ascontiguousarray(sigma, np.complex128), first, last, nthreads)
+    if e:
+        raise RefError("forward_sample", e)

@@ -259,6 +259,37 @@ contains
     ref_forward = errnr
   end function ref_forward
 
+  !> A bounded sample of the forward loop for timing: systems s = (k-1)*eanz + l
+  !> in [c_first, c_last] (1-based), each assembled, scaled, factored and
+  !> solved as in src/fem.f90:355-380, OpenMP over the systems.  No output.
+  integer(c_int) function ref_forward_sample(c_sigma, c_first, c_last, c_nthreads) &
+       bind(C, name='ref_forward_sample')
+    use omp_lib
+    complex(c_double_complex), intent(in) :: c_sigma(*)
+    integer(c_int), value :: c_first, c_last, c_nthreads
+    complex(kind(0d0)), allocatable :: my_a(:), my_b(:), my_pot(:)
+    real(kind(0d0)), allocatable :: my_fak(:)
+    integer :: s, k, l
+    call set_sigma(c_sigma)
+    if (c_nthreads > 0) call omp_set_num_threads(c_nthreads)
+    !$OMP PARALLEL DEFAULT(shared) PRIVATE(my_a, my_b, my_pot, my_fak, s, k, l)
+    allocate(my_a((mb+1)*sanz), my_b(sanz), my_pot(sanz), my_fak(sanz))
+    !$OMP DO SCHEDULE(dynamic, 1)
+    do s = c_first, c_last
+       k = (s - 1) / eanz + 1
+       l = mod(s - 1, eanz) + 1
+       call kompab(l, k, my_a, my_b)
+       if (lrandb) call randb(my_a, my_b)
+       call scalab(my_a, my_b, my_fak)
+       call chol(my_a)
+       call vre(my_a, my_b, my_pot)
+    end do
+    !$OMP END DO
+    deallocate(my_a, my_b, my_pot, my_fak)
+    !$OMP END PARALLEL
+    ref_forward_sample = errnr
+  end function ref_forward_sample
+
   !> Reference bvolt (CRMod, mode 0) or bvolti (CRTomo, mode 1).
   integer(c_int) function ref_volt(c_mode, c_volt, c_sigmaa) bind(C, name='ref_volt')
     integer(c_int), value :: c_mode

@@ -1,0 +1,80 @@
+import os
+import sys
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+ROOT = Path(__file__).resolve().parents[1]
+sys.path.insert(0, str(ROOT))
+
+GOLDEN = ROOT / "tests" / "golden"
+FIXTURES = GOLDEN / "fixtures"
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs an MI355X (HIP device) and the built libcrt_hip.so")
+    config.addinivalue_line("markers", "slow: larger CPU cases")
+
+
+@pytest.fixture(scope="session")
+def oracle_lib():
+    from oracle import cpu
+    cpu.build()
+    return cpu
+
+
+def load_fixture(name):
+    from titatest_amd import formats as F
+    d = FIXTURES / name
+    m = F.read_elem(d / "elem.dat")
+    m.enr = F.read_elec(d / "elec.dat")
+    sig = F.read_rho(d / "rho.dat", m.elanz)
+    st, vn, mag, pha = F.read_volt(d / "volt.dat")
+    return m, sig, st, vn, mag, pha
+
+
+# (name, swrtr, lsink, nsink)
+FIXTURE_CASES = [
+    ("minimal", 1, False, 0),
+    ("mixed_surf", 1, False, 0),
+    ("decoupling", 0, True, 373),
+    ("rect_complex", 1, False, 0),
+]
+
+
+def golden(name):
+    return np.load(GOLDEN / f"ref_{name}.npz")
+
+
+def rel_err(a, b):
+    a = np.asarray(a)
+    b = np.asarray(b)
+    return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+
+
+def gpu_available():
+    try:
+        import torch
+        return torch.cuda.is_available()
+    except Exception:
+        return False
+
+
+@pytest.fixture(scope="session")
+def hip_lib():
+    if not gpu_available():
+        pytest.skip("no HIP device")
+    from titatest_amd import crt
+    return crt.load_library()
+
+
+def pytest_collection_modifyitems(config, items):
+    if os.environ.get("CRT_FORCE_GPU_TESTS"):
+        return
+    if gpu_available():
+        return
+    skip = pytest.mark.skip(reason="no HIP device in this environment")
+    for it in items:
+        if "gpu" in it.keywords:
+            it.add_marker(skip)

@@ -1,0 +1,43 @@
+"""The C-ABI library loads and exports every entry point include/crt_hip.h
+declares (no compute calls: these run without a GPU)."""
+import ctypes
+import re
+from pathlib import Path
+
+import pytest
+
+from titatest_amd import crt
+
+HEADER = Path(__file__).resolve().parents[1] / "include" / "crt_hip.h"
+
+
+def declared():
+    text = HEADER.read_text()
+    return sorted(set(re.findall(r"^int (crt_\w+)\(", text, re.M)))
+
+
+def test_header_declares_the_binding_set():
+    assert declared() == sorted(crt.EXPORTS)
+
+
+def test_library_exports_every_declared_symbol():
+    if not crt.LIB_PATH.exists():
+        pytest.fail(f"{crt.LIB_PATH} not built; run python -m titatest_amd.build")
+    lib = ctypes.CDLL(str(crt.LIB_PATH))
+    missing = [s for s in declared() if not hasattr(lib, s)]
+    assert not missing, missing
+
+
+def test_binding_configures_all_symbols():
+    lib = crt.load_library()
+    for s in crt.EXPORTS:
+        assert getattr(lib, s).argtypes is not None
+
+
+def test_null_arguments_are_rejected_without_a_device():
+    lib = crt.load_library()
+    # argument validation happens before any HIP call
+    assert lib.crt_create(None, None) == 1000
+    assert lib.crt_destroy(None) == 0
+    assert lib.crt_rwaven(None, 0, None, None, None, None, None) == 1000
+    assert lib.crt_forward(None, None, None, None) == 1000

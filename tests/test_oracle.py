@@ -1,0 +1,357 @@
+"""Pin the C oracle (oracle/crt_oracle.c) before trusting it.
+
+Checks, per stage:
+* against the reference's own routines (golden vectors written by
+  tests/golden/make_golden.py from oracle/_ref, bit-for-bit where the
+  arithmetic is the same),
+* against the FP32 volt.dat outputs shipped with the reference's tests,
+* against closed forms that a dropped term, sign or index would break
+  (Gauss rules vs numpy, the wavenumber integral of K0, element row sums and
+  areas, Bessel functions vs scipy, dense complex solves, L L^T = A, the
+  Euler sum rule and reciprocity of the sensitivities, the exact normal-equation
+  solution for CG).
+"""
+import math
+
+import numpy as np
+import pytest
+import scipy.special as sps
+
+from conftest import FIXTURE_CASES, golden, load_fixture, rel_err
+from titatest_amd import formats as F
+from titatest_amd import meshgen
+
+
+@pytest.fixture(scope="module")
+def small(oracle_lib):
+    w = meshgen.small_block()
+    o = oracle_lib.Oracle(w.mesh)
+    o.rwaven(w.strnr, w.vnr)
+    o.precal()
+    o.forward(w.sigma)
+    return w, o
+
+
+@pytest.fixture(scope="module")
+def small_dir(oracle_lib):
+    w = meshgen.small_dirichlet()
+    o = oracle_lib.Oracle(w.mesh)
+    o.rwaven(w.strnr, w.vnr)
+    o.precal()
+    o.forward(w.sigma)
+    return w, o
+
+
+# ------------------------------------------------------------------ A1 rwaven
+def test_rwaven_matches_reference(small):
+    w, o = small
+    g = golden("small_block")
+    assert o.K == len(g["kwn"]) == 10
+    np.testing.assert_array_equal(o.kwn, g["kwn"])
+    np.testing.assert_array_equal(o.kwnwi, g["kwnwi"])
+
+
+def test_rwaven_gauss_rules_against_numpy(small):
+    w, o = small
+    ganz = o.K - 4
+    k0 = 1.0 / (2.0 * o.amin)
+    x, wt = np.polynomial.legendre.leggauss(ganz)
+    xs = 0.5 * (x + 1.0)
+    ws = 0.5 * wt
+    np.testing.assert_allclose(o.kwn[:ganz], k0 * xs ** 2, rtol=1e-13)
+    np.testing.assert_allclose(o.kwnwi[:ganz], 2 * k0 * ws * xs, rtol=1e-13)
+    xl, wl = np.polynomial.laguerre.laggauss(4)
+    np.testing.assert_allclose(o.kwn[ganz:], k0 * (xl + 1.0), rtol=1e-13)
+    np.testing.assert_allclose(o.kwnwi[ganz:], k0 * wl * np.exp(xl), rtol=1e-12)
+
+
+def test_rwaven_integrates_k0(small):
+    """The rule is built so that (1/pi) sum w_k K0(k r) ~ 1/(2r) within ~1 %
+    for r in [amin, amax] (src/rwaven.f90:16-18)."""
+    w, o = small
+    for r in np.geomspace(o.amin, o.amax, 7):
+        approx = np.sum(o.kwnwi * sps.k0(o.kwn * r))
+        exact = math.pi / (2 * r)
+        assert abs(approx - exact) / exact < 0.01
+
+
+def test_rwaven_count_rule(oracle_lib):
+    """ganz = int(real(6*log10(amax/amin))) with bounds 2 and 26."""
+    w = meshgen.large_grf()
+    o = oracle_lib.Oracle(w.mesh)
+    o.rwaven(w.strnr, w.vnr)
+    assert o.K == int(np.float32(6 * math.log10(o.amax / o.amin))) + 4 == 14
+
+
+# ------------------------------------------------------------- A2/A3 precal
+def test_precal_matches_reference(small):
+    w, o = small
+    g = golden("small_block")
+    m = w.mesh
+    K = o.K
+    elbg = o.elbg.reshape((m.elanz, o.nimn, K), order="F")
+    np.testing.assert_array_equal(elbg[:64], g["elbg"])
+    kg = o.kg.reshape((m.relanz, m.eanz, K), order="F")
+    relbg = o.relbg.reshape((m.relanz, o.nimn), order="F")
+    btyp = np.concatenate([np.full(n, t) for t, n in zip(m.typ, m.nelanz) if t > 10])
+    mixed = btyp == 11
+    np.testing.assert_array_equal(kg[mixed][:64], g["kg"])
+    np.testing.assert_array_equal(relbg[mixed][:, :3], g["relbg"])
+
+
+def _unpack(vals, nk):
+    S = np.zeros((nk, nk))
+    p = 0
+    for a in range(nk):
+        for b in range(a + 1):
+            S[a, b] = S[b, a] = vals[p]
+            p += 1
+    return S
+
+
+def test_precal_element_closed_forms(oracle_lib):
+    """k = 0: constant potentials carry no energy (row sums 0); small k: the
+    k^2 part integrates to the element area (1^T S 1 = k^2 area)."""
+    w = meshgen.small_block()
+    o = oracle_lib.Oracle(w.mesh)
+    kk = 1e-4
+    o.set_waven([0.0, kk], [1.0, 1.0])
+    o.precal()
+    m = w.mesh
+    elbg = o.elbg.reshape((m.elanz, o.nimn, 2), order="F")
+    x, z = m.node_xy(m.nrel[:m.elanz, :4])
+    area = 0.5 * np.abs((x[:, 2] - x[:, 0]) * (z[:, 3] - z[:, 1]) - (x[:, 3] - x[:, 1]) * (z[:, 2] - z[:, 0]))
+    for e in (0, 5, 399, 799):
+        S0 = _unpack(elbg[e, :, 0], 4)
+        np.testing.assert_allclose(S0.sum(1), 0.0, atol=1e-13)
+        S1 = _unpack(elbg[e, :, 1], 4)
+        np.testing.assert_allclose(S1.sum() / kk ** 2, area[e], rtol=1e-6)
+
+
+def test_precal_triangles_closed_forms(oracle_lib):
+    w = meshgen.topo_triangles(nx_nodes=30, nz_nodes=12, eanz=8)
+    o = oracle_lib.Oracle(w.mesh)
+    kk = 0.7
+    o.set_waven([0.0, kk], [1.0, 1.0])
+    o.precal()
+    m = w.mesh
+    elbg = o.elbg.reshape((m.elanz, o.nimn, 2), order="F")
+    x, z = m.node_xy(m.nrel[:m.elanz, :3])
+    area = 0.5 * ((x[:, 1] - x[:, 0]) * (z[:, 2] - z[:, 0]) - (x[:, 2] - x[:, 0]) * (z[:, 1] - z[:, 0]))
+    for e in range(0, m.elanz, 97):
+        S0 = _unpack(elbg[e, :, 0], 3)
+        np.testing.assert_allclose(S0.sum(1), 0.0, atol=1e-12)
+        M2 = (_unpack(elbg[e, :, 1], 3) - S0) / kk ** 2
+        np.testing.assert_allclose(M2.sum(), area[e], rtol=1e-12)
+        np.testing.assert_allclose(M2, area[e] / 12 * (np.ones((3, 3)) + np.eye(3)), rtol=1e-12)
+
+
+def test_beta_against_scipy_bessel(small):
+    """beta = k (cos_m K1(k r_m) + cos_p K1(k r_p)) / (K0(k r_m) + K0(k r_p)),
+    with the A&S polynomials accurate to ~2e-7 (sytop = 0)."""
+    w, o = small
+    m = w.mesh
+    kg = o.kg.reshape((m.relanz, m.eanz, o.K), order="F")
+    elanz = m.elanz
+    btyp = np.concatenate([np.full(n, t) for t, n in zip(m.typ, m.nelanz) if t > 10])
+    for rel in np.nonzero(btyp == 11)[0][::17]:
+        n1, n2 = m.nrel[elanz + rel, :2]
+        x1, y1 = m.node_xy(n1)
+        x2, y2 = m.node_xy(n2)
+        for l in (0, 7, 19):
+            xs, ys = m.node_xy(m.enr[l])
+            x3 = 0.5 * (x1 + x2) - xs
+            y3m = 0.5 * (y1 + y2) - ys
+            y3p = 0.5 * (y1 + y2) + ys
+            rm, rp = math.hypot(x3, y3m), math.hypot(x3, y3p)
+            x4, y4 = y1 - y2, x2 - x1
+            r2 = math.hypot(x4, y4)
+            cm = (x3 * x4 + y3m * y4) / rm / r2
+            cp = (x3 * x4 + y3p * y4) / rp / r2
+            for k in range(o.K):
+                kw = o.kwn[k]
+                exp = kw * (cm * sps.k1(kw * rm) + cp * sps.k1(kw * rp)) / (sps.k0(kw * rm) + sps.k0(kw * rp))
+                assert abs(kg[rel, l, k] - exp) <= 2e-6 * abs(kw) + 1e-12
+
+
+# -------------------------------------------------------- A4-A9 forward solve
+def _dense_from_band(a_band, n, mb):
+    A = np.zeros((n, n), complex)
+    for i in range(n):
+        for t in range(mb + 1):
+            j = i - mb + t
+            if j >= 0:
+                A[i, j] = A[j, i] = a_band[i * (mb + 1) + t]
+    return A
+
+
+def test_chol_reconstructs_matrix(small):
+    """L L^T (no conjugation) equals the scaled band matrix."""
+    w, o = small
+    n, mb = w.mesh.sanz, w.mesh.mb
+    a, b, fak = o.assemble(w.sigma, 4, 3)
+    A = _dense_from_band(a, n, mb)
+    Lb = o.chol(a)
+    L = np.tril(_dense_from_band(Lb, n, mb))
+    assert rel_err(L @ L.T, A) < 1e-13
+    # scaling: unit-modulus diagonal
+    np.testing.assert_allclose(np.abs(np.diag(A)), 1.0, rtol=1e-15)
+
+
+def test_forward_against_dense_solve(small):
+    """kpot(:,l,k) solves the unscaled system A x = e_l (A x + b = 0, b = -e_l)."""
+    w, o = small
+    n, mb = w.mesh.sanz, w.mesh.mb
+    for (l, k) in [(0, 0), (11, 6), (19, 9)]:
+        a, b, fak = o.assemble(w.sigma, l, k)
+        A = _dense_from_band(a, n, mb)
+        D = np.diag(1.0 / fak)
+        Au = D @ A @ D          # undo D A D
+        rhs = np.zeros(n, complex)
+        rhs[w.mesh.enr[l] - 1] = 1.0
+        x = np.linalg.solve(Au, rhs)
+        assert rel_err(o.kpot[:, l, k], x) < 1e-11
+
+
+def test_forward_matches_reference(small):
+    w, o = small
+    g = golden("small_block")
+    enr = w.mesh.enr - 1
+    np.testing.assert_array_equal(o.kpot[enr], g["kpot_elec"])
+    np.testing.assert_array_equal(o.kpot[:, 0, :], g["kpot_col"])
+    np.testing.assert_array_equal(o.hpot[enr], g["hpot_elec"])
+
+
+def test_forward_dirichlet_reuse_matches_reference(small_dir):
+    """No mixed boundaries: one factor per wavenumber (kompb), randb rows."""
+    w, o = small_dir
+    g = golden("small_dirichlet")
+    assert o.lbeta == 0 and o.lrandb == 1
+    enr = w.mesh.enr - 1
+    np.testing.assert_array_equal(o.kpot[enr], g["kpot_elec"])
+    np.testing.assert_array_equal(o.hpot[enr], g["hpot_elec"])
+
+
+# ------------------------------------------------------------- A11/A12 volt
+def test_volt_matches_reference(small):
+    w, o = small
+    g = golden("small_block")
+    v, sa = o.volt(w.strnr, w.vnr, 0)
+    np.testing.assert_array_equal(v, g["volt"])
+    np.testing.assert_array_equal(sa, g["sigmaa0"])
+    v1, sa1 = o.volt(w.strnr, w.vnr, 1)
+    assert rel_err(sa1, g["sigmaa1"]) < 1e-15
+
+
+def test_volt_reciprocity(small):
+    """Swapping current and potential dipoles leaves V unchanged (symmetric A,
+    away from the mixed-boundary terms' source dependence to ~1e-3)."""
+    w, o = small
+    st = np.array([10002, 40003])
+    vn = np.array([40003, 10002])
+    v, _ = o.volt(st, vn, 0)
+    assert abs(v[0] - v[1]) / abs(v[0]) < 5e-3
+
+
+@pytest.mark.parametrize("case", FIXTURE_CASES, ids=[c[0] for c in FIXTURE_CASES])
+def test_volt_against_reference_fixtures(oracle_lib, case):
+    """The reference's shipped volt.dat (FP32 text) and the full-precision
+    reference on the same inputs."""
+    name, swrtr, lsink, nsink = case
+    m, sig, st, vn, mag, pha = load_fixture(name)
+    o = oracle_lib.Oracle(m, lsink=lsink, nsink=nsink)
+    if swrtr:
+        o.rwaven(st, vn)
+    else:
+        o.set_waven([0.0], [1.0], swrtr=0)
+    o.precal()
+    o.forward(sig)
+    v, sa = o.volt(st, vn, 0)
+    bet, pm, vout = F.volt_fields(st, vn, sa)
+    tol = 1e-5 if name == "mixed_surf" else 2e-7
+    assert np.max(np.abs(bet - mag) / mag) < tol
+    assert np.max(np.abs(pm - pha)) < 1e-4
+    g = golden(name)
+    np.testing.assert_array_equal(v, g["volt"])
+
+
+# ------------------------------------------------------------- A14 sens
+def test_sens_matches_reference(small):
+    w, o = small
+    g = golden("small_block")
+    o.volt(w.strnr, w.vnr, 0)
+    s0 = o.sens(0)
+    np.testing.assert_array_equal(s0[:8], g["sens0"])
+    o.volt(w.strnr, w.vnr, 1)
+    s1 = o.sens(1)
+    assert rel_err(s1[:8], g["sens1"]) < 1e-14
+
+
+def test_sens_euler_rule_and_fd(small_dir, oracle_lib):
+    """Without mixed boundaries A is homogeneous of degree 1 in sigma, so
+    sum_e sigma_e dV/dsigma_e = -V; and dV/dsigma_e matches central differences."""
+    w, o = small_dir
+    v, _ = o.volt(w.strnr, w.vnr, 0)
+    s = o.sens(0)
+    lhs = s @ w.sigma
+    assert rel_err(lhs, -v) < 1e-10
+    # central differences on an element with a large sensitivity (the FD
+    # quotient of far elements is round-off dominated), real and imaginary steps
+    e = 100
+    o2 = oracle_lib.Oracle(w.mesh)
+    o2.set_waven(o.kwn, o.kwnwi)
+    o2.precal()
+    for dirn in (1.0, 1j):
+        h = 1e-4 * abs(w.sigma[e]) * dirn
+        sp = w.sigma.copy(); sp[e] += h
+        sm = w.sigma.copy(); sm[e] -= h
+        o2.forward(sp); vp, _ = o2.volt(w.strnr, w.vnr, 0)
+        o2.forward(sm); vm, _ = o2.volt(w.strnr, w.vnr, 0)
+        fd = (vp - vm) / (2 * h)
+        assert rel_err(fd, s[:, e]) < 1e-6
+
+
+# ------------------------------------------------------------- A16/A17 CG
+def _normal_data(w, o, seed=3):
+    rng = np.random.default_rng(seed)
+    v, sa = o.volt(w.strnr, w.vnr, 1)
+    J = o.sens(1)
+    N, M = J.shape
+    dat = sa + 0.01 * (rng.standard_normal(N) + 1j * rng.standard_normal(N))
+    wmatd = 1.0 / (0.02 + 1e-4 / np.abs(v)) ** 2 * 1e-4
+    wdfak = np.ones(N, np.int32)
+    return J, dat, sa, wmatd, wdfak
+
+
+def test_cg_converges_to_normal_equations(small, oracle_lib):
+    w, o = small
+    J, dat, sa, wmatd, wdfak = _normal_data(w, o)
+    N, M = J.shape
+    sm = oracle_lib.bsmatmreg(o, 40, 20, 1.0, 1.0)
+    lam = 10.0
+    par = np.zeros(M, complex)
+    rm = oracle_lib.rm_reg(sm, 40, par)
+    cgfac, bvec = oracle_lib.update_rhs(J, wmatd, wdfak, dat, sa, lam, 0, sm[:, 0], rm)
+    dpar, cgres, ncg = oracle_lib.cjggra(J, wmatd, wdfak, cgfac, bvec, lam, 0, 40, sm, 4000, 1e-30)
+    # dense normal equations of the scaled system: (C (J^H W J + lam R) C) y = bvec
+    W = wmatd * wdfak
+    R = np.diag(sm[:, 0]).astype(complex)
+    for i in range(M):
+        if i + 1 < M:
+            R[i, i + 1] = R[i + 1, i] = sm[i, 1]
+        if i + 40 < M:
+            R[i, i + 40] = R[i + 40, i] = sm[i, 2]
+    Cm = np.diag(cgfac)
+    Amat = Cm @ (J.conj().T @ (W[:, None] * J) + lam * R) @ Cm
+    y = np.linalg.solve(Amat, bvec)
+    assert rel_err(dpar, y) < 1e-6
+    # preconditioner: unit diagonal of the scaled operator
+    np.testing.assert_allclose(np.real(np.diag(Amat)), 1.0, rtol=1e-12)
+
+
+def test_smoothness_operator_rows_sum_to_zero(small, oracle_lib):
+    """R = Wx^T Wx + Wz^T Wz annihilates constant models."""
+    w, o = small
+    sm = oracle_lib.bsmatmreg(o, 40, 20, 2.0, 0.5)
+    one = np.ones(o.elanz, complex)
+    np.testing.assert_allclose(oracle_lib.rm_reg(sm, 40, one), 0.0, atol=1e-12)

@@ -1,0 +1,1029 @@
+// Host side of the C-ABI (include/crt_hip.h): mesh plan, wavenumbers, device
+// buffers, kernel launches and timing.  One HIP stream per handle.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "../../include/crt_hip.h"
+#include "crt_kernels.h"
+#include "crt_post.h"
+
+namespace {
+
+constexpr int kErrArg = 1000;
+constexpr int kErrHip = 1001;
+constexpr int kErrOrder = 1002;
+constexpr int kErrBand = 1003;
+constexpr int kNB = 16;
+
+struct Timer {
+  std::string name;
+  hipEvent_t a, b;
+};
+
+}  // namespace
+
+struct crt_handle {
+  // mesh (host copies, reference layout)
+  int n = 0, mb = 0, E = 0, typanz = 0, elanz = 0, relanz = 0, maxnodes = 0, nimn = 0, nel = 0;
+  std::vector<int> typ, nelanz, selanz, nrel, rnr, snr, enr;
+  std::vector<double> sx, sy;
+  int lsytop = 0, lsink = 0, nsink = 0;
+  bool has11 = false, has13 = false;
+  double sytop = 0.0;
+  // wavenumbers
+  int K = 0, swrtr = 1;
+  std::vector<double> kwn, kwnwi;
+  // stage state
+  bool precal_done = false, forward_done = false, volt_done = false, sens_done = false;
+  int sens_mode = -1;
+  int nanz = 0;
+  std::vector<int> strnr, vnr;
+  // device
+  hipStream_t stream = nullptr;
+  int ncu = 0;
+  size_t lds_max = 0;
+  hipError_t hip_err = hipSuccess;
+  std::map<std::string, void*> buf;
+  std::map<std::string, size_t> bufsz;
+  int slots = 0;
+  int last_err = 0, last_k = 0, last_l = 0;
+  // timing
+  bool timing = false;
+  std::vector<Timer> pending;
+  std::map<std::string, std::pair<double, int>> times;
+
+  template <class T>
+  T* get(const std::string& name, size_t count) {
+    size_t bytes = std::max<size_t>(count * sizeof(T), 16);
+    auto it = buf.find(name);
+    if (it != buf.end() && bufsz[name] >= bytes) return static_cast<T*>(it->second);
+    if (it != buf.end()) {
+      hipFree(it->second);
+      buf.erase(it);
+    }
+    void* p = nullptr;
+    if (hipMalloc(&p, bytes) != hipSuccess) return nullptr;
+    buf[name] = p;
+    bufsz[name] = bytes;
+    return static_cast<T*>(p);
+  }
+  template <class T>
+  T* at(const std::string& name) {
+    auto it = buf.find(name);
+    return it == buf.end() ? nullptr : static_cast<T*>(it->second);
+  }
+};
+
+namespace {
+
+#define HIPCHK(x)                          \
+  do {                                     \
+    hipError_t e_ = (x);                   \
+    if (e_ != hipSuccess) {                \
+      h->hip_err = e_;                     \
+      return kErrHip;                      \
+    }                                      \
+  } while (0)
+
+#define NEED(ptr)                 \
+  do {                            \
+    if (!(ptr)) return kErrHip;   \
+  } while (0)
+
+void tstart(crt_handle* h, const char* name) {
+  if (!h->timing) return;
+  Timer t;
+  t.name = name;
+  hipEventCreate(&t.a);
+  hipEventCreate(&t.b);
+  hipEventRecord(t.a, h->stream);
+  h->pending.push_back(t);
+}
+void tstop(crt_handle* h) {
+  if (!h->timing || h->pending.empty()) return;
+  hipEventRecord(h->pending.back().b, h->stream);
+}
+void tcollect(crt_handle* h) {
+  for (auto& t : h->pending) {
+    float ms = 0.f;
+    hipEventSynchronize(t.b);
+    hipEventElapsedTime(&ms, t.a, t.b);
+    auto& slot = h->times[t.name];
+    slot.first += ms;
+    slot.second += 1;
+    hipEventDestroy(t.a);
+    hipEventDestroy(t.b);
+  }
+  h->pending.clear();
+}
+
+int finish(crt_handle* h) {
+  hipError_t e = hipStreamSynchronize(h->stream);
+  tcollect(h);
+  if (e != hipSuccess) {
+    h->hip_err = e;
+    return kErrHip;
+  }
+  e = hipGetLastError();
+  if (e != hipSuccess) {
+    h->hip_err = e;
+    return kErrHip;
+  }
+  return 0;
+}
+
+int read_err(crt_handle* h) {
+  int herr[2] = {0, 0};
+  int* d = h->at<int>("err");
+  if (hipMemcpyAsync(herr, d, sizeof(herr), hipMemcpyDeviceToHost, h->stream) != hipSuccess) return kErrHip;
+  if (hipStreamSynchronize(h->stream) != hipSuccess) return kErrHip;
+  if (herr[0]) {
+    h->last_err = herr[0];
+    if (h->E > 0) {
+      h->last_k = herr[1] / h->E + 1;
+      h->last_l = herr[1] % h->E + 1;
+    }
+  }
+  return herr[0];
+}
+
+int clear_err(crt_handle* h) {
+  int* d = h->get<int>("err", 2);
+  NEED(d);
+  HIPCHK(hipMemsetAsync(d, 0, 2 * sizeof(int), h->stream));
+  return 0;
+}
+
+template <class T>
+int upload(crt_handle* h, const std::string& name, const T* src, size_t count, T** out) {
+  T* d = h->get<T>(name, count);
+  NEED(d);
+  if (count) HIPCHK(hipMemcpyAsync(d, src, count * sizeof(T), hipMemcpyHostToDevice, h->stream));
+  *out = d;
+  return 0;
+}
+
+// ---- wavenumbers (src/rwaven.f90, gauleg.f90, gaulag.f90, gammln.f90) ----
+double gammln(double xx) {
+  static const double cof[6] = {76.18009172947146, -86.50532032941677, 24.01409824083091,
+                                -1.231739572450155, .1208650973866179e-2, -.5395239384953e-5};
+  const double stp = 2.5066282746310005;
+  double x = xx, y = x, tmp = x + 5.5;
+  tmp = (x + 0.5) * std::log(tmp) - tmp;
+  double ser = 1.000000000190015;
+  for (int j = 0; j < 6; ++j) {
+    y = y + 1.0;
+    ser = ser + cof[j] / y;
+  }
+  return tmp + std::log(stp * ser / x);
+}
+
+void gauss_legendre(double x1, double x2, double* x, double* w, int n) {
+  const double eps = 3.0e-14;
+  const int m = (n + 1) / 2;
+  const double xm = 0.5 * (x2 + x1), xl = 0.5 * (x2 - x1);
+  for (int i = 1; i <= m; ++i) {
+    double z = std::cos(3.141592653589793 * ((double)i - .25) / ((double)n + .5));
+    double z1, p1, p2, p3, pp;
+    do {
+      p1 = 1.0;
+      p2 = 0.0;
+      for (int j = 1; j <= n; ++j) {
+        p3 = p2;
+        p2 = p1;
+        p1 = ((2.0 * (double)j - 1.0) * z * p2 - ((double)j - 1.0) * p3) / (double)j;
+      }
+      pp = (double)n * (z * p1 - p2) / (z * z - 1.0);
+      z1 = z;
+      z = z1 - p1 / pp;
+    } while (std::fabs(z - z1) > eps);
+    x[i - 1] = xm - xl * z;
+    x[n - i] = xm + xl * z;
+    w[i - 1] = 2.0 * xl / ((1.0 - z * z) * pp * pp);
+    w[n - i] = w[i - 1];
+  }
+}
+
+void gauss_laguerre(double* x, double* w, int n, double alf) {
+  const double eps = 3.0e-14;
+  double z = 0.0, z1, p1 = 0, p2 = 0, p3, pp = 0;
+  for (int i = 1; i <= n; ++i) {
+    if (i == 1) {
+      z = (1.0 + alf) * (3.0 + .92 * alf) / (1.0 + 2.4 * n + 1.8 * alf);
+    } else if (i == 2) {
+      z = z + (15.0 + 6.25 * alf) / (1.0 + .9 * alf + 2.5 * n);
+    } else {
+      double ai = (double)(i - 2);
+      z = z + ((1.0 + 2.55 * ai) / (1.9 * ai) + 1.26 * ai * alf / (1.0 + 3.5 * ai)) * (z - x[i - 3]) /
+                  (1.0 + .3 * alf);
+    }
+    for (int its = 1; its <= 10; ++its) {
+      p1 = 1.0;
+      p2 = 0.0;
+      for (int j = 1; j <= n; ++j) {
+        p3 = p2;
+        p2 = p1;
+        p1 = (((double)(2 * j - 1) + alf - z) * p2 - ((double)(j - 1) + alf) * p3) / (double)j;
+      }
+      pp = ((double)n * p1 - ((double)n + alf) * p2) / z;
+      z1 = z;
+      z = z1 - p1 / pp;
+      if (std::fabs(z - z1) <= eps) break;
+    }
+    x[i - 1] = z;
+    w[i - 1] = -std::exp(gammln(alf + (double)n) - gammln((double)n)) / (pp * (double)n * p2);
+  }
+}
+
+double node_x(const crt_handle* h, int node1) { return h->sx[h->snr[node1 - 1] - 1]; }
+double node_y(const crt_handle* h, int node1) { return h->sy[h->snr[node1 - 1] - 1]; }
+
+}  // namespace
+
+extern "C" {
+
+int crt_create(crt_handle** out, const crt_mesh* m) {
+  if (!out || !m || m->sanz <= 0 || m->typanz <= 0 || m->mb <= 0 || m->eanz < 0) return kErrArg;
+  crt_handle* h = new crt_handle();
+  h->n = m->sanz;
+  h->mb = m->mb;
+  h->E = m->eanz;
+  h->typanz = m->typanz;
+  h->typ.assign(m->typ, m->typ + m->typanz);
+  h->nelanz.assign(m->nelanz, m->nelanz + m->typanz);
+  h->selanz.assign(m->selanz, m->selanz + m->typanz);
+  bool seen_bnd = false;
+  for (int i = 0; i < h->typanz; ++i) {
+    const int t = h->typ[i];
+    if (t > 10) {
+      seen_bnd = true;
+      h->relanz += h->nelanz[i];
+      if (t == 11) h->has11 = true;
+      if (t == 13) h->has13 = true;
+      if (h->selanz[i] != 2 || (t != 11 && t != 12 && t != 13)) { delete h; return kErrArg; }
+    } else {
+      if (seen_bnd) { delete h; return kErrArg; }  // area elements must precede boundaries
+      if (!((t == 3 && h->selanz[i] == 3) || ((t == 5 || t == 8) && h->selanz[i] == 4))) {
+        delete h;
+        return 18;
+      }
+      h->elanz += h->nelanz[i];
+    }
+    h->maxnodes = std::max(h->maxnodes, h->selanz[i]);
+  }
+  h->nel = h->elanz + h->relanz;
+  h->nimn = h->maxnodes * (h->maxnodes + 1) / 2;
+  h->nrel.assign(m->nrel, m->nrel + (size_t)h->nel * h->maxnodes);
+  h->rnr.assign(m->rnr, m->rnr + h->relanz);
+  h->snr.assign(m->snr, m->snr + h->n);
+  h->sx.assign(m->sx, m->sx + h->n);
+  h->sy.assign(m->sy, m->sy + h->n);
+  h->enr.assign(m->enr, m->enr + h->E);
+  h->lsytop = m->lsytop;
+  h->lsink = m->lsink;
+  h->nsink = m->nsink;
+  for (int v : h->snr)
+    if (v < 1 || v > h->n) { delete h; return kErrArg; }
+  for (int v : h->enr)
+    if (v < 1 || v > h->n) { delete h; return 29; }  // relectr errnr 29
+  for (int v : h->rnr)
+    if (v < 1 || v > h->elanz) { delete h; return 111; }
+  for (size_t i = 0; i < h->nrel.size(); ++i) {
+    const int iel = (int)(i % h->nel), q = (int)(i / h->nel);
+    int ty = 0, nk = 0, acc = 0;
+    for (int t = 0; t < h->typanz; ++t) {
+      if (iel < acc + h->nelanz[t]) { ty = h->typ[t]; nk = h->selanz[t]; break; }
+      acc += h->nelanz[t];
+    }
+    (void)ty;
+    if (q < nk && (h->nrel[i] < 1 || h->nrel[i] > h->n)) { delete h; return kErrArg; }
+  }
+  if (h->lsink && (h->nsink < 1 || h->nsink > h->n)) { delete h; return kErrArg; }
+
+  int dev = 0;
+  hipDeviceProp_t prop;
+  if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess) {
+    delete h;
+    return kErrHip;
+  }
+  h->ncu = prop.multiProcessorCount;
+  h->lds_max = prop.sharedMemPerBlock;
+  if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete h;
+    return kErrHip;
+  }
+
+  // ---- row-assembly plan (kompab's accumulation, src/kompab.f90:79-129) ----
+  struct Term {
+    int row, t, e, imn;
+  };
+  std::vector<Term> terms;
+  terms.reserve((size_t)h->nel * 10);
+  std::vector<uint8_t> isdir(h->n, 0);
+  int iel = 0;
+  for (int ti = 0; ti < h->typanz; ++ti) {
+    const int ty = h->typ[ti], nk = h->selanz[ti];
+    for (int j = 0; j < h->nelanz[ti]; ++j, ++iel) {
+      if (ty == 13) {
+        for (int q = 0; q < nk; ++q) isdir[h->nrel[iel + (size_t)q * h->nel] - 1] = 1;
+      }
+      if (ty > 11) continue;
+      int ikl = 0;
+      for (int kk = 0; kk < nk; ++kk) {
+        const int nzp = h->nrel[iel + (size_t)kk * h->nel];
+        for (int ll = 0; ll <= kk; ++ll, ++ikl) {
+          const int nnp = h->nrel[iel + (size_t)ll * h->nel];
+          if (std::abs(nzp - nnp) > h->mb) {
+            crt_destroy(h);
+            return 19;
+          }
+          const int row = std::max(nzp, nnp) - 1, col = std::min(nzp, nnp) - 1;
+          const int code = (ty == 11) ? -((iel - h->elanz) + 1) : iel;
+          terms.push_back({row, col - row + h->mb, code, ikl});
+        }
+      }
+    }
+  }
+  if (h->has13) {
+    std::vector<Term> kept;
+    kept.reserve(terms.size());
+    for (const Term& t : terms) {
+      const int col = t.row - h->mb + t.t;
+      if (isdir[t.row] || isdir[col]) continue;
+      kept.push_back(t);
+    }
+    for (int r = 0; r < h->n; ++r)
+      if (isdir[r]) kept.push_back({r, h->mb, kDirichletUnit, 0});
+    terms.swap(kept);
+  }
+  std::stable_sort(terms.begin(), terms.end(), [](const Term& a, const Term& b) {
+    return a.row != b.row ? a.row < b.row : a.t < b.t;
+  });
+  std::vector<int> rowptr(h->n + 1, 0), te(terms.size());
+  std::vector<int16_t> tt(terms.size());
+  std::vector<int8_t> timn(terms.size());
+  for (size_t p = 0; p < terms.size(); ++p) {
+    rowptr[terms[p].row + 1]++;
+    tt[p] = (int16_t)terms[p].t;
+    timn[p] = (int8_t)terms[p].imn;
+    te[p] = terms[p].e;
+  }
+  for (int r = 0; r < h->n; ++r) rowptr[r + 1] += rowptr[r];
+
+  // ---- element / boundary / node arrays for the kernels ----
+  std::vector<int> el_type(h->elanz), el_nodes((size_t)h->elanz * 4), el_nk(h->elanz);
+  std::vector<int> bnd_type(std::max(h->relanz, 1)), bnd_nodes((size_t)std::max(h->relanz, 1) * 2);
+  iel = 0;
+  for (int ti = 0; ti < h->typanz; ++ti) {
+    const int ty = h->typ[ti], nk = h->selanz[ti];
+    for (int j = 0; j < h->nelanz[ti]; ++j, ++iel) {
+      if (ty <= 10) {
+        el_type[iel] = ty;
+        el_nk[iel] = nk;
+        for (int q = 0; q < 4; ++q) el_nodes[(size_t)iel * 4 + q] = h->nrel[iel + (size_t)std::min(q, nk - 1) * h->nel] - 1;
+      } else {
+        const int rel = iel - h->elanz;
+        bnd_type[rel] = ty;
+        bnd_nodes[2 * rel] = h->nrel[iel] - 1;
+        bnd_nodes[2 * rel + 1] = h->nrel[iel + (size_t)h->nel] - 1;
+      }
+    }
+  }
+  std::vector<double> nx(h->n), ny(h->n);
+  for (int j = 0; j < h->n; ++j) {
+    nx[j] = node_x(h, j + 1);
+    ny[j] = node_y(h, j + 1);
+  }
+  std::vector<int> enr0(std::max(h->E, 1)), rnr0(std::max(h->relanz, 1));
+  for (int i = 0; i < h->E; ++i) enr0[i] = h->enr[i] - 1;
+  for (int i = 0; i < h->relanz; ++i) rnr0[i] = h->rnr[i] - 1;
+
+  int* di;
+  int16_t* d16;
+  int8_t* d8;
+  double* dd;
+  uint8_t* du;
+  int rc = 0;
+  rc |= upload(h, "rowptr", rowptr.data(), rowptr.size(), &di);
+  rc |= upload(h, "term_t", tt.data(), tt.size(), &d16);
+  rc |= upload(h, "term_imn", timn.data(), timn.size(), &d8);
+  rc |= upload(h, "term_e", te.data(), te.size(), &di);
+  rc |= upload(h, "el_type", el_type.data(), el_type.size(), &di);
+  rc |= upload(h, "el_nodes", el_nodes.data(), el_nodes.size(), &di);
+  rc |= upload(h, "el_nk", el_nk.data(), el_nk.size(), &di);
+  rc |= upload(h, "bnd_type", bnd_type.data(), bnd_type.size(), &di);
+  rc |= upload(h, "bnd_nodes", bnd_nodes.data(), bnd_nodes.size(), &di);
+  rc |= upload(h, "nx", nx.data(), nx.size(), &dd);
+  rc |= upload(h, "ny", ny.data(), ny.size(), &dd);
+  rc |= upload(h, "enr", enr0.data(), enr0.size(), &di);
+  rc |= upload(h, "rnr", rnr0.data(), rnr0.size(), &di);
+  if (h->has13) rc |= upload(h, "isdir", isdir.data(), isdir.size(), &du);
+  rc |= clear_err(h);
+  if (rc || hipStreamSynchronize(h->stream) != hipSuccess) {
+    crt_destroy(h);
+    return kErrHip;
+  }
+  // bsytop (src/bsytop.f90:35-69): running mean of the no-flow element centres
+  if (h->lsytop) {
+    double st = 0.0;
+    iel = 0;
+    for (int ti = 0; ti < h->typanz; ++ti) {
+      if (h->typ[ti] == 12) {
+        const int nk = h->selanz[ti];
+        st = 0.0;
+        for (int j = 1; j <= h->nelanz[ti]; ++j, ++iel) {
+          double sp = 0.0;
+          for (int q = 0; q < nk; ++q) sp = sp + node_y(h, h->nrel[iel + (size_t)q * h->nel]);
+          sp = sp / (double)nk;
+          st = (j == 1) ? sp : (st * (double)(j - 1) + sp) / (double)j;
+        }
+      } else {
+        iel += h->nelanz[ti];
+      }
+    }
+    h->sytop = st;
+  }
+  *out = h;
+  return 0;
+}
+
+int crt_destroy(crt_handle* h) {
+  if (!h) return 0;
+  if (h->stream) hipStreamSynchronize(h->stream);
+  for (auto& kv : h->buf) hipFree(kv.second);
+  for (auto& t : h->pending) {
+    hipEventDestroy(t.a);
+    hipEventDestroy(t.b);
+  }
+  if (h->stream) hipStreamDestroy(h->stream);
+  delete h;
+  return 0;
+}
+
+int crt_rwaven(crt_handle* h, int32_t nanz, const int32_t* strnr, const int32_t* vnr, int32_t* kwnanz,
+               double* kwn, double* kwnwi) {
+  if (!h || nanz <= 0 || !strnr || !vnr) return kErrArg;
+  double amin = 1e9, amax = 0.0;
+  double xke[4] = {0, 0, 0, 0}, yke[4] = {0, 0, 0, 0};
+  for (int i = 0; i < nanz; ++i) {
+    int el[4];
+    el[0] = strnr[i] % 10000;
+    el[1] = (strnr[i] - el[0]) / 10000;
+    el[2] = vnr[i] % 10000;
+    el[3] = (vnr[i] - el[2]) / 10000;
+    for (int j = 0; j < 4; ++j) {
+      if (el[j] > h->E || el[j] < 0) return 46;
+      if (el[j] > 0) {
+        xke[j] = node_x(h, h->enr[el[j] - 1]);
+        yke[j] = node_y(h, h->enr[el[j] - 1]);
+      }
+    }
+    for (int q = 2; q < 4; ++q) {
+      if (el[q] <= 0) continue;
+      for (int j = 0; j < 2; ++j) {
+        if (el[j] > 0) {
+          const double x21 = xke[q] - xke[j], y21 = yke[q] - yke[j];
+          const double dum = std::sqrt(x21 * x21 + y21 * y21);
+          amin = std::fmin(dum, amin);
+          amax = std::fmax(dum, amax);
+        }
+      }
+    }
+  }
+  if (amin < 1e-12 || amax < 1e-12) return 73;
+  const int lanz = 4;
+  int ganz = (int)(float)(6.0 * std::log10(amax / amin));  // int(real(...)), single precision
+  ganz = std::max(ganz, 2);
+  ganz = std::min(ganz, 30 - lanz);
+  const int K = ganz + lanz;
+  const double kwn0 = 1.0 / (2.0 * amin);
+  std::vector<double> x(30, 0.0), w(30, 0.0);
+  gauss_legendre(0.0, 1.0, x.data(), w.data(), ganz);
+  for (int i = 0; i < ganz; ++i) {
+    w[i] = 2.0 * kwn0 * w[i] * x[i];
+    x[i] = kwn0 * x[i] * x[i];
+  }
+  gauss_laguerre(x.data() + ganz, w.data() + ganz, lanz, 0.0);
+  for (int i = ganz; i < K; ++i) {
+    w[i] = kwn0 * w[i] * std::exp(x[i]);
+    x[i] = kwn0 * (x[i] + 1.0);
+  }
+  h->K = K;
+  h->swrtr = 1;
+  h->kwn.assign(x.begin(), x.begin() + K);
+  h->kwnwi.assign(w.begin(), w.begin() + K);
+  h->precal_done = h->forward_done = h->volt_done = h->sens_done = false;
+  if (kwnanz) *kwnanz = K;
+  if (kwn) std::copy(h->kwn.begin(), h->kwn.end(), kwn);
+  if (kwnwi) std::copy(h->kwnwi.begin(), h->kwnwi.end(), kwnwi);
+  return 0;
+}
+
+int crt_set_wavenumbers(crt_handle* h, int32_t kwnanz, const double* kwn, const double* kwnwi, int32_t swrtr) {
+  if (!h || kwnanz <= 0 || kwnanz > 30 || !kwn || !kwnwi) return kErrArg;
+  h->K = kwnanz;
+  h->swrtr = swrtr ? 1 : 0;
+  h->kwn.assign(kwn, kwn + kwnanz);
+  h->kwnwi.assign(kwnwi, kwnwi + kwnanz);
+  h->precal_done = h->forward_done = h->volt_done = h->sens_done = false;
+  return 0;
+}
+
+int crt_precal(crt_handle* h, int32_t* lbeta, int32_t* lrandb) {
+  if (!h) return kErrArg;
+  if (h->K <= 0) return kErrOrder;
+  if (h->has11 && h->has13) return 101;
+  double *dkwn, *dkwnwi;
+  int rc = upload(h, "kwn", h->kwn.data(), h->K, &dkwn);
+  rc |= upload(h, "kwnwi", h->kwnwi.data(), h->K, &dkwnwi);
+  if (rc) return kErrHip;
+  PrecalArgs P;
+  P.elanz = h->elanz;
+  P.relanz = h->relanz;
+  P.eanz = h->E;
+  P.K = h->K;
+  P.nimn = h->nimn;
+  P.el_type = h->at<int>("el_type");
+  P.el_nodes = h->at<int>("el_nodes");
+  P.bnd_type = h->at<int>("bnd_type");
+  P.bnd_nodes = h->at<int>("bnd_nodes");
+  P.nx = h->at<double>("nx");
+  P.ny = h->at<double>("ny");
+  P.enr = h->at<int>("enr");
+  P.kwn = dkwn;
+  P.sytop = h->sytop;
+  P.elbg = h->get<double>("elbg", (size_t)h->K * h->elanz * h->nimn);
+  P.relbg = h->get<double>("relbg", (size_t)std::max(h->relanz, 1) * h->nimn);
+  P.kg = h->get<double>("kg", (size_t)h->K * h->E * std::max(h->relanz, 1));
+  NEED(P.elbg);
+  NEED(P.relbg);
+  NEED(P.kg);
+  if (clear_err(h)) return kErrHip;
+  P.err = h->at<int>("err");
+  HIPCHK(hipMemsetAsync(P.relbg, 0, sizeof(double) * std::max(h->relanz, 1) * h->nimn, h->stream));
+  const int nt = 128;
+  tstart(h, "precal_area");
+  long na = (long)h->elanz * h->K;
+  hipLaunchKernelGGL(crt_precal_area_kernel, dim3((unsigned)((na + nt - 1) / nt)), dim3(nt), 0, h->stream, P);
+  tstop(h);
+  long nb = (long)h->relanz * h->E * h->K;
+  if (nb > 0) {
+    tstart(h, "precal_bnd");
+    hipLaunchKernelGGL(crt_precal_bnd_kernel, dim3((unsigned)((nb + nt - 1) / nt)), dim3(nt), 0, h->stream, P);
+    tstop(h);
+  }
+  int rc2 = finish(h);
+  if (rc2) return rc2;
+  int e = read_err(h);
+  if (e) return e;
+  h->precal_done = true;
+  h->forward_done = h->volt_done = h->sens_done = false;
+  if (lbeta) *lbeta = h->has11 ? 1 : 0;
+  if (lrandb) *lrandb = h->has13 ? 1 : 0;
+  return 0;
+}
+
+int crt_get_precal(crt_handle* h, double* elbg, double* relbg, double* kg) {
+  if (!h) return kErrArg;
+  if (!h->precal_done) return kErrOrder;
+  const int K = h->K, el = h->elanz, rl = h->relanz, nimn = h->nimn, E = h->E;
+  if (elbg) {
+    std::vector<double> t((size_t)K * el * nimn);
+    HIPCHK(hipMemcpy(t.data(), h->at<double>("elbg"), t.size() * 8, hipMemcpyDeviceToHost));
+    for (int k = 0; k < K; ++k)
+      for (int e = 0; e < el; ++e)
+        for (int q = 0; q < nimn; ++q) elbg[e + (size_t)el * (q + (size_t)nimn * k)] = t[((size_t)k * el + e) * nimn + q];
+  }
+  if (relbg && rl > 0) {
+    std::vector<double> t((size_t)rl * nimn);
+    HIPCHK(hipMemcpy(t.data(), h->at<double>("relbg"), t.size() * 8, hipMemcpyDeviceToHost));
+    for (int r = 0; r < rl; ++r)
+      for (int q = 0; q < nimn; ++q) relbg[r + (size_t)rl * q] = t[(size_t)r * nimn + q];
+  }
+  if (kg && rl > 0) {
+    // device layout [K][E][relanz] is the reference's kg(relanz,eanz,kwnanz)
+    HIPCHK(hipMemcpy(kg, h->at<double>("kg"), (size_t)K * E * rl * 8, hipMemcpyDeviceToHost));
+  }
+  return 0;
+}
+
+static int run_forward(crt_handle* h) {
+  const int n = h->n, mb = h->mb, E = h->E, K = h->K;
+  const size_t lds = crt_factor_lds_bytes(mb, kNB);
+  if (lds > h->lds_max) return kErrBand;
+  if (mb > 124) return kErrBand;  // two 4x4 trailing tiles per thread
+  auto kern = crt_factor_kernel<kNB>;
+  hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  int per_cu = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, 256, lds) != hipSuccess || per_cu < 1) per_cu = 1;
+  const int nsys = K * E;
+  h->slots = std::min(nsys, h->ncu * per_cu);
+  FactorArgs A;
+  A.n = n;
+  A.mb = mb;
+  A.eanz = E;
+  A.elanz = h->elanz;
+  A.relanz = h->relanz;
+  A.nimn = h->nimn;
+  A.nsys = nsys;
+  A.sys_list = nullptr;
+  A.rowptr = h->at<int>("rowptr");
+  A.term_t = h->at<int16_t>("term_t");
+  A.term_imn = h->at<int8_t>("term_imn");
+  A.term_e = h->at<int>("term_e");
+  A.elbg = h->at<double>("elbg");
+  A.relbg = h->at<double>("relbg");
+  A.kg = h->at<double>("kg");
+  A.sigma = h->at<double2>("sigma");
+  A.rnr = h->at<int>("rnr");
+  A.enr = h->at<int>("enr");
+  A.isdir = h->has13 ? h->at<uint8_t>("isdir") : nullptr;
+  A.lsink = h->lsink;
+  A.nsink = h->nsink - 1;
+  A.L = h->get<double2>("L", (size_t)h->slots * n * (mb + 1));
+  A.y = h->get<double2>("y", (size_t)h->slots * n);
+  A.fak = h->get<double>("fak", (size_t)h->slots * n);
+  A.kpot = h->get<double2>("kpot", (size_t)K * E * n);
+  NEED(A.L);
+  NEED(A.y);
+  NEED(A.fak);
+  NEED(A.kpot);
+  if (clear_err(h)) return kErrHip;
+  A.err = h->at<int>("err");
+  tstart(h, "factor_solve");
+  hipLaunchKernelGGL(kern, dim3(h->slots), dim3(256), lds, h->stream, A);
+  tstop(h);
+  // electrode slice of the back-transform, used by the voltage stage
+  double2* he = h->get<double2>("he", (size_t)E * E);
+  NEED(he);
+  tstart(h, "rtrafo_elec");
+  hipLaunchKernelGGL(crt_rtrafo_elec_kernel, dim3((E * E + 255) / 256), dim3(256), 0, h->stream,
+                     (const double2*)A.kpot, (const double*)h->at<double>("kwnwi"), (const int*)A.enr, n, E, K,
+                     h->swrtr, he);
+  tstop(h);
+  int rc = finish(h);
+  if (rc) return rc;
+  int e = read_err(h);
+  if (e) return e;
+  h->forward_done = true;
+  h->volt_done = h->sens_done = false;
+  return 0;
+}
+
+int crt_forward(crt_handle* h, const double* sigma, double* kpot, double* hpot) {
+  if (!h || !sigma) return kErrArg;
+  if (!h->precal_done) return kErrOrder;
+  double2* ds;
+  if (upload(h, "sigma", reinterpret_cast<const double2*>(sigma), h->elanz, &ds)) return kErrHip;
+  int rc = run_forward(h);
+  if (rc) return rc;
+  const int n = h->n, E = h->E, K = h->K;
+  if (kpot) HIPCHK(hipMemcpy(kpot, h->at<double2>("kpot"), (size_t)K * E * n * 16, hipMemcpyDeviceToHost));
+  if (hpot) {
+    double2* dh = h->get<double2>("hpot", (size_t)E * n);
+    NEED(dh);
+    tstart(h, "rtrafo");
+    hipLaunchKernelGGL(crt_rtrafo_kernel, dim3((unsigned)(((size_t)n * E + 255) / 256)), dim3(256), 0, h->stream,
+                       (const double2*)h->at<double2>("kpot"), (const double*)h->at<double>("kwnwi"), n, E, K,
+                       h->swrtr, dh);
+    tstop(h);
+    rc = finish(h);
+    if (rc) return rc;
+    HIPCHK(hipMemcpy(hpot, dh, (size_t)E * n * 16, hipMemcpyDeviceToHost));
+  }
+  return 0;
+}
+
+int crt_forward_dev(crt_handle* h, const double* sigma_dev) {
+  if (!h || !sigma_dev) return kErrArg;
+  if (!h->precal_done) return kErrOrder;
+  double2* ds = h->get<double2>("sigma", h->elanz);
+  NEED(ds);
+  HIPCHK(hipMemcpyAsync(ds, sigma_dev, (size_t)h->elanz * 16, hipMemcpyDeviceToDevice, h->stream));
+  return run_forward(h);
+}
+
+static int run_voltages(crt_handle* h, int32_t nanz, const int32_t* strnr, const int32_t* vnr, int32_t mode,
+                        const double* kfak) {
+  if (nanz <= 0 || !strnr || !vnr) return kErrArg;
+  if (!h->forward_done) return kErrOrder;
+  for (int i = 0; i < nanz; ++i) {
+    const int e1 = strnr[i] % 10000, e2 = strnr[i] / 10000, e3 = vnr[i] % 10000, e4 = vnr[i] / 10000;
+    if (e1 < 0 || e1 > h->E || e2 < 0 || e2 > h->E || e3 < 0 || e3 > h->E || e4 < 0 || e4 > h->E) return 46;
+  }
+  h->nanz = nanz;
+  h->strnr.assign(strnr, strnr + nanz);
+  h->vnr.assign(vnr, vnr + nanz);
+  int *ds, *dv;
+  double* dk = nullptr;
+  int rc = upload(h, "strnr", strnr, nanz, &ds);
+  rc |= upload(h, "vnr", vnr, nanz, &dv);
+  if (kfak) rc |= upload(h, "kfak", kfak, nanz, &dk);
+  if (rc) return kErrHip;
+  double2* vo = h->get<double2>("volt", nanz);
+  double2* sa = h->get<double2>("sigmaa", nanz);
+  NEED(vo);
+  NEED(sa);
+  if (clear_err(h)) return kErrHip;
+  tstart(h, "volt");
+  hipLaunchKernelGGL(crt_volt_kernel, dim3((nanz + 255) / 256), dim3(256), 0, h->stream,
+                     (const double2*)h->at<double2>("he"), h->E, nanz, (const int*)ds, (const int*)dv, mode,
+                     (const double*)dk, vo, sa, h->at<int>("err"));
+  tstop(h);
+  rc = finish(h);
+  if (rc) return rc;
+  int e = read_err(h);
+  if (e) return e;
+  h->volt_done = true;
+  h->sens_done = false;
+  return 0;
+}
+
+int crt_voltages(crt_handle* h, int32_t nanz, const int32_t* strnr, const int32_t* vnr, int32_t mode,
+                 const double* kfak, double* volt, double* sigmaa) {
+  if (!h) return kErrArg;
+  int rc = run_voltages(h, nanz, strnr, vnr, mode, kfak);
+  if (rc) return rc;
+  if (volt) HIPCHK(hipMemcpy(volt, h->at<double2>("volt"), (size_t)nanz * 16, hipMemcpyDeviceToHost));
+  if (sigmaa) HIPCHK(hipMemcpy(sigmaa, h->at<double2>("sigmaa"), (size_t)nanz * 16, hipMemcpyDeviceToHost));
+  return 0;
+}
+
+int crt_voltages_dev(crt_handle* h, int32_t nanz, const int32_t* strnr, const int32_t* vnr, int32_t mode,
+                     double* volt_dev, double* sigmaa_dev) {
+  if (!h) return kErrArg;
+  int rc = run_voltages(h, nanz, strnr, vnr, mode, nullptr);
+  if (rc) return rc;
+  if (volt_dev) HIPCHK(hipMemcpyAsync(volt_dev, h->at<double2>("volt"), (size_t)nanz * 16, hipMemcpyDeviceToDevice, h->stream));
+  if (sigmaa_dev)
+    HIPCHK(hipMemcpyAsync(sigmaa_dev, h->at<double2>("sigmaa"), (size_t)nanz * 16, hipMemcpyDeviceToDevice, h->stream));
+  return finish(h);
+}
+
+int crt_sensitivities(crt_handle* h, int32_t mode, double* sens) {
+  if (!h || (mode != 0 && mode != 1)) return kErrArg;
+  if (!h->volt_done) return kErrOrder;
+  SensArgs S;
+  S.n = h->n;
+  S.elanz = h->elanz;
+  S.eanz = h->E;
+  S.K = h->K;
+  S.nimn = h->nimn;
+  S.nanz = h->nanz;
+  S.swrtr = h->swrtr;
+  S.mode = mode;
+  S.el_nk = h->at<int>("el_nk");
+  S.el_nodes = h->at<int>("el_nodes");
+  S.elbg = h->at<double>("elbg");
+  S.kwnwi = h->at<double>("kwnwi");
+  S.kpot = h->at<double2>("kpot");
+  S.sigma = h->at<double2>("sigma");
+  S.volt = h->at<double2>("volt");
+  S.strnr = h->at<int>("strnr");
+  S.vnr = h->at<int>("vnr");
+  S.sens = h->get<double2>("sens", (size_t)h->nanz * h->elanz);
+  NEED(S.sens);
+  const size_t lds = (size_t)h->maxnodes * h->E * 16;
+  tstart(h, "sens");
+  hipLaunchKernelGGL(crt_sens_kernel, dim3(h->elanz), dim3(256), lds, h->stream, S);
+  tstop(h);
+  int rc = finish(h);
+  if (rc) return rc;
+  h->sens_done = true;
+  h->sens_mode = mode;
+  if (sens) HIPCHK(hipMemcpy(sens, S.sens, (size_t)h->nanz * h->elanz * 16, hipMemcpyDeviceToHost));
+  return 0;
+}
+
+static int run_adjoint(crt_handle* h, int32_t mode, const double* w_dev, const double2* r_dev, double2* g_dev) {
+  const int n = h->n, E = h->E, K = h->K, N = h->nanz;
+  double2* gamma = h->get<double2>("adj_gamma", N);
+  double2* C = h->get<double2>("adj_C", (size_t)E * E);
+  double2* Y = h->get<double2>("adj_Y", (size_t)K * n * E);
+  NEED(gamma);
+  NEED(C);
+  NEED(Y);
+  tstart(h, "adj_gamma");
+  hipLaunchKernelGGL(crt_adj_gamma_kernel, dim3((N + 255) / 256), dim3(256), 0, h->stream, N, mode, w_dev, r_dev,
+                     (const double2*)h->at<double2>("volt"), gamma);
+  tstop(h);
+  tstart(h, "adj_cmat");
+  hipLaunchKernelGGL(crt_adj_cmat_kernel, dim3((E * E + 255) / 256), dim3(256), 0, h->stream, N, E,
+                     (const int*)h->at<int>("strnr"), (const int*)h->at<int>("vnr"), (const double2*)gamma, C);
+  tstop(h);
+  const size_t lds = (size_t)E * kAdjColsHost * 16;
+  tstart(h, "adj_gemm");
+  hipLaunchKernelGGL(crt_adj_gemm_kernel, dim3((n + 255) / 256, (E + kAdjColsHost - 1) / kAdjColsHost, K), dim3(256),
+                     lds, h->stream, (const double2*)h->at<double2>("kpot"), (const double2*)C, n, E, Y);
+  tstop(h);
+  AdjArgs A;
+  A.n = n;
+  A.elanz = h->elanz;
+  A.eanz = E;
+  A.K = K;
+  A.nimn = h->nimn;
+  A.swrtr = h->swrtr;
+  A.mode = mode;
+  A.el_nk = h->at<int>("el_nk");
+  A.el_nodes = h->at<int>("el_nodes");
+  A.elbg = h->at<double>("elbg");
+  A.kwnwi = h->at<double>("kwnwi");
+  A.kpot = h->at<double2>("kpot");
+  A.Y = Y;
+  A.sigma = h->at<double2>("sigma");
+  A.grad = g_dev;
+  tstart(h, "adj_elem");
+  hipLaunchKernelGGL(crt_adj_elem_kernel, dim3((unsigned)(((size_t)h->elanz * 64 + 255) / 256)), dim3(256), 0,
+                     h->stream, A);
+  tstop(h);
+  return finish(h);
+}
+
+int crt_adjoint_gradient(crt_handle* h, int32_t mode, const double* w, const double* resid, double* grad) {
+  if (!h || !w || !resid || !grad || (mode != 0 && mode != 1)) return kErrArg;
+  if (!h->volt_done) return kErrOrder;
+  double* dw;
+  double2* dr;
+  int rc = upload(h, "adj_w", w, h->nanz, &dw);
+  rc |= upload(h, "adj_r", reinterpret_cast<const double2*>(resid), h->nanz, &dr);
+  if (rc) return kErrHip;
+  double2* dg = h->get<double2>("adj_g", h->elanz);
+  NEED(dg);
+  rc = run_adjoint(h, mode, dw, dr, dg);
+  if (rc) return rc;
+  HIPCHK(hipMemcpy(grad, dg, (size_t)h->elanz * 16, hipMemcpyDeviceToHost));
+  return 0;
+}
+
+int crt_adjoint_gradient_dev(crt_handle* h, int32_t mode, const double* w_dev, const double* resid_dev,
+                             double* grad_dev) {
+  if (!h || !w_dev || !resid_dev || !grad_dev || (mode != 0 && mode != 1)) return kErrArg;
+  if (!h->volt_done) return kErrOrder;
+  return run_adjoint(h, mode, w_dev, reinterpret_cast<const double2*>(resid_dev),
+                     reinterpret_cast<double2*>(grad_dev));
+}
+
+int crt_jhw_r(crt_handle* h, const double* w, const double* resid, double* grad) {
+  if (!h || !w || !resid || !grad) return kErrArg;
+  if (!h->sens_done) return kErrOrder;
+  double* dw;
+  double2* dr;
+  int rc = upload(h, "adj_w", w, h->nanz, &dw);
+  rc |= upload(h, "adj_r", reinterpret_cast<const double2*>(resid), h->nanz, &dr);
+  if (rc) return kErrHip;
+  double2* dg = h->get<double2>("jhwr_g", h->elanz);
+  NEED(dg);
+  tstart(h, "jhwr");
+  hipLaunchKernelGGL(crt_jhwr_kernel, dim3((unsigned)(((size_t)h->elanz * 64 + 255) / 256)), dim3(256), 0, h->stream,
+                     (const double2*)h->at<double2>("sens"), h->nanz, h->elanz, (const double*)dw,
+                     (const double2*)dr, dg);
+  tstop(h);
+  rc = finish(h);
+  if (rc) return rc;
+  HIPCHK(hipMemcpy(grad, dg, (size_t)h->elanz * 16, hipMemcpyDeviceToHost));
+  return 0;
+}
+
+int crt_cg_solve(crt_handle* h, const double* w, const double* dat, const double* sigmaa, double lam, int32_t ltri,
+                 int32_t nx, const double* smatm, int32_t smatm_cols, const int32_t* nachbar, int32_t maxnodes,
+                 const double* rdiag, const double* rm, int32_t ncgmax, double eps, double* dpar, double* cgfac,
+                 double* cgres) {
+  if (!h || !w || !dat || !sigmaa || !smatm || !rdiag || !dpar || ncgmax <= 0 || smatm_cols <= 0) return kErrArg;
+  if (!h->sens_done) return kErrOrder;
+  const bool tri_op = !(ltri == 0 || ltri == 3 || ltri == 4);
+  if (ltri == 15) return kErrArg;  // the stochastic (dense) operator is not supported
+  if (tri_op && (!nachbar || maxnodes <= 0 || smatm_cols < maxnodes + 1)) return kErrArg;
+  if (ltri == 0 && smatm_cols < 3) return kErrArg;
+  const int N = h->nanz, M = h->elanz;
+  CgArgs G;
+  G.nanz = N;
+  G.manz = M;
+  G.ltri = ltri;
+  G.nx = nx;
+  G.maxnodes = maxnodes;
+  G.lam = lam;
+  G.eps = eps;
+  G.J = h->at<double2>("sens");
+  double* dw;
+  double* drd;
+  double* dsm;
+  double2 *ddat, *dsa, *drm = nullptr;
+  int* dnb = nullptr;
+  int rc = upload(h, "cg_w", w, N, &dw);
+  rc |= upload(h, "cg_rdiag", rdiag, M, &drd);
+  rc |= upload(h, "cg_smatm", smatm, (size_t)M * smatm_cols, &dsm);
+  rc |= upload(h, "cg_dat", reinterpret_cast<const double2*>(dat), N, &ddat);
+  rc |= upload(h, "cg_sigmaa", reinterpret_cast<const double2*>(sigmaa), N, &dsa);
+  if (rm) rc |= upload(h, "cg_rm", reinterpret_cast<const double2*>(rm), M, &drm);
+  if (tri_op) rc |= upload(h, "cg_nachbar", nachbar, (size_t)M * maxnodes, &dnb);
+  if (rc) return kErrHip;
+  G.w = dw;
+  G.rdiag = drd;
+  G.smatm = dsm;
+  G.nachbar = dnb;
+  G.cgfac = h->get<double>("cg_cgfac", M);
+  G.bvec = h->get<double2>("cg_bvec", M);
+  G.rvec = h->get<double2>("cg_rvec", M);
+  G.pvec = h->get<double2>("cg_pvec", M);
+  G.ap = h->get<double2>("cg_ap", N);
+  G.dpar = h->get<double2>("cg_dpar", M);
+  G.cgres = h->get<double>("cg_cgres", ncgmax + 2);
+  G.scal = h->get<double>("cg_scal", 8);
+  NEED(G.cgfac);
+  NEED(G.bvec);
+  NEED(G.rvec);
+  NEED(G.pvec);
+  NEED(G.ap);
+  NEED(G.dpar);
+  NEED(G.cgres);
+  NEED(G.scal);
+  HIPCHK(hipMemsetAsync(G.dpar, 0, (size_t)M * 16, h->stream));
+  HIPCHK(hipMemsetAsync(G.pvec, 0, (size_t)M * 16, h->stream));
+  HIPCHK(hipMemsetAsync(G.scal, 0, 8 * sizeof(double), h->stream));
+  HIPCHK(hipMemsetAsync(G.cgres, 0, (ncgmax + 2) * sizeof(double), h->stream));
+  tstart(h, "cg_rhs");
+  hipLaunchKernelGGL(crt_cg_rhs_kernel, dim3((unsigned)(((size_t)M * 64 + 255) / 256)), dim3(256), 0, h->stream, G,
+                     (const double2*)ddat, (const double2*)dsa, (const double2*)drm);
+  tstop(h);
+  HIPCHK(hipMemcpyAsync(G.rvec, G.bvec, (size_t)M * 16, hipMemcpyDeviceToDevice, h->stream));
+  for (int it = 0; it < ncgmax; ++it) {
+    tstart(h, "cg_scalar");
+    hipLaunchKernelGGL(crt_cg_scalar_kernel, dim3(1), dim3(1024), 0, h->stream, G, 0, it);
+    tstop(h);
+    tstart(h, "cg_bap");
+    hipLaunchKernelGGL(crt_cg_bap_kernel, dim3((N + 255) / 256), dim3(256), 0, h->stream, G);
+    tstop(h);
+    tstart(h, "cg_bb");
+    hipLaunchKernelGGL(crt_cg_bb_kernel, dim3((unsigned)(((size_t)M * 64 + 255) / 256)), dim3(256), 0, h->stream, G);
+    tstop(h);
+    tstart(h, "cg_scalar");
+    hipLaunchKernelGGL(crt_cg_scalar_kernel, dim3(1), dim3(1024), 0, h->stream, G, 1, it);
+    tstop(h);
+  }
+  rc = finish(h);
+  if (rc) return rc;
+  std::vector<double> cf(M), sc(8);
+  std::vector<double> dp(2 * (size_t)M);
+  HIPCHK(hipMemcpy(cf.data(), G.cgfac, M * 8, hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(dp.data(), G.dpar, (size_t)M * 16, hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(sc.data(), G.scal, 64, hipMemcpyDeviceToHost));
+  if (cgres) {
+    HIPCHK(hipMemcpy(cgres, G.cgres, (ncgmax + 2) * sizeof(double), hipMemcpyDeviceToHost));
+    cgres[0] = sc[6];
+  }
+  // dpar = dpar * cgfac (src/update.f90:307)
+  for (int j = 0; j < M; ++j) {
+    dpar[2 * j] = dp[2 * j] * cf[j];
+    dpar[2 * j + 1] = dp[2 * j + 1] * cf[j];
+  }
+  if (cgfac) std::copy(cf.begin(), cf.end(), cgfac);
+  return 0;
+}
+
+int crt_error_info(crt_handle* h, int32_t* code, int32_t* k, int32_t* l) {
+  if (!h) return kErrArg;
+  if (code) *code = h->last_err;
+  if (k) *k = h->last_k;
+  if (l) *l = h->last_l;
+  return 0;
+}
+
+int crt_set_timing(crt_handle* h, int32_t enable) {
+  if (!h) return kErrArg;
+  h->timing = enable != 0;
+  h->times.clear();
+  return 0;
+}
+
+int crt_stage_times(crt_handle* h, int32_t n_max, char* names, double* ms, int32_t* launches) {
+  if (!h) return -kErrArg;
+  int i = 0;
+  for (auto& kv : h->times) {
+    if (i >= n_max) break;
+    if (names) {
+      std::strncpy(names + 48 * i, kv.first.c_str(), 47);
+      names[48 * i + 47] = 0;
+    }
+    if (ms) ms[i] = kv.second.first;
+    if (launches) launches[i] = kv.second.second;
+    ++i;
+  }
+  return i;
+}
+
+int crt_kpot_device(crt_handle* h, const double** kpot_dev) {
+  if (!h || !kpot_dev) return kErrArg;
+  if (!h->forward_done) return kErrOrder;
+  *kpot_dev = reinterpret_cast<const double*>(h->at<double2>("kpot"));
+  return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,90 @@
+// Argument blocks of the post-solve kernels (precal, sensitivities, adjoint, CG).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+struct PrecalArgs {
+  int elanz, relanz, eanz, K, nimn;
+  const int* el_type;     // elanz (3, 5 or 8)
+  const int* el_nodes;    // elanz x 4, 0-based
+  const int* bnd_type;    // relanz (11, 12, 13)
+  const int* bnd_nodes;   // relanz x 2, 0-based
+  const double* nx;       // node coordinates (through snr)
+  const double* ny;
+  const int* enr;         // eanz, 0-based
+  const double* kwn;      // K
+  double sytop;
+  double* elbg;           // [K][elanz][nimn]
+  double* relbg;          // [relanz][nimn]
+  double* kg;             // [K][eanz][relanz]
+  int* err;
+};
+
+struct SensArgs {
+  int n, elanz, eanz, K, nimn, nanz, swrtr, mode;
+  const int* el_nk;
+  const int* el_nodes;
+  const double* elbg;
+  const double* kwnwi;
+  const double2* kpot;    // [K][eanz][n]
+  const double2* sigma;
+  const double2* volt;
+  const int* strnr;
+  const int* vnr;
+  double2* sens;          // nanz x elanz, column major
+};
+
+struct AdjArgs {
+  int n, elanz, eanz, K, nimn, swrtr, mode;
+  const int* el_nk;
+  const int* el_nodes;
+  const double* elbg;
+  const double* kwnwi;
+  const double2* kpot;    // [K][eanz][n]
+  const double2* Y;       // [K][n][eanz]
+  const double2* sigma;
+  double2* grad;          // elanz
+};
+
+struct CgArgs {
+  int nanz, manz, ltri, nx, maxnodes;
+  double lam, eps;
+  const double2* J;       // nanz x manz, column major
+  const double* w;        // wmatd * wdfak
+  const double* rdiag;    // diagonal of R used by the preconditioner
+  const double* smatm;    // manz x (3 | maxnodes+1), column major
+  const int* nachbar;     // manz x maxnodes (1-based, 0 = none)
+  double* cgfac;
+  double2* bvec;
+  double2* rvec;
+  double2* pvec;
+  double2* ap;
+  double2* dpar;
+  double* cgres;          // ncgmax + 2
+  double* scal;           // 8 device scalars of the recurrence
+};
+
+__global__ void crt_precal_area_kernel(PrecalArgs P);
+__global__ void crt_precal_bnd_kernel(PrecalArgs P);
+__global__ void crt_rtrafo_kernel(const double2* kpot, const double* kwnwi, int n, int eanz, int K,
+                                  int swrtr, double2* hpot);
+__global__ void crt_rtrafo_elec_kernel(const double2* kpot, const double* kwnwi, const int* enr, int n,
+                                       int eanz, int K, int swrtr, double2* he);
+__global__ void crt_volt_kernel(const double2* he, int eanz, int nanz, const int* strnr, const int* vnr,
+                                int mode, const double* kfak, double2* volt, double2* sigmaa, int* err);
+__global__ void crt_sens_kernel(SensArgs S);
+__global__ void crt_adj_gamma_kernel(int nanz, int mode, const double* w, const double2* resid,
+                                     const double2* volt, double2* gamma);
+__global__ void crt_adj_cmat_kernel(int nanz, int eanz, const int* strnr, const int* vnr,
+                                    const double2* gamma, double2* C);
+__global__ void crt_adj_gemm_kernel(const double2* kpot, const double2* C, int n, int eanz, double2* Y);
+__global__ void crt_adj_elem_kernel(AdjArgs A);
+__global__ void crt_jhwr_kernel(const double2* J, int nanz, int manz, const double* w, const double2* r,
+                                double2* g);
+__global__ void crt_cg_rhs_kernel(CgArgs G, const double2* dat, const double2* sigmaa, const double2* rm);
+__global__ void crt_cg_bap_kernel(CgArgs G);
+__global__ void crt_cg_bb_kernel(CgArgs G);
+__global__ void crt_cg_scalar_kernel(CgArgs G, int stage, int iter);
+
+constexpr int kAdjColsHost = 32;

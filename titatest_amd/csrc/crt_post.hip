@@ -1,0 +1,642 @@
+// Element pre-computation, back-transform, voltage pick-off, sensitivities,
+// adjoint gradient and CG normal-equation products (gfx950, complex FP64).
+#include "crt_device.h"
+#include "crt_post.h"
+
+// ---------------------------------------------------------------- precal --
+// Per-element matrices (src/precal.f90:99-205): elem3 (src/elem3.f90:27-99),
+// elem5 (src/elem5.f90:1-72), elem8 with centre-node condensation
+// (src/elem8.f90:18-205).  One thread per (area element, wavenumber); output
+// elbg[k][e][imn], packed lower triangle imn = m(m+1)/2 + n (0-based, n <= m).
+__constant__ int c_s1[9] = {1, -1, 0, -1, 1, 0, 0, 0, 0};
+__constant__ int c_s2[9] = {2, -1, -1, -1, 0, 1, -1, 1, 0};
+__constant__ int c_s3[9] = {1, 0, -1, 0, 0, 0, -1, 0, 1};
+__constant__ int c_s4[9] = {2, 1, 1, 1, 2, 1, 1, 1, 2};
+
+__device__ int elem8_dev(const double* xk, const double* yk, double kw, double* kel /*4x4 col*/) {
+  double xkd[5], ykd[5], elmas[25], elmam[25];
+  for (int i = 0; i < 25; ++i) { elmas[i] = 0.0; elmam[i] = 0.0; }
+  double a1 = xk[2] - xk[0], a2 = xk[3] - xk[1];
+  if (fabs(a1) <= 1e-12 || fabs(a2) <= 1e-12) return 26;
+  a1 = (yk[2] - yk[0]) / a1;
+  a2 = (yk[3] - yk[1]) / a2;
+  if (fabs(a1 - a2) <= 1e-12) return 26;
+  xkd[4] = (a1 * xk[0] - yk[0] - a2 * xk[1] + yk[1]) / (a1 - a2);
+  ykd[4] = yk[0] + a1 * (xkd[4] - xk[0]);
+  for (int i = 0; i < 4; ++i) { xkd[i] = xk[i]; ykd[i] = yk[i]; }
+  const int IK[4][3] = {{0, 1, 4}, {1, 2, 4}, {4, 2, 3}, {0, 4, 3}};
+  for (int k = 0; k < 4; ++k) {
+    const int* ik = IK[k];
+    double x21 = xkd[ik[1]] - xkd[ik[0]], x31 = xkd[ik[2]] - xkd[ik[0]];
+    double y21 = ykd[ik[1]] - ykd[ik[0]], y31 = ykd[ik[2]] - ykd[ik[0]];
+    double det = x21 * y31 - x31 * y21;
+    if (det <= 1e-12) return 26;
+    double a = (x31 * x31 + y31 * y31) / det;
+    double b = -(x31 * x21 + y31 * y21) / det;
+    double c = (x21 * x21 + y21 * y21) / det;
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) {
+        int p = ik[i] + 5 * ik[j];
+        elmas[p] = elmas[p] + (a * (double)c_s1[i + 3 * j] + b * (double)c_s2[i + 3 * j] +
+                               c * (double)c_s3[i + 3 * j]) / 2.0;
+        elmam[p] = elmam[p] + det * (double)c_s4[i + 3 * j] / 2.4e1;
+      }
+  }
+  for (int i = 0; i < 25; ++i) elmas[i] = elmas[i] + kw * kw * elmam[i];
+  for (int i = 0; i < 4; ++i)
+    for (int j = 0; j < 4; ++j) kel[i + 4 * j] = elmas[i + 5 * j] - elmas[i + 20] * elmas[4 + 5 * j] / elmas[24];
+  return 0;
+}
+
+__global__ void crt_precal_area_kernel(PrecalArgs P) {
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= P.elanz * P.K) return;
+  const int e = idx % P.elanz, k = idx / P.elanz;
+  const int ty = P.el_type[e];
+  const int nk = (ty == 3) ? 3 : 4;
+  double xk[4], yk[4];
+  for (int q = 0; q < nk; ++q) {
+    int node = P.el_nodes[e * 4 + q];
+    xk[q] = P.nx[node];
+    yk[q] = P.ny[node];
+  }
+  const double kw = P.kwn[k];
+  double* out = P.elbg + ((size_t)k * P.elanz + e) * P.nimn;
+  if (ty == 8) {
+    double kel[16];
+    int err = elem8_dev(xk, yk, kw, kel);
+    if (err) { atomicCAS(P.err, 0, err); return; }
+    int imn = 0;
+    for (int m = 0; m < 4; ++m)
+      for (int n = 0; n <= m; ++n) out[imn++] = kel[m + 4 * n];
+  } else if (ty == 3) {
+    double x21 = xk[1] - xk[0], x31 = xk[2] - xk[0];
+    double y21 = yk[1] - yk[0], y31 = yk[2] - yk[0];
+    double det = x21 * y31 - x31 * y21;
+    if (det <= 0.0) { atomicCAS(P.err, 0, 26); return; }
+    double a = (x31 * x31 + y31 * y31) / det;
+    double b = -(x31 * x21 + y31 * y21) / det;
+    double c = (x21 * x21 + y21 * y21) / det;
+    int imn = 0;
+    for (int m = 0; m < 3; ++m)
+      for (int n = 0; n <= m; ++n) {
+        double es = (a * (double)c_s1[m + 3 * n] + b * (double)c_s2[m + 3 * n] + c * (double)c_s3[m + 3 * n]) / 2.0;
+        double em = det * (double)c_s4[m + 3 * n] / 2.4e1;
+        out[imn++] = es + em * kw * kw;
+      }
+  } else if (ty == 5) {
+    const int s1[16] = {2, -2, -1, 1, -2, 2, 1, -1, -1, 1, 2, -2, 1, -1, -2, 2};
+    const int s2[16] = {1, 0, -1, 0, 0, -1, 0, 1, -1, 0, 1, 0, 0, 1, 0, -1};
+    const int s3[16] = {2, 1, -1, -2, 1, 2, -2, -1, -1, -2, 2, 1, -2, -1, 1, 2};
+    const int s4[16] = {4, 2, 1, 2, 2, 4, 2, 1, 1, 2, 4, 2, 2, 1, 2, 4};
+    double x21 = xk[1] - xk[0], x31 = xk[3] - xk[0];
+    double y21 = yk[1] - yk[0], y31 = yk[3] - yk[0];
+    double det = x21 * y31 - x31 * y21;
+    if (det <= 0.0) { atomicCAS(P.err, 0, 26); return; }
+    double a = (x31 * x31 + y31 * y31) / det;
+    double b = -(x31 * x21 + y31 * y21) / det;
+    double c = (x21 * x21 + y21 * y21) / det;
+    int imn = 0;
+    for (int m = 0; m < 4; ++m)
+      for (int n = 0; n <= m; ++n) {
+        double es = (a * (double)s1[m + 4 * n] + 3.0 * b * (double)s2[m + 4 * n] + c * (double)s3[m + 4 * n]) / 6.0;
+        double em = det * (double)s4[m + 4 * n] / 3.6e1;
+        out[imn++] = es + em * kw * kw;
+      }
+  } else {
+    atomicCAS(P.err, 0, 18);
+  }
+}
+
+// Bessel K0/K1 via the Abramowitz & Stegun 9.8.1-9.8.8 polynomials with the
+// reference's coefficients (src/bessk0.f90:11-24, bessk1.f90:10-23,
+// bessi0.f90, bessi1.f90); K = 0 above x = 500.
+__device__ double bessi0_dev(double x) {
+  if (fabs(x) < 3.75) {
+    double y = (x / 3.75) * (x / 3.75);
+    return 1.0 + y * (3.5156229 + y * (3.0899424 + y * (1.2067492 + y * (0.2659732 +
+           y * (0.360768e-1 + y * 0.45813e-2)))));
+  }
+  double ax = fabs(x), y = 3.75 / ax;
+  return (exp(ax) / sqrt(ax)) * (0.39894228 + y * (0.1328592e-1 + y * (0.225319e-2 +
+         y * (-0.157565e-2 + y * (0.916281e-2 + y * (-0.2057706e-1 + y * (0.2635537e-1 +
+         y * (-0.1647633e-1 + y * 0.392377e-2))))))));
+}
+__device__ double bessi1_dev(double x) {
+  if (fabs(x) < 3.75) {
+    double y = (x / 3.75) * (x / 3.75);
+    return x * (0.5 + y * (0.87890594 + y * (0.51498869 + y * (0.15084934 +
+           y * (0.2658733e-1 + y * (0.301532e-2 + y * 0.32411e-3))))));
+  }
+  double ax = fabs(x), y = 3.75 / ax;
+  return (exp(ax) / sqrt(ax)) * (0.39894228 + y * (-0.3988024e-1 + y * (-0.362018e-2 +
+         y * (0.163801e-2 + y * (-0.1031555e-1 + y * (0.2282967e-1 + y * (-0.2895312e-1 +
+         y * (0.1787654e-1 + y * -0.420059e-2))))))));
+}
+__device__ double bessk0_dev(double x) {
+  if (x <= 2.0) {
+    double y = x * x / 4.0;
+    return (-log(x / 2.0) * bessi0_dev(x)) + (-0.57721566 + y * (0.42278420 + y * (0.23069756 +
+           y * (0.3488590e-1 + y * (0.262698e-2 + y * (0.10750e-3 + y * 0.74e-5))))));
+  } else if (x > 5e2) {
+    return 0.0;
+  }
+  double y = 2.0 / x;
+  return (exp(-x) / sqrt(x)) * (1.25331414 + y * (-0.7832358e-1 + y * (0.2189568e-1 +
+         y * (-0.1062446e-1 + y * (0.587872e-2 + y * (-0.251540e-2 + y * 0.53208e-3))))));
+}
+__device__ double bessk1_dev(double x) {
+  if (x <= 2.0) {
+    double y = x * x / 4.0;
+    return (log(x / 2.0) * bessi1_dev(x)) + (1.0 / x) * (1.0 + y * (0.15443144 + y * (-0.67278579 +
+           y * (-0.18156897 + y * (-0.1919402e-1 + y * (-0.110404e-2 + y * -0.4686e-4))))));
+  } else if (x > 5e2) {
+    return 0.0;
+  }
+  double y = 2.0 / x;
+  return (exp(-x) / sqrt(x)) * (1.25331414 + y * (0.23498619 + y * (-0.3655620e-1 +
+         y * (0.1504268e-1 + y * (-0.780353e-2 + y * (0.325614e-2 + y * -0.68245e-3))))));
+}
+
+// Mixed-boundary factor kg[k][l][rel] = beta(l, k) (src/beta.f90:38-135) and the
+// boundary mass matrix relbg = L/6 [[2,1],[1,2]] (src/elem1.f90:53-64) for
+// type-11 boundary elements.  One thread per (boundary element, electrode, k).
+__global__ void crt_precal_bnd_kernel(PrecalArgs P) {
+  const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long total = (long)P.relanz * P.eanz * P.K;
+  if (idx >= total) return;
+  const int rel = (int)(idx % P.relanz);
+  const int l = (int)((idx / P.relanz) % P.eanz);
+  const int k = (int)(idx / ((long)P.relanz * P.eanz));
+  double* kgo = P.kg + ((size_t)k * P.eanz + l) * P.relanz + rel;
+  if (P.bnd_type[rel] != 11) { *kgo = 0.0; return; }
+  const int n1 = P.bnd_nodes[2 * rel], n2 = P.bnd_nodes[2 * rel + 1];
+  const double xk0 = P.nx[n1], xk1 = P.nx[n2], yk0 = P.ny[n1], yk1 = P.ny[n2];
+  if (l == 0 && k == 0) {
+    double x21 = xk1 - xk0, y21 = yk1 - yk0;
+    double len = sqrt(x21 * x21 + y21 * y21);
+    double* rb = P.relbg + (size_t)rel * P.nimn;
+    rb[0] = len * 2.0 / 6.0;
+    rb[1] = len * 1.0 / 6.0;
+    rb[2] = len * 2.0 / 6.0;
+  }
+  const int en = P.enr[l];
+  const double xs = P.nx[en], ys = P.ny[en];
+  const double xr = (xk0 + xk1) / 2.0, yr = (yk0 + yk1) / 2.0;
+  const double sytop = P.sytop;
+  const double x3 = xr - xs;
+  const double y3m = (yr - sytop) - (ys - sytop);
+  const double y3p = (yr - sytop) + (ys - sytop);
+  const double r1m = sqrt(x3 * x3 + y3m * y3m);
+  const double r1p = sqrt(x3 * x3 + y3p * y3p);
+  const double x4 = yk0 - yk1, y4 = xk1 - xk0;
+  const double r2 = sqrt(x4 * x4 + y4 * y4);
+  if (r1m < 1e-12 || r1p < 1e-12 || r2 < 1e-12) { atomicCAS(P.err, 0, 35); *kgo = 0.0; return; }
+  const double cthetm = (x3 * x4 + y3m * y4) / r1m / r2;
+  const double cthetp = (x3 * x4 + y3p * y4) / r1p / r2;
+  const double kw = P.kwn[k];
+  const double bk0m = bessk0_dev(r1m * kw), bk0p = bessk0_dev(r1p * kw);
+  const double bk1m = bessk1_dev(r1m * kw), bk1p = bessk1_dev(r1p * kw);
+  double b;
+  if (bk0m + bk0p == 0.0) b = (cthetm + cthetp) / 2.0;
+  else b = (cthetm * bk1m + cthetp * bk1p) / (bk0m + bk0p);
+  *kgo = kw * b;
+}
+
+// ---------------------------------------------------------- back-transform --
+// hpot(j,l) = sum_k kpot(j,l,k) * kwnwi(k) / pi  (src/rtrafo.f90:73-88), or
+// hpot = kpot (2D, swrtr = 0).  kpot[k][l][n], hpot[l][n].
+__global__ void crt_rtrafo_kernel(const double2* __restrict__ kpot, const double* __restrict__ kwnwi,
+                                  int n, int eanz, int K, int swrtr, double2* __restrict__ hpot) {
+  const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (long)n * eanz) return;
+  if (swrtr == 0) { hpot[idx] = kpot[idx]; return; }
+  cplx s = mkc(0.0, 0.0);
+  const size_t stride = (size_t)n * eanz;
+  for (int k = 0; k < K; ++k) s = cadd(s, rmul(kwnwi[k], ldc(kpot + idx + k * stride)));
+  stc(hpot + idx, cdiv(s, mkc(M_PI, 0.0)));
+}
+
+// Electrode slice of the back-transform: he[l][m] = hpot(enr(m), l).
+__global__ void crt_rtrafo_elec_kernel(const double2* __restrict__ kpot, const double* __restrict__ kwnwi,
+                                       const int* __restrict__ enr, int n, int eanz, int K, int swrtr,
+                                       double2* __restrict__ he) {
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= eanz * eanz) return;
+  const int m = idx % eanz, l = idx / eanz;
+  const size_t base = (size_t)l * n + enr[m];
+  if (swrtr == 0) { he[idx] = kpot[base]; return; }
+  cplx s = mkc(0.0, 0.0);
+  const size_t stride = (size_t)n * eanz;
+  for (int k = 0; k < K; ++k) s = cadd(s, rmul(kwnwi[k], ldc(kpot + base + k * stride)));
+  stc(he + idx, cdiv(s, mkc(M_PI, 0.0)));
+}
+
+// Voltage superposition (src/bvolt.f90:54-63) from the electrode slice;
+// mode 0: sigmaa = kfak*V (bvolt); mode 1: sigmaa = -ln V (bvolti :88).
+__global__ void crt_volt_kernel(const double2* __restrict__ he, int eanz, int nanz,
+                                const int* __restrict__ strnr, const int* __restrict__ vnr, int mode,
+                                const double* __restrict__ kfak, double2* __restrict__ volt,
+                                double2* __restrict__ sigmaa, int* err) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nanz) return;
+  const int e1 = strnr[i] % 10000, e2 = (strnr[i] - e1) / 10000;
+  const int e3 = vnr[i] % 10000, e4 = (vnr[i] - e3) / 10000;
+  auto H = [&](int ep, int es) {
+    const double msk = (double)(min(ep, 1) * min(es, 1));
+    return rmul(msk, ldc(he + (size_t)(max(es, 1) - 1) * eanz + (max(ep, 1) - 1)));
+  };
+  const cplx d1 = H(e4, e2), d2 = H(e4, e1), d3 = H(e3, e2), d4 = H(e3, e1);
+  const cplx v = csub(csub(d1, d2), csub(d3, d4));
+  stc(volt + i, v);
+  if (mode == 0) {
+    stc(sigmaa + i, rmul(kfak ? kfak[i] : 1.0, v));
+  } else {
+    if (cabs_(v) == 0.0) { atomicCAS(err, 0, 82); stc(sigmaa + i, mkc(0.0, 0.0)); return; }
+    cplx lg = clog_(v);
+    stc(sigmaa + i, mkc(-lg.re, -lg.im));
+  }
+}
+
+// ---------------------------------------------------------- sensitivities --
+// bsens (mode 0: sens(i,e) = -dV_i/dsigma_e summed the reference's way,
+// src/bsens.f90:124-187) and bsensi (mode 1: +dum*sigma_e/V_i,
+// src/bsensi.f90:119-184).  One workgroup per area element: the element's
+// node potentials for all electrodes at one wavenumber are staged in LDS and
+// every thread evaluates h_k for its measurements in the reference's (mi, mj)
+// order, accumulating sum_k w_k h_k in registers.
+constexpr int kSensMaxQ = 16;  // measurements per thread per pass
+
+__global__ __launch_bounds__(256) void crt_sens_kernel(SensArgs S) {
+  extern __shared__ double2 sm[];
+  const int e = blockIdx.x;
+  const int nk = S.el_nk[e];
+  const int E = S.eanz;
+  cplx* P = reinterpret_cast<cplx*>(sm);  // [q][E]
+  __shared__ double sel[10];
+  __shared__ int nodes[4];
+  if (threadIdx.x < 4) nodes[threadIdx.x] = S.el_nodes[e * 4 + threadIdx.x];
+  const double pi = M_PI;
+  const cplx sig = ldc(S.sigma + e);
+  for (int i0 = 0; i0 < S.nanz; i0 += kSensMaxQ * blockDim.x) {
+    cplx acc[kSensMaxQ];
+    int e1[kSensMaxQ], e2[kSensMaxQ], e3[kSensMaxQ], e4[kSensMaxQ];
+#pragma unroll
+    for (int q = 0; q < kSensMaxQ; ++q) {
+      acc[q] = mkc(0.0, 0.0);
+      const int i = i0 + threadIdx.x + q * blockDim.x;
+      if (i < S.nanz) {
+        e1[q] = S.strnr[i] % 10000; e2[q] = (S.strnr[i] - e1[q]) / 10000;
+        e3[q] = S.vnr[i] % 10000;   e4[q] = (S.vnr[i] - e3[q]) / 10000;
+      } else {
+        e1[q] = e2[q] = e3[q] = e4[q] = 0;
+      }
+    }
+    for (int k = 0; k < S.K; ++k) {
+      __syncthreads();
+      for (int t = threadIdx.x; t < nk * E; t += blockDim.x) {
+        const int q = t / E, a = t - q * E;
+        P[t] = ldc(S.kpot + ((size_t)k * E + a) * S.n + nodes[q]);
+      }
+      if (threadIdx.x < S.nimn) sel[threadIdx.x] = S.elbg[((size_t)k * S.elanz + e) * S.nimn + threadIdx.x];
+      __syncthreads();
+      const double wk = S.kwnwi[k];
+#pragma unroll
+      for (int q = 0; q < kSensMaxQ; ++q) {
+        const int i = i0 + threadIdx.x + q * blockDim.x;
+        if (i >= S.nanz) continue;
+        cplx h = mkc(0.0, 0.0);
+        const cplx zero = mkc(0.0, 0.0);
+        for (int mi = 0; mi < nk; ++mi) {
+          const cplx s3 = e3[q] > 0 ? P[mi * E + e3[q] - 1] : zero;
+          const cplx s4 = e4[q] > 0 ? P[mi * E + e4[q] - 1] : zero;
+          const cplx v = csub(s4, s3);
+          for (int mj = 0; mj < nk; ++mj) {
+            const cplx s1 = e1[q] > 0 ? P[mj * E + e1[q] - 1] : zero;
+            const cplx s2 = e2[q] > 0 ? P[mj * E + e2[q] - 1] : zero;
+            const int imax = max(mi, mj), imin = min(mi, mj);
+            const double sv = sel[(imax * (imax + 1)) / 2 + imin];
+            h = cadd(h, rmul(sv, cmul(csub(s2, s1), v)));
+          }
+        }
+        if (S.swrtr == 0) acc[q] = h;
+        else acc[q] = cadd(acc[q], rmul(wk, h));
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < kSensMaxQ; ++q) {
+      const int i = i0 + threadIdx.x + q * blockDim.x;
+      if (i >= S.nanz) continue;
+      cplx out;
+      if (S.mode == 0) {
+        cplx dum = (S.swrtr == 0) ? acc[q] : rmul(1.0 / pi, acc[q]);
+        out = mkc(0.0 - dum.re, 0.0 - dum.im);
+      } else {
+        cplx dum = (S.swrtr == 0) ? acc[q] : cdiv(acc[q], mkc(pi, 0.0));
+        out = cdiv(cmul(dum, sig), ldc(S.volt + i));
+      }
+      stc(S.sens + (size_t)e * S.nanz + i, out);
+    }
+  }
+}
+
+// ------------------------------------------------------- adjoint gradient --
+// g_e = sum_i conj(J_ie) w_i r_i without forming J.  With J_ie = alpha_i beta_e
+// (1/pi) sum_k w_k h_ik(e) (bsens: alpha=-1, beta=1; bsensi: alpha=1/V_i,
+// beta=sigma_e) and h_ik(e) = sum_{mi,mj} S_k[mi,mj] v_i(n_mi) u_i(n_mj):
+//   g_e = conj(beta_e (1/pi) sum_k w_k sum_{mi,mj} S_k[mi,mj]
+//                 sum_b Y_k(n_mi, b) P_k(n_mj, b)),   Y_k = P_k C,
+//   C[a][b] = sum_i gamma_i (d_{a,e4} - d_{a,e3}) (d_{b,e2} - d_{b,e1}),
+//   gamma_i = alpha_i conj(w_i r_i).
+__global__ void crt_adj_gamma_kernel(int nanz, int mode, const double* __restrict__ w,
+                                     const double2* __restrict__ resid, const double2* __restrict__ volt,
+                                     double2* __restrict__ gamma) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nanz) return;
+  cplx r = ldc(resid + i);
+  cplx wr = cconj(rmul(w[i], r));
+  cplx g = (mode == 0) ? mkc(-wr.re, -wr.im) : cdiv(wr, ldc(volt + i));
+  stc(gamma + i, g);
+}
+
+// C[a][b] (a = receiver electrode, b = source electrode), one thread per entry,
+// summing over measurements in order (deterministic).
+__global__ void crt_adj_cmat_kernel(int nanz, int eanz, const int* __restrict__ strnr,
+                                    const int* __restrict__ vnr, const double2* __restrict__ gamma,
+                                    double2* __restrict__ C) {
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= eanz * eanz) return;
+  const int a = idx / eanz + 1, b = idx % eanz + 1;
+  cplx s = mkc(0.0, 0.0);
+  for (int i = 0; i < nanz; ++i) {
+    const int e1 = strnr[i] % 10000, e2 = strnr[i] / 10000;
+    const int e3 = vnr[i] % 10000, e4 = vnr[i] / 10000;
+    const int ca = (a == e4) - (a == e3);
+    const int cb = (b == e2) - (b == e1);
+    const int c = ca * cb;
+    if (c) s = cadd(s, rmul((double)c, ldc(gamma + i)));
+  }
+  stc(C + (size_t)(a - 1) * eanz + (b - 1), s);
+}
+
+// Y_k[j][b] = sum_a P_k[j][a] C[a][b] for all k (P_k = kpot[k][.][.], n x E with
+// leading dimension n).  Output layout Y[k][j][b] (row per node).  Block: 256
+// rows j x a 32-column slab of C held in LDS.
+constexpr int kAdjCols = 32;
+__global__ __launch_bounds__(256) void crt_adj_gemm_kernel(const double2* __restrict__ kpot,
+                                                         const double2* __restrict__ C, int n, int eanz,
+                                                         double2* __restrict__ Y) {
+  extern __shared__ double2 csm[];
+  const int k = blockIdx.z;
+  const int b0 = blockIdx.y * kAdjCols;
+  const int nb = min(kAdjCols, eanz - b0);
+  for (int t = threadIdx.x; t < eanz * kAdjCols; t += blockDim.x) {
+    const int a = t / kAdjCols, bb = t - a * kAdjCols;
+    csm[t] = (bb < nb) ? C[(size_t)a * eanz + b0 + bb] : make_double2(0.0, 0.0);
+  }
+  __syncthreads();
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  double ar[kAdjCols], ai[kAdjCols];
+#pragma unroll
+  for (int bb = 0; bb < kAdjCols; ++bb) { ar[bb] = 0.0; ai[bb] = 0.0; }
+  const double2* Pk = kpot + (size_t)k * eanz * n;
+  for (int a = 0; a < eanz; ++a) {
+    const double2 p = Pk[(size_t)a * n + j];
+#pragma unroll
+    for (int bb = 0; bb < kAdjCols; ++bb) {
+      const double2 c = csm[a * kAdjCols + bb];
+      ar[bb] += p.x * c.x - p.y * c.y;
+      ai[bb] += p.x * c.y + p.y * c.x;
+    }
+  }
+  double2* out = Y + ((size_t)k * n + j) * eanz + b0;
+  for (int bb = 0; bb < nb; ++bb) out[bb] = make_double2(ar[bb], ai[bb]);
+}
+
+// Element contraction: one wavefront per element, lanes over source
+// electrodes b; each lane accumulates sum_k w_k sum_{mi,mj} S Y P for its b,
+// then a wave reduction.
+__global__ __launch_bounds__(256) void crt_adj_elem_kernel(AdjArgs A) {
+  const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int lane = threadIdx.x & 63;
+  if (wave >= A.elanz) return;
+  const int e = wave;
+  const int nk = A.el_nk[e];
+  int nodes[4];
+  for (int q = 0; q < 4; ++q) nodes[q] = A.el_nodes[e * 4 + q];
+  const int E = A.eanz;
+  double accr = 0.0, acci = 0.0;
+  for (int k = 0; k < A.K; ++k) {
+    const double* sel = A.elbg + ((size_t)k * A.elanz + e) * A.nimn;
+    const double wk = (A.swrtr == 0) ? 1.0 : A.kwnwi[k];
+    const double2* Yk = A.Y + (size_t)k * A.n * E;
+    const double2* Pk = A.kpot + (size_t)k * E * A.n;
+    for (int b = lane; b < E; b += 64) {
+      cplx yv[4], pv[4];
+      for (int q = 0; q < nk; ++q) {
+        yv[q] = ldc(Yk + (size_t)nodes[q] * E + b);
+        pv[q] = ldc(Pk + (size_t)b * A.n + nodes[q]);
+      }
+      double sr = 0.0, si = 0.0;
+      for (int mi = 0; mi < nk; ++mi)
+        for (int mj = 0; mj < nk; ++mj) {
+          const int imax = max(mi, mj), imin = min(mi, mj);
+          const double sv = sel[(imax * (imax + 1)) / 2 + imin];
+          const cplx t = cmul(yv[mi], pv[mj]);
+          sr += sv * t.re;
+          si += sv * t.im;
+        }
+      accr += wk * sr;
+      acci += wk * si;
+    }
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    accr += __shfl_xor(accr, off, 64);
+    acci += __shfl_xor(acci, off, 64);
+  }
+  if (lane == 0) {
+    cplx s = mkc(accr, acci);
+    if (A.swrtr != 0) s = rmul(1.0 / M_PI, s);
+    const cplx beta = (A.mode == 0) ? mkc(1.0, 0.0) : ldc(A.sigma + e);
+    stc(A.grad + e, cconj(cmul(beta, s)));
+  }
+}
+
+// Data term of the model update from the explicit Jacobian:
+// g_j = sum_i conj(J_ij) w_i r_i (src/update.f90:263-266 without cgfac).
+// One wavefront per column j.
+__global__ __launch_bounds__(256) void crt_jhwr_kernel(const double2* __restrict__ J, int nanz, int manz,
+                                                      const double* __restrict__ w,
+                                                      const double2* __restrict__ r, double2* __restrict__ g) {
+  const int j = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int lane = threadIdx.x & 63;
+  if (j >= manz) return;
+  const double2* col = J + (size_t)j * nanz;
+  double sr = 0.0, si = 0.0;
+  for (int i = lane; i < nanz; i += 64) {
+    const cplx s = cconj(ldc(col + i));
+    const cplx a = rmul(w[i], ldc(r + i));
+    sr += s.re * a.re - s.im * a.im;
+    si += s.re * a.im + s.im * a.re;
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    sr += __shfl_xor(sr, off, 64);
+    si += __shfl_xor(si, off, 64);
+  }
+  if (lane == 0) stc(g + j, mkc(sr, si));
+}
+
+// ------------------------------------------------------- CG products ------
+// Preconditioner cgfac_j = 1/sqrt(sum_i |J_ij|^2 w_i + lam*rdiag_j) and the
+// scaled RHS (src/update.f90:179-293).  One wavefront per column j (J is
+// column major N x M).  Reductions use a fixed lane order (deterministic).
+__global__ __launch_bounds__(256) void crt_cg_rhs_kernel(CgArgs G, const double2* __restrict__ dat,
+                                                        const double2* __restrict__ sigmaa,
+                                                        const double2* __restrict__ rm) {
+  const int j = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int lane = threadIdx.x & 63;
+  if (j >= G.manz) return;
+  const double2* col = G.J + (size_t)j * G.nanz;
+  double d = 0.0, br = 0.0, bi = 0.0;
+  for (int i = lane; i < G.nanz; i += 64) {
+    const cplx s = ldc(col + i);
+    const double w = G.w[i];
+    d += (s.re * s.re + s.im * s.im) * w;
+    const cplx r = csub(ldc(dat + i), ldc(sigmaa + i));
+    const cplx t = cmul(rmul(w, cconj(s)), r);
+    br += t.re;
+    bi += t.im;
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    d += __shfl_xor(d, off, 64);
+    br += __shfl_xor(br, off, 64);
+    bi += __shfl_xor(bi, off, 64);
+  }
+  if (lane == 0) {
+    const double cg = 1.0 / sqrt(d + G.lam * G.rdiag[j]);
+    G.cgfac[j] = cg;
+    cplx b = mkc(br, bi);
+    if (rm && !(G.ltri == 3 || G.ltri == 4)) b = csub(b, rmul(G.lam, ldc(rm + j)));
+    stc(G.bvec + j, rmul(cg, b));
+  }
+}
+
+// ap_i = sum_j p_j J_ij cgfac_j (bap, src/cg_mod.f90:520-545).  Thread per
+// row i, coalesced over i for each column.
+__global__ void crt_cg_bap_kernel(CgArgs G) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= G.nanz) return;
+  double sr = 0.0, si = 0.0;
+  for (int j = 0; j < G.manz; ++j) {
+    const cplx p = rmul(G.cgfac[j], ldc(G.pvec + j));
+    const cplx s = ldc(G.J + (size_t)j * G.nanz + i);
+    sr += p.re * s.re - p.im * s.im;
+    si += p.re * s.im + p.im * s.re;
+  }
+  stc(G.ap + i, mkc(sr, si));
+}
+
+// bvec = R (cgfac o p) (bp / bptri / bplma, src/cg_mod.f90:547-640), then
+// bvec_j = cgfac_j (sum_i conj(J_ij) w_i ap_i + lam bvec_j) (bb, :699-726).
+__global__ __launch_bounds__(256) void crt_cg_bb_kernel(CgArgs G) {
+  const int j = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int lane = threadIdx.x & 63;
+  if (j >= G.manz) return;
+  const double2* col = G.J + (size_t)j * G.nanz;
+  double sr = 0.0, si = 0.0;
+  for (int i = lane; i < G.nanz; i += 64) {
+    const cplx s = cconj(ldc(col + i));
+    const cplx a = rmul(G.w[i], ldc(G.ap + i));
+    sr += s.re * a.re - s.im * a.im;
+    si += s.re * a.im + s.im * a.re;
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    sr += __shfl_xor(sr, off, 64);
+    si += __shfl_xor(si, off, 64);
+  }
+  if (lane != 0) return;
+  // R * (cgfac o p)
+  cplx rp = mkc(0.0, 0.0);
+  if (G.ltri == 0) {
+    const int M = G.manz, nx = G.nx;
+    const double* sm = G.smatm;
+    if (j > 0) rp = rmul(sm[(j - 1) + M] * G.cgfac[j - 1], ldc(G.pvec + j - 1));
+    if (j < M - 1) rp = cadd(rp, rmul(sm[j + M] * G.cgfac[j + 1], ldc(G.pvec + j + 1)));
+    if (j + 1 > nx) rp = cadd(rp, rmul(sm[(j - nx) + 2 * M] * G.cgfac[j - nx], ldc(G.pvec + j - nx)));
+    if (j + 1 < M - nx + 1) rp = cadd(rp, rmul(sm[j + 2 * M] * G.cgfac[j + nx], ldc(G.pvec + j + nx)));
+    rp = cadd(rp, rmul(sm[j] * G.cgfac[j], ldc(G.pvec + j)));
+  } else if (G.ltri == 3 || G.ltri == 4) {
+    rp = rmul(G.smatm[j], rmul(G.cgfac[j], ldc(G.pvec + j)));
+  } else {
+    const int M = G.manz;
+    for (int q = 0; q < G.maxnodes; ++q) {
+      const int nb = G.nachbar[j + (size_t)M * q];
+      if (nb != 0) rp = cadd(rp, rmul(G.cgfac[nb - 1], rmul(G.smatm[j + (size_t)M * q], ldc(G.pvec + nb - 1))));
+    }
+    rp = cadd(rp, rmul(G.cgfac[j], rmul(G.smatm[j + (size_t)M * G.maxnodes], ldc(G.pvec + j))));
+  }
+  cplx b = cadd(mkc(sr, si), rmul(G.lam, rp));
+  stc(G.bvec + j, rmul(G.cgfac[j], b));
+}
+
+// Dot products of the CG recurrence and the vector updates, on one workgroup
+// so the scalars never leave the device (the loop can be graph-captured).
+// stage 0: dr = sum |r|^2; beta; p = r + beta p
+// stage 1: dr1 = Re sum conj(p) b; alpha = dr/dr1; dpar += alpha p; r -= alpha b
+__global__ __launch_bounds__(1024) void crt_cg_scalar_kernel(CgArgs G, int stage, int iter) {
+  __shared__ double red[1024];
+  const int tid = threadIdx.x;
+  double s = 0.0;
+  if (stage == 0) {
+    for (int j = tid; j < G.manz; j += blockDim.x) {
+      const cplx r = ldc(G.rvec + j);
+      s += r.re * r.re + r.im * r.im;
+    }
+  } else {
+    for (int j = tid; j < G.manz; j += blockDim.x) {
+      const cplx p = ldc(G.pvec + j), b = ldc(G.bvec + j);
+      s += p.re * b.re + p.im * b.im;
+    }
+  }
+  red[tid] = s;
+  __syncthreads();
+  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+    if (tid < w) red[tid] += red[tid + w];
+    __syncthreads();
+  }
+  const double tot = red[0];
+  double* sc = G.scal;  // [0]=dr [1]=dr0 [2]=dr1 [3]=alpha [4]=beta [5]=done [6]=ncg
+  if (stage == 0) {
+    if (sc[5] != 0.0) return;  // uniform: every thread reads the same flag
+    const double dr0 = (iter == 0) ? tot * G.eps : sc[1];
+    if (iter > 0 && tot <= dr0) {
+      __syncthreads();
+      if (tid == 0) { sc[5] = 1.0; sc[6] = (double)iter; }
+      return;
+    }
+    const double beta = (iter == 0) ? 0.0 : tot / sc[2];
+    __syncthreads();
+    if (tid == 0) { sc[0] = tot; sc[1] = dr0; sc[4] = beta; }
+    for (int j = tid; j < G.manz; j += blockDim.x) {
+      const cplx p = ldc(G.pvec + j), r = ldc(G.rvec + j);
+      stc(G.pvec + j, cadd(r, rmul(beta, p)));
+    }
+  } else {
+    if (sc[5] != 0.0) return;
+    const double dr = sc[0];
+    const double alpha = dr / tot;
+    __syncthreads();
+    for (int j = tid; j < G.manz; j += blockDim.x) {
+      const cplx p = ldc(G.pvec + j), b = ldc(G.bvec + j);
+      stc(G.dpar + j, cadd(ldc(G.dpar + j), rmul(alpha, p)));
+      stc(G.rvec + j, csub(ldc(G.rvec + j), rmul(alpha, b)));
+    }
+    if (tid == 0) {
+      sc[3] = alpha;
+      sc[2] = dr;
+      G.cgres[iter + 1] = (double)(float)(G.eps * dr / sc[1]);
+      sc[6] = (double)(iter + 1);
+    }
+  }
+}

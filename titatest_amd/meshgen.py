@@ -28,10 +28,11 @@ import numpy as np
 from .formats import Mesh
 
 
-def _quad_mesh(xn: np.ndarray, zn: np.ndarray, mixed: bool = True) -> Mesh:
+def _quad_mesh(xn: np.ndarray, zn: np.ndarray, side_type: int = 11) -> Mesh:
     """Quadrilateral type-8 mesh on the node lines xn (left->right) and zn (top
     surface first, increasingly negative).  Node numbering runs down each column
-    (node = ix*(nz+1) + iz + 1), so mb = nz + 2."""
+    (node = ix*(nz+1) + iz + 1), so mb = nz + 2.  The top is a no-flow (type 12)
+    boundary; left, right and bottom are of ``side_type`` (11 mixed, 13 Dirichlet)."""
     nx = len(xn) - 1
     nz = len(zn) - 1
     ncol = nz + 1
@@ -58,12 +59,8 @@ def _quad_mesh(xn: np.ndarray, zn: np.ndarray, mixed: bool = True) -> Mesh:
     bottom = [(node(i + 1, nz), node(i, nz), elem_of(i, nz - 1)) for i in range(nx - 1, -1, -1)]
     left = [(node(0, j + 1), node(0, j), elem_of(0, j)) for j in range(nz - 1, -1, -1)]
     mixed_b = right + bottom + left
-    btype = 11 if mixed else 12
-    typ = np.array([8, 12, btype], np.int32) if mixed else np.array([8, 12], np.int32)
-    if mixed:
-        bnd = [top, mixed_b]
-    else:
-        bnd = [top + mixed_b]
+    typ = np.array([8, 12, side_type], np.int32)
+    bnd = [top, mixed_b]
     nel = nx * nz + sum(len(b) for b in bnd)
     nrel = np.zeros((nel, 4), np.int32)
     nrel[: nx * nz] = quads
@@ -183,7 +180,7 @@ def small_block() -> Workload:
     """40x20 quads of 1 m x 0.5 m, 20 electrodes at x = 10..29 m, block anomaly."""
     xn = np.arange(41, dtype=np.float64)
     zn = -0.5 * np.arange(21, dtype=np.float64)
-    mesh = _quad_mesh(xn, zn, mixed=True)
+    mesh = _quad_mesh(xn, zn)
     mesh.enr = surface_nodes(20, np.arange(10, 30))
     cx, cz = centroids(mesh)
     mag = np.full(mesh.elanz, 100.0)
@@ -193,6 +190,17 @@ def small_block() -> Workload:
     pha[blk] = -20.0
     st, vn = dipole_dipole(20, [1], range(1, 9))
     return Workload("small_block", mesh, mag, pha, st, vn)
+
+
+def small_dirichlet() -> Workload:
+    """As small_block but with homogeneous Dirichlet (type 13) sides and bottom:
+    no mixed boundaries, so one factor per wavenumber serves every electrode."""
+    w = small_block()
+    xn = np.arange(41, dtype=np.float64)
+    zn = -0.5 * np.arange(21, dtype=np.float64)
+    mesh = _quad_mesh(xn, zn, side_type=13)
+    mesh.enr = w.mesh.enr
+    return Workload("small_dirichlet", mesh, w.mag, w.pha_mrad, w.strnr, w.vnr)
 
 
 def _graded_axis(n_uniform, dx, n_side, factor):
@@ -231,7 +239,7 @@ def large_grf(nx_side=72, n_uniform=96, nz=80, eanz=96, seed=1001) -> Workload:
     """240x80 quads, 96 electrodes at 1.25 m, GRF model, dipole-dipole a=1..4, n=1..10."""
     xn = _graded_axis(n_uniform, 1.25, nx_side, 1.1)
     zn = _depth_axis(nz, 0.3125, 1.05)
-    mesh = _quad_mesh(xn, zn, mixed=True)
+    mesh = _quad_mesh(xn, zn)
     mesh.enr = surface_nodes(nz, nx_side + np.arange(eanz))
     mag, pha = grf_model(mesh, seed, seed + 1)
     st, vn = dipole_dipole(eanz, range(1, 5), range(1, 11))
@@ -251,7 +259,7 @@ def uneven_batch(count=64, seed=1006):
         n_uniform = nx - 2 * side
         xn = _graded_axis(n_uniform, 1.25, side, 1.1)
         zn = _depth_axis(nz, 0.3125, 1.05)
-        mesh = _quad_mesh(xn, zn, mixed=True)
+        mesh = _quad_mesh(xn, zn)
         mesh.enr = surface_nodes(nz, side + np.arange(eanz))
         mag, pha = grf_model(mesh, seed + 10 + 2 * m, seed + 11 + 2 * m)
         st, vn = dipole_dipole(eanz, range(1, 5), range(1, 11))
