@@ -235,14 +235,18 @@ def test_large_mesh_systems_match_oracle(crtmod, oracle_lib):
     o = oracle_lib.Oracle(w.mesh)
     o.rwaven(w.strnr, w.vnr)
     o.precal()
+    o.forward(w.sigma)
+    # Per-wavenumber potentials: the lowest wavenumber's system is the worst
+    # conditioned; two FP64 evaluations of the reference's own algorithm (with
+    # and without fused multiply-add) already differ by 4.4e-11 there, so the
+    # potentials are held to 1e-9 and the voltages (the parity target) to 1e-10.
     E = w.mesh.eanz
-    sample = [0, 95, 5 * E + 40, 13 * E + 95]
-    for s in sample:
-        o.forward(w.sigma, sys_range=(s, s + 1))
+    for s in [0, 95, 5 * E + 40, 13 * E + 95]:
         k, l = divmod(s, E)
-        assert rel_err(kpot[:, l, k], o.kpot[:, l, k]) < VTOL, (k, l)
+        assert rel_err(kpot[:, l, k], o.kpot[:, l, k]) < 1e-9, (k, l)
     v, sa = g.bvolt(w.strnr, w.vnr)
-    assert np.isfinite(v).all() and (np.abs(v) > 0).all()
+    vo, _ = o.volt(w.strnr, w.vnr, 0)
+    check_volt(v, vo)
 
 
 def test_triangles_large_band_match_oracle(crtmod, oracle_lib):
@@ -261,3 +265,28 @@ def test_triangles_large_band_match_oracle(crtmod, oracle_lib):
         o.forward(w.sigma, sys_range=(s, s + 1))
         k, l = divmod(s, E)
         assert rel_err(kpot[:, l, k], o.kpot[:, l, k]) < VTOL, (k, l)
+
+
+def test_fortran_host_crmod_fixture(crtmod, tmp_path):
+    """The Fortran host (iso_c_binding over the C-ABI) on the reference's
+    tests/minimal tomodir writes the reference's voltages."""
+    import subprocess
+    from pathlib import Path
+    from titatest_amd import formats as F
+    exe = Path(crtmod.__file__).resolve().parent / "crmod_hip"
+    if not exe.exists():
+        pytest.fail("Fortran host not built")
+    m, sig, st, vn, mag, pha = load_fixture("minimal")
+    from conftest import FIXTURES
+    d = FIXTURES / "minimal"
+    F.write_config(tmp_path / "config.dat", st, vn)
+    say("fortran host minimal")
+    r = subprocess.run([str(exe), str(d / "elem.dat"), str(d / "elec.dat"), str(d / "rho.dat"),
+                        str(tmp_path / "config.dat"), str(tmp_path / "volt.dat")],
+                       capture_output=True, text=True, timeout=120, cwd=tmp_path)
+    assert r.returncode == 0, r.stdout + r.stderr
+    s2, v2, mag2, pha2 = F.read_volt(tmp_path / "volt.dat")
+    ge = golden("minimal")["volt"]
+    np.testing.assert_array_equal(s2, st)
+    assert np.max(np.abs(mag2 - np.abs(ge)) / np.abs(ge)) < VTOL
+    assert np.max(np.abs(mag2 - mag) / mag) < 2e-7

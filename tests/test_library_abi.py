@@ -41,3 +41,12 @@ def test_null_arguments_are_rejected_without_a_device():
     assert lib.crt_destroy(None) == 0
     assert lib.crt_rwaven(None, 0, None, None, None, None, None) == 1000
     assert lib.crt_forward(None, None, None, None) == 1000
+
+
+def test_fortran_host_built_and_linked():
+    import subprocess
+    exe = crt.LIB_PATH.parent / "crmod_hip"
+    if not exe.exists():
+        pytest.skip("no Fortran compiler in this environment")
+    r = subprocess.run([str(exe)], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 1 and "usage" in r.stdout

@@ -55,7 +55,38 @@ def build(verbose: bool = False) -> Path:
     os.replace(tmp, OUT)
     if verbose:
         print(f"built {OUT}")
+    build_fortran_host(verbose)
     return OUT
+
+
+FORTRAN_HOST = PKG / "crmod_hip"
+
+
+def flang() -> str | None:
+    for cand in (os.environ.get("FC"), "/opt/rocm/bin/amdflang", "/opt/rocm/llvm/bin/flang"):
+        if cand and os.path.exists(cand):
+            return cand
+    return None
+
+
+def build_fortran_host(verbose: bool = False):
+    """The Fortran host (iso_c_binding module + CRMod-style driver), linked
+    against libcrt_hip.so with an $ORIGIN rpath.  Skipped without a Fortran
+    compiler."""
+    fc = flang()
+    if fc is None:
+        return None
+    fdir = PKG / "fortran"
+    cmd = [fc, "-O2", "-J", str(BUILD), "-o", str(FORTRAN_HOST),
+           str(fdir / "crt_hip_mod.f90"), str(fdir / "crmod_hip.f90"),
+           "-L", str(PKG), "-lcrt_hip", "-Wl,-rpath,$ORIGIN"]
+    r = subprocess.run(cmd, capture_output=True, text=True, cwd=str(BUILD))
+    if r.returncode != 0:
+        sys.stderr.write(r.stderr)
+        raise RuntimeError("Fortran host build failed")
+    if verbose:
+        print(f"built {FORTRAN_HOST}")
+    return FORTRAN_HOST
 
 
 if __name__ == "__main__":

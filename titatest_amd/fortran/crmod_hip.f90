@@ -1,0 +1,204 @@
+!> CRMod-style forward modelling on the MI355X: reads a tomodir's elem.dat,
+!> elec.dat, rho.dat and config.dat, runs the forward path through the C-ABI
+!> and writes volt.dat the way the reference's wdatm does (src/wdatm.f90:55-79),
+!> at full precision.
+!>
+!>   crmod_hip elem.dat elec.dat rho.dat config.dat volt.dat [swrtr [nsink]]
+!>
+!> swrtr = 1 (default) is the 2.5D forward; 0 is 2-D (one wavenumber 0).
+!> nsink > 0 adds the fictitious sink at that node (crmod.cfg lsink/nsink).
+!> The readers follow src/relem.f90:62-151 (with the rnr repair of :209-238),
+!> src/relectr.f90:46-68, src/rsigma.f90:91-189 and src/rdatm.f90:54-115.
+program crmod_hip
+  use iso_c_binding
+  use crt_hip
+  implicit none
+
+  integer(c_int32_t) :: sanz, typanz, mb, eanz, nanz, elanz, relanz, maxn, nel
+  integer(c_int32_t), allocatable, target :: typ(:), nelanz(:), selanz(:), snr(:), nrel(:,:), rnr(:), enr(:)
+  integer(c_int32_t), allocatable :: strnr(:), vnr(:)
+  real(c_double), allocatable, target :: sx(:), sy(:)
+  complex(c_double_complex), allocatable :: sigma(:), volt(:), sigmaa(:)
+  real(c_double) :: kwn(30), kwnwi(30), bet, pha, pi, npi
+  integer(c_int32_t) :: kwnanz, lbeta, lrandb, swrtr, nsink, code, kk, ll
+  type(crt_mesh) :: mesh
+  type(c_ptr) :: h
+  character(len=512) :: f_elem, f_elec, f_rho, f_cfg, f_volt, arg
+  integer :: i, j, k, idum, e1, e2, ios, rc, ie1, ie2, ic, jk1, jk2, l
+  real(c_double) :: rb, rp
+
+  if (command_argument_count() < 5) then
+     write (*, '(a)') 'usage: crmod_hip elem.dat elec.dat rho.dat config.dat volt.dat [swrtr [nsink]]'
+     stop 1
+  end if
+  call get_command_argument(1, f_elem)
+  call get_command_argument(2, f_elec)
+  call get_command_argument(3, f_rho)
+  call get_command_argument(4, f_cfg)
+  call get_command_argument(5, f_volt)
+  swrtr = 1
+  nsink = 0
+  if (command_argument_count() >= 6) then
+     call get_command_argument(6, arg)
+     read (arg, *) swrtr
+  end if
+  if (command_argument_count() >= 7) then
+     call get_command_argument(7, arg)
+     read (arg, *) nsink
+  end if
+  pi = acos(-1d0)
+
+  ! ---- elem.dat
+  open (10, file=trim(f_elem), status='old', action='read')
+  read (10, *) sanz, typanz, mb
+  allocate (typ(typanz), nelanz(typanz), selanz(typanz))
+  read (10, *) (typ(i), nelanz(i), selanz(i), i = 1, typanz)
+  maxn = maxval(selanz)
+  elanz = 0
+  relanz = 0
+  do i = 1, typanz
+     if (typ(i) > 10) then
+        relanz = relanz + nelanz(i)
+     else
+        elanz = elanz + nelanz(i)
+     end if
+  end do
+  nel = elanz + relanz
+  allocate (snr(sanz), sx(sanz), sy(sanz), nrel(nel, maxn), rnr(max(relanz, 1)))
+  read (10, *) (snr(i), sx(i), sy(i), i = 1, sanz)
+  nrel = 0
+  idum = 0
+  do i = 1, typanz
+     do j = 1, nelanz(i)
+        read (10, *) (nrel(idum + j, k), k = 1, selanz(i))
+     end do
+     idum = idum + nelanz(i)
+  end do
+  if (relanz > 0) read (10, *) (rnr(i), i = 1, relanz)
+  close (10)
+  ! link each boundary element to the area element holding both its nodes
+  do i = 1, relanz
+     do k = 1, elanz
+        jk1 = 0
+        jk2 = 0
+        do l = 1, selanz(1)
+           if (nrel(k, l) == nrel(elanz + i, 1)) jk1 = 1
+           if (nrel(k, l) == nrel(elanz + i, 2)) jk2 = 1
+        end do
+        if (jk1 == 1 .and. jk2 == 1) rnr(i) = k
+     end do
+  end do
+
+  ! ---- elec.dat
+  open (11, file=trim(f_elec), status='old', action='read')
+  read (11, *) eanz
+  allocate (enr(eanz))
+  do i = 1, eanz
+     read (11, *) enr(i)
+  end do
+  close (11)
+
+  ! ---- rho.dat: sigma = (cos(phi)/|rho|, -sin(phi)/|rho|), phi in mrad
+  open (12, file=trim(f_rho), status='old', action='read')
+  read (12, *) idum
+  if (idum /= elanz) then
+     write (*, '(a)') 'rho.dat does not match the mesh (errnr 47)'
+     stop 47
+  end if
+  allocate (sigma(elanz))
+  do i = 1, elanz
+     read (12, *) rb, rp
+     rp = 1d-3 * rp
+     sigma(i) = dcmplx(dcos(rp) / rb, -dsin(rp) / rb)
+  end do
+  close (12)
+
+  ! ---- config.dat (A*10^4+B  M*10^4+N, or A B M N)
+  open (13, file=trim(f_cfg), status='old', action='read')
+  read (13, *) nanz
+  allocate (strnr(nanz), vnr(nanz), volt(nanz), sigmaa(nanz))
+  read (13, *) idum
+  backspace (13)
+  do i = 1, nanz
+     if (idum >= 10000) then
+        read (13, *) strnr(i), vnr(i)
+     else
+        read (13, *) e1, e2, ie1, ie2
+        strnr(i) = e1 * 10000 + e2
+        vnr(i) = ie1 * 10000 + ie2
+     end if
+  end do
+  close (13)
+
+  ! ---- the forward path on the device
+  mesh%sanz = sanz
+  mesh%typanz = typanz
+  mesh%mb = mb
+  mesh%eanz = eanz
+  mesh%typ = c_loc(typ)
+  mesh%nelanz = c_loc(nelanz)
+  mesh%selanz = c_loc(selanz)
+  mesh%nrel = c_loc(nrel)
+  mesh%rnr = c_loc(rnr)
+  mesh%snr = c_loc(snr)
+  mesh%sx = c_loc(sx)
+  mesh%sy = c_loc(sy)
+  mesh%enr = c_loc(enr)
+  mesh%lsytop = 0
+  mesh%lsink = merge(1, 0, nsink > 0)
+  mesh%nsink = nsink
+  rc = crt_create(h, mesh)
+  call check(rc, 'crt_create')
+  if (swrtr == 1) then
+     rc = crt_rwaven(h, nanz, strnr, vnr, kwnanz, kwn, kwnwi)
+     call check(rc, 'crt_rwaven')
+  else
+     kwn(1) = 0d0
+     kwnwi(1) = 1d0
+     rc = crt_set_wavenumbers(h, 1, kwn, kwnwi, 0)
+     call check(rc, 'crt_set_wavenumbers')
+  end if
+  rc = crt_precal(h, lbeta, lrandb)
+  call check(rc, 'crt_precal')
+  rc = crt_forward(h, sigma, c_null_ptr, c_null_ptr)
+  call check(rc, 'crt_forward')
+  rc = crt_voltages(h, nanz, strnr, vnr, 0, c_null_ptr, volt, sigmaa)
+  call check(rc, 'crt_voltages')
+  rc = crt_destroy(h)
+
+  ! ---- volt.dat as wdatm writes it (polarity swap when the phase is near pi)
+  open (14, file=trim(f_volt), status='replace', action='write')
+  write (14, *) nanz
+  do i = 1, nanz
+     bet = abs(sigmaa(i))
+     pha = atan2(dimag(sigmaa(i)), dble(sigmaa(i)))
+     npi = dnint(pha / pi) * pi
+     if (dabs(npi) > 1d-12) then
+        pha = pha - npi
+        ie1 = mod(vnr(i), 10000)
+        ie2 = (vnr(i) - ie1) / 10000
+        vnr(i) = ie1 * 10000 + ie2
+     end if
+     write (14, '(2I12,2ES26.17)') strnr(i), vnr(i), bet, 1d3 * pha
+  end do
+  close (14)
+  ic = 0
+
+contains
+
+  subroutine check(rcode, where)
+    integer, intent(in) :: rcode
+    character(len=*), intent(in) :: where
+    if (rcode /= 0) then
+       code = 0
+       kk = 0
+       ll = 0
+       write (*, '(a,a,i6)') trim(where), ': error ', rcode
+       open (15, file='error.dat', status='replace')
+       write (15, '(i6)') rcode
+       close (15)
+       stop 2
+    end if
+  end subroutine check
+
+end program crmod_hip

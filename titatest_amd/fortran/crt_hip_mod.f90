@@ -1,0 +1,108 @@
+!> iso_c_binding interface to libcrt_hip.so (include/crt_hip.h).
+!>
+!> This is the module a CRMod/CRTomo host `USE`s to hand its stages to the
+!> MI355X: the arrays it passes are its own module arrays (elemmod, electrmod,
+!> sigmamod, datmod, alloci), unchanged, column major with 1-based index values.
+!> Complex arrays are COMPLEX(KIND(0D0)) and travel as C double pointers.
+module crt_hip
+  use iso_c_binding
+  implicit none
+
+  !> Mirrors struct crt_mesh.
+  type, bind(C) :: crt_mesh
+     integer(c_int32_t) :: sanz, typanz, mb, eanz
+     type(c_ptr) :: typ, nelanz, selanz, nrel, rnr, snr, sx, sy, enr
+     integer(c_int32_t) :: lsytop, lsink, nsink
+  end type crt_mesh
+
+  interface
+     integer(c_int) function crt_create(h, mesh) bind(C, name='crt_create')
+       import :: c_int, c_ptr, crt_mesh
+       type(c_ptr) :: h
+       type(crt_mesh), intent(in) :: mesh
+     end function crt_create
+
+     integer(c_int) function crt_destroy(h) bind(C, name='crt_destroy')
+       import :: c_int, c_ptr
+       type(c_ptr), value :: h
+     end function crt_destroy
+
+     integer(c_int) function crt_rwaven(h, nanz, strnr, vnr, kwnanz, kwn, kwnwi) &
+          bind(C, name='crt_rwaven')
+       import :: c_int, c_ptr, c_int32_t, c_double
+       type(c_ptr), value :: h
+       integer(c_int32_t), value :: nanz
+       integer(c_int32_t), intent(in) :: strnr(*), vnr(*)
+       integer(c_int32_t), intent(out) :: kwnanz
+       real(c_double), intent(out) :: kwn(30), kwnwi(30)
+     end function crt_rwaven
+
+     integer(c_int) function crt_set_wavenumbers(h, kwnanz, kwn, kwnwi, swrtr) &
+          bind(C, name='crt_set_wavenumbers')
+       import :: c_int, c_ptr, c_int32_t, c_double
+       type(c_ptr), value :: h
+       integer(c_int32_t), value :: kwnanz, swrtr
+       real(c_double), intent(in) :: kwn(*), kwnwi(*)
+     end function crt_set_wavenumbers
+
+     integer(c_int) function crt_precal(h, lbeta, lrandb) bind(C, name='crt_precal')
+       import :: c_int, c_ptr, c_int32_t
+       type(c_ptr), value :: h
+       integer(c_int32_t), intent(out) :: lbeta, lrandb
+     end function crt_precal
+
+     integer(c_int) function crt_forward(h, sigma, kpot, hpot) bind(C, name='crt_forward')
+       import :: c_int, c_ptr, c_double_complex
+       type(c_ptr), value :: h
+       complex(c_double_complex), intent(in) :: sigma(*)
+       type(c_ptr), value :: kpot, hpot
+     end function crt_forward
+
+     integer(c_int) function crt_voltages(h, nanz, strnr, vnr, mode, kfak, volt, sigmaa) &
+          bind(C, name='crt_voltages')
+       import :: c_int, c_ptr, c_int32_t, c_double_complex
+       type(c_ptr), value :: h
+       integer(c_int32_t), value :: nanz, mode
+       integer(c_int32_t), intent(in) :: strnr(*), vnr(*)
+       type(c_ptr), value :: kfak
+       complex(c_double_complex), intent(out) :: volt(*), sigmaa(*)
+     end function crt_voltages
+
+     integer(c_int) function crt_sensitivities(h, mode, sens) bind(C, name='crt_sensitivities')
+       import :: c_int, c_ptr, c_int32_t
+       type(c_ptr), value :: h
+       integer(c_int32_t), value :: mode
+       type(c_ptr), value :: sens
+     end function crt_sensitivities
+
+     integer(c_int) function crt_adjoint_gradient(h, mode, w, resid, grad) &
+          bind(C, name='crt_adjoint_gradient')
+       import :: c_int, c_ptr, c_int32_t, c_double, c_double_complex
+       type(c_ptr), value :: h
+       integer(c_int32_t), value :: mode
+       real(c_double), intent(in) :: w(*)
+       complex(c_double_complex), intent(in) :: resid(*)
+       complex(c_double_complex), intent(out) :: grad(*)
+     end function crt_adjoint_gradient
+
+     integer(c_int) function crt_cg_solve(h, w, dat, sigmaa, lam, ltri, nx, smatm, smatm_cols, &
+          nachbar, maxnodes, rdiag, rm, ncgmax, eps, dpar, cgfac, cgres) bind(C, name='crt_cg_solve')
+       import :: c_int, c_ptr, c_int32_t, c_double, c_double_complex
+       type(c_ptr), value :: h
+       real(c_double), intent(in) :: w(*)
+       complex(c_double_complex), intent(in) :: dat(*), sigmaa(*)
+       real(c_double), value :: lam, eps
+       integer(c_int32_t), value :: ltri, nx, smatm_cols, maxnodes, ncgmax
+       real(c_double), intent(in) :: smatm(*), rdiag(*)
+       type(c_ptr), value :: nachbar, rm
+       complex(c_double_complex), intent(out) :: dpar(*)
+       real(c_double), intent(out) :: cgfac(*), cgres(*)
+     end function crt_cg_solve
+
+     integer(c_int) function crt_error_info(h, code, k, l) bind(C, name='crt_error_info')
+       import :: c_int, c_ptr, c_int32_t
+       type(c_ptr), value :: h
+       integer(c_int32_t), intent(out) :: code, k, l
+     end function crt_error_info
+  end interface
+end module crt_hip
