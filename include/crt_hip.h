@@ -129,6 +129,12 @@ int crt_error_info(crt_handle *h, int32_t *code, int32_t *k, int32_t *l);
  * handle's stream).  names: n_max slots of 48 chars; returns the count. */
 int crt_set_timing(crt_handle *h, int32_t enable);
 int crt_stage_times(crt_handle *h, int32_t n_max, char *names, double *ms, int32_t *launches);
+/* Phase clock of the factor kernel: enable != 0 turns it on for the next
+ * forwards; ticks8 (may be NULL) receives the accumulated wall-clock ticks
+ * (100 MHz) of its phases summed over workgroups since the last call: scaling
+ * pre-pass, first window, panel factorisation, L store, trailing update, row
+ * assembly, back substitution. */
+int crt_factor_phase_ticks(crt_handle *h, int32_t enable, uint64_t *ticks8);
 /* Device pointer of the last forward's potentials kpot (sanz x eanz x K complex). */
 int crt_kpot_device(crt_handle *h, const double **kpot_dev);
 

@@ -36,7 +36,8 @@ EXPORTS = (
     "crt_get_precal", "crt_forward", "crt_forward_dev", "crt_voltages", "crt_voltages_dev",
     "crt_sensitivities", "crt_adjoint_gradient", "crt_adjoint_gradient_dev", "crt_jhw_r",
     "crt_cg_solve",
-    "crt_error_info", "crt_set_timing", "crt_stage_times", "crt_kpot_device",
+    "crt_error_info", "crt_set_timing", "crt_stage_times", "crt_factor_phase_ticks",
+    "crt_kpot_device",
 )
 
 ERRORS = {
@@ -98,6 +99,7 @@ def load_library(path: Path = LIB_PATH):
     L.crt_set_timing.argtypes = [vp, i32]
     L.crt_stage_times.argtypes = [vp, i32, vp, vp, vp]
     L.crt_kpot_device.argtypes = [vp, C.POINTER(vp)]
+    L.crt_factor_phase_ticks.argtypes = [vp, i32, vp]
     _lib = L
     return L
 
@@ -295,6 +297,16 @@ class CRT:
             nm = raw[48 * i:48 * (i + 1)].split(b"\0", 1)[0].decode()
             out[nm] = (float(ms[i]), int(cnt[i]))
         return out
+
+    PHASES = ("scaling", "first_window", "panel", "store_L", "trailing", "assembly",
+              "backsolve", "unused")
+
+    def factor_phases(self, enable=True):
+        """Return the factor kernel's phase times (ms summed over workgroups)
+        accumulated since the previous call, and switch the clock on/off."""
+        t = np.zeros(8, np.uint64)
+        self._check("factor_phase_ticks", self.lib.crt_factor_phase_ticks(self.h, int(enable), _ptr(t)))
+        return {k: float(v) * 1e-5 for k, v in zip(self.PHASES, t) if k != "unused"}
 
     def error_info(self):
         c, k, l = C.c_int32(), C.c_int32(), C.c_int32()

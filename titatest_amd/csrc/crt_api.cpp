@@ -54,7 +54,12 @@ struct crt_handle {
   std::map<std::string, void*> buf;
   std::map<std::string, size_t> bufsz;
   int slots = 0;
+  int R = 1;
+  size_t batch_bytes_max = (size_t)64 << 30;  // device bytes for factors in flight
   int last_err = 0, last_k = 0, last_l = 0;
+  // factor-kernel phase clock (wall-clock ticks summed over workgroups)
+  bool kprof = false;
+  unsigned long long kprof_ticks[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   // timing
   bool timing = false;
   std::vector<Timer> pending;
@@ -367,16 +372,50 @@ int crt_create(crt_handle** out, const crt_mesh* m) {
   std::stable_sort(terms.begin(), terms.end(), [](const Term& a, const Term& b) {
     return a.row != b.row ? a.row < b.row : a.t < b.t;
   });
-  std::vector<int> rowptr(h->n + 1, 0), te(terms.size());
-  std::vector<int16_t> tt(terms.size());
-  std::vector<int8_t> timn(terms.size());
-  for (size_t p = 0; p < terms.size(); ++p) {
-    rowptr[terms[p].row + 1]++;
-    tt[p] = (int16_t)terms[p].t;
-    timn[p] = (int8_t)terms[p].imn;
-    te[p] = terms[p].e;
+  // row pattern (distinct band offsets, ascending) and the split into area
+  // terms (summed once per wavenumber) and mixed-boundary terms (per source)
+  std::vector<std::vector<int>> pat(h->n);
+  for (const Term& t : terms)
+    if (pat[t.row].empty() || pat[t.row].back() != t.t) pat[t.row].push_back(t.t);
+  for (int r = 0; r < h->n; ++r)
+    if (pat[r].empty() || pat[r].back() != h->mb) pat[r].push_back(h->mb);  // diagonal slot
+  int R = 1;
+  for (auto& v : pat) R = std::max<int>(R, (int)v.size());
+  if (R > kMaxR) {
+    crt_destroy(h);
+    return kErrBand;
   }
-  for (int r = 0; r < h->n; ++r) rowptr[r + 1] += rowptr[r];
+  h->R = R;
+  std::vector<int16_t> pat_t((size_t)h->n * R, -1);
+  std::vector<int8_t> pat_diag(h->n, 0);
+  for (int r = 0; r < h->n; ++r)
+    for (size_t q = 0; q < pat[r].size(); ++q) {
+      pat_t[(size_t)r * R + q] = (int16_t)pat[r][q];
+      if (pat[r][q] == h->mb) pat_diag[r] = (int8_t)q;
+    }
+  std::vector<int> rowptr(h->n + 1, 0), te, bptr(h->n + 1, 0), bcode;
+  std::vector<int16_t> tt;
+  std::vector<int8_t> timn, bslot, bimn;
+  for (const Term& t : terms) {
+    if (t.e < 0 && t.e != kDirichletUnit) {
+      const auto& pr = pat[t.row];
+      bptr[t.row + 1]++;
+      bslot.push_back((int8_t)(std::find(pr.begin(), pr.end(), t.t) - pr.begin()));
+      bcode.push_back(t.e);
+      bimn.push_back((int8_t)t.imn);
+    } else {
+      rowptr[t.row + 1]++;
+      tt.push_back((int16_t)t.t);
+      timn.push_back((int8_t)t.imn);
+      te.push_back(t.e);
+    }
+  }
+  for (int r = 0; r < h->n; ++r) {
+    rowptr[r + 1] += rowptr[r];
+    bptr[r + 1] += bptr[r];
+  }
+  if (te.empty()) { tt.push_back(0); timn.push_back(0); te.push_back(0); }
+  if (bcode.empty()) { bslot.push_back(0); bcode.push_back(-1); bimn.push_back(0); }
 
   // ---- element / boundary / node arrays for the kernels ----
   std::vector<int> el_type(h->elanz), el_nodes((size_t)h->elanz * 4), el_nk(h->elanz);
@@ -416,6 +455,12 @@ int crt_create(crt_handle** out, const crt_mesh* m) {
   rc |= upload(h, "term_t", tt.data(), tt.size(), &d16);
   rc |= upload(h, "term_imn", timn.data(), timn.size(), &d8);
   rc |= upload(h, "term_e", te.data(), te.size(), &di);
+  rc |= upload(h, "pat_t", pat_t.data(), pat_t.size(), &d16);
+  rc |= upload(h, "pat_diag", pat_diag.data(), pat_diag.size(), &d8);
+  rc |= upload(h, "bptr", bptr.data(), bptr.size(), &di);
+  rc |= upload(h, "bslot", bslot.data(), bslot.size(), &d8);
+  rc |= upload(h, "bcode", bcode.data(), bcode.size(), &di);
+  rc |= upload(h, "bimn", bimn.data(), bimn.size(), &d8);
   rc |= upload(h, "el_type", el_type.data(), el_type.size(), &di);
   rc |= upload(h, "el_nodes", el_nodes.data(), el_nodes.size(), &di);
   rc |= upload(h, "el_nk", el_nk.data(), el_nk.size(), &di);
@@ -618,14 +663,14 @@ int crt_get_precal(crt_handle* h, double* elbg, double* relbg, double* kg) {
 static int run_forward(crt_handle* h) {
   const int n = h->n, mb = h->mb, E = h->E, K = h->K;
   const size_t lds = crt_factor_lds_bytes(mb, kNB);
-  if (lds > h->lds_max) return kErrBand;
-  if (mb > 124) return kErrBand;  // two 4x4 trailing tiles per thread
-  auto kern = crt_factor_kernel<kNB>;
+  if (lds > h->lds_max || mb + kNB > 256 || mb > 124 || mb > 255) return kErrBand;
+  // trailing update: 4x4 tiles of the (mb x mb) triangle, one or two per thread
+  const int ntile = ((mb + 3) / 4) * ((mb + 3) / 4 + 1) / 2;
+  auto kern = (ntile <= 256) ? crt_factor_kernel<kNB, 1> : crt_factor_kernel<kNB, 2>;
   hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   int per_cu = 0;
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, 256, lds) != hipSuccess || per_cu < 1) per_cu = 1;
   const int nsys = K * E;
-  h->slots = std::min(nsys, h->ncu * per_cu);
   FactorArgs A;
   A.n = n;
   A.mb = mb;
@@ -633,12 +678,17 @@ static int run_forward(crt_handle* h) {
   A.elanz = h->elanz;
   A.relanz = h->relanz;
   A.nimn = h->nimn;
-  A.nsys = nsys;
-  A.sys_list = nullptr;
   A.rowptr = h->at<int>("rowptr");
   A.term_t = h->at<int16_t>("term_t");
   A.term_imn = h->at<int8_t>("term_imn");
   A.term_e = h->at<int>("term_e");
+  A.R = h->R;
+  A.pat_t = h->at<int16_t>("pat_t");
+  A.pat_diag = h->at<int8_t>("pat_diag");
+  A.bptr = h->at<int>("bptr");
+  A.bslot = h->at<int8_t>("bslot");
+  A.bcode = h->at<int>("bcode");
+  A.bimn = h->at<int8_t>("bimn");
   A.elbg = h->at<double>("elbg");
   A.relbg = h->at<double>("relbg");
   A.kg = h->at<double>("kg");
@@ -648,31 +698,81 @@ static int run_forward(crt_handle* h) {
   A.isdir = h->has13 ? h->at<uint8_t>("isdir") : nullptr;
   A.lsink = h->lsink;
   A.nsink = h->nsink - 1;
-  A.L = h->get<double2>("L", (size_t)h->slots * n * (mb + 1));
-  A.y = h->get<double2>("y", (size_t)h->slots * n);
-  A.fak = h->get<double>("fak", (size_t)h->slots * n);
-  A.kpot = h->get<double2>("kpot", (size_t)K * E * n);
+  double2* av = h->get<double2>("av", (size_t)K * n * h->R);
+  double2* kpot = h->get<double2>("kpot", (size_t)K * E * n);
+  NEED(av);
+  NEED(kpot);
+  A.av = av;
+  if (clear_err(h)) return kErrHip;
+  A.err = h->at<int>("err");
+  A.prof = nullptr;
+  if (h->kprof) {
+    A.prof = h->get<unsigned long long>("kprof", 8);
+    NEED(A.prof);
+    HIPCHK(hipMemsetAsync(A.prof, 0, 8 * sizeof(unsigned long long), h->stream));
+  }
+  // per-wavenumber area sums
+  tstart(h, "area_values");
+  hipLaunchKernelGGL(crt_area_values_kernel, dim3((unsigned)(((size_t)K * n + 255) / 256)), dim3(256), 0, h->stream,
+                     A, K, av);
+  tstop(h);
+  // systems per batch: L, y and the scaling factors of every system in the
+  // batch stay in device memory until its back substitution
+  const size_t per_sys = (size_t)n * (mb + 1) * 16 + (size_t)n * 24;
+  size_t free_b = 0, total_b = 0;
+  if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) free_b = (size_t)8 << 30;
+  size_t budget = std::min<size_t>(free_b / 2, h->batch_bytes_max);
+  const size_t have = h->bufsz.count("L") ? h->bufsz["L"] + h->bufsz["y"] + h->bufsz["fak"] : 0;
+  budget = std::max(budget, have);
+  int batch = (int)std::max<size_t>(1, std::min<size_t>((size_t)nsys, budget / per_sys));
+  A.L = h->get<double2>("L", (size_t)batch * n * (mb + 1));
+  A.y = h->get<double2>("y", (size_t)batch * n);
+  A.fak = h->get<double>("fak", (size_t)batch * n);
   NEED(A.L);
   NEED(A.y);
   NEED(A.fak);
-  NEED(A.kpot);
-  if (clear_err(h)) return kErrHip;
-  A.err = h->at<int>("err");
-  tstart(h, "factor_solve");
-  hipLaunchKernelGGL(kern, dim3(h->slots), dim3(256), lds, h->stream, A);
-  tstop(h);
+  BacksolveArgs B;
+  B.n = n;
+  B.mb = mb;
+  B.eanz = E;
+  B.L = A.L;
+  B.y = A.y;
+  B.fak = A.fak;
+  B.kpot = kpot;
+  for (int first = 0; first < nsys; first += batch) {
+    const int cnt = std::min(batch, nsys - first);
+    A.sys_first = first;
+    A.nsys = cnt;
+    h->slots = std::min(cnt, h->ncu * per_cu);
+    tstart(h, "factor_solve");
+    hipLaunchKernelGGL(kern, dim3(h->slots), dim3(256), lds, h->stream, A);
+    tstop(h);
+    B.sys_first = first;
+    B.nsys = cnt;
+    tstart(h, "backsolve");
+    if (mb < 128)
+      hipLaunchKernelGGL((crt_backsolve_kernel<2, 8>), dim3(cnt), dim3(64), 0, h->stream, B);
+    else
+      hipLaunchKernelGGL((crt_backsolve_kernel<4, 4>), dim3(cnt), dim3(64), 0, h->stream, B);
+    tstop(h);
+  }
   // electrode slice of the back-transform, used by the voltage stage
   double2* he = h->get<double2>("he", (size_t)E * E);
   NEED(he);
   tstart(h, "rtrafo_elec");
   hipLaunchKernelGGL(crt_rtrafo_elec_kernel, dim3((E * E + 255) / 256), dim3(256), 0, h->stream,
-                     (const double2*)A.kpot, (const double*)h->at<double>("kwnwi"), (const int*)A.enr, n, E, K,
+                     (const double2*)kpot, (const double*)h->at<double>("kwnwi"), (const int*)A.enr, n, E, K,
                      h->swrtr, he);
   tstop(h);
   int rc = finish(h);
   if (rc) return rc;
   int e = read_err(h);
   if (e) return e;
+  if (h->kprof) {
+    unsigned long long t[8];
+    HIPCHK(hipMemcpy(t, A.prof, sizeof(t), hipMemcpyDeviceToHost));
+    for (int i = 0; i < 8; ++i) h->kprof_ticks[i] += t[i];
+  }
   h->forward_done = true;
   h->volt_done = h->sens_done = false;
   return 0;
@@ -1017,6 +1117,15 @@ int crt_stage_times(crt_handle* h, int32_t n_max, char* names, double* ms, int32
     ++i;
   }
   return i;
+}
+
+int crt_factor_phase_ticks(crt_handle* h, int32_t enable, uint64_t* ticks8) {
+  if (!h) return kErrArg;
+  if (ticks8)
+    for (int i = 0; i < 8; ++i) ticks8[i] = h->kprof_ticks[i];
+  h->kprof = enable != 0;
+  for (int i = 0; i < 8; ++i) h->kprof_ticks[i] = 0;
+  return 0;
 }
 
 int crt_kpot_device(crt_handle* h, const double** kpot_dev) {

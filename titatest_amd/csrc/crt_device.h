@@ -37,17 +37,20 @@ __device__ __forceinline__ cplx rmul(double r, cplx z) { return mkc(r * z.re, r 
 __device__ __forceinline__ cplx cconj(cplx z) { return mkc(z.re, -z.im); }
 __device__ __forceinline__ double cabs_(cplx z) { return hypot(z.re, z.im); }
 
+// logb(m) for finite nonzero m is frexp's exponent minus one; the power-of-two
+// scalings are exact (v_frexp_exp_i32_f64, v_ldexp_f64).
 __device__ __forceinline__ cplx cdiv(cplx a, cplx b) {
   double c = b.re, d = b.im;
-  double lg = logb(fmax(fabs(c), fabs(d)));
+  const double m = fmax(fabs(c), fabs(d));
   int il = 0;
-  if (isfinite(lg)) {
-    il = (int)lg;
-    c = scalbn(c, -il);
-    d = scalbn(d, -il);
+  if (m != 0.0 && isfinite(m)) {
+    il = __builtin_amdgcn_frexp_exp(m) - 1;
+    c = __builtin_amdgcn_ldexp(c, -il);
+    d = __builtin_amdgcn_ldexp(d, -il);
   }
-  double den = c * c + d * d;
-  return mkc(scalbn((a.re * c + a.im * d) / den, -il), scalbn((a.im * c - a.re * d) / den, -il));
+  const double den = c * c + d * d;
+  return mkc(__builtin_amdgcn_ldexp((a.re * c + a.im * d) / den, -il),
+             __builtin_amdgcn_ldexp((a.im * c - a.re * d) / den, -il));
 }
 
 __device__ __forceinline__ cplx csqrt_(cplx z) {
@@ -59,6 +62,15 @@ __device__ __forceinline__ cplx csqrt_(cplx z) {
 }
 
 __device__ __forceinline__ cplx clog_(cplx z) { return mkc(log(cabs_(z)), atan2(z.im, z.re)); }
+
+// Workgroup barrier ordering LDS only.  __syncthreads() also waits for every
+// outstanding global store of the calling wave; kernels whose threads only
+// communicate through LDS use this instead so stores stream out behind it.
+__device__ __forceinline__ void lds_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
 
 __device__ __forceinline__ cplx ldc(const double2* p) {
   double2 v = *p;

@@ -8,18 +8,30 @@
 
 // Assembly term marking "this row is a Dirichlet row: unit diagonal".
 constexpr int kDirichletUnit = -0x7fffffff;
+// Stored band entries per matrix row (pattern slots) the factor kernel keeps
+// in registers.
+constexpr int kMaxR = 8;
 
-// Arguments of the factor + forward-sweep + back-sweep kernel.  Device
-// pointers; all node / element indices here are 0-based.
+// Arguments of the assembly + factor + forward-sweep kernel.  Device pointers;
+// node / element indices are 0-based.
 struct FactorArgs {
   int n, mb, eanz, elanz, relanz, nimn;
-  int nsys;                 // systems to solve
-  const int* sys_list;      // system ids (k*eanz + l), or null for 0..nsys-1
-  // row assembly plan (CSR over rows, terms sorted by band offset then element)
+  int nsys;                 // systems in this batch
+  int sys_first;            // system id (k*eanz + l) of batch entry 0
+  // area-term plan (CSR over rows, terms sorted by band offset then element)
   const int* rowptr;        // n+1
   const int16_t* term_t;    // band offset t = c - r + mb
   const int8_t* term_imn;   // packed element-matrix index
-  const int* term_e;        // >= 0 area element, < 0: -(boundary element + 1)
+  const int* term_e;        // >= 0 area element, kDirichletUnit for Dirichlet rows
+  // row pattern: R stored entries per row (slots in ascending t, -1 = unused)
+  int R;
+  const int16_t* pat_t;     // n x R
+  const int8_t* pat_diag;   // n: slot of the diagonal
+  // mixed-boundary terms per row (CSR, element order)
+  const int* bptr;          // n+1
+  const int8_t* bslot;      // pattern slot
+  const int* bcode;         // -(boundary element + 1)
+  const int8_t* bimn;       // packed boundary-matrix index
   const double* elbg;       // [K][elanz][nimn]
   const double* relbg;      // [relanz][nimn]
   const double* kg;         // [K][eanz][relanz]
@@ -28,26 +40,31 @@ struct FactorArgs {
   const int* enr;           // eanz, electrode nodes
   const uint8_t* isdir;     // n Dirichlet flags, or null
   int lsink, nsink;
-  // per-slot scratch (slot = workgroup)
-  double2* L;               // [slots][n][mb+1]
-  double2* y;               // [slots][n]
-  double* fak;              // [slots][n]
-  // output potentials kpot[k][l][n] (the reference's kpot(sanz,eanz,kwnanz))
-  double2* kpot;
-  int* err;                 // first error code (0 = ok)
+  const double2* av;        // [K][n][R] area sums (crt_area_values_kernel)
+  // per-system outputs of the batch (index = system - sys_first)
+  double2* L;               // [nsys][n][mb+1]
+  double2* y;               // [nsys][n]
+  double* fak;              // [nsys][n]
+  int* err;                 // first error code (0 = ok) and the failing system
+  unsigned long long* prof; // optional: 8 phase-time accumulators (wall clock ticks)
 };
 
-template <int NB>
+struct BacksolveArgs {
+  int n, mb, eanz, nsys, sys_first;
+  const double2* L;
+  const double2* y;
+  const double* fak;
+  double2* kpot;            // [K][eanz][n]
+};
+
+__global__ void crt_area_values_kernel(FactorArgs A, int K, double2* av);
+template <int NB, int MAXT>
 __global__ void crt_factor_kernel(FactorArgs A);
+template <int SLOTS, int DEPTH>
+__global__ void crt_backsolve_kernel(BacksolveArgs B);
 
-__device__ void crt_backsolve_block(const FactorArgs& A, int sys, int slot, cplx* smem);
-
-// LDS bytes the factor kernel needs for bandwidth mb.
+// LDS bytes the factor kernel needs for bandwidth mb and panel width nb.
 inline size_t crt_factor_lds_bytes(int mb, int nb) {
   size_t W = (size_t)mb + nb;
-  size_t fac = (W * (W + 1) / 2 + W) * 16;
-  size_t P = 1;
-  while (P < (size_t)mb + 16 + 1) P <<= 1;
-  size_t back = (P + 16 * ((size_t)mb + 1)) * 16;
-  return fac > back ? fac : back;
+  return (W * (W + 1) / 2 + W + 2 * (size_t)nb + 4) * 16 + W * 8;
 }
