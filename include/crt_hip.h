@@ -131,9 +131,11 @@ int crt_set_timing(crt_handle *h, int32_t enable);
 int crt_stage_times(crt_handle *h, int32_t n_max, char *names, double *ms, int32_t *launches);
 /* Phase clock of the factor kernel: enable != 0 turns it on for the next
  * forwards; ticks8 (may be NULL) receives the accumulated wall-clock ticks
- * (100 MHz) of its phases summed over workgroups since the last call: scaling
- * pre-pass, first window, panel factorisation, L store, trailing update, row
- * assembly, back substitution. */
+ * (100 MHz) since the last call, summed over workgroups: for the panel wave
+ * (wave 0) the prologue, the wait for the look-ahead update, the panel and the
+ * wait for the trailing update; for the trailing waves (wave 1) the prologue
+ * with the look-ahead update, its barrier wait, the trailing update and the
+ * wait for the panel. */
 int crt_factor_phase_ticks(crt_handle *h, int32_t enable, uint64_t *ticks8);
 /* Device pointer of the last forward's potentials kpot (sanz x eanz x K complex). */
 int crt_kpot_device(crt_handle *h, const double **kpot_dev);

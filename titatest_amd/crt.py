@@ -298,15 +298,17 @@ class CRT:
             out[nm] = (float(ms[i]), int(cnt[i]))
         return out
 
-    PHASES = ("scaling", "first_window", "panel", "store_L", "trailing", "assembly",
-              "backsolve", "unused")
+    # wave 0 (panel wave) then waves 1-3 (trailing waves), per stage phase
+    # wave 0 (panel wave) then waves 1-3 (trailing waves), per stage phase
+    PHASES = ("w0_prologue", "w0_wait_lookahead", "w0_panel", "w0_wait_trailing",
+              "w1_prologue_lookahead", "w1_wait_lookahead", "w1_trailing", "w1_wait_panel")
 
     def factor_phases(self, enable=True):
         """Return the factor kernel's phase times (ms summed over workgroups)
         accumulated since the previous call, and switch the clock on/off."""
         t = np.zeros(8, np.uint64)
         self._check("factor_phase_ticks", self.lib.crt_factor_phase_ticks(self.h, int(enable), _ptr(t)))
-        return {k: float(v) * 1e-5 for k, v in zip(self.PHASES, t) if k != "unused"}
+        return {k: float(v) * 1e-5 for k, v in zip(self.PHASES, t)}
 
     def error_info(self):
         c, k, l = C.c_int32(), C.c_int32(), C.c_int32()

@@ -21,7 +21,6 @@ constexpr int kErrArg = 1000;
 constexpr int kErrHip = 1001;
 constexpr int kErrOrder = 1002;
 constexpr int kErrBand = 1003;
-constexpr int kNB = 16;
 
 struct Timer {
   std::string name;
@@ -662,11 +661,20 @@ int crt_get_precal(crt_handle* h, double* elbg, double* relbg, double* kg) {
 
 static int run_forward(crt_handle* h) {
   const int n = h->n, mb = h->mb, E = h->E, K = h->K;
-  const size_t lds = crt_factor_lds_bytes(mb, kNB);
-  if (lds > h->lds_max || mb + kNB > 256 || mb > 124 || mb > 255) return kErrBand;
-  // trailing update: 4x4 tiles of the (mb x mb) triangle, one or two per thread
-  const int ntile = ((mb + 3) / 4) * ((mb + 3) / 4 + 1) / 2;
-  auto kern = (ntile <= 256) ? crt_factor_kernel<kNB, 1> : crt_factor_kernel<kNB, 2>;
+  const size_t lds = crt_factor_lds_bytes(mb);
+  if (lds > h->lds_max || mb > 124) return kErrBand;
+  // wave 0 holds the W = mb + kNB panel rows, ROWS per lane; waves 1-3 own the
+  // trailing window's 4x4 tiles beyond the look-ahead columns, MAXT each
+  const int W = mb + kNB, tw = (mb + 3) / 4;
+  const int nrest = tw > 2 ? ((tw - 2) * (tw - 1)) / 2 : 0;
+  const int rows = (W + 63) / 64, maxt = std::max(1, (nrest + 191) / 192);
+  void (*kern)(FactorArgs) = nullptr;
+  if (rows == 1 && maxt == 1) kern = crt_factor_kernel<1, 1>;
+  else if (rows == 2 && maxt == 1) kern = crt_factor_kernel<2, 1>;
+  else if (rows == 2 && maxt == 2) kern = crt_factor_kernel<2, 2>;
+  else if (rows == 2 && maxt == 3) kern = crt_factor_kernel<2, 3>;
+  else if (rows == 3 && maxt <= 3) kern = crt_factor_kernel<3, 3>;
+  if (!kern) return kErrBand;
   hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   int per_cu = 0;
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, 256, lds) != hipSuccess || per_cu < 1) per_cu = 1;
@@ -716,19 +724,23 @@ static int run_forward(crt_handle* h) {
   hipLaunchKernelGGL(crt_area_values_kernel, dim3((unsigned)(((size_t)K * n + 255) / 256)), dim3(256), 0, h->stream,
                      A, K, av);
   tstop(h);
-  // systems per batch: L, y and the scaling factors of every system in the
-  // batch stay in device memory until its back substitution
-  const size_t per_sys = (size_t)n * (mb + 1) * 16 + (size_t)n * 24;
+  // systems per batch: the scaled rows, L, y and the scaling factors of every
+  // system in the batch stay in device memory until its back substitution
+  const size_t per_sys = (size_t)n * (mb + 1) * 16 + (size_t)n * h->R * 16 + (size_t)n * 24;
   size_t free_b = 0, total_b = 0;
   if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) free_b = (size_t)8 << 30;
   size_t budget = std::min<size_t>(free_b / 2, h->batch_bytes_max);
-  const size_t have = h->bufsz.count("L") ? h->bufsz["L"] + h->bufsz["y"] + h->bufsz["fak"] : 0;
+  const size_t have = h->bufsz.count("L") && h->bufsz.count("S")
+                          ? h->bufsz["L"] + h->bufsz["S"] + h->bufsz["y"] + h->bufsz["fak"]
+                          : 0;
   budget = std::max(budget, have);
   int batch = (int)std::max<size_t>(1, std::min<size_t>((size_t)nsys, budget / per_sys));
   A.L = h->get<double2>("L", (size_t)batch * n * (mb + 1));
+  A.S = h->get<double2>("S", (size_t)batch * n * h->R);
   A.y = h->get<double2>("y", (size_t)batch * n);
   A.fak = h->get<double>("fak", (size_t)batch * n);
   NEED(A.L);
+  NEED(A.S);
   NEED(A.y);
   NEED(A.fak);
   BacksolveArgs B;
@@ -743,6 +755,11 @@ static int run_forward(crt_handle* h) {
     const int cnt = std::min(batch, nsys - first);
     A.sys_first = first;
     A.nsys = cnt;
+    const unsigned rgrid = (unsigned)(((size_t)cnt * n + 255) / 256);
+    tstart(h, "assemble_scale");
+    hipLaunchKernelGGL(crt_scale_kernel, dim3(rgrid), dim3(256), 0, h->stream, A);
+    hipLaunchKernelGGL(crt_rows_kernel, dim3(rgrid), dim3(256), 0, h->stream, A);
+    tstop(h);
     h->slots = std::min(cnt, h->ncu * per_cu);
     tstart(h, "factor_solve");
     hipLaunchKernelGGL(kern, dim3(h->slots), dim3(256), lds, h->stream, A);

@@ -29,36 +29,75 @@ __device__ __forceinline__ cplx cmul(cplx a, cplx b) {
 __device__ __forceinline__ cplx cmsub(cplx a, cplx b, cplx c) {
   return mkc(a.re - (b.re * c.re - b.im * c.im), a.im - (b.re * c.im + b.im * c.re));
 }
+// a - b*c as four fused multiply-adds.  The real part subtracts b.re c.re
+// first, in one rounding with the exact product, so when a.re and b.re c.re
+// nearly cancel (the usual case in an elimination update) the error is
+// relative to the small difference, not to a.re.
+__device__ __forceinline__ cplx cfms(cplx a, cplx b, cplx c) {
+  return mkc(fma(b.im, c.im, fma(-b.re, c.re, a.re)), fma(-b.im, c.re, fma(-b.re, c.im, a.im)));
+}
 // a + b*c
 __device__ __forceinline__ cplx cmadd(cplx a, cplx b, cplx c) {
   return mkc(a.re + (b.re * c.re - b.im * c.im), a.im + (b.re * c.im + b.im * c.re));
 }
 __device__ __forceinline__ cplx rmul(double r, cplx z) { return mkc(r * z.re, r * z.im); }
+// Component-wise select.  A ?: between two cplx values may be lowered to a
+// select of their addresses and a load, which keeps arrays of cplx in scratch
+// memory instead of registers.
+__device__ __forceinline__ cplx csel(bool c, cplx a, cplx b) { return mkc(c ? a.re : b.re, c ? a.im : b.im); }
 __device__ __forceinline__ cplx cconj(cplx z) { return mkc(z.re, -z.im); }
 __device__ __forceinline__ double cabs_(cplx z) { return hypot(z.re, z.im); }
 
-// logb(m) for finite nonzero m is frexp's exponent minus one; the power-of-two
-// scalings are exact (v_frexp_exp_i32_f64, v_ldexp_f64).
-__device__ __forceinline__ cplx cdiv(cplx a, cplx b) {
-  double c = b.re, d = b.im;
-  const double m = fmax(fabs(c), fabs(d));
-  int il = 0;
+// Division by b, split so that several numerators can share the divisor's
+// scaling: b is scaled by 2^-il (il = logb(max(|b.re|, |b.im|)), exact, via
+// v_frexp_exp_i32_f64 / v_ldexp_f64), den = |b 2^-il|^2, and each quotient is
+// rescaled by 2^-il.
+struct cdiv_by {
+  double c, d, den;
+  int il;
+};
+__device__ __forceinline__ cdiv_by cdiv_prep(cplx b) {
+  cdiv_by p;
+  p.c = b.re;
+  p.d = b.im;
+  const double m = fmax(fabs(p.c), fabs(p.d));
+  p.il = 0;
   if (m != 0.0 && isfinite(m)) {
-    il = __builtin_amdgcn_frexp_exp(m) - 1;
-    c = __builtin_amdgcn_ldexp(c, -il);
-    d = __builtin_amdgcn_ldexp(d, -il);
+    p.il = __builtin_amdgcn_frexp_exp(m) - 1;
+    p.c = __builtin_amdgcn_ldexp(p.c, -p.il);
+    p.d = __builtin_amdgcn_ldexp(p.d, -p.il);
   }
-  const double den = c * c + d * d;
-  return mkc(__builtin_amdgcn_ldexp((a.re * c + a.im * d) / den, -il),
-             __builtin_amdgcn_ldexp((a.im * c - a.re * d) / den, -il));
+  p.den = p.c * p.c + p.d * p.d;
+  return p;
+}
+__device__ __forceinline__ cplx cdiv_apply(cplx a, const cdiv_by& p) {
+  return mkc(__builtin_amdgcn_ldexp((a.re * p.c + a.im * p.d) / p.den, -p.il),
+             __builtin_amdgcn_ldexp((a.im * p.c - a.re * p.d) / p.den, -p.il));
+}
+__device__ __forceinline__ cplx cdiv(cplx a, cplx b) { return cdiv_apply(a, cdiv_prep(b)); }
+
+// 1/z = conj(z') / |z'|^2 * 2^-il with z' = z 2^-il (exact scaling, so |z'|^2
+// lies in [1, 8)); the reciprocal of |z'|^2 is v_rcp_f64 refined by two Newton
+// steps (within one ulp).
+__device__ __forceinline__ cplx crecip(cplx z) {
+  const cdiv_by p = cdiv_prep(z);
+  double r = __builtin_amdgcn_rcp(p.den);
+  double e = fma(-p.den, r, 1.0);
+  r = fma(r, e, r);
+  e = fma(-p.den, r, 1.0);
+  r = fma(r, e, r);
+  return mkc(__builtin_amdgcn_ldexp(p.c * r, -p.il), __builtin_amdgcn_ldexp(-p.d * r, -p.il));
 }
 
+// Principal square root: t = sqrt((|x| + |z|)/2), then (t, y/2t) for x >= 0
+// and (|y|/2t, sign(y) t) otherwise -- one division either way.
 __device__ __forceinline__ cplx csqrt_(cplx z) {
-  double x = z.re, y = z.im;
+  const double x = z.re, y = z.im;
   if (x == 0.0 && y == 0.0) return mkc(0.0, y);
-  double t = sqrt((fabs(x) + hypot(x, y)) * 0.5);
-  if (x >= 0.0) return mkc(t, y / (2.0 * t));
-  return mkc(fabs(y) / (2.0 * t), copysign(t, y));
+  const double t = sqrt((fabs(x) + hypot(x, y)) * 0.5);
+  const bool pos = x >= 0.0;
+  const double u = (pos ? y : fabs(y)) / (2.0 * t);
+  return pos ? mkc(t, u) : mkc(u, copysign(t, y));
 }
 
 __device__ __forceinline__ cplx clog_(cplx z) { return mkc(log(cabs_(z)), atan2(z.im, z.re)); }

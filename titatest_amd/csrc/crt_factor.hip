@@ -1,24 +1,29 @@
-// Batched complex-symmetric band factorisation with fused assembly, scaling
-// and forward sweep (gfx950), plus the wavefront-per-system back substitution.
+// Batched complex-symmetric band factorisation with fused forward sweep
+// (gfx950), the assembly/scaling kernels that feed it, and the
+// wavefront-per-system back substitution.
 //
 // crt_area_values_kernel: per wavenumber k and row r, the sums of the area
 //   element contributions to each stored band entry of row r, in element order
 //   (the first part of kompab's accumulation, src/kompab.f90:79-129).  They do
 //   not depend on the source electrode.
 //
-// crt_factor_kernel: one workgroup factors one (k, l) system at a time and
-//   loops over systems (persistent).  Rows enter an LDS window as the factor
-//   front reaches them: the area sums are completed with the mixed-boundary
-//   terms of the row (boundary elements follow all area elements in the
-//   reference's element order, so the sum order is the reference's), then
-//   scaled (scalab, src/scalab.f90:52-76).  The band L L^T without conjugation
-//   (chol, src/chol.f90:44-78) is factored right-looking in panels of NB
-//   columns; each L_ij is divided by L_jj exactly once and every entry receives
-//   -L_ik L_jk in ascending k (the reference's dot-product order).  The forward
-//   substitution L y = b (vre, src/vre.f90:45-56) runs inside the panel loop.
-//   Finished rows of L (band-row layout: row i holds columns i-mb..i), y and the
-//   scaling factors go to global memory; threads communicate only through LDS,
-//   so the barriers order LDS alone and the stores stream out behind them.
+// crt_scale_kernel / crt_rows_kernel: per system (k, l) and row, the area sums
+//   completed with the row's mixed-boundary terms (boundary elements follow all
+//   area elements in the reference's element order, so the sum order is the
+//   reference's), the scaling factor f = 1/sqrt|a_rr| and the scaled entries
+//   a_rr/|a_rr|, a_rc f_r f_c (scalab, src/scalab.f90:52-76).  One record of R
+//   values per row (the row's distinct band offsets).
+//
+// crt_factor_kernel: band L L^T without conjugation (chol, src/chol.f90:44-78)
+//   plus the forward substitution L y = b (vre, src/vre.f90:45-56), one
+//   workgroup per system, persistent over systems.  The matrix is processed in
+//   stages of kNB columns.  Stage J: waves 1-3 first apply panel J-1 to the
+//   kNB columns of panel J (the look-ahead), then, while wave 0 factors panel J
+//   in registers without any workgroup barrier, they apply panel J-1 to the
+//   rest of the trailing window.  Two barriers per stage.  Every entry receives
+//   -L_ik L_jk in ascending k and is divided by L_jj once (the reference's
+//   arithmetic per entry).  Rows enter from the scaled-row records one batch of
+//   kNB rows per stage, staged in LDS a stage ahead.
 //
 // crt_backsolve_kernel: x = -L^{-T} y (vre, src/vre.f90:58-69) and
 //   kpot = x * f (src/fem.f90:385), one wavefront per system.
@@ -27,16 +32,20 @@
 
 namespace {
 
-__device__ __forceinline__ int tri(int r, int c) { return (r * (r + 1)) / 2 + c; }
+constexpr int kStg = 3;  // staged row batches: the stage's previous, current and next
 
-// Optional per-phase timing (thread 0 of each workgroup, constant-rate clock).
+// Optional per-phase timing (constant-rate clock), kept by thread 0 (wave 0:
+// accumulators 0-3) and thread 64 (waves 1-3: accumulators 4-7).
 struct PhaseClock {
   unsigned long long* out;
-  unsigned long long t, acc[8];
-  __device__ void start() {
+  int base;
+  unsigned long long t, acc[4];
+  __device__ void start(unsigned long long* o, int b) {
+    out = o;
+    base = b;
     if (!out) return;
     t = wall_clock64();
-    for (int i = 0; i < 8; ++i) acc[i] = 0;
+    for (int i = 0; i < 4; ++i) acc[i] = 0;
   }
   __device__ void mark(int phase) {
     if (!out) return;
@@ -46,7 +55,7 @@ struct PhaseClock {
   }
   __device__ void flush() {
     if (!out) return;
-    for (int i = 0; i < 8; ++i) atomicAdd(out + i, acc[i]);
+    for (int i = 0; i < 4; ++i) atomicAdd(out + base + i, acc[i]);
   }
 };
 
@@ -56,54 +65,120 @@ __device__ __forceinline__ cplx bnd_term(const FactorArgs& A, int k, int l, int 
   return rmul(dum, ldc(A.sigma + A.rnr[rel]));
 }
 
-__device__ __forceinline__ cplx rhs_value(const FactorArgs& A, int l, int r, double fr) {
-  // b(enr(l)) = -1, b(nsink) = +1 (kompab :162,:183), Dirichlet rows zeroed
-  // (randb :49), then scaled by f (scalab :65).
-  cplx b = mkc(0.0, 0.0);
-  if (r == A.enr[l]) b = mkc(-1.0, 0.0);
-  if (A.lsink && r == A.nsink) b = mkc(1.0, 0.0);
-  if (A.isdir && A.isdir[r]) b = mkc(0.0, 0.0);
-  return rmul(fr, b);
+// Trailing window storage.  The window holds rows and columns [o, o+mb); its
+// diagonal t = r - c (0 <= t < mb) has live entries for c in [o, o+mb-t).
+// Diagonals t and mb-1-t share one ring of mb+1 slots: diagonal t at slot
+// c mod (mb+1), its partner at slot (r+1) mod (mb+1).  For every origin o the
+// two live ranges are complementary arcs of the ring, so advancing the window
+// moves no data, and the slot of a departing column's entry is the slot of an
+// entering row's entry.  Arguments are window coordinates rw = r - o, cw = c - o.
+struct Ring {
+  int mb, M1, half, om;  // om = o mod (mb+1)
+  __device__ __forceinline__ int pos(int rw, int cw) const {
+    const int t = rw - cw;
+    if (t < half) {
+      int p = cw + om;
+      p = p >= M1 ? p - M1 : p;
+      return t * M1 + p;
+    }
+    int p = rw + 1 + om;
+    p = p >= M1 ? p - M1 : p;
+    return (mb - 1 - t) * M1 + p;
+  }
+};
+
+// Panel buffer: W rows of kNB columns, column index swizzled by the row's
+// 4-row group so that tiles reading one column from different rows spread
+// over the LDS banks.
+__device__ __forceinline__ int pix(int rw, int k) { return rw * kNB + (k ^ ((rw >> 2) & (kNB - 1))); }
+
+// Value at band offset t of row i of a staged batch (0 when t is not one of
+// the row's stored offsets).
+__device__ __forceinline__ cplx staged(const cplx* sv, const int16_t* so, int i, int t) {
+  int qf = -1;
+#pragma unroll
+  for (int q = 0; q < kMaxR; ++q) qf = (so[i * kMaxR + q] == t) ? q : qf;
+  return csel(qf >= 0, sv[i * kMaxR + max(qf, 0)], mkc(0.0, 0.0));
 }
 
-// Row r (window row rw, window columns from j0w): write the unscaled band
-// entries -- the per-wavenumber area sums av (R slots) plus the row's
-// mixed-boundary terms -- into the zeroed window row, and return the scaling
-// factor 1/sqrt(|a_rr|) (scalab :56-63).
-__device__ double put_row_unscaled(const FactorArgs& A, int k, int l, int r, int rw, int j0w,
-                                   const double2* av, cplx* __restrict__ T, int sys) {
-  cplx* row = T + tri(rw, 0);
-  const int base = r - A.mb - j0w;
-  for (int q = 0; q < A.R; ++q) {
-    const int t = A.pat_t[(size_t)r * A.R + q];
-    if (t >= 0) row[base + t] = ldc(av + (size_t)r * A.R + q);
+__device__ __forceinline__ double rl(double x, int l) {
+  const long long b = __double_as_longlong(x);
+  const int lo = __builtin_amdgcn_readlane((int)(unsigned)(b & 0xffffffffLL), l);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+  return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+__device__ __forceinline__ cplx rlc(cplx z, int l) { return mkc(rl(z.re, l), rl(z.im, l)); }
+
+// Scaled-row record slot q of row r (value and band offset), or (0, -1).
+__device__ __forceinline__ void row_slot(const FactorArgs& A, const double2* Sg, int r, int q, cplx& v, int& t) {
+  v = mkc(0.0, 0.0);
+  t = -1;
+  if (r >= 0 && r < A.n && q < A.R) {
+    t = A.pat_t[(size_t)r * A.R + q];
+    if (t >= 0) v = ldc(Sg + (size_t)r * A.R + q);
   }
-  const int b1 = A.bptr[r + 1];
-  for (int p = A.bptr[r]; p < b1; ++p) {
-    const int t = A.pat_t[(size_t)r * A.R + A.bslot[p]];
-    row[base + t] = cadd(row[base + t], bnd_term(A, k, l, A.bcode[p], A.bimn[p]));
-  }
-  const double ad = cabs_(row[base + A.mb]);
-  if (!(ad > 0.0)) {
-    if (atomicCAS(A.err, 0, 27) == 0) A.err[1] = sys;
-    return 1.0;
-  }
-  return 1.0 / sqrt(ad);
 }
 
-// Scale the stored entries of row r in place: a_rr/|a_rr| on the diagonal,
-// a_rc f_r f_c elsewhere (scalab :58-73).
-__device__ void scale_row(const FactorArgs& A, int r, int rw, int j0w, double fr, const double* fw,
-                          cplx* __restrict__ T) {
-  cplx* row = T + tri(rw, 0);
-  const int base = r - A.mb - j0w;
-  for (int q = 0; q < A.R; ++q) {
-    const int t = A.pat_t[(size_t)r * A.R + q];
-    if (t < 0) continue;
-    const cplx v = row[base + t];
-    if (t == A.mb) row[base + t] = cdiv(v, mkc(cabs_(v), 0.0));
-    else row[base + t] = rmul(fr * fw[base + t], v);
+struct Stage {
+  int n, mb, W, o;
+  Ring ring;
+  bool apply;              // panel J-1 exists
+  const cplx* Pp;          // panel J-1 (its window coordinates: stage rows + kNB)
+  const cplx* sv_old;      // batch J-1 = stage rows [mb - kNB, mb)
+  const int16_t* so_old;
+};
+
+// Entry (rw, cw) of a tile exists: lower triangle, inside the window and the
+// matrix.
+__device__ __forceinline__ bool tile_ok(const Stage& C, int rw, int cw) {
+  return cw <= rw && rw < C.mb && C.o + rw < C.n;
+}
+
+// TR x TC tile of the stage window at (rw0, cw0): the current values (ring, or
+// the staged batch for the window's last kNB rows) minus panel J-1's rank-kNB
+// product, accumulated in ascending k.
+template <int TR, int TC>
+__device__ __forceinline__ void tile_update(const Stage& C, const cplx* Tt, int rw0, int cw0, cplx acc[TR][TC]) {
+#pragma unroll
+  for (int i = 0; i < TR; ++i)
+#pragma unroll
+    for (int j = 0; j < TC; ++j) {
+      const int rw = rw0 + i, cw = cw0 + j;
+      cplx v = mkc(0.0, 0.0);
+      if (tile_ok(C, rw, cw)) {
+        const int ib = rw - (C.mb - kNB);
+        v = ib >= 0 ? staged(C.sv_old, C.so_old, ib, cw - rw + C.mb) : Tt[C.ring.pos(rw, cw)];
+      }
+      acc[i][j] = v;
+    }
+  if (!C.apply) return;
+  // operand rows of panel J-1 (stage rows + kNB)
+  int ra[TR], ca[TC];
+#pragma unroll
+  for (int i = 0; i < TR; ++i) ra[i] = min(rw0 + i + kNB, C.W - 1);
+#pragma unroll
+  for (int j = 0; j < TC; ++j) ca[j] = min(cw0 + j + kNB, C.W - 1);
+#pragma unroll 1
+  for (int k = 0; k < kNB; ++k) {
+    cplx a[TR], b[TC];
+#pragma unroll
+    for (int i = 0; i < TR; ++i) a[i] = C.Pp[pix(ra[i], k)];
+#pragma unroll
+    for (int j = 0; j < TC; ++j) b[j] = C.Pp[pix(ca[j], k)];
+#pragma unroll
+    for (int i = 0; i < TR; ++i)
+#pragma unroll
+      for (int j = 0; j < TC; ++j) acc[i][j] = cmsub(acc[i][j], a[i], b[j]);
   }
+}
+
+// lower-triangle tile index x -> (ti, tj), x = ti (ti + 1) / 2 + tj
+__device__ __forceinline__ void tri_tile(int x, int& ti, int& tj) {
+  int i = (int)((sqrt(8.0 * x + 1.0) - 1.0) * 0.5);
+  while ((i * (i + 1)) / 2 > x) --i;
+  while (((i + 1) * (i + 2)) / 2 <= x) ++i;
+  ti = i;
+  tj = x - (i * (i + 1)) / 2;
 }
 
 }  // namespace
@@ -133,224 +208,333 @@ __global__ void crt_area_values_kernel(FactorArgs A, int K, double2* __restrict_
 }
 
 // ---------------------------------------------------------------------------
-template <int NB, int MAXT>
-__global__ __launch_bounds__(256) void crt_factor_kernel(FactorArgs A) {
+// f_r = 1/sqrt(|a_rr|) of every row of every system of the batch (scalab
+// :56-63; errnr 27 when |a_rr| = 0).
+__global__ void crt_scale_kernel(FactorArgs A) {
+  const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (long)A.nsys * A.n) return;
+  const int s = (int)(idx / A.n), r = (int)(idx % A.n);
+  const int sys = A.sys_first + s;
+  const int k = sys / A.eanz, l = sys % A.eanz;
+  const int qd = A.pat_diag[r];
+  cplx v = ldc(A.av + ((size_t)k * A.n + r) * A.R + qd);
+  const int b1 = A.bptr[r + 1];
+  for (int p = A.bptr[r]; p < b1; ++p)
+    if (A.bslot[p] == qd) v = cadd(v, bnd_term(A, k, l, A.bcode[p], A.bimn[p]));
+  const double ad = cabs_(v);
+  double f = 1.0 / sqrt(ad);
+  if (ad <= 0.0) {  // scalab :56-61
+    f = 1.0;
+    if (atomicCAS(A.err, 0, 27) == 0) A.err[1] = sys;
+  }
+  A.fak[(size_t)s * A.n + r] = f;
+}
+
+// Scaled band entries of every row of every system of the batch: a_rr/|a_rr|
+// on the diagonal, (f_r f_c) a_rc elsewhere (scalab :58-73).
+__global__ void crt_rows_kernel(FactorArgs A) {
+  const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (long)A.nsys * A.n) return;
+  const int s = (int)(idx / A.n), r = (int)(idx % A.n);
+  const int sys = A.sys_first + s;
+  const int k = sys / A.eanz, l = sys % A.eanz;
+  const double* fk = A.fak + (size_t)s * A.n;
+  const double fr = fk[r];
+  const int b0 = A.bptr[r], b1 = A.bptr[r + 1];
+  double2* out = A.S + ((size_t)s * A.n + r) * A.R;
+  for (int q = 0; q < A.R; ++q) {
+    const int t = A.pat_t[(size_t)r * A.R + q];
+    cplx v = mkc(0.0, 0.0);
+    if (t >= 0) {
+      v = ldc(A.av + ((size_t)k * A.n + r) * A.R + q);
+      for (int p = b0; p < b1; ++p)
+        if (A.bslot[p] == q) v = cadd(v, bnd_term(A, k, l, A.bcode[p], A.bimn[p]));
+      if (t == A.mb) v = cdiv(v, mkc(cabs_(v), 0.0));
+      else v = rmul(fr * fk[r - A.mb + t], v);
+    }
+    out[q] = make_double2(v.re, v.im);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Lookahead band factorisation.  ROWS: panel rows per lane of wave 0
+// (W = mb + kNB <= 64 ROWS); MAXT: trailing tiles per thread of waves 1-3.
+template <int ROWS, int MAXT>
+__global__ __launch_bounds__(256, 2) void crt_factor_kernel(FactorArgs A) {
   extern __shared__ double2 smem[];
-  const int mb = A.mb, n = A.n;
-  const int W = mb + NB;
-  const int ntri = (W * (W + 1)) / 2;
-  cplx* T = reinterpret_cast<cplx*>(smem);
-  cplx* yw = T + ntri;                              // W: right-hand side / y
-  cplx* colb = yw + W;                              // 2 x NB: one L column of the panel
-  cplx* diagb = colb + 2 * NB;                      // 2: next pivot value
-  cplx* ycb = diagb + 2;                            // 2: y of the pivot row
-  double* fw = reinterpret_cast<double*>(ycb + 2);  // W: scaling factors of window rows
-  const int tid = threadIdx.x;
-  const int nthr = blockDim.x;
+  const int n = A.n, mb = A.mb, R = A.R;
+  const int W = mb + kNB, M1 = mb + 1, half = (mb + 1) / 2;
+  cplx* Tt = reinterpret_cast<cplx*>(smem);  // half x M1 ring slots
+  cplx* P0 = Tt + half * M1;                  // 2 panels of W x kNB
+  cplx* sv = P0 + 2 * W * kNB;                // kStg batches of kNB x kMaxR values
+  int16_t* so = reinterpret_cast<int16_t*>(sv + kStg * kNB * kMaxR);  // kStg x kNB x kMaxR offsets
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int x3 = tid - 64;                    // thread index within waves 1-3
+  const int nst = (n + kNB - 1) / kNB;
+  const int tw = (mb + 3) / 4;                          // 4x4 tiles per window side
+  // look-ahead: 2x2 tiles over window rows [0, mb) x columns [0, kNB), column
+  // by column of tiles; tile column j holds the tile rows i >= j
+  const int t2 = (mb + 1) / 2;
+  int nla = 0;
+#pragma unroll
+  for (int j = 0; j < kNB / 2; ++j) nla += max(t2 - j, 0);
+  const int nrest = tw > 2 ? ((tw - 2) * (tw - 1)) / 2 : 0;  // 4x4 tiles, tile columns 2..
+
+  // trailing 4x4 tiles of this thread (fixed for the kernel)
+  int rs_r0[MAXT], rs_c0[MAXT];
+#pragma unroll
+  for (int q = 0; q < MAXT; ++q) {
+    rs_r0[q] = -1;
+    rs_c0[q] = 0;
+    const int x = x3 + 192 * q;
+    if (wave > 0 && x < nrest) {
+      int ti, tj;
+      tri_tile(x, ti, tj);
+      rs_r0[q] = 4 * (ti + 2);
+      rs_c0[q] = 4 * (tj + 2);
+    }
+  }
 
   PhaseClock clk;
-  clk.out = (tid == 0) ? A.prof : nullptr;
-  clk.start();
+  clk.start(tid == 0 ? A.prof : (tid == 64 ? A.prof : nullptr), tid == 0 ? 0 : 4);
 
   for (int s = blockIdx.x; s < A.nsys; s += gridDim.x) {
     const int sys = A.sys_first + s;
-    const int k = sys / A.eanz, l = sys % A.eanz;
-    double* fak = A.fak + (size_t)s * n;
-    double2* Lg = A.L + (size_t)s * n * (mb + 1);
+    const int l = sys % A.eanz;
+    double2* Lg = A.L + (size_t)s * n * M1;
     double2* yg = A.y + (size_t)s * n;
-    const double2* av = A.av + (size_t)k * n * A.R;
+    const double2* Sg = A.S + (size_t)s * n * R;
+    const double* fk = A.fak + (size_t)s * n;
 
-    // ---- first window: rows [0, min(W, n)); unscaled entries and scaling
-    // factors, then the scaled entries
-    for (int i = tid; i < ntri; i += nthr) T[i] = mkc(0.0, 0.0);
+    // ---- prologue: ring zeroed, rows [0, mb - kNB) in place, batches
+    // [mb - kNB, mb) and [mb, mb + kNB) staged, wave 0's right-hand side
+    for (int i = tid; i < half * M1; i += 256) Tt[i] = mkc(0.0, 0.0);
     lds_barrier();
     {
-      const int rend = min(W, n);
-      double fr = 0.0;
-      if (tid < rend) {
-        fr = put_row_unscaled(A, k, l, tid, tid, 0, av, T, sys);
-        fw[tid] = fr;
-        fak[tid] = fr;
-      }
-      lds_barrier();
-      if (tid < rend) {
-        scale_row(A, tid, tid, 0, fr, fw, T);
-        yw[tid] = rhs_value(A, l, tid, fr);
+      Ring ring0{mb, M1, half, 0};
+      for (int r = tid; r < min(mb - kNB, n); r += 256)
+        for (int q = 0; q < R; ++q) {
+          cplx v;
+          int t;
+          row_slot(A, Sg, r, q, v, t);
+          if (t >= 0) Tt[ring0.pos(r, r - mb + t)] = v;
+        }
+      for (int x = tid; x < 2 * kNB * kMaxR; x += 256) {
+        const int b = x / (kNB * kMaxR), i = (x / kMaxR) % kNB, q = x % kMaxR;
+        const int r = (b == 0 ? mb - kNB : mb) + i;
+        const int slot = (b == 0 ? kStg - 1 : 0) * kNB * kMaxR + i * kMaxR + q;
+        cplx v;
+        int t;
+        row_slot(A, Sg, r, q, v, t);
+        sv[slot] = v;
+        so[slot] = (int16_t)t;
       }
     }
+    // right-hand side b(enr(l)) = -1, b(nsink) = +1 (kompab :162,:183), zero on
+    // Dirichlet rows (randb :49), scaled by f (scalab :65)
+    const int rel = A.enr[l];
+    const int rsink = A.lsink ? A.nsink : -1;
+    double be = 0.0, bs = 0.0;
+    if (wave == 0) {
+      if (!(A.isdir && A.isdir[rel])) be = -fk[rel];
+      if (rsink >= 0 && !(A.isdir && A.isdir[rsink])) bs = fk[rsink];
+    }
+    auto rhs = [&](int r) -> cplx { return mkc(r == rsink ? bs : (r == rel ? be : 0.0), 0.0); };
+    cplx yv[ROWS];
+#pragma unroll
+    for (int q = 0; q < ROWS; ++q) {
+      const int rw = lane + 64 * q;
+      yv[q] = csel(wave == 0 && rw < W && rw < n, rhs(rw), mkc(0.0, 0.0));
+    }
     lds_barrier();
-    clk.mark(1);
+    clk.mark(0);
 
-    for (int j0 = 0; j0 < n; j0 += NB) {
-      const int nbc = min(NB, n - j0);
-      const int rows = min(W, n - j0);  // window rows present
+    for (int J = 0; J < nst; ++J) {
+      const int o = J * kNB;
+      const int nbc = min(kNB, n - o);
+      Stage C;
+      C.n = n;
+      C.mb = mb;
+      C.W = W;
+      C.o = o;
+      C.ring = Ring{mb, M1, half, o % M1};
+      C.apply = J > 0;
+      C.Pp = P0 + ((J + 1) & 1) * W * kNB;
+      const int bo = (J + kStg - 1) % kStg, bc = J % kStg, bn = (J + 1) % kStg;
+      C.sv_old = sv + bo * kNB * kMaxR;
+      C.so_old = so + bo * kNB * kMaxR;
+      cplx* Pc = P0 + (J & 1) * W * kNB;
 
-      // ---- panel factorisation.  During the panel each thread owns its window
-      // row; columns are broadcast through double-buffered LDS slots, one
-      // barrier per column.
-      const int r = tid;
-      const bool live = r < rows;
-      cplx* __restrict__ myrow = T + tri(min(r, W - 1), 0);
-      cplx yr = live ? yw[r] : mkc(0.0, 0.0);
-      if (tid == 0) diagb[0] = T[0];
+      // The tile coordinates are loop invariant; re-declaring them as changed
+      // each stage keeps the position arithmetic of 16 entries per tile from
+      // being hoisted out of the stage loop (it would hold ~100 registers).
+#pragma unroll
+      for (int q = 0; q < MAXT; ++q) asm volatile("" : "+v"(rs_r0[q]), "+v"(rs_c0[q]));
+
+      // ---- phase A: waves 1-3 apply panel J-1 to panel J's columns; wave 0
+      // copies the entering batch J into panel J's last kNB rows
+      if (wave > 0) {
+        for (int x = x3; x < nla; x += 192) {
+          int j = 0, i = x;
+#pragma unroll
+          for (int jj = 0; jj < kNB / 2 - 1; ++jj)
+            if (i >= t2 - j) {
+              i -= t2 - j;
+              ++j;
+            }
+          const int rw0 = 2 * (i + j), cw0 = 2 * j;
+          cplx acc[2][2];
+          tile_update<2, 2>(C, Tt, rw0, cw0, acc);
+#pragma unroll
+          for (int a = 0; a < 2; ++a)
+#pragma unroll
+            for (int b = 0; b < 2; ++b)
+              if (tile_ok(C, rw0 + a, cw0 + b)) Pc[pix(rw0 + a, cw0 + b)] = acc[a][b];
+        }
+      } else {
+        const int i = lane / kNB, cw = lane % kNB, rw = mb + i;
+        const bool ok = o + rw < n && cw >= i;
+        Pc[pix(rw, cw)] = csel(ok, staged(sv + bc * kNB * kMaxR, so + bc * kNB * kMaxR, i, cw - i), mkc(0.0, 0.0));
+      }
+      clk.mark(0);
       lds_barrier();
-      for (int c = 0; c < nbc; ++c) {
-        const int buf = c & 1;
-        const cplx d = diagb[buf];
-        if (tid == 0 && !(cabs_(d) > 0.0) && atomicCAS(A.err, 0, 28) == 0) A.err[1] = sys;
-        const cplx lcc = csqrt_(d);
-        const bool below = live && r > c && r - c <= mb;
-        cplx lrc = mkc(0.0, 0.0);
-        if (below) {
-          lrc = cdiv(myrow[c], lcc);
-          myrow[c] = lrc;
-          if (r < nbc) colb[buf * NB + r] = lrc;
-          if (r == c + 1 && r < nbc) {
-            // the next pivot takes its -L L now so it travels with this column
-            const cplx dn = cmsub(myrow[r], lrc, lrc);
-            myrow[r] = dn;
-            diagb[buf ^ 1] = dn;
+      clk.mark(1);
+
+      // ---- phase B
+      if (wave == 0) {
+        // panel J in registers: lane holds panel rows lane + 64 q
+        const int rows = min(W, n - o);
+        cplx v[ROWS][kNB];
+#pragma unroll
+        for (int q = 0; q < ROWS; ++q) {
+          const int rw = lane + 64 * q;
+#pragma unroll
+          for (int k = 0; k < kNB; ++k) {
+            const bool ok = rw < rows && k < nbc && k <= rw && rw - k <= mb;
+            v[q][k] = csel(ok, Pc[pix(min(rw, W - 1), k)], mkc(0.0, 0.0));
           }
         }
-        if (r == c) {
-          myrow[c] = lcc;
-          const cplx yc = cdiv(yr, lcc);
-          yr = yc;
-          ycb[buf] = yc;
-          yg[j0 + c] = make_double2(yc.re, yc.im);
-        }
-        lds_barrier();
-        if (below) {
-          // the row's remaining panel entries: all loads first, then the
-          // updates (static indices, registers), then the stores
-          const cplx* cb = colb + buf * NB;
-          const int cend = min(nbc - 1, r);
-          const int cskip = (r == c + 1) ? c + 1 : -1;
-          constexpr int H = NB / 2;
+        // Column c, in the L D L^T form of the same elimination: with d the
+        // pivot and s_rc the column's updated entries, l_rc = s_rc / d and the
+        // later columns take -l_rc s_c2c (= -L_rc L_c2c).  The next pivot,
+        // d_{c+1} = s_{c+1,c+1} - l_{c+1,c} s_{c+1,c}, needs no square root, so
+        // the column-to-column dependency is a reciprocal and two products;
+        // L_cc = sqrt(d), L_rc = l_rc L_cc and y_c = y'_c / L_cc follow off it.
+        // lrc is 0 for rows at or above c, so every update is unconditional
+        // (a - 0 * x leaves a unchanged).
+        cplx piv = rlc(v[0][0], 0);
+        bool zero_pivot = false;
 #pragma unroll
-          for (int h0 = 0; h0 < NB; h0 += H) {
-            cplx rv[H], cv[H];
+        for (int c = 0; c < kNB; ++c) {
+          if (c < nbc) {
+            // chol :66 -- |s| <= 0 is an exact zero (a NaN pivot passes, as there)
+            zero_pivot |= piv.re == 0.0 && piv.im == 0.0;
+            const cplx inv = crecip(piv);
+            cplx lrc[ROWS];
+            bool below[ROWS];
 #pragma unroll
-            for (int u = 0; u < H; ++u) {
-              const int c2 = h0 + u;
-              const bool upd = c2 > c && c2 <= cend && c2 != cskip;
-              rv[u] = upd ? myrow[c2] : mkc(0.0, 0.0);
-              cv[u] = upd ? cb[c2] : mkc(0.0, 0.0);
+            for (int q = 0; q < ROWS; ++q) {
+              below[q] = lane + 64 * q > c;
+              lrc[q] = csel(below[q], cmul(v[q][c], inv), mkc(0.0, 0.0));
+            }
+            const int cn = c + 1 < kNB ? c + 1 : kNB - 1;
+            cplx pivn = mkc(0.0, 0.0);
+            if (c + 1 < nbc) pivn = rlc(cmsub(v[0][cn], lrc[0], v[0][c]), cn);
+            const cplx yp = rlc(yv[0], c);  // y'_c, final
+            // s_{c2,c} of the panel's diagonal block, read from lane c2
+#pragma unroll
+            for (int c2 = c + 1; c2 < kNB; ++c2) {
+              if (c2 < nbc) {
+                const cplx s2 = rlc(v[0][c], c2);
+#pragma unroll
+                for (int q = 0; q < ROWS; ++q) v[q][c2] = cmsub(v[q][c2], lrc[q], s2);
+              }
             }
 #pragma unroll
-            for (int u = 0; u < H; ++u) {
-              const int c2 = h0 + u;
-              const bool upd = c2 > c && c2 <= cend && c2 != cskip;
-              if (upd) myrow[c2] = cmsub(rv[u], lrc, cv[u]);
+            for (int q = 0; q < ROWS; ++q) yv[q] = cmsub(yv[q], lrc[q], yp);
+            const cplx lcc = csqrt_(piv);
+#pragma unroll
+            for (int q = 0; q < ROWS; ++q) v[q][c] = csel(below[q], cmul(lrc[q], lcc), v[q][c]);
+            v[0][c] = csel(lane == c, lcc, v[0][c]);
+            yv[0] = csel(lane == c, cmul(cmul(yp, lcc), inv), yv[0]);
+            piv = pivn;
+          }
+        }
+        if (zero_pivot && lane == 0 && atomicCAS(A.err, 0, 28) == 0) A.err[1] = sys;
+        // L of panel J: back to the panel buffer (stage J+1's operand) and out
+        // to the band rows; y of its columns is final
+#pragma unroll
+        for (int q = 0; q < ROWS; ++q) {
+          const int rw = lane + 64 * q;
+          if (rw < W) {
+#pragma unroll
+            for (int k = 0; k < kNB; ++k) {
+              const bool ok = rw < rows && k < nbc && k <= rw && rw - k <= mb;
+              Pc[pix(rw, k)] = csel(ok, v[q][k], mkc(0.0, 0.0));
+              if (ok) Lg[(size_t)(o + rw) * M1 + (k - rw + mb)] = make_double2(v[q][k].re, v[q][k].im);
             }
           }
-          yr = cmsub(yr, lrc, ycb[buf]);
         }
-      }
-      // the panel's columns are final: out to L (band-row layout)
-      for (int idx = tid; idx < rows * NB; idx += nthr) {
-        const int rr = idx / NB, cc = idx - rr * NB;
-        if (cc < nbc && cc <= rr && rr - cc <= mb) {
-          const cplx v = T[tri(rr, cc)];
-          Lg[(size_t)(j0 + rr) * (mb + 1) + (cc - rr + mb)] = make_double2(v.re, v.im);
+        if (lane < nbc) yg[o + lane] = make_double2(yv[0].re, yv[0].im);
+        // y of rows [o + kNB, o + W) moves up kNB rows; entering rows get b
+        const int src = (lane + kNB) & 63;
+        cplx ny[ROWS];
+#pragma unroll
+        for (int q = 0; q < ROWS; ++q) {
+          cplx a, b = mkc(0.0, 0.0);
+          a.re = __shfl(yv[q].re, src, 64);
+          a.im = __shfl(yv[q].im, src, 64);
+          if (q + 1 < ROWS) {
+            b.re = __shfl(yv[q + 1].re, src, 64);
+            b.im = __shfl(yv[q + 1].im, src, 64);
+          }
+          ny[q] = csel(lane + kNB < 64, a, b);
+          const int rwn = lane + 64 * q;
+          if (rwn >= mb) {
+            const int r = o + kNB + rwn;
+            ny[q] = csel(rwn < W && r < n, rhs(r), mkc(0.0, 0.0));
+          }
         }
-      }
-      if (live) yw[r] = yr;
-      lds_barrier();
-      clk.mark(2);
-      if (j0 + NB >= n) break;  // last panel: nothing trails it
-
-      const int j0w = j0 + NB;
-      const int r_new = j0w + W - NB + tid;  // threads 0..NB-1: one entering row each
-      const bool has_new = tid < NB && r_new < n;
-
-      // ---- trailing update of the (rows-NB)^2 lower triangle, moved up by NB
-      const int m = rows - NB;
-      const int nt = (m + 3) / 4;
-      const int ntiles = (nt * (nt + 1)) / 2;
-      cplx acc[MAXT][4][4];
-      int tr0[MAXT], tc0[MAXT];
 #pragma unroll
-      for (int q = 0; q < MAXT; ++q) {
-        const int tile = tid + q * nthr;
-        tr0[q] = -1;
-        tc0[q] = 0;
-        if (tile < ntiles) {
-          int ti = (int)((sqrt(8.0 * tile + 1.0) - 1.0) * 0.5);
-          while ((ti * (ti + 1)) / 2 > tile) --ti;
-          while (((ti + 1) * (ti + 2)) / 2 <= tile) ++ti;
-          const int tj = tile - (ti * (ti + 1)) / 2;
-          tr0[q] = NB + 4 * ti;
-          tc0[q] = NB + 4 * tj;
+        for (int q = 0; q < ROWS; ++q) yv[q] = ny[q];
+        clk.mark(2);
+      } else {
+        // batch J+1 = rows [o + kNB + mb, o + 2 kNB + mb): loads issued first,
+        // stored to LDS after the tiles
+        cplx nv = mkc(0.0, 0.0);
+        int nt = -1;
+        if (x3 < kNB * kMaxR) row_slot(A, Sg, o + kNB + mb + x3 / kMaxR, x3 % kMaxR, nv, nt);
+        // the rest of the window: panel J-1's update in place
 #pragma unroll
-          for (int i = 0; i < 4; ++i)
-#pragma unroll
-            for (int j = 0; j < 4; ++j) {
-              const int rr = tr0[q] + i, cc = tc0[q] + j;
-              acc[q][i][j] = (rr < rows && cc <= rr) ? T[tri(rr, cc)] : mkc(0.0, 0.0);
-            }
-          for (int kk = 0; kk < NB; ++kk) {
-            cplx a[4], b[4];
-#pragma unroll
-            for (int i = 0; i < 4; ++i) {
-              a[i] = T[tri(min(tr0[q] + i, rows - 1), kk)];
-              b[i] = T[tri(min(tc0[q] + i, rows - 1), kk)];
-            }
+        for (int q = 0; q < MAXT; ++q) {
+          if (rs_r0[q] >= 0) {
+            cplx acc[4][4];
+            tile_update<4, 4>(C, Tt, rs_r0[q], rs_c0[q], acc);
 #pragma unroll
             for (int i = 0; i < 4; ++i)
 #pragma unroll
-              for (int j = 0; j < 4; ++j) acc[q][i][j] = cmsub(acc[q][i][j], a[i], b[j]);
+              for (int j = 0; j < 4; ++j)
+                if (tile_ok(C, rs_r0[q] + i, rs_c0[q] + j)) Tt[C.ring.pos(rs_r0[q] + i, rs_c0[q] + j)] = acc[i][j];
           }
         }
-      }
-      cplx ysh = mkc(0.0, 0.0);
-      double fsh = 0.0;
-      if (NB + tid < rows) {
-        ysh = yw[NB + tid];
-        fsh = fw[NB + tid];
-      }
-      lds_barrier();
-#pragma unroll
-      for (int q = 0; q < MAXT; ++q) {
-        if (tr0[q] < 0) continue;
-#pragma unroll
-        for (int i = 0; i < 4; ++i)
-#pragma unroll
-          for (int j = 0; j < 4; ++j) {
-            const int rr = tr0[q] + i, cc = tc0[q] + j;
-            if (rr < rows && cc <= rr) T[tri(rr - NB, cc - NB)] = acc[q][i][j];
-          }
-      }
-      if (NB + tid < rows) {
-        yw[tid] = ysh;
-        fw[tid] = fsh;
-      }
-      for (int i = tri(W - NB, 0) + tid; i < ntri; i += nthr) T[i] = mkc(0.0, 0.0);
-      lds_barrier();
-      clk.mark(4);
-
-      // ---- the entering rows: unscaled entries and scaling factors, then scaled
-      double fr = 0.0;
-      if (has_new) {
-        fr = put_row_unscaled(A, k, l, r_new, r_new - j0w, j0w, av, T, sys);
-        fw[r_new - j0w] = fr;
-        fak[r_new] = fr;
+        if (x3 < kNB * kMaxR) {
+          sv[bn * kNB * kMaxR + x3] = nv;
+          so[bn * kNB * kMaxR + x3] = (int16_t)nt;
+        }
+        clk.mark(2);
       }
       lds_barrier();
-      if (has_new) {
-        scale_row(A, r_new, r_new - j0w, j0w, fr, fw, T);
-        yw[r_new - j0w] = rhs_value(A, l, r_new, fr);
-      }
-      lds_barrier();
-      clk.mark(5);
+      clk.mark(3);
     }
-    lds_barrier();
   }
   clk.flush();
 }
 
-template __global__ void crt_factor_kernel<16, 1>(FactorArgs);
-template __global__ void crt_factor_kernel<16, 2>(FactorArgs);
+template __global__ void crt_factor_kernel<1, 1>(FactorArgs);
+template __global__ void crt_factor_kernel<2, 1>(FactorArgs);
+template __global__ void crt_factor_kernel<2, 2>(FactorArgs);
+template __global__ void crt_factor_kernel<2, 3>(FactorArgs);
+template __global__ void crt_factor_kernel<3, 3>(FactorArgs);
 
 // ---------------------------------------------------------------------------
 // Back substitution, one wavefront per system.  Lane j (mod 64) keeps the

@@ -8,12 +8,13 @@
 
 // Assembly term marking "this row is a Dirichlet row: unit diagonal".
 constexpr int kDirichletUnit = -0x7fffffff;
-// Stored band entries per matrix row (pattern slots) the factor kernel keeps
-// in registers.
+// Stored band entries per matrix row (pattern slots) of the scaled-row records.
 constexpr int kMaxR = 8;
+// Columns per stage of the factor kernel (panel width).
+constexpr int kNB = 8;
 
-// Arguments of the assembly + factor + forward-sweep kernel.  Device pointers;
-// node / element indices are 0-based.
+// Arguments of the assembly, scaling and factor + forward-sweep kernels.
+// Device pointers; node / element indices are 0-based.
 struct FactorArgs {
   int n, mb, eanz, elanz, relanz, nimn;
   int nsys;                 // systems in this batch
@@ -41,10 +42,11 @@ struct FactorArgs {
   const uint8_t* isdir;     // n Dirichlet flags, or null
   int lsink, nsink;
   const double2* av;        // [K][n][R] area sums (crt_area_values_kernel)
-  // per-system outputs of the batch (index = system - sys_first)
+  // per-system arrays of the batch (index = system - sys_first)
+  double* fak;              // [nsys][n] scaling factors (crt_scale_kernel)
+  double2* S;               // [nsys][n][R] scaled row entries (crt_rows_kernel)
   double2* L;               // [nsys][n][mb+1]
   double2* y;               // [nsys][n]
-  double* fak;              // [nsys][n]
   int* err;                 // first error code (0 = ok) and the failing system
   unsigned long long* prof; // optional: 8 phase-time accumulators (wall clock ticks)
 };
@@ -58,13 +60,16 @@ struct BacksolveArgs {
 };
 
 __global__ void crt_area_values_kernel(FactorArgs A, int K, double2* av);
-template <int NB, int MAXT>
+__global__ void crt_scale_kernel(FactorArgs A);
+__global__ void crt_rows_kernel(FactorArgs A);
+template <int ROWS, int MAXT>
 __global__ void crt_factor_kernel(FactorArgs A);
 template <int SLOTS, int DEPTH>
 __global__ void crt_backsolve_kernel(BacksolveArgs B);
 
-// LDS bytes the factor kernel needs for bandwidth mb and panel width nb.
-inline size_t crt_factor_lds_bytes(int mb, int nb) {
-  size_t W = (size_t)mb + nb;
-  return (W * (W + 1) / 2 + W + 2 * (size_t)nb + 4) * 16 + W * 8;
+// LDS bytes of the factor kernel for bandwidth mb: the trailing-window rings,
+// two panels and three staged row batches.
+inline size_t crt_factor_lds_bytes(int mb) {
+  const size_t M1 = (size_t)mb + 1, half = M1 / 2, W = (size_t)mb + kNB;
+  return (half * M1 + 2 * W * kNB + 3 * kNB * kMaxR) * 16 + 3 * kNB * kMaxR * 2;
 }
