@@ -49,6 +49,7 @@ def lib():
         L.orc_chol.argtypes = [C.c_int, C.c_int, _z]
         L.orc_vre.argtypes = [C.c_int, C.c_int, _z, _z, _z]
         L.orc_vre.restype = None
+        L.orc_solve_ld.argtypes = [C.c_int, C.c_int, _z, _z, _z]
         L.orc_forward.argtypes = [P, C.c_int, _d, C.c_int, _d, _d, _d, C.c_int, C.c_int, _z,
                                   _z, _z, C.c_int, C.c_int, C.c_int]
         L.orc_volt.argtypes = [P, C.c_int, _i, _i, _z, C.c_int, C.c_void_p, _z, _z]
@@ -163,6 +164,17 @@ class Oracle:
         pot = np.zeros(self.mesh.sanz, np.complex128)
         lib().orc_vre(self.mesh.sanz, self.mesh.mb, a, np.ascontiguousarray(b), pot)
         return pot
+
+    def solve_ld(self, sigma, l, k):
+        """kpot(:, l, k) of the assembled system solved in long double (a
+        verification aid: the exact solution of the discrete system to ~1e-19
+        times its condition number)."""
+        a, b, fak = self.assemble(sigma, l, k)
+        pot = np.zeros(self.mesh.sanz, np.complex128)
+        e = lib().orc_solve_ld(self.mesh.sanz, self.mesh.mb, a, b, pot)
+        if e:
+            raise OracleError("solve_ld", e)
+        return pot * fak
 
     # --- A4..A11 ------------------------------------------------------------
     def forward(self, sigma, nthreads=0, sys_range=None):

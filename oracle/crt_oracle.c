@@ -598,6 +598,72 @@ void orc_vre(int sanz, int mb, const orc_c *a, const orc_c *b, orc_c *pot) {
   }
 }
 
+/* Verification aid, not a restatement: the same band system (scaled matrix a,
+ * right-hand side b, both as orc_assemble returns them) factored and solved in
+ * x87 long double (64-bit significand), pot = x (A x + b = 0).  Comparing the
+ * double-precision oracle and the HIP path with this solution measures how far
+ * each is from the exact solution of the discrete system, i.e. the
+ * conditioning floor that any FP64 evaluation order shares.  Plain complex
+ * long double arithmetic; divisions by the pivot use the textbook formula. */
+typedef struct { long double re, im; } ld_c;
+static inline ld_c ld_mul(ld_c a, ld_c b) {
+  ld_c r = {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
+  return r;
+}
+static inline ld_c ld_div(ld_c a, ld_c b) {
+  long double den = b.re * b.re + b.im * b.im;
+  ld_c r = {(a.re * b.re + a.im * b.im) / den, (a.im * b.re - a.re * b.im) / den};
+  return r;
+}
+static inline ld_c ld_sqrt(ld_c z) {
+  long double t = sqrtl((fabsl(z.re) + hypotl(z.re, z.im)) * 0.5L);
+  ld_c r;
+  if (t == 0.0L) { r.re = 0.0L; r.im = z.im; return r; }
+  if (z.re >= 0.0L) { r.re = t; r.im = z.im / (2.0L * t); }
+  else { r.re = fabsl(z.im) / (2.0L * t); r.im = copysignl(t, z.im); }
+  return r;
+}
+int orc_solve_ld(int sanz, int mb, const orc_c *a_in, const orc_c *b, orc_c *pot) {
+  const int m1 = mb + 1;
+  ld_c *a = (ld_c *)malloc(sizeof(ld_c) * (size_t)m1 * sanz);
+  ld_c *x = (ld_c *)malloc(sizeof(ld_c) * (size_t)sanz);
+  if (!a || !x) { free(a); free(x); return -1; }
+  for (size_t i = 0; i < (size_t)m1 * sanz; i++) { a[i].re = a_in[i].re; a[i].im = a_in[i].im; }
+  /* row-oriented band L L^T, row i holds columns i-mb..i (diagonal last) */
+  for (int i = 0; i < sanz; i++) {
+    const int fi = i - mb > 0 ? i - mb : 0;
+    ld_c *ri = a + (size_t)i * m1 + (mb - i);  /* ri[j] = entry (i, j) */
+    for (int j = fi; j <= i; j++) {
+      ld_c *rj = a + (size_t)j * m1 + (mb - j);
+      ld_c s = ri[j];
+      for (int k = fi; k < j; k++) {
+        ld_c p = ld_mul(ri[k], rj[k]);
+        s.re -= p.re; s.im -= p.im;
+      }
+      if (j < i) ri[j] = ld_div(s, rj[j]);
+      else { if (s.re == 0.0L && s.im == 0.0L) { free(a); free(x); return 28; } ri[i] = ld_sqrt(s); }
+    }
+  }
+  for (int i = 0; i < sanz; i++) {
+    const int fi = i - mb > 0 ? i - mb : 0;
+    const ld_c *ri = a + (size_t)i * m1 + (mb - i);
+    ld_c s = {b[i].re, b[i].im};
+    for (int j = fi; j < i; j++) { ld_c p = ld_mul(ri[j], x[j]); s.re -= p.re; s.im -= p.im; }
+    x[i] = ld_div(s, ri[i]);
+  }
+  for (int i = sanz - 1; i >= 0; i--) {
+    const int fi = i - mb > 0 ? i - mb : 0;
+    const ld_c *ri = a + (size_t)i * m1 + (mb - i);
+    ld_c q = ld_div(x[i], ri[i]);
+    x[i].re = -q.re; x[i].im = -q.im;
+    for (int j = fi; j < i; j++) { ld_c p = ld_mul(ri[j], x[i]); x[j].re += p.re; x[j].im += p.im; }
+  }
+  for (int i = 0; i < sanz; i++) { pot[i].re = (double)x[i].re; pot[i].im = (double)x[i].im; }
+  free(a);
+  free(x);
+  return 0;
+}
+
 int orc_assemble(const orc_mesh *m, int K, const double *elbg, const double *relbg,
                  const double *kg, int lrandb, const orc_c *sigma, int l, int k,
                  orc_c *a, orc_c *b, double *fak) {

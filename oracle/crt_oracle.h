@@ -29,6 +29,7 @@ int orc_assemble(const orc_mesh *m, int kwnanz, const double *elbg, const double
                  orc_c *a, orc_c *b, double *fak);
 int orc_chol(int sanz, int mb, orc_c *a);
 void orc_vre(int sanz, int mb, const orc_c *a, const orc_c *b, orc_c *pot);
+int orc_solve_ld(int sanz, int mb, const orc_c *a, const orc_c *b, orc_c *pot);
 int orc_forward(const orc_mesh *m, int kwnanz, const double *kwnwi, int swrtr,
                 const double *elbg, const double *relbg, const double *kg, int lbeta,
                 int lrandb, const orc_c *sigma, orc_c *kpot, orc_c *hpot, int nthreads,

@@ -58,15 +58,11 @@ struct cdiv_by {
 };
 __device__ __forceinline__ cdiv_by cdiv_prep(cplx b) {
   cdiv_by p;
-  p.c = b.re;
-  p.d = b.im;
-  const double m = fmax(fabs(p.c), fabs(p.d));
-  p.il = 0;
-  if (m != 0.0 && isfinite(m)) {
-    p.il = __builtin_amdgcn_frexp_exp(m) - 1;
-    p.c = __builtin_amdgcn_ldexp(p.c, -p.il);
-    p.d = __builtin_amdgcn_ldexp(p.d, -p.il);
-  }
+  const double m = fmax(fabs(b.re), fabs(b.im));
+  // branch-free: il = 0 (no scaling) for a zero or non-finite divisor
+  p.il = (m != 0.0 && isfinite(m)) ? __builtin_amdgcn_frexp_exp(m) - 1 : 0;
+  p.c = __builtin_amdgcn_ldexp(b.re, -p.il);
+  p.d = __builtin_amdgcn_ldexp(b.im, -p.il);
   p.den = p.c * p.c + p.d * p.d;
   return p;
 }
@@ -93,11 +89,12 @@ __device__ __forceinline__ cplx crecip(cplx z) {
 // and (|y|/2t, sign(y) t) otherwise -- one division either way.
 __device__ __forceinline__ cplx csqrt_(cplx z) {
   const double x = z.re, y = z.im;
-  if (x == 0.0 && y == 0.0) return mkc(0.0, y);
   const double t = sqrt((fabs(x) + hypot(x, y)) * 0.5);
   const bool pos = x >= 0.0;
   const double u = (pos ? y : fabs(y)) / (2.0 * t);
-  return pos ? mkc(t, u) : mkc(u, copysign(t, y));
+  // branch-free selects; z = 0 gives (0, y)
+  const bool zero = x == 0.0 && y == 0.0;
+  return mkc(zero ? 0.0 : (pos ? t : u), zero ? y : (pos ? u : copysign(t, y)));
 }
 
 __device__ __forceinline__ cplx clog_(cplx z) { return mkc(log(cabs_(z)), atan2(z.im, z.re)); }

@@ -267,7 +267,11 @@ __global__ __launch_bounds__(256, 2) void crt_factor_kernel(FactorArgs A) {
   cplx* Tt = reinterpret_cast<cplx*>(smem);  // half x M1 ring slots
   cplx* P0 = Tt + half * M1;                  // 2 panels of W x kNB
   cplx* sv = P0 + 2 * W * kNB;                // kStg batches of kNB x kMaxR values
-  int16_t* so = reinterpret_cast<int16_t*>(sv + kStg * kNB * kMaxR);  // kStg x kNB x kMaxR offsets
+  cplx* xb = sv + kStg * kNB * kMaxR;         // kNB: column broadcast of the panel
+  cplx* xd = xb + kNB;                        // kNB: the panel's pivots d_c
+  cplx* xi = xd + kNB;                        // kNB: 1/d_c
+  cplx* xl = xi + kNB;                        // kNB: L_cc = sqrt(d_c)
+  int16_t* so = reinterpret_cast<int16_t*>(xl + kNB);  // kStg x kNB x kMaxR offsets
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int x3 = tid - 64;                    // thread index within waves 1-3
   const int nst = (n + kNB - 1) / kNB;
@@ -422,6 +426,12 @@ __global__ __launch_bounds__(256, 2) void crt_factor_kernel(FactorArgs A) {
         // L_cc = sqrt(d), L_rc = l_rc L_cc and y_c = y'_c / L_cc follow off it.
         // lrc is 0 for rows at or above c, so every update is unconditional
         // (a - 0 * x leaves a unchanged).
+        // Columns past the matrix end (the last stage only) are skipped; their
+        // pivot slots hold 1 so the square roots below stay finite.  The loop leaves
+        // l_rc in place (0 at and above the diagonal); the square roots of all
+        // kNB pivots are taken at once afterwards (lane c) and turn l into L
+        // and y' into y.  Broadcasts go through LDS (one wavefront: its LDS
+        // operations complete in issue order), except the next pivot and y'_c.
         cplx piv = rlc(v[0][0], 0);
         bool zero_pivot = false;
 #pragma unroll
@@ -430,36 +440,48 @@ __global__ __launch_bounds__(256, 2) void crt_factor_kernel(FactorArgs A) {
             // chol :66 -- |s| <= 0 is an exact zero (a NaN pivot passes, as there)
             zero_pivot |= piv.re == 0.0 && piv.im == 0.0;
             const cplx inv = crecip(piv);
+            xd[c] = piv;  // every lane stores the same value
+            xi[c] = inv;
             cplx lrc[ROWS];
-            bool below[ROWS];
 #pragma unroll
-            for (int q = 0; q < ROWS; ++q) {
-              below[q] = lane + 64 * q > c;
-              lrc[q] = csel(below[q], cmul(v[q][c], inv), mkc(0.0, 0.0));
-            }
+            for (int q = 0; q < ROWS; ++q) lrc[q] = csel(lane + 64 * q > c, cmul(v[q][c], inv), mkc(0.0, 0.0));
             const int cn = c + 1 < kNB ? c + 1 : kNB - 1;
-            cplx pivn = mkc(0.0, 0.0);
+            cplx pivn = mkc(1.0, 0.0);
             if (c + 1 < nbc) pivn = rlc(cmsub(v[0][cn], lrc[0], v[0][c]), cn);
             const cplx yp = rlc(yv[0], c);  // y'_c, final
-            // s_{c2,c} of the panel's diagonal block, read from lane c2
+            // s_{c2,c} of the panel's diagonal block from lanes c+1..kNB-1
+            if (lane > c && lane < kNB) xb[lane] = v[0][c];
 #pragma unroll
             for (int c2 = c + 1; c2 < kNB; ++c2) {
               if (c2 < nbc) {
-                const cplx s2 = rlc(v[0][c], c2);
+                const cplx s2 = xb[c2];
 #pragma unroll
                 for (int q = 0; q < ROWS; ++q) v[q][c2] = cmsub(v[q][c2], lrc[q], s2);
               }
             }
 #pragma unroll
-            for (int q = 0; q < ROWS; ++q) yv[q] = cmsub(yv[q], lrc[q], yp);
-            const cplx lcc = csqrt_(piv);
-#pragma unroll
-            for (int q = 0; q < ROWS; ++q) v[q][c] = csel(below[q], cmul(lrc[q], lcc), v[q][c]);
-            v[0][c] = csel(lane == c, lcc, v[0][c]);
-            yv[0] = csel(lane == c, cmul(cmul(yp, lcc), inv), yv[0]);
+            for (int q = 0; q < ROWS; ++q) {
+              yv[q] = cmsub(yv[q], lrc[q], yp);
+              v[q][c] = lrc[q];
+            }
             piv = pivn;
+          } else {
+            xd[c] = mkc(1.0, 0.0);
+            xi[c] = mkc(1.0, 0.0);
           }
         }
+        const int lk = min(lane, kNB - 1);
+        const cplx dd = xd[lk], invd = xi[lk];
+        const cplx lcc = csqrt_(dd);
+        if (lane < kNB) xl[lane] = lcc;
+#pragma unroll
+        for (int c = 0; c < kNB; ++c) {
+          const cplx lc = xl[c];
+#pragma unroll
+          for (int q = 0; q < ROWS; ++q) v[q][c] = cmul(v[q][c], lc);
+          v[0][c] = csel(lane == c, lc, v[0][c]);
+        }
+        yv[0] = csel(lane < kNB, cmul(cmul(yv[0], lcc), invd), yv[0]);
         if (zero_pivot && lane == 0 && atomicCAS(A.err, 0, 28) == 0) A.err[1] = sys;
         // L of panel J: back to the panel buffer (stage J+1's operand) and out
         // to the band rows; y of its columns is final

@@ -68,8 +68,8 @@ template <int SLOTS, int DEPTH>
 __global__ void crt_backsolve_kernel(BacksolveArgs B);
 
 // LDS bytes of the factor kernel for bandwidth mb: the trailing-window rings,
-// two panels and three staged row batches.
+// two panels, three staged row batches and the panel's broadcast slots.
 inline size_t crt_factor_lds_bytes(int mb) {
   const size_t M1 = (size_t)mb + 1, half = M1 / 2, W = (size_t)mb + kNB;
-  return (half * M1 + 2 * W * kNB + 3 * kNB * kMaxR) * 16 + 3 * kNB * kMaxR * 2;
+  return (half * M1 + 2 * W * kNB + 3 * kNB * kMaxR + 4 * kNB) * 16 + 3 * kNB * kMaxR * 2;
 }
