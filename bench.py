@@ -155,6 +155,16 @@ def main():
     achieved = flops / fac_avg_s / 1e12
     lbytes = nsys * 2.0 * 16.0 * n * (mb + 1)
 
+    # HBM bytes per factor launch from the committed rocprofv3 FETCH_SIZE /
+    # WRITE_SIZE passes of this workload (scaled by the copy-kernel calibration)
+    traffic = None
+    tf = ROOT / "profiles" / "factor_traffic_large_grf.json"
+    if tf.exists():
+        try:
+            traffic = json.loads(tf.read_text())["factor"]["hbm_bytes"]
+        except (ValueError, KeyError):
+            traffic = None
+
     result = {
         "metric": "modelled quadrupoles/s and adjoint-gradient ms vs explicit J, FP64, 1 GPU",
         "value": value,
@@ -173,7 +183,7 @@ def main():
                    "systems": nsys, "nodes": n, "bandwidth": mb, "electrodes": E,
                    "quadrupoles": N, "wavenumbers": K},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP64_TFLOPS,
-                     "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None,
+                     "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_TFLOPS, "traffic": traffic,
                      "kernel": "crt_factor_kernel", "avg_launch_ms": fac_ms / fac_n,
                      "algorithmic_flops": flops, "algorithmic_bytes": lbytes},
         "stage_ms_per_step": {k: v[0] / v[1] for k, v in times.items()},

@@ -244,6 +244,12 @@ def test_large_mesh_systems_match_oracle(crtmod, oracle_lib):
     for s in [0, 95, 5 * E + 40, 13 * E + 95]:
         k, l = divmod(s, E)
         assert rel_err(kpot[:, l, k], o.kpot[:, l, k]) < 1e-9, (k, l)
+    # against the exact solution of the same system (long double): the HIP
+    # path's error is within 3x the FP64 oracle's own
+    kl = o.solve_ld(w.sigma, 0, 0)
+    e_hip, e_orc = rel_err(kpot[:, 0, 0], kl), rel_err(o.kpot[:, 0, 0], kl)
+    say("k=0 l=0 distance from the long-double solution: hip", e_hip, "oracle", e_orc)
+    assert e_hip < 3 * e_orc, (e_hip, e_orc)
     v, sa = g.bvolt(w.strnr, w.vnr)
     vo, _ = o.volt(w.strnr, w.vnr, 0)
     check_volt(v, vo)

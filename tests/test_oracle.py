@@ -213,6 +213,25 @@ def test_forward_against_dense_solve(small):
         assert rel_err(o.kpot[:, l, k], x) < 1e-11
 
 
+def test_long_double_solve_pins_conditioning(small, oracle_lib):
+    """The same assembled system solved in long double: on the small mesh the
+    FP64 oracle agrees to ~1e-13; on the 240x80 mesh its lowest-wavenumber
+    system (the worst conditioned) is ~1e-10 from the exact solution, which is
+    the floor any FP64 evaluation order shares (the HIP path is held to a
+    small multiple of it)."""
+    w, o = small
+    for (l, k) in [(0, 0), (11, 6)]:
+        kl = o.solve_ld(w.sigma, l, k)
+        assert rel_err(o.kpot[:, l, k], kl) < 1e-12
+    big = meshgen.large_grf()
+    ob = oracle_lib.Oracle(big.mesh)
+    ob.rwaven(big.strnr, big.vnr)
+    ob.precal()
+    ob.forward(big.sigma, sys_range=(0, 1))
+    e0 = rel_err(ob.kpot[:, 0, 0], ob.solve_ld(big.sigma, 0, 0))
+    assert 1e-12 < e0 < 3e-10, e0
+
+
 def test_forward_matches_reference(small):
     w, o = small
     g = golden("small_block")
