@@ -154,7 +154,9 @@ def test_cg_solve_matches_oracle(small_pair, oracle_lib):
     assert rel_err(dpar, dpar_o * cgfac_o) < STOL
 
 
-def test_dirichlet_factor_reuse(crtmod):
+def test_dirichlet_factor_reuse(crtmod, oracle_lib):
+    """No mixed boundaries: one factor per wavenumber serves every source
+    (kompb), each source takes its own forward and back sweep."""
     w = meshgen.small_dirichlet()
     say("small_dirichlet")
     g = crtmod.CRT(w.mesh)
@@ -164,6 +166,11 @@ def test_dirichlet_factor_reuse(crtmod):
     (kpot,) = g.forward(w.sigma, want_kpot=True)
     ge = golden("small_dirichlet")
     assert rel_err(kpot[w.mesh.enr - 1], ge["kpot_elec"]) < VTOL
+    o = oracle_lib.Oracle(w.mesh)
+    o.rwaven(w.strnr, w.vnr)
+    o.precal()
+    o.forward(w.sigma)
+    assert rel_err(kpot, o.kpot) < VTOL
     v, _ = g.bvolt(w.strnr, w.vnr)
     check_volt(v, ge["volt"])
 

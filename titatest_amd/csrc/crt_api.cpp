@@ -724,17 +724,27 @@ static int run_forward(crt_handle* h) {
   hipLaunchKernelGGL(crt_area_values_kernel, dim3((unsigned)(((size_t)K * n + 255) / 256)), dim3(256), 0, h->stream,
                      A, K, av);
   tstop(h);
-  // systems per batch: the scaled rows, L, y and the scaling factors of every
-  // system in the batch stay in device memory until its back substitution
-  const size_t per_sys = (size_t)n * (mb + 1) * 16 + (size_t)n * h->R * 16 + (size_t)n * 24;
+  // Without mixed boundaries the matrix does not depend on the source: one
+  // factor per wavenumber serves all eanz sources (src/fem.f90:372-376,
+  // kompb), each source then takes both sweeps.  Otherwise every (k, l) system
+  // is factored with its own forward sweep fused in.
+  const bool reuse = !h->has11;
+  const int nfac = reuse ? K : nsys;
+  const int rpf = reuse ? E : 1;  // right-hand sides per factor
+  A.sys_stride = reuse ? E : 1;
+  // factors per batch: the scaled rows, L, y and the scaling factors of every
+  // factor in the batch stay in device memory until its back substitution
+  const size_t per_sys = (size_t)n * (mb + 1) * 16 + (size_t)n * h->R * 16 + (size_t)n * 24 +
+                         (reuse ? (size_t)E * n * 16 : 0);
   size_t free_b = 0, total_b = 0;
   if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) free_b = (size_t)8 << 30;
   size_t budget = std::min<size_t>(free_b / 2, h->batch_bytes_max);
   const size_t have = h->bufsz.count("L") && h->bufsz.count("S")
-                          ? h->bufsz["L"] + h->bufsz["S"] + h->bufsz["y"] + h->bufsz["fak"]
+                          ? h->bufsz["L"] + h->bufsz["S"] + h->bufsz["y"] + h->bufsz["fak"] +
+                                (h->bufsz.count("yr") ? h->bufsz["yr"] : 0)
                           : 0;
   budget = std::max(budget, have);
-  int batch = (int)std::max<size_t>(1, std::min<size_t>((size_t)nsys, budget / per_sys));
+  int batch = (int)std::max<size_t>(1, std::min<size_t>((size_t)nfac, budget / per_sys));
   A.L = h->get<double2>("L", (size_t)batch * n * (mb + 1));
   A.S = h->get<double2>("S", (size_t)batch * n * h->R);
   A.y = h->get<double2>("y", (size_t)batch * n);
@@ -743,16 +753,34 @@ static int run_forward(crt_handle* h) {
   NEED(A.S);
   NEED(A.y);
   NEED(A.fak);
+  double2* yr = nullptr;
+  if (reuse) {
+    yr = h->get<double2>("yr", (size_t)batch * E * n);
+    NEED(yr);
+  }
   BacksolveArgs B;
   B.n = n;
   B.mb = mb;
   B.eanz = E;
+  B.rhs_per_fac = rpf;
+  B.fstride = A.sys_stride;
   B.L = A.L;
-  B.y = A.y;
+  B.y = reuse ? yr : A.y;
   B.fak = A.fak;
   B.kpot = kpot;
-  for (int first = 0; first < nsys; first += batch) {
-    const int cnt = std::min(batch, nsys - first);
+  FwdArgs F;
+  F.n = n;
+  F.mb = mb;
+  F.eanz = E;
+  F.L = A.L;
+  F.fak = A.fak;
+  F.enr = A.enr;
+  F.isdir = A.isdir;
+  F.lsink = A.lsink;
+  F.nsink = A.nsink;
+  F.y = yr;
+  for (int first = 0; first < nfac; first += batch) {
+    const int cnt = std::min(batch, nfac - first);
     A.sys_first = first;
     A.nsys = cnt;
     const unsigned rgrid = (unsigned)(((size_t)cnt * n + 255) / 256);
@@ -764,13 +792,24 @@ static int run_forward(crt_handle* h) {
     tstart(h, "factor_solve");
     hipLaunchKernelGGL(kern, dim3(h->slots), dim3(256), lds, h->stream, A);
     tstop(h);
-    B.sys_first = first;
-    B.nsys = cnt;
+    const int nrhs = cnt * rpf;
+    if (reuse) {
+      F.nrhs = nrhs;
+      F.fac_first = first;
+      tstart(h, "forward_sweep");
+      if (mb < 128)
+        hipLaunchKernelGGL((crt_fwdsweep_kernel<2, 8>), dim3(nrhs), dim3(64), 0, h->stream, F);
+      else
+        hipLaunchKernelGGL((crt_fwdsweep_kernel<4, 4>), dim3(nrhs), dim3(64), 0, h->stream, F);
+      tstop(h);
+    }
+    B.nrhs = nrhs;
+    B.fac_first = first;
     tstart(h, "backsolve");
     if (mb < 128)
-      hipLaunchKernelGGL((crt_backsolve_kernel<2, 8>), dim3(cnt), dim3(64), 0, h->stream, B);
+      hipLaunchKernelGGL((crt_backsolve_kernel<2, 8>), dim3(nrhs), dim3(64), 0, h->stream, B);
     else
-      hipLaunchKernelGGL((crt_backsolve_kernel<4, 4>), dim3(cnt), dim3(64), 0, h->stream, B);
+      hipLaunchKernelGGL((crt_backsolve_kernel<4, 4>), dim3(nrhs), dim3(64), 0, h->stream, B);
     tstop(h);
   }
   // electrode slice of the back-transform, used by the voltage stage

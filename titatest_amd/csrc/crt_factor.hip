@@ -214,7 +214,7 @@ __global__ void crt_scale_kernel(FactorArgs A) {
   const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= (long)A.nsys * A.n) return;
   const int s = (int)(idx / A.n), r = (int)(idx % A.n);
-  const int sys = A.sys_first + s;
+  const int sys = (A.sys_first + s) * A.sys_stride;
   const int k = sys / A.eanz, l = sys % A.eanz;
   const int qd = A.pat_diag[r];
   cplx v = ldc(A.av + ((size_t)k * A.n + r) * A.R + qd);
@@ -236,7 +236,7 @@ __global__ void crt_rows_kernel(FactorArgs A) {
   const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= (long)A.nsys * A.n) return;
   const int s = (int)(idx / A.n), r = (int)(idx % A.n);
-  const int sys = A.sys_first + s;
+  const int sys = (A.sys_first + s) * A.sys_stride;
   const int k = sys / A.eanz, l = sys % A.eanz;
   const double* fk = A.fak + (size_t)s * A.n;
   const double fr = fk[r];
@@ -303,7 +303,7 @@ __global__ __launch_bounds__(256, 2) void crt_factor_kernel(FactorArgs A) {
   clk.start(tid == 0 ? A.prof : (tid == 64 ? A.prof : nullptr), tid == 0 ? 0 : 4);
 
   for (int s = blockIdx.x; s < A.nsys; s += gridDim.x) {
-    const int sys = A.sys_first + s;
+    const int sys = (A.sys_first + s) * A.sys_stride;
     const int l = sys % A.eanz;
     double2* Lg = A.L + (size_t)s * n * M1;
     double2* yg = A.y + (size_t)s * n;
@@ -569,15 +569,16 @@ template __global__ void crt_factor_kernel<3, 3>(FactorArgs);
 template <int SLOTS, int DEPTH>
 __global__ __launch_bounds__(64) void crt_backsolve_kernel(BacksolveArgs B) {
   const int lane = threadIdx.x;
-  const int s = blockIdx.x;
-  if (s >= B.nsys) return;
-  const int sys = B.sys_first + s;
+  const int r = blockIdx.x;
+  if (r >= B.nrhs) return;
+  const int f = r / B.rhs_per_fac;
+  const int sys = (B.fac_first + f) * B.fstride + r % B.rhs_per_fac;
   const int k = sys / B.eanz, l = sys % B.eanz;
   const int n = B.n, mb = B.mb, m1 = mb + 1;
   constexpr int P = 64 * SLOTS;
-  const double2* Lg = B.L + (size_t)s * n * m1;
-  const double2* yg = B.y + (size_t)s * n;
-  const double* fak = B.fak + (size_t)s * n;
+  const double2* Lg = B.L + (size_t)f * n * m1;
+  const double2* yg = B.y + (size_t)r * n;
+  const double* fak = B.fak + (size_t)f * n;
   double2* out = B.kpot + ((size_t)k * B.eanz + l) * n;
 
   // class q = lane + 64*sl holds row j(i, q) = i - ((i - q) mod P) at step i
@@ -647,3 +648,101 @@ __global__ __launch_bounds__(64) void crt_backsolve_kernel(BacksolveArgs B) {
 
 template __global__ void crt_backsolve_kernel<2, 8>(BacksolveArgs);
 template __global__ void crt_backsolve_kernel<4, 4>(BacksolveArgs);
+
+// ---------------------------------------------------------------------------
+// Forward sweep L y = b, one wavefront per right-hand side, for sources that
+// share one factor (no mixed boundaries: the factor of source 1 serves all,
+// src/fem.f90:372-376 with kompb, src/kompb.f90).  b has at most two non-zero
+// rows (the source electrode, the sink), so y is zero up to the first of them
+// and the sweep starts there.  Lane j (mod 64) keeps the partial sums of the
+// rows i = q (mod 64*SLOTS) of the band ahead of the current row; row j's
+// pivot lane divides, the value is broadcast and every lane subtracts
+// L_ij y_j from its rows i in (j, j+mb] -- for each i the terms arrive in
+// ascending j, the order of vre's row sums.  Column j of L is read from the
+// rows below it, DEPTH columns ahead.
+template <int SLOTS, int DEPTH>
+__global__ __launch_bounds__(64) void crt_fwdsweep_kernel(FwdArgs F) {
+  const int lane = threadIdx.x;
+  const int r = blockIdx.x;
+  if (r >= F.nrhs) return;
+  const int f = r / F.eanz, l = r % F.eanz;
+  const int n = F.n, mb = F.mb, m1 = mb + 1;
+  constexpr int P = 64 * SLOTS;
+  const double2* Lg = F.L + (size_t)f * n * m1;
+  const double* fak = F.fak + (size_t)f * n;
+  double2* yg = F.y + (size_t)r * n;
+  // b = -f at the source electrode, +f at the sink (the sink wins a tie),
+  // zero on Dirichlet rows (kompb, randb)
+  const int re = F.enr[l], rs = F.lsink ? F.nsink : -1;
+  auto bval = [&](int i) -> cplx {
+    double v = 0.0;
+    if (i == rs) v = 1.0;
+    else if (i == re) v = -1.0;
+    if (F.isdir && F.isdir[i]) v = 0.0;
+    return mkc(v * fak[i], 0.0);
+  };
+  const int j0 = rs >= 0 ? min(re, rs) : re;
+  for (int i = lane; i < j0; i += 64) yg[i] = make_double2(0.0, 0.0);
+  // class q = lane + 64*sl holds row i(j, q) = j + ((q - j) mod P) at step j
+  cplx acc[SLOTS];
+#pragma unroll
+  for (int sl = 0; sl < SLOTS; ++sl) {
+    const int q = lane + 64 * sl;
+    const int i = j0 + ((((q - j0) % P) + P) % P);
+    acc[sl] = i < n ? bval(i) : mkc(0.0, 0.0);
+  }
+  // column j fetched DEPTH steps ahead: this lane's L_ij, and for the pivot
+  // lane L_jj and the refill value b_{j+P}
+  cplx lc[DEPTH][SLOTS];
+  cplx ljj[DEPTH], bnx[DEPTH];
+  auto fetch = [&](int j, cplx* dst, cplx& djj, cplx& dbn) {
+#pragma unroll
+    for (int sl = 0; sl < SLOTS; ++sl) {
+      const int q = lane + 64 * sl;
+      const int i = j + ((((q - j) % P) + P) % P);
+      const int t = j - i + mb;
+      dst[sl] = (j < n && i < n && i > j && t >= 0) ? ldc(Lg + (size_t)i * m1 + t) : mkc(0.0, 0.0);
+    }
+    const bool piv = j < n && lane == (j % P) % 64;
+    djj = piv ? ldc(Lg + (size_t)j * m1 + mb) : mkc(1.0, 0.0);
+    dbn = (piv && j + P < n) ? bval(j + P) : mkc(0.0, 0.0);
+  };
+#pragma unroll
+  for (int d = 0; d < DEPTH; ++d) fetch(j0 + d, lc[d], ljj[d], bnx[d]);
+
+  for (int jb = j0; jb < n; jb += DEPTH) {
+#pragma unroll
+    for (int d = 0; d < DEPTH; ++d) {
+      const int j = jb + d;
+      if (j < n) {
+        const int qp = j % P;
+        const int plane = qp % 64, pslot = qp / 64;
+        cplx yj = mkc(0.0, 0.0);
+        if (lane == plane) {
+          cplx s = acc[0];
+#pragma unroll
+          for (int sl = 1; sl < SLOTS; ++sl)
+            if (sl == pslot) s = acc[sl];
+          yj = cdiv(s, ljj[d]);
+          yg[j] = make_double2(yj.re, yj.im);
+          // the freed slot takes row j+P, first reached by column j+P-mb
+#pragma unroll
+          for (int sl = 0; sl < SLOTS; ++sl)
+            if (sl == pslot) acc[sl] = bnx[d];
+        }
+        yj.re = __shfl(yj.re, plane, 64);
+        yj.im = __shfl(yj.im, plane, 64);
+#pragma unroll
+        for (int sl = 0; sl < SLOTS; ++sl) {
+          const int q = lane + 64 * sl;
+          const int i = j + ((((q - j) % P) + P) % P);
+          if (i > j && i - j <= mb && i < n) acc[sl] = cmsub(acc[sl], lc[d][sl], yj);
+        }
+        fetch(j + DEPTH, lc[d], ljj[d], bnx[d]);
+      }
+    }
+  }
+}
+
+template __global__ void crt_fwdsweep_kernel<2, 8>(FwdArgs);
+template __global__ void crt_fwdsweep_kernel<4, 4>(FwdArgs);

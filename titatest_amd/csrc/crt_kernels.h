@@ -18,7 +18,8 @@ constexpr int kNB = 8;
 struct FactorArgs {
   int n, mb, eanz, elanz, relanz, nimn;
   int nsys;                 // systems in this batch
-  int sys_first;            // system id (k*eanz + l) of batch entry 0
+  int sys_first;            // batch entry s is system (sys_first + s) * sys_stride
+  int sys_stride;           // 1, or eanz when one factor serves every source (k*eanz + l)
   // area-term plan (CSR over rows, terms sorted by band offset then element)
   const int* rowptr;        // n+1
   const int16_t* term_t;    // band offset t = c - r + mb
@@ -51,12 +52,26 @@ struct FactorArgs {
   unsigned long long* prof; // optional: 8 phase-time accumulators (wall clock ticks)
 };
 
+// Substitution sweeps over right-hand sides: rhs r of the batch uses factor
+// f = r / rhs_per_fac of the batch and is system (fac_first + f) * fstride +
+// r % rhs_per_fac (one rhs per factor, or eanz sources sharing one factor).
 struct BacksolveArgs {
-  int n, mb, eanz, nsys, sys_first;
-  const double2* L;
-  const double2* y;
-  const double* fak;
+  int n, mb, eanz, nrhs, rhs_per_fac, fac_first, fstride;
+  const double2* L;         // [factor][n][mb+1]
+  const double2* y;         // [rhs][n]
+  const double* fak;        // [factor][n]
   double2* kpot;            // [K][eanz][n]
+};
+
+// Forward sweep L y = b for every source of a shared factor (vre :45-56).
+struct FwdArgs {
+  int n, mb, eanz, nrhs, fac_first;
+  const double2* L;         // [factor][n][mb+1]
+  const double* fak;        // [factor][n]
+  const int* enr;
+  const uint8_t* isdir;
+  int lsink, nsink;
+  double2* y;               // [rhs][n], rhs r = factor r / eanz, source r % eanz
 };
 
 __global__ void crt_area_values_kernel(FactorArgs A, int K, double2* av);
@@ -66,6 +81,8 @@ template <int ROWS, int MAXT>
 __global__ void crt_factor_kernel(FactorArgs A);
 template <int SLOTS, int DEPTH>
 __global__ void crt_backsolve_kernel(BacksolveArgs B);
+template <int SLOTS, int DEPTH>
+__global__ void crt_fwdsweep_kernel(FwdArgs F);
 
 // LDS bytes of the factor kernel for bandwidth mb: the trailing-window rings,
 // two panels, three staged row batches and the panel's broadcast slots.
