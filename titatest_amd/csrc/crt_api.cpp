@@ -809,7 +809,7 @@ static int run_forward(crt_handle* h) {
     if (mb < 128)
       hipLaunchKernelGGL((crt_backsolve_kernel<2, 8>), dim3(nrhs), dim3(64), 0, h->stream, B);
     else
-      hipLaunchKernelGGL((crt_backsolve_kernel<4, 4>), dim3(nrhs), dim3(64), 0, h->stream, B);
+      hipLaunchKernelGGL((crt_backsolve_kernel<4, 8>), dim3(nrhs), dim3(64), 0, h->stream, B);
     tstop(h);
   }
   // electrode slice of the back-transform, used by the voltage stage

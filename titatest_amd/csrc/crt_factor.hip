@@ -559,13 +559,18 @@ template __global__ void crt_factor_kernel<2, 3>(FactorArgs);
 template __global__ void crt_factor_kernel<3, 3>(FactorArgs);
 
 // ---------------------------------------------------------------------------
-// Back substitution, one wavefront per system.  Lane j (mod 64) keeps the
-// working values of the rows j = q (mod 64*SLOTS) of the band window in SLOTS
-// registers; row i's pivot lane divides, the value is broadcast and every lane
-// adds L_ij x_i to its rows j in [i-mb, i) -- for each j the updates arrive in
-// descending i, the reference's order.  A slot freed by x_i is refilled with
-// y_{i - 64*SLOTS}, which enters the window 64*SLOTS - mb rows later.  L rows
-// are fetched DEPTH rows ahead.
+// Back substitution, one wavefront per right-hand side.  Lane j (mod 64) keeps
+// the working values of the rows j = q (mod 64*SLOTS) of the band window in
+// SLOTS registers; row i's pivot lane divides, the value is broadcast and every
+// lane adds L_ij x_i to its rows j in [i-mb, i) -- for each j the updates
+// arrive in descending i, the reference's order.  A slot freed by x_i is
+// refilled with y_{i - 64*SLOTS}, which enters the window 64*SLOTS - mb rows
+// later.  Rows of L are fetched DEPTH rows ahead; the pivot lane's own slot of
+// row i is L_ii.  The pivot lane of row i is lane i mod 64, so f_i and the
+// refill value are loaded per 64-row block, one per lane, a block ahead, and
+// kpot is stored per block, one row per lane.  The loop has no lane-divergent
+// branch: loads read clamped in-range addresses and select afterwards, so the
+// compiler can wait for each prefetched row alone.
 template <int SLOTS, int DEPTH>
 __global__ __launch_bounds__(64) void crt_backsolve_kernel(BacksolveArgs B) {
   const int lane = threadIdx.x;
@@ -581,73 +586,114 @@ __global__ __launch_bounds__(64) void crt_backsolve_kernel(BacksolveArgs B) {
   const double* fak = B.fak + (size_t)f * n;
   double2* out = B.kpot + ((size_t)k * B.eanz + l) * n;
 
-  // class q = lane + 64*sl holds row j(i, q) = i - ((i - q) mod P) at step i
   cplx ya[SLOTS];
-#pragma unroll
-  for (int sl = 0; sl < SLOTS; ++sl) {
-    const int q = lane + 64 * sl;
-    const int i = n - 1;
-    const int j = i - (((i - q) % P) + P) % P;
-    ya[sl] = (j >= 0) ? ldc(yg + j) : mkc(0.0, 0.0);
-  }
-  // rows fetched DEPTH steps ahead: this lane's L entries, and for the pivot
-  // lane L_ii, f_i and the refill value y_{i-P}
+  // row i of L in this lane's slots: L_ij of its rows j <= i in the band,
+  // loaded from clamped addresses; entries outside the band are masked where
+  // they are used (a select right after the load would wait for it)
   cplx lr[DEPTH][SLOTS];
-  cplx lii[DEPTH], ynx[DEPTH];
-  double fr[DEPTH];
-  auto fetch = [&](int i, cplx* dst, cplx& dii, cplx& dyn, double& fdst) {
+  auto fetch = [&](int i, cplx* dst) __attribute__((always_inline)) {
+    const int ic = min(max(i, 0), n - 1);
 #pragma unroll
     for (int sl = 0; sl < SLOTS; ++sl) {
       const int q = lane + 64 * sl;
       const int j = i - (((i - q) % P) + P) % P;
       const int t = j - i + mb;
-      dst[sl] = (i >= 0 && j >= 0 && t >= 0 && j < i) ? ldc(Lg + (size_t)i * m1 + t) : mkc(0.0, 0.0);
+      dst[sl] = ldc(Lg + (size_t)ic * m1 + min(max(t, 0), mb));
     }
-    const bool piv = i >= 0 && lane == (i % P) % 64;
-    dii = piv ? ldc(Lg + (size_t)i * m1 + mb) : mkc(1.0, 0.0);
-    fdst = piv ? fak[i] : 0.0;
-    dyn = (piv && i - P >= 0) ? ldc(yg + (i - P)) : mkc(0.0, 0.0);
   };
+  // f_i and y_{i-P} of the 64-row block b, lane = row mod 64
+  // (raw loads, masked when the block becomes current)
+  auto load_blk = [&](int b, double& fb, cplx& yb) __attribute__((always_inline)) {
+    const int i = 64 * b + lane;
+    const int ic = min(max(i, 0), n - 1), iy = min(max(i - P, 0), n - 1);
+    fb = fak[ic];
+    yb = ldc(yg + iy);
+  };
+  // (the refill row i-P is a real row also when i is a padding row above n)
+  auto mask_blk = [&](int b, double& fb, cplx& yb) __attribute__((always_inline)) {
+    const int i = 64 * b + lane;
+    fb = (b >= 0 && i < n) ? fb : 0.0;
+    yb = csel(b >= 0 && i - P >= 0 && i - P < n, yb, mkc(0.0, 0.0));
+  };
+  // Rows are taken in 64-row blocks from the top; the top block is padded
+  // above n with rows that change nothing (unit pivot, x = 0).
+  const int btop = (n - 1) >> 6;
+  double fcur, fnxt;
+  cplx ycur, ynxt, xo = mkc(0.0, 0.0);
+  load_blk(btop, fcur, ycur);
+  mask_blk(btop, fcur, ycur);
+  load_blk(btop - 1, fnxt, ynxt);
+  const int itop = 64 * btop + 63;
+  // class q = lane + 64*sl holds row j(i, q) = i - ((i - q) mod P) at step i
 #pragma unroll
-  for (int d = 0; d < DEPTH; ++d) fetch(n - 1 - d, lr[d], lii[d], ynx[d], fr[d]);
+  for (int sl = 0; sl < SLOTS; ++sl) {
+    const int q = lane + 64 * sl;
+    const int j = itop - (((itop - q) % P) + P) % P;
+    ya[sl] = csel(j >= 0 && j < n, ldc(yg + min(max(j, 0), n - 1)), mkc(0.0, 0.0));
+  }
+#pragma unroll
+  for (int d = 0; d < DEPTH; ++d) fetch(itop - d, lr[d]);
 
-  for (int ib = n - 1; ib >= 0; ib -= DEPTH) {
+  auto step = [&](int i, int d) __attribute__((always_inline)) {
+    const int qp = i % P;
+    const int plane = qp % 64, pslot = qp / 64;
+    cplx lv[SLOTS];
 #pragma unroll
-    for (int d = 0; d < DEPTH; ++d) {
-      const int i = ib - d;
-      if (i >= 0) {
-        const int qp = i % P;
-        const int plane = qp % 64, pslot = qp / 64;
-        cplx x = mkc(0.0, 0.0);
-        if (lane == plane) {
-          cplx yi = ya[0];
-#pragma unroll
-          for (int sl = 1; sl < SLOTS; ++sl)
-            if (sl == pslot) yi = ya[sl];
-          const cplx dq = cdiv(yi, lii[d]);
-          x = mkc(-dq.re, -dq.im);
-          out[i] = make_double2(fr[d] * x.re, fr[d] * x.im);
-          // the freed slot takes y_{i-P}, first updated by row i-P+mb
-#pragma unroll
-          for (int sl = 0; sl < SLOTS; ++sl)
-            if (sl == pslot) ya[sl] = ynx[d];
-        }
-        x.re = __shfl(x.re, plane, 64);
-        x.im = __shfl(x.im, plane, 64);
-#pragma unroll
-        for (int sl = 0; sl < SLOTS; ++sl) {
-          const int q = lane + 64 * sl;
-          const int j = i - (((i - q) % P) + P) % P;
-          if (j < i && i - j <= mb && j >= 0) ya[sl] = cmadd(ya[sl], lr[d][sl], x);
-        }
-        fetch(i - DEPTH, lr[d], lii[d], ynx[d], fr[d]);
-      }
+    for (int sl = 0; sl < SLOTS; ++sl) {
+      const int q = lane + 64 * sl;
+      const int j = i - (((i - q) % P) + P) % P;
+      lv[sl] = csel(i < n && j >= 0 && j - i + mb >= 0, lr[d][sl], mkc(0.0, 0.0));
     }
+    cplx yi = ya[0], lii = lv[0];
+#pragma unroll
+    for (int sl = 1; sl < SLOTS; ++sl) {
+      yi = csel(sl == pslot, ya[sl], yi);
+      lii = csel(sl == pslot, lv[sl], lii);
+    }
+    const bool real_row = i < n;
+    lii = csel(real_row, lii, mkc(1.0, 0.0));
+    const cplx dq = cdiv(yi, lii);  // meaningful in the pivot lane
+    cplx x = csel(real_row, mkc(-dq.re, -dq.im), mkc(0.0, 0.0));
+    xo = csel(lane == plane, x, xo);
+    x.re = __shfl(x.re, plane, 64);
+    x.im = __shfl(x.im, plane, 64);
+#pragma unroll
+    for (int sl = 0; sl < SLOTS; ++sl) {
+      const int q = lane + 64 * sl;
+      const int j = i - (((i - q) % P) + P) % P;
+      // rows j < i take L_ij x_i; the pivot's own slot takes y_{i-P}
+      ya[sl] = csel(j < i, cmadd(ya[sl], lv[sl], x), csel(lane == plane && sl == pslot, ycur, ya[sl]));
+    }
+    fetch(i - DEPTH, lr[d]);
+    // the fetch stays here, DEPTH rows ahead of its use (the scheduler would
+    // otherwise sink it next to the use)
+    __builtin_amdgcn_sched_barrier(0);
+  };
+  auto block = [&](int b) __attribute__((always_inline)) {
+#pragma unroll 1
+    for (int c = 0; c < 64; c += DEPTH) {
+#pragma unroll
+      for (int d = 0; d < DEPTH; ++d) step(64 * b + 63 - c - d, d);
+    }
+  };
+  // top block (its store is the only one that can pass n)
+  block(btop);
+  {
+    const int io = 64 * btop + lane;
+    if (io < n) out[io] = make_double2(fcur * xo.re, fcur * xo.im);
+  }
+  for (int b = btop - 1; b >= 0; --b) {
+    fcur = fnxt;
+    ycur = ynxt;
+    mask_blk(b, fcur, ycur);
+    load_blk(b - 1, fnxt, ynxt);
+    block(b);
+    out[64 * b + lane] = make_double2(fcur * xo.re, fcur * xo.im);
   }
 }
 
 template __global__ void crt_backsolve_kernel<2, 8>(BacksolveArgs);
-template __global__ void crt_backsolve_kernel<4, 4>(BacksolveArgs);
+template __global__ void crt_backsolve_kernel<4, 8>(BacksolveArgs);
 
 // ---------------------------------------------------------------------------
 // Forward sweep L y = b, one wavefront per right-hand side, for sources that
