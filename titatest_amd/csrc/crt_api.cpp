@@ -662,7 +662,8 @@ int crt_get_precal(crt_handle* h, double* elbg, double* relbg, double* kg) {
 static int run_forward(crt_handle* h) {
   const int n = h->n, mb = h->mb, E = h->E, K = h->K;
   const size_t lds = crt_factor_lds_bytes(mb);
-  if (lds > h->lds_max || mb > 124) return kErrBand;
+  // the substitutions address one factor with 32-bit byte offsets
+  if (lds > h->lds_max || mb > 124 || (size_t)n * (mb + 1) * 16 >= (size_t(1) << 31)) return kErrBand;
   // wave 0 holds the W = mb + kNB panel rows, ROWS per lane; waves 1-3 own the
   // trailing window's 4x4 tiles beyond the look-ahead columns, MAXT each
   const int W = mb + kNB, tw = (mb + 3) / 4;
@@ -806,10 +807,10 @@ static int run_forward(crt_handle* h) {
     B.nrhs = nrhs;
     B.fac_first = first;
     tstart(h, "backsolve");
-    if (mb < 128)
-      hipLaunchKernelGGL((crt_backsolve_kernel<2, 8>), dim3(nrhs), dim3(64), 0, h->stream, B);
+    if (mb <= 120)
+      hipLaunchKernelGGL((crt_backsolve_kernel<8, 8>), dim3(nrhs), dim3(64), 0, h->stream, B);
     else
-      hipLaunchKernelGGL((crt_backsolve_kernel<4, 8>), dim3(nrhs), dim3(64), 0, h->stream, B);
+      hipLaunchKernelGGL((crt_backsolve_kernel<8, 4>), dim3(nrhs), dim3(64), 0, h->stream, B);
     tstop(h);
   }
   // electrode slice of the back-transform, used by the voltage stage

@@ -108,6 +108,22 @@ __device__ __forceinline__ void lds_barrier() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
+// Raw buffer access (no stride, no swizzle; dword 3 of the descriptor is the
+// gfx9 data-format word).  A 16-byte load at a byte offset at or past the
+// buffer's size returns zero -- a range check done by the memory pipeline, so
+// a masked element costs one select of its offset instead of one per value.
+typedef __amdgpu_buffer_rsrc_t brsrc;
+typedef double dbl2v __attribute__((ext_vector_type(2)));
+typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
+constexpr unsigned kBufOut = 0xFFFFFFF0u;  // an offset past every buffer made here
+__device__ __forceinline__ brsrc make_rsrc(const void* p, unsigned bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, (int)bytes, 0x00020000);
+}
+__device__ __forceinline__ cplx bldc(brsrc r, unsigned off) {
+  const dbl2v v = __builtin_bit_cast(dbl2v, __builtin_amdgcn_raw_buffer_load_b128(r, (int)off, 0, 0));
+  return mkc(v.x, v.y);
+}
+
 __device__ __forceinline__ cplx ldc(const double2* p) {
   double2 v = *p;
   return mkc(v.x, v.y);

@@ -27,6 +27,8 @@
 //
 // crt_backsolve_kernel: x = -L^{-T} y (vre, src/vre.f90:58-69) and
 //   kpot = x * f (src/fem.f90:385), one wavefront per system.
+#include <type_traits>
+
 #include "crt_device.h"
 #include "crt_kernels.h"
 
@@ -559,20 +561,27 @@ template __global__ void crt_factor_kernel<2, 3>(FactorArgs);
 template __global__ void crt_factor_kernel<3, 3>(FactorArgs);
 
 // ---------------------------------------------------------------------------
-// Back substitution, one wavefront per right-hand side.  Lane j (mod 64) keeps
-// the working values of the rows j = q (mod 64*SLOTS) of the band window in
-// SLOTS registers; row i's pivot lane divides, the value is broadcast and every
-// lane adds L_ij x_i to its rows j in [i-mb, i) -- for each j the updates
-// arrive in descending i, the reference's order.  A slot freed by x_i is
-// refilled with y_{i - 64*SLOTS}, which enters the window 64*SLOTS - mb rows
-// later.  Rows of L are fetched DEPTH rows ahead; the pivot lane's own slot of
-// row i is L_ii.  The pivot lane of row i is lane i mod 64, so f_i and the
-// refill value are loaded per 64-row block, one per lane, a block ahead, and
-// kpot is stored per block, one row per lane.  The loop has no lane-divergent
-// branch: loads read clamped in-range addresses and select afterwards, so the
-// compiler can wait for each prefetched row alone.
-template <int SLOTS, int DEPTH>
+// Back substitution, one wavefront per right-hand side.  Lane q (q < 64) keeps
+// the working values of the rows j = q and j = q + 64 (mod 128) of the band
+// window in two slots (128 > mb): slot value y_j + sum_{i>j} L_ij x_i.  Row
+// i's pivot lane forms x_i = -(its value) / L_ii, the value is broadcast and
+// every lane adds L_ij x_i to its rows j in [i-mb, i) -- for each j the
+// updates arrive in descending i, the reference's order.  Rows are taken in
+// 64-row blocks.  Slot A of a lane holds the class of the block's own rows
+// (lane + 64 (b & 1)), slot B the other; the two swap between blocks.  Row i's
+// pivot is slot A of lane i mod 64, which holds -1/L_ii of its block row
+// (formed once per block from a block-wide load, with f_i and the refill
+// values y_{i-128}).  The pivot lane's slot A takes no update at its own step
+// (the load of L_ii is masked), so it keeps row i's final value; every G rows
+// the group's lanes keep x (kpot is stored per block, one row per lane) and
+// refill slot A with y_{i-128}, which row i-128+mb first updates -- G < 129 -
+// mb keeps the refill ahead of it.  Rows of L are fetched DEPTH rows ahead
+// with buffer loads whose offsets for out-of-band, above-n and below-0 entries
+// lie past the end of L, so they read zero: no lane-divergent branch and no
+// per-value mask in the loop.
+template <int DEPTH, int G>
 __global__ __launch_bounds__(64) void crt_backsolve_kernel(BacksolveArgs B) {
+  static_assert(DEPTH % G == 0 && 64 % DEPTH == 0, "row groups tile the block");
   const int lane = threadIdx.x;
   const int r = blockIdx.x;
   if (r >= B.nrhs) return;
@@ -580,120 +589,109 @@ __global__ __launch_bounds__(64) void crt_backsolve_kernel(BacksolveArgs B) {
   const int sys = (B.fac_first + f) * B.fstride + r % B.rhs_per_fac;
   const int k = sys / B.eanz, l = sys % B.eanz;
   const int n = B.n, mb = B.mb, m1 = mb + 1;
-  constexpr int P = 64 * SLOTS;
-  const double2* Lg = B.L + (size_t)f * n * m1;
+  // n m1 16 < 2^31 (checked by the host): offsets of rows in [0, n + 64) fit,
+  // offsets of rows below 0 wrap past the end
+  const brsrc Lr = make_rsrc(B.L + (size_t)f * n * m1, (unsigned)(n * m1 * 16));
   const double2* yg = B.y + (size_t)r * n;
   const double* fak = B.fak + (size_t)f * n;
   double2* out = B.kpot + ((size_t)k * B.eanz + l) * n;
 
-  cplx ya[SLOTS];
-  // row i of L in this lane's slots: L_ij of its rows j <= i in the band,
-  // loaded from clamped addresses; entries outside the band are masked where
-  // they are used (a select right after the load would wait for it)
-  cplx lr[DEPTH][SLOTS];
-  auto fetch = [&](int i, cplx* dst) __attribute__((always_inline)) {
-    const int ic = min(max(i, 0), n - 1);
-#pragma unroll
-    for (int sl = 0; sl < SLOTS; ++sl) {
-      const int q = lane + 64 * sl;
-      const int j = i - (((i - q) % P) + P) % P;
-      const int t = j - i + mb;
-      dst[sl] = ldc(Lg + (size_t)ic * m1 + min(max(t, 0), mb));
-    }
+  // Entries of row t of L for both slots: the slot's row is j = t - rf,
+  // rf = (t - q) mod 128, at band offset mb - rf, in range for 1 <= rf <=
+  // min(mb, t).  For slot A (q = lane + (t & 64)) rf = (t mod 64 - lane) mod
+  // 128; slot B has rf ^ 64.  rf in 16-byte units.
+  const int m16 = m1 * 16;
+  auto fetch = [&](int t, cplx* dst) __attribute__((always_inline)) {
+    const int base = t * m16 + mb * 16;
+    const unsigned lim = (unsigned)max(min(mb, t), 0) * 16u;
+    const int ra = (((t & 63) - lane) * 16) & 0x7F0;
+    const int rb = ra ^ 0x400;
+    dst[0] = bldc(Lr, (unsigned)(ra - 16) < lim ? (unsigned)(base - ra) : kBufOut);
+    dst[1] = bldc(Lr, (unsigned)(rb - 16) < lim ? (unsigned)(base - rb) : kBufOut);
   };
-  // f_i and y_{i-P} of the 64-row block b, lane = row mod 64
-  // (raw loads, masked when the block becomes current)
-  auto load_blk = [&](int b, double& fb, cplx& yb) __attribute__((always_inline)) {
+  // f_i, L_ii and y_{i-128} of block b's row i = 64 b + lane (raw loads;
+  // masked when the block becomes current)
+  auto load_blk = [&](int b, double& fb, cplx& db, cplx& yb) __attribute__((always_inline)) {
     const int i = 64 * b + lane;
-    const int ic = min(max(i, 0), n - 1), iy = min(max(i - P, 0), n - 1);
-    fb = fak[ic];
-    yb = ldc(yg + iy);
+    fb = fak[min(max(i, 0), n - 1)];
+    db = bldc(Lr, (unsigned)(i * m16 + mb * 16));
+    yb = ldc(yg + min(max(i - 128, 0), n - 1));
   };
-  // (the refill row i-P is a real row also when i is a padding row above n)
-  auto mask_blk = [&](int b, double& fb, cplx& yb) __attribute__((always_inline)) {
-    const int i = 64 * b + lane;
-    fb = (b >= 0 && i < n) ? fb : 0.0;
-    yb = csel(b >= 0 && i - P >= 0 && i - P < n, yb, mkc(0.0, 0.0));
-  };
+  cplx ya, yb;  // slots A and B
+  cplx lr[DEPTH][2];
   // Rows are taken in 64-row blocks from the top; the top block is padded
-  // above n with rows that change nothing (unit pivot, x = 0).
+  // above n with rows that change nothing (x = 0: their -1/L_ii is zero).
   const int btop = (n - 1) >> 6;
   double fcur, fnxt;
-  cplx ycur, ynxt, xo = mkc(0.0, 0.0);
-  load_blk(btop, fcur, ycur);
-  mask_blk(btop, fcur, ycur);
-  load_blk(btop - 1, fnxt, ynxt);
-  const int itop = 64 * btop + 63;
-  // class q = lane + 64*sl holds row j(i, q) = i - ((i - q) mod P) at step i
-#pragma unroll
-  for (int sl = 0; sl < SLOTS; ++sl) {
-    const int q = lane + 64 * sl;
-    const int j = itop - (((itop - q) % P) + P) % P;
-    ya[sl] = csel(j >= 0 && j < n, ldc(yg + min(max(j, 0), n - 1)), mkc(0.0, 0.0));
-  }
-#pragma unroll
-  for (int d = 0; d < DEPTH; ++d) fetch(itop - d, lr[d]);
-
-  auto step = [&](int i, int d) __attribute__((always_inline)) {
-    const int qp = i % P;
-    const int plane = qp % 64, pslot = qp / 64;
-    cplx lv[SLOTS];
-#pragma unroll
-    for (int sl = 0; sl < SLOTS; ++sl) {
-      const int q = lane + 64 * sl;
-      const int j = i - (((i - q) % P) + P) % P;
-      lv[sl] = csel(i < n && j >= 0 && j - i + mb >= 0, lr[d][sl], mkc(0.0, 0.0));
-    }
-    cplx yi = ya[0], lii = lv[0];
-#pragma unroll
-    for (int sl = 1; sl < SLOTS; ++sl) {
-      yi = csel(sl == pslot, ya[sl], yi);
-      lii = csel(sl == pslot, lv[sl], lii);
-    }
-    const bool real_row = i < n;
-    lii = csel(real_row, lii, mkc(1.0, 0.0));
-    const cplx dq = cdiv(yi, lii);  // meaningful in the pivot lane
-    cplx x = csel(real_row, mkc(-dq.re, -dq.im), mkc(0.0, 0.0));
-    xo = csel(lane == plane, x, xo);
-    x.re = __shfl(x.re, plane, 64);
-    x.im = __shfl(x.im, plane, 64);
-#pragma unroll
-    for (int sl = 0; sl < SLOTS; ++sl) {
-      const int q = lane + 64 * sl;
-      const int j = i - (((i - q) % P) + P) % P;
-      // rows j < i take L_ij x_i; the pivot's own slot takes y_{i-P}
-      ya[sl] = csel(j < i, cmadd(ya[sl], lv[sl], x), csel(lane == plane && sl == pslot, ycur, ya[sl]));
-    }
-    fetch(i - DEPTH, lr[d]);
-    // the fetch stays here, DEPTH rows ahead of its use (the scheduler would
-    // otherwise sink it next to the use)
-    __builtin_amdgcn_sched_barrier(0);
+  cplx dcur, dnxt, ycur, ynxt, nl, xo = mkc(0.0, 0.0);
+  auto enter_blk = [&](int b) __attribute__((always_inline)) {
+    const int i = 64 * b + lane;
+    fcur = i < n ? fcur : 0.0;
+    ycur = csel(i - 128 >= 0 && i - 128 < n, ycur, mkc(0.0, 0.0));
+    const cplx rc = crecip(dcur);
+    nl = csel(i < n, mkc(-rc.re, -rc.im), mkc(0.0, 0.0));
   };
-  auto block = [&](int b) __attribute__((always_inline)) {
+  load_blk(btop, fcur, dcur, ycur);
+  load_blk(btop - 1, fnxt, dnxt, ynxt);
+  const int itop = 64 * btop + 63;
+  {
+    // slot A's class at the top block is lane + 64 (btop & 1)
+    const int qa = lane + (itop & 64), qb = qa ^ 64;
+    const int ja = itop - (((itop - qa) % 128) + 128) % 128;
+    const int jb = itop - (((itop - qb) % 128) + 128) % 128;
+    ya = csel(ja >= 0 && ja < n, ldc(yg + min(max(ja, 0), n - 1)), mkc(0.0, 0.0));
+    yb = csel(jb >= 0 && jb < n, ldc(yg + min(max(jb, 0), n - 1)), mkc(0.0, 0.0));
+  }
+  // in row order, as the loop refills them: the loop's wait for row i's
+  // entries then counts the same loads behind them on entry as on the back edge
+#pragma unroll
+  for (int d = 0; d < DEPTH; ++d) {
+    fetch(itop - d, lr[d]);
+    __builtin_amdgcn_sched_barrier(0);
+  }
+  enter_blk(btop);
+
+  for (int b = btop; b >= 0; --b) {
+    if (b < btop) {
+      fcur = fnxt;
+      dcur = dnxt;
+      ycur = ynxt;
+      enter_blk(b);
+      load_blk(b - 1, fnxt, dnxt, ynxt);
+    }
 #pragma unroll 1
     for (int c = 0; c < 64; c += DEPTH) {
 #pragma unroll
-      for (int d = 0; d < DEPTH; ++d) step(64 * b + 63 - c - d, d);
+      for (int d = 0; d < DEPTH; ++d) {
+        const int i = 64 * b + 63 - c - d;
+        const cplx xb = rlc(cmul(ya, nl), i & 63);
+        ya = cmadd(ya, lr[d][0], xb);
+        yb = cmadd(yb, lr[d][1], xb);
+        // the updates are formed before lr[d] is refilled (otherwise they
+        // may sink below the refill, which then needs a copy and a wait)
+        asm volatile("" : "+v"(ya.re), "+v"(ya.im), "+v"(yb.re), "+v"(yb.im));
+        fetch(i - DEPTH, lr[d]);
+        // the fetch stays here, DEPTH rows ahead of its use (the scheduler
+        // would otherwise sink it next to the use)
+        __builtin_amdgcn_sched_barrier(0);
+        if ((d + 1) % G == 0) {
+          // rows i+G-1 .. i done: their lanes keep x and refill slot A
+          const bool g = (unsigned)(lane - (i & 63)) < (unsigned)G;
+          xo = csel(g, cmul(ya, nl), xo);
+          ya = csel(g, ycur, ya);
+        }
+      }
     }
-  };
-  // top block (its store is the only one that can pass n)
-  block(btop);
-  {
-    const int io = 64 * btop + lane;
+    const int io = 64 * b + lane;
     if (io < n) out[io] = make_double2(fcur * xo.re, fcur * xo.im);
-  }
-  for (int b = btop - 1; b >= 0; --b) {
-    fcur = fnxt;
-    ycur = ynxt;
-    mask_blk(b, fcur, ycur);
-    load_blk(b - 1, fnxt, ynxt);
-    block(b);
-    out[64 * b + lane] = make_double2(fcur * xo.re, fcur * xo.im);
+    const cplx t = ya;
+    ya = yb;
+    yb = t;
   }
 }
 
-template __global__ void crt_backsolve_kernel<2, 8>(BacksolveArgs);
-template __global__ void crt_backsolve_kernel<4, 8>(BacksolveArgs);
+template __global__ void crt_backsolve_kernel<8, 8>(BacksolveArgs);
+template __global__ void crt_backsolve_kernel<8, 4>(BacksolveArgs);
 
 // ---------------------------------------------------------------------------
 // Forward sweep L y = b, one wavefront per right-hand side, for sources that

@@ -79,7 +79,7 @@ __global__ void crt_scale_kernel(FactorArgs A);
 __global__ void crt_rows_kernel(FactorArgs A);
 template <int ROWS, int MAXT>
 __global__ void crt_factor_kernel(FactorArgs A);
-template <int SLOTS, int DEPTH>
+template <int DEPTH, int G>
 __global__ void crt_backsolve_kernel(BacksolveArgs B);
 template <int SLOTS, int DEPTH>
 __global__ void crt_fwdsweep_kernel(FwdArgs F);
