@@ -65,6 +65,14 @@ def lib():
         L.orc_bsmatmreg.restype = None
         L.orc_rm_reg.argtypes = [C.c_int, C.c_int, _d, _z, _z]
         L.orc_rm_reg.restype = None
+        L.orc_assemble_dc.argtypes = [P, _d, _d, _d, C.c_int, _z, C.c_int, C.c_int, _d, _d, _d]
+        L.orc_choldc.argtypes = [C.c_int, C.c_int, _d]
+        L.orc_vredc.argtypes = [C.c_int, C.c_int, _d, _d, _d]
+        L.orc_vredc.restype = None
+        L.orc_forward_dc.argtypes = [P, C.c_int, _d, C.c_int, _d, _d, _d, C.c_int, C.c_int, _z,
+                                     _d, _d, C.c_int]
+        L.orc_sens_dc.argtypes = [P, C.c_int, _i, _i, C.c_int, _d, C.c_int, _d, _d, _z, _z, _d,
+                                  C.c_int]
         L.orc_adjoint_grad.argtypes = [C.c_int, C.c_int, _z, _d, _i, _z, _z]
         L.orc_adjoint_grad.restype = None
         _lib = L
@@ -192,6 +200,61 @@ class Oracle:
         self.hpot = hpot.reshape((n, E), order="F")
         self.sigma = np.ascontiguousarray(sigma, np.complex128)
         return self.kpot, self.hpot
+
+    # --- A19: DC twins --------------------------------------------------------
+    def assemble_dc(self, sigma, l, k):
+        """kompadc + randdc + scaldc: the scaled real band matrix, rhs and fak."""
+        n, mb = self.mesh.sanz, self.mesh.mb
+        a = np.zeros((mb + 1) * n)
+        b = np.zeros(n)
+        fak = np.zeros(n)
+        e = lib().orc_assemble_dc(C.byref(self.cm), self.elbg, self.relbg, self.kg, self.lrandb,
+                                  np.ascontiguousarray(sigma, np.complex128), l, k, a, b, fak)
+        if e:
+            raise OracleError("assemble_dc", e)
+        return a, b, fak
+
+    def choldc(self, a):
+        a = a.copy()
+        e = lib().orc_choldc(self.mesh.sanz, self.mesh.mb, a)
+        if e:
+            raise OracleError("choldc", e)
+        return a
+
+    def vredc(self, a, b):
+        pot = np.zeros(self.mesh.sanz)
+        lib().orc_vredc(self.mesh.sanz, self.mesh.mb, a, np.ascontiguousarray(b, np.float64), pot)
+        return pot
+
+    def forward_dc(self, sigma, nthreads=0):
+        """kpotdc (sanz, eanz, K) and hpotdc (sanz, eanz), real; the handle's
+        hpot/kpot become their complex copies (dcmplx), as bvolti/bsendc read
+        them in the DC case."""
+        n, E, K = self.mesh.sanz, self.mesh.eanz, self.K
+        kp = np.zeros(n * E * K)
+        hp = np.zeros(n * E)
+        e = lib().orc_forward_dc(C.byref(self.cm), K, self.kwnwi, self.swrtr, self.elbg, self.relbg,
+                                 self.kg, self.lbeta, self.lrandb,
+                                 np.ascontiguousarray(sigma, np.complex128), kp, hp, nthreads)
+        if e:
+            raise OracleError("forward_dc", e)
+        self.kpotdc = kp.reshape((n, E, K), order="F")
+        self.hpotdc = hp.reshape((n, E), order="F")
+        self.kpot = self.kpotdc.astype(np.complex128)
+        self.hpot = self.hpotdc.astype(np.complex128)
+        self.sigma = np.ascontiguousarray(sigma, np.complex128)
+        return self.kpotdc, self.hpotdc
+
+    def sens_dc(self, nthreads=0):
+        """bsendc: real sensdc (nanz, elanz)."""
+        N = len(self.strnr)
+        out = np.zeros(N * self.elanz)
+        kp = np.asfortranarray(self.kpotdc).ravel(order="F")
+        e = lib().orc_sens_dc(C.byref(self.cm), N, self.strnr, self.vnr, self.K, self.kwnwi,
+                              self.swrtr, self.elbg, kp, self.sigma, self.voltv, out, nthreads)
+        if e:
+            raise OracleError("sens_dc", e)
+        return out.reshape((N, self.elanz), order="F")
 
     # --- A12 ----------------------------------------------------------------
     def volt(self, strnr, vnr, mode=0):

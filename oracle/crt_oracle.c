@@ -859,6 +859,269 @@ int orc_sens(const orc_mesh *m, int nanz, const int *strnr, const int *vnr, int 
   return 0;
 }
 
+/* ---------------------------------------------------------------- DC twins
+ * The real (ldc) variants of the forward path: kompadc
+ * (src/kompadc.f90:66-160), randdc (src/randdc.f90, the randb loop on real
+ * arrays), scaldc (src/scaldc.f90:49-73), choldc (src/choldc.f90:44-77),
+ * vredc (src/vredc.f90:45-77), kompbdc (src/kompbdc.f90:39-59), kpotdc and
+ * hpotdc (src/inv.f90:393-399), rtrafo's DC branch (src/rtrafo.f90:48-66) and
+ * bsendc (src/bsendc.f90:74-184).  Real arithmetic throughout; sigma enters
+ * through its real part (DBLE(sigma)). */
+static int kompadc(const orc_mesh *m, const double *elbg, const double *relbg, const double *kg,
+                   const orc_c *sigma, int l, int k, double *a, double *b) {
+  int elanz = m_elanz(m), relanz = m_relanz(m), nel = elanz + relanz;
+  int maxn = m_maxnodes(m), nimn = maxn * (maxn + 1) / 2, mb = m->mb;
+  memset(a, 0, sizeof(double) * (size_t)(mb + 1) * m->sanz);
+  memset(b, 0, sizeof(double) * (size_t)m->sanz);
+  int iel = 0;
+  for (int i = 0; i < m->typanz; i++) {
+    int ntyp = m->typ[i], nkel = m->selanz[i];
+    for (int j = 0; j < m->nelanz[i]; j++, iel++) {
+      if (ntyp > 11) continue;
+      int ikl = 0;
+      for (int kk = 0; kk < nkel; kk++) {
+        int nzp = NREL(m, nel, iel, kk);
+        for (int ll = 0; ll <= kk; ll++) {
+          int nnp = NREL(m, nel, iel, ll);
+          if (abs(nzp - nnp) > mb) return 19;
+          int imax = nzp > nnp ? nzp : nnp, imin = nzp < nnp ? nzp : nnp;
+          size_t im = (size_t)imax * mb + imin; /* 1-based */
+          double dum, dum2;
+          if (ntyp == 11) {
+            int rel = iel - elanz;
+            dum = relbg[rel + (size_t)relanz * ikl] * kg[rel + (size_t)relanz * (l + (size_t)m->eanz * k)];
+            dum2 = sigma[m->rnr[rel] - 1].re;
+          } else {
+            dum = elbg[iel + (size_t)elanz * (ikl + (size_t)nimn * k)];
+            dum2 = sigma[iel].re;
+          }
+          ikl++;
+          a[im - 1] += dum * dum2;
+        }
+      }
+    }
+  }
+  b[m->enr[l] - 1] = -1.0;
+  if (m->lsink) b[m->nsink - 1] = 1.0;
+  return 0;
+}
+
+static void randdc(const orc_mesh *m, double *a, double *b) {
+  int elanz = m_elanz(m), relanz = m_relanz(m), nel = elanz + relanz;
+  int mb = m->mb, m1 = mb + 1, sanz = m->sanz;
+  int iel = 0;
+  for (int i2 = 0; i2 < m->typanz; i2++) {
+    int ntyp = m->typ[i2], nkel = m->selanz[i2];
+    for (int j2 = 0; j2 < m->nelanz[i2]; j2++, iel++) {
+      if (ntyp != 13) continue;
+      for (int ir = 0; ir < nkel; ir++) {
+        int k = NREL(m, nel, iel, ir);
+        b[k - 1] = 0.0;
+        int idk = k * m1;
+        a[idk - 1] = 1.0;
+        if (k != 1) {
+          int ia = (1 > mb + 2 - k) ? 1 : mb + 2 - k;
+          for (int i = ia; i <= mb; i++) a[idk + i - m1 - 1] = 0.0;
+        }
+        if (k == sanz) continue;
+        int ia = (1 > k - sanz + m1) ? 1 : k - sanz + m1;
+        for (int i = ia; i <= mb; i++) {
+          int j = k - i + m1;
+          a[(j - 1) * m1 + i - 1] = 0.0;
+        }
+      }
+    }
+  }
+}
+
+static int scaldc(int sanz, int mb, double *a, double *b, double *fak) {
+  for (int i = 1; i <= sanz; i++) {
+    int idi = i * (mb + 1);
+    double dum = a[idi - 1];
+    if (dum <= 0.0) return 27;
+    a[idi - 1] = 1.0;
+    fak[i - 1] = 1.0 / sqrt(dum);
+    b[i - 1] = b[i - 1] * fak[i - 1];
+    if (i == 1) continue;
+    int i0 = i * mb;
+    int ja = (1 > i - mb) ? 1 : i - mb;
+    for (int j = ja; j <= i - 1; j++) a[i0 + j - 1] = a[i0 + j - 1] * fak[i - 1] * fak[j - 1];
+  }
+  return 0;
+}
+
+int orc_choldc(int sanz, int mb, double *a) {
+  int m1 = mb + 1;
+  for (int i = 1; i <= sanz; i++) {
+    int idi = i * m1;
+    int fi = (1 > i - mb) ? 1 : i - mb;
+    int i0 = idi - i;
+    for (int j = fi; j <= i; j++) {
+      int ij = i0 + j, j0 = j * mb;
+      double s = a[ij - 1];
+      for (int k = fi; k <= j - 1; k++) s = s - a[i0 + k - 1] * a[j0 + k - 1];
+      if (j < i) {
+        a[ij - 1] = s / a[j * m1 - 1];
+      } else {
+        if (s <= 0.0) return 28;
+        a[idi - 1] = sqrt(s);
+      }
+    }
+  }
+  return 0;
+}
+
+void orc_vredc(int sanz, int mb, const double *a, const double *b, double *pot) {
+  int m1 = mb + 1, idi = 0;
+  for (int i = 1; i <= sanz; i++) {
+    idi = i * m1;
+    double s = b[i - 1];
+    int i0 = idi - i;
+    int jlow = (1 > i - mb) ? 1 : i - mb;
+    for (int j = jlow; j <= i - 1; j++) s = s - a[i0 + j - 1] * pot[j - 1];
+    pot[i - 1] = s / a[idi - 1];
+  }
+  for (int i = sanz; i >= 1; i--) {
+    pot[i - 1] = -pot[i - 1] / a[idi - 1];
+    int jlow = (1 > i - mb) ? 1 : i - mb;
+    int i0 = idi - i;
+    for (int j = jlow; j <= i - 1; j++) pot[j - 1] = pot[j - 1] + a[i0 + j - 1] * pot[i - 1];
+    idi = idi - m1;
+  }
+}
+
+int orc_assemble_dc(const orc_mesh *m, const double *elbg, const double *relbg, const double *kg,
+                    int lrandb, const orc_c *sigma, int l, int k, double *a, double *b, double *fak) {
+  int e = kompadc(m, elbg, relbg, kg, sigma, l, k, a, b);
+  if (e) return e;
+  if (lrandb) randdc(m, a, b);
+  return scaldc(m->sanz, m->mb, a, b, fak);
+}
+
+/* The DC loop of src/inv.f90:343-400 (kpotdc = pot*fak) and rtrafo's DC
+ * branch (hpotdc = sum_k kpotdc*kwnwi / pi, src/rtrafo.f90:54-62; a copy of
+ * k = 1 for swrtr = 0).  kpotdc: sanz x eanz x K, hpotdc: sanz x eanz. */
+int orc_forward_dc(const orc_mesh *m, int K, const double *kwnwi, int swrtr, const double *elbg,
+                   const double *relbg, const double *kg, int lbeta, int lrandb, const orc_c *sigma,
+                   double *kpotdc, double *hpotdc, int nthreads) {
+  int sanz = m->sanz, mb = m->mb, E = m->eanz;
+  int err = 0;
+#ifdef _OPENMP
+  if (nthreads > 0) omp_set_num_threads(nthreads);
+#else
+  (void)nthreads;
+#endif
+#pragma omp parallel
+  {
+    double *a = malloc(sizeof(double) * (size_t)(mb + 1) * sanz);
+    double *b = malloc(sizeof(double) * (size_t)sanz);
+    double *pot = malloc(sizeof(double) * (size_t)sanz);
+    double *fak = malloc(sizeof(double) * (size_t)sanz);
+#pragma omp for schedule(dynamic, 1)
+    for (int k = 0; k < K; k++) {
+      int failed = 0;
+      for (int l = 0; l < E && !failed; l++) {
+        if (lbeta || l == 0) {
+          int e = orc_assemble_dc(m, elbg, relbg, kg, lrandb, sigma, l, k, a, b, fak);
+          if (!e) e = orc_choldc(sanz, mb, a);
+          if (e) {
+#pragma omp atomic write
+            err = e;
+            failed = 1;
+            continue;
+          }
+        } else {
+          memset(b, 0, sizeof(double) * (size_t)sanz); /* kompbdc */
+          b[m->enr[l] - 1] = -fak[m->enr[l] - 1];
+          if (m->lsink) b[m->nsink - 1] = fak[m->nsink - 1];
+        }
+        orc_vredc(sanz, mb, a, b, pot);
+        double *dst = kpotdc + (size_t)sanz * (l + (size_t)E * k);
+        for (int j = 0; j < sanz; j++) dst[j] = pot[j] * fak[j];
+      }
+    }
+    free(a); free(b); free(pot); free(fak);
+  }
+  if (err) return err;
+  if (swrtr == 0) {
+    memcpy(hpotdc, kpotdc, sizeof(double) * (size_t)sanz * E);
+  } else {
+    double pi = acos(-1.0);
+#pragma omp parallel for collapse(2)
+    for (int l = 0; l < E; l++)
+      for (int j = 0; j < sanz; j++) {
+        double s = 0.0;
+        for (int k = 0; k < K; k++) s = s + kpotdc[j + (size_t)sanz * (l + (size_t)E * k)] * kwnwi[k];
+        hpotdc[j + (size_t)sanz * l] = s / pi;
+      }
+  }
+  return 0;
+}
+
+/* bsendc: sensdc(i, e) += dum * DBLE(sigma(e) / volt(i)) with the real h_k
+ * sums of kpotdc; sensdc: nanz x elanz column major. */
+int orc_sens_dc(const orc_mesh *m, int nanz, const int *strnr, const int *vnr, int K,
+                const double *kwnwi, int swrtr, const double *elbg, const double *kpotdc,
+                const orc_c *sigma, const orc_c *volt, double *sensdc, int nthreads) {
+  int elanz = m_elanz(m), relanz = m_relanz(m), nel = elanz + relanz;
+  int maxn = m_maxnodes(m), nimn = maxn * (maxn + 1) / 2;
+  int sanz = m->sanz, E = m->eanz;
+  double pi = acos(-1.0);
+  memset(sensdc, 0, sizeof(double) * (size_t)nanz * elanz);
+#ifdef _OPENMP
+  if (nthreads > 0) omp_set_num_threads(nthreads);
+#else
+  (void)nthreads;
+#endif
+#pragma omp parallel
+  {
+    double *hsens = malloc(sizeof(double) * (size_t)K);
+#pragma omp for schedule(guided, 16)
+    for (int i = 0; i < nanz; i++) {
+      int e1 = strnr[i] % 10000, e2 = (strnr[i] - e1) / 10000;
+      int e3 = vnr[i] % 10000, e4 = (vnr[i] - e3) / 10000;
+      double sup[4] = {0, 0, 0, 0};
+      int iel = 0;
+      for (int ityp = 0; ityp < m->typanz; ityp++) {
+        int ntyp = m->typ[ityp], nkel = m->selanz[ityp];
+        if (ntyp > 10) continue;
+        for (int jn = 0; jn < m->nelanz[ityp]; jn++, iel++) {
+          for (int k = 0; k < K; k++) {
+            double h = 0.0;
+            const double *kp = kpotdc + (size_t)sanz * (size_t)E * k;
+            for (int mi = 0; mi < nkel; mi++) {
+              int nzp = NREL(m, nel, iel, mi);
+              for (int mj = 0; mj < nkel; mj++) {
+                int nnp = NREL(m, nel, iel, mj);
+                int imax = (mi > mj ? mi : mj) + 1, imin = (mi < mj ? mi : mj) + 1;
+                int imn = imax * (imax - 1) / 2 + imin;
+                if (e1 > 0) sup[0] = kp[(nnp - 1) + (size_t)sanz * (e1 - 1)];
+                if (e2 > 0) sup[1] = kp[(nnp - 1) + (size_t)sanz * (e2 - 1)];
+                if (e3 > 0) sup[2] = kp[(nzp - 1) + (size_t)sanz * (e3 - 1)];
+                if (e4 > 0) sup[3] = kp[(nzp - 1) + (size_t)sanz * (e4 - 1)];
+                double dum = (sup[1] - sup[0]) * (sup[3] - sup[2]);
+                h = h + elbg[iel + (size_t)elanz * ((imn - 1) + (size_t)nimn * k)] * dum;
+              }
+            }
+            hsens[k] = h;
+          }
+          double dum;
+          if (swrtr == 0) {
+            dum = hsens[0];
+          } else {
+            dum = 0.0;
+            for (int k = 0; k < K; k++) dum = dum + hsens[k] * kwnwi[k];
+            dum = dum / pi;
+          }
+          sensdc[i + (size_t)nanz * iel] += dum * cdiv(sigma[iel], volt[i]).re;
+        }
+      }
+    }
+    free(hsens);
+  }
+  return 0;
+}
+
 /* Preconditioner and RHS of update: src/update.f90:179-293 (ldc, lfpi, ldiff
  * and the reference-model term off).  rdiag[j] = the diagonal entry of R that
  * update adds for the active ltri (smatm(j,1) for ltri 0/3/4,

@@ -50,6 +50,16 @@ int orc_cjggra(int nanz, int manz, const orc_c *sens, const double *wmatd, const
 void orc_bsmatmreg(const orc_mesh *m, int nx, int nz, double alfx, double alfz,
                    double *smatm);
 void orc_rm_reg(int manz, int nx, const double *smatm, const orc_c *par, orc_c *out);
+int orc_assemble_dc(const orc_mesh *m, const double *elbg, const double *relbg, const double *kg,
+                    int lrandb, const orc_c *sigma, int l, int k, double *a, double *b, double *fak);
+int orc_choldc(int sanz, int mb, double *a);
+void orc_vredc(int sanz, int mb, const double *a, const double *b, double *pot);
+int orc_forward_dc(const orc_mesh *m, int kwnanz, const double *kwnwi, int swrtr,
+                   const double *elbg, const double *relbg, const double *kg, int lbeta,
+                   int lrandb, const orc_c *sigma, double *kpotdc, double *hpotdc, int nthreads);
+int orc_sens_dc(const orc_mesh *m, int nanz, const int *strnr, const int *vnr, int kwnanz,
+                const double *kwnwi, int swrtr, const double *elbg, const double *kpotdc,
+                const orc_c *sigma, const orc_c *volt, double *sensdc, int nthreads);
 void orc_adjoint_grad(int nanz, int manz, const orc_c *sens, const double *wmatd,
                       const int *wdfak, const orc_c *resid, orc_c *grad);
 

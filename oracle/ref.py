@@ -43,6 +43,10 @@ def lib():
         L.ref_forward.argtypes = [_z, _z, _z, C.c_int]
         L.ref_volt.argtypes = [C.c_int, _z, _z]
         L.ref_sens.argtypes = [C.c_int, _z]
+        L.ref_forward_dc.argtypes = [_z, _d, _d, C.c_int]
+        L.ref_assemble_dc.argtypes = [C.c_int, C.c_int, _z, _d, _d, _d]
+        L.ref_choldc.argtypes = [_d]
+        L.ref_sens_dc.argtypes = [_d]
         _lib = L
     return _lib
 
@@ -152,6 +156,41 @@ class Reference:
         e = lib().ref_sens(mode, out)
         if e:
             raise RefError("sens", e)
+        return out.reshape((self.nanz, self.elanz), order="F")
+
+    # DC twins (ldc): kompadc/randdc/scaldc/choldc/vredc/kompbdc, rtrafo's DC
+    # branch, bvolti's DC branch (through volt(1)) and bsendc
+    def forward_dc(self, sigma, nthreads=0):
+        n, E, K = self.mesh.sanz, self.mesh.eanz, self.K
+        kp = np.zeros(n * E * K)
+        hp = np.zeros(n * E)
+        e = lib().ref_forward_dc(np.ascontiguousarray(sigma, np.complex128), kp, hp, nthreads)
+        if e:
+            raise RefError("forward_dc", e)
+        return kp.reshape((n, E, K), order="F"), hp.reshape((n, E), order="F")
+
+    def assemble_dc(self, sigma, l, k):
+        n, mb = self.mesh.sanz, self.mesh.mb
+        a = np.zeros((mb + 1) * n)
+        b = np.zeros(n)
+        fak = np.zeros(n)
+        e = lib().ref_assemble_dc(l + 1, k + 1, np.ascontiguousarray(sigma, np.complex128), a, b, fak)
+        if e:
+            raise RefError("assemble_dc", e)
+        return a, b, fak
+
+    def choldc(self, a):
+        a = a.copy()
+        e = lib().ref_choldc(a)
+        if e:
+            raise RefError("choldc", e)
+        return a
+
+    def sens_dc(self):
+        out = np.zeros(self.nanz * self.elanz)
+        e = lib().ref_sens_dc(out)
+        if e:
+            raise RefError("sens_dc", e)
         return out.reshape((self.nanz, self.elanz), order="F")
 
 

@@ -97,6 +97,8 @@ contains
     if (allocated(kg)) deallocate(kg)
     if (allocated(kpot)) deallocate(kpot)
     if (allocated(hpot)) deallocate(hpot)
+    if (allocated(kpotdc)) deallocate(kpotdc)
+    if (allocated(hpotdc)) deallocate(hpotdc)
   end subroutine ref_free_mesh
 
   !> Measurement configurations (replaces rdatm, src/rdatm.f90:54-115).
@@ -224,6 +226,7 @@ contains
     real(kind(0d0)), allocatable :: my_fak(:)
     integer :: k, l, j, nerr
     call set_sigma(c_sigma)
+    ldc = .false.
     if (allocated(kpot)) deallocate(kpot)
     if (allocated(hpot)) deallocate(hpot)
     allocate(kpot(sanz, eanz, kwnanz), hpot(sanz, eanz))
@@ -258,6 +261,94 @@ contains
     c_hpot(1:sanz*eanz) = reshape(hpot, (/sanz*eanz/))
     ref_forward = errnr
   end function ref_forward
+
+  !> The DC forward loop of src/inv.f90:343-400 (kompadc -> randdc -> scaldc
+  !> -> choldc, or kompbdc when the factor is reused, then vredc and kpotdc =
+  !> pot*fak), then rtrafo's DC branch when swrtr=1.  Leaves ldc set, so that
+  !> ref_volt (bvolti) and ref_sens_dc (bsendc) take their DC branches.
+  !> kpotdc_out: sanz x eanz x kwnanz, hpotdc_out: sanz x eanz (real).
+  integer(c_int) function ref_forward_dc(c_sigma, c_kpotdc, c_hpotdc, c_nthreads) &
+       bind(C, name='ref_forward_dc')
+    use omp_lib
+    complex(c_double_complex), intent(in) :: c_sigma(*)
+    real(c_double), intent(out) :: c_kpotdc(*), c_hpotdc(*)
+    integer(c_int), value :: c_nthreads
+    real(kind(0d0)), allocatable :: my_a(:), my_b(:), my_fak(:)
+    complex(kind(0d0)), allocatable :: my_pot(:)
+    integer :: k, l, j
+    call set_sigma(c_sigma)
+    ldc = .true.
+    if (allocated(kpotdc)) deallocate(kpotdc)
+    if (allocated(hpotdc)) deallocate(hpotdc)
+    allocate(kpotdc(sanz, eanz, kwnanz), hpotdc(sanz, eanz))
+    if (c_nthreads > 0) call omp_set_num_threads(c_nthreads)
+    !$OMP PARALLEL DEFAULT(shared) PRIVATE(my_a, my_b, my_pot, my_fak, k, l, j)
+    allocate(my_a((mb+1)*sanz), my_b(sanz), my_pot(sanz), my_fak(sanz))
+    !$OMP DO
+    do k = 1, kwnanz
+       do l = 1, eanz
+          if (lbeta .or. l == 1) then
+             call kompadc(l, k, my_a, my_b)
+             if (lrandb) call randdc(my_a, my_b)
+             call scaldc(my_a, my_b, my_fak)
+             call choldc(my_a)
+          else
+             call kompbdc(l, my_b, my_fak)
+          end if
+          call vredc(my_a, my_b, my_pot)
+          do j = 1, sanz
+             kpotdc(j, l, k) = dble(my_pot(j)) * my_fak(j)
+             if (swrtr == 0) hpotdc(j, l) = kpotdc(j, l, k)
+          end do
+       end do
+    end do
+    !$OMP END DO
+    deallocate(my_a, my_b, my_pot, my_fak)
+    !$OMP END PARALLEL
+    if (swrtr == 1) call rtrafo()
+    c_kpotdc(1:sanz*eanz*kwnanz) = reshape(kpotdc, (/sanz*eanz*kwnanz/))
+    c_hpotdc(1:sanz*eanz) = reshape(hpotdc, (/sanz*eanz/))
+    ref_forward_dc = errnr
+  end function ref_forward_dc
+
+  !> One (k,l) system as the DC loop assembles it: kompadc, randdc, scaldc.
+  integer(c_int) function ref_assemble_dc(c_l, c_k, c_sigma, c_a, c_b, c_fak) bind(C, name='ref_assemble_dc')
+    integer(c_int), value :: c_l, c_k
+    complex(c_double_complex), intent(in) :: c_sigma(*)
+    real(c_double), intent(out) :: c_a(*), c_b(*), c_fak(*)
+    real(kind(0d0)), allocatable :: my_a(:), my_b(:), my_fak(:)
+    call set_sigma(c_sigma)
+    allocate(my_a((mb+1)*sanz), my_b(sanz), my_fak(sanz))
+    call kompadc(c_l, c_k, my_a, my_b)
+    if (errnr /= 0) then
+       ref_assemble_dc = errnr
+       return
+    end if
+    if (lrandb) call randdc(my_a, my_b)
+    call scaldc(my_a, my_b, my_fak)
+    ref_assemble_dc = errnr
+    c_a(1:(mb+1)*sanz) = my_a
+    c_b(1:sanz) = my_b
+    c_fak(1:sanz) = my_fak
+  end function ref_assemble_dc
+
+  !> In-place reference choldc (src/choldc.f90:7-86).
+  integer(c_int) function ref_choldc(c_a) bind(C, name='ref_choldc')
+    real(c_double), intent(inout) :: c_a(*)
+    call choldc(c_a)
+    ref_choldc = errnr
+  end function ref_choldc
+
+  !> Reference bsendc (src/bsendc.f90:1-187) after ref_forward_dc and ref_volt(1).
+  !> sensdc_out: nanz x elanz column major (real).
+  integer(c_int) function ref_sens_dc(c_sensdc) bind(C, name='ref_sens_dc')
+    real(c_double), intent(out) :: c_sensdc(*)
+    if (allocated(sensdc)) deallocate(sensdc)
+    allocate(sensdc(nanz, manz))
+    call bsendc(.false.)
+    c_sensdc(1:nanz*manz) = reshape(sensdc, (/nanz*manz/))
+    ref_sens_dc = errnr
+  end function ref_sens_dc
 
   !> A bounded sample of the forward loop for timing: systems s = (k-1)*eanz + l
   !> in [c_first, c_last] (1-based), each assembled, scaled, factored and

@@ -68,6 +68,30 @@ def synthetic(w, lsytop=False):
     print(f"wrote ref_{w.name}.npz  K={K} N={len(v0)}")
 
 
+def synthetic_dc(w):
+    """The DC twins (kompadc ... bsendc) on the same case: kpotdc at the
+    electrode nodes, hpotdc electrode rows, DC voltages and sensdc rows, plus
+    one scaled system and its choldc factor."""
+    m = w.mesh
+    r = Reference(m)
+    r.load_data(w.strnr, w.vnr)
+    r.rwaven()
+    r.precal()
+    kp, hp = r.forward_dc(w.sigma, nthreads=8)
+    v1, sa1 = r.volt(1)
+    sdc = r.sens_dc()
+    a, b, fak = r.assemble_dc(w.sigma, 0, 0)
+    L = r.choldc(a)
+    np.savez_compressed(
+        OUT / f"ref_{w.name}_dc.npz",
+        kpotdc_elec=electrode_slice(kp, m.enr), kpotdc_col=kp[:, 0, :],
+        hpotdc_elec=hp[np.asarray(m.enr) - 1, :], volt=v1, sigmaa1=sa1,
+        sensdc=sdc[:SENS_ROWS], sensdc_rowsum=sdc.sum(axis=1),
+        a00=a, b00=b, fak00=fak, chol00=L,
+    )
+    print(f"wrote ref_{w.name}_dc.npz  N={len(v1)}")
+
+
 def fixture(name, swrtr, lsink=False, nsink=0):
     d = FIX / name
     m = F.read_elem(d / "elem.dat")
@@ -93,6 +117,8 @@ def main():
         sys.exit("oracle/_ref/libcrtref.so is not built (make -C oracle ref)")
     synthetic(meshgen.small_block())
     synthetic(meshgen.small_dirichlet())
+    synthetic_dc(meshgen.small_block())
+    synthetic_dc(meshgen.small_dirichlet())
     fixture("minimal", 1)
     fixture("mixed_surf", 1)
     fixture("decoupling", 0, lsink=True, nsink=373)
