@@ -80,6 +80,16 @@ int crt_forward(crt_handle *h, const double *sigma, double *kpot, double *hpot);
 /* Same with sigma already in device memory (elanz complex). */
 int crt_forward_dev(crt_handle *h, const double *sigma_dev);
 
+/* The DC twins (CRTomo with ldc): kompadc (real part of sigma), randdc, scaldc
+ * (unit diagonal; error 27 when a_ii <= 0), choldc (error 28 when a pivot is
+ * <= 0), vredc or kompbdc + vredc for a shared factor, kpotdc = pot*fak, then
+ * rtrafo's DC branch; replaces src/inv.f90:343-400 and src/rtrafo.f90:48-66.
+ * sigma: elanz complex (as the reference's sigma array; kept for bsendc).
+ * kpotdc (sanz x eanz x K) and hpotdc (sanz x eanz) are optional real host
+ * outputs.  crt_voltages then gives bvolti's DC branch (src/bvolti.f90:56-66,
+ * volt = dcmplx(V)). */
+int crt_forward_dc(crt_handle *h, const double *sigma, double *kpotdc, double *hpotdc);
+
 /* Voltages and apparent values, src/bvolt.f90 (mode 0: sigmaa = kfak*V) and
  * src/bvolti.f90 (mode 1: sigmaa = -ln V, error 82 on V = 0).  kfak may be NULL
  * (1).  The configuration is kept in h for the sensitivity/gradient stages. */
@@ -93,6 +103,10 @@ int crt_voltages_dev(crt_handle *h, int32_t nanz, const int32_t *strnr, const in
  * (mode 1: d(-ln V)/d(ln sigma)); sens: nanz x elanz column major complex, or NULL
  * to keep it on the device for crt_cg_solve. */
 int crt_sensitivities(crt_handle *h, int32_t mode, double *sens);
+
+/* bsendc (src/bsendc.f90:23-187) after crt_forward_dc and crt_voltages(mode 1):
+ * sensdc nanz x elanz column major, real. */
+int crt_sensitivities_dc(crt_handle *h, double *sensdc);
 
 /* Data-misfit gradient without forming J: grad_e = sum_i conj(J_ie) w_i r_i with
  * J as crt_sensitivities(mode) defines it (the data term of src/update.f90:235-289).

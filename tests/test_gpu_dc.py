@@ -1,0 +1,114 @@
+"""DC twins on the device (crt_forward_dc, crt_sensitivities_dc) against the
+oracle's DC restatement and the reference's DC golden vectors.
+
+* kpotdc / hpotdc at 1e-10 relative, the DC voltages at 1e-10 in magnitude
+  and 1e-9 rad in phase, sensdc at 1e-8 relative -- on a mesh with mixed
+  boundaries (one factor per system) and on one without (one factor per
+  wavenumber shared by every source, kompbdc);
+* the DC error rules from the host side: -sigma makes every diagonal
+  negative (scaldc: 27) and a striped sigma of +1 / -0.5 by element column
+  keeps every diagonal positive while the matrix is indefinite (choldc: 28);
+  the complex path runs both without error.
+"""
+import numpy as np
+import pytest
+
+from conftest import golden, rel_err
+from titatest_amd import meshgen
+
+pytestmark = pytest.mark.gpu
+
+TOL_POT = 1e-10
+TOL_MAG = 1e-10
+TOL_PH = 1e-9
+TOL_SENS = 1e-8
+
+
+def say(*a):
+    print("[gpu-case]", *a, flush=True)
+
+
+@pytest.fixture(scope="module")
+def crtmod(hip_lib):
+    from titatest_amd import crt
+    return crt
+
+
+def _pair(crt, oracle_lib, w):
+    g = crt.CRT(w.mesh)
+    g.rwaven(w.strnr, w.vnr)
+    g.precal()
+    o = oracle_lib.Oracle(w.mesh)
+    o.rwaven(w.strnr, w.vnr)
+    o.precal()
+    return g, o
+
+
+@pytest.mark.parametrize("case", ["small_block", "small_dirichlet"])
+def test_dc_forward_voltages_sensitivities(crtmod, oracle_lib, case):
+    w = getattr(meshgen, case)()
+    say(case, "DC forward + bvolti + bsendc")
+    g, o = _pair(crtmod, oracle_lib, w)
+    kp, hp = g.forward_dc(w.sigma, want_kpot=True, want_hpot=True)
+    kpo, hpo = o.forward_dc(w.sigma)
+    assert kp.dtype == np.float64
+    assert rel_err(kp, kpo) < TOL_POT
+    assert rel_err(hp, hpo) < TOL_POT
+    v, sa = g.bvolti(w.strnr, w.vnr)
+    gd = golden(f"{case}_dc")
+    vr = gd["volt"]
+    assert np.max(np.abs(v.imag)) == 0.0
+    assert np.max(np.abs(np.abs(v) - np.abs(vr)) / np.abs(vr)) < TOL_MAG
+    assert np.max(np.abs(np.angle(v / vr))) < TOL_PH
+    assert rel_err(sa, gd["sigmaa1"]) < TOL_MAG
+    vo, _ = o.volt(w.strnr, w.vnr, 1)
+    s = g.bsendc()
+    so = o.sens_dc()
+    assert s.dtype == np.float64
+    assert rel_err(s, so) < TOL_SENS
+    assert rel_err(s[:8], gd["sensdc"]) < TOL_SENS
+
+
+def test_dc_error_rules_on_device(crtmod, oracle_lib):
+    w = meshgen.small_block()
+    say("small_block DC error rules (27 from -sigma, 28 from a striped sigma)")
+    g, o = _pair(crtmod, oracle_lib, w)
+    with pytest.raises(crtmod.CRTError) as ex:
+        g.forward_dc(-w.sigma)
+    assert ex.value.code == 27
+    g.forward(-w.sigma)  # the complex scaling takes |a_ii|
+    m = w.mesh
+    nrel = np.asarray(m.nrel)
+    xc = np.array([np.mean([m.sx[m.snr[nrel[e, k] - 1] - 1] for k in range(4)]) for e in range(m.elanz)])
+    col = np.round(xc - xc.min()).astype(int)
+    sig = np.where((col % 2 == 1) & (col > 1) & (col < col.max() - 1), -0.5, 1.0).astype(np.complex128) * 0.01
+    with pytest.raises(oracle_lib.OracleError) as exo:
+        o.forward_dc(sig)
+    assert exo.value.code == 28
+    with pytest.raises(crtmod.CRTError) as ex:
+        g.forward_dc(sig)
+    assert ex.value.code == 28
+    g.forward(sig)  # complex pivots are only checked for zero
+    # the handle recovers: a valid DC forward after the failures
+    (kp,) = g.forward_dc(w.sigma, want_kpot=True)
+    kpo, _ = o.forward_dc(w.sigma)
+    assert rel_err(kp, kpo) < TOL_POT
+
+
+def test_dc_state_guards(crtmod):
+    """bsendc needs a DC forward; the complex J and the adjoint need a complex one."""
+    w = meshgen.small_block()
+    say("small_block DC/complex stage order")
+    g = crtmod.CRT(w.mesh)
+    g.rwaven(w.strnr, w.vnr)
+    g.precal()
+    g.forward(w.sigma)
+    g.bvolti(w.strnr, w.vnr)
+    with pytest.raises(crtmod.CRTError) as ex:
+        g.bsendc()
+    assert ex.value.code == 1002
+    g.forward_dc(w.sigma)
+    g.bvolti(w.strnr, w.vnr)
+    with pytest.raises(crtmod.CRTError) as ex:
+        g.bsensi()
+    assert ex.value.code == 1002

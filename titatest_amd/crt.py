@@ -35,7 +35,7 @@ EXPORTS = (
     "crt_create", "crt_destroy", "crt_rwaven", "crt_set_wavenumbers", "crt_precal",
     "crt_get_precal", "crt_forward", "crt_forward_dev", "crt_voltages", "crt_voltages_dev",
     "crt_sensitivities", "crt_adjoint_gradient", "crt_adjoint_gradient_dev", "crt_jhw_r",
-    "crt_cg_solve",
+    "crt_cg_solve", "crt_forward_dc", "crt_sensitivities_dc",
     "crt_error_info", "crt_set_timing", "crt_stage_times", "crt_factor_phase_ticks",
     "crt_kpot_device",
 )
@@ -87,6 +87,8 @@ def load_library(path: Path = LIB_PATH):
     L.crt_get_precal.argtypes = [vp, vp, vp, vp]
     L.crt_forward.argtypes = [vp, vp, vp, vp]
     L.crt_forward_dev.argtypes = [vp, vp]
+    L.crt_forward_dc.argtypes = [vp, vp, vp, vp]
+    L.crt_sensitivities_dc.argtypes = [vp, vp]
     L.crt_voltages.argtypes = [vp, i32, vp, vp, i32, vp, vp, vp]
     L.crt_voltages_dev.argtypes = [vp, i32, vp, vp, i32, vp, vp]
     L.crt_sensitivities.argtypes = [vp, i32, vp]
@@ -206,6 +208,29 @@ class CRT:
         if want_hpot:
             out.append(hpot.reshape((n, E), order="F"))
         return tuple(out)
+
+    # A19: DC twins ----------------------------------------------------------
+    def forward_dc(self, sigma, want_kpot=False, want_hpot=False):
+        """kompadc ... vredc on the device (crt_forward_dc): real kpotdc
+        (sanz, eanz, K) and hpotdc (sanz, eanz) as requested.  bvolti then
+        gives the DC voltages and choldc's error rules apply (27, 28)."""
+        n, E, K = self.mesh.sanz, self.mesh.eanz, self.K
+        sig = _c128(sigma)
+        kpot = np.zeros(n * E * K) if want_kpot else None
+        hpot = np.zeros(n * E) if want_hpot else None
+        self._check("forward_dc", self.lib.crt_forward_dc(self.h, _ptr(sig), _ptr(kpot), _ptr(hpot)))
+        out = []
+        if want_kpot:
+            out.append(kpot.reshape((n, E, K), order="F"))
+        if want_hpot:
+            out.append(hpot.reshape((n, E), order="F"))
+        return tuple(out)
+
+    def bsendc(self):
+        """Real DC sensitivities (nanz, elanz) after forward_dc and bvolti."""
+        out = np.zeros(self.nanz * self.elanz)
+        self._check("sensitivities_dc", self.lib.crt_sensitivities_dc(self.h, _ptr(out)))
+        return out.reshape((self.nanz, self.elanz), order="F")
 
     # A12 ------------------------------------------------------------------
     def voltages(self, strnr, vnr, mode=0, kfak=None):

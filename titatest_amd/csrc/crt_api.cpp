@@ -42,6 +42,7 @@ struct crt_handle {
   std::vector<double> kwn, kwnwi;
   // stage state
   bool precal_done = false, forward_done = false, volt_done = false, sens_done = false;
+  bool dc = false;  // the last forward was the DC twin (crt_forward_dc)
   int sens_mode = -1;
   int nanz = 0;
   std::vector<int> strnr, vnr;
@@ -707,6 +708,7 @@ static int run_forward(crt_handle* h) {
   A.isdir = h->has13 ? h->at<uint8_t>("isdir") : nullptr;
   A.lsink = h->lsink;
   A.nsink = h->nsink - 1;
+  A.dc = h->dc ? 1 : 0;
   double2* av = h->get<double2>("av", (size_t)K * n * h->R);
   double2* kpot = h->get<double2>("kpot", (size_t)K * E * n);
   NEED(av);
@@ -840,6 +842,7 @@ int crt_forward(crt_handle* h, const double* sigma, double* kpot, double* hpot) 
   if (!h->precal_done) return kErrOrder;
   double2* ds;
   if (upload(h, "sigma", reinterpret_cast<const double2*>(sigma), h->elanz, &ds)) return kErrHip;
+  h->dc = false;
   int rc = run_forward(h);
   if (rc) return rc;
   const int n = h->n, E = h->E, K = h->K;
@@ -865,7 +868,52 @@ int crt_forward_dev(crt_handle* h, const double* sigma_dev) {
   double2* ds = h->get<double2>("sigma", h->elanz);
   NEED(ds);
   HIPCHK(hipMemcpyAsync(ds, sigma_dev, (size_t)h->elanz * 16, hipMemcpyDeviceToDevice, h->stream));
+  h->dc = false;
   return run_forward(h);
+}
+
+// Real parts of n complex device values into a host array, through a device
+// staging buffer.
+static int copy_real(crt_handle* h, const double2* src, size_t n, double* host) {
+  double* d = h->get<double>("dc_real", n);
+  NEED(d);
+  hipLaunchKernelGGL(crt_real_part_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, h->stream, src, n, d);
+  int rc = finish(h);
+  if (rc) return rc;
+  HIPCHK(hipMemcpy(host, d, n * 8, hipMemcpyDeviceToHost));
+  return 0;
+}
+
+int crt_forward_dc(crt_handle* h, const double* sigma, double* kpotdc, double* hpotdc) {
+  if (!h || !sigma) return kErrArg;
+  if (!h->precal_done) return kErrOrder;
+  // the caller's sigma stays for bsendc's sigma/V; the assembly takes its real part
+  double2* din;
+  if (upload(h, "sigma_dc", reinterpret_cast<const double2*>(sigma), h->elanz, &din)) return kErrHip;
+  double2* ds = h->get<double2>("sigma", h->elanz);
+  NEED(ds);
+  hipLaunchKernelGGL(crt_real_sigma_kernel, dim3((h->elanz + 255) / 256), dim3(256), 0, h->stream,
+                     (const double2*)din, h->elanz, ds);
+  h->dc = true;
+  int rc = run_forward(h);
+  if (rc) return rc;
+  const int n = h->n, E = h->E, K = h->K;
+  if (kpotdc) {
+    rc = copy_real(h, h->at<double2>("kpot"), (size_t)K * E * n, kpotdc);
+    if (rc) return rc;
+  }
+  if (hpotdc) {
+    double2* dh = h->get<double2>("hpot", (size_t)E * n);
+    NEED(dh);
+    tstart(h, "rtrafo");
+    hipLaunchKernelGGL(crt_rtrafo_kernel, dim3((unsigned)(((size_t)n * E + 255) / 256)), dim3(256), 0, h->stream,
+                       (const double2*)h->at<double2>("kpot"), (const double*)h->at<double>("kwnwi"), n, E, K,
+                       h->swrtr, dh);
+    tstop(h);
+    rc = copy_real(h, dh, (size_t)E * n, hpotdc);
+    if (rc) return rc;
+  }
+  return 0;
 }
 
 static int run_voltages(crt_handle* h, int32_t nanz, const int32_t* strnr, const int32_t* vnr, int32_t mode,
@@ -925,9 +973,7 @@ int crt_voltages_dev(crt_handle* h, int32_t nanz, const int32_t* strnr, const in
   return finish(h);
 }
 
-int crt_sensitivities(crt_handle* h, int32_t mode, double* sens) {
-  if (!h || (mode != 0 && mode != 1)) return kErrArg;
-  if (!h->volt_done) return kErrOrder;
+static int run_sens(crt_handle* h, int32_t mode, const double2* sigma) {
   SensArgs S;
   S.n = h->n;
   S.elanz = h->elanz;
@@ -942,7 +988,7 @@ int crt_sensitivities(crt_handle* h, int32_t mode, double* sens) {
   S.elbg = h->at<double>("elbg");
   S.kwnwi = h->at<double>("kwnwi");
   S.kpot = h->at<double2>("kpot");
-  S.sigma = h->at<double2>("sigma");
+  S.sigma = sigma;
   S.volt = h->at<double2>("volt");
   S.strnr = h->at<int>("strnr");
   S.vnr = h->at<int>("vnr");
@@ -952,12 +998,30 @@ int crt_sensitivities(crt_handle* h, int32_t mode, double* sens) {
   tstart(h, "sens");
   hipLaunchKernelGGL(crt_sens_kernel, dim3(h->elanz), dim3(256), lds, h->stream, S);
   tstop(h);
-  int rc = finish(h);
+  return finish(h);
+}
+
+int crt_sensitivities(crt_handle* h, int32_t mode, double* sens) {
+  if (!h || (mode != 0 && mode != 1)) return kErrArg;
+  if (!h->volt_done || h->dc) return kErrOrder;
+  int rc = run_sens(h, mode, h->at<double2>("sigma"));
   if (rc) return rc;
   h->sens_done = true;
   h->sens_mode = mode;
-  if (sens) HIPCHK(hipMemcpy(sens, S.sens, (size_t)h->nanz * h->elanz * 16, hipMemcpyDeviceToHost));
+  if (sens) HIPCHK(hipMemcpy(sens, h->at<double2>("sens"), (size_t)h->nanz * h->elanz * 16, hipMemcpyDeviceToHost));
   return 0;
+}
+
+// bsendc: the bsensi sums of the real potentials times sigma/V, of which the
+// real part is kept (src/bsendc.f90:183-184).
+int crt_sensitivities_dc(crt_handle* h, double* sensdc) {
+  if (!h || !sensdc) return kErrArg;
+  if (!h->volt_done || !h->dc) return kErrOrder;
+  int rc = run_sens(h, 1, h->at<double2>("sigma_dc"));
+  if (rc) return rc;
+  // the complex J of this call is not the J the CG stage expects
+  h->sens_done = false;
+  return copy_real(h, h->at<double2>("sens"), (size_t)h->nanz * h->elanz, sensdc);
 }
 
 static int run_adjoint(crt_handle* h, int32_t mode, const double* w_dev, const double2* r_dev, double2* g_dev) {
@@ -1006,7 +1070,7 @@ static int run_adjoint(crt_handle* h, int32_t mode, const double* w_dev, const d
 
 int crt_adjoint_gradient(crt_handle* h, int32_t mode, const double* w, const double* resid, double* grad) {
   if (!h || !w || !resid || !grad || (mode != 0 && mode != 1)) return kErrArg;
-  if (!h->volt_done) return kErrOrder;
+  if (!h->volt_done || h->dc) return kErrOrder;
   double* dw;
   double2* dr;
   int rc = upload(h, "adj_w", w, h->nanz, &dw);
@@ -1023,7 +1087,7 @@ int crt_adjoint_gradient(crt_handle* h, int32_t mode, const double* w, const dou
 int crt_adjoint_gradient_dev(crt_handle* h, int32_t mode, const double* w_dev, const double* resid_dev,
                              double* grad_dev) {
   if (!h || !w_dev || !resid_dev || !grad_dev || (mode != 0 && mode != 1)) return kErrArg;
-  if (!h->volt_done) return kErrOrder;
+  if (!h->volt_done || h->dc) return kErrOrder;
   return run_adjoint(h, mode, w_dev, reinterpret_cast<const double2*>(resid_dev),
                      reinterpret_cast<double2*>(grad_dev));
 }

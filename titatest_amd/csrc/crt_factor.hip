@@ -211,7 +211,8 @@ __global__ void crt_area_values_kernel(FactorArgs A, int K, double2* __restrict_
 
 // ---------------------------------------------------------------------------
 // f_r = 1/sqrt(|a_rr|) of every row of every system of the batch (scalab
-// :56-63; errnr 27 when |a_rr| = 0).
+// :56-63; errnr 27 when |a_rr| = 0), or 1/sqrt(a_rr) for the DC twin (scaldc
+// :53-61; errnr 27 when a_rr <= 0).
 __global__ void crt_scale_kernel(FactorArgs A) {
   const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= (long)A.nsys * A.n) return;
@@ -223,7 +224,8 @@ __global__ void crt_scale_kernel(FactorArgs A) {
   const int b1 = A.bptr[r + 1];
   for (int p = A.bptr[r]; p < b1; ++p)
     if (A.bslot[p] == qd) v = cadd(v, bnd_term(A, k, l, A.bcode[p], A.bimn[p]));
-  const double ad = cabs_(v);
+  // DC (scaldc :53-59): a_rr is real and must be positive
+  const double ad = A.dc ? v.re : cabs_(v);
   double f = 1.0 / sqrt(ad);
   if (ad <= 0.0) {  // scalab :56-61
     f = 1.0;
@@ -251,7 +253,7 @@ __global__ void crt_rows_kernel(FactorArgs A) {
       v = ldc(A.av + ((size_t)k * A.n + r) * A.R + q);
       for (int p = b0; p < b1; ++p)
         if (A.bslot[p] == q) v = cadd(v, bnd_term(A, k, l, A.bcode[p], A.bimn[p]));
-      if (t == A.mb) v = cdiv(v, mkc(cabs_(v), 0.0));
+      if (t == A.mb) v = A.dc ? mkc(1.0, 0.0) : cdiv(v, mkc(cabs_(v), 0.0));  // scaldc :60
       else v = rmul(fr * fk[r - A.mb + t], v);
     }
     out[q] = make_double2(v.re, v.im);
@@ -439,8 +441,9 @@ __global__ __launch_bounds__(256, 2) void crt_factor_kernel(FactorArgs A) {
 #pragma unroll
         for (int c = 0; c < kNB; ++c) {
           if (c < nbc) {
-            // chol :66 -- |s| <= 0 is an exact zero (a NaN pivot passes, as there)
-            zero_pivot |= piv.re == 0.0 && piv.im == 0.0;
+            // chol :66 -- |s| <= 0 is an exact zero (a NaN pivot passes, as
+            // there); choldc :66 -- s <= 0 for the real DC pivots
+            zero_pivot |= A.dc ? piv.re <= 0.0 : (piv.re == 0.0 && piv.im == 0.0);
             const cplx inv = crecip(piv);
             xd[c] = piv;  // every lane stores the same value
             xi[c] = inv;

@@ -48,6 +48,7 @@ struct FactorArgs {
   double2* S;               // [nsys][n][R] scaled row entries (crt_rows_kernel)
   double2* L;               // [nsys][n][mb+1]
   double2* y;               // [nsys][n]
+  int dc;                   // DC twins (scaldc, choldc): real sigma, a_ii <= 0 -> 27, pivot <= 0 -> 28
   int* err;                 // first error code (0 = ok) and the failing system
   unsigned long long* prof; // optional: 8 phase-time accumulators (wall clock ticks)
 };

@@ -69,6 +69,8 @@ __global__ void crt_precal_area_kernel(PrecalArgs P);
 __global__ void crt_precal_bnd_kernel(PrecalArgs P);
 __global__ void crt_rtrafo_kernel(const double2* kpot, const double* kwnwi, int n, int eanz, int K,
                                   int swrtr, double2* hpot);
+__global__ void crt_real_sigma_kernel(const double2* in, int n, double2* out);
+__global__ void crt_real_part_kernel(const double2* in, size_t n, double* out);
 __global__ void crt_rtrafo_elec_kernel(const double2* kpot, const double* kwnwi, const int* enr, int n,
                                        int eanz, int K, int swrtr, double2* he);
 __global__ void crt_volt_kernel(const double2* he, int eanz, int nanz, const int* strnr, const int* vnr,

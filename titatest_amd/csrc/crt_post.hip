@@ -217,6 +217,20 @@ __global__ void crt_rtrafo_kernel(const double2* __restrict__ kpot, const double
   stc(hpot + idx, cdiv(s, mkc(M_PI, 0.0)));
 }
 
+// The DC twins assemble with the real part of sigma (kompadc :120-124,
+// DBLE(sigma)): out = (Re sigma, 0).
+__global__ void crt_real_sigma_kernel(const double2* __restrict__ in, int n, double2* __restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = make_double2(in[i].x, 0.0);
+}
+
+// Real parts of a complex array (the DC outputs kpotdc, hpotdc, sensdc, whose
+// imaginary parts are zero or, for sensdc, dropped as DBLE does in bsendc :184).
+__global__ void crt_real_part_kernel(const double2* __restrict__ in, size_t n, double* __restrict__ out) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = in[i].x;
+}
+
 // Electrode slice of the back-transform: he[l][m] = hpot(enr(m), l).
 __global__ void crt_rtrafo_elec_kernel(const double2* __restrict__ kpot, const double* __restrict__ kwnwi,
                                        const int* __restrict__ enr, int n, int eanz, int K, int swrtr,
