@@ -58,6 +58,14 @@ module crt_hip
        type(c_ptr), value :: kpot, hpot
      end function crt_forward
 
+     ! DC twins (ldc): kpotdc/hpotdc real, or c_null_ptr
+     integer(c_int) function crt_forward_dc(h, sigma, kpotdc, hpotdc) bind(C, name='crt_forward_dc')
+       import :: c_int, c_ptr, c_double_complex
+       type(c_ptr), value :: h
+       complex(c_double_complex), intent(in) :: sigma(*)
+       type(c_ptr), value :: kpotdc, hpotdc
+     end function crt_forward_dc
+
      integer(c_int) function crt_voltages(h, nanz, strnr, vnr, mode, kfak, volt, sigmaa) &
           bind(C, name='crt_voltages')
        import :: c_int, c_ptr, c_int32_t, c_double_complex
@@ -74,6 +82,12 @@ module crt_hip
        integer(c_int32_t), value :: mode
        type(c_ptr), value :: sens
      end function crt_sensitivities
+
+     integer(c_int) function crt_sensitivities_dc(h, sensdc) bind(C, name='crt_sensitivities_dc')
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: h
+       real(c_double), intent(out) :: sensdc(*)
+     end function crt_sensitivities_dc
 
      integer(c_int) function crt_adjoint_gradient(h, mode, w, resid, grad) &
           bind(C, name='crt_adjoint_gradient')
