@@ -66,6 +66,14 @@ int crt_set_wavenumbers(crt_handle *h, int32_t kwnanz, const double *kwn,
 /* Element and mixed-boundary pre-computation on the device, src/precal.f90:13-222
  * (elem1/3/5/8, beta with the reference's Bessel polynomials, bsytop). */
 int crt_precal(crt_handle *h, int32_t *lbeta, int32_t *lrandb);
+/* Prescribed boundary potentials (lrandb2: rrandb's rwdanz, rwdnr, rwd,
+ * src/rrandb.f90), applied by randb2 (src/randb2.f90) after kompab/randb in
+ * every assembled system: unit row and column at each node, b = -value there,
+ * and the coupling terms moved to the right-hand side.  rwd: rwdanz complex
+ * values (the DC twins take the real parts, as randbdc2 reads rwddc).
+ * rwdanz = 0 switches them off.  Returns 30 for a node outside 1..sanz. */
+int crt_set_boundary_values(crt_handle *h, int32_t rwdanz, const int32_t *rwdnr, const double *rwd);
+
 /* Copy the precomputed arrays back in the reference layout (for checking):
  * elbg elanz x nimn x K, relbg relanz x nimn, kg relanz x eanz x K (column major,
  * nimn = maxnodes(maxnodes+1)/2).  Any pointer may be NULL. */

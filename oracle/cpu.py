@@ -21,7 +21,8 @@ class OrcMesh(C.Structure):
                 ("typ", C.c_void_p), ("nelanz", C.c_void_p), ("selanz", C.c_void_p),
                 ("nrel", C.c_void_p), ("rnr", C.c_void_p), ("snr", C.c_void_p),
                 ("sx", C.c_void_p), ("sy", C.c_void_p), ("enr", C.c_void_p),
-                ("lsytop", C.c_int), ("lsink", C.c_int), ("nsink", C.c_int)]
+                ("lsytop", C.c_int), ("lsink", C.c_int), ("nsink", C.c_int),
+                ("rwdanz", C.c_int), ("rwdnr", C.c_void_p), ("rwd", C.c_void_p)]
 
 
 def build(force=False):
@@ -105,11 +106,20 @@ class Oracle:
         self.cm = OrcMesh(mesh.sanz, mesh.typanz, mesh.mb, mesh.eanz,
                           *(k[n].ctypes.data for n in ("typ", "nelanz", "selanz", "nrel", "rnr",
                                                        "snr", "sx", "sy", "enr")),
-                          int(lsytop), int(lsink), int(nsink))
+                          int(lsytop), int(lsink), int(nsink), 0, None, None)
         self.elanz = mesh.elanz
         self.relanz = mesh.relanz
         self.nimn = mesh.maxnodes * (mesh.maxnodes + 1) // 2
         self.swrtr = 1
+
+    def set_boundary_values(self, rwdnr, rwd):
+        """Prescribed boundary values (rrandb's rwdnr/rwd, applied by randb2 /
+        randbdc2 after randb); empty arrays switch them off."""
+        self._keep["rwdnr"] = np.ascontiguousarray(rwdnr, np.int32)
+        self._keep["rwd"] = np.ascontiguousarray(rwd, np.complex128)
+        self.cm.rwdanz = len(self._keep["rwdnr"])
+        self.cm.rwdnr = self._keep["rwdnr"].ctypes.data if self.cm.rwdanz else None
+        self.cm.rwd = self._keep["rwd"].ctypes.data if self.cm.rwdanz else None
 
     # --- A1 -----------------------------------------------------------------
     def rwaven(self, strnr, vnr):

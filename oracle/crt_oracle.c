@@ -538,6 +538,36 @@ static void randb(const orc_mesh *m, orc_c *a, orc_c *b) {
   }
 }
 
+/* randb2 (prescribed boundary values): src/randb2.f90:28-73.  Row and column
+ * k are zeroed with unit diagonal, b(k) = -value, and the column's entries
+ * move to the right-hand side (b(j) += value * a(k,j)), node by node in the
+ * order of the boundary file. */
+static void randb2(const orc_mesh *m, orc_c *a, orc_c *b) {
+  int mb = m->mb, m1 = mb + 1, sanz = m->sanz;
+  for (int ir = 0; ir < m->rwdanz; ir++) {
+    int k = m->rwdnr[ir];
+    orc_c rw = m->rwd[ir];
+    b[k - 1] = cc(-rw.re, -rw.im);
+    int idk = k * m1;
+    a[idk - 1] = cc(1, 0);
+    if (k != 1) {
+      int ia = (1 > mb + 2 - k) ? 1 : mb + 2 - k;
+      for (int i = ia; i <= mb; i++) {
+        int j = k + i - m1, ki = idk + i - m1;
+        b[j - 1] = cadd(b[j - 1], cmul(rw, a[ki - 1]));
+        a[ki - 1] = cc(0, 0);
+      }
+    }
+    if (k == sanz) continue;
+    int ia = (1 > k - sanz + m1) ? 1 : k - sanz + m1;
+    for (int i = ia; i <= mb; i++) {
+      int j = k - i + m1, ji = (j - 1) * m1 + i;
+      b[j - 1] = cadd(b[j - 1], cmul(rw, a[ji - 1]));
+      a[ji - 1] = cc(0, 0);
+    }
+  }
+}
+
 /* scalab: src/scalab.f90:52-76 */
 static int scalab(int sanz, int mb, orc_c *a, orc_c *b, double *fak) {
   for (int i = 1; i <= sanz; i++) {
@@ -670,6 +700,7 @@ int orc_assemble(const orc_mesh *m, int K, const double *elbg, const double *rel
   int e = kompab(m, K, elbg, relbg, kg, sigma, l, k, a, b);
   if (e) return e;
   if (lrandb) randb(m, a, b);
+  if (m->rwdanz > 0) randb2(m, a, b);
   return scalab(m->sanz, m->mb, a, b, fak);
 }
 
@@ -934,6 +965,33 @@ static void randdc(const orc_mesh *m, double *a, double *b) {
   }
 }
 
+/* randbdc2: src/randbdc2.f90, randb2 on the real arrays with the real values */
+static void randbdc2(const orc_mesh *m, double *a, double *b) {
+  int mb = m->mb, m1 = mb + 1, sanz = m->sanz;
+  for (int ir = 0; ir < m->rwdanz; ir++) {
+    int k = m->rwdnr[ir];
+    double rw = m->rwd[ir].re;
+    b[k - 1] = -rw;
+    int idk = k * m1;
+    a[idk - 1] = 1.0;
+    if (k != 1) {
+      int ia = (1 > mb + 2 - k) ? 1 : mb + 2 - k;
+      for (int i = ia; i <= mb; i++) {
+        int j = k + i - m1, ki = idk + i - m1;
+        b[j - 1] = b[j - 1] + rw * a[ki - 1];
+        a[ki - 1] = 0.0;
+      }
+    }
+    if (k == sanz) continue;
+    int ia = (1 > k - sanz + m1) ? 1 : k - sanz + m1;
+    for (int i = ia; i <= mb; i++) {
+      int j = k - i + m1, ji = (j - 1) * m1 + i;
+      b[j - 1] = b[j - 1] + rw * a[ji - 1];
+      a[ji - 1] = 0.0;
+    }
+  }
+}
+
 static int scaldc(int sanz, int mb, double *a, double *b, double *fak) {
   for (int i = 1; i <= sanz; i++) {
     int idi = i * (mb + 1);
@@ -995,6 +1053,7 @@ int orc_assemble_dc(const orc_mesh *m, const double *elbg, const double *relbg, 
   int e = kompadc(m, elbg, relbg, kg, sigma, l, k, a, b);
   if (e) return e;
   if (lrandb) randdc(m, a, b);
+  if (m->rwdanz > 0) randbdc2(m, a, b);
   return scaldc(m->sanz, m->mb, a, b, fak);
 }
 

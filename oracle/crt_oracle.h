@@ -16,9 +16,12 @@ typedef struct {
   const double *sx, *sy;            /* sanz */
   const int *enr;                   /* eanz */
   int lsytop, lsink, nsink;
+  int rwdanz;                       /* prescribed boundary values (randb2), 0 = none */
+  const int *rwdnr;                 /* rwdanz nodes (1-based) */
+  const struct orc_c_ *rwd;         /* rwdanz values (complex; the DC twins take the real part) */
 } orc_mesh;
 
-typedef struct { double re, im; } orc_c;
+typedef struct orc_c_ { double re, im; } orc_c;
 
 int orc_rwaven(const orc_mesh *m, int nanz, const int *strnr, const int *vnr,
                int *kwnanz, double *kwn, double *kwnwi, double *amin, double *amax);

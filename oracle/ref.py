@@ -44,6 +44,7 @@ def lib():
         L.ref_volt.argtypes = [C.c_int, _z, _z]
         L.ref_sens.argtypes = [C.c_int, _z]
         L.ref_forward_dc.argtypes = [_z, _d, _d, C.c_int]
+        L.ref_set_randb2.argtypes = [C.c_int, _i, _z]
         L.ref_assemble_dc.argtypes = [C.c_int, C.c_int, _z, _d, _d, _d]
         L.ref_choldc.argtypes = [_d]
         L.ref_sens_dc.argtypes = [_d]
@@ -82,6 +83,12 @@ class Reference:
         self.nanz = len(strnr)
         lib().ref_load_data(self.nanz, np.ascontiguousarray(strnr, np.int32),
                             np.ascontiguousarray(vnr, np.int32))
+
+    def set_boundary_values(self, rwdnr, rwd):
+        rn = np.ascontiguousarray(rwdnr, np.int32)
+        rv = np.ascontiguousarray(rwd, np.complex128)
+        lib().ref_set_randb2(len(rn), rn if len(rn) else np.zeros(1, np.int32),
+                             rv if len(rv) else np.zeros(1, np.complex128))
 
     def rwaven(self):
         K = C.c_int()

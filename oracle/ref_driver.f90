@@ -114,6 +114,27 @@ contains
     ref_load_data = 0
   end function ref_load_data
 
+  !> Prescribed boundary values (replaces rrandb, src/rrandb.f90): rwdnr/rwd
+  !> for randb2, rwddc = their real parts for randbdc2; c_n = 0 switches
+  !> lrandb2 off.
+  integer(c_int) function ref_set_randb2(c_n, c_rwdnr, c_rwd) bind(C, name='ref_set_randb2')
+    integer(c_int), value :: c_n
+    integer(c_int), intent(in) :: c_rwdnr(*)
+    complex(c_double_complex), intent(in) :: c_rwd(*)
+    if (allocated(rwdnr)) deallocate(rwdnr)
+    if (allocated(rwd)) deallocate(rwd)
+    if (allocated(rwddc)) deallocate(rwddc)
+    rwdanz = c_n
+    lrandb2 = (c_n > 0)
+    if (c_n > 0) then
+       allocate(rwdnr(c_n), rwd(c_n), rwddc(c_n))
+       rwdnr = c_rwdnr(1:c_n)
+       rwd = c_rwd(1:c_n)
+       rwddc = dble(rwd)
+    end if
+    ref_set_randb2 = 0
+  end function ref_set_randb2
+
   !> Run the reference rwaven (src/rwaven.f90:22-171); arrays sized >= 30.
   integer(c_int) function ref_rwaven(c_kwnanz, c_kwn, c_kwnwi, c_amin, c_amax) bind(C, name='ref_rwaven')
     integer(c_int), intent(out) :: c_kwnanz
@@ -185,6 +206,7 @@ contains
        return
     end if
     if (lrandb) call randb(my_a, my_b)
+    if (lrandb2) call randb2(my_a, my_b)
     call scalab(my_a, my_b, my_fak)
     ref_assemble = errnr
     c_a(1:(mb+1)*sanz) = my_a
@@ -290,6 +312,7 @@ contains
           if (lbeta .or. l == 1) then
              call kompadc(l, k, my_a, my_b)
              if (lrandb) call randdc(my_a, my_b)
+             if (lrandb2) call randbdc2(my_a, my_b)
              call scaldc(my_a, my_b, my_fak)
              call choldc(my_a)
           else
@@ -325,6 +348,7 @@ contains
        return
     end if
     if (lrandb) call randdc(my_a, my_b)
+    if (lrandb2) call randbdc2(my_a, my_b)
     call scaldc(my_a, my_b, my_fak)
     ref_assemble_dc = errnr
     c_a(1:(mb+1)*sanz) = my_a

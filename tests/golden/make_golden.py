@@ -92,6 +92,31 @@ def synthetic_dc(w):
     print(f"wrote ref_{w.name}_dc.npz  N={len(v1)}")
 
 
+def synthetic_rb2(w, inset):
+    """Prescribed boundary potentials (randb2 / randbdc2) on the same case,
+    values from meshgen.boundary_values(w, inset=inset)."""
+    m = w.mesh
+    rn, rv = meshgen.boundary_values(w, inset=inset)
+    r = Reference(m)
+    r.load_data(w.strnr, w.vnr)
+    r.rwaven()
+    r.precal()
+    r.set_boundary_values(rn, rv)
+    a, b, fak = r.assemble(w.sigma, 0, 0)
+    kpot, hpot = r.forward(w.sigma, nthreads=8)
+    v1, sa1 = r.volt(1)
+    kd, hd = r.forward_dc(w.sigma, nthreads=8)
+    vd, _ = r.volt(1)
+    r.set_boundary_values([], [])
+    np.savez_compressed(
+        OUT / f"ref_{w.name}_rb2.npz", rwdnr=rn, rwd=rv, a00=a, b00=b, fak00=fak,
+        kpot_elec=electrode_slice(kpot, m.enr), kpot_col=kpot[:, 0, :],
+        hpot_elec=hpot[np.asarray(m.enr) - 1, :], volt=v1, sigmaa1=sa1,
+        kpotdc_elec=electrode_slice(kd, m.enr), hpotdc_elec=hd[np.asarray(m.enr) - 1, :], volt_dc=vd,
+    )
+    print(f"wrote ref_{w.name}_rb2.npz  N={len(v1)}")
+
+
 def fixture(name, swrtr, lsink=False, nsink=0):
     d = FIX / name
     m = F.read_elem(d / "elem.dat")
@@ -119,6 +144,8 @@ def main():
     synthetic(meshgen.small_dirichlet())
     synthetic_dc(meshgen.small_block())
     synthetic_dc(meshgen.small_dirichlet())
+    synthetic_rb2(meshgen.small_block(), 0)
+    synthetic_rb2(meshgen.small_dirichlet(), 1)
     fixture("minimal", 1)
     fixture("mixed_surf", 1)
     fixture("decoupling", 0, lsink=True, nsink=373)

@@ -35,7 +35,7 @@ EXPORTS = (
     "crt_create", "crt_destroy", "crt_rwaven", "crt_set_wavenumbers", "crt_precal",
     "crt_get_precal", "crt_forward", "crt_forward_dev", "crt_voltages", "crt_voltages_dev",
     "crt_sensitivities", "crt_adjoint_gradient", "crt_adjoint_gradient_dev", "crt_jhw_r",
-    "crt_cg_solve", "crt_forward_dc", "crt_sensitivities_dc",
+    "crt_cg_solve", "crt_forward_dc", "crt_sensitivities_dc", "crt_set_boundary_values",
     "crt_error_info", "crt_set_timing", "crt_stage_times", "crt_factor_phase_ticks",
     "crt_kpot_device",
 )
@@ -88,6 +88,7 @@ def load_library(path: Path = LIB_PATH):
     L.crt_forward.argtypes = [vp, vp, vp, vp]
     L.crt_forward_dev.argtypes = [vp, vp]
     L.crt_forward_dc.argtypes = [vp, vp, vp, vp]
+    L.crt_set_boundary_values.argtypes = [vp, i32, vp, vp]
     L.crt_sensitivities_dc.argtypes = [vp, vp]
     L.crt_voltages.argtypes = [vp, i32, vp, vp, i32, vp, vp, vp]
     L.crt_voltages_dev.argtypes = [vp, i32, vp, vp, i32, vp, vp]
@@ -194,6 +195,15 @@ class CRT:
         kg = np.zeros(max(self.relanz, 1) * E * K)
         self._check("get_precal", self.lib.crt_get_precal(self.h, _ptr(elbg), _ptr(relbg), _ptr(kg)))
         return elbg, relbg, kg
+
+    def set_boundary_values(self, rwdnr, rwd):
+        """Prescribed boundary potentials for randb2 (rrandb's rwdnr/rwd);
+        empty arrays switch them off."""
+        rn = np.ascontiguousarray(rwdnr, np.int32)
+        rv = np.ascontiguousarray(rwd, np.complex128)
+        self._check("set_boundary_values",
+                    self.lib.crt_set_boundary_values(self.h, len(rn), _ptr(rn) if len(rn) else None,
+                                                     _ptr(rv) if len(rv) else None))
 
     # A4..A11 --------------------------------------------------------------
     def forward(self, sigma, want_kpot=False, want_hpot=False):

@@ -36,6 +36,9 @@ struct crt_handle {
   std::vector<double> sx, sy;
   int lsytop = 0, lsink = 0, nsink = 0;
   bool has11 = false, has13 = false;
+  // prescribed boundary values (randb2): nodes (1-based) and complex values
+  std::vector<int> rwdnr;
+  std::vector<double2> rwd;
   double sytop = 0.0;
   // wavenumbers
   int K = 0, swrtr = 1;
@@ -255,6 +258,167 @@ double node_y(const crt_handle* h, int node1) { return h->sy[h->snr[node1 - 1] -
 
 extern "C" {
 
+// Row-assembly plan (kompab's accumulation, src/kompab.f90:79-129), with the
+// Dirichlet rows of randb (type-13 nodes) and of randb2 (prescribed nodes) as
+// unit rows whose couplings are dropped, and for randb2 the terms that move to
+// the right-hand side: b(j) += value_k * a(k,j) for every free node j coupled
+// to a prescribed node k (src/randb2.f90:41-71), grouped by j in the order of
+// the boundary file, each a(k,j) as the list of its element terms in element
+// order.  Uploads the plan arrays; rebuilt when the boundary values change.
+static int build_plan(crt_handle* h) {
+  struct Term {
+    int row, t, e, imn;
+  };
+  std::vector<Term> terms;
+  terms.reserve((size_t)h->nel * 10);
+  std::vector<uint8_t> is13(h->n, 0);
+  int iel = 0;
+  for (int ti = 0; ti < h->typanz; ++ti) {
+    const int ty = h->typ[ti], nk = h->selanz[ti];
+    for (int j = 0; j < h->nelanz[ti]; ++j, ++iel) {
+      if (ty == 13) {
+        for (int q = 0; q < nk; ++q) is13[h->nrel[iel + (size_t)q * h->nel] - 1] = 1;
+      }
+      if (ty > 11) continue;
+      int ikl = 0;
+      for (int kk = 0; kk < nk; ++kk) {
+        const int nzp = h->nrel[iel + (size_t)kk * h->nel];
+        for (int ll = 0; ll <= kk; ++ll, ++ikl) {
+          const int nnp = h->nrel[iel + (size_t)ll * h->nel];
+          if (std::abs(nzp - nnp) > h->mb) return 19;
+          const int row = std::max(nzp, nnp) - 1, col = std::min(nzp, nnp) - 1;
+          const int code = (ty == 11) ? -((iel - h->elanz) + 1) : iel;
+          terms.push_back({row, col - row + h->mb, code, ikl});
+        }
+      }
+    }
+  }
+  // randb2: first and last position of each prescribed node in the file
+  const int nrw = (int)h->rwdnr.size();
+  std::vector<int> first_ir(h->n, -1), last_ir(h->n, -1);
+  for (int ir = 0; ir < nrw; ++ir) {
+    const int k = h->rwdnr[ir] - 1;
+    if (first_ir[k] < 0) first_ir[k] = ir;
+    last_ir[k] = ir;
+  }
+  std::vector<uint8_t> unit(h->n, 0);
+  for (int r = 0; r < h->n; ++r) unit[r] = is13[r] || last_ir[r] >= 0;
+  // moved terms: a(k,j) with k prescribed (and not zeroed before by randb)
+  // and j free, used at k's first position in the file
+  struct Move {
+    int j, ir, e, imn, seq;
+  };
+  std::vector<Move> moves;
+  if (nrw > 0) {
+    int seq = 0;
+    for (const Term& t : terms) {
+      const int r = t.row, c = t.row - h->mb + t.t;
+      if (r == c) continue;
+      const int ends[2][2] = {{r, c}, {c, r}};
+      for (const auto& kj : ends) {
+        const int k = kj[0], j = kj[1];
+        if (first_ir[k] >= 0 && !is13[k] && !unit[j]) moves.push_back({j, first_ir[k], t.e, t.imn, seq++});
+      }
+    }
+    std::stable_sort(moves.begin(), moves.end(), [](const Move& a, const Move& b) {
+      return a.j != b.j ? a.j < b.j : (a.ir != b.ir ? a.ir < b.ir : a.seq < b.seq);
+    });
+  }
+  {
+    std::vector<Term> kept;
+    kept.reserve(terms.size());
+    for (const Term& t : terms) {
+      const int col = t.row - h->mb + t.t;
+      if (unit[t.row] || unit[col]) continue;
+      kept.push_back(t);
+    }
+    for (int r = 0; r < h->n; ++r)
+      if (unit[r]) kept.push_back({r, h->mb, kDirichletUnit, 0});
+    terms.swap(kept);
+  }
+  std::stable_sort(terms.begin(), terms.end(), [](const Term& a, const Term& b) {
+    return a.row != b.row ? a.row < b.row : a.t < b.t;
+  });
+  // row pattern (distinct band offsets, ascending) and the split into area
+  // terms (summed once per wavenumber) and mixed-boundary terms (per source)
+  std::vector<std::vector<int>> pat(h->n);
+  for (const Term& t : terms)
+    if (pat[t.row].empty() || pat[t.row].back() != t.t) pat[t.row].push_back(t.t);
+  for (int r = 0; r < h->n; ++r)
+    if (pat[r].empty() || pat[r].back() != h->mb) pat[r].push_back(h->mb);  // diagonal slot
+  int R = 1;
+  for (auto& v : pat) R = std::max<int>(R, (int)v.size());
+  if (R > kMaxR) return kErrBand;
+  h->R = R;
+  std::vector<int16_t> pat_t((size_t)h->n * R, -1);
+  std::vector<int8_t> pat_diag(h->n, 0);
+  for (int r = 0; r < h->n; ++r)
+    for (size_t q = 0; q < pat[r].size(); ++q) {
+      pat_t[(size_t)r * R + q] = (int16_t)pat[r][q];
+      if (pat[r][q] == h->mb) pat_diag[r] = (int8_t)q;
+    }
+  std::vector<int> rowptr(h->n + 1, 0), te, bptr(h->n + 1, 0), bcode;
+  std::vector<int16_t> tt;
+  std::vector<int8_t> timn, bslot, bimn;
+  for (const Term& t : terms) {
+    if (t.e < 0 && t.e != kDirichletUnit) {
+      const auto& pr = pat[t.row];
+      bptr[t.row + 1]++;
+      bslot.push_back((int8_t)(std::find(pr.begin(), pr.end(), t.t) - pr.begin()));
+      bcode.push_back(t.e);
+      bimn.push_back((int8_t)t.imn);
+    } else {
+      rowptr[t.row + 1]++;
+      tt.push_back((int16_t)t.t);
+      timn.push_back((int8_t)t.imn);
+      te.push_back(t.e);
+    }
+  }
+  for (int r = 0; r < h->n; ++r) {
+    rowptr[r + 1] += rowptr[r];
+    bptr[r + 1] += bptr[r];
+  }
+  if (te.empty()) { tt.push_back(0); timn.push_back(0); te.push_back(0); }
+  if (bcode.empty()) { bslot.push_back(0); bcode.push_back(-1); bimn.push_back(0); }
+  int* di;
+  int16_t* d16;
+  int8_t* d8;
+  uint8_t* du;
+  int rc = 0;
+  rc |= upload(h, "rowptr", rowptr.data(), rowptr.size(), &di);
+  rc |= upload(h, "term_t", tt.data(), tt.size(), &d16);
+  rc |= upload(h, "term_imn", timn.data(), timn.size(), &d8);
+  rc |= upload(h, "term_e", te.data(), te.size(), &di);
+  rc |= upload(h, "pat_t", pat_t.data(), pat_t.size(), &d16);
+  rc |= upload(h, "pat_diag", pat_diag.data(), pat_diag.size(), &d8);
+  rc |= upload(h, "bptr", bptr.data(), bptr.size(), &di);
+  rc |= upload(h, "bslot", bslot.data(), bslot.size(), &d8);
+  rc |= upload(h, "bcode", bcode.data(), bcode.size(), &di);
+  rc |= upload(h, "bimn", bimn.data(), bimn.size(), &d8);
+  if (h->has13) rc |= upload(h, "isdir", is13.data(), is13.size(), &du);
+  if (nrw > 0) {
+    std::vector<int> mv_ptr(h->n + 1, 0), mv_ir, mv_e;
+    std::vector<int8_t> mv_imn;
+    for (const Move& mv : moves) {
+      mv_ptr[mv.j + 1]++;
+      mv_ir.push_back(mv.ir);
+      mv_e.push_back(mv.e);
+      mv_imn.push_back((int8_t)mv.imn);
+    }
+    for (int r = 0; r < h->n; ++r) mv_ptr[r + 1] += mv_ptr[r];
+    if (mv_ir.empty()) { mv_ir.push_back(0); mv_e.push_back(0); mv_imn.push_back(0); }
+    double2* dz;
+    rc |= upload(h, "mv_ptr", mv_ptr.data(), mv_ptr.size(), &di);
+    rc |= upload(h, "mv_ir", mv_ir.data(), mv_ir.size(), &di);
+    rc |= upload(h, "mv_e", mv_e.data(), mv_e.size(), &di);
+    rc |= upload(h, "mv_imn", mv_imn.data(), mv_imn.size(), &d8);
+    rc |= upload(h, "rb_last", last_ir.data(), last_ir.size(), &di);
+    rc |= upload(h, "rwd", h->rwd.data(), h->rwd.size(), &dz);
+  }
+  if (rc || hipStreamSynchronize(h->stream) != hipSuccess) return kErrHip;
+  return 0;
+}
+
 int crt_create(crt_handle** out, const crt_mesh* m) {
   if (!out || !m || m->sanz <= 0 || m->typanz <= 0 || m->mb <= 0 || m->eanz < 0) return kErrArg;
   crt_handle* h = new crt_handle();
@@ -326,101 +490,17 @@ int crt_create(crt_handle** out, const crt_mesh* m) {
     return kErrHip;
   }
 
-  // ---- row-assembly plan (kompab's accumulation, src/kompab.f90:79-129) ----
-  struct Term {
-    int row, t, e, imn;
-  };
-  std::vector<Term> terms;
-  terms.reserve((size_t)h->nel * 10);
-  std::vector<uint8_t> isdir(h->n, 0);
-  int iel = 0;
-  for (int ti = 0; ti < h->typanz; ++ti) {
-    const int ty = h->typ[ti], nk = h->selanz[ti];
-    for (int j = 0; j < h->nelanz[ti]; ++j, ++iel) {
-      if (ty == 13) {
-        for (int q = 0; q < nk; ++q) isdir[h->nrel[iel + (size_t)q * h->nel] - 1] = 1;
-      }
-      if (ty > 11) continue;
-      int ikl = 0;
-      for (int kk = 0; kk < nk; ++kk) {
-        const int nzp = h->nrel[iel + (size_t)kk * h->nel];
-        for (int ll = 0; ll <= kk; ++ll, ++ikl) {
-          const int nnp = h->nrel[iel + (size_t)ll * h->nel];
-          if (std::abs(nzp - nnp) > h->mb) {
-            crt_destroy(h);
-            return 19;
-          }
-          const int row = std::max(nzp, nnp) - 1, col = std::min(nzp, nnp) - 1;
-          const int code = (ty == 11) ? -((iel - h->elanz) + 1) : iel;
-          terms.push_back({row, col - row + h->mb, code, ikl});
-        }
-      }
+  {
+    const int rc = build_plan(h);
+    if (rc) {
+      crt_destroy(h);
+      return rc;
     }
   }
-  if (h->has13) {
-    std::vector<Term> kept;
-    kept.reserve(terms.size());
-    for (const Term& t : terms) {
-      const int col = t.row - h->mb + t.t;
-      if (isdir[t.row] || isdir[col]) continue;
-      kept.push_back(t);
-    }
-    for (int r = 0; r < h->n; ++r)
-      if (isdir[r]) kept.push_back({r, h->mb, kDirichletUnit, 0});
-    terms.swap(kept);
-  }
-  std::stable_sort(terms.begin(), terms.end(), [](const Term& a, const Term& b) {
-    return a.row != b.row ? a.row < b.row : a.t < b.t;
-  });
-  // row pattern (distinct band offsets, ascending) and the split into area
-  // terms (summed once per wavenumber) and mixed-boundary terms (per source)
-  std::vector<std::vector<int>> pat(h->n);
-  for (const Term& t : terms)
-    if (pat[t.row].empty() || pat[t.row].back() != t.t) pat[t.row].push_back(t.t);
-  for (int r = 0; r < h->n; ++r)
-    if (pat[r].empty() || pat[r].back() != h->mb) pat[r].push_back(h->mb);  // diagonal slot
-  int R = 1;
-  for (auto& v : pat) R = std::max<int>(R, (int)v.size());
-  if (R > kMaxR) {
-    crt_destroy(h);
-    return kErrBand;
-  }
-  h->R = R;
-  std::vector<int16_t> pat_t((size_t)h->n * R, -1);
-  std::vector<int8_t> pat_diag(h->n, 0);
-  for (int r = 0; r < h->n; ++r)
-    for (size_t q = 0; q < pat[r].size(); ++q) {
-      pat_t[(size_t)r * R + q] = (int16_t)pat[r][q];
-      if (pat[r][q] == h->mb) pat_diag[r] = (int8_t)q;
-    }
-  std::vector<int> rowptr(h->n + 1, 0), te, bptr(h->n + 1, 0), bcode;
-  std::vector<int16_t> tt;
-  std::vector<int8_t> timn, bslot, bimn;
-  for (const Term& t : terms) {
-    if (t.e < 0 && t.e != kDirichletUnit) {
-      const auto& pr = pat[t.row];
-      bptr[t.row + 1]++;
-      bslot.push_back((int8_t)(std::find(pr.begin(), pr.end(), t.t) - pr.begin()));
-      bcode.push_back(t.e);
-      bimn.push_back((int8_t)t.imn);
-    } else {
-      rowptr[t.row + 1]++;
-      tt.push_back((int16_t)t.t);
-      timn.push_back((int8_t)t.imn);
-      te.push_back(t.e);
-    }
-  }
-  for (int r = 0; r < h->n; ++r) {
-    rowptr[r + 1] += rowptr[r];
-    bptr[r + 1] += bptr[r];
-  }
-  if (te.empty()) { tt.push_back(0); timn.push_back(0); te.push_back(0); }
-  if (bcode.empty()) { bslot.push_back(0); bcode.push_back(-1); bimn.push_back(0); }
-
   // ---- element / boundary / node arrays for the kernels ----
   std::vector<int> el_type(h->elanz), el_nodes((size_t)h->elanz * 4), el_nk(h->elanz);
   std::vector<int> bnd_type(std::max(h->relanz, 1)), bnd_nodes((size_t)std::max(h->relanz, 1) * 2);
-  iel = 0;
+  int iel = 0;
   for (int ti = 0; ti < h->typanz; ++ti) {
     const int ty = h->typ[ti], nk = h->selanz[ti];
     for (int j = 0; j < h->nelanz[ti]; ++j, ++iel) {
@@ -446,21 +526,8 @@ int crt_create(crt_handle** out, const crt_mesh* m) {
   for (int i = 0; i < h->relanz; ++i) rnr0[i] = h->rnr[i] - 1;
 
   int* di;
-  int16_t* d16;
-  int8_t* d8;
   double* dd;
-  uint8_t* du;
   int rc = 0;
-  rc |= upload(h, "rowptr", rowptr.data(), rowptr.size(), &di);
-  rc |= upload(h, "term_t", tt.data(), tt.size(), &d16);
-  rc |= upload(h, "term_imn", timn.data(), timn.size(), &d8);
-  rc |= upload(h, "term_e", te.data(), te.size(), &di);
-  rc |= upload(h, "pat_t", pat_t.data(), pat_t.size(), &d16);
-  rc |= upload(h, "pat_diag", pat_diag.data(), pat_diag.size(), &d8);
-  rc |= upload(h, "bptr", bptr.data(), bptr.size(), &di);
-  rc |= upload(h, "bslot", bslot.data(), bslot.size(), &d8);
-  rc |= upload(h, "bcode", bcode.data(), bcode.size(), &di);
-  rc |= upload(h, "bimn", bimn.data(), bimn.size(), &d8);
   rc |= upload(h, "el_type", el_type.data(), el_type.size(), &di);
   rc |= upload(h, "el_nodes", el_nodes.data(), el_nodes.size(), &di);
   rc |= upload(h, "el_nk", el_nk.data(), el_nk.size(), &di);
@@ -470,7 +537,6 @@ int crt_create(crt_handle** out, const crt_mesh* m) {
   rc |= upload(h, "ny", ny.data(), ny.size(), &dd);
   rc |= upload(h, "enr", enr0.data(), enr0.size(), &di);
   rc |= upload(h, "rnr", rnr0.data(), rnr0.size(), &di);
-  if (h->has13) rc |= upload(h, "isdir", isdir.data(), isdir.size(), &du);
   rc |= clear_err(h);
   if (rc || hipStreamSynchronize(h->stream) != hipSuccess) {
     crt_destroy(h);
@@ -636,6 +702,19 @@ int crt_precal(crt_handle* h, int32_t* lbeta, int32_t* lrandb) {
   return 0;
 }
 
+int crt_set_boundary_values(crt_handle* h, int32_t rwdanz, const int32_t* rwdnr, const double* rwd) {
+  if (!h || rwdanz < 0 || (rwdanz > 0 && (!rwdnr || !rwd))) return kErrArg;
+  for (int i = 0; i < rwdanz; ++i)
+    if (rwdnr[i] < 1 || rwdnr[i] > h->n) return 30;  // rrandb errnr 30
+  h->rwdnr.assign(rwdnr, rwdnr + rwdanz);
+  h->rwd.resize(rwdanz);
+  for (int i = 0; i < rwdanz; ++i) h->rwd[i] = make_double2(rwd[2 * i], rwd[2 * i + 1]);
+  const int rc = build_plan(h);
+  if (rc) return rc;
+  h->forward_done = h->volt_done = h->sens_done = false;
+  return 0;
+}
+
 int crt_get_precal(crt_handle* h, double* elbg, double* relbg, double* kg) {
   if (!h) return kErrArg;
   if (!h->precal_done) return kErrOrder;
@@ -737,8 +816,9 @@ static int run_forward(crt_handle* h) {
   A.sys_stride = reuse ? E : 1;
   // factors per batch: the scaled rows, L, y and the scaling factors of every
   // factor in the batch stay in device memory until its back substitution
+  const bool rb2 = !h->rwdnr.empty();
   const size_t per_sys = (size_t)n * (mb + 1) * 16 + (size_t)n * h->R * 16 + (size_t)n * 24 +
-                         (reuse ? (size_t)E * n * 16 : 0);
+                         (reuse ? (size_t)E * n * 16 : 0) + (rb2 ? (size_t)n * 16 : 0);
   size_t free_b = 0, total_b = 0;
   if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) free_b = (size_t)8 << 30;
   size_t budget = std::min<size_t>(free_b / 2, h->batch_bytes_max);
@@ -782,6 +862,22 @@ static int run_forward(crt_handle* h) {
   F.lsink = A.lsink;
   F.nsink = A.nsink;
   F.y = yr;
+  // prescribed boundary values: one assembled right-hand side per system
+  A.mv_ptr = A.mv_ir = A.mv_e = A.rb_last = nullptr;
+  A.mv_imn = nullptr;
+  A.rwd = nullptr;
+  A.rhs = nullptr;
+  if (rb2) {
+    A.mv_ptr = h->at<int>("mv_ptr");
+    A.mv_ir = h->at<int>("mv_ir");
+    A.mv_e = h->at<int>("mv_e");
+    A.mv_imn = h->at<int8_t>("mv_imn");
+    A.rb_last = h->at<int>("rb_last");
+    A.rwd = h->at<double2>("rwd");
+    A.rhs = h->get<double2>("rhs", (size_t)batch * n);
+    NEED(A.rhs);
+  }
+  F.rhs = A.rhs;
   for (int first = 0; first < nfac; first += batch) {
     const int cnt = std::min(batch, nfac - first);
     A.sys_first = first;
@@ -790,6 +886,7 @@ static int run_forward(crt_handle* h) {
     tstart(h, "assemble_scale");
     hipLaunchKernelGGL(crt_scale_kernel, dim3(rgrid), dim3(256), 0, h->stream, A);
     hipLaunchKernelGGL(crt_rows_kernel, dim3(rgrid), dim3(256), 0, h->stream, A);
+    if (rb2) hipLaunchKernelGGL(crt_rhs_kernel, dim3(rgrid), dim3(256), 0, h->stream, A);
     tstop(h);
     h->slots = std::min(cnt, h->ncu * per_cu);
     tstart(h, "factor_solve");

@@ -260,6 +260,46 @@ __global__ void crt_rows_kernel(FactorArgs A) {
   }
 }
 
+// Right-hand side of every system of the batch when boundary values are
+// prescribed: kompab's source and sink (:162, :183; the sink wins a tie), zero
+// on type-13 rows (randb :49), then randb2 (src/randb2.f90:30-71): -value on a
+// prescribed row, and on a free row the moved terms value_k * a(k,j) in file
+// order, each a(k,j) summed from its element terms in element order; scaled by
+// f (scalab :65).  The DC twin (randbdc2) takes the real parts of the values.
+__global__ void crt_rhs_kernel(FactorArgs A) {
+  const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (long)A.nsys * A.n) return;
+  const int s = (int)(idx / A.n), r = (int)(idx % A.n);
+  const int sys = (A.sys_first + s) * A.sys_stride;
+  const int k = sys / A.eanz, l = sys % A.eanz;
+  const int rel = A.enr[l], rsink = A.lsink ? A.nsink : -1;
+  cplx b = mkc(r == rsink ? 1.0 : (r == rel ? -1.0 : 0.0), 0.0);
+  if (A.isdir && A.isdir[r]) b = mkc(0.0, 0.0);
+  auto value = [&](int ir) {
+    const cplx v = ldc(A.rwd + ir);
+    return A.dc ? mkc(v.re, 0.0) : v;
+  };
+  const int last = A.rb_last[r];
+  if (last >= 0) {
+    const cplx v = value(last);
+    b = mkc(-v.re, -v.im);
+  } else {
+    int p = A.mv_ptr[r];
+    const int p1 = A.mv_ptr[r + 1];
+    while (p < p1) {
+      const int ir = A.mv_ir[p];
+      cplx a = mkc(0.0, 0.0);
+      for (; p < p1 && A.mv_ir[p] == ir; ++p) {
+        const int e = A.mv_e[p];
+        if (e >= 0) a = cadd(a, rmul(A.elbg[((size_t)k * A.elanz + e) * A.nimn + A.mv_imn[p]], ldc(A.sigma + e)));
+        else a = cadd(a, bnd_term(A, k, l, e, A.mv_imn[p]));
+      }
+      b = cadd(b, cmul(value(ir), a));
+    }
+  }
+  stc(A.rhs + (size_t)s * A.n + r, rmul(A.fak[(size_t)s * A.n + r], b));
+}
+
 // ---------------------------------------------------------------------------
 // Lookahead band factorisation.  ROWS: panel rows per lane of wave 0
 // (W = mb + kNB <= 64 ROWS); MAXT: trailing tiles per thread of waves 1-3.
@@ -347,7 +387,12 @@ __global__ __launch_bounds__(256, 2) void crt_factor_kernel(FactorArgs A) {
       if (!(A.isdir && A.isdir[rel])) be = -fk[rel];
       if (rsink >= 0 && !(A.isdir && A.isdir[rsink])) bs = fk[rsink];
     }
-    auto rhs = [&](int r) -> cplx { return mkc(r == rsink ? bs : (r == rel ? be : 0.0), 0.0); };
+    // prescribed boundary values: the right-hand side comes from crt_rhs_kernel
+    const double2* Bs = A.rhs ? A.rhs + (size_t)s * n : nullptr;
+    auto rhs = [&](int r) -> cplx {
+      if (Bs) return ldc(Bs + min(max(r, 0), n - 1));
+      return mkc(r == rsink ? bs : (r == rel ? be : 0.0), 0.0);
+    };
     cplx yv[ROWS];
 #pragma unroll
     for (int q = 0; q < ROWS; ++q) {
@@ -718,17 +763,22 @@ __global__ __launch_bounds__(64) void crt_fwdsweep_kernel(FwdArgs F) {
   const double2* Lg = F.L + (size_t)f * n * m1;
   const double* fak = F.fak + (size_t)f * n;
   double2* yg = F.y + (size_t)r * n;
-  // b = -f at the source electrode, +f at the sink (the sink wins a tie),
-  // zero on Dirichlet rows (kompb, randb)
+  // b = -f at the source electrode, +f at the sink (the sink wins a tie).
+  // Source 0 is assembled with the matrix (kompab, randb: zero on Dirichlet
+  // rows; with prescribed values the whole vector comes from crt_rhs_kernel);
+  // the others take kompb's vector (src/kompb.f90:39-59), which leaves
+  // Dirichlet rows as they are.
   const int re = F.enr[l], rs = F.lsink ? F.nsink : -1;
+  const double2* B0 = (l == 0 && F.rhs) ? F.rhs + (size_t)f * n : nullptr;
   auto bval = [&](int i) -> cplx {
+    if (B0) return ldc(B0 + i);
     double v = 0.0;
     if (i == rs) v = 1.0;
     else if (i == re) v = -1.0;
-    if (F.isdir && F.isdir[i]) v = 0.0;
+    if (l == 0 && F.isdir && F.isdir[i]) v = 0.0;
     return mkc(v * fak[i], 0.0);
   };
-  const int j0 = rs >= 0 ? min(re, rs) : re;
+  const int j0 = B0 ? 0 : (rs >= 0 ? min(re, rs) : re);
   for (int i = lane; i < j0; i += 64) yg[i] = make_double2(0.0, 0.0);
   // class q = lane + 64*sl holds row i(j, q) = j + ((q - j) mod P) at step j
   cplx acc[SLOTS];

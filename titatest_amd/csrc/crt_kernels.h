@@ -49,6 +49,14 @@ struct FactorArgs {
   double2* L;               // [nsys][n][mb+1]
   double2* y;               // [nsys][n]
   int dc;                   // DC twins (scaldc, choldc): real sigma, a_ii <= 0 -> 27, pivot <= 0 -> 28
+  // prescribed boundary values (randb2), all null when there are none
+  const int* mv_ptr;        // n+1: moved terms of each free row (CSR)
+  const int* mv_ir;         // position of the prescribed node in the boundary file
+  const int* mv_e;          // area element, or -(boundary element + 1)
+  const int8_t* mv_imn;     // packed element-matrix index
+  const int* rb_last;       // n: last file position of a prescribed node, -1 otherwise
+  const double2* rwd;       // file values
+  double2* rhs;             // [nsys][n] scaled right-hand sides (crt_rhs_kernel), or null
   int* err;                 // first error code (0 = ok) and the failing system
   unsigned long long* prof; // optional: 8 phase-time accumulators (wall clock ticks)
 };
@@ -72,12 +80,14 @@ struct FwdArgs {
   const int* enr;
   const uint8_t* isdir;
   int lsink, nsink;
+  const double2* rhs;       // [factor][n] scaled right-hand side of source 0 (randb2), or null
   double2* y;               // [rhs][n], rhs r = factor r / eanz, source r % eanz
 };
 
 __global__ void crt_area_values_kernel(FactorArgs A, int K, double2* av);
 __global__ void crt_scale_kernel(FactorArgs A);
 __global__ void crt_rows_kernel(FactorArgs A);
+__global__ void crt_rhs_kernel(FactorArgs A);
 template <int ROWS, int MAXT>
 __global__ void crt_factor_kernel(FactorArgs A);
 template <int DEPTH, int G>

@@ -203,6 +203,23 @@ def small_dirichlet() -> Workload:
     return Workload("small_dirichlet", mesh, w.mag, w.pha_mrad, w.strnr, w.vnr)
 
 
+def boundary_values(w: Workload, every=4, seed=7, inset=0):
+    """Prescribed potentials for randb2 (the rwdnr/rwd arrays rrandb reads,
+    src/rrandb.f90): every ``every``-th node of the bottom boundary and the
+    three lowest nodes of the left side of a column-numbered quad mesh (moved
+    ``inset`` node rows up and columns right), with complex values of the
+    order of the source potentials there."""
+    m = w.mesh
+    ncol = int(round(m.sanz / (len(np.unique(m.sx)))))   # nodes per column
+    nxn = m.sanz // ncol
+    bottom = [ix * ncol + ncol - inset for ix in range(inset, nxn - inset, every)]
+    left = [inset * ncol + ncol - 1 - inset - j for j in range(1, 4)]
+    rwdnr = np.array(bottom + left, np.int32)
+    rng = np.random.default_rng(seed)
+    rwd = 0.05 * rng.uniform(0.5, 1.5, len(rwdnr)) * np.exp(1j * rng.uniform(-0.3, 0.3, len(rwdnr)))
+    return rwdnr, rwd.astype(np.complex128)
+
+
 def _graded_axis(n_uniform, dx, n_side, factor):
     """Node coordinates: n_uniform cells of width dx, n_side cells growing by
     `factor` on each side."""
