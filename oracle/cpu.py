@@ -65,6 +65,10 @@ def lib():
         L.orc_bsmatmreg.argtypes = [P, C.c_int, C.c_int, C.c_double, C.c_double, _d]
         L.orc_bsmatmreg.restype = None
         L.orc_rm_reg.argtypes = [C.c_int, C.c_int, _d, _z, _z]
+        L.orc_bnachbar.argtypes = [P, _i]
+        L.orc_bnachbar.restype = None
+        L.orc_bsmatmtri.argtypes = [P, C.c_double, C.c_double, _i, _d]
+        L.orc_bsmatmtri.restype = None
         L.orc_rm_reg.restype = None
         L.orc_assemble_dc.argtypes = [P, _d, _d, _d, C.c_int, _z, C.c_int, C.c_int, _d, _d, _d]
         L.orc_choldc.argtypes = [C.c_int, C.c_int, _d]
@@ -297,6 +301,23 @@ def bsmatmreg(orc: Oracle, nx, nz, alfx, alfz):
     sm = np.zeros(orc.elanz * 3)
     lib().orc_bsmatmreg(C.byref(orc.cm), nx, nz, alfx, alfz, sm)
     return sm.reshape((orc.elanz, 3), order="F")
+
+
+def bnachbar(orc: Oracle):
+    """Edge neighbours (manz, maxnodes+1), 1-based, last column = count."""
+    mx = orc.mesh.maxnodes
+    nb = np.zeros(orc.elanz * (mx + 1), np.int32)
+    lib().orc_bnachbar(C.byref(orc.cm), nb)
+    return nb.reshape((orc.elanz, mx + 1), order="F")
+
+
+def bsmatmtri(orc: Oracle, alfx, alfz, nachbar):
+    """Triangular smoothness operator smatm (manz, maxnodes+1)."""
+    mx = orc.mesh.maxnodes
+    sm = np.zeros(orc.elanz * (mx + 1))
+    lib().orc_bsmatmtri(C.byref(orc.cm), alfx, alfz,
+                        np.asfortranarray(nachbar, np.int32).ravel(order="F"), sm)
+    return sm.reshape((orc.elanz, mx + 1), order="F")
 
 
 def rm_reg(smatm, nx, par):

@@ -1286,6 +1286,71 @@ void orc_bsmatmreg(const orc_mesh *m, int nx, int nz, double alfx, double alfz, 
 
 /* R m for the rectangular operator (the ltri = 0 branch of update,
  * src/update.f90:63-84, no prior model). */
+/* bnachbar: src/bnachbar.f90:25-66.  nachbar(i,k) = the element that shares
+ * edge k (nodes k, k+1 mod maxnodes) of element i (the last one found), and
+ * nachbar(i, maxnodes+1) = the number of edges with a neighbour; manz x
+ * (maxnodes+1) column major, 1-based element numbers, 0 = none. */
+void orc_bnachbar(const orc_mesh *m, int *nachbar) {
+  int elanz = m_elanz(m), nel = elanz + m_relanz(m), mx = m_maxnodes(m);
+  memset(nachbar, 0, sizeof(int) * (size_t)elanz * (mx + 1));
+  for (int i = 0; i < elanz; i++)
+    for (int ik = 0; ik < mx; ik++) {
+      int ik1 = NREL(m, nel, i, ik), ik2 = NREL(m, nel, i, (ik + 1) % mx);
+      for (int j = 0; j < elanz; j++) {
+        if (j == i) continue;
+        for (int jk = 0; jk < mx; jk++) {
+          int jk1 = NREL(m, nel, j, jk), jk2 = NREL(m, nel, j, (jk + 1) % mx);
+          if ((ik1 == jk1 && ik2 == jk2) || (ik1 == jk2 && ik2 == jk1)) {
+            nachbar[i + (size_t)elanz * ik] = j + 1;
+            nachbar[i + (size_t)elanz * mx] += 1;
+          }
+        }
+      }
+    }
+}
+
+/* bsmatmtri: src/bsmatm_mod.f90:364-461 without decoupled edges.  For each
+ * edge k of element i with a neighbour: w = edge length / centroid distance *
+ * sqrt((alfx cos a)^2 + (alfz sin a)^2), a the angle of the centroid
+ * difference; smatm(i,k) = -w, smatm(i,maxnodes+1) += w.  Centroids as relem
+ * sets them (src/relem.f90:134-143).  smatm: manz x (maxnodes+1). */
+void orc_bsmatmtri(const orc_mesh *m, double alfx, double alfz, const int *nachbar, double *smatm) {
+  int elanz = m_elanz(m), nel = elanz + m_relanz(m), mx = m_maxnodes(m);
+  double *ex = calloc((size_t)elanz, sizeof(double)), *ey = calloc((size_t)elanz, sizeof(double));
+  int iel = 0;
+  for (int ti = 0; ti < m->typanz; ti++)
+    for (int j = 0; j < m->nelanz[ti]; j++, iel++) {
+      if (m->typ[ti] >= 10) continue;
+      int nk = m->selanz[ti];
+      for (int k = 0; k < nk; k++) {
+        ex[iel] = ex[iel] + SX(m, NREL(m, nel, iel, k));
+        ey[iel] = ey[iel] + SY(m, NREL(m, nel, iel, k));
+      }
+      ex[iel] = ex[iel] / nk;
+      ey[iel] = ey[iel] / nk;
+    }
+  memset(smatm, 0, sizeof(double) * (size_t)elanz * (mx + 1));
+  for (int i = 0; i < elanz; i++)
+    for (int k = 0; k < mx; k++) {
+      int a = NREL(m, nel, i, k), b = NREL(m, nel, i, (k + 1) % mx);
+      double dx = SX(m, a) - SX(m, b), dy = SY(m, a) - SY(m, b);
+      double edglen = sqrt(dx * dx + dy * dy);
+      int nb = nachbar[i + (size_t)elanz * k];
+      if (nb > 0) {
+        double px = ex[i] - ex[nb - 1], py = ey[i] - ey[nb - 1];
+        double dist = sqrt(px * px + py * py);
+        double ang = atan2(py, px);
+        double cx = alfx * cos(ang), cz = alfz * sin(ang);
+        double alfgeo = sqrt(cx * cx + cz * cz);
+        double w = edglen / dist * alfgeo;
+        smatm[i + (size_t)elanz * k] = -w;
+        smatm[i + (size_t)elanz * mx] = smatm[i + (size_t)elanz * mx] + w;
+      }
+    }
+  free(ex);
+  free(ey);
+}
+
 void orc_rm_reg(int manz, int nx, const double *smatm, const orc_c *par, orc_c *out) {
   for (int i = 0; i < manz; i++) {
     orc_c cdum = cc(0, 0);

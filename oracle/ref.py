@@ -45,6 +45,7 @@ def lib():
         L.ref_sens.argtypes = [C.c_int, _z]
         L.ref_forward_dc.argtypes = [_z, _d, _d, C.c_int]
         L.ref_set_randb2.argtypes = [C.c_int, _i, _z]
+        L.ref_bsmatmtri.argtypes = [C.c_double, C.c_double, _d, _i]
         L.ref_assemble_dc.argtypes = [C.c_int, C.c_int, _z, _d, _d, _d]
         L.ref_choldc.argtypes = [_d]
         L.ref_sens_dc.argtypes = [_d]
@@ -192,6 +193,16 @@ class Reference:
         if e:
             raise RefError("choldc", e)
         return a
+
+    def bsmatmtri(self, alfx, alfz):
+        """bnachbar + bsmatm(ltri=1): smatm and nachbar, (manz, maxnodes+1)."""
+        mx = self.mesh.maxnodes
+        sm = np.zeros(self.elanz * (mx + 1))
+        nb = np.zeros(self.elanz * (mx + 1), np.int32)
+        e = lib().ref_bsmatmtri(alfx, alfz, sm, nb)
+        if e:
+            raise RefError("bsmatmtri", e)
+        return sm.reshape((self.elanz, mx + 1), order="F"), nb.reshape((self.elanz, mx + 1), order="F")
 
     def sens_dc(self):
         out = np.zeros(self.nanz * self.elanz)

@@ -374,6 +374,52 @@ contains
     ref_sens_dc = errnr
   end function ref_sens_dc
 
+  !> Neighbours and the triangular smoothness operator: bnachbar
+  !> (src/bnachbar.f90) and bsmatm with ltri = 1, i.e. bsmatmtri
+  !> (src/bsmatm_mod.f90:364-461), with the element centroids as relem sets
+  !> them (src/relem.f90:134-143) and no decoupled edges.
+  !> smatm_out and nachbar_out: manz x (maxnodes+1), column major.
+  integer(c_int) function ref_bsmatmtri(c_alfx, c_alfz, c_smatm, c_nachbar) bind(C, name='ref_bsmatmtri')
+    use bsmatm_mod, only: bsmatm
+    use konvmod, only: ltri, alfx, alfz
+    real(c_double), value :: c_alfx, c_alfz
+    real(c_double), intent(out) :: c_smatm(*)
+    integer(c_int), intent(out) :: c_nachbar(*)
+    logical :: lb
+    integer :: k, iel, ti, j, ncol
+    if (allocated(espx)) deallocate(espx)
+    if (allocated(espy)) deallocate(espy)
+    allocate(espx(elanz), espy(elanz))
+    espx = 0d0
+    espy = 0d0
+    iel = 0
+    do ti = 1, typanz
+       do j = 1, nelanz(ti)
+          iel = iel + 1
+          if (typ(ti) < 10) then
+             do k = 1, selanz(ti)
+                espx(iel) = espx(iel) + sx(snr(nrel(iel, k)))
+                espy(iel) = espy(iel) + sy(snr(nrel(iel, k)))
+             end do
+             espx(iel) = espx(iel) / selanz(ti)
+             espy(iel) = espy(iel) / selanz(ti)
+          end if
+       end do
+    end do
+    if (allocated(nachbar)) deallocate(nachbar)
+    call bnachbar()
+    decanz = 0
+    ltri = 1
+    alfx = c_alfx
+    alfz = c_alfz
+    if (allocated(smatm)) deallocate(smatm)
+    call bsmatm(1, lb)
+    ncol = max_nr_element_nodes + 1
+    c_smatm(1:manz*ncol) = reshape(smatm, (/manz*ncol/))
+    c_nachbar(1:manz*ncol) = reshape(nachbar, (/manz*ncol/))
+    ref_bsmatmtri = errnr
+  end function ref_bsmatmtri
+
   !> A bounded sample of the forward loop for timing: systems s = (k-1)*eanz + l
   !> in [c_first, c_last] (1-based), each assembled, scaled, factored and
   !> solved as in src/fem.f90:355-380, OpenMP over the systems.  No output.

@@ -117,6 +117,16 @@ def synthetic_rb2(w, inset):
     print(f"wrote ref_{w.name}_rb2.npz  N={len(v1)}")
 
 
+def triangular_smoothness():
+    """bnachbar + bsmatmtri (ltri = 1) on a small terrain-following
+    triangular mesh, alfx = 1, alfz = 0.5."""
+    w = meshgen.topo_triangles(nx_nodes=25, nz_nodes=10, eanz=12)
+    r = Reference(w.mesh)
+    sm, nb = r.bsmatmtri(1.0, 0.5)
+    np.savez_compressed(OUT / "ref_tri_small_smooth.npz", smatm=sm, nachbar=nb)
+    print(f"wrote ref_tri_small_smooth.npz  manz={sm.shape[0]}")
+
+
 def fixture(name, swrtr, lsink=False, nsink=0):
     d = FIX / name
     m = F.read_elem(d / "elem.dat")
@@ -146,6 +156,7 @@ def main():
     synthetic_dc(meshgen.small_dirichlet())
     synthetic_rb2(meshgen.small_block(), 0)
     synthetic_rb2(meshgen.small_dirichlet(), 1)
+    triangular_smoothness()
     fixture("minimal", 1)
     fixture("mixed_surf", 1)
     fixture("decoupling", 0, lsink=True, nsink=373)

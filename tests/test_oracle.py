@@ -374,3 +374,42 @@ def test_smoothness_operator_rows_sum_to_zero(small, oracle_lib):
     sm = oracle_lib.bsmatmreg(o, 40, 20, 2.0, 0.5)
     one = np.ones(o.elanz, complex)
     np.testing.assert_allclose(oracle_lib.rm_reg(sm, 40, one), 0.0, atol=1e-12)
+
+
+def _tri_operator(sm, nb):
+    """Dense R from smatm/nachbar (the product bptri applies,
+    src/cg_mod.f90:579-613)."""
+    M, c = sm.shape
+    mx = c - 1
+    R = np.zeros((M, M))
+    for i in range(M):
+        R[i, i] += sm[i, mx]
+        for k in range(mx):
+            if nb[i, k] > 0:
+                R[i, nb[i, k] - 1] += sm[i, k]
+    return R
+
+
+def test_triangular_smoothness_matches_reference(oracle_lib):
+    """bnachbar and bsmatmtri (ltri = 1) bit for bit against the reference's
+    own routines; R is symmetric, annihilates constants, and the neighbour
+    relation is symmetric with the edge count in the last column."""
+    w = meshgen.topo_triangles(nx_nodes=25, nz_nodes=10, eanz=12)
+    o = oracle_lib.Oracle(w.mesh)
+    nb = oracle_lib.bnachbar(o)
+    sm = oracle_lib.bsmatmtri(o, 1.0, 0.5, nb)
+    g = golden("tri_small_smooth")
+    np.testing.assert_array_equal(nb, g["nachbar"])
+    np.testing.assert_array_equal(sm, g["smatm"])
+    R = _tri_operator(sm, nb)
+    assert np.max(np.abs(R - R.T)) < 1e-13 * np.max(np.abs(R))
+    np.testing.assert_allclose(R @ np.ones(len(R)), 0.0, atol=1e-12)
+    mx = w.mesh.maxnodes
+    np.testing.assert_array_equal(nb[:, mx], (nb[:, :mx] > 0).sum(axis=1))
+    for i in range(len(nb)):
+        for j in nb[i, :mx]:
+            if j > 0:
+                assert i + 1 in nb[j - 1, :mx]
+    # isotropic weights on a regular pair of triangles: interior elements have
+    # three neighbours, corner triangles one or two
+    assert set(np.unique(nb[:, mx])) <= {1, 2, 3}
