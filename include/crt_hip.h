@@ -129,6 +129,16 @@ int crt_adjoint_gradient_dev(crt_handle *h, int32_t mode, const double *w_dev,
  * cgfac scaling).  The explicit-J path the adjoint gradient is compared with. */
 int crt_jhw_r(crt_handle *h, const double *w, const double *resid, double *grad);
 
+/* The regularisation's smoothness operator, built on the host once per
+ * inversion (bsmatm, src/bsmatm_mod.f90): ltri = 0 the rectangular stencil of
+ * bsmatmreg (:190-362; elements numbered row by row, nx x nz; smatm manz x 3),
+ * ltri = 1 or 2 the triangular operator of bsmatmtri (:364-461, no decoupled
+ * edges; smatm manz x (maxnodes+1)) with the edge neighbours of bnachbar
+ * (src/bnachbar.f90) in nachbar (manz x (maxnodes+1), may be NULL).  Column
+ * major, the reference's layout; the input crt_cg_solve takes. */
+int crt_smoothness(crt_handle *h, int32_t ltri, double alfx, double alfz, int32_t nx, int32_t nz,
+                   double *smatm, int32_t *nachbar);
+
 /* One model update's linear solve, src/update.f90:179-307 with cjggra
  * (src/cg_mod.f90:426-518): preconditioner cgfac, scaled RHS, then Fletcher-Reeves
  * CG on (J^H W J + lam R) with products bap/bp/bptri/bplma/bb, using the J of the

@@ -1284,8 +1284,6 @@ void orc_bsmatmreg(const orc_mesh *m, int nx, int nz, double alfx, double alfz, 
 #undef NY_
 }
 
-/* R m for the rectangular operator (the ltri = 0 branch of update,
- * src/update.f90:63-84, no prior model). */
 /* bnachbar: src/bnachbar.f90:25-66.  nachbar(i,k) = the element that shares
  * edge k (nodes k, k+1 mod maxnodes) of element i (the last one found), and
  * nachbar(i, maxnodes+1) = the number of edges with a neighbour; manz x
@@ -1351,6 +1349,8 @@ void orc_bsmatmtri(const orc_mesh *m, double alfx, double alfz, const int *nachb
   free(ey);
 }
 
+/* R m for the rectangular operator (the ltri = 0 branch of update,
+ * src/update.f90:63-84, no prior model). */
 void orc_rm_reg(int manz, int nx, const double *smatm, const orc_c *par, orc_c *out) {
   for (int i = 0; i < manz; i++) {
     orc_c cdum = cc(0, 0);

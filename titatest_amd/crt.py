@@ -36,6 +36,7 @@ EXPORTS = (
     "crt_get_precal", "crt_forward", "crt_forward_dev", "crt_voltages", "crt_voltages_dev",
     "crt_sensitivities", "crt_adjoint_gradient", "crt_adjoint_gradient_dev", "crt_jhw_r",
     "crt_cg_solve", "crt_forward_dc", "crt_sensitivities_dc", "crt_set_boundary_values",
+    "crt_smoothness",
     "crt_error_info", "crt_set_timing", "crt_stage_times", "crt_factor_phase_ticks",
     "crt_kpot_device",
 )
@@ -89,6 +90,7 @@ def load_library(path: Path = LIB_PATH):
     L.crt_forward_dev.argtypes = [vp, vp]
     L.crt_forward_dc.argtypes = [vp, vp, vp, vp]
     L.crt_set_boundary_values.argtypes = [vp, i32, vp, vp]
+    L.crt_smoothness.argtypes = [vp, i32, f64, f64, i32, i32, vp, vp]
     L.crt_sensitivities_dc.argtypes = [vp, vp]
     L.crt_voltages.argtypes = [vp, i32, vp, vp, i32, vp, vp, vp]
     L.crt_voltages_dev.argtypes = [vp, i32, vp, vp, i32, vp, vp]
@@ -291,6 +293,18 @@ class CRT:
         return g
 
     # A16/A17 ----------------------------------------------------------------
+    def smoothness(self, ltri, alfx, alfz, nx=0, nz=0):
+        """bsmatm's smoothness operator: smatm (manz, 3) for ltri 0, or
+        (smatm, nachbar), both (manz, maxnodes+1), for ltri 1/2."""
+        M, mx = self.elanz, self.mesh.maxnodes
+        ncol = 3 if ltri == 0 else mx + 1
+        sm = np.zeros(M * ncol)
+        nb = None if ltri == 0 else np.zeros(M * (mx + 1), np.int32)
+        self._check("smoothness", self.lib.crt_smoothness(self.h, ltri, float(alfx), float(alfz),
+                                                          nx, nz, _ptr(sm), _ptr(nb)))
+        sm = sm.reshape((M, ncol), order="F")
+        return sm if ltri == 0 else (sm, nb.reshape((M, mx + 1), order="F"))
+
     def cg_solve(self, w, dat, sigmaa, lam, ltri, smatm, rdiag, ncgmax, eps=1e-4, nx=0,
                  nachbar=None, rm=None):
         M = self.elanz

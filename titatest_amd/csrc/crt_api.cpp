@@ -1306,6 +1306,119 @@ int crt_cg_solve(crt_handle* h, const double* w, const double* dat, const double
   return 0;
 }
 
+// ---- smoothness operator (bsmatm), host side, once per inversion ----------
+// ltri = 0: the rectangular 5-point stencil of bsmatmreg
+// (src/bsmatm_mod.f90:190-362).  Elements are numbered row by row from the
+// surface (element (i, j) = (i-1) nx + j), corners in the order lower-left,
+// lower-right, upper-right, upper-left.  smatm(:,1) is the diagonal,
+// smatm(:,2) the coupling to the right, smatm(:,3) the one downwards;
+// horizontal weights alfx * dz / |dx of the centroids|, vertical ones
+// alfz * dx / |dz of the centroids|.
+static int smooth_rect(crt_handle* h, int nx, int nz, double alfx, double alfz, double* sm) {
+  const int manz = h->elanz;
+  if (nx < 1 || nz < 1 || nx * nz != manz) return kErrArg;
+  for (int t = 0; t < h->typanz; ++t)
+    if (h->typ[t] <= 10 && h->selanz[t] != 4) return kErrArg;
+  std::fill(sm, sm + (size_t)manz * 3, 0.0);
+  auto X = [&](int e, int c) { return node_x(h, h->nrel[e + (size_t)c * h->nel]); };
+  auto Y = [&](int e, int c) { return node_y(h, h->nrel[e + (size_t)c * h->nel]); };
+  auto cx = [&](int e) { double v = 0.0; for (int c = 0; c < 4; ++c) v = v + X(e, c); return v / 4.0; };
+  auto cy = [&](int e) { double v = 0.0; for (int c = 0; c < 4; ++c) v = v + Y(e, c); return v / 4.0; };
+  for (int i = 1; i <= nz; ++i)
+    for (int j = 1; j <= nx; ++j) {
+      const int e = (i - 1) * nx + j - 1;
+      const double xm = cx(e), zm = cy(e);
+      if (j > 1) sm[e] = 1.0 * alfx * std::fabs(Y(e, 3) - Y(e, 0)) / std::fabs(xm - cx(e - 1));
+      if (j < nx) {
+        const double w = 1.0 * alfx * std::fabs(Y(e, 2) - Y(e, 1)) / std::fabs(cx(e + 1) - xm);
+        sm[e] = sm[e] + w;
+        sm[e + (size_t)manz] = -w;
+      }
+      if (i > 1) sm[e] = sm[e] + 1.0 * alfz * std::fabs(X(e, 2) - X(e, 3)) / std::fabs(cy(e - nx) - zm);
+      if (i < nz) {
+        const double w = 1.0 * alfz * std::fabs(X(e, 1) - X(e, 0)) / std::fabs(zm - cy(e + nx));
+        sm[e] = sm[e] + w;
+        sm[e + 2 * (size_t)manz] = -w;
+      }
+    }
+  return 0;
+}
+
+// Edge neighbours of the area elements (bnachbar, src/bnachbar.f90:25-66):
+// nachbar(i,k) = the element sharing edge k (nodes k and k+1 mod maxnodes)
+// of element i, the highest-numbered one if several do, and nachbar(i,
+// maxnodes+1) the number of matches; found through a map of edges instead of
+// the reference's all-pairs scan.
+static void neighbours(crt_handle* h, std::vector<int>& nb) {
+  const int manz = h->elanz, mx = h->maxnodes;
+  nb.assign((size_t)manz * (mx + 1), 0);
+  auto node = [&](int e, int c) { return h->nrel[e + (size_t)c * h->nel]; };
+  std::map<std::pair<int, int>, std::vector<int>> edges;
+  for (int j = 0; j < manz; ++j)
+    for (int c = 0; c < mx; ++c) {
+      const int a = node(j, c), b = node(j, (c + 1) % mx);
+      edges[{std::min(a, b), std::max(a, b)}].push_back(j);
+    }
+  for (int i = 0; i < manz; ++i)
+    for (int c = 0; c < mx; ++c) {
+      const int a = node(i, c), b = node(i, (c + 1) % mx);
+      for (int j : edges[{std::min(a, b), std::max(a, b)}]) {
+        if (j == i) continue;
+        nb[i + (size_t)manz * c] = j + 1;
+        nb[i + (size_t)manz * mx] += 1;
+      }
+    }
+}
+
+// ltri = 1, 2: bsmatmtri (src/bsmatm_mod.f90:364-461) without decoupled
+// edges.  Across edge k of element i with neighbour n: w = edge length /
+// centroid distance * sqrt((alfx cos a)^2 + (alfz sin a)^2), a the direction
+// of the centroid difference; smatm(i,k) = -w, smatm(i,maxnodes+1) += w.
+static void smooth_tri(crt_handle* h, double alfx, double alfz, const std::vector<int>& nb, double* sm) {
+  const int manz = h->elanz, mx = h->maxnodes;
+  std::vector<double> ex(manz, 0.0), ey(manz, 0.0);
+  int iel = 0;
+  for (int t = 0; t < h->typanz; ++t)
+    for (int j = 0; j < h->nelanz[t]; ++j, ++iel) {
+      if (h->typ[t] > 10) continue;
+      const int nk = h->selanz[t];
+      for (int c = 0; c < nk; ++c) {
+        ex[iel] = ex[iel] + node_x(h, h->nrel[iel + (size_t)c * h->nel]);
+        ey[iel] = ey[iel] + node_y(h, h->nrel[iel + (size_t)c * h->nel]);
+      }
+      ex[iel] = ex[iel] / nk;
+      ey[iel] = ey[iel] / nk;
+    }
+  std::fill(sm, sm + (size_t)manz * (mx + 1), 0.0);
+  for (int i = 0; i < manz; ++i)
+    for (int c = 0; c < mx; ++c) {
+      const int a = h->nrel[i + (size_t)c * h->nel], b = h->nrel[i + (size_t)((c + 1) % mx) * h->nel];
+      const double dx = node_x(h, a) - node_x(h, b), dy = node_y(h, a) - node_y(h, b);
+      const double edglen = std::sqrt(dx * dx + dy * dy);
+      const int n = nb[i + (size_t)manz * c];
+      if (n <= 0) continue;
+      const double px = ex[i] - ex[n - 1], py = ey[i] - ey[n - 1];
+      const double dist = std::sqrt(px * px + py * py);
+      const double ang = std::atan2(py, px);
+      const double ax = alfx * std::cos(ang), az = alfz * std::sin(ang);
+      const double w = edglen / dist * std::sqrt(ax * ax + az * az);
+      sm[i + (size_t)manz * c] = -w;
+      sm[i + (size_t)manz * mx] = sm[i + (size_t)manz * mx] + w;
+    }
+}
+
+int crt_smoothness(crt_handle* h, int32_t ltri, double alfx, double alfz, int32_t nx, int32_t nz,
+                   double* smatm, int32_t* nachbar) {
+  if (!h || !smatm) return kErrArg;
+  if (ltri == 0) return smooth_rect(h, nx, nz, alfx, alfz, smatm);
+  if (ltri != 1 && ltri != 2) return kErrArg;
+  std::vector<int> nb;
+  neighbours(h, nb);
+  smooth_tri(h, alfx, alfz, nb, smatm);
+  if (nachbar) std::copy(nb.begin(), nb.end(), nachbar);
+  return 0;
+}
+
 int crt_error_info(crt_handle* h, int32_t* code, int32_t* k, int32_t* l) {
   if (!h) return kErrArg;
   if (code) *code = h->last_err;
