@@ -58,6 +58,25 @@ module crt_hip
        type(c_ptr), value :: kpot, hpot
      end function crt_forward
 
+     integer(c_int) function crt_set_boundary_values(h, rwdanz, rwdnr, rwd) &
+          bind(C, name='crt_set_boundary_values')
+       import :: c_int, c_ptr, c_int32_t, c_double_complex
+       type(c_ptr), value :: h
+       integer(c_int32_t), value :: rwdanz
+       integer(c_int32_t), intent(in) :: rwdnr(*)
+       complex(c_double_complex), intent(in) :: rwd(*)
+     end function crt_set_boundary_values
+
+     integer(c_int) function crt_smoothness(h, ltri, alfx, alfz, nx, nz, smatm, nachbar) &
+          bind(C, name='crt_smoothness')
+       import :: c_int, c_ptr, c_int32_t, c_double
+       type(c_ptr), value :: h
+       integer(c_int32_t), value :: ltri, nx, nz
+       real(c_double), value :: alfx, alfz
+       real(c_double), intent(out) :: smatm(*)
+       type(c_ptr), value :: nachbar
+     end function crt_smoothness
+
      ! DC twins (ldc): kpotdc/hpotdc real, or c_null_ptr
      integer(c_int) function crt_forward_dc(h, sigma, kpotdc, hpotdc) bind(C, name='crt_forward_dc')
        import :: c_int, c_ptr, c_double_complex
