@@ -303,3 +303,48 @@ def test_fortran_host_crmod_fixture(crtmod, tmp_path):
     np.testing.assert_array_equal(s2, st)
     assert np.max(np.abs(mag2 - np.abs(ge)) / np.abs(ge)) < VTOL
     assert np.max(np.abs(mag2 - mag) / mag) < 2e-7
+
+
+def test_fortran_host_writes_sensitivity_files(crtmod, oracle_lib, tmp_path):
+    """wsens through the Fortran host: one file per measurement named with
+    the zero-padded measurement number before the extension, header
+    |sum_j s_ij| and phase [mrad], then centroid x z and Re/Im of dV/dsigma
+    (bsens) per cell, all FP32 -- checked against the oracle's bsens."""
+    import subprocess
+    from pathlib import Path
+    from titatest_amd import formats as F
+    exe = Path(crtmod.__file__).resolve().parent / "crmod_hip"
+    if not exe.exists():
+        pytest.fail("Fortran host not built")
+    m, sig, st, vn, mag, pha = load_fixture("minimal")
+    from conftest import FIXTURES
+    d = FIXTURES / "minimal"
+    F.write_config(tmp_path / "config.dat", st, vn)
+    (tmp_path / "sens").mkdir()
+    say("fortran host minimal wsens")
+    r = subprocess.run([str(exe), str(d / "elem.dat"), str(d / "elec.dat"), str(d / "rho.dat"),
+                        str(tmp_path / "config.dat"), str(tmp_path / "volt.dat"), "1", "0",
+                        str(tmp_path / "sens" / "sens.dat")],
+                       capture_output=True, text=True, timeout=120, cwd=tmp_path)
+    assert r.returncode == 0, r.stdout + r.stderr
+    N = len(st)
+    nd = len(str(N))
+    names = sorted(p.name for p in (tmp_path / "sens").iterdir())
+    assert names == [f"sens{i:0{nd}d}.dat" for i in range(1, N + 1)]
+    o = oracle_lib.Oracle(m)
+    o.rwaven(st, vn)
+    o.precal()
+    o.forward(sig)
+    o.volt(st, vn, 0)
+    so = o.sens(0)
+    cx = np.array([m.sx[m.snr[np.asarray(m.nrel)[e, :4] - 1] - 1].mean() for e in range(m.elanz)])
+    for i in (0, N - 1):
+        rows = np.loadtxt(tmp_path / "sens" / f"sens{i + 1:0{nd}d}.dat", skiprows=1)
+        head = np.loadtxt(tmp_path / "sens" / f"sens{i + 1:0{nd}d}.dat", max_rows=1)
+        assert rows.shape == (m.elanz, 4)
+        np.testing.assert_allclose(rows[:, 0], cx, rtol=1e-6)
+        s = rows[:, 2] + 1j * rows[:, 3]
+        assert rel_err(s, so[i]) < 1e-6
+        tot = so[i].sum()
+        np.testing.assert_allclose(head[0], abs(tot), rtol=1e-6)
+        np.testing.assert_allclose(head[1], 1e3 * np.angle(tot), rtol=1e-5, atol=1e-4)

@@ -3,10 +3,15 @@
 !> and writes volt.dat the way the reference's wdatm does (src/wdatm.f90:55-79),
 !> at full precision.
 !>
-!>   crmod_hip elem.dat elec.dat rho.dat config.dat volt.dat [swrtr [nsink]]
+!>   crmod_hip elem.dat elec.dat rho.dat config.dat volt.dat [swrtr [nsink [sens.dat]]]
 !>
 !> swrtr = 1 (default) is the 2.5D forward; 0 is 2-D (one wavenumber 0).
 !> nsink > 0 adds the fictitious sink at that node (crmod.cfg lsink/nsink).
+!> With a sensitivity file name (crmod.cfg's lsens and sens path), the
+!> sensitivities dV/dsigma (bsens) are written one file per measurement as the
+!> reference's wsens does (src/wsens.f90:40-98): the measurement number,
+!> zero-padded to the digits of nanz, goes before the extension; first line
+!> |sum_j s_ij| and its phase [mrad], then x z Re Im per cell, in FP32.
 !> The readers follow src/relem.f90:62-151 (with the rnr repair of :209-238),
 !> src/relectr.f90:46-68, src/rsigma.f90:91-189 and src/rdatm.f90:54-115.
 program crmod_hip
@@ -19,16 +24,17 @@ program crmod_hip
   integer(c_int32_t), allocatable :: strnr(:), vnr(:)
   real(c_double), allocatable, target :: sx(:), sy(:)
   complex(c_double_complex), allocatable :: sigma(:), volt(:), sigmaa(:)
+  complex(c_double_complex), allocatable, target :: sens(:,:)
   real(c_double) :: kwn(30), kwnwi(30), bet, pha, pi, npi
   integer(c_int32_t) :: kwnanz, lbeta, lrandb, swrtr, nsink, code, kk, ll
   type(crt_mesh) :: mesh
   type(c_ptr) :: h
-  character(len=512) :: f_elem, f_elec, f_rho, f_cfg, f_volt, arg
+  character(len=512) :: f_elem, f_elec, f_rho, f_cfg, f_volt, f_sens, arg
   integer :: i, j, k, idum, e1, e2, ios, rc, ie1, ie2, ic, jk1, jk2, l
   real(c_double) :: rb, rp
 
   if (command_argument_count() < 5) then
-     write (*, '(a)') 'usage: crmod_hip elem.dat elec.dat rho.dat config.dat volt.dat [swrtr [nsink]]'
+     write (*, '(a)') 'usage: crmod_hip elem.dat elec.dat rho.dat config.dat volt.dat [swrtr [nsink [sens.dat]]]'
      stop 1
   end if
   call get_command_argument(1, f_elem)
@@ -46,6 +52,8 @@ program crmod_hip
      call get_command_argument(7, arg)
      read (arg, *) nsink
   end if
+  f_sens = ''
+  if (command_argument_count() >= 8) call get_command_argument(8, f_sens)
   pi = acos(-1d0)
 
   ! ---- elem.dat
@@ -164,6 +172,12 @@ program crmod_hip
   call check(rc, 'crt_forward')
   rc = crt_voltages(h, nanz, strnr, vnr, 0, c_null_ptr, volt, sigmaa)
   call check(rc, 'crt_voltages')
+  if (len_trim(f_sens) > 0) then
+     allocate (sens(nanz, elanz))
+     rc = crt_sensitivities(h, 0, c_loc(sens))
+     call check(rc, 'crt_sensitivities')
+     call write_sens(trim(f_sens))
+  end if
   rc = crt_destroy(h)
 
   ! ---- volt.dat as wdatm writes it (polarity swap when the phase is near pi)
@@ -185,6 +199,56 @@ program crmod_hip
   ic = 0
 
 contains
+
+  ! wsens (src/wsens.f90:40-98): one file per measurement, the centroids as
+  ! relem sets them (src/relem.f90:134-143)
+  subroutine write_sens(pattern)
+    character(len=*), intent(in) :: pattern
+    real(c_double), allocatable :: ex(:), ey(:)
+    complex(c_double_complex) :: summe
+    character(len=16) :: digits, fmt
+    character(len=600) :: fname
+    integer :: nd, ls, ld, ii, jj, it, iel, kk2
+    allocate (ex(elanz), ey(elanz))
+    ex = 0d0
+    ey = 0d0
+    iel = 0
+    do it = 1, typanz
+       do jj = 1, nelanz(it)
+          iel = iel + 1
+          if (typ(it) < 10) then
+             do kk2 = 1, selanz(it)
+                ex(iel) = ex(iel) + sx(snr(nrel(iel, kk2)))
+                ey(iel) = ey(iel) + sy(snr(nrel(iel, kk2)))
+             end do
+             ex(iel) = ex(iel) / selanz(it)
+             ey(iel) = ey(iel) / selanz(it)
+          end if
+       end do
+    end do
+    nd = int(log10(real(nanz))) + 1
+    ls = index(pattern, '/', back=.true.)
+    ld = ls + index(pattern(ls + 1:), '.') - 1
+    write (fmt, '(a,i0,a,i0,a)') '(i', nd, '.', nd, ')'
+    do ii = 1, nanz
+       write (digits, fmt) ii
+       fname = pattern(1:ld) // digits(1:nd) // pattern(ld + 1:min(ld + 4, len(pattern)))
+       open (16, file=trim(fname), status='replace', action='write')
+       summe = dcmplx(0d0)
+       do jj = 1, elanz
+          summe = summe + sens(ii, jj)
+       end do
+       if (abs(summe) > 0d0) then
+          write (16, *) real(abs(summe)), real(1d3 * atan2(dimag(summe), dble(summe)))
+       else
+          write (16, *) 0., 0.
+       end if
+       do jj = 1, elanz
+          write (16, *) real(ex(jj)), real(ey(jj)), real(dble(sens(ii, jj))), real(dimag(sens(ii, jj)))
+       end do
+       close (16)
+    end do
+  end subroutine write_sens
 
   subroutine check(rcode, where)
     integer, intent(in) :: rcode
