@@ -138,21 +138,65 @@ __device__ __forceinline__ bool tile_ok(const Stage& C, int rw, int cw) {
 
 // TR x TC tile of the stage window at (rw0, cw0): the current values (ring, or
 // the staged batch for the window's last kNB rows) minus panel J-1's rank-kNB
-// product, accumulated in ascending k.
+// product, accumulated in ascending k.  For the 2x2 look-ahead tiles the
+// current values are fetched without branches, in two rounds of LDS loads
+// whose latencies overlap: the ring entries and each tile row's offset record
+// of the staged batch (kMaxR int16 offsets, one 16-byte load), then the
+// staged values at the matching slots; every index is clamped in range and the
+// choice is a select.
 template <int TR, int TC>
 __device__ __forceinline__ void tile_update(const Stage& C, const cplx* Tt, int rw0, int cw0, cplx acc[TR][TC]) {
+  if constexpr (TR * TC <= 4) {
+    // look-ahead tiles (on the critical path): branch-free two-round fetch
+    int slot[TR][TC];
 #pragma unroll
-  for (int i = 0; i < TR; ++i)
+    for (int i = 0; i < TR; ++i) {
+      const int rw = rw0 + i;
+      const int ibc = min(max(rw - (C.mb - kNB), 0), kNB - 1);
+      const u32x4 rec = *reinterpret_cast<const u32x4*>(C.so_old + ibc * kMaxR);
+      const int rwc = min(max(rw, 0), C.mb - 1);
 #pragma unroll
-    for (int j = 0; j < TC; ++j) {
-      const int rw = rw0 + i, cw = cw0 + j;
-      cplx v = mkc(0.0, 0.0);
-      if (tile_ok(C, rw, cw)) {
-        const int ib = rw - (C.mb - kNB);
-        v = ib >= 0 ? staged(C.sv_old, C.so_old, ib, cw - rw + C.mb) : Tt[C.ring.pos(rw, cw)];
+      for (int j = 0; j < TC; ++j) {
+        const int cw = cw0 + j;
+        acc[i][j] = Tt[C.ring.pos(rwc, min(max(cw, 0), rwc))];
+        const int t = cw - rw + C.mb;
+        int q = -1;
+#pragma unroll
+        for (int sl = 0; sl < kMaxR; ++sl) {
+          const int off = (int)(short)(unsigned short)(rec[sl >> 1] >> (16 * (sl & 1)));
+          q = off == t ? sl : q;
+        }
+        slot[i][j] = q;
       }
-      acc[i][j] = v;
     }
+#pragma unroll
+    for (int i = 0; i < TR; ++i) {
+      const int rw = rw0 + i;
+      const int ib = rw - (C.mb - kNB);
+      const int ibc = min(max(ib, 0), kNB - 1);
+#pragma unroll
+      for (int j = 0; j < TC; ++j) {
+        const cplx sv = C.sv_old[ibc * kMaxR + max(slot[i][j], 0)];
+        const cplx v = ib >= 0 ? csel(slot[i][j] >= 0, sv, mkc(0.0, 0.0)) : acc[i][j];
+        acc[i][j] = csel(tile_ok(C, rw, cw0 + j), v, mkc(0.0, 0.0));
+      }
+    }
+  } else {
+    // trailing tiles: only the few on the window's last kNB rows take the
+    // staged batch's slot search
+#pragma unroll
+    for (int i = 0; i < TR; ++i)
+#pragma unroll
+      for (int j = 0; j < TC; ++j) {
+        const int rw = rw0 + i, cw = cw0 + j;
+        cplx v = mkc(0.0, 0.0);
+        if (tile_ok(C, rw, cw)) {
+          const int ib = rw - (C.mb - kNB);
+          v = ib >= 0 ? staged(C.sv_old, C.so_old, ib, cw - rw + C.mb) : Tt[C.ring.pos(rw, cw)];
+        }
+        acc[i][j] = v;
+      }
+  }
   if (!C.apply) return;
   // operand rows of panel J-1 (stage rows + kNB)
   int ra[TR], ca[TC];
@@ -160,7 +204,9 @@ __device__ __forceinline__ void tile_update(const Stage& C, const cplx* Tt, int 
   for (int i = 0; i < TR; ++i) ra[i] = min(rw0 + i + kNB, C.W - 1);
 #pragma unroll
   for (int j = 0; j < TC; ++j) ca[j] = min(cw0 + j + kNB, C.W - 1);
-#pragma unroll 1
+  // the 2x2 look-ahead tiles issue all their operand loads at once; the
+  // 4x4 trailing tiles keep one k in registers at a time
+#pragma unroll (TR * TC <= 4 ? kNB : 1)
   for (int k = 0; k < kNB; ++k) {
     cplx a[TR], b[TC];
 #pragma unroll
@@ -427,6 +473,8 @@ __global__ __launch_bounds__(256, 2) void crt_factor_kernel(FactorArgs A) {
       // ---- phase A: waves 1-3 apply panel J-1 to panel J's columns; wave 0
       // copies the entering batch J into panel J's last kNB rows
       if (wave > 0) {
+        // the look-ahead precedes the panel on the critical path as well
+        __builtin_amdgcn_s_setprio(2);
         for (int x = x3; x < nla; x += 192) {
           int j = 0, i = x;
 #pragma unroll
@@ -449,12 +497,16 @@ __global__ __launch_bounds__(256, 2) void crt_factor_kernel(FactorArgs A) {
         const bool ok = o + rw < n && cw >= i;
         Pc[pix(rw, cw)] = csel(ok, staged(sv + bc * kNB * kMaxR, so + bc * kNB * kMaxR, i, cw - i), mkc(0.0, 0.0));
       }
+      __builtin_amdgcn_s_setprio(0);
       clk.mark(0);
       lds_barrier();
       clk.mark(1);
 
       // ---- phase B
       if (wave == 0) {
+        // the panel is the stage's critical path: its wave takes issue
+        // precedence over the other workgroup's trailing wave on its SIMD
+        __builtin_amdgcn_s_setprio(3);
         // panel J in registers: lane holds panel rows lane + 64 q
         const int rows = min(W, n - o);
         cplx v[ROWS][kNB];
@@ -569,6 +621,7 @@ __global__ __launch_bounds__(256, 2) void crt_factor_kernel(FactorArgs A) {
         }
 #pragma unroll
         for (int q = 0; q < ROWS; ++q) yv[q] = ny[q];
+        __builtin_amdgcn_s_setprio(0);
         clk.mark(2);
       } else {
         // batch J+1 = rows [o + kNB + mb, o + 2 kNB + mb): loads issued first,
