@@ -74,6 +74,46 @@ def cpu_baseline(w, nsys, K, sample=64):
     }
 
 
+def uneven_batch(args):
+    """COUNT independent models of uneven mesh and electrode counts (nx x nz
+    quadrilaterals, nx ~ U[30,160], nz ~ U[15,50], 24-72 electrodes): one
+    Gauss-Newton step each -- forward, -ln V, adjoint gradient, explicit J and
+    20 CG iterations on J^H W J + lam R (lam = 10) -- ``threads`` models in
+    flight.  Times the batch with the CG iterations as captured HIP graphs and
+    with per-launch CG (the per-model path); the batch's handles, element
+    matrices and observed data are prepared before the timed region."""
+    import numpy as np
+    from titatest_amd import batch, meshgen
+    ws = meshgen.uneven_batch(count=args.uneven_batch, seed=1006)
+    mods = []
+    for i, w in enumerate(ws):
+        nx = len(np.unique(w.mesh.sx)) - 1
+        mods.append(batch.prepare(w, nx, w.mesh.elanz // nx, seed=1003 + i))
+    print(f"prepared {len(mods)} models", file=sys.stderr, flush=True)
+    for m in mods:  # warm-up: buffers, graph capture
+        batch.gn_step(m, use_graph=True)
+        batch.gn_step(m, use_graph=False)
+    t_graph = []
+    t_eager = []
+    for _ in range(max(1, args.steps // 2)):
+        _, t = batch.run_batch(mods, threads=args.threads, use_graph=True)
+        t_graph.append(t)
+        _, t = batch.run_batch(mods, threads=args.threads, use_graph=False)
+        t_eager.append(t)
+    tg, te = min(t_graph), min(t_eager)
+    cells = sum(w.mesh.elanz for w in ws)
+    quads = sum(w.nanz for w in ws)
+    print(json.dumps({
+        "metric": "Gauss-Newton steps/s over an uneven batch (adjoint gradient + 20 CG iterations), FP64, 1 GPU",
+        "value": len(ws) / tg, "unit": "model steps/s", "n_gpus": 1, "steps": len(t_graph),
+        "warmup": 1, "ms_per_step": tg * 1e3, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "c128", "data": "synthetic (seeded random-field models, generated meshes)",
+        "config": {"workload": f"{len(ws)} uneven models, {cells} cells, {quads} quadrupoles in total",
+                   "threads": args.threads},
+        "per_model_path_ms": te * 1e3, "graph_over_per_model": te / tg,
+    }))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -84,6 +124,10 @@ def main():
     ap.add_argument("--skip-adjoint", action="store_true")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (calibrates HBM byte counters)")
+    ap.add_argument("--uneven-batch", type=int, default=0, metavar="COUNT",
+                    help="instead of the forward: one Gauss-Newton step on each of COUNT uneven models "
+                         "(adjoint gradient + 20 CG iterations), graph-captured CG against the per-launch path")
+    ap.add_argument("--threads", type=int, default=8, help="models in flight for --uneven-batch")
     args = ap.parse_args()
     if args.copy_probe:
         import torch
@@ -107,6 +151,9 @@ def main():
 
     if not torch.cuda.is_available():
         sys.exit("bench.py needs a HIP device")
+    if args.uneven_batch:
+        uneven_batch(args)
+        return
     dev = torch.device("cuda:0")
 
     w = meshgen.large_grf()

@@ -153,6 +153,19 @@ int crt_cg_solve(crt_handle *h, const double *wmatd_wdfak, const double *dat,
                  int32_t maxnodes, const double *rdiag, const double *rm, int32_t ncgmax,
                  double eps, double *dpar, double *cgfac, double *cgres);
 
+/* No reference counterpart.  Caps the device memory a forward keeps for its
+ * factors in flight (band factors, scaled rows, sweeps; default 64 GiB, and
+ * never more than half the free memory): a larger problem runs in several
+ * factor batches.  For callers keeping many handles alive at once. */
+int crt_set_device_budget(crt_handle *h, uint64_t bytes);
+
+/* No reference counterpart.  enable != 0: crt_cg_solve runs its iterations as
+ * one HIP graph, captured on the first call and replayed while the problem's
+ * sizes, buffers and parameters stay the same (for callers running many short
+ * solves, where the per-kernel launches dominate).  The results are those of
+ * the launch-by-launch path, bit for bit. */
+int crt_set_graph(crt_handle *h, int32_t enable);
+
 /* First failing system of the last call: error code, wavenumber and electrode
  * (1-based) where known. */
 int crt_error_info(crt_handle *h, int32_t *code, int32_t *k, int32_t *l);

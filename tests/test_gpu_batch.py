@@ -1,0 +1,106 @@
+"""The graph-captured CG and the uneven batch of Gauss-Newton steps.
+
+* crt_set_graph: the CG iterations replayed from one captured HIP graph give
+  the launch-by-launch results bit for bit, on every replay, with eager solves
+  on the same handle in between, and a changed parameter recaptures;
+* titatest_amd.batch: one Gauss-Newton step on models of different mesh and
+  electrode counts, run concurrently (threads, graph-captured CG) against the
+  same steps run one by one with launch-by-launch CG (bit for bit), and the
+  step against the oracle's chain (bvolti, bsensi, update's right-hand side
+  and cgfac, cjggra with a fixed iteration count) to the sensitivity
+  tolerance, with the adjoint gradient against J^H W r.  The oracle
+  comparison runs 10 CG iterations: with eps = 0 the iterations past
+  convergence amplify rounding in the inputs -- on these models the oracle's
+  own 20-iteration update moves by up to 6e-10 when J is perturbed by 1e-15
+  (3e-15 after 10 iterations) -- so the 20-iteration steps are checked bit for
+  bit across the graph, threaded and per-model paths instead.
+"""
+import numpy as np
+import pytest
+
+from conftest import rel_err
+from titatest_amd import meshgen
+
+pytestmark = pytest.mark.gpu
+
+STOL = 1e-8
+
+
+def say(*a):
+    print("[gpu-case]", *a, flush=True)
+
+
+@pytest.fixture(scope="module")
+def crtmod(hip_lib):
+    from titatest_amd import crt
+    return crt
+
+
+def test_cg_graph_replays_match_eager(crtmod, oracle_lib):
+    w = meshgen.small_block()
+    say("small_block cg graph vs eager")
+    g = crtmod.CRT(w.mesh)
+    g.rwaven(w.strnr, w.vnr)
+    g.precal()
+    g.forward(w.sigma)
+    v, sa = g.bvolti(w.strnr, w.vnr)
+    g.sensitivities(1, to_host=False)
+    rng = np.random.default_rng(5)
+    N = len(v)
+    dat = sa + 0.01 * (rng.standard_normal(N) + 1j * rng.standard_normal(N))
+    wgt = 1.0 / (0.02 + 1e-4 / np.abs(v)) ** 2
+    sm = g.smoothness(0, 1.0, 1.0, nx=40, nz=20)
+
+    def solve(lam, graph):
+        g.set_graph(graph)
+        return g.cg_solve(wgt, dat, sa, lam, 0, sm, sm[:, 0], 20, 1e-4, nx=40)
+
+    ref = solve(10.0, False)
+    for rep in range(3):
+        got = solve(10.0, True)
+        for a, b in zip(got, ref):
+            np.testing.assert_array_equal(a, b)
+        eager = solve(10.0, False)  # eager work on the handle between replays
+        np.testing.assert_array_equal(eager[0], ref[0])
+    ref2 = solve(3.0, False)
+    got2 = solve(3.0, True)  # a changed parameter recaptures
+    np.testing.assert_array_equal(got2[0], ref2[0])
+    assert rel_err(got2[0], ref[0]) > 1e-6
+
+
+def _small_models(count=2):
+    ws = meshgen.uneven_batch(count=8, seed=1006)
+    ws = sorted(ws, key=lambda w: w.mesh.elanz)[:count]
+    out = []
+    for w in ws:
+        nx = len(np.unique(w.mesh.sx)) - 1
+        out.append((w, nx, w.mesh.elanz // nx))
+    return out
+
+
+def test_uneven_batch_gn_step(crtmod, oracle_lib):
+    from titatest_amd import batch
+    cases = _small_models(2)
+    say("uneven batch", [(w.mesh.sanz, w.mesh.eanz, w.nanz) for w, _, _ in cases])
+    mods = [batch.prepare(w, nx, nz, seed=i) for i, (w, nx, nz) in enumerate(cases)]
+    seq = [batch.gn_step(m, use_graph=False) for m in mods]
+    par, _ = batch.run_batch(mods, threads=2, use_graph=True)
+    for (d1, g1, _), (d2, g2, _) in zip(seq, par):
+        np.testing.assert_array_equal(d1, d2)
+        np.testing.assert_array_equal(g1, g2)
+    for (w, nx, nz), m in zip(cases, mods):
+        dpar, grad, _ = batch.gn_step(m, ncg=10, use_graph=True)
+        o = oracle_lib.Oracle(w.mesh)
+        o.rwaven(w.strnr, w.vnr)
+        o.precal()
+        o.forward(w.sigma)
+        v, sa = o.volt(w.strnr, w.vnr, 1)
+        J = o.sens(1)
+        N = len(v)
+        ones = np.ones(N, np.int32)
+        gj = J.conj().T @ (m.wgt * (m.dobs - sa))
+        assert rel_err(grad, gj) < STOL
+        cgfac, bvec = oracle_lib.update_rhs(J, m.wgt, ones, m.dobs, sa, 10.0, 0, m.smatm[:, 0], m.rm)
+        dpo, _, ncg = oracle_lib.cjggra(J, m.wgt, ones, cgfac, bvec, 10.0, 0, nx, m.smatm, 10, 0.0)
+        assert ncg == 10
+        assert rel_err(dpar, dpo * cgfac) < STOL

@@ -45,6 +45,8 @@ def test_null_arguments_are_rejected_without_a_device():
     assert lib.crt_sensitivities_dc(None, None) == 1000
     assert lib.crt_set_boundary_values(None, 0, None, None) == 1000
     assert lib.crt_smoothness(None, 0, 1.0, 1.0, 1, 1, None, None) == 1000
+    assert lib.crt_set_graph(None, 1) == 1000
+    assert lib.crt_set_device_budget(None, 1 << 30) == 1000
 
 
 def test_fortran_host_built_and_linked():

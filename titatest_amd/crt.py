@@ -36,7 +36,7 @@ EXPORTS = (
     "crt_get_precal", "crt_forward", "crt_forward_dev", "crt_voltages", "crt_voltages_dev",
     "crt_sensitivities", "crt_adjoint_gradient", "crt_adjoint_gradient_dev", "crt_jhw_r",
     "crt_cg_solve", "crt_forward_dc", "crt_sensitivities_dc", "crt_set_boundary_values",
-    "crt_smoothness",
+    "crt_smoothness", "crt_set_graph", "crt_set_device_budget",
     "crt_error_info", "crt_set_timing", "crt_stage_times", "crt_factor_phase_ticks",
     "crt_kpot_device",
 )
@@ -91,6 +91,8 @@ def load_library(path: Path = LIB_PATH):
     L.crt_forward_dc.argtypes = [vp, vp, vp, vp]
     L.crt_set_boundary_values.argtypes = [vp, i32, vp, vp]
     L.crt_smoothness.argtypes = [vp, i32, f64, f64, i32, i32, vp, vp]
+    L.crt_set_graph.argtypes = [vp, i32]
+    L.crt_set_device_budget.argtypes = [vp, C.c_uint64]
     L.crt_sensitivities_dc.argtypes = [vp, vp]
     L.crt_voltages.argtypes = [vp, i32, vp, vp, i32, vp, vp, vp]
     L.crt_voltages_dev.argtypes = [vp, i32, vp, vp, i32, vp, vp]
@@ -304,6 +306,14 @@ class CRT:
                                                           nx, nz, _ptr(sm), _ptr(nb)))
         sm = sm.reshape((M, ncol), order="F")
         return sm if ltri == 0 else (sm, nb.reshape((M, mx + 1), order="F"))
+
+    def set_device_budget(self, nbytes):
+        """Cap the device memory a forward keeps for factors in flight."""
+        self._check("set_device_budget", self.lib.crt_set_device_budget(self.h, int(nbytes)))
+
+    def set_graph(self, on=True):
+        """Run crt_cg_solve's iterations as one captured HIP graph."""
+        self._check("set_graph", self.lib.crt_set_graph(self.h, int(bool(on))))
 
     def cg_solve(self, w, dat, sigmaa, lam, ltri, smatm, rdiag, ncgmax, eps=1e-4, nx=0,
                  nachbar=None, rm=None):

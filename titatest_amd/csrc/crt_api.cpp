@@ -46,6 +46,12 @@ struct crt_handle {
   // stage state
   bool precal_done = false, forward_done = false, volt_done = false, sens_done = false;
   bool dc = false;  // the last forward was the DC twin (crt_forward_dc)
+  // CG iterations as one captured HIP graph (crt_set_graph), with the kernel
+  // arguments and iteration count it was captured for
+  bool use_graph = false;
+  hipGraph_t cg_graph = nullptr;
+  hipGraphExec_t cg_exec = nullptr;
+  std::vector<unsigned char> cg_key;
   int sens_mode = -1;
   int nanz = 0;
   std::vector<int> strnr, vnr;
@@ -569,6 +575,8 @@ int crt_create(crt_handle** out, const crt_mesh* m) {
 int crt_destroy(crt_handle* h) {
   if (!h) return 0;
   if (h->stream) hipStreamSynchronize(h->stream);
+  if (h->cg_exec) hipGraphExecDestroy(h->cg_exec);
+  if (h->cg_graph) hipGraphDestroy(h->cg_graph);
   for (auto& kv : h->buf) hipFree(kv.second);
   for (auto& t : h->pending) {
     hipEventDestroy(t.a);
@@ -1222,6 +1230,7 @@ int crt_cg_solve(crt_handle* h, const double* w, const double* dat, const double
   if (ltri == 0 && smatm_cols < 3) return kErrArg;
   const int N = h->nanz, M = h->elanz;
   CgArgs G;
+  std::memset(&G, 0, sizeof(G));  // the bytes key the captured graph
   G.nanz = N;
   G.manz = M;
   G.ltri = ltri;
@@ -1272,19 +1281,45 @@ int crt_cg_solve(crt_handle* h, const double* w, const double* dat, const double
                      (const double2*)ddat, (const double2*)dsa, (const double2*)drm);
   tstop(h);
   HIPCHK(hipMemcpyAsync(G.rvec, G.bvec, (size_t)M * 16, hipMemcpyDeviceToDevice, h->stream));
-  for (int it = 0; it < ncgmax; ++it) {
-    tstart(h, "cg_scalar");
-    hipLaunchKernelGGL(crt_cg_scalar_kernel, dim3(1), dim3(1024), 0, h->stream, G, 0, it);
+  // the iterations: kernels only, the recurrence's scalars and the stopping
+  // test stay on the device (later iterations return at once once converged)
+  auto iterations = [&](bool timed) {
+    for (int it = 0; it < ncgmax; ++it) {
+      if (timed) tstart(h, "cg_scalar");
+      hipLaunchKernelGGL(crt_cg_scalar_kernel, dim3(1), dim3(1024), 0, h->stream, G, 0, it);
+      if (timed) tstop(h);
+      if (timed) tstart(h, "cg_bap");
+      hipLaunchKernelGGL(crt_cg_bap_kernel, dim3((N + 255) / 256), dim3(256), 0, h->stream, G);
+      if (timed) tstop(h);
+      if (timed) tstart(h, "cg_bb");
+      hipLaunchKernelGGL(crt_cg_bb_kernel, dim3((unsigned)(((size_t)M * 64 + 255) / 256)), dim3(256), 0,
+                         h->stream, G);
+      if (timed) tstop(h);
+      if (timed) tstart(h, "cg_scalar");
+      hipLaunchKernelGGL(crt_cg_scalar_kernel, dim3(1), dim3(1024), 0, h->stream, G, 1, it);
+      if (timed) tstop(h);
+    }
+  };
+  if (h->use_graph) {
+    std::vector<unsigned char> key(sizeof(CgArgs) + sizeof(int));
+    std::memcpy(key.data(), &G, sizeof(CgArgs));
+    std::memcpy(key.data() + sizeof(CgArgs), &ncgmax, sizeof(int));
+    if (!h->cg_exec || key != h->cg_key) {
+      if (h->cg_exec) hipGraphExecDestroy(h->cg_exec);
+      if (h->cg_graph) hipGraphDestroy(h->cg_graph);
+      h->cg_exec = nullptr;
+      h->cg_graph = nullptr;
+      HIPCHK(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+      iterations(false);
+      HIPCHK(hipStreamEndCapture(h->stream, &h->cg_graph));
+      HIPCHK(hipGraphInstantiate(&h->cg_exec, h->cg_graph, nullptr, nullptr, 0));
+      h->cg_key = key;
+    }
+    tstart(h, "cg_graph");
+    HIPCHK(hipGraphLaunch(h->cg_exec, h->stream));
     tstop(h);
-    tstart(h, "cg_bap");
-    hipLaunchKernelGGL(crt_cg_bap_kernel, dim3((N + 255) / 256), dim3(256), 0, h->stream, G);
-    tstop(h);
-    tstart(h, "cg_bb");
-    hipLaunchKernelGGL(crt_cg_bb_kernel, dim3((unsigned)(((size_t)M * 64 + 255) / 256)), dim3(256), 0, h->stream, G);
-    tstop(h);
-    tstart(h, "cg_scalar");
-    hipLaunchKernelGGL(crt_cg_scalar_kernel, dim3(1), dim3(1024), 0, h->stream, G, 1, it);
-    tstop(h);
+  } else {
+    iterations(true);
   }
   rc = finish(h);
   if (rc) return rc;
@@ -1416,6 +1451,18 @@ int crt_smoothness(crt_handle* h, int32_t ltri, double alfx, double alfz, int32_
   neighbours(h, nb);
   smooth_tri(h, alfx, alfz, nb, smatm);
   if (nachbar) std::copy(nb.begin(), nb.end(), nachbar);
+  return 0;
+}
+
+int crt_set_device_budget(crt_handle* h, uint64_t bytes) {
+  if (!h || bytes < ((uint64_t)1 << 20)) return kErrArg;
+  h->batch_bytes_max = (size_t)bytes;
+  return 0;
+}
+
+int crt_set_graph(crt_handle* h, int32_t enable) {
+  if (!h) return kErrArg;
+  h->use_graph = enable != 0;
   return 0;
 }
 
