@@ -1099,7 +1099,14 @@ static int run_sens(crt_handle* h, int32_t mode, const double2* sigma) {
   S.vnr = h->at<int>("vnr");
   S.sens = h->get<double2>("sens", (size_t)h->nanz * h->elanz);
   NEED(S.sens);
-  const size_t lds = (size_t)h->maxnodes * h->E * 16;
+  double2* kpt = h->get<double2>("kpt", (size_t)h->K * h->n * h->E);
+  NEED(kpt);
+  S.kpt = kpt;
+  tstart(h, "kpot_transpose");
+  hipLaunchKernelGGL(crt_kpot_transpose_kernel, dim3((h->n + 63) / 64, (h->E + 15) / 16, h->K), dim3(256), 0,
+                     h->stream, (const double2*)S.kpot, h->n, h->E, kpt);
+  tstop(h);
+  const size_t lds = (size_t)4 * h->E * 16;
   tstart(h, "sens");
   hipLaunchKernelGGL(crt_sens_kernel, dim3(h->elanz), dim3(256), lds, h->stream, S);
   tstop(h);

@@ -28,6 +28,7 @@ struct SensArgs {
   const double* elbg;
   const double* kwnwi;
   const double2* kpot;    // [K][eanz][n]
+  const double2* kpt;     // [K][n][eanz] (crt_kpot_transpose_kernel)
   const double2* sigma;
   const double2* volt;
   const int* strnr;
@@ -76,6 +77,7 @@ __global__ void crt_rtrafo_elec_kernel(const double2* kpot, const double* kwnwi,
 __global__ void crt_volt_kernel(const double2* he, int eanz, int nanz, const int* strnr, const int* vnr,
                                 int mode, const double* kfak, double2* volt, double2* sigmaa, int* err);
 __global__ void crt_sens_kernel(SensArgs S);
+__global__ void crt_kpot_transpose_kernel(const double2* kpot, int n, int E, double2* kpt);
 __global__ void crt_adj_gamma_kernel(int nanz, int mode, const double* w, const double2* resid,
                                      const double2* volt, double2* gamma);
 __global__ void crt_adj_cmat_kernel(int nanz, int eanz, const int* strnr, const int* vnr,

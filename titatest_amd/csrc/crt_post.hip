@@ -279,18 +279,48 @@ __global__ void crt_volt_kernel(const double2* __restrict__ he, int eanz, int na
 // node potentials for all electrodes at one wavenumber are staged in LDS and
 // every thread evaluates h_k for its measurements in the reference's (mi, mj)
 // order, accumulating sum_k w_k h_k in registers.
-constexpr int kSensMaxQ = 16;  // measurements per thread per pass
+constexpr int kSensMaxQ = 8;  // measurements per thread per pass
 
+// kpot[K][E][n] -> kpt[K][n][E]: the sensitivity kernel reads one element's
+// nodes for every electrode, which the transposed copy makes contiguous.
+// Tiles of 64 nodes x 16 electrodes through LDS, both sides coalesced.
+__global__ __launch_bounds__(256) void crt_kpot_transpose_kernel(const double2* __restrict__ kpot, int n, int E,
+                                                                  double2* __restrict__ kpt) {
+  __shared__ double2 tile[16][65];
+  const int k = blockIdx.z;
+  const int n0 = blockIdx.x * 64, a0 = blockIdx.y * 16;
+  const int t = threadIdx.x;
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const int a = a0 + r * 4 + t / 64, node = n0 + (t & 63);
+    if (a < E && node < n) tile[r * 4 + t / 64][t & 63] = kpot[((size_t)k * E + a) * n + node];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const int node = n0 + r * 16 + t / 16, a = a0 + (t & 15);
+    if (a < E && node < n) kpt[((size_t)k * n + node) * E + a] = tile[t & 15][r * 16 + t / 16];
+  }
+}
+
+// Explicit Jacobian, one workgroup per area element e, threads over the
+// measurements i: for every wavenumber k
+//   h_k = sum_{a,b} S_ab(e,k) (phi_e2 - phi_e1)(n_b) (phi_e4 - phi_e3)(n_a)
+// (bsens :137-167, electrode number 0 contributing zero), evaluated as
+// u_a = sum_b S_ab d12_b, h_k = sum_a d34_a u_a, then sum_k w_k h_k / pi
+// (:170-184) and the mode's final form: -that (bsens :187, dV/dsigma) or
+// that * sigma_e / V_i (bsensi :184, d(-ln V)/d(ln sigma)).  The element's
+// potentials of wavenumber k (its <= 4 nodes x all electrodes) and S(e,k)
+// are staged in LDS; rows beyond the element's node count are zero.
 __global__ __launch_bounds__(256) void crt_sens_kernel(SensArgs S) {
   extern __shared__ double2 sm[];
   const int e = blockIdx.x;
   const int nk = S.el_nk[e];
   const int E = S.eanz;
-  cplx* P = reinterpret_cast<cplx*>(sm);  // [q][E]
-  __shared__ double sel[10];
+  cplx* P = reinterpret_cast<cplx*>(sm);  // [4][E]
+  __shared__ double sel[4][4];
   __shared__ int nodes[4];
   if (threadIdx.x < 4) nodes[threadIdx.x] = S.el_nodes[e * 4 + threadIdx.x];
-  const double pi = M_PI;
   const cplx sig = ldc(S.sigma + e);
   for (int i0 = 0; i0 < S.nanz; i0 += kSensMaxQ * blockDim.x) {
     cplx acc[kSensMaxQ];
@@ -298,43 +328,48 @@ __global__ __launch_bounds__(256) void crt_sens_kernel(SensArgs S) {
 #pragma unroll
     for (int q = 0; q < kSensMaxQ; ++q) {
       acc[q] = mkc(0.0, 0.0);
-      const int i = i0 + threadIdx.x + q * blockDim.x;
-      if (i < S.nanz) {
-        e1[q] = S.strnr[i] % 10000; e2[q] = (S.strnr[i] - e1[q]) / 10000;
-        e3[q] = S.vnr[i] % 10000;   e4[q] = (S.vnr[i] - e3[q]) / 10000;
-      } else {
-        e1[q] = e2[q] = e3[q] = e4[q] = 0;
-      }
+      const int i = min(i0 + (int)threadIdx.x + q * (int)blockDim.x, S.nanz - 1);
+      const int st = S.strnr[i], vn = S.vnr[i];
+      e1[q] = st % 10000;
+      e2[q] = st / 10000;
+      e3[q] = vn % 10000;
+      e4[q] = vn / 10000;
     }
     for (int k = 0; k < S.K; ++k) {
       __syncthreads();
-      for (int t = threadIdx.x; t < nk * E; t += blockDim.x) {
+      for (int t = threadIdx.x; t < 4 * E; t += blockDim.x) {
         const int q = t / E, a = t - q * E;
-        P[t] = ldc(S.kpot + ((size_t)k * E + a) * S.n + nodes[q]);
+        P[t] = q < nk ? ldc(S.kpt + ((size_t)k * S.n + nodes[q]) * E + a) : mkc(0.0, 0.0);
       }
-      if (threadIdx.x < S.nimn) sel[threadIdx.x] = S.elbg[((size_t)k * S.elanz + e) * S.nimn + threadIdx.x];
+      if (threadIdx.x < 16) {
+        const int a = threadIdx.x >> 2, b = threadIdx.x & 3;
+        const int imax = max(a, b), imin = min(a, b);
+        sel[a][b] = (a < nk && b < nk) ? S.elbg[((size_t)k * S.elanz + e) * S.nimn + (imax * (imax + 1)) / 2 + imin]
+                                       : 0.0;
+      }
       __syncthreads();
       const double wk = S.kwnwi[k];
 #pragma unroll
       for (int q = 0; q < kSensMaxQ; ++q) {
-        const int i = i0 + threadIdx.x + q * blockDim.x;
-        if (i >= S.nanz) continue;
-        cplx h = mkc(0.0, 0.0);
-        const cplx zero = mkc(0.0, 0.0);
-        for (int mi = 0; mi < nk; ++mi) {
-          const cplx s3 = e3[q] > 0 ? P[mi * E + e3[q] - 1] : zero;
-          const cplx s4 = e4[q] > 0 ? P[mi * E + e4[q] - 1] : zero;
-          const cplx v = csub(s4, s3);
-          for (int mj = 0; mj < nk; ++mj) {
-            const cplx s1 = e1[q] > 0 ? P[mj * E + e1[q] - 1] : zero;
-            const cplx s2 = e2[q] > 0 ? P[mj * E + e2[q] - 1] : zero;
-            const int imax = max(mi, mj), imin = min(mi, mj);
-            const double sv = sel[(imax * (imax + 1)) / 2 + imin];
-            h = cadd(h, rmul(sv, cmul(csub(s2, s1), v)));
-          }
+        cplx d12[4], d34[4];
+#pragma unroll
+        for (int m = 0; m < 4; ++m) {
+          const cplx s1 = csel(e1[q] > 0, P[m * E + max(e1[q] - 1, 0)], mkc(0.0, 0.0));
+          const cplx s2 = csel(e2[q] > 0, P[m * E + max(e2[q] - 1, 0)], mkc(0.0, 0.0));
+          const cplx s3 = csel(e3[q] > 0, P[m * E + max(e3[q] - 1, 0)], mkc(0.0, 0.0));
+          const cplx s4 = csel(e4[q] > 0, P[m * E + max(e4[q] - 1, 0)], mkc(0.0, 0.0));
+          d12[m] = csub(s2, s1);
+          d34[m] = csub(s4, s3);
         }
-        if (S.swrtr == 0) acc[q] = h;
-        else acc[q] = cadd(acc[q], rmul(wk, h));
+        cplx h = mkc(0.0, 0.0);
+#pragma unroll
+        for (int a = 0; a < 4; ++a) {
+          cplx u = mkc(0.0, 0.0);
+#pragma unroll
+          for (int b = 0; b < 4; ++b) u = cadd(u, rmul(sel[a][b], d12[b]));
+          h = cadd(h, cmul(d34[a], u));
+        }
+        acc[q] = S.swrtr == 0 ? h : cadd(acc[q], rmul(wk, h));
       }
     }
 #pragma unroll
@@ -343,10 +378,10 @@ __global__ __launch_bounds__(256) void crt_sens_kernel(SensArgs S) {
       if (i >= S.nanz) continue;
       cplx out;
       if (S.mode == 0) {
-        cplx dum = (S.swrtr == 0) ? acc[q] : rmul(1.0 / pi, acc[q]);
+        cplx dum = (S.swrtr == 0) ? acc[q] : rmul(1.0 / M_PI, acc[q]);
         out = mkc(0.0 - dum.re, 0.0 - dum.im);
       } else {
-        cplx dum = (S.swrtr == 0) ? acc[q] : cdiv(acc[q], mkc(pi, 0.0));
+        cplx dum = (S.swrtr == 0) ? acc[q] : cdiv(acc[q], mkc(M_PI, 0.0));
         out = cdiv(cmul(dum, sig), ldc(S.volt + i));
       }
       stc(S.sens + (size_t)e * S.nanz + i, out);
