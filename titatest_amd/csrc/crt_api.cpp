@@ -46,6 +46,7 @@ struct crt_handle {
   // stage state
   bool precal_done = false, forward_done = false, volt_done = false, sens_done = false;
   bool dc = false;  // the last forward was the DC twin (crt_forward_dc)
+  bool kpt_valid = false;  // "kpt" holds the last forward's kpot transposed
   // CG iterations as one captured HIP graph (crt_set_graph), with the kernel
   // arguments and iteration count it was captured for
   bool use_graph = false;
@@ -939,6 +940,7 @@ static int run_forward(crt_handle* h) {
   }
   h->forward_done = true;
   h->volt_done = h->sens_done = false;
+  h->kpt_valid = false;
   return 0;
 }
 
@@ -1078,6 +1080,21 @@ int crt_voltages_dev(crt_handle* h, int32_t nanz, const int32_t* strnr, const in
   return finish(h);
 }
 
+// kpot transposed to [K][n][E] for the element-wise kernels; built once per
+// forward
+static double2* transposed_kpot(crt_handle* h) {
+  double2* kpt = h->get<double2>("kpt", (size_t)h->K * h->n * h->E);
+  if (!kpt) return nullptr;
+  if (!h->kpt_valid) {
+    tstart(h, "kpot_transpose");
+    hipLaunchKernelGGL(crt_kpot_transpose_kernel, dim3((h->n + 63) / 64, (h->E + 15) / 16, h->K), dim3(256), 0,
+                       h->stream, (const double2*)h->at<double2>("kpot"), h->n, h->E, kpt);
+    tstop(h);
+    h->kpt_valid = true;
+  }
+  return kpt;
+}
+
 static int run_sens(crt_handle* h, int32_t mode, const double2* sigma) {
   SensArgs S;
   S.n = h->n;
@@ -1099,13 +1116,8 @@ static int run_sens(crt_handle* h, int32_t mode, const double2* sigma) {
   S.vnr = h->at<int>("vnr");
   S.sens = h->get<double2>("sens", (size_t)h->nanz * h->elanz);
   NEED(S.sens);
-  double2* kpt = h->get<double2>("kpt", (size_t)h->K * h->n * h->E);
-  NEED(kpt);
-  S.kpt = kpt;
-  tstart(h, "kpot_transpose");
-  hipLaunchKernelGGL(crt_kpot_transpose_kernel, dim3((h->n + 63) / 64, (h->E + 15) / 16, h->K), dim3(256), 0,
-                     h->stream, (const double2*)S.kpot, h->n, h->E, kpt);
-  tstop(h);
+  S.kpt = transposed_kpot(h);
+  NEED(S.kpt);
   const size_t lds = (size_t)4 * h->E * 16;
   tstart(h, "sens");
   hipLaunchKernelGGL(crt_sens_kernel, dim3(h->elanz), dim3(256), lds, h->stream, S);
@@ -1148,9 +1160,30 @@ static int run_adjoint(crt_handle* h, int32_t mode, const double* w_dev, const d
   hipLaunchKernelGGL(crt_adj_gamma_kernel, dim3((N + 255) / 256), dim3(256), 0, h->stream, N, mode, w_dev, r_dev,
                      (const double2*)h->at<double2>("volt"), gamma);
   tstop(h);
+  // measurements per potential electrode (those with e3 or e4 = a), ascending
+  std::vector<int> qptr(E + 1, 0), qlist;
+  for (int i = 0; i < N; ++i) {
+    const int e3 = h->vnr[i] % 10000, e4 = h->vnr[i] / 10000;
+    if (e3 > 0) qptr[e3]++;
+    if (e4 > 0 && e4 != e3) qptr[e4]++;
+  }
+  for (int a = 0; a < E; ++a) qptr[a + 1] += qptr[a];
+  qlist.resize(std::max(qptr[E], 1));
+  {
+    std::vector<int> fill(qptr.begin(), qptr.end() - 1);
+    for (int i = 0; i < N; ++i) {
+      const int e3 = h->vnr[i] % 10000, e4 = h->vnr[i] / 10000;
+      if (e3 > 0) qlist[fill[e3 - 1]++] = i;
+      if (e4 > 0 && e4 != e3) qlist[fill[e4 - 1]++] = i;
+    }
+  }
+  int *dqp, *dql;
+  if (upload(h, "adj_qptr", qptr.data(), qptr.size(), &dqp) || upload(h, "adj_qlist", qlist.data(), qlist.size(), &dql))
+    return kErrHip;
   tstart(h, "adj_cmat");
   hipLaunchKernelGGL(crt_adj_cmat_kernel, dim3((E * E + 255) / 256), dim3(256), 0, h->stream, N, E,
-                     (const int*)h->at<int>("strnr"), (const int*)h->at<int>("vnr"), (const double2*)gamma, C);
+                     (const int*)h->at<int>("strnr"), (const int*)h->at<int>("vnr"), (const int*)dqp,
+                     (const int*)dql, (const double2*)gamma, C);
   tstop(h);
   const size_t lds = (size_t)E * kAdjColsHost * 16;
   tstart(h, "adj_gemm");
@@ -1170,6 +1203,8 @@ static int run_adjoint(crt_handle* h, int32_t mode, const double* w_dev, const d
   A.elbg = h->at<double>("elbg");
   A.kwnwi = h->at<double>("kwnwi");
   A.kpot = h->at<double2>("kpot");
+  A.kpt = transposed_kpot(h);
+  NEED(A.kpt);
   A.Y = Y;
   A.sigma = h->at<double2>("sigma");
   A.grad = g_dev;

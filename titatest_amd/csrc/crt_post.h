@@ -43,6 +43,7 @@ struct AdjArgs {
   const double* elbg;
   const double* kwnwi;
   const double2* kpot;    // [K][eanz][n]
+  const double2* kpt;     // [K][n][eanz] (crt_kpot_transpose_kernel)
   const double2* Y;       // [K][n][eanz]
   const double2* sigma;
   double2* grad;          // elanz
@@ -80,8 +81,8 @@ __global__ void crt_sens_kernel(SensArgs S);
 __global__ void crt_kpot_transpose_kernel(const double2* kpot, int n, int E, double2* kpt);
 __global__ void crt_adj_gamma_kernel(int nanz, int mode, const double* w, const double2* resid,
                                      const double2* volt, double2* gamma);
-__global__ void crt_adj_cmat_kernel(int nanz, int eanz, const int* strnr, const int* vnr,
-                                    const double2* gamma, double2* C);
+__global__ void crt_adj_cmat_kernel(int nanz, int eanz, const int* strnr, const int* vnr, const int* qptr,
+                                    const int* qlist, const double2* gamma, double2* C);
 __global__ void crt_adj_gemm_kernel(const double2* kpot, const double2* C, int n, int eanz, double2* Y);
 __global__ void crt_adj_elem_kernel(AdjArgs A);
 __global__ void crt_jhwr_kernel(const double2* J, int nanz, int manz, const double* w, const double2* r,

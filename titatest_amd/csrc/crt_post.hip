@@ -411,13 +411,17 @@ __global__ void crt_adj_gamma_kernel(int nanz, int mode, const double* __restric
 // C[a][b] (a = receiver electrode, b = source electrode), one thread per entry,
 // summing over measurements in order (deterministic).
 __global__ void crt_adj_cmat_kernel(int nanz, int eanz, const int* __restrict__ strnr,
-                                    const int* __restrict__ vnr, const double2* __restrict__ gamma,
+                                    const int* __restrict__ vnr, const int* __restrict__ qptr,
+                                    const int* __restrict__ qlist, const double2* __restrict__ gamma,
                                     double2* __restrict__ C) {
   const int idx = blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= eanz * eanz) return;
   const int a = idx / eanz + 1, b = idx % eanz + 1;
   cplx s = mkc(0.0, 0.0);
-  for (int i = 0; i < nanz; ++i) {
+  // only the measurements with a among their potential electrodes, in
+  // ascending order (qlist is sorted per electrode)
+  for (int p = qptr[a - 1]; p < qptr[a]; ++p) {
+    const int i = qlist[p];
     const int e1 = strnr[i] % 10000, e2 = strnr[i] / 10000;
     const int e3 = vnr[i] % 10000, e4 = vnr[i] / 10000;
     const int ca = (a == e4) - (a == e3);
@@ -425,6 +429,7 @@ __global__ void crt_adj_cmat_kernel(int nanz, int eanz, const int* __restrict__ 
     const int c = ca * cb;
     if (c) s = cadd(s, rmul((double)c, ldc(gamma + i)));
   }
+  (void)nanz;
   stc(C + (size_t)(a - 1) * eanz + (b - 1), s);
 }
 
@@ -473,31 +478,43 @@ __global__ __launch_bounds__(256) void crt_adj_elem_kernel(AdjArgs A) {
   const int e = wave;
   const int nk = A.el_nk[e];
   int nodes[4];
-  for (int q = 0; q < 4; ++q) nodes[q] = A.el_nodes[e * 4 + q];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) nodes[q] = A.el_nodes[e * 4 + min(q, nk - 1)];
   const int E = A.eanz;
   double accr = 0.0, acci = 0.0;
   for (int k = 0; k < A.K; ++k) {
     const double* sel = A.elbg + ((size_t)k * A.elanz + e) * A.nimn;
+    double sm[4][4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+      for (int b = 0; b < 4; ++b) {
+        const int imax = max(a, b), imin = min(a, b);
+        sm[a][b] = (a < nk && b < nk) ? sel[(imax * (imax + 1)) / 2 + imin] : 0.0;
+      }
     const double wk = (A.swrtr == 0) ? 1.0 : A.kwnwi[k];
+    // Y and the transposed potentials are both [K][n][E]: every load below
+    // reads a contiguous run of electrodes
     const double2* Yk = A.Y + (size_t)k * A.n * E;
-    const double2* Pk = A.kpot + (size_t)k * E * A.n;
+    const double2* Pk = A.kpt + (size_t)k * A.n * E;
     for (int b = lane; b < E; b += 64) {
       cplx yv[4], pv[4];
-      for (int q = 0; q < nk; ++q) {
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
         yv[q] = ldc(Yk + (size_t)nodes[q] * E + b);
-        pv[q] = ldc(Pk + (size_t)b * A.n + nodes[q]);
+        pv[q] = ldc(Pk + (size_t)nodes[q] * E + b);
       }
-      double sr = 0.0, si = 0.0;
-      for (int mi = 0; mi < nk; ++mi)
-        for (int mj = 0; mj < nk; ++mj) {
-          const int imax = max(mi, mj), imin = min(mi, mj);
-          const double sv = sel[(imax * (imax + 1)) / 2 + imin];
-          const cplx t = cmul(yv[mi], pv[mj]);
-          sr += sv * t.re;
-          si += sv * t.im;
-        }
-      accr += wk * sr;
-      acci += wk * si;
+      // sum_{a,b} S_ab Y_a P_b as sum_a Y_a (S P)_a
+      cplx t = mkc(0.0, 0.0);
+#pragma unroll
+      for (int a = 0; a < 4; ++a) {
+        cplx u = mkc(0.0, 0.0);
+#pragma unroll
+        for (int c = 0; c < 4; ++c) u = cadd(u, rmul(sm[a][c], pv[c]));
+        t = cadd(t, cmul(yv[a], u));
+      }
+      accr += wk * t.re;
+      acci += wk * t.im;
     }
   }
   for (int off = 32; off > 0; off >>= 1) {
