@@ -1187,8 +1187,12 @@ static int run_adjoint(crt_handle* h, int32_t mode, const double* w_dev, const d
   tstop(h);
   const size_t lds = (size_t)E * kAdjColsHost * 16;
   tstart(h, "adj_gemm");
-  hipLaunchKernelGGL(crt_adj_gemm_kernel, dim3((n + 255) / 256, (E + kAdjColsHost - 1) / kAdjColsHost, K), dim3(256),
-                     lds, h->stream, (const double2*)h->at<double2>("kpot"), (const double2*)C, n, E, Y);
+  if (E <= 16 * kAdjTilesHost)
+    hipLaunchKernelGGL(crt_adj_gemm_mfma_kernel, dim3((n + 63) / 64, K), dim3(256), 0, h->stream,
+                       (const double2*)h->at<double2>("kpot"), (const double2*)C, n, E, Y);
+  else
+    hipLaunchKernelGGL(crt_adj_gemm_kernel, dim3((n + 255) / 256, (E + kAdjColsHost - 1) / kAdjColsHost, K),
+                       dim3(256), lds, h->stream, (const double2*)h->at<double2>("kpot"), (const double2*)C, n, E, Y);
   tstop(h);
   AdjArgs A;
   A.n = n;

@@ -93,3 +93,5 @@ __global__ void crt_cg_bb_kernel(CgArgs G);
 __global__ void crt_cg_scalar_kernel(CgArgs G, int stage, int iter);
 
 constexpr int kAdjColsHost = 32;
+constexpr int kAdjTilesHost = 8;  // electrodes up to 128 take the matrix-core GEMM
+__global__ void crt_adj_gemm_mfma_kernel(const double2* kpot, const double2* C, int n, int eanz, double2* Y);

@@ -437,6 +437,64 @@ __global__ void crt_adj_cmat_kernel(int nanz, int eanz, const int* __restrict__ 
 // leading dimension n).  Output layout Y[k][j][b] (row per node).  Block: 256
 // rows j x a 32-column slab of C held in LDS.
 constexpr int kAdjCols = 32;
+// Y[k] = P[k]^T C on the matrix cores: v_mfma_f64_16x16x4f64, whose operand
+// A[i][kk] sits in lane i + 16 kk, B[kk][j] in lane j + 16 kk, and whose
+// result D[i][j] sits in lane j + 16 (i mod 4), register i / 4 (lane l,
+// register v: row l / 16 + 4 v, column l % 16).  One wavefront owns 16 nodes and
+// all electrode columns (up to kAdjTiles tiles of 16), so every potential is
+// read once; the complex product takes four real products per 4-wide step
+// (re += Pr Cr - Pi Ci, im += Pr Ci + Pi Cr).  C is read through the cache.
+constexpr int kAdjTiles = 8;
+typedef double v4d __attribute__((ext_vector_type(4)));
+__global__ __launch_bounds__(256) void crt_adj_gemm_mfma_kernel(const double2* __restrict__ kpot,
+                                                              const double2* __restrict__ C, int n, int eanz,
+                                                              double2* __restrict__ Y) {
+  const int k = blockIdx.y;
+  const int lane = threadIdx.x & 63;
+  const int j0 = (blockIdx.x * 4 + (threadIdx.x >> 6)) * 16;
+  if (j0 >= n) return;
+  const int ntile = (eanz + 15) / 16;
+  const int col = lane & 15, kk = lane >> 4;
+  v4d dr[kAdjTiles], di[kAdjTiles];
+#pragma unroll
+  for (int t = 0; t < kAdjTiles; ++t) {
+    dr[t] = v4d{0.0, 0.0, 0.0, 0.0};
+    di[t] = v4d{0.0, 0.0, 0.0, 0.0};
+  }
+  const double2* Pk = kpot + (size_t)k * eanz * n;
+  const int jA = min(j0 + col, n - 1);  // A operand: node j0 + (lane % 16)
+  for (int a0 = 0; a0 < eanz; a0 += 4) {
+    const int a = a0 + kk;
+    const bool aok = a < eanz;
+    const double2 p = Pk[(size_t)min(a, eanz - 1) * n + jA];
+    const double pr = aok && j0 + col < n ? p.x : 0.0, pi = aok && j0 + col < n ? p.y : 0.0;
+#pragma unroll
+    for (int t = 0; t < kAdjTiles; ++t) {
+      if (t < ntile) {
+        const int b = 16 * t + col;
+        const double2 c = C[(size_t)min(a, eanz - 1) * eanz + min(b, eanz - 1)];
+        const bool ok = aok && b < eanz;
+        const double cr = ok ? c.x : 0.0, ci = ok ? c.y : 0.0;
+        dr[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(pr, cr, dr[t], 0, 0, 0);
+        dr[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(pi, -ci, dr[t], 0, 0, 0);
+        di[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(pr, ci, di[t], 0, 0, 0);
+        di[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(pi, cr, di[t], 0, 0, 0);
+      }
+    }
+  }
+#pragma unroll
+  for (int t = 0; t < kAdjTiles; ++t) {
+    if (t < ntile) {
+      const int b = 16 * t + col;
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int j = j0 + kk + 4 * v;
+        if (j < n && b < eanz) Y[((size_t)k * n + j) * eanz + b] = make_double2(dr[t][v], di[t][v]);
+      }
+    }
+  }
+}
+
 __global__ __launch_bounds__(256) void crt_adj_gemm_kernel(const double2* __restrict__ kpot,
                                                          const double2* __restrict__ C, int n, int eanz,
                                                          double2* __restrict__ Y) {
