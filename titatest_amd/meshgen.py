@@ -252,11 +252,12 @@ def grf_model(mesh, seed_mag, seed_pha):
     return 10.0 ** lmag, pha
 
 
-def large_grf(nx_side=72, n_uniform=96, nz=80, eanz=96, seed=1001) -> Workload:
-    """240x80 quads, 96 electrodes at 1.25 m, GRF model, dipole-dipole a=1..4, n=1..10."""
+def large_grf(nx_side=72, n_uniform=96, nz=80, eanz=96, seed=1001, side_type=11) -> Workload:
+    """240x80 quads, 96 electrodes at 1.25 m, GRF model, dipole-dipole a=1..4, n=1..10.
+    side_type 13 gives Dirichlet sides and bottom (one factor per wavenumber)."""
     xn = _graded_axis(n_uniform, 1.25, nx_side, 1.1)
     zn = _depth_axis(nz, 0.3125, 1.05)
-    mesh = _quad_mesh(xn, zn)
+    mesh = _quad_mesh(xn, zn, side_type=side_type)
     mesh.enr = surface_nodes(nz, nx_side + np.arange(eanz))
     mag, pha = grf_model(mesh, seed, seed + 1)
     st, vn = dipole_dipole(eanz, range(1, 5), range(1, 11))
