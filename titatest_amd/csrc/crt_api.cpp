@@ -827,13 +827,14 @@ static int run_forward(crt_handle* h) {
   // factor in the batch stay in device memory until its back substitution
   const bool rb2 = !h->rwdnr.empty();
   const size_t per_sys = (size_t)n * (mb + 1) * 16 + (size_t)n * h->R * 16 + (size_t)n * 24 +
-                         (reuse ? (size_t)E * n * 16 : 0) + (rb2 ? (size_t)n * 16 : 0);
+                         (reuse ? (size_t)E * n * 16 + (size_t)n * (mb + 1) * 16 : 0) + (rb2 ? (size_t)n * 16 : 0);
   size_t free_b = 0, total_b = 0;
   if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) free_b = (size_t)8 << 30;
   size_t budget = std::min<size_t>(free_b / 2, h->batch_bytes_max);
   const size_t have = h->bufsz.count("L") && h->bufsz.count("S")
                           ? h->bufsz["L"] + h->bufsz["S"] + h->bufsz["y"] + h->bufsz["fak"] +
-                                (h->bufsz.count("yr") ? h->bufsz["yr"] : 0)
+                                (h->bufsz.count("yr") ? h->bufsz["yr"] : 0) +
+                                (h->bufsz.count("Lc") ? h->bufsz["Lc"] : 0)
                           : 0;
   budget = std::max(budget, have);
   int batch = (int)std::max<size_t>(1, std::min<size_t>((size_t)nfac, budget / per_sys));
@@ -846,9 +847,12 @@ static int run_forward(crt_handle* h) {
   NEED(A.y);
   NEED(A.fak);
   double2* yr = nullptr;
+  double2* Lc = nullptr;
   if (reuse) {
     yr = h->get<double2>("yr", (size_t)batch * E * n);
+    Lc = h->get<double2>("Lc", (size_t)batch * n * (mb + 1));
     NEED(yr);
+    NEED(Lc);
   }
   BacksolveArgs B;
   B.n = n;
@@ -864,7 +868,7 @@ static int run_forward(crt_handle* h) {
   F.n = n;
   F.mb = mb;
   F.eanz = E;
-  F.L = A.L;
+  F.Lc = Lc;
   F.fak = A.fak;
   F.enr = A.enr;
   F.isdir = A.isdir;
@@ -905,11 +909,17 @@ static int run_forward(crt_handle* h) {
     if (reuse) {
       F.nrhs = nrhs;
       F.fac_first = first;
+      tstart(h, "column_copy");
+      const size_t ncopy = (size_t)cnt * n * (mb + 1);
+      hipLaunchKernelGGL(crt_lcol_kernel, dim3((unsigned)((ncopy + 255) / 256)), dim3(256), 0, h->stream,
+                         (const double2*)A.L, Lc, n, mb, cnt);
+      tstop(h);
       tstart(h, "forward_sweep");
-      if (mb < 128)
-        hipLaunchKernelGGL((crt_fwdsweep_kernel<2, 8>), dim3(nrhs), dim3(64), 0, h->stream, F);
+      // row groups G < 129 - mb (the refill stays ahead of its first update)
+      if (mb <= 120)
+        hipLaunchKernelGGL((crt_fwdsweep_kernel<8, 8>), dim3(nrhs), dim3(64), 0, h->stream, F);
       else
-        hipLaunchKernelGGL((crt_fwdsweep_kernel<4, 4>), dim3(nrhs), dim3(64), 0, h->stream, F);
+        hipLaunchKernelGGL((crt_fwdsweep_kernel<8, 4>), dim3(nrhs), dim3(64), 0, h->stream, F);
       tstop(h);
     }
     B.nrhs = nrhs;

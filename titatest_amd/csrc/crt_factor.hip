@@ -795,104 +795,165 @@ template __global__ void crt_backsolve_kernel<8, 8>(BacksolveArgs);
 template __global__ void crt_backsolve_kernel<8, 4>(BacksolveArgs);
 
 // ---------------------------------------------------------------------------
+// Column-band copy of shared factors for the forward sweep: row j of Lc holds
+// column j of L from the diagonal down, Lc[j][s] = L_{j+s, j} for s <= mb
+// (zero past row n-1).  One thread per entry of Lc, consecutive threads along
+// a row of Lc (coalesced stores); the reads walk an anti-diagonal of L whose
+// lines neighbouring rows of Lc share in cache.
+__global__ void crt_lcol_kernel(const double2* __restrict__ L, double2* __restrict__ Lc, int n, int mb,
+                                int nfac) {
+  const int m1 = mb + 1;
+  const size_t per = (size_t)n * m1;
+  const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= per * nfac) return;
+  const size_t f = idx / per;
+  const int rem = (int)(idx - f * per);
+  const int j = rem / m1, s = rem - j * m1;
+  double2 v = make_double2(0.0, 0.0);
+  if (j + s < n) v = L[f * per + (size_t)(j + s) * m1 + (mb - s)];
+  Lc[idx] = v;
+}
+
+// ---------------------------------------------------------------------------
 // Forward sweep L y = b, one wavefront per right-hand side, for sources that
 // share one factor (no mixed boundaries: the factor of source 1 serves all,
 // src/fem.f90:372-376 with kompb, src/kompb.f90).  b has at most two non-zero
 // rows (the source electrode, the sink), so y is zero up to the first of them
-// and the sweep starts there.  Lane j (mod 64) keeps the partial sums of the
-// rows i = q (mod 64*SLOTS) of the band ahead of the current row; row j's
-// pivot lane divides, the value is broadcast and every lane subtracts
-// L_ij y_j from its rows i in (j, j+mb] -- for each i the terms arrive in
-// ascending j, the order of vre's row sums.  Column j of L is read from the
-// rows below it, DEPTH columns ahead.
-template <int SLOTS, int DEPTH>
+// and the sweep starts at that row's 64-row block.  The mirror image of the
+// back substitution above, on the column-band copy Lc: lane q (q < 64) keeps
+// the partial sums b_i - sum_{j'<j} L_ij' y_j' of the rows i = q and q + 64
+// (mod 128) ahead of step j in two slots.  Row j's pivot lane forms w = -y_j =
+// -(its sum) / L_jj, w is broadcast and every lane adds L_ij w to its rows i
+// in (j, j+mb] -- for each i the terms arrive in ascending j, the order of
+// vre's row sums (src/vre.f90).  Slot A of a lane holds the current block's
+// row 64 b + lane, slot B the next block's; the two swap between blocks.  The
+// pivot lane's slot A takes no update at its own step (the load of its entry
+// is masked), so it keeps row j's final sum; every G rows the group's lanes
+// keep y and refill slot A with b_{j+128}, which column j+128-mb first
+// updates -- G < 129 - mb keeps the refill ahead of it.  Columns of Lc are
+// fetched DEPTH steps ahead with buffer loads whose offsets for entries past
+// the band or past row n-1 lie beyond the end of Lc, so they read zero.
+template <int DEPTH, int G>
 __global__ __launch_bounds__(64) void crt_fwdsweep_kernel(FwdArgs F) {
+  static_assert(DEPTH % G == 0 && 64 % DEPTH == 0, "row groups tile the block");
   const int lane = threadIdx.x;
   const int r = blockIdx.x;
   if (r >= F.nrhs) return;
   const int f = r / F.eanz, l = r % F.eanz;
   const int n = F.n, mb = F.mb, m1 = mb + 1;
-  constexpr int P = 64 * SLOTS;
-  const double2* Lg = F.L + (size_t)f * n * m1;
-  const double* fak = F.fak + (size_t)f * n;
+  // n m1 16 < 2^31 (checked by the host): offsets of rows in [0, n + 128) fit
+  const brsrc Lr = make_rsrc(F.Lc + (size_t)f * n * m1, (unsigned)(n * m1 * 16));
   double2* yg = F.y + (size_t)r * n;
   // b = -f at the source electrode, +f at the sink (the sink wins a tie).
   // Source 0 is assembled with the matrix (kompab, randb: zero on Dirichlet
   // rows; with prescribed values the whole vector comes from crt_rhs_kernel);
   // the others take kompb's vector (src/kompb.f90:39-59), which leaves
   // Dirichlet rows as they are.
-  const int re = F.enr[l], rs = F.lsink ? F.nsink : -1;
   const double2* B0 = (l == 0 && F.rhs) ? F.rhs + (size_t)f * n : nullptr;
-  auto bval = [&](int i) -> cplx {
-    if (B0) return ldc(B0 + i);
-    double v = 0.0;
-    if (i == rs) v = 1.0;
-    else if (i == re) v = -1.0;
-    if (l == 0 && F.isdir && F.isdir[i]) v = 0.0;
-    return mkc(v * fak[i], 0.0);
+  const brsrc Br = make_rsrc(B0 ? (const void*)B0 : (const void*)F.fak, B0 ? (unsigned)(n * 16) : 0u);
+  const int re = F.enr[l], rs = F.lsink ? F.nsink : -1;
+  const double* fak = F.fak + (size_t)f * n;
+  const bool dz = l == 0 && F.isdir;
+  const double bre = B0 || (dz && F.isdir[re]) ? 0.0 : -fak[re];
+  const double brs = B0 || rs < 0 || (dz && F.isdir[rs]) ? 0.0 : fak[rs];
+  // b_i, zero outside [0, n): one buffer load (empty without an assembled
+  // vector) plus the electrode term, no branch.  Split in two so that the
+  // refill values are combined only at the refill (a use next to the load
+  // would wait for every fetch in flight).
+  auto bload = [&](int i) __attribute__((always_inline)) -> cplx {
+    return bldc(Br, (unsigned)i < (unsigned)n ? (unsigned)i * 16u : kBufOut);
+  };
+  auto bterm = [&](int i) __attribute__((always_inline)) -> double {
+    return i == rs ? brs : (i == re ? bre : 0.0);
   };
   const int j0 = B0 ? 0 : (rs >= 0 ? min(re, rs) : re);
-  for (int i = lane; i < j0; i += 64) yg[i] = make_double2(0.0, 0.0);
-  // class q = lane + 64*sl holds row i(j, q) = j + ((q - j) mod P) at step j
-  cplx acc[SLOTS];
-#pragma unroll
-  for (int sl = 0; sl < SLOTS; ++sl) {
-    const int q = lane + 64 * sl;
-    const int i = j0 + ((((q - j0) % P) + P) % P);
-    acc[sl] = i < n ? bval(i) : mkc(0.0, 0.0);
-  }
-  // column j fetched DEPTH steps ahead: this lane's L_ij, and for the pivot
-  // lane L_jj and the refill value b_{j+P}
-  cplx lc[DEPTH][SLOTS];
-  cplx ljj[DEPTH], bnx[DEPTH];
-  auto fetch = [&](int j, cplx* dst, cplx& djj, cplx& dbn) {
-#pragma unroll
-    for (int sl = 0; sl < SLOTS; ++sl) {
-      const int q = lane + 64 * sl;
-      const int i = j + ((((q - j) % P) + P) % P);
-      const int t = j - i + mb;
-      dst[sl] = (j < n && i < n && i > j && t >= 0) ? ldc(Lg + (size_t)i * m1 + t) : mkc(0.0, 0.0);
-    }
-    const bool piv = j < n && lane == (j % P) % 64;
-    djj = piv ? ldc(Lg + (size_t)j * m1 + mb) : mkc(1.0, 0.0);
-    dbn = (piv && j + P < n) ? bval(j + P) : mkc(0.0, 0.0);
-  };
-#pragma unroll
-  for (int d = 0; d < DEPTH; ++d) fetch(j0 + d, lc[d], ljj[d], bnx[d]);
+  const int b0 = j0 >> 6, btop = (n - 1) >> 6;
+  for (int i = lane; i < 64 * b0; i += 64) yg[i] = make_double2(0.0, 0.0);
 
-  for (int jb = j0; jb < n; jb += DEPTH) {
+  const int m16 = m1 * 16;
+  // Entries of column j for both slots: slot A's row is j + rf, rf = (lane -
+  // j mod 64) mod 128, slot B's j + (rf ^ 64); in range for 1 <= rf <=
+  // min(mb, n-1-j).  rf in 16-byte units.
+  auto fetch = [&](int j, cplx* dst) __attribute__((always_inline)) {
+    const unsigned base = (unsigned)j * (unsigned)m16;
+    const unsigned lim = (unsigned)max(min(mb, n - 1 - j), 0) * 16u;
+    const int ra = ((lane - (j & 63)) * 16) & 0x7F0;
+    const int rb = ra ^ 0x400;
+    dst[0] = bldc(Lr, (unsigned)(ra - 16) < lim ? base + (unsigned)ra : kBufOut);
+    dst[1] = bldc(Lr, (unsigned)(rb - 16) < lim ? base + (unsigned)rb : kBufOut);
+  };
+  // L_jj and b_{j+128} of block b's row j = 64 b + lane (L_jj reads zero for
+  // rows past n-1; masked when the block becomes current)
+  auto load_blk = [&](int b, cplx& db, cplx& bb) __attribute__((always_inline)) {
+    const int j = 64 * b + lane;
+    db = bldc(Lr, (unsigned)j * (unsigned)m16);
+    bb = bload(j + 128);
+  };
+  cplx dcur, dnxt, bcur, bnxt, nl, yo = mkc(0.0, 0.0);
+  double ecur;
+  // rows past n-1 (padding of the last block) change nothing: w = 0
+  auto enter_blk = [&](int b) __attribute__((always_inline)) {
+    const cplx rc = crecip(dcur);
+    nl = csel(64 * b + lane < n, mkc(-rc.re, -rc.im), mkc(0.0, 0.0));
+    ecur = bterm(64 * b + lane + 128);
+  };
+  load_blk(b0, dcur, bcur);
+  load_blk(b0 + 1, dnxt, bnxt);
+  cplx ya = bload(64 * b0 + lane), yb = bload(64 * b0 + 64 + lane);  // slots A and B
+  ya.re += bterm(64 * b0 + lane);
+  yb.re += bterm(64 * b0 + 64 + lane);
+  cplx lr[DEPTH][2];
+  // in step order, as the loop refills them: the loop's wait for step j's
+  // entries then counts the same loads behind them on entry as on the back edge
 #pragma unroll
-    for (int d = 0; d < DEPTH; ++d) {
-      const int j = jb + d;
-      if (j < n) {
-        const int qp = j % P;
-        const int plane = qp % 64, pslot = qp / 64;
-        cplx yj = mkc(0.0, 0.0);
-        if (lane == plane) {
-          cplx s = acc[0];
+  for (int d = 0; d < DEPTH; ++d) {
+    fetch(64 * b0 + d, lr[d]);
+    __builtin_amdgcn_sched_barrier(0);
+  }
+  enter_blk(b0);
+
+  for (int b = b0; b <= btop; ++b) {
+    if (b > b0) {
+      dcur = dnxt;
+      bcur = bnxt;
+      enter_blk(b);
+      // the copy out of bnxt comes first, so the load can write its registers
+      // directly (a load into a temporary needs a copy after it, and the copy
+      // a wait for every fetch in flight)
+      asm volatile("" : "+v"(bcur.re), "+v"(bcur.im));
+      __builtin_amdgcn_sched_barrier(0);
+      load_blk(b + 1, dnxt, bnxt);
+    }
+#pragma unroll 1
+    for (int c = 0; c < 64; c += DEPTH) {
 #pragma unroll
-          for (int sl = 1; sl < SLOTS; ++sl)
-            if (sl == pslot) s = acc[sl];
-          yj = cdiv(s, ljj[d]);
-          yg[j] = make_double2(yj.re, yj.im);
-          // the freed slot takes row j+P, first reached by column j+P-mb
-#pragma unroll
-          for (int sl = 0; sl < SLOTS; ++sl)
-            if (sl == pslot) acc[sl] = bnx[d];
+      for (int d = 0; d < DEPTH; ++d) {
+        const int j = 64 * b + c + d;
+        const cplx w = rlc(cmul(ya, nl), j & 63);
+        ya = cmadd(ya, lr[d][0], w);
+        yb = cmadd(yb, lr[d][1], w);
+        // the updates are formed before lr[d] is refilled (otherwise they
+        // may sink below the refill, which then needs a copy and a wait)
+        asm volatile("" : "+v"(ya.re), "+v"(ya.im), "+v"(yb.re), "+v"(yb.im));
+        fetch(j + DEPTH, lr[d]);
+        // the fetch stays here, DEPTH steps ahead of its use (the scheduler
+        // would otherwise sink it next to the use)
+        __builtin_amdgcn_sched_barrier(0);
+        if ((d + 1) % G == 0) {
+          // rows j-G+1 .. j done: their lanes keep y and refill slot A
+          const bool g = (unsigned)(lane - ((j & 63) - (G - 1))) < (unsigned)G;
+          yo = csel(g, cmul(ya, nl), yo);
+          ya = csel(g, mkc(bcur.re + ecur, bcur.im), ya);
         }
-        yj.re = __shfl(yj.re, plane, 64);
-        yj.im = __shfl(yj.im, plane, 64);
-#pragma unroll
-        for (int sl = 0; sl < SLOTS; ++sl) {
-          const int q = lane + 64 * sl;
-          const int i = j + ((((q - j) % P) + P) % P);
-          if (i > j && i - j <= mb && i < n) acc[sl] = cmsub(acc[sl], lc[d][sl], yj);
-        }
-        fetch(j + DEPTH, lc[d], ljj[d], bnx[d]);
       }
     }
+    const int jo = 64 * b + lane;
+    if (jo < n) yg[jo] = make_double2(-yo.re, -yo.im);
+    const cplx t = ya;
+    ya = yb;
+    yb = t;
   }
 }
 
-template __global__ void crt_fwdsweep_kernel<2, 8>(FwdArgs);
-template __global__ void crt_fwdsweep_kernel<4, 4>(FwdArgs);
+template __global__ void crt_fwdsweep_kernel<8, 8>(FwdArgs);
+template __global__ void crt_fwdsweep_kernel<8, 4>(FwdArgs);

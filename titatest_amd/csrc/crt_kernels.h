@@ -75,7 +75,7 @@ struct BacksolveArgs {
 // Forward sweep L y = b for every source of a shared factor (vre :45-56).
 struct FwdArgs {
   int n, mb, eanz, nrhs, fac_first;
-  const double2* L;         // [factor][n][mb+1]
+  const double2* Lc;        // [factor][n][mb+1] column-band copy: Lc[j][s] = L_{j+s, j}
   const double* fak;        // [factor][n]
   const int* enr;
   const uint8_t* isdir;
@@ -92,7 +92,8 @@ template <int ROWS, int MAXT>
 __global__ void crt_factor_kernel(FactorArgs A);
 template <int DEPTH, int G>
 __global__ void crt_backsolve_kernel(BacksolveArgs B);
-template <int SLOTS, int DEPTH>
+__global__ void crt_lcol_kernel(const double2* L, double2* Lc, int n, int mb, int nfac);
+template <int DEPTH, int G>
 __global__ void crt_fwdsweep_kernel(FwdArgs F);
 
 // LDS bytes of the factor kernel for bandwidth mb: the trailing-window rings,
