@@ -372,3 +372,80 @@ def adjoint_grad(sens, wmatd, wdfak, resid):
 
 def nthreads_default():
     return int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+
+
+# --- inversion-loop pieces: dmisft / chkpo2, bpar, bsigma, update step --------
+
+def _inv_lib():
+    L = lib()
+    if not getattr(L, "_inv_bound", False):
+        L.orc_chkpo2.argtypes = [_OrcC, _OrcC, C.c_int, C.POINTER(_OrcC), C.POINTER(_OrcC),
+                                 C.POINTER(C.c_int)]
+        L.orc_dmisft.argtypes = [C.c_int, _z, _z, _d, _d, _d, C.c_void_p, C.c_void_p, C.c_int,
+                                 C.c_double, _d, _i]
+        L.orc_bpar.argtypes = [C.c_int, C.c_void_p, _z, _z]
+        L.orc_bsigma.argtypes = [C.c_int, C.c_void_p, _z, _z]
+        L.orc_update_step.argtypes = [C.c_int, _z, _z, C.c_double, C.c_int, _d]
+        L._inv_bound = True
+    return L
+
+
+class _OrcC(C.Structure):
+    _fields_ = [("re", C.c_double), ("im", C.c_double)]
+
+
+def chkpo2(dati, sigi, ldum):
+    """(cdat, csig, wdlok) of src/dmisft.f90:213-300 for one datum."""
+    cd, cs, wd = _OrcC(), _OrcC(), C.c_int(1)
+    _inv_lib().orc_chkpo2(_OrcC(dati.real, dati.imag), _OrcC(sigi.real, sigi.imag), int(bool(ldum)),
+                          C.byref(cd), C.byref(cs), C.byref(wd))
+    return complex(cd.re, cd.im), complex(cs.re, cs.im), wd.value
+
+
+DMISFT_LPOL, DMISFT_LFPI, DMISFT_LDIFF, DMISFT_LROBUST = 1, 2, 4, 8
+
+
+def dmisft(dat, sigmaa, wmatd, wmatdr, wmatdp, flags=0, l1min=1.0, d0=None, fm0=None):
+    """Returns (stats, wmatd_out, wdlok); stats = nrmsd, betrms, pharms,
+    rmssum, l1rat, npol."""
+    N = len(dat)
+    w = np.array(wmatd, np.float64)
+    st = np.zeros(6)
+    wd = np.zeros(N, np.int32)
+    d0p = np.ascontiguousarray(d0, np.complex128) if d0 is not None else None
+    fmp = np.ascontiguousarray(fm0, np.complex128) if fm0 is not None else None
+    e = _inv_lib().orc_dmisft(N, np.ascontiguousarray(dat, np.complex128),
+                              np.ascontiguousarray(sigmaa, np.complex128), w,
+                              np.ascontiguousarray(wmatdr, np.float64), np.ascontiguousarray(wmatdp, np.float64),
+                              d0p.ctypes.data if d0p is not None else None,
+                              fmp.ctypes.data if fmp is not None else None, int(flags), float(l1min), st, wd)
+    if e:
+        raise OracleError("dmisft", e)
+    return st, w, wd
+
+
+def bpar(sigma, manz=None, mnr=None, par=None):
+    sigma = np.ascontiguousarray(sigma, np.complex128)
+    manz = len(sigma) if manz is None else manz
+    out = np.zeros(manz, np.complex128) if par is None else np.array(par, np.complex128)
+    m = np.ascontiguousarray(mnr, np.int32) if mnr is not None else None
+    _inv_lib().orc_bpar(len(sigma), m.ctypes.data if m is not None else None, sigma, out)
+    return out
+
+
+def bsigma(par, elanz=None, mnr=None):
+    par = np.ascontiguousarray(par, np.complex128)
+    elanz = len(par) if elanz is None else elanz
+    out = np.zeros(elanz, np.complex128)
+    m = np.ascontiguousarray(mnr, np.int32) if mnr is not None else None
+    _inv_lib().orc_bsigma(elanz, m.ctypes.data if m is not None else None, par, out)
+    return out
+
+
+def update_step(par, dpar, step, lphi0=False):
+    """(par_new, bdpar, forced) of src/update.f90:332-351."""
+    p = np.array(par, np.complex128)
+    out = np.zeros(2)
+    _inv_lib().orc_update_step(len(p), p, np.ascontiguousarray(dpar, np.complex128), float(step),
+                               int(bool(lphi0)), out)
+    return p, out[0], int(out[1])

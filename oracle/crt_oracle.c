@@ -1439,3 +1439,135 @@ done:
   free(rvec); free(pvec); free(bvec); free(ap);
   return ncg;
 }
+
+/* ------------------------------------------------------------------------
+ * Inversion-loop pieces around the path: data misfit with polarity folding,
+ * model parameters from and to conductivities, the model update step.
+ * ------------------------------------------------------------------------ */
+
+/* chkpo2 (src/dmisft.f90:213-300): phase-range classes of datum and model
+ * response (imag parts are phases in rad), folded by pi as there; wdlok = 0
+ * marks a polarity mismatch. */
+void orc_chkpo2(orc_c dati, orc_c sigi, int ldum, orc_c *cdat, orc_c *csig, int *wdlok) {
+  const double pi = acos(-1.0);
+  double redat = dati.re, imdat = dati.im, resig = sigi.re, imsig = sigi.im;
+  int idat, isig;
+  if (imdat > pi / 2.0) idat = -1;          /* :254-260 */
+  else if (imdat <= -pi / 2.0) idat = 1;
+  else idat = 0;
+  if (imsig > pi / 2.0) isig = -1;          /* :262-268 */
+  else if (imsig <= -pi / 2.0) isig = 1;
+  else isig = 0;
+  if (idat == 0 && isig != 0) {             /* :270-277 */
+    imsig = imsig + (double)isig * pi;
+    if (!ldum) imdat = imdat - copysign(pi, imdat);
+    *wdlok = 0;
+  } else if (idat != 0 && isig == 0) {      /* :279-285 */
+    if (ldum) imdat = imdat + (double)idat * pi;
+    *wdlok = 0;
+  } else if (idat != 0 && isig != 0) {      /* :287-291 */
+    imsig = imsig + (double)isig * pi;
+    imdat = imdat + (double)idat * pi;
+  }
+  *cdat = cc(redat, imdat);
+  *csig = cc(resig, imsig);
+}
+
+/* dmisft (src/dmisft.f90:17-208) without its text output.  flags: 1 lpol,
+ * 2 lfpi, 4 ldiff (d0, fm0 used), 8 lrobust.  stats: nrmsd, betrms, pharms,
+ * rmssum, l1rat (0 unless robust), npol.  wmatd is updated in place by the
+ * robust reweighting when l1rat > l1min (:140-191).  wdlok (optional) gets
+ * the per-datum polarity flags.  Returns 99 when every datum is excluded. */
+int orc_dmisft(int nanz, const orc_c *dat, const orc_c *sigmaa, double *wmatd,
+               const double *wmatdr, const double *wmatdp, const orc_c *d0, const orc_c *fm0,
+               int flags, double l1min, double *stats, int *wdlok_out) {
+  const int lpol = flags & 1, lfpi = flags & 2, ldiff = flags & 4, lrobust = flags & 8;
+  double *psi = (double *)malloc(sizeof(double) * (size_t)(nanz > 0 ? nanz : 1));
+  int *wdlok = (int *)malloc(sizeof(int) * (size_t)(nanz > 0 ? nanz : 1));
+  double nrmsd = 0.0, betrms = 0.0, pharms = 0.0, l1rat = 0.0;
+  int idum = 0;
+  for (int i = 0; i < nanz; ++i) {          /* :85-123 */
+    orc_c cdat, csig, cdum;
+    wdlok[i] = 1;
+    orc_chkpo2(dat[i], sigmaa[i], lpol, &cdat, &csig, &wdlok[i]);
+    if (!ldiff) cdum = csub(cdat, csig);
+    else cdum = csub(csub(cdat, csig), csub(d0[i], fm0[i]));
+    psi[i] = lfpi ? sqrt(wmatd[i]) * fabs(cdum.im) : sqrt(wmatd[i]) * cabs_(cdum);
+    idum += wdlok[i];
+    nrmsd += psi[i] * psi[i] * (double)wdlok[i];
+    betrms += wmatdr[i] * (double)wdlok[i] * cdum.re * cdum.re;
+    pharms += wmatdp[i] * (double)wdlok[i] * cdum.im * cdum.im;
+  }
+  if (idum == 0) {
+    free(psi);
+    free(wdlok);
+    return 99;
+  }
+  stats[3] = nrmsd;                         /* rmssum, :133-137 */
+  stats[5] = (double)(nanz - idum);         /* npol */
+  nrmsd = sqrt(nrmsd / (double)idum);
+  betrms = sqrt(betrms / (double)idum);
+  pharms = sqrt(pharms / (double)idum);
+  if (lrobust) {                            /* :140-191 */
+    double *eps2 = (double *)malloc(sizeof(double) * (size_t)nanz);
+    double norm = 0.0, norm2 = 0.0;
+    for (int i = 0; i < nanz; ++i) {
+      double dum = 1.0 / sqrt(wmatd[i]);
+      eps2[i] = dum * sqrt(psi[i]);
+      norm = norm + psi[i] * (double)wdlok[i];
+      norm2 = norm2 + psi[i] * (double)wdlok[i] * dum / eps2[i];
+    }
+    for (int i = 0; i < nanz; ++i) eps2[i] = eps2[i] * norm2 / norm;
+    norm2 = 0.0;
+    for (int i = 0; i < nanz; ++i) {
+      double dum = 1.0 / sqrt(wmatd[i]);
+      eps2[i] = fmax(dum, eps2[i]);
+      norm2 = norm2 + psi[i] * (double)wdlok[i] * dum / eps2[i];
+    }
+    l1rat = norm / norm2;
+    if (l1rat > l1min)
+      for (int i = 0; i < nanz; ++i) wmatd[i] = 1.0 / (eps2[i] * eps2[i]);
+    free(eps2);
+  }
+  stats[0] = nrmsd;
+  stats[1] = betrms;
+  stats[2] = pharms;
+  stats[4] = l1rat;
+  if (wdlok_out) memcpy(wdlok_out, wdlok, sizeof(int) * (size_t)nanz);
+  free(psi);
+  free(wdlok);
+  return 0;
+}
+
+/* bpar (src/bpar.f90): par(mnr(i)) = log(sigma(i)) for every element, in
+ * element order (mnr 1-based; NULL = identity). */
+void orc_bpar(int elanz, const int *mnr, const orc_c *sigma, orc_c *par) {
+  for (int i = 0; i < elanz; ++i) par[(mnr ? mnr[i] : i + 1) - 1] = clog_(sigma[i]);
+}
+
+/* bsigma (src/bsigma.f90): sigma(i) = exp(par(mnr(i))). */
+void orc_bsigma(int elanz, const int *mnr, const orc_c *par, orc_c *sigma) {
+  for (int i = 0; i < elanz; ++i) {
+    const orc_c p = par[(mnr ? mnr[i] : i + 1) - 1];
+    const double e = exp(p.re);
+    sigma[i] = cc(e * cos(p.im), e * sin(p.im));
+  }
+}
+
+/* The model update of src/update.f90:332-351: par += step dpar, with the
+ * lphi0 rule (a negative conductivity phase is set to zero, counted), and
+ * bdpar = step * sum |dpar|^2.  out: bdpar, count. */
+void orc_update_step(int manz, orc_c *par, const orc_c *dpar, double step, int lphi0, double *out) {
+  double bdpar = 0.0;
+  int in = 0;
+  for (int j = 0; j < manz; ++j) {
+    par[j] = cadd(par[j], cmul(cc(step, 0.0), dpar[j]));
+    if (lphi0 && par[j].im < 0.0) {
+      par[j] = cc(par[j].re, 0.0);
+      in = in + 1;
+    }
+    bdpar = bdpar + cmul(dpar[j], cconj(dpar[j])).re;
+  }
+  out[0] = bdpar * step;
+  out[1] = (double)in;
+}

@@ -68,4 +68,12 @@ int orc_sens_dc(const orc_mesh *m, int nanz, const int *strnr, const int *vnr, i
 void orc_adjoint_grad(int nanz, int manz, const orc_c *sens, const double *wmatd,
                       const int *wdfak, const orc_c *resid, orc_c *grad);
 
+void orc_chkpo2(orc_c dati, orc_c sigi, int ldum, orc_c *cdat, orc_c *csig, int *wdlok);
+int orc_dmisft(int nanz, const orc_c *dat, const orc_c *sigmaa, double *wmatd,
+               const double *wmatdr, const double *wmatdp, const orc_c *d0, const orc_c *fm0,
+               int flags, double l1min, double *stats, int *wdlok);
+void orc_bpar(int elanz, const int *mnr, const orc_c *sigma, orc_c *par);
+void orc_bsigma(int elanz, const int *mnr, const orc_c *par, orc_c *sigma);
+void orc_update_step(int manz, orc_c *par, const orc_c *dpar, double step, int lphi0, double *out);
+
 #endif

@@ -219,3 +219,63 @@ def forward_sample(sigma, first, last, nthreads=0):
     e = L.ref_forward_sample(np.ascontiguousarray(sigma, np.complex128), first, last, nthreads)
     if e:
         raise RefError("forward_sample", e)
+
+
+# --- inversion-loop pieces (src/dmisft.f90, src/bpar.f90, src/bsigma.f90) ----
+
+def chkpo2(dati, sigi, ldum):
+    L = lib()
+    L.ref_chkpo2.argtypes = [_z, _z, C.c_int, _z, _z, _i]
+    cd = np.zeros(1, np.complex128)
+    cs = np.zeros(1, np.complex128)
+    wd = np.zeros(1, np.int32)
+    L.ref_chkpo2(np.array([dati], np.complex128), np.array([sigi], np.complex128), int(bool(ldum)), cd, cs, wd)
+    return complex(cd[0]), complex(cs[0]), int(wd[0])
+
+
+def dmisft(dat, sigmaa, wmatd, wmatdr, wmatdp, flags=0, l1min=1.0, d0=None, fm0=None, workdir=None):
+    """The reference dmisft; its control files go to ``workdir`` (a short
+    temporary directory by default).  Returns (stats, wmatd_out) with stats =
+    nrmsd, betrms, pharms, rmssum, l1rat, npol."""
+    import tempfile
+    L = lib()
+    L.ref_dmisft.argtypes = [C.c_int, _z, _z, _d, _d, _d, _z, _z, C.c_int, C.c_double, C.c_char_p,
+                             C.c_int, _d]
+    N = len(dat)
+    z = np.zeros(N, np.complex128)
+    w = np.array(wmatd, np.float64)
+    out = np.zeros(6)
+    with tempfile.TemporaryDirectory(dir="/tmp", prefix="crtr") as td:
+        d = (workdir or td).encode()
+        assert len(d) < 60
+        e = L.ref_dmisft(N, np.ascontiguousarray(dat, np.complex128), np.ascontiguousarray(sigmaa, np.complex128),
+                         w, np.ascontiguousarray(wmatdr, np.float64), np.ascontiguousarray(wmatdp, np.float64),
+                         np.ascontiguousarray(d0 if d0 is not None else z, np.complex128),
+                         np.ascontiguousarray(fm0 if fm0 is not None else z, np.complex128),
+                         int(flags), float(l1min), d, len(d), out)
+    if e:
+        raise RefError("dmisft", e)
+    return out, w
+
+
+def bpar(ref_mesh, sigma, manz, mnr, par=None):
+    """bpar for the loaded mesh (a Reference instance holds it)."""
+    del ref_mesh
+    L = lib()
+    L.ref_bpar.argtypes = [C.c_int, _i, _z, _z]
+    out = np.zeros(manz, np.complex128) if par is None else np.array(par, np.complex128)
+    e = L.ref_bpar(manz, np.ascontiguousarray(mnr, np.int32), np.ascontiguousarray(sigma, np.complex128), out)
+    if e:
+        raise RefError("bpar", e)
+    return out
+
+
+def bsigma(ref_mesh, par, elanz, mnr):
+    del ref_mesh
+    L = lib()
+    L.ref_bsigma.argtypes = [C.c_int, _i, _z, _z]
+    out = np.zeros(elanz, np.complex128)
+    e = L.ref_bsigma(len(par), np.ascontiguousarray(mnr, np.int32), np.ascontiguousarray(par, np.complex128), out)
+    if e:
+        raise RefError("bsigma", e)
+    return out

@@ -484,4 +484,124 @@ contains
     c1 = 0d0
   end function ref_sens
 
+  !> Reference chkpo2 (src/dmisft.f90:213-300) on one datum.
+  integer(c_int) function ref_chkpo2(c_dati, c_sigi, c_ldum, c_cdat, c_csig, c_wdlok) &
+       bind(C, name='ref_chkpo2')
+    complex(c_double_complex), intent(in) :: c_dati(1), c_sigi(1)
+    integer(c_int), value :: c_ldum
+    complex(c_double_complex), intent(out) :: c_cdat(1), c_csig(1)
+    integer(c_int), intent(out) :: c_wdlok(1)
+    complex(kind(0d0)) :: cdat, csig
+    integer(kind=4) :: wd
+    logical :: ld
+    wd = 1
+    ld = c_ldum /= 0
+    call chkpo2(c_dati(1), c_sigi(1), cdat, csig, wd, ld)
+    c_cdat(1) = cdat
+    c_csig(1) = csig
+    c_wdlok(1) = wd
+    ref_chkpo2 = 0
+  end function ref_chkpo2
+
+  !> Reference dmisft (src/dmisft.f90:17-208) with llam = lsetup = .false.
+  !> (no text output).  Its control files eps.ctr and run.ctr are created
+  !> empty in the caller's directory c_dir (at most 59 characters), which
+  !> becomes ramd.  flags: 1 lpol, 2 lfpi, 4 ldiff, 8 lrobust.  c_out:
+  !> nrmsd, betrms, pharms, rmssum, l1rat, npol; c_wmatd is updated as dmisft
+  !> updates wmatd.
+  integer(c_int) function ref_dmisft(c_nanz, c_dat, c_sigmaa, c_wmatd, c_wmatdr, c_wmatdp, &
+       c_d0, c_fm0, c_flags, c_l1min, c_dir, c_dirlen, c_out) bind(C, name='ref_dmisft')
+    use invmod, only: dat, wmatd, d0, fm0, lfpi
+    use konvmod, only: nrmsd, betrms, pharms, rmssum, l1rat, l1min, lrobust, ldiff, llam, &
+         lstep, it
+    use pathmod, only: ramd, lnramd, slash
+    integer(c_int), value :: c_nanz, c_flags, c_dirlen
+    complex(c_double_complex), intent(in) :: c_dat(c_nanz), c_sigmaa(c_nanz), c_d0(c_nanz), &
+         c_fm0(c_nanz)
+    real(c_double), intent(inout) :: c_wmatd(c_nanz)
+    real(c_double), intent(in) :: c_wmatdr(c_nanz), c_wmatdp(c_nanz)
+    real(c_double), value :: c_l1min
+    character(kind=c_char), intent(in) :: c_dir(c_dirlen)
+    real(c_double), intent(out) :: c_out(6)
+    integer :: i
+    nanz = c_nanz
+    if (allocated(dat)) deallocate(dat)
+    if (allocated(sigmaa)) deallocate(sigmaa)
+    if (allocated(wmatd)) deallocate(wmatd)
+    if (allocated(wmatdr)) deallocate(wmatdr)
+    if (allocated(wmatdp)) deallocate(wmatdp)
+    if (allocated(d0)) deallocate(d0)
+    if (allocated(fm0)) deallocate(fm0)
+    allocate(dat(nanz), sigmaa(nanz), wmatd(nanz), wmatdr(nanz), wmatdp(nanz), d0(nanz), fm0(nanz))
+    dat = c_dat; sigmaa = c_sigmaa; wmatd = c_wmatd; wmatdr = c_wmatdr; wmatdp = c_wmatdp
+    d0 = c_d0; fm0 = c_fm0
+    lpol = iand(c_flags, 1) /= 0
+    lfpi = iand(c_flags, 2) /= 0
+    ldiff = iand(c_flags, 4) /= 0
+    lrobust = iand(c_flags, 8) /= 0
+    l1min = c_l1min
+    llam = .false.; lstep = .false.; it = 1
+    ramd = ' '
+    do i = 1, c_dirlen
+       ramd(i:i) = c_dir(i)
+    end do
+    lnramd = c_dirlen
+    slash = '/'
+    fprun = 10; fpeps = 15
+    open(fpeps, file=ramd(1:lnramd)//'/eps.ctr', status='replace')
+    close(fpeps)
+    open(fprun, file=ramd(1:lnramd)//'/run.ctr', status='replace')
+    close(fprun)
+    l1rat = 0d0
+    call dmisft(.false.)
+    close(fpeps)
+    close(fprun)
+    c_out = (/ nrmsd, betrms, pharms, rmssum, l1rat, dble(npol) /)
+    c_wmatd = wmatd
+    ref_dmisft = errnr
+  end function ref_dmisft
+
+  !> Reference bpar (src/bpar.f90) and bsigma (src/bsigma.f90) for the loaded
+  !> mesh's elanz elements with the caller's element-to-parameter map (1-based).
+  integer(c_int) function ref_bpar(c_manz, c_mnr, c_sigma, c_par) bind(C, name='ref_bpar')
+    use invmod, only: par
+    integer(c_int), value :: c_manz
+    integer(c_int), intent(in) :: c_mnr(*)
+    complex(c_double_complex), intent(in) :: c_sigma(*)
+    complex(c_double_complex), intent(inout) :: c_par(c_manz)
+    manz = c_manz
+    if (allocated(mnr)) deallocate(mnr)
+    allocate(mnr(elanz))
+    mnr = c_mnr(1:elanz)
+    if (allocated(sigma)) deallocate(sigma)
+    allocate(sigma(elanz))
+    sigma = c_sigma(1:elanz)
+    if (allocated(par)) deallocate(par)
+    allocate(par(manz))
+    par = c_par
+    call bpar()
+    c_par = par
+    ref_bpar = errnr
+  end function ref_bpar
+
+  integer(c_int) function ref_bsigma(c_manz, c_mnr, c_par, c_sigma) bind(C, name='ref_bsigma')
+    use invmod, only: par
+    integer(c_int), value :: c_manz
+    integer(c_int), intent(in) :: c_mnr(*)
+    complex(c_double_complex), intent(in) :: c_par(c_manz)
+    complex(c_double_complex), intent(out) :: c_sigma(*)
+    manz = c_manz
+    if (allocated(mnr)) deallocate(mnr)
+    allocate(mnr(elanz))
+    mnr = c_mnr(1:elanz)
+    if (allocated(sigma)) deallocate(sigma)
+    allocate(sigma(elanz))
+    if (allocated(par)) deallocate(par)
+    allocate(par(manz))
+    par = c_par
+    call bsigma()
+    c_sigma(1:elanz) = sigma
+    ref_bsigma = errnr
+  end function ref_bsigma
+
 end module crt_ref_driver

@@ -147,9 +147,71 @@ def fixture(name, swrtr, lsink=False, nsink=0):
     print(f"wrote ref_{name}.npz  N={len(v0)}")
 
 
+def inversion_steps():
+    """chkpo2 over a grid of datum/response phases spanning its range
+    classes (both ldum), dmisft on seeded data under every switch combination
+    (lpol, lfpi, ldiff, lrobust), and bpar / bsigma on the small block mesh
+    with an identity and a zoned element-to-parameter map."""
+    from oracle import ref
+    pi = np.pi
+    ph = np.array([0.0, 0.1, -0.1, 1.5, -1.5, pi / 2, -pi / 2, np.nextafter(pi / 2, 4.0),
+                   np.nextafter(-pi / 2, -4.0), 2.0, -2.0, 3.1, -3.1])
+    cin = []
+    cout = []
+    for a in ph:
+        for b in ph:
+            for ld in (0, 1):
+                d, s = complex(1.3, a), complex(-0.7, b)
+                cd, cs, wd = ref.chkpo2(d, s, ld)
+                cin.append((d, s, ld))
+                cout.append((cd, cs, wd))
+    rng = np.random.default_rng(2024)
+    N = 400
+    dat = rng.normal(0, 1, N) + 1j * rng.uniform(-2.5, 2.5, N)
+    sig = dat + 0.05 * (rng.normal(size=N) + 1j * rng.normal(size=N))
+    sig[::17] += 2.0j
+    wmatd = 1.0 / (0.02 + 1e-3 * rng.uniform(size=N)) ** 2
+    wmatdr = rng.uniform(0.5, 2, N) * wmatd
+    wmatdp = rng.uniform(0.5, 2, N) * wmatd
+    d0 = rng.normal(size=N) + 0.01j * rng.normal(size=N)
+    fm0 = d0 + 0.01 * rng.normal(size=N)
+    flag_sets = np.arange(16, dtype=np.int32)
+    l1min = 1.0
+    stats = np.zeros((16, 6))
+    wout = np.zeros((16, N))
+    for f in flag_sets:
+        stats[f], wout[f] = ref.dmisft(dat, sig, wmatd, wmatdr, wmatdp, int(f), l1min, d0, fm0)
+    w = meshgen.small_block()
+    r = Reference(w.mesh)
+    el = w.mesh.elanz
+    mnr_id = np.arange(1, el + 1, dtype=np.int32)
+    mnr_z = (np.arange(el) // 3 + 1).astype(np.int32)
+    par_id = ref.bpar(r, w.sigma, el, mnr_id)
+    par_z = ref.bpar(r, w.sigma, el // 3 + 1 if el % 3 else el // 3, mnr_z)
+    par_step = par_z + 0.05 * (rng.normal(size=len(par_z)) + 1j * rng.normal(size=len(par_z)))
+    sig_z = ref.bsigma(r, par_step, el, mnr_z)
+    np.savez_compressed(
+        OUT / "ref_inversion_steps.npz",
+        chk_in_d=np.array([c[0] for c in cin]), chk_in_s=np.array([c[1] for c in cin]),
+        chk_in_ld=np.array([c[2] for c in cin], np.int32),
+        chk_cdat=np.array([c[0] for c in cout]), chk_csig=np.array([c[1] for c in cout]),
+        chk_wdlok=np.array([c[2] for c in cout], np.int32),
+        dat=dat, sigmaa=sig, wmatd=wmatd, wmatdr=wmatdr, wmatdp=wmatdp, d0=d0, fm0=fm0,
+        l1min=l1min, flags=flag_sets, stats=stats, wmatd_out=wout,
+        mnr_z=mnr_z, par_id=par_id, par_z=par_z, par_step=par_step, sigma_z=sig_z)
+    print(f"wrote ref_inversion_steps.npz  chkpo2 cases={len(cin)} N={N}")
+
+
+CASES = {"inversion": inversion_steps}
+
+
 def main():
     if not available():
         sys.exit("oracle/_ref/libcrtref.so is not built (make -C oracle ref)")
+    if len(sys.argv) > 1:  # named cases only
+        for name in sys.argv[1:]:
+            CASES[name]()
+        return
     synthetic(meshgen.small_block())
     synthetic(meshgen.small_dirichlet())
     synthetic_dc(meshgen.small_block())
@@ -161,6 +223,7 @@ def main():
     fixture("mixed_surf", 1)
     fixture("decoupling", 0, lsink=True, nsink=373)
     fixture("rect_complex", 1)
+    inversion_steps()
 
 
 if __name__ == "__main__":
