@@ -185,6 +185,59 @@ int crt_factor_phase_ticks(crt_handle *h, int32_t enable, uint64_t *ticks8);
 /* Device pointer of the last forward's potentials kpot (sanz x eanz x K complex). */
 int crt_kpot_device(crt_handle *h, const double **kpot_dev);
 
+/* ---- Inversion-loop pieces around the path (one Gauss-Newton iteration can
+ * stay on the device: forward -> voltages -> misfit -> update -> sigma).
+ * Complex arrays are interleaved (re, im) doubles. ---- */
+
+/* dmisft switches: the reference's lpol, lfpi, ldiff, lrobust */
+#define CRT_MISFIT_LPOL 1
+#define CRT_MISFIT_LFPI 2
+#define CRT_MISFIT_LDIFF 4
+#define CRT_MISFIT_LROBUST 8
+
+/* Replaces dmisft (src/dmisft.f90:17-208, its chkpo2 polarity folding at
+ * :213-300) as called from src/inv.f90:524 and :677, without the eps.ctr text
+ * output.  dat, sigmaa (nanz complex): measured and modelled -ln V; wmatd
+ * (in/out: the robust reweighting replaces it when l1rat > l1min), wmatdr,
+ * wmatdp: the error weights of datmod; d0, fm0: difference-inversion data
+ * (read only with CRT_MISFIT_LDIFF, may be NULL otherwise).  stats (6):
+ * nrmsd, betrms, pharms, rmssum, l1rat, npol.  wdlok (nanz, may be NULL): 0
+ * for a datum excluded by its polarity.  Returns 99 (the reference's errnr)
+ * when every datum is excluded.  The sums run in a fixed tree order, not
+ * sequentially: results agree with the reference to rounding. */
+int crt_dmisft(crt_handle *h, int32_t nanz, const double *dat, const double *sigmaa, double *wmatd,
+               const double *wmatdr, const double *wmatdp, const double *d0, const double *fm0,
+               int32_t flags, double l1min, double *stats, int32_t *wdlok);
+/* The same on device pointers, queued on the handle's stream without a host
+ * synchronisation (capturable).  wdlok is required; stats[5] == nanz
+ * signals errnr 99. */
+int crt_dmisft_dev(crt_handle *h, int32_t nanz, const double *dat_dev, const double *sigmaa_dev,
+                   double *wmatd_dev, const double *wmatdr_dev, const double *wmatdp_dev, const double *d0_dev,
+                   const double *fm0_dev, int32_t flags, double l1min, double *stats_dev, int32_t *wdlok_dev);
+
+/* Element-to-parameter map of modelmod (mnr, 1-based, elanz entries; NULL =
+ * identity with manz = elanz), used by crt_bpar and crt_bsigma.  Replaces the
+ * set-up of modelmod::mnr and manz (src/rall.f90:686-692, :719-728). */
+int crt_set_model_map(crt_handle *h, int32_t manz, const int32_t *mnr);
+/* Replaces bpar (src/bpar.f90, called at src/inv.f90:924): par(mnr(i)) =
+ * log(sigma(i)) in element order (the last element of a parameter wins);
+ * parameters without elements keep their value in par (in/out, manz
+ * complex). */
+int crt_bpar(crt_handle *h, const double *sigma, double *par);
+int crt_bpar_dev(crt_handle *h, const double *sigma_dev, double *par_dev);
+/* Replaces bsigma (src/bsigma.f90, called at src/inv.f90:933): sigma(i) =
+ * exp(par(mnr(i))). */
+int crt_bsigma(crt_handle *h, const double *par, double *sigma);
+int crt_bsigma_dev(crt_handle *h, const double *par_dev, double *sigma_dev);
+/* Replaces the model update of update (src/update.f90:332-351, called at
+ * src/inv.f90:929): par += step dpar (manz complex, rounded as the reference's
+ * complex product then sum), with lphi0 a negative phase set to zero; out
+ * (2): bdpar = step sum |dpar|^2 and the count of zeroed phases. */
+int crt_update_step(crt_handle *h, int32_t manz, double *par, const double *dpar, double step, int32_t lphi0,
+                    double *out);
+int crt_update_step_dev(crt_handle *h, int32_t manz, double *par_dev, const double *dpar_dev, double step,
+                        int32_t lphi0, double *out_dev);
+
 #ifdef __cplusplus
 }
 #endif

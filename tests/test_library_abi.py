@@ -47,6 +47,12 @@ def test_null_arguments_are_rejected_without_a_device():
     assert lib.crt_smoothness(None, 0, 1.0, 1.0, 1, 1, None, None) == 1000
     assert lib.crt_set_graph(None, 1) == 1000
     assert lib.crt_set_device_budget(None, 1 << 30) == 1000
+    assert lib.crt_dmisft(None, 4, None, None, None, None, None, None, None, 0, 1.0, None, None) == 1000
+    assert lib.crt_dmisft_dev(None, 4, None, None, None, None, None, None, None, 0, 1.0, None, None) == 1000
+    assert lib.crt_set_model_map(None, 0, None) == 1000
+    assert lib.crt_bpar(None, None, None) == 1000
+    assert lib.crt_bsigma_dev(None, None, None) == 1000
+    assert lib.crt_update_step(None, 4, None, None, 1.0, 0, None) == 1000
 
 
 def test_fortran_host_built_and_linked():

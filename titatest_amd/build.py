@@ -18,7 +18,7 @@ PKG = Path(__file__).resolve().parent
 CSRC = PKG / "csrc"
 OUT = PKG / "libcrt_hip.so"
 BUILD = PKG / "_build"
-SOURCES = ["crt_factor.hip", "crt_post.hip", "crt_api.cpp"]
+SOURCES = ["crt_factor.hip", "crt_post.hip", "crt_inv.hip", "crt_api.cpp"]
 ARCH = os.environ.get("CRT_OFFLOAD_ARCH", "gfx950")
 
 

@@ -38,14 +38,19 @@ EXPORTS = (
     "crt_cg_solve", "crt_forward_dc", "crt_sensitivities_dc", "crt_set_boundary_values",
     "crt_smoothness", "crt_set_graph", "crt_set_device_budget",
     "crt_error_info", "crt_set_timing", "crt_stage_times", "crt_factor_phase_ticks",
-    "crt_kpot_device",
+    "crt_kpot_device", "crt_dmisft", "crt_dmisft_dev", "crt_set_model_map", "crt_bpar", "crt_bpar_dev",
+    "crt_bsigma", "crt_bsigma_dev", "crt_update_step", "crt_update_step_dev",
 )
+
+# dmisft switches (the reference's lpol, lfpi, ldiff, lrobust)
+MISFIT_LPOL, MISFIT_LFPI, MISFIT_LDIFF, MISFIT_LROBUST = 1, 2, 4, 8
 
 ERRORS = {
     18: "unknown element type", 19: "band width too small for an element",
     26: "element geometry (non-positive determinant)", 27: "zero diagonal in scaling",
     28: "zero pivot in the Cholesky factorisation", 29: "electrode node out of range",
     35: "mixed-boundary geometry", 46: "electrode number out of range",
+    99: "every datum excluded by its polarity",
     73: "duplicate electrodes", 82: "zero voltage", 101: "mixed and Dirichlet boundaries",
     111: "boundary element not linked to an area element",
     1000: "invalid argument", 1001: "HIP runtime failure", 1002: "stage called out of order",
@@ -107,6 +112,15 @@ def load_library(path: Path = LIB_PATH):
     L.crt_stage_times.argtypes = [vp, i32, vp, vp, vp]
     L.crt_kpot_device.argtypes = [vp, C.POINTER(vp)]
     L.crt_factor_phase_ticks.argtypes = [vp, i32, vp]
+    L.crt_dmisft.argtypes = [vp, i32, vp, vp, vp, vp, vp, vp, vp, i32, f64, vp, vp]
+    L.crt_dmisft_dev.argtypes = [vp, i32, vp, vp, vp, vp, vp, vp, vp, i32, f64, vp, vp]
+    L.crt_set_model_map.argtypes = [vp, i32, vp]
+    L.crt_bpar.argtypes = [vp, vp, vp]
+    L.crt_bpar_dev.argtypes = [vp, vp, vp]
+    L.crt_bsigma.argtypes = [vp, vp, vp]
+    L.crt_bsigma_dev.argtypes = [vp, vp, vp]
+    L.crt_update_step.argtypes = [vp, i32, vp, vp, f64, i32, vp]
+    L.crt_update_step_dev.argtypes = [vp, i32, vp, vp, f64, i32, vp]
     _lib = L
     return L
 
@@ -310,6 +324,47 @@ class CRT:
     def set_device_budget(self, nbytes):
         """Cap the device memory a forward keeps for factors in flight."""
         self._check("set_device_budget", self.lib.crt_set_device_budget(self.h, int(nbytes)))
+
+    # inversion-loop pieces -----------------------------------------------
+    def dmisft(self, dat, sigmaa, wmatd, wmatdr, wmatdp, flags=0, l1min=1.0, d0=None, fm0=None):
+        """Data misfit of src/dmisft.f90: returns (stats, wmatd_out, wdlok),
+        stats = nrmsd, betrms, pharms, rmssum, l1rat, npol."""
+        N = len(dat)
+        w = np.array(wmatd, np.float64)
+        st = np.zeros(6)
+        wd = np.zeros(N, np.int32)
+        d0 = None if d0 is None else _c128(d0)
+        fm0 = None if fm0 is None else _c128(fm0)
+        self._check("dmisft", self.lib.crt_dmisft(
+            self.h, N, _ptr(_c128(dat)), _ptr(_c128(sigmaa)), _ptr(w),
+            _ptr(np.ascontiguousarray(wmatdr, np.float64)), _ptr(np.ascontiguousarray(wmatdp, np.float64)),
+            _ptr(d0), _ptr(fm0), int(flags), float(l1min), _ptr(st), _ptr(wd)))
+        return st, w, wd
+
+    def set_model_map(self, manz=0, mnr=None):
+        """Element-to-parameter map (1-based, elanz entries), None = identity."""
+        self._mnr = None if mnr is None else np.ascontiguousarray(mnr, np.int32)
+        self._manz = self.elanz if mnr is None else int(manz)
+        self._check("set_model_map", self.lib.crt_set_model_map(self.h, self._manz, _ptr(self._mnr)))
+
+    def bpar(self, sigma, par=None):
+        manz = getattr(self, "_manz", self.elanz)
+        out = np.zeros(manz, np.complex128) if par is None else np.array(par, np.complex128)
+        self._check("bpar", self.lib.crt_bpar(self.h, _ptr(_c128(sigma)), _ptr(out)))
+        return out
+
+    def bsigma(self, par):
+        out = np.zeros(self.elanz, np.complex128)
+        self._check("bsigma", self.lib.crt_bsigma(self.h, _ptr(_c128(par)), _ptr(out)))
+        return out
+
+    def update_step(self, par, dpar, step, lphi0=False):
+        """par + step dpar (src/update.f90:332-351): (par_new, bdpar, forced)."""
+        p = np.array(par, np.complex128)
+        out = np.zeros(2)
+        self._check("update_step", self.lib.crt_update_step(self.h, len(p), _ptr(p), _ptr(_c128(dpar)),
+                                                            float(step), int(bool(lphi0)), _ptr(out)))
+        return p, out[0], int(out[1])
 
     def set_graph(self, on=True):
         """Run crt_cg_solve's iterations as one captured HIP graph."""

@@ -12,6 +12,7 @@
 #include <vector>
 
 #include "../../include/crt_hip.h"
+#include "crt_inv.h"
 #include "crt_kernels.h"
 #include "crt_post.h"
 
@@ -50,6 +51,9 @@ struct crt_handle {
   // CG iterations as one captured HIP graph (crt_set_graph), with the kernel
   // arguments and iteration count it was captured for
   bool use_graph = false;
+  // element-to-parameter map (modelmod::mnr) for bpar/bsigma; empty = identity
+  int manz_map = 0;
+  std::vector<int> mnr_host;
   hipGraph_t cg_graph = nullptr;
   hipGraphExec_t cg_exec = nullptr;
   std::vector<unsigned char> cg_key;
@@ -1570,3 +1574,173 @@ int crt_kpot_device(crt_handle* h, const double** kpot_dev) {
 }
 
 }  // extern "C"
+
+// ---- inversion-loop pieces (src/dmisft.f90, src/bpar.f90, src/bsigma.f90,
+// src/update.f90:332-351) ----
+
+int crt_dmisft_dev(crt_handle* h, int32_t nanz, const double* dat_dev, const double* sigmaa_dev, double* wmatd_dev,
+                   const double* wmatdr_dev, const double* wmatdp_dev, const double* d0_dev, const double* fm0_dev,
+                   int32_t flags, double l1min, double* stats_dev, int32_t* wdlok_dev) {
+  if (!h || nanz <= 0 || !dat_dev || !sigmaa_dev || !wmatd_dev || !wmatdr_dev || !wmatdp_dev || !stats_dev ||
+      !wdlok_dev || (flags & ~15))
+    return kErrArg;
+  if ((flags & kMisfitLdiff) && (!d0_dev || !fm0_dev)) return kErrArg;
+  MisfitArgs A;
+  A.nanz = nanz;
+  A.flags = flags;
+  A.l1min = l1min;
+  A.dat = reinterpret_cast<const double2*>(dat_dev);
+  A.sigmaa = reinterpret_cast<const double2*>(sigmaa_dev);
+  A.wmatd = wmatd_dev;
+  A.wmatdr = wmatdr_dev;
+  A.wmatdp = wmatdp_dev;
+  A.d0 = reinterpret_cast<const double2*>(d0_dev);
+  A.fm0 = reinterpret_cast<const double2*>(fm0_dev);
+  A.psi = h->get<double>("mis_psi", nanz);
+  NEED(A.psi);
+  A.wdlok = wdlok_dev;
+  A.stats = stats_dev;
+  tstart(h, "misfit");
+  hipLaunchKernelGGL(crt_misfit_kernel, dim3(1), dim3(kInvThreads), 0, h->stream, A);
+  tstop(h);
+  return hipGetLastError() == hipSuccess ? 0 : kErrHip;
+}
+
+int crt_dmisft(crt_handle* h, int32_t nanz, const double* dat, const double* sigmaa, double* wmatd,
+               const double* wmatdr, const double* wmatdp, const double* d0, const double* fm0, int32_t flags,
+               double l1min, double* stats, int32_t* wdlok) {
+  if (!h || nanz <= 0 || !dat || !sigmaa || !wmatd || !wmatdr || !wmatdp || !stats || (flags & ~15))
+    return kErrArg;
+  const bool ldiff = flags & kMisfitLdiff;
+  if (ldiff && (!d0 || !fm0)) return kErrArg;
+  double2 *dd, *ds, *d0d = nullptr, *f0d = nullptr;
+  double *dw, *dwr, *dwp;
+  int rc = upload(h, "mis_dat", reinterpret_cast<const double2*>(dat), nanz, &dd);
+  rc |= upload(h, "mis_sig", reinterpret_cast<const double2*>(sigmaa), nanz, &ds);
+  rc |= upload(h, "mis_w", wmatd, nanz, &dw);
+  rc |= upload(h, "mis_wr", wmatdr, nanz, &dwr);
+  rc |= upload(h, "mis_wp", wmatdp, nanz, &dwp);
+  if (ldiff) {
+    rc |= upload(h, "mis_d0", reinterpret_cast<const double2*>(d0), nanz, &d0d);
+    rc |= upload(h, "mis_fm0", reinterpret_cast<const double2*>(fm0), nanz, &f0d);
+  }
+  if (rc) return kErrHip;
+  int* dwd = h->get<int>("mis_wdlok", nanz);
+  double* dst = h->get<double>("mis_stats", 6);
+  NEED(dwd);
+  NEED(dst);
+  rc = crt_dmisft_dev(h, nanz, reinterpret_cast<const double*>(dd), reinterpret_cast<const double*>(ds), dw, dwr,
+                      dwp, reinterpret_cast<const double*>(d0d), reinterpret_cast<const double*>(f0d), flags, l1min,
+                      dst, dwd);
+  if (rc) return rc;
+  rc = finish(h);
+  if (rc) return rc;
+  double st[6];
+  HIPCHK(hipMemcpy(st, dst, sizeof(st), hipMemcpyDeviceToHost));
+  if (st[5] == (double)nanz) return 99;  // dmisft :125-130: every datum excluded
+  std::memcpy(stats, st, sizeof(st));
+  HIPCHK(hipMemcpy(wmatd, dw, (size_t)nanz * 8, hipMemcpyDeviceToHost));
+  if (wdlok) HIPCHK(hipMemcpy(wdlok, dwd, (size_t)nanz * 4, hipMemcpyDeviceToHost));
+  return 0;
+}
+
+int crt_set_model_map(crt_handle* h, int32_t manz, const int32_t* mnr) {
+  if (!h) return kErrArg;
+  const int el = h->elanz;
+  if (!mnr) {
+    if (manz != el && manz != 0) return kErrArg;
+    h->manz_map = el;
+    h->mnr_host.clear();
+    return 0;
+  }
+  if (manz <= 0) return kErrArg;
+  std::vector<int> last(manz, -1);
+  for (int i = 0; i < el; ++i) {
+    if (mnr[i] < 1 || mnr[i] > manz) return kErrArg;
+    last[mnr[i] - 1] = i;  // bpar's element loop: the last element wins
+  }
+  int* dm;
+  int* dl;
+  if (upload(h, "model_mnr", mnr, el, &dm) || upload(h, "model_last", last.data(), manz, &dl)) return kErrHip;
+  HIPCHK(hipStreamSynchronize(h->stream));  // the host vector above goes out of scope
+  h->manz_map = manz;
+  h->mnr_host.assign(mnr, mnr + el);
+  return 0;
+}
+
+static int model_manz(crt_handle* h) { return h->mnr_host.empty() ? h->elanz : h->manz_map; }
+
+int crt_bpar_dev(crt_handle* h, const double* sigma_dev, double* par_dev) {
+  if (!h || !sigma_dev || !par_dev) return kErrArg;
+  const int manz = model_manz(h);
+  const int* last = h->mnr_host.empty() ? nullptr : h->at<int>("model_last");
+  hipLaunchKernelGGL(crt_bpar_kernel, dim3((manz + 255) / 256), dim3(256), 0, h->stream, manz, last,
+                     reinterpret_cast<const double2*>(sigma_dev), reinterpret_cast<double2*>(par_dev));
+  return hipGetLastError() == hipSuccess ? 0 : kErrHip;
+}
+
+int crt_bsigma_dev(crt_handle* h, const double* par_dev, double* sigma_dev) {
+  if (!h || !par_dev || !sigma_dev) return kErrArg;
+  const int* mnr = h->mnr_host.empty() ? nullptr : h->at<int>("model_mnr");
+  hipLaunchKernelGGL(crt_bsigma_kernel, dim3((h->elanz + 255) / 256), dim3(256), 0, h->stream, h->elanz, mnr,
+                     reinterpret_cast<const double2*>(par_dev), reinterpret_cast<double2*>(sigma_dev));
+  return hipGetLastError() == hipSuccess ? 0 : kErrHip;
+}
+
+int crt_bpar(crt_handle* h, const double* sigma, double* par) {
+  if (!h || !sigma || !par) return kErrArg;
+  const int manz = model_manz(h);
+  double2 *ds, *dp;
+  if (upload(h, "bp_sigma", reinterpret_cast<const double2*>(sigma), h->elanz, &ds) ||
+      upload(h, "bp_par", reinterpret_cast<const double2*>(par), manz, &dp))
+    return kErrHip;
+  int rc = crt_bpar_dev(h, reinterpret_cast<const double*>(ds), reinterpret_cast<double*>(dp));
+  if (rc) return rc;
+  rc = finish(h);
+  if (rc) return rc;
+  HIPCHK(hipMemcpy(par, dp, (size_t)manz * 16, hipMemcpyDeviceToHost));
+  return 0;
+}
+
+int crt_bsigma(crt_handle* h, const double* par, double* sigma) {
+  if (!h || !par || !sigma) return kErrArg;
+  const int manz = model_manz(h);
+  double2* dp;
+  if (upload(h, "bp_par", reinterpret_cast<const double2*>(par), manz, &dp)) return kErrHip;
+  double2* ds = h->get<double2>("bp_sigma", h->elanz);
+  NEED(ds);
+  int rc = crt_bsigma_dev(h, reinterpret_cast<const double*>(dp), reinterpret_cast<double*>(ds));
+  if (rc) return rc;
+  rc = finish(h);
+  if (rc) return rc;
+  HIPCHK(hipMemcpy(sigma, ds, (size_t)h->elanz * 16, hipMemcpyDeviceToHost));
+  return 0;
+}
+
+int crt_update_step_dev(crt_handle* h, int32_t manz, double* par_dev, const double* dpar_dev, double step,
+                        int32_t lphi0, double* out_dev) {
+  if (!h || manz <= 0 || !par_dev || !dpar_dev || !out_dev) return kErrArg;
+  hipLaunchKernelGGL(crt_update_step_kernel, dim3(1), dim3(kInvThreads), 0, h->stream, manz,
+                     reinterpret_cast<double2*>(par_dev), reinterpret_cast<const double2*>(dpar_dev), step,
+                     lphi0 ? 1 : 0, out_dev);
+  return hipGetLastError() == hipSuccess ? 0 : kErrHip;
+}
+
+int crt_update_step(crt_handle* h, int32_t manz, double* par, const double* dpar, double step, int32_t lphi0,
+                    double* out) {
+  if (!h || manz <= 0 || !par || !dpar || !out) return kErrArg;
+  double2 *dp, *dd;
+  if (upload(h, "up_par", reinterpret_cast<const double2*>(par), manz, &dp) ||
+      upload(h, "up_dpar", reinterpret_cast<const double2*>(dpar), manz, &dd))
+    return kErrHip;
+  double* dout = h->get<double>("up_out", 2);
+  NEED(dout);
+  int rc = crt_update_step_dev(h, manz, reinterpret_cast<double*>(dp), reinterpret_cast<const double*>(dd), step,
+                               lphi0, dout);
+  if (rc) return rc;
+  rc = finish(h);
+  if (rc) return rc;
+  HIPCHK(hipMemcpy(par, dp, (size_t)manz * 16, hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(out, dout, 16, hipMemcpyDeviceToHost));
+  return 0;
+}
