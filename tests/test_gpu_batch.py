@@ -104,3 +104,30 @@ def test_uneven_batch_gn_step(crtmod, oracle_lib):
         dpo, _, ncg = oracle_lib.cjggra(J, m.wgt, ones, cgfac, bvec, 10.0, 0, nx, m.smatm, 10, 0.0)
         assert ncg == 10
         assert rel_err(dpar, dpo * cgfac) < STOL
+
+
+def test_gn_iterations_reduce_the_misfit(crtmod, oracle_lib):
+    """Three fixed-lambda Gauss-Newton iterations (forward, misfit, J, CG,
+    bpar -> update -> bsigma, all through the C-ABI) move the model towards
+    the data: the misfit falls, and each iteration's misfit is the oracle's
+    dmisft of the same modelled data."""
+    from titatest_amd import batch
+    from oracle import cpu
+    (w, nx, nz), = _small_models(1)
+    say("gn iterations", w.mesh.sanz, w.mesh.eanz, w.nanz)
+    m = batch.prepare(w, nx, nz, seed=3, contrast=2.0)
+    # from the homogeneous model of the same mean log conductivity; lam keeps
+    # the steps in the linear range (no line search); the oracle's chain on
+    # this model gives nrmsd 43.5, 13.3, 10.5, 9.48
+    s0 = np.full_like(w.sigma, np.exp(np.mean(np.log(w.sigma))))
+    hist, sig = batch.gn_iterate(m, iters=3, lam=1e4, sigma0=s0)
+    nr = [h[0] for h in hist]
+    assert nr[0] > nr[1] > nr[2] > nr[3], nr
+    assert abs(nr[0] - 43.4673) < 1e-3 and nr[3] < 0.25 * nr[0], nr
+    assert np.all(np.isfinite(sig)) and not np.array_equal(sig, w.sigma)
+    g = m.g
+    g.forward(sig)
+    _, sa = g.bvolti(w.strnr, w.vnr)
+    so, _, _ = cpu.dmisft(m.dobs, sa, m.wgt, m.wgt, m.wgt)
+    assert abs(nr[3] - so[0]) <= 1e-13 * so[0]
+    assert abs(hist[3][1] - so[1]) <= 1e-13 * so[1] and abs(hist[3][2] - so[2]) <= 1e-13 * so[2]

@@ -26,7 +26,7 @@ import numpy as np
 from . import crt as _crt
 from .meshgen import Workload, centroids
 
-__all__ = ["GNModel", "prepare", "gn_step", "run_batch", "rm_rect"]
+__all__ = ["GNModel", "prepare", "gn_step", "gn_iterate", "run_batch", "rm_rect"]
 
 
 def rm_rect(smatm: np.ndarray, nx: int, m: np.ndarray) -> np.ndarray:
@@ -56,11 +56,12 @@ class GNModel:
 
 
 def prepare(w: Workload, nx: int, nz: int, seed: int = 0, alfx: float = 1.0, alfz: float = 1.0,
-            device_budget: int = 2 << 30) -> GNModel:
+            device_budget: int = 2 << 30, contrast: float = 1.2) -> GNModel:
     """Handle, wavenumbers, element matrices and the smoothness operator of a
     model on nx x nz quadrilaterals; synthetic observed data from a perturbed
-    model (+20 % conductivity in a disc) with 2 % + 1e-4 Ohm Gaussian noise,
-    and the error weights of CRTomo's error model (src/rdati.f90:254,315-320)."""
+    model (conductivity times ``contrast`` in a disc) with 2 % + 1e-4 Ohm
+    Gaussian noise, and the error weights of CRTomo's error model
+    (src/rdati.f90:254,315-320)."""
     g = _crt.CRT(w.mesh)
     g.set_device_budget(device_budget)
     g.rwaven(w.strnr, w.vnr)
@@ -71,7 +72,7 @@ def prepare(w: Workload, nx: int, nz: int, seed: int = 0, alfx: float = 1.0, alf
     x0 = rng.uniform(cx.min(), cx.max())
     z0 = rng.uniform(cz.min() * 0.5, 0.0)
     blob = (cx - x0) ** 2 + (cz - z0) ** 2 < (0.2 * (cx.max() - cx.min())) ** 2
-    sig_true = w.sigma * np.where(blob, 1.2, 1.0)
+    sig_true = w.sigma * np.where(blob, contrast, 1.0)
     g.forward(sig_true)
     v, _ = g.bvolti(w.strnr, w.vnr)
     r = np.abs(v) * (1.0 + 0.02 * rng.standard_normal(len(v))) + 1e-4 * rng.standard_normal(len(v))
@@ -96,6 +97,41 @@ def gn_step(mod: GNModel, lam: float = 10.0, ncg: int = 20, use_graph: bool = Fa
     dpar, _, cgres = g.cg_solve(mod.wgt, mod.dobs, sa, lam, 0, mod.smatm, mod.smatm[:, 0], ncg, eps=0.0,
                                 nx=mod.nx, rm=mod.rm)
     return dpar, grad, cgres
+
+
+def gn_iterate(mod: GNModel, iters: int = 3, lam: float = 10.0, ncg: int = 20, step: float = 1.0,
+               eps: float = 0.0, lphi0: bool = False, use_graph: bool = False, sigma0=None):
+    """Fixed-lambda Gauss-Newton iterations in the order of src/inv.f90: the
+    forward and -ln V of the current model, the data misfit (dmisft,
+    :524/:677), the explicit Jacobian, the CG update of update (right-hand
+    side J^H W (d_obs - d) - lam R m with m = ln sigma), then bpar, the model
+    update par + step dpar and bsigma (:924-933).  Starts from ``sigma0``
+    (default: the model's conductivities, left unchanged); there is no line
+    search (the reference halves a step that raises the misfit and fits a
+    parabola to the step length, :872-912),
+    so lam must keep the steps in the linear range.  Returns (misfits, sigma) with
+    misfits[i] = (nrmsd, betrms, pharms) of the model before iteration i and
+    the final entry that of the last model."""
+    g, w = mod.g, mod.w
+    g.set_graph(use_graph)
+    g.set_model_map()
+    sigma = np.array(w.sigma if sigma0 is None else sigma0, np.complex128)
+    hist = []
+    for it in range(iters + 1):
+        g.forward(sigma)
+        _, sa = g.bvolti(w.strnr, w.vnr)
+        st, _, _ = g.dmisft(mod.dobs, sa, mod.wgt, mod.wgt, mod.wgt)
+        hist.append(tuple(st[:3]))
+        if it == iters:
+            break
+        g.sensitivities(1, to_host=False)
+        rm = rm_rect(mod.smatm, mod.nx, np.log(sigma))
+        dpar, _, _ = g.cg_solve(mod.wgt, mod.dobs, sa, lam, 0, mod.smatm, mod.smatm[:, 0], ncg, eps=eps,
+                                nx=mod.nx, rm=rm)
+        par = g.bpar(sigma)
+        par, _, _ = g.update_step(par, dpar, step, lphi0)
+        sigma = g.bsigma(par)
+    return hist, sigma
 
 
 def run_batch(models, threads: int = 8, **kw):
