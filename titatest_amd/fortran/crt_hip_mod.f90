@@ -132,6 +132,53 @@ module crt_hip
        real(c_double), intent(out) :: cgfac(*), cgres(*)
      end function crt_cg_solve
 
+     ! inversion-loop pieces (src/dmisft.f90, src/bpar.f90, src/bsigma.f90,
+     ! src/update.f90:332-351); d0/fm0 and wdlok may be c_null_ptr
+     integer(c_int) function crt_dmisft(h, nanz, dat, sigmaa, wmatd, wmatdr, wmatdp, d0, fm0, flags, &
+          l1min, stats, wdlok) bind(C, name='crt_dmisft')
+       import :: c_int, c_ptr, c_int32_t, c_double, c_double_complex
+       type(c_ptr), value :: h
+       integer(c_int32_t), value :: nanz, flags
+       complex(c_double_complex), intent(in) :: dat(*), sigmaa(*)
+       real(c_double), intent(inout) :: wmatd(*)
+       real(c_double), intent(in) :: wmatdr(*), wmatdp(*)
+       type(c_ptr), value :: d0, fm0, wdlok
+       real(c_double), value :: l1min
+       real(c_double), intent(out) :: stats(6)
+     end function crt_dmisft
+
+     integer(c_int) function crt_set_model_map(h, manz, mnr) bind(C, name='crt_set_model_map')
+       import :: c_int, c_ptr, c_int32_t
+       type(c_ptr), value :: h
+       integer(c_int32_t), value :: manz
+       integer(c_int32_t), intent(in) :: mnr(*)
+     end function crt_set_model_map
+
+     integer(c_int) function crt_bpar(h, sigma, par) bind(C, name='crt_bpar')
+       import :: c_int, c_ptr, c_double_complex
+       type(c_ptr), value :: h
+       complex(c_double_complex), intent(in) :: sigma(*)
+       complex(c_double_complex), intent(inout) :: par(*)
+     end function crt_bpar
+
+     integer(c_int) function crt_bsigma(h, par, sigma) bind(C, name='crt_bsigma')
+       import :: c_int, c_ptr, c_double_complex
+       type(c_ptr), value :: h
+       complex(c_double_complex), intent(in) :: par(*)
+       complex(c_double_complex), intent(out) :: sigma(*)
+     end function crt_bsigma
+
+     integer(c_int) function crt_update_step(h, manz, par, dpar, step, lphi0, out) &
+          bind(C, name='crt_update_step')
+       import :: c_int, c_ptr, c_int32_t, c_double, c_double_complex
+       type(c_ptr), value :: h
+       integer(c_int32_t), value :: manz, lphi0
+       complex(c_double_complex), intent(inout) :: par(*)
+       complex(c_double_complex), intent(in) :: dpar(*)
+       real(c_double), value :: step
+       real(c_double), intent(out) :: out(2)
+     end function crt_update_step
+
      integer(c_int) function crt_error_info(h, code, k, l) bind(C, name='crt_error_info')
        import :: c_int, c_ptr, c_int32_t
        type(c_ptr), value :: h
