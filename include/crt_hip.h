@@ -238,6 +238,15 @@ int crt_update_step(crt_handle *h, int32_t manz, double *par, const double *dpar
 int crt_update_step_dev(crt_handle *h, int32_t manz, double *par_dev, const double *dpar_dev, double step,
                         int32_t lphi0, double *out_dev);
 
+/* Replaces blam0 (src/blam0.f90:69-170, called at src/inv.f90:813) after
+ * crt_sensitivities (or crt_sensitivities_dc): the starting regularisation
+ * weight lammax = mean_j |(J^H W J 1)_j| * 2 / (alfx + alfz) * 5, formed as
+ * J^H (w o (J 1)) in O(nanz manz) instead of the reference's O(nanz manz^2)
+ * loop.  w = wmatd * wdfak; mode 0 complex J, 1 real parts (the lfpi branch,
+ * and the ldc branch after crt_sensitivities_dc).  The preset-lambda branches
+ * (:47-62) stay with the caller. */
+int crt_blam0(crt_handle *h, const double *w, int32_t mode, double alfx, double alfz, double *lammax);
+
 #ifdef __cplusplus
 }
 #endif

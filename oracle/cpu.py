@@ -449,3 +449,15 @@ def update_step(par, dpar, step, lphi0=False):
     _inv_lib().orc_update_step(len(p), p, np.ascontiguousarray(dpar, np.complex128), float(step),
                                int(bool(lphi0)), out)
     return p, out[0], int(out[1])
+
+
+def blam0(sens, w, mode=0, alfx=1.0, alfz=1.0):
+    """Starting lambda of src/blam0.f90 (reference loop order, O(N M^2));
+    mode 0 complex, 1 real parts (FPI / DC)."""
+    L = lib()
+    L.orc_blam0.restype = C.c_double
+    L.orc_blam0.argtypes = [C.c_int, C.c_int, _z, _d, C.c_int, C.c_double, C.c_double]
+    S = np.asfortranarray(sens, np.complex128)
+    N, M = S.shape
+    return L.orc_blam0(N, M, np.ascontiguousarray(S.ravel(order="F")), np.ascontiguousarray(w, np.float64),
+                       int(mode), float(alfx), float(alfz))

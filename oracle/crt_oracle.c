@@ -1571,3 +1571,39 @@ void orc_update_step(int manz, orc_c *par, const orc_c *dpar, double step, int l
   out[0] = bdpar * step;
   out[1] = (double)in;
 }
+
+/* blam0 (src/blam0.f90:69-170): the starting regularisation weight from the
+ * column sums of |J^H W J| row j, in the reference's loop order (j, then i,
+ * then k: O(N M^2)).  mode 0: complex J (cdabs of the sum); mode 1: real
+ * parts of J (the FPI branch, and the DC branch on sensdc).  sens is N x M
+ * column major; w = wmatd * wdfak.  Presets of a fixed lambda (:47-62) are
+ * the caller's. */
+double orc_blam0(int nanz, int manz, const orc_c *sens, const double *w, int mode, double alfx,
+                 double alfz) {
+  double *jtj = (double *)malloc(sizeof(double) * (size_t)manz);
+#pragma omp parallel for schedule(guided)
+  for (int j = 0; j < manz; ++j) {
+    if (mode == 0) {
+      orc_c cdum = cc(0.0, 0.0);
+      for (int i = 0; i < nanz; ++i) {
+        const orc_c sij = cconj(sens[(size_t)j * nanz + i]);
+        const orc_c wi = cc(w[i], 0.0);
+        for (int k = 0; k < manz; ++k)
+          cdum = cadd(cdum, cmul(cmul(sij, sens[(size_t)k * nanz + i]), wi));
+      }
+      jtj[j] = cabs_(cdum);
+    } else {
+      double dum = 0.0;
+      for (int i = 0; i < nanz; ++i)
+        for (int k = 0; k < manz; ++k)
+          dum = dum + sens[(size_t)j * nanz + i].re * sens[(size_t)k * nanz + i].re * w[i];
+      jtj[j] = fabs(dum);
+    }
+  }
+  double s = 0.0;
+  for (int j = 0; j < manz; ++j) s += jtj[j];
+  free(jtj);
+  double lammax = s / (double)manz;
+  lammax = lammax * 2.0 / (alfx + alfz);
+  return lammax * 5.0;
+}

@@ -75,5 +75,7 @@ int orc_dmisft(int nanz, const orc_c *dat, const orc_c *sigmaa, double *wmatd,
 void orc_bpar(int elanz, const int *mnr, const orc_c *sigma, orc_c *par);
 void orc_bsigma(int elanz, const int *mnr, const orc_c *par, orc_c *sigma);
 void orc_update_step(int manz, orc_c *par, const orc_c *dpar, double step, int lphi0, double *out);
+double orc_blam0(int nanz, int manz, const orc_c *sens, const double *w, int mode, double alfx,
+                 double alfz);
 
 #endif

@@ -279,3 +279,15 @@ def bsigma(ref_mesh, par, elanz, mnr):
     if e:
         raise RefError("bsigma", e)
     return out
+
+
+def blam0(sens, w, mode=0, alfx=1.0, alfz=1.0):
+    """The reference blam0 on J (N x M): mode 0 complex, 1 lfpi, 2 ldc."""
+    L = lib()
+    L.ref_blam0.argtypes = [C.c_int, C.c_int, _z, _d, C.c_int, C.c_double, C.c_double, C.POINTER(C.c_double)]
+    S = np.asfortranarray(sens, np.complex128)
+    N, M = S.shape
+    out = C.c_double()
+    L.ref_blam0(N, M, np.ascontiguousarray(S.ravel(order="F")), np.ascontiguousarray(w, np.float64), int(mode),
+                float(alfx), float(alfz), C.byref(out))
+    return out.value

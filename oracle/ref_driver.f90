@@ -604,4 +604,36 @@ contains
     ref_bsigma = errnr
   end function ref_bsigma
 
+  !> Reference blam0 (src/blam0.f90) on the caller's J (nanz x manz, column
+  !> major): mode 0 the complex branch, 1 lfpi (real parts of sens), 2 ldc
+  !> (sensdc = real parts).  c_w = wmatd with wdfak = 1.  No preset lambda.
+  integer(c_int) function ref_blam0(c_nanz, c_manz, c_sens, c_w, c_mode, c_alfx, c_alfz, c_lammax) &
+       bind(C, name='ref_blam0')
+    use invmod, only: wmatd, wdfak, lfpi
+    use konvmod, only: lammax, lamnull_cri, lamnull_fpi, llamf, alfx, alfz, nz, lverb
+    use femmod, only: ldc
+    integer(c_int), value :: c_nanz, c_manz, c_mode
+    complex(c_double_complex), intent(in) :: c_sens(c_nanz, c_manz)
+    real(c_double), intent(in) :: c_w(c_nanz)
+    real(c_double), value :: c_alfx, c_alfz
+    real(c_double), intent(out) :: c_lammax
+    nanz = c_nanz; manz = c_manz
+    if (allocated(sens)) deallocate(sens)
+    if (allocated(sensdc)) deallocate(sensdc)
+    if (allocated(wmatd)) deallocate(wmatd)
+    if (allocated(wdfak)) deallocate(wdfak)
+    allocate(sens(nanz, manz), sensdc(nanz, manz), wmatd(nanz), wdfak(nanz))
+    sens = c_sens
+    sensdc = dble(c_sens)
+    wmatd = c_w
+    wdfak = 1
+    ldc = c_mode == 2
+    lfpi = c_mode == 1
+    llamf = 0; lamnull_cri = 0d0; lamnull_fpi = 0d0; nz = 0; lverb = .false.
+    alfx = c_alfx; alfz = c_alfz
+    call blam0()
+    c_lammax = lammax
+    ref_blam0 = 0
+  end function ref_blam0
+
 end module crt_ref_driver

@@ -202,7 +202,25 @@ def inversion_steps():
     print(f"wrote ref_inversion_steps.npz  chkpo2 cases={len(cin)} N={N}")
 
 
-CASES = {"inversion": inversion_steps}
+def blam0_case():
+    """blam0 (complex, lfpi and ldc branches) on the small block mesh's bsensi
+    Jacobian with the error weights of its voltages, alfx = 1, alfz = 0.5."""
+    from oracle import ref
+    w = meshgen.small_block()
+    r = Reference(w.mesh)
+    r.load_data(w.strnr, w.vnr)
+    r.rwaven()
+    r.precal()
+    r.forward(w.sigma, nthreads=8)
+    v, _ = r.volt(1)
+    J = r.sens(1)
+    wt = 1.0 / (0.02 + 1e-4 / np.abs(v)) ** 2
+    lam = np.array([ref.blam0(J, wt, m, 1.0, 0.5) for m in (0, 1, 2)])
+    np.savez_compressed(OUT / "ref_blam0.npz", w=wt, alfx=1.0, alfz=0.5, lammax=lam)
+    print(f"wrote ref_blam0.npz  lammax={lam}")
+
+
+CASES = {"inversion": inversion_steps, "blam0": blam0_case}
 
 
 def main():
@@ -224,6 +242,7 @@ def main():
     fixture("decoupling", 0, lsink=True, nsink=373)
     fixture("rect_complex", 1)
     inversion_steps()
+    blam0_case()
 
 
 if __name__ == "__main__":

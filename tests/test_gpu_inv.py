@@ -1,6 +1,7 @@
 """The inversion-loop pieces on the device (crt_dmisft, crt_bpar, crt_bsigma,
-crt_update_step and their device-pointer variants) against the oracle and the
-reference's golden vectors (tests/golden/ref_inversion_steps.npz).
+crt_update_step and their device-pointer variants, crt_blam0) against the
+oracle and the reference's golden vectors (tests/golden/ref_inversion_steps.npz,
+ref_blam0.npz).
 
 Tolerances: dmisft's sums run in a fixed tree order instead of the
 reference's sequential one, and the device's sqrt/hypot/log/exp may differ
@@ -126,3 +127,23 @@ def test_device_variants_equal_host_variants(handle, crtmod):
     np.testing.assert_array_equal(par.cpu().numpy(), p_h)
     assert out[0].item() == bd_h and int(out[1].item()) == n_h
     np.testing.assert_array_equal(sg2.cpu().numpy(), g.bsigma(p_h))
+
+
+def test_blam0_matches_reference(crtmod):
+    """blam0 from the device Jacobian, J^H (w o (J 1)) in O(N M), against the
+    reference's O(N M^2) loop (golden vectors).  Held to the sensitivity
+    tolerance: the device J itself agrees with the reference's to 1e-8."""
+    w = meshgen.small_block()
+    d = golden("blam0")
+    say("blam0 small_block")
+    g = crtmod.CRT(w.mesh)
+    g.rwaven(w.strnr, w.vnr)
+    g.precal()
+    g.forward(w.sigma)
+    g.bvolti(w.strnr, w.vnr)
+    g.sensitivities(1, to_host=False)
+    for mode, ref_mode in ((0, 0), (1, 1)):
+        lam = g.blam0(d["w"], mode, float(d["alfx"]), float(d["alfz"]))
+        assert abs(lam - d["lammax"][ref_mode]) <= 1e-8 * d["lammax"][ref_mode], (mode, lam)
+    with pytest.raises(crtmod.CRTError):
+        g.blam0(d["w"], 2)

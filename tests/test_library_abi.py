@@ -53,6 +53,7 @@ def test_null_arguments_are_rejected_without_a_device():
     assert lib.crt_bpar(None, None, None) == 1000
     assert lib.crt_bsigma_dev(None, None, None) == 1000
     assert lib.crt_update_step(None, 4, None, None, 1.0, 0, None) == 1000
+    assert lib.crt_blam0(None, None, 0, 1.0, 1.0, None) == 1000
 
 
 def test_fortran_host_built_and_linked():

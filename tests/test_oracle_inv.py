@@ -147,3 +147,33 @@ def test_update_step_exact_cases():
     expect[neg] = expect[neg].real
     np.testing.assert_array_equal(p, expect)
     assert n == int(neg.sum()) == 2
+
+
+def _small_block_j():
+    from titatest_amd import meshgen
+    w = meshgen.small_block()
+    o = cpu.Oracle(w.mesh)
+    o.rwaven(w.strnr, w.vnr)
+    o.precal()
+    o.forward(w.sigma)
+    o.volt(w.strnr, w.vnr, 1)
+    return o.sens(1)
+
+
+def test_blam0_matches_reference_bitwise():
+    d = golden("blam0")
+    J = _small_block_j()
+    for mode, ref_mode in ((0, 0), (1, 1), (1, 2)):  # complex, lfpi, ldc (real parts)
+        assert cpu.blam0(J, d["w"], mode, float(d["alfx"]), float(d["alfz"])) == d["lammax"][ref_mode]
+
+
+def test_blam0_is_the_mean_of_jhwj1():
+    # the O(N M^2) loop of the reference equals |J^H W (J 1)| averaged over
+    # the columns, times 2/(alfx+alfz)*5
+    d = golden("blam0")
+    J = _small_block_j()
+    w = d["w"]
+    for mode in (0, 1):
+        jj = J.real if mode else J
+        want = np.mean(np.abs((jj.conj().T * w) @ jj.sum(axis=1))) * 2.0 / 1.5 * 5.0
+        assert cpu.blam0(J, w, mode, 1.0, 0.5) == pytest.approx(want, rel=1e-13)

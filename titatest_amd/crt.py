@@ -39,7 +39,7 @@ EXPORTS = (
     "crt_smoothness", "crt_set_graph", "crt_set_device_budget",
     "crt_error_info", "crt_set_timing", "crt_stage_times", "crt_factor_phase_ticks",
     "crt_kpot_device", "crt_dmisft", "crt_dmisft_dev", "crt_set_model_map", "crt_bpar", "crt_bpar_dev",
-    "crt_bsigma", "crt_bsigma_dev", "crt_update_step", "crt_update_step_dev",
+    "crt_bsigma", "crt_bsigma_dev", "crt_update_step", "crt_update_step_dev", "crt_blam0",
 )
 
 # dmisft switches (the reference's lpol, lfpi, ldiff, lrobust)
@@ -121,6 +121,7 @@ def load_library(path: Path = LIB_PATH):
     L.crt_bsigma_dev.argtypes = [vp, vp, vp]
     L.crt_update_step.argtypes = [vp, i32, vp, vp, f64, i32, vp]
     L.crt_update_step_dev.argtypes = [vp, i32, vp, vp, f64, i32, vp]
+    L.crt_blam0.argtypes = [vp, vp, i32, f64, f64, C.POINTER(f64)]
     _lib = L
     return L
 
@@ -365,6 +366,13 @@ class CRT:
         self._check("update_step", self.lib.crt_update_step(self.h, len(p), _ptr(p), _ptr(_c128(dpar)),
                                                             float(step), int(bool(lphi0)), _ptr(out)))
         return p, out[0], int(out[1])
+
+    def blam0(self, w, mode=0, alfx=1.0, alfz=1.0):
+        """Starting lambda of src/blam0.f90 from the explicit J on the device."""
+        out = C.c_double()
+        self._check("blam0", self.lib.crt_blam0(self.h, _ptr(np.ascontiguousarray(w, np.float64)), int(mode),
+                                                float(alfx), float(alfz), C.byref(out)))
+        return out.value
 
     def set_graph(self, on=True):
         """Run crt_cg_solve's iterations as one captured HIP graph."""

@@ -1744,3 +1744,38 @@ int crt_update_step(crt_handle* h, int32_t manz, double* par, const double* dpar
   HIPCHK(hipMemcpy(out, dout, 16, hipMemcpyDeviceToHost));
   return 0;
 }
+
+// blam0 (src/blam0.f90:69-170) in O(N M): jtj_j = |sum_i conj(J_ij) w_i
+// sum_k J_ik| = |(J^H W (J 1))_j| (mode 1: real parts of J, the FPI and DC
+// branches), lammax = mean_j jtj_j * 2 / (alfx + alfz) * 5.  The reference's
+// triple loop forms the same sums in O(N M^2) and another order.
+int crt_blam0(crt_handle* h, const double* w, int32_t mode, double alfx, double alfz, double* lammax) {
+  if (!h || !w || !lammax || (mode != 0 && mode != 1) || !(alfx + alfz != 0.0)) return kErrArg;
+  if (!h->sens_done) return kErrOrder;
+  const int N = h->nanz, M = h->elanz;
+  double* dw;
+  if (upload(h, "bl_w", w, N, &dw)) return kErrHip;
+  double2* part = h->get<double2>("bl_part", (size_t)kRowSlices * N);
+  double2* s = h->get<double2>("bl_s", N);
+  double2* t = h->get<double2>("bl_t", M);
+  NEED(part);
+  NEED(s);
+  NEED(t);
+  const double2* J = h->at<double2>("sens");
+  tstart(h, "blam0");
+  hipLaunchKernelGGL(crt_rowsum_partial_kernel, dim3((N + 255) / 256, kRowSlices), dim3(256), 0, h->stream, J, N, M,
+                     (int)mode, part);
+  hipLaunchKernelGGL(crt_rowsum_final_kernel, dim3((N + 255) / 256), dim3(256), 0, h->stream, (const double2*)part,
+                     N, s);
+  hipLaunchKernelGGL(crt_jhwr_kernel, dim3((unsigned)(((size_t)M * 64 + 255) / 256)), dim3(256), 0, h->stream, J, N,
+                     M, (const double*)dw, (const double2*)s, t);
+  tstop(h);
+  int rc = finish(h);
+  if (rc) return rc;
+  std::vector<double2> th(M);
+  HIPCHK(hipMemcpy(th.data(), t, (size_t)M * 16, hipMemcpyDeviceToHost));
+  double sum = 0.0;
+  for (int j = 0; j < M; ++j) sum += mode ? std::fabs(th[j].x) : std::hypot(th[j].x, th[j].y);
+  *lammax = sum / (double)M * 2.0 / (alfx + alfz) * 5.0;
+  return 0;
+}

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 constexpr int kInvThreads = 1024;  // one workgroup: fixed-order reductions
+constexpr int kRowSlices = 64;     // column slices of the row-sum first stage
 
 // dmisft switches (the reference's lpol, lfpi, ldiff, lrobust)
 constexpr int kMisfitLpol = 1, kMisfitLfpi = 2, kMisfitLdiff = 4, kMisfitLrobust = 8;
@@ -32,3 +33,6 @@ __global__ void crt_bsigma_kernel(int elanz, const int* mnr, const double2* par,
 // par += step dpar, lphi0 rule, out = {bdpar, forced count}
 __global__ void crt_update_step_kernel(int manz, double2* par, const double2* dpar, double step, int lphi0,
                                        double* out);
+// blam0's row sums s_i = sum_j J_ij (mode 1: real parts), two stages
+__global__ void crt_rowsum_partial_kernel(const double2* J, int nanz, int manz, int mode, double2* part);
+__global__ void crt_rowsum_final_kernel(const double2* part, int nanz, double2* s);

@@ -1,8 +1,9 @@
 // Inversion-loop pieces around the forward/sensitivity path, on the device so
 // that a Gauss-Newton iteration needs no host round trip (and can be captured
 // in a graph): the data misfit with polarity folding (src/dmisft.f90:17-300),
-// bpar (src/bpar.f90), bsigma (src/bsigma.f90) and the model update step
-// (src/update.f90:332-351).  Reductions run on one workgroup: each thread sums
+// bpar (src/bpar.f90), bsigma (src/bsigma.f90), the model update step
+// (src/update.f90:332-351) and the row sums for blam0's starting lambda
+// (src/blam0.f90).  Reductions run on one workgroup: each thread sums
 // its strided share in index order, then a fixed tree over the workgroup, so
 // results are deterministic (their summation order is not the reference's
 // sequential one; the difference is rounding).
@@ -174,4 +175,34 @@ __global__ __launch_bounds__(kInvThreads) void crt_update_step_kernel(int manz, 
     out[0] = bd * step;
     out[1] = nf;
   }
+}
+
+// Row sums of J for blam0's J^H W (J 1): s_i = sum_j J_ij (mode 1: real parts
+// only).  J is N x M column major.  Stage 1: slice q of kRowSlices sums the
+// columns j = q, q + kRowSlices, ... for rows i (coalesced over i); stage 2
+// adds the slices in order (deterministic).
+__global__ void crt_rowsum_partial_kernel(const double2* __restrict__ J, int nanz, int manz, int mode,
+                                          double2* __restrict__ part) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int q = blockIdx.y;
+  if (i >= nanz) return;
+  double sr = 0.0, si = 0.0;
+  for (int j = q; j < manz; j += kRowSlices) {
+    const cplx v = ldc(J + (size_t)j * nanz + i);
+    sr += v.re;
+    si += v.im;
+  }
+  stc(part + (size_t)q * nanz + i, mkc(sr, mode ? 0.0 : si));
+}
+
+__global__ void crt_rowsum_final_kernel(const double2* __restrict__ part, int nanz, double2* __restrict__ s) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nanz) return;
+  double sr = 0.0, si = 0.0;
+  for (int q = 0; q < kRowSlices; ++q) {
+    const cplx v = ldc(part + (size_t)q * nanz + i);
+    sr += v.re;
+    si += v.im;
+  }
+  stc(s + i, mkc(sr, si));
 }

@@ -179,6 +179,15 @@ module crt_hip
        real(c_double), intent(out) :: out(2)
      end function crt_update_step
 
+     integer(c_int) function crt_blam0(h, w, mode, alfx, alfz, lammax) bind(C, name='crt_blam0')
+       import :: c_int, c_ptr, c_int32_t, c_double
+       type(c_ptr), value :: h
+       real(c_double), intent(in) :: w(*)
+       integer(c_int32_t), value :: mode
+       real(c_double), value :: alfx, alfz
+       real(c_double), intent(out) :: lammax
+     end function crt_blam0
+
      integer(c_int) function crt_error_info(h, code, k, l) bind(C, name='crt_error_info')
        import :: c_int, c_ptr, c_int32_t
        type(c_ptr), value :: h
