@@ -128,6 +128,9 @@ def main():
                     help="instead of the forward: one Gauss-Newton step on each of COUNT uneven models "
                          "(adjoint gradient + 20 CG iterations), graph-captured CG against the per-launch path")
     ap.add_argument("--threads", type=int, default=8, help="models in flight for --uneven-batch")
+    ap.add_argument("--tri", action="store_true",
+                    help="the terrain-following triangular mesh instead: 257x100 nodes (25,700), 50,688 "
+                         "triangles, mb 101, 128 electrodes, 7,074 quadrupoles, mixed sides and bottom")
     ap.add_argument("--dirichlet", action="store_true",
                     help="the same 240x80 model with Dirichlet sides and bottom: one shared factor per "
                          "wavenumber, then a forward and a backward sweep per source (kompb)")
@@ -159,7 +162,10 @@ def main():
         return
     dev = torch.device("cuda:0")
 
-    w = meshgen.large_grf(side_type=13 if args.dirichlet else 11)
+    if args.tri:
+        w = meshgen.topo_triangles()
+    else:
+        w = meshgen.large_grf(side_type=13 if args.dirichlet else 11)
     m = w.mesh
     g = crt.CRT(m)
     g.rwaven(w.strnr, w.vnr)
@@ -167,7 +173,8 @@ def main():
     n, mb, E, K, N = m.sanz, m.mb, m.eanz, g.K, w.nanz
     nsys = K * E
     nfac = K if args.dirichlet else nsys   # factorisations per forward
-    log(f"workload large_grf: n={n} mb={mb} E={E} K={K} N={N} systems={nsys} factors={nfac}")
+    log(f"workload {'topo_triangles' if args.tri else 'large_grf'}: n={n} mb={mb} E={E} K={K} N={N} "
+        f"systems={nsys} factors={nfac}")
     sigma_dev = torch.tensor(w.sigma, dtype=torch.complex128, device=dev)
     st = np.ascontiguousarray(w.strnr, np.int32)
     vn = np.ascontiguousarray(w.vnr, np.int32)
@@ -201,18 +208,21 @@ def main():
     value = N / (ms_per_step / 1e3)
 
     # the factor kernel: 4 n mb^2 flops per band factorisation plus 8 n mb
-    # for the forward sweep fused into it; L written once
+    # for the forward sweep fused into it; L written once.  A forward may take
+    # several launches (factors in flight are capped by the device budget):
+    # per launch = per step / launches per step.
     fac_ms, fac_n = times.get("factor_solve", (float("nan"), 1))
     fac_avg_s = fac_ms / fac_n / 1e3
-    flops = nfac * (4.0 * n * mb * mb + 8.0 * n * mb)
+    launches = fac_n / args.steps
+    flops = nfac * (4.0 * n * mb * mb + 8.0 * n * mb) / launches
     achieved = flops / fac_avg_s / 1e12
-    lbytes = nfac * 16.0 * n * (mb + 1)
+    lbytes = nfac * 16.0 * n * (mb + 1) / launches
 
     # HBM bytes per factor launch from the committed rocprofv3 FETCH_SIZE /
     # WRITE_SIZE passes of this workload (scaled by the copy-kernel calibration)
     traffic = None
     tf = ROOT / "profiles" / "factor_traffic_large_grf.json"
-    if tf.exists() and not args.dirichlet:
+    if tf.exists() and not (args.dirichlet or args.tri):
         try:
             traffic = json.loads(tf.read_text())["factor"]["hbm_bytes"]
         except (ValueError, KeyError):
@@ -231,17 +241,21 @@ def main():
         "vs_baseline": None,
         "dtype": "c128",
         "data": "synthetic (seeded Gaussian random-field model, generated mesh)",
-        "config": {"workload": "240x80 quads, 19521 nodes, mb 82, 96 electrodes, "
-                               "3090 dipole-dipole quadrupoles, 14 wavenumbers x 96 sources"
-                               + (", Dirichlet sides and bottom (one factor per wavenumber)"
-                                  if args.dirichlet else ", mixed sides and bottom"),
+        "config": {"workload": (f"terrain-following triangles, {m.elanz} elements, {n} nodes, mb {mb}, "
+                                f"{E} electrodes, {N} dipole-dipole quadrupoles, {K} wavenumbers x {E} sources, "
+                                "mixed sides and bottom" if args.tri else
+                                "240x80 quads, 19521 nodes, mb 82, 96 electrodes, "
+                                "3090 dipole-dipole quadrupoles, 14 wavenumbers x 96 sources"
+                                + (", Dirichlet sides and bottom (one factor per wavenumber)"
+                                   if args.dirichlet else ", mixed sides and bottom")),
                    "systems": nsys, "factorisations": nfac, "nodes": n, "bandwidth": mb, "electrodes": E,
                    "quadrupoles": N, "wavenumbers": K},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP64_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_TFLOPS, "traffic": traffic,
                      "kernel": "crt_factor_kernel", "avg_launch_ms": fac_ms / fac_n,
+                     "launches_per_step": launches,
                      "algorithmic_flops": flops, "algorithmic_bytes": lbytes},
-        "stage_ms_per_step": {k: v[0] / v[1] for k, v in times.items()},
+        "stage_ms_per_step": {k: v[0] / args.steps for k, v in times.items()},
     }
 
     if not args.skip_adjoint:
@@ -278,7 +292,7 @@ def main():
         }
         log(f"adjoint {adj_ms:.3f} ms, explicit J {exp_ms:.3f} ms")
 
-    if not args.no_cpu_baseline and not args.dirichlet:
+    if not args.no_cpu_baseline and not (args.dirichlet or args.tri):
         log("cpu baseline sample")
         result["cpu_baseline"] = cpu_baseline(w, nsys, K, args.cpu_sample)
     print(json.dumps(result), flush=True)

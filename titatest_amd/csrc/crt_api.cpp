@@ -757,21 +757,30 @@ static int run_forward(crt_handle* h) {
   const size_t lds = crt_factor_lds_bytes(mb);
   // the substitutions address one factor with 32-bit byte offsets
   if (lds > h->lds_max || mb > 124 || (size_t)n * (mb + 1) * 16 >= (size_t(1) << 31)) return kErrBand;
-  // wave 0 holds the W = mb + kNB panel rows, ROWS per lane; waves 1-3 own the
-  // trailing window's 4x4 tiles beyond the look-ahead columns, MAXT each
+  // wave 0 holds the W = mb + kNB panel rows, ROWS per lane; the other waves
+  // own the trailing window's 4x4 tiles beyond the look-ahead columns, MAXT
+  // each.  When one workgroup's LDS leaves no room for a second on the CU, the
+  // workgroup takes 512 threads instead of 256.
   const int W = mb + kNB, tw = (mb + 3) / 4;
   const int nrest = tw > 2 ? ((tw - 2) * (tw - 1)) / 2 : 0;
-  const int rows = (W + 63) / 64, maxt = std::max(1, (nrest + 191) / 192);
+  const int nt = 2 * lds > h->lds_max ? 512 : 256;
+  const int rows = (W + 63) / 64, maxt = std::max(1, (nrest + nt - 65) / (nt - 64));
   void (*kern)(FactorArgs) = nullptr;
-  if (rows == 1 && maxt == 1) kern = crt_factor_kernel<1, 1>;
-  else if (rows == 2 && maxt == 1) kern = crt_factor_kernel<2, 1>;
-  else if (rows == 2 && maxt == 2) kern = crt_factor_kernel<2, 2>;
-  else if (rows == 2 && maxt == 3) kern = crt_factor_kernel<2, 3>;
-  else if (rows == 3 && maxt <= 3) kern = crt_factor_kernel<3, 3>;
+  if (nt == 256) {
+    if (rows == 1 && maxt == 1) kern = crt_factor_kernel<1, 1, 256>;
+    else if (rows == 2 && maxt == 1) kern = crt_factor_kernel<2, 1, 256>;
+    else if (rows == 2 && maxt == 2) kern = crt_factor_kernel<2, 2, 256>;
+    else if (rows == 2 && maxt == 3) kern = crt_factor_kernel<2, 3, 256>;
+    else if (rows == 3 && maxt <= 3) kern = crt_factor_kernel<3, 3, 256>;
+  } else {
+    if (rows <= 2 && maxt == 1) kern = crt_factor_kernel<2, 1, 512>;
+    else if (rows <= 2 && maxt == 2) kern = crt_factor_kernel<2, 2, 512>;
+    else if (rows == 3 && maxt <= 2) kern = crt_factor_kernel<3, 2, 512>;
+  }
   if (!kern) return kErrBand;
   hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   int per_cu = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, 256, lds) != hipSuccess || per_cu < 1) per_cu = 1;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, nt, lds) != hipSuccess || per_cu < 1) per_cu = 1;
   const int nsys = K * E;
   FactorArgs A;
   A.n = n;
@@ -907,7 +916,7 @@ static int run_forward(crt_handle* h) {
     tstop(h);
     h->slots = std::min(cnt, h->ncu * per_cu);
     tstart(h, "factor_solve");
-    hipLaunchKernelGGL(kern, dim3(h->slots), dim3(256), lds, h->stream, A);
+    hipLaunchKernelGGL(kern, dim3(h->slots), dim3(nt), lds, h->stream, A);
     tstop(h);
     const int nrhs = cnt * rpf;
     if (reuse) {

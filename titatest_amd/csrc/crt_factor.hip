@@ -348,9 +348,13 @@ __global__ void crt_rhs_kernel(FactorArgs A) {
 
 // ---------------------------------------------------------------------------
 // Lookahead band factorisation.  ROWS: panel rows per lane of wave 0
-// (W = mb + kNB <= 64 ROWS); MAXT: trailing tiles per thread of waves 1-3.
-template <int ROWS, int MAXT>
-__global__ __launch_bounds__(256, 2) void crt_factor_kernel(FactorArgs A) {
+// (W = mb + kNB <= 64 ROWS); MAXT: trailing tiles per thread of the other
+// waves; NT threads per workgroup: 256 (two workgroups per CU) or, when the
+// LDS of one workgroup leaves no room for a second (mb above ~98), 512, so
+// that the CU still holds 8 wavefronts (7 for the look-ahead and trailing
+// update).
+template <int ROWS, int MAXT, int NT>
+__global__ __launch_bounds__(NT, 512 / NT) void crt_factor_kernel(FactorArgs A) {
   extern __shared__ double2 smem[];
   const int n = A.n, mb = A.mb, R = A.R;
   const int W = mb + kNB, M1 = mb + 1, half = (mb + 1) / 2;
@@ -380,7 +384,7 @@ __global__ __launch_bounds__(256, 2) void crt_factor_kernel(FactorArgs A) {
   for (int q = 0; q < MAXT; ++q) {
     rs_r0[q] = -1;
     rs_c0[q] = 0;
-    const int x = x3 + 192 * q;
+    const int x = x3 + (NT - 64) * q;
     if (wave > 0 && x < nrest) {
       int ti, tj;
       tri_tile(x, ti, tj);
@@ -402,18 +406,18 @@ __global__ __launch_bounds__(256, 2) void crt_factor_kernel(FactorArgs A) {
 
     // ---- prologue: ring zeroed, rows [0, mb - kNB) in place, batches
     // [mb - kNB, mb) and [mb, mb + kNB) staged, wave 0's right-hand side
-    for (int i = tid; i < half * M1; i += 256) Tt[i] = mkc(0.0, 0.0);
+    for (int i = tid; i < half * M1; i += NT) Tt[i] = mkc(0.0, 0.0);
     lds_barrier();
     {
       Ring ring0{mb, M1, half, 0};
-      for (int r = tid; r < min(mb - kNB, n); r += 256)
+      for (int r = tid; r < min(mb - kNB, n); r += NT)
         for (int q = 0; q < R; ++q) {
           cplx v;
           int t;
           row_slot(A, Sg, r, q, v, t);
           if (t >= 0) Tt[ring0.pos(r, r - mb + t)] = v;
         }
-      for (int x = tid; x < 2 * kNB * kMaxR; x += 256) {
+      for (int x = tid; x < 2 * kNB * kMaxR; x += NT) {
         const int b = x / (kNB * kMaxR), i = (x / kMaxR) % kNB, q = x % kMaxR;
         const int r = (b == 0 ? mb - kNB : mb) + i;
         const int slot = (b == 0 ? kStg - 1 : 0) * kNB * kMaxR + i * kMaxR + q;
@@ -475,7 +479,7 @@ __global__ __launch_bounds__(256, 2) void crt_factor_kernel(FactorArgs A) {
       if (wave > 0) {
         // the look-ahead precedes the panel on the critical path as well
         __builtin_amdgcn_s_setprio(2);
-        for (int x = x3; x < nla; x += 192) {
+        for (int x = x3; x < nla; x += NT - 64) {
           int j = 0, i = x;
 #pragma unroll
           for (int jj = 0; jj < kNB / 2 - 1; ++jj)
@@ -655,11 +659,14 @@ __global__ __launch_bounds__(256, 2) void crt_factor_kernel(FactorArgs A) {
   clk.flush();
 }
 
-template __global__ void crt_factor_kernel<1, 1>(FactorArgs);
-template __global__ void crt_factor_kernel<2, 1>(FactorArgs);
-template __global__ void crt_factor_kernel<2, 2>(FactorArgs);
-template __global__ void crt_factor_kernel<2, 3>(FactorArgs);
-template __global__ void crt_factor_kernel<3, 3>(FactorArgs);
+template __global__ void crt_factor_kernel<1, 1, 256>(FactorArgs);
+template __global__ void crt_factor_kernel<2, 1, 256>(FactorArgs);
+template __global__ void crt_factor_kernel<2, 2, 256>(FactorArgs);
+template __global__ void crt_factor_kernel<2, 3, 256>(FactorArgs);
+template __global__ void crt_factor_kernel<3, 3, 256>(FactorArgs);
+template __global__ void crt_factor_kernel<2, 1, 512>(FactorArgs);
+template __global__ void crt_factor_kernel<2, 2, 512>(FactorArgs);
+template __global__ void crt_factor_kernel<3, 2, 512>(FactorArgs);
 
 // ---------------------------------------------------------------------------
 // Back substitution, one wavefront per right-hand side.  Lane q (q < 64) keeps

@@ -285,12 +285,14 @@ def uneven_batch(count=64, seed=1006):
     return out
 
 
-def topo_triangles(nx_nodes=250, nz_nodes=100, eanz=128, length=160.0, depth=60.0,
+def topo_triangles(nx_nodes=257, nz_nodes=100, eanz=128, length=160.0, depth=60.0,
                    seed=1011) -> Workload:
     """Terrain-following triangular mesh (type 3), z_top = 2 sin(2 pi x / 40).
 
     Each structured cell is split into two counter-clockwise triangles; node
-    numbering is column-wise, so mb = nz_nodes + 1."""
+    numbering is column-wise, so mb = nz_nodes + 1.  The default 257 x 100
+    nodes (0.625 m columns over 160 m) put the 128 electrodes on every second
+    surface node, 1.25 m apart."""
     xn = np.linspace(0.0, length, nx_nodes)
     s = np.linspace(0.0, 1.0, nz_nodes) ** 1.5
     ncol = nz_nodes
@@ -333,6 +335,8 @@ def topo_triangles(nx_nodes=250, nz_nodes=100, eanz=128, length=160.0, depth=60.
     spacing = xn[1] - xn[0]
     step = max(1, int(round(1.25 / spacing)))
     first = (nx_nodes - step * (eanz - 1)) // 2
+    if first < 0:
+        raise ValueError(f"{eanz} electrodes every {step} columns do not fit on {nx_nodes} node columns")
     cols = first + step * np.arange(eanz)
     mesh.enr = (cols * ncol + 1).astype(np.int32)
     mag, pha = grf_model(mesh, seed, seed + 1)
