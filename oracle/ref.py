@@ -291,3 +291,23 @@ def blam0(sens, w, mode=0, alfx=1.0, alfz=1.0):
     L.ref_blam0(N, M, np.ascontiguousarray(S.ravel(order="F")), np.ascontiguousarray(w, np.float64), int(mode),
                 float(alfx), float(alfz), C.byref(out))
     return out.value
+
+
+def rdati(path, eanz, ldc=False, model=(5.0, 1e-3, 0.0, 0.0, 0.0, 0.0), nmax=100000):
+    """The reference rdati on a data file: (errnr, dict of arrays)."""
+    L = lib()
+    L.ref_rdati.argtypes = [C.c_char_p, C.c_int, C.c_int, C.c_int, _d, C.c_int, _i, _i, _i, _z, _d, _d, _d, _i]
+    p = str(path).encode()
+    info = np.zeros(2, np.int32)
+    st = np.zeros(nmax, np.int32)
+    vn = np.zeros(nmax, np.int32)
+    wd = np.zeros(nmax, np.int32)
+    dat = np.zeros(nmax, np.complex128)
+    wr = np.zeros(nmax)
+    wp = np.zeros(nmax)
+    wc = np.zeros(nmax)
+    e = L.ref_rdati(p, len(p), int(eanz), int(bool(ldc)), np.asarray(model, np.float64), nmax, info, st, vn, dat,
+                    wr, wp, wc, wd)
+    n = int(info[0])
+    return e, dict(strnr=st[:n], vnr=vn[:n], dat=dat[:n], wmatdr=wr[:n], wmatdp=wp[:n], wmatd_cri=wc[:n],
+                   wdfak=wd[:n], lindiv=bool(info[1]))

@@ -636,4 +636,59 @@ contains
     ref_blam0 = 0
   end function ref_blam0
 
+  !> Reference rdati (src/rdati.f90:84-363) on the file c_path (no synthetic
+  !> noise): the data vector and weights as rdati leaves them.  c_model: the
+  !> error model stabw0, stabm0, stabpA1, stabpB, stabpA2, stabp0.  Outputs are
+  !> sized nanz_max; c_info returns nanz, lindiv.
+  integer(c_int) function ref_rdati(c_path, c_plen, c_eanz, c_ldc, c_model, c_nmax, c_info, c_strnr, &
+       c_vnr, c_dat, c_wmatdr, c_wmatdp, c_wmatdcri, c_wdfak) bind(C, name='ref_rdati')
+    use invmod, only: dat, wmatd, wdfak, wmatd2
+    use konvmod, only: lverb
+    use femmod, only: ldc
+    integer(c_int), value :: c_plen, c_eanz, c_ldc, c_nmax
+    character(kind=c_char), intent(in) :: c_path(c_plen)
+    real(c_double), intent(in) :: c_model(6)
+    integer(c_int), intent(out) :: c_info(2), c_strnr(c_nmax), c_vnr(c_nmax), c_wdfak(c_nmax)
+    complex(c_double_complex), intent(out) :: c_dat(c_nmax)
+    real(c_double), intent(out) :: c_wmatdr(c_nmax), c_wmatdp(c_nmax), c_wmatdcri(c_nmax)
+    character(len=256) :: path
+    integer :: i, kanal
+    path = ' '
+    do i = 1, c_plen
+       path(i:i) = c_path(i)
+    end do
+    eanz = c_eanz
+    ldc = c_ldc /= 0
+    lnse = .false.; lverb = .false.
+    stabw0 = c_model(1); stabm0 = c_model(2); stabpA1 = c_model(3); stabpB = c_model(4)
+    stabpA2 = c_model(5); stabp0 = c_model(6)
+    if (allocated(strnr)) deallocate(strnr)
+    if (allocated(strom)) deallocate(strom)
+    if (allocated(volt)) deallocate(volt)
+    if (allocated(sigmaa)) deallocate(sigmaa)
+    if (allocated(kfak)) deallocate(kfak)
+    if (allocated(wmatdr)) deallocate(wmatdr)
+    if (allocated(wmatdp)) deallocate(wmatdp)
+    if (allocated(vnr)) deallocate(vnr)
+    if (allocated(dat)) deallocate(dat)
+    if (allocated(wmatd)) deallocate(wmatd)
+    if (allocated(wmatd2)) deallocate(wmatd2)
+    if (allocated(sgmaa2)) deallocate(sgmaa2)
+    if (allocated(wdfak)) deallocate(wdfak)
+    if (allocated(wmatd_cri)) deallocate(wmatd_cri)
+    kanal = 77
+    nanz = 0
+    ! rdati sets lindiv only when the file's first record carries it: start
+    ! from a fresh program's value
+    lindiv = .false.
+    call rdati(kanal, path)
+    ref_rdati = errnr
+    c_info(1) = nanz
+    c_info(2) = merge(1, 0, lindiv)
+    if (errnr /= 0 .or. nanz > c_nmax) return
+    c_strnr(1:nanz) = strnr; c_vnr(1:nanz) = vnr; c_dat(1:nanz) = dat
+    c_wmatdr(1:nanz) = wmatdr; c_wmatdp(1:nanz) = wmatdp; c_wmatdcri(1:nanz) = wmatd_cri
+    c_wdfak(1:nanz) = wdfak
+  end function ref_rdati
+
 end module crt_ref_driver

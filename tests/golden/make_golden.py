@@ -220,7 +220,47 @@ def blam0_case():
     print(f"wrote ref_blam0.npz  lammax={lam}")
 
 
-CASES = {"inversion": inversion_steps, "blam0": blam0_case}
+# data files for rdati: (name, eanz, ldc, error model, text)
+RDATI_MODEL = (3.0, 1e-3, 0.5, -0.8, 2.0, 0.1)
+RDATI_FILES = [
+    ("crt_model", 8, False, "4\n10002 40003 1.2345 -12.5\n20003 50004 0.5 -3.25\n"
+                            "30004 60005 2.0e-2 -45.0\n10003 50006 7.5 1.5\n"),
+    ("abmn_model", 8, False, "3\n1 2 4 3 0.8 -7.0\n2 3 5 4 1.25d0 -20.5 99 99\n3 4 8 7 3.5E-1 2.0\n"),
+    ("crt_indiv", 8, False, "3 T\n10002 40003 1.2345 -12.5 0.05 0.4\n20003 50004 0.5 -3.25 0.01 0.2\n"
+                            "30004 60005 2.0e-2 -45.0 0.003 1.5\n1.5 2.0\n"),
+    ("dc_indiv", 6, True, "3 .TRUE.\n10002 40003 1.2345 0.05\n20003 50004 0.5 0.01\n"
+                          "30004 60005 2.0e-2 0.003\n0.75\n"),
+    ("dc_model", 6, True, "3 F\n10002 40003 1.2345 -12.5\n20003 50004 0.5\n30004 60005 2.0e-2 0 0\n"),
+    ("spanning", 8, False, "2\n10002 40003\n 1.5 -2.0\n20003\n50004 0.25\n -1.0 extra\n"),
+    ("err_dup", 8, False, "1 T\n10002 20003 1.0 -1.0 0.1 0.1\n1.0 1.0\n"),
+    ("err_range", 4, False, "1\n10002 40005 1.0 -1.0\n"),
+    ("err_phase_error", 8, False, "1 T\n10002 40003 1.0 -1.0 0.1 0.0\n1.0 1.0\n"),
+    ("err_resistance", 8, False, "2\n10002 40003 1.0 -1.0\n20003 50004 -0.5 -1.0\n"),
+    ("err_phase", 8, False, "1\n10002 40003 1.0 -3200.0\n"),
+    ("err_eof", 8, False, "3\n10002 40003 1.0 -1.0\n20003 50004 0.5 -1.0\n"),
+]
+
+
+def rdati_cases():
+    """rdati on the data files above (written to tests/golden/rdati/), with
+    the error model RDATI_MODEL; error numbers or the arrays rdati fills."""
+    from oracle import ref
+    d = OUT / "rdati"
+    d.mkdir(exist_ok=True)
+    out = {}
+    for name, eanz, ldc, text in RDATI_FILES:
+        f = d / f"{name}.dat"
+        f.write_text(text)
+        e, arr = ref.rdati(f, eanz, ldc, RDATI_MODEL)
+        out[f"{name}__errnr"] = np.int32(e)
+        if e == 0:
+            for k, v in arr.items():
+                out[f"{name}__{k}"] = np.asarray(v)
+    np.savez_compressed(OUT / "ref_rdati.npz", **out)
+    print("wrote ref_rdati.npz", {n: int(out[f"{n}__errnr"]) for n, *_ in RDATI_FILES})
+
+
+CASES = {"inversion": inversion_steps, "blam0": blam0_case, "rdati": rdati_cases}
 
 
 def main():
@@ -243,6 +283,7 @@ def main():
     fixture("rect_complex", 1)
     inversion_steps()
     blam0_case()
+    rdati_cases()
 
 
 if __name__ == "__main__":

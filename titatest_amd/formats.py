@@ -10,7 +10,11 @@ tomodir prepared for CRMod reads the same here:
   with phi given in mrad (src/rsigma.f90:187).
 * ``config.dat`` -- src/rdatm.f90:54-115 (CRTomo ``A*10^4+B`` form or ``A B M N``).
 * ``volt.dat``  -- src/wdatm.f90:55-79 (writer, including the polarity swap of the
-  potential pair when the phase is near +-pi) and src/rdati.f90:84-240 (reader).
+  potential pair when the phase is near +-pi); ``read_volt`` gives the columns
+  and ``read_data`` reads a measured data file the way CRTomo's rdati does
+  (src/rdati.f90:84-363: format detection, individual errors or the error
+  model, validation with the reference's error numbers, the data vector and
+  weights).
 
 All node / element / electrode numbers are kept 1-based, as in the files and in
 the Fortran module arrays the C-ABI mirrors.
@@ -283,3 +287,180 @@ def write_volt(path, strnr, vnr, sigmaa, full_precision=True) -> None:
     body = "".join(fmt.format(int(a), int(b), float(m), float(p))
                    for a, b, m, p in zip(strnr, vout, bet, pha_mrad))
     Path(path).write_text(f"{len(strnr)}\n" + body)
+
+
+# ---- measured data (CRTomo's rdati) -----------------------------------------
+
+class FormatError(ValueError):
+    """A file the reference rejects, with the reference's error number."""
+
+    def __init__(self, errnr: int, msg: str):
+        super().__init__(f"{msg} (errnr {errnr})")
+        self.errnr = errnr
+
+
+class _Records:
+    """List-directed reads in the manner of Fortran READ(kanal,*): each read
+    starts at the next record, may continue over following records until it
+    has its values, and skips the rest of its last record."""
+
+    def __init__(self, text: str):
+        self.lines = [ln.replace(",", " ").split() for ln in text.splitlines()]
+        self.pos = 0
+
+    def read(self, n: int):
+        vals = []
+        while len(vals) < n:
+            if self.pos >= len(self.lines):
+                raise EOFError
+            rec = self.lines[self.pos]
+            self.pos += 1
+            vals.extend(rec[: n - len(vals)])
+        return vals
+
+    def backspace(self):
+        self.pos = max(0, self.pos - 1)
+
+
+def _f_logical(tok: str):
+    t = tok.lstrip(".").lower()
+    if t[:1] == "t":
+        return True
+    if t[:1] == "f":
+        return False
+    raise ValueError(tok)
+
+
+def _f_int(tok: str) -> int:
+    return int(tok)
+
+
+def _f_real(tok: str) -> float:
+    return float(tok.lower().replace("d", "e"))
+
+
+@dataclasses.dataclass
+class DataSet:
+    """Measured data as rdati stores it (datmod/invmod arrays)."""
+    strnr: np.ndarray      # (nanz,) int32, A*10^4 + B (current electrodes)
+    vnr: np.ndarray        # (nanz,) int32, M*10^4 + N (potential electrodes)
+    dat: np.ndarray        # (nanz,) complex128: -ln|R| - i phase[rad]
+    wmatdr: np.ndarray     # 1 / sigma_{ln|R|}^2 (times the file's scale, lindiv)
+    wmatdp: np.ndarray     # 1 / sigma_phase^2 (0 for DC)
+    wmatd_cri: np.ndarray  # 1 / (sigma_{ln|R|}^2 + sigma_phase^2) (0 for DC)
+    wdfak: np.ndarray      # (nanz,) int32, 1
+    lindiv: bool           # individual errors in the file
+
+
+def read_data(path, eanz: int, ldc: bool = False, stabw0: float = 5.0, stabm0: float = 1e-3,
+              stabpA1: float = 0.0, stabpB: float = 0.0, stabpA2: float = 0.0, stabp0: float = 0.0) -> DataSet:
+    """Read a measured data file (volt.dat layout) with the semantics of
+    src/rdati.f90:84-363 (its synthetic-noise option excepted).
+
+    * First record ``nanz [lindiv]``; without a logical second value the
+      error model applies (:84-88).  The first datum's first value tells the
+      CRTomo ``A*10^4+B M*10^4+N`` form (>= 10000) from ``A B M N`` (:90-94).
+    * Individual errors (lindiv): each datum carries the magnitude error in
+      Ohm and, unless DC, the phase error in mrad; a last record holds global
+      factors applied to the weights (:144-214, :344-363).
+    * Error model: sigma_{ln|R|} = stabw0/100 + stabm0/|R| and sigma_phase =
+      (stabpA1 |R|^stabpB + stabpA2/100 |phase| + stabp0) mrad (:216-254).
+    * Errors as the reference numbers them: 73 duplicate electrodes (lindiv
+      only, as there), 46 electrode out of range, 88 non-positive error, 94
+      non-positive |R|, 95 |phase| > 1000 pi mrad, 2 premature end of file,
+      3 unreadable value.
+    """
+    pi = math.acos(-1.0)
+    rec = _Records(Path(path).read_text())
+    try:
+        first = rec.read(1)
+        nanz = _f_int(first[0])
+        lindiv = False
+        line0 = rec.lines[0]
+        if len(line0) >= 2:
+            try:
+                lindiv = _f_logical(line0[1])
+            except ValueError:
+                lindiv = False
+        probe = rec.read(1)
+        rec.backspace()
+        crtf = _f_int(probe[0]) - 10000 >= 0
+        strnr = np.zeros(nanz, np.int32)
+        vnr = np.zeros(nanz, np.int32)
+        dat = np.zeros(nanz, np.complex128)
+        wmatdr = np.zeros(nanz)
+        wmatdp = np.zeros(nanz)
+        wcri = np.zeros(nanz)
+        ncol = (2 if crtf else 4) + (1 if ldc else 2) + ((1 if ldc else 2) if lindiv else 0)
+        pha = 0.0
+        for i in range(nanz):
+            v = rec.read(ncol)
+            if crtf:
+                strnr[i], vnr[i] = _f_int(v[0]), _f_int(v[1])
+                k = 2
+            else:
+                e1, e2, e3, e4 = (_f_int(x) for x in v[:4])
+                strnr[i], vnr[i] = e1 * 10000 + e2, e3 * 10000 + e4
+                k = 4
+            bet = _f_real(v[k])
+            if not ldc:
+                pha = _f_real(v[k + 1])
+            stabwp = 0.0
+            if lindiv:
+                stabw = _f_real(v[k + (1 if ldc else 2)])
+                if not ldc:
+                    stabwp = _f_real(v[k + 3])
+                    if stabwp <= 0.0:
+                        raise FormatError(88, f"non-positive phase error of datum {i + 1}")
+                    stabwp = 1e-3 * stabwp
+                with np.errstate(divide="ignore", invalid="ignore"):
+                    stabw = float(np.float64(stabw) / np.float64(bet))
+                e1, e2 = int(strnr[i]) % 10000, int(strnr[i]) // 10000
+                e3, e4 = int(vnr[i]) % 10000, int(vnr[i]) // 10000
+                if (e1 == e2 or e3 == e4 or ((e1 == e3 or e1 == e4) and e1 != 0)
+                        or ((e2 == e3 or e2 == e4) and e2 != 0)):
+                    raise FormatError(73, f"duplicate electrodes for reading {i + 1}")
+                if min(e1, e2, e3, e4) < 0 or max(e1, e2, e3, e4) > eanz:
+                    raise FormatError(46, f"electrode pair {i + 1} not correct")
+                if stabw <= 0.0:
+                    raise FormatError(88, f"non-positive magnitude error of datum {i + 1}")
+            else:
+                if not ldc:
+                    stabwp = (stabpA1 * bet ** stabpB + 1e-2 * stabpA2 * abs(pha) + stabp0) * 1e-3
+                if bet <= 0.0:
+                    raise FormatError(94, f"non-positive resistance of datum {i + 1}")
+                stabw = 1e-2 * stabw0 + stabm0 / bet
+            if bet <= 0.0:
+                raise FormatError(94, f"non-positive resistance of datum {i + 1}")
+            if ldc:
+                pha = 0.0
+            elif abs(pha) > 1e3 * pi:
+                raise FormatError(95, f"phase of datum {i + 1} beyond 1000 pi mrad")
+            dat[i] = complex(-math.log(bet), -pha / 1e3)
+            wmatdr[i] = 1.0 / (stabw ** 2.0)
+            if not ldc:
+                wcri[i] = 1.0 / (stabw ** 2.0 + stabwp ** 2.0)
+                wmatdp[i] = 1.0 / (stabwp ** 2.0)
+            e1, e2 = int(strnr[i]) % 10000, int(strnr[i]) // 10000
+            e3, e4 = int(vnr[i]) % 10000, int(vnr[i]) // 10000
+            if min(e1, e2, e3, e4) < 0 or max(e1, e2, e3, e4) > eanz:
+                raise FormatError(46, f"electrode pair {i + 1} not correct")
+        if lindiv:
+            if ldc:
+                s = _f_real(rec.read(1)[0])
+                if s <= 0.0:
+                    raise FormatError(88, "non-positive global error factor")
+                wmatdr = wmatdr * s * s
+            else:
+                s, sp = (_f_real(x) for x in rec.read(2))
+                if s <= 0.0 or sp <= 0.0:
+                    raise FormatError(88, "non-positive global error factor")
+                wmatdr = wmatdr * s * s
+                wmatdp = wmatdp * sp * sp
+    except EOFError:
+        raise FormatError(2, "premature end of the data file") from None
+    except (ValueError, IndexError) as e:
+        if isinstance(e, FormatError):
+            raise
+        raise FormatError(3, f"unreadable value in the data file: {e}") from None
+    return DataSet(strnr, vnr, dat, wmatdr, wmatdp, wcri, np.ones(nanz, np.int32), lindiv)
