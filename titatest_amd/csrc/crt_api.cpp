@@ -911,7 +911,8 @@ static int run_forward(crt_handle* h) {
     const unsigned rgrid = (unsigned)(((size_t)cnt * n + 255) / 256);
     tstart(h, "assemble_scale");
     hipLaunchKernelGGL(crt_scale_kernel, dim3(rgrid), dim3(256), 0, h->stream, A);
-    hipLaunchKernelGGL(crt_rows_kernel, dim3(rgrid), dim3(256), 0, h->stream, A);
+    hipLaunchKernelGGL(crt_rows_kernel, dim3((unsigned)(((size_t)cnt * n * h->R + 255) / 256)), dim3(256), 0,
+                       h->stream, A);
     if (rb2) hipLaunchKernelGGL(crt_rhs_kernel, dim3(rgrid), dim3(256), 0, h->stream, A);
     tstop(h);
     h->slots = std::min(cnt, h->ncu * per_cu);

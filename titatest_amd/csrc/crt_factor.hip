@@ -281,29 +281,29 @@ __global__ void crt_scale_kernel(FactorArgs A) {
 }
 
 // Scaled band entries of every row of every system of the batch: a_rr/|a_rr|
-// on the diagonal, (f_r f_c) a_rc elsewhere (scalab :58-73).
+// on the diagonal, (f_r f_c) a_rc elsewhere (scalab :58-73).  One thread per
+// entry (system, row, slot), so that consecutive threads read av and write S
+// at consecutive addresses.
 __global__ void crt_rows_kernel(FactorArgs A) {
   const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= (long)A.nsys * A.n) return;
-  const int s = (int)(idx / A.n), r = (int)(idx % A.n);
+  if (idx >= (long)A.nsys * A.n * A.R) return;
+  const long rr = idx / A.R;
+  const int q = (int)(idx - rr * A.R);
+  const int s = (int)(rr / A.n), r = (int)(rr % A.n);
   const int sys = (A.sys_first + s) * A.sys_stride;
   const int k = sys / A.eanz, l = sys % A.eanz;
   const double* fk = A.fak + (size_t)s * A.n;
-  const double fr = fk[r];
-  const int b0 = A.bptr[r], b1 = A.bptr[r + 1];
-  double2* out = A.S + ((size_t)s * A.n + r) * A.R;
-  for (int q = 0; q < A.R; ++q) {
-    const int t = A.pat_t[(size_t)r * A.R + q];
-    cplx v = mkc(0.0, 0.0);
-    if (t >= 0) {
-      v = ldc(A.av + ((size_t)k * A.n + r) * A.R + q);
-      for (int p = b0; p < b1; ++p)
-        if (A.bslot[p] == q) v = cadd(v, bnd_term(A, k, l, A.bcode[p], A.bimn[p]));
-      if (t == A.mb) v = A.dc ? mkc(1.0, 0.0) : cdiv(v, mkc(cabs_(v), 0.0));  // scaldc :60
-      else v = rmul(fr * fk[r - A.mb + t], v);
-    }
-    out[q] = make_double2(v.re, v.im);
+  const int t = A.pat_t[(size_t)r * A.R + q];
+  cplx v = mkc(0.0, 0.0);
+  if (t >= 0) {
+    v = ldc(A.av + ((size_t)k * A.n + r) * A.R + q);
+    const int b0 = A.bptr[r], b1 = A.bptr[r + 1];
+    for (int p = b0; p < b1; ++p)
+      if (A.bslot[p] == q) v = cadd(v, bnd_term(A, k, l, A.bcode[p], A.bimn[p]));
+    if (t == A.mb) v = A.dc ? mkc(1.0, 0.0) : cdiv(v, mkc(cabs_(v), 0.0));  // scaldc :60
+    else v = rmul(fk[r] * fk[r - A.mb + t], v);
   }
+  A.S[idx] = make_double2(v.re, v.im);
 }
 
 // Right-hand side of every system of the batch when boundary values are
