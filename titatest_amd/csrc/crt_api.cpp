@@ -51,6 +51,10 @@ struct crt_handle {
   // CG iterations as one captured HIP graph (crt_set_graph), with the kernel
   // arguments and iteration count it was captured for
   bool use_graph = false;
+  // distinct electrode pairs of the loaded measurements (sensitivity kernel)
+  std::vector<int> pairs_key;
+  std::vector<int2> sens_cp, sens_pp, sens_mp;
+  std::vector<int> sens_ord;
   // element-to-parameter map (modelmod::mnr) for bpar/bsigma; empty = identity
   int manz_map = 0;
   std::vector<int> mnr_host;
@@ -1142,9 +1146,60 @@ static int run_sens(crt_handle* h, int32_t mode, const double2* sigma) {
   NEED(S.sens);
   S.kpt = transposed_kpot(h);
   NEED(S.kpt);
-  const size_t lds = (size_t)4 * h->E * 16;
+  // distinct current and potential pairs (rebuilt when the measurements change)
+  std::vector<int> key(h->strnr);
+  key.insert(key.end(), h->vnr.begin(), h->vnr.end());
+  if (key != h->pairs_key) {
+    std::map<int, int> ci, pi;
+    for (int v : h->strnr) ci.emplace(v, 0);
+    for (int v : h->vnr) pi.emplace(v, 0);
+    h->sens_cp.clear();
+    h->sens_pp.clear();
+    for (auto& kv : ci) {
+      kv.second = (int)h->sens_cp.size();
+      h->sens_cp.push_back(make_int2(kv.first % 10000, kv.first / 10000));
+    }
+    for (auto& kv : pi) {
+      kv.second = (int)h->sens_pp.size();
+      h->sens_pp.push_back(make_int2(kv.first % 10000, kv.first / 10000));
+    }
+    h->sens_mp.resize(h->nanz);
+    for (int i = 0; i < h->nanz; ++i) h->sens_mp[i] = make_int2(ci[h->strnr[i]], pi[h->vnr[i]]);
+    h->sens_ord.resize(h->nanz);
+    std::iota(h->sens_ord.begin(), h->sens_ord.end(), 0);
+    std::stable_sort(h->sens_ord.begin(), h->sens_ord.end(), [&](int a, int b) {
+      const int2 x = h->sens_mp[a], y = h->sens_mp[b];
+      return x.x != y.x ? x.x < y.x : x.y < y.y;
+    });
+    int2 *dc, *dp, *dm;
+    int* dord;
+    if (upload(h, "sens_cp", h->sens_cp.data(), h->sens_cp.size(), &dc) ||
+        upload(h, "sens_pp", h->sens_pp.data(), h->sens_pp.size(), &dp) ||
+        upload(h, "sens_mp", h->sens_mp.data(), h->sens_mp.size(), &dm) ||
+        upload(h, "sens_ord", h->sens_ord.data(), h->sens_ord.size(), &dord))
+      return kErrHip;
+    h->pairs_key.swap(key);
+  }
+  SensPairs Q;
+  Q.ncp = (int)h->sens_cp.size();
+  Q.npp = (int)h->sens_pp.size();
+  const size_t lds_pairs = (size_t)(4 * h->E + 4 * Q.ncp + 4 * Q.npp) * 16;
   tstart(h, "sens");
-  hipLaunchKernelGGL(crt_sens_kernel, dim3(h->elanz), dim3(256), lds, h->stream, S);
+  // the pair form: 256 threads while its LDS leaves room for two workgroups
+  // per CU, else 512 in one
+  if (lds_pairs <= h->lds_max) {
+    const int nt = 2 * lds_pairs <= h->lds_max ? 256 : 512;
+    Q.cp = h->at<int2>("sens_cp");
+    Q.pp = h->at<int2>("sens_pp");
+    Q.mp = h->at<int2>("sens_mp");
+    Q.ord = h->at<int>("sens_ord");
+    hipFuncSetAttribute(reinterpret_cast<const void*>(crt_sens_pairs_kernel),
+                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_pairs);
+    hipLaunchKernelGGL(crt_sens_pairs_kernel, dim3(h->elanz), dim3(nt), lds_pairs, h->stream, S, Q);
+  } else {
+    const size_t lds = (size_t)4 * h->E * 16;
+    hipLaunchKernelGGL(crt_sens_kernel, dim3(h->elanz), dim3(256), lds, h->stream, S);
+  }
   tstop(h);
   return finish(h);
 }

@@ -36,6 +36,19 @@ struct SensArgs {
   double2* sens;          // nanz x elanz, column major
 };
 
+// Distinct electrode pairs of the measurements for crt_sens_pairs_kernel:
+// current pairs cp (one per distinct strnr), potential pairs pp (one per
+// distinct vnr), and per measurement its (cp, pp) indices.
+struct SensPairs {
+  int ncp, npp;
+  const int2* cp;  // (e1, e2), 1-based electrode numbers, 0 = none
+  const int2* pp;  // (e3, e4)
+  const int2* mp;  // [nanz] (cp index, pp index)
+  const int* ord;  // [nanz] measurements in (cp, pp) order: the lanes of a
+                   // wavefront then mostly share u and read neighbouring d34
+};
+constexpr int kSensPairQ = 16;  // measurements per thread per pass
+
 struct AdjArgs {
   int n, elanz, eanz, K, nimn, swrtr, mode;
   const int* el_nk;
@@ -78,6 +91,7 @@ __global__ void crt_rtrafo_elec_kernel(const double2* kpot, const double* kwnwi,
 __global__ void crt_volt_kernel(const double2* he, int eanz, int nanz, const int* strnr, const int* vnr,
                                 int mode, const double* kfak, double2* volt, double2* sigmaa, int* err);
 __global__ void crt_sens_kernel(SensArgs S);
+__global__ void crt_sens_pairs_kernel(SensArgs S, SensPairs Q);
 __global__ void crt_kpot_transpose_kernel(const double2* kpot, int n, int E, double2* kpt);
 __global__ void crt_adj_gamma_kernel(int nanz, int mode, const double* w, const double2* resid,
                                      const double2* volt, double2* gamma);

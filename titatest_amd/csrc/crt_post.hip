@@ -335,12 +335,33 @@ __global__ __launch_bounds__(256) void crt_sens_kernel(SensArgs S) {
       e3[q] = vn % 10000;
       e4[q] = vn / 10000;
     }
+    // the element's potentials of wavenumber k + 1 are loaded into registers
+    // while k is evaluated (kSensPre per thread covers 4 E for E <= 128 at
+    // 256 threads); larger E loads the rest when staging
+    constexpr int kSensPre = 2;
+    cplx pre[kSensPre];
+    auto fetch = [&](int k) {
+#pragma unroll
+      for (int r = 0; r < kSensPre; ++r) {
+        const int t = threadIdx.x + r * blockDim.x;
+        const int q = t / E, a = t - q * E;
+        pre[r] = (t < 4 * E && q < nk) ? ldc(S.kpt + ((size_t)k * S.n + nodes[q]) * E + a) : mkc(0.0, 0.0);
+      }
+    };
+    __syncthreads();  // nodes[]
+    fetch(0);
     for (int k = 0; k < S.K; ++k) {
       __syncthreads();
-      for (int t = threadIdx.x; t < 4 * E; t += blockDim.x) {
+#pragma unroll
+      for (int r = 0; r < kSensPre; ++r) {
+        const int t = threadIdx.x + r * blockDim.x;
+        if (t < 4 * E) P[t] = pre[r];
+      }
+      for (int t = threadIdx.x + kSensPre * blockDim.x; t < 4 * E; t += blockDim.x) {
         const int q = t / E, a = t - q * E;
         P[t] = q < nk ? ldc(S.kpt + ((size_t)k * S.n + nodes[q]) * E + a) : mkc(0.0, 0.0);
       }
+      if (k + 1 < S.K) fetch(k + 1);
       if (threadIdx.x < 16) {
         const int a = threadIdx.x >> 2, b = threadIdx.x & 3;
         const int imax = max(a, b), imin = min(a, b);
@@ -376,6 +397,119 @@ __global__ __launch_bounds__(256) void crt_sens_kernel(SensArgs S) {
     for (int q = 0; q < kSensMaxQ; ++q) {
       const int i = i0 + threadIdx.x + q * blockDim.x;
       if (i >= S.nanz) continue;
+      cplx out;
+      if (S.mode == 0) {
+        cplx dum = (S.swrtr == 0) ? acc[q] : rmul(1.0 / M_PI, acc[q]);
+        out = mkc(0.0 - dum.re, 0.0 - dum.im);
+      } else {
+        cplx dum = (S.swrtr == 0) ? acc[q] : cdiv(acc[q], mkc(M_PI, 0.0));
+        out = cdiv(cmul(dum, sig), ldc(S.volt + i));
+      }
+      stc(S.sens + (size_t)e * S.nanz + i, out);
+    }
+  }
+}
+
+// The explicit Jacobian with the per-pair work hoisted: d12 = P(:,e2) -
+// P(:,e1) and u_a = sum_b S_ab d12_b depend only on the measurement's current
+// pair, d34 = P(:,e4) - P(:,e3) only on its potential pair.  Per element and
+// wavenumber, u of every distinct current pair and d34 of every distinct
+// potential pair are formed once in LDS (the same operations, in the same
+// order, as the per-measurement form of crt_sens_kernel); each measurement then
+// takes h_k = sum_a d34_a u_a from two LDS rows.  The measurements are taken in
+// (current pair, potential pair) order and written to their own rows.  LDS:
+// (4 E + 4 ncp + 4 npp) complex values.
+__global__ __launch_bounds__(512) void crt_sens_pairs_kernel(SensArgs S, SensPairs Q) {
+  extern __shared__ double2 sm[];
+  const int e = blockIdx.x;
+  const int nk = S.el_nk[e];
+  const int E = S.eanz, ncp = Q.ncp, npp = Q.npp;
+  cplx* P = reinterpret_cast<cplx*>(sm);  // [4][E]
+  cplx* U = P + 4 * E;                    // [4][ncp]
+  cplx* D = U + 4 * ncp;                  // [4][npp]
+  __shared__ double sel[4][4];
+  __shared__ int nodes[4];
+  if (threadIdx.x < 4) nodes[threadIdx.x] = S.el_nodes[e * 4 + threadIdx.x];
+  const cplx sig = ldc(S.sigma + e);
+  auto pick = [&](int m, int el) { return csel(el > 0, P[m * E + max(el - 1, 0)], mkc(0.0, 0.0)); };
+  for (int i0 = 0; i0 < S.nanz; i0 += kSensPairQ * blockDim.x) {
+    cplx acc[kSensPairQ];
+    int ci[kSensPairQ], pi[kSensPairQ];
+#pragma unroll
+    for (int q = 0; q < kSensPairQ; ++q) {
+      acc[q] = mkc(0.0, 0.0);
+      const int i = Q.ord[min(i0 + (int)threadIdx.x + q * (int)blockDim.x, S.nanz - 1)];
+      const int2 m = Q.mp[i];
+      ci[q] = m.x;
+      pi[q] = m.y;
+    }
+    // the element's potentials of wavenumber k + 1 are loaded into registers
+    // while k is evaluated (kSensPre per thread covers 4 E for E <= 128 at
+    // 256 threads); larger E loads the rest when staging
+    constexpr int kSensPre = 2;
+    cplx pre[kSensPre];
+    auto fetch = [&](int k) {
+#pragma unroll
+      for (int r = 0; r < kSensPre; ++r) {
+        const int t = threadIdx.x + r * blockDim.x;
+        const int q = t / E, a = t - q * E;
+        pre[r] = (t < 4 * E && q < nk) ? ldc(S.kpt + ((size_t)k * S.n + nodes[q]) * E + a) : mkc(0.0, 0.0);
+      }
+    };
+    __syncthreads();  // nodes[]
+    fetch(0);
+    for (int k = 0; k < S.K; ++k) {
+      __syncthreads();
+#pragma unroll
+      for (int r = 0; r < kSensPre; ++r) {
+        const int t = threadIdx.x + r * blockDim.x;
+        if (t < 4 * E) P[t] = pre[r];
+      }
+      for (int t = threadIdx.x + kSensPre * blockDim.x; t < 4 * E; t += blockDim.x) {
+        const int q = t / E, a = t - q * E;
+        P[t] = q < nk ? ldc(S.kpt + ((size_t)k * S.n + nodes[q]) * E + a) : mkc(0.0, 0.0);
+      }
+      if (k + 1 < S.K) fetch(k + 1);
+      if (threadIdx.x < 16) {
+        const int a = threadIdx.x >> 2, b = threadIdx.x & 3;
+        const int imax = max(a, b), imin = min(a, b);
+        sel[a][b] = (a < nk && b < nk) ? S.elbg[((size_t)k * S.elanz + e) * S.nimn + (imax * (imax + 1)) / 2 + imin]
+                                       : 0.0;
+      }
+      __syncthreads();
+      for (int c = threadIdx.x; c < ncp; c += blockDim.x) {
+        const int2 pr = Q.cp[c];
+        cplx d12[4];
+#pragma unroll
+        for (int m = 0; m < 4; ++m) d12[m] = csub(pick(m, pr.y), pick(m, pr.x));
+#pragma unroll
+        for (int a = 0; a < 4; ++a) {
+          cplx u = mkc(0.0, 0.0);
+#pragma unroll
+          for (int b = 0; b < 4; ++b) u = cadd(u, rmul(sel[a][b], d12[b]));
+          U[a * ncp + c] = u;
+        }
+      }
+      for (int p = threadIdx.x; p < npp; p += blockDim.x) {
+        const int2 pr = Q.pp[p];
+#pragma unroll
+        for (int m = 0; m < 4; ++m) D[m * npp + p] = csub(pick(m, pr.y), pick(m, pr.x));
+      }
+      __syncthreads();
+      const double wk = S.kwnwi[k];
+#pragma unroll
+      for (int q = 0; q < kSensPairQ; ++q) {
+        cplx h = mkc(0.0, 0.0);
+#pragma unroll
+        for (int a = 0; a < 4; ++a) h = cadd(h, cmul(D[a * npp + pi[q]], U[a * ncp + ci[q]]));
+        acc[q] = S.swrtr == 0 ? h : cadd(acc[q], rmul(wk, h));
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < kSensPairQ; ++q) {
+      const int j = i0 + threadIdx.x + q * blockDim.x;
+      if (j >= S.nanz) continue;
+      const int i = Q.ord[j];
       cplx out;
       if (S.mode == 0) {
         cplx dum = (S.swrtr == 0) ? acc[q] : rmul(1.0 / M_PI, acc[q]);
