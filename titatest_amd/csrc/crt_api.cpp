@@ -66,6 +66,10 @@ struct crt_handle {
   std::vector<int> strnr, vnr;
   // device
   hipStream_t stream = nullptr;
+  // second stream (default priority, like the first): back substitutions that
+  // overlap the factorisation's last round; joined back before the forward ends
+  hipStream_t stream2 = nullptr;
+  hipEvent_t ev_fac = nullptr, ev_join = nullptr;
   int ncu = 0;
   size_t lds_max = 0;
   hipError_t hip_err = hipSuccess;
@@ -133,6 +137,20 @@ void tstart(crt_handle* h, const char* name) {
 void tstop(crt_handle* h) {
   if (!h->timing || h->pending.empty()) return;
   hipEventRecord(h->pending.back().b, h->stream);
+}
+// A stage on another stream: returns its slot for tstop_on.
+int tstart_on(crt_handle* h, const char* name, hipStream_t s) {
+  if (!h->timing) return -1;
+  Timer t;
+  t.name = name;
+  hipEventCreate(&t.a);
+  hipEventCreate(&t.b);
+  hipEventRecord(t.a, s);
+  h->pending.push_back(t);
+  return (int)h->pending.size() - 1;
+}
+void tstop_on(crt_handle* h, int slot, hipStream_t s) {
+  if (slot >= 0 && slot < (int)h->pending.size()) hipEventRecord(h->pending[slot].b, s);
 }
 void tcollect(crt_handle* h) {
   for (auto& t : h->pending) {
@@ -504,6 +522,12 @@ int crt_create(crt_handle** out, const crt_mesh* m) {
     delete h;
     return kErrHip;
   }
+  if (hipStreamCreateWithFlags(&h->stream2, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreateWithFlags(&h->ev_fac, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&h->ev_join, hipEventDisableTiming) != hipSuccess) {
+    crt_destroy(h);
+    return kErrHip;
+  }
 
   {
     const int rc = build_plan(h);
@@ -584,6 +608,7 @@ int crt_create(crt_handle** out, const crt_mesh* m) {
 int crt_destroy(crt_handle* h) {
   if (!h) return 0;
   if (h->stream) hipStreamSynchronize(h->stream);
+  if (h->stream2) hipStreamSynchronize(h->stream2);
   if (h->cg_exec) hipGraphExecDestroy(h->cg_exec);
   if (h->cg_graph) hipGraphDestroy(h->cg_graph);
   for (auto& kv : h->buf) hipFree(kv.second);
@@ -591,6 +616,9 @@ int crt_destroy(crt_handle* h) {
     hipEventDestroy(t.a);
     hipEventDestroy(t.b);
   }
+  if (h->ev_fac) hipEventDestroy(h->ev_fac);
+  if (h->ev_join) hipEventDestroy(h->ev_join);
+  if (h->stream2) hipStreamDestroy(h->stream2);
   if (h->stream) hipStreamDestroy(h->stream);
   delete h;
   return 0;
@@ -920,10 +948,51 @@ static int run_forward(crt_handle* h) {
     if (rb2) hipLaunchKernelGGL(crt_rhs_kernel, dim3(rgrid), dim3(256), 0, h->stream, A);
     tstop(h);
     h->slots = std::min(cnt, h->ncu * per_cu);
+    const int nrhs = cnt * rpf;
+    // Own factors per system: the systems of the persistent kernel's full
+    // rounds are factored first, then the partial last round, during which
+    // the full rounds' back substitutions run on the second stream (a CU in
+    // the last round holds fewer factor workgroups and has room for them).
+    const int full = (cnt / h->slots) * h->slots;
+    if (!reuse && full > 0 && full < cnt) {
+      for (int part = 0; part < 2; ++part) {
+        const int p0 = part ? full : 0, pc = part ? cnt - full : full;
+        FactorArgs Ap = A;
+        Ap.sys_first = first + p0;
+        Ap.nsys = pc;
+        Ap.L = A.L + (size_t)p0 * n * (mb + 1);
+        Ap.S = A.S + (size_t)p0 * n * h->R;
+        Ap.y = A.y + (size_t)p0 * n;
+        Ap.fak = A.fak + (size_t)p0 * n;
+        if (A.rhs) Ap.rhs = A.rhs + (size_t)p0 * n;
+        tstart(h, "factor_solve");
+        hipLaunchKernelGGL(kern, dim3(std::min(pc, h->slots)), dim3(nt), lds, h->stream, Ap);
+        tstop(h);
+        BacksolveArgs Bp = B;
+        Bp.nrhs = pc;
+        Bp.fac_first = first + p0;
+        Bp.L = Ap.L;
+        Bp.y = Ap.y;
+        Bp.fak = Ap.fak;
+        hipStream_t bs = part ? h->stream : h->stream2;
+        if (!part) {
+          HIPCHK(hipEventRecord(h->ev_fac, h->stream));
+          HIPCHK(hipStreamWaitEvent(h->stream2, h->ev_fac, 0));
+        }
+        const int slot = tstart_on(h, "backsolve", bs);
+        if (mb <= 120)
+          hipLaunchKernelGGL((crt_backsolve_kernel<8, 8>), dim3(pc), dim3(64), 0, bs, Bp);
+        else
+          hipLaunchKernelGGL((crt_backsolve_kernel<8, 4>), dim3(pc), dim3(64), 0, bs, Bp);
+        tstop_on(h, slot, bs);
+      }
+      HIPCHK(hipEventRecord(h->ev_join, h->stream2));
+      HIPCHK(hipStreamWaitEvent(h->stream, h->ev_join, 0));
+      continue;
+    }
     tstart(h, "factor_solve");
     hipLaunchKernelGGL(kern, dim3(h->slots), dim3(nt), lds, h->stream, A);
     tstop(h);
-    const int nrhs = cnt * rpf;
     if (reuse) {
       F.nrhs = nrhs;
       F.fac_first = first;
