@@ -55,6 +55,7 @@ struct crt_handle {
   std::vector<int> pairs_key;
   std::vector<int2> sens_cp, sens_pp, sens_mp;
   std::vector<int> sens_ord;
+  size_t sens_lds_set = 0;  // dynamic LDS limit set for the pair kernel
   // element-to-parameter map (modelmod::mnr) for bpar/bsigma; empty = identity
   int manz_map = 0;
   std::vector<int> mnr_host;
@@ -953,8 +954,10 @@ static int run_forward(crt_handle* h) {
     // rounds are factored first, then the partial last round, during which
     // the full rounds' back substitutions run on the second stream (a CU in
     // the last round holds fewer factor workgroups and has room for them).
+    // Only for forwards of at least two full rounds: smaller ones leave the
+    // device to other handles' work, which the cross-stream wait would hold.
     const int full = (cnt / h->slots) * h->slots;
-    if (!reuse && full > 0 && full < cnt) {
+    if (!reuse && full >= 2 * h->slots && full < cnt) {
       for (int part = 0; part < 2; ++part) {
         const int p0 = part ? full : 0, pc = part ? cnt - full : full;
         FactorArgs Ap = A;
@@ -1262,8 +1265,11 @@ static int run_sens(crt_handle* h, int32_t mode, const double2* sigma) {
     Q.pp = h->at<int2>("sens_pp");
     Q.mp = h->at<int2>("sens_mp");
     Q.ord = h->at<int>("sens_ord");
-    hipFuncSetAttribute(reinterpret_cast<const void*>(crt_sens_pairs_kernel),
-                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_pairs);
+    if (lds_pairs > h->sens_lds_set) {
+      HIPCHK(hipFuncSetAttribute(reinterpret_cast<const void*>(crt_sens_pairs_kernel),
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_pairs));
+      h->sens_lds_set = lds_pairs;
+    }
     hipLaunchKernelGGL(crt_sens_pairs_kernel, dim3(h->elanz), dim3(nt), lds_pairs, h->stream, S, Q);
   } else {
     const size_t lds = (size_t)4 * h->E * 16;
@@ -1458,6 +1464,15 @@ int crt_cg_solve(crt_handle* h, const double* w, const double* dat, const double
   G.dpar = h->get<double2>("cg_dpar", M);
   G.cgres = h->get<double>("cg_cgres", ncgmax + 2);
   G.scal = h->get<double>("cg_scal", 8);
+  // J (cgfac o p) in column slices: about 2048 workgroups over the row blocks
+  {
+    const int rb = (N + 255) / 256;
+    G.nslice = std::max(1, std::min(std::min(128, M), (2048 + rb - 1) / rb));
+    G.chunk = (M + G.nslice - 1) / G.nslice;
+    G.nslice = (M + G.chunk - 1) / G.chunk;
+  }
+  G.part = h->get<double2>("cg_part", (size_t)G.nslice * N);
+  NEED(G.part);
   NEED(G.cgfac);
   NEED(G.bvec);
   NEED(G.rvec);
@@ -1483,7 +1498,8 @@ int crt_cg_solve(crt_handle* h, const double* w, const double* dat, const double
       hipLaunchKernelGGL(crt_cg_scalar_kernel, dim3(1), dim3(1024), 0, h->stream, G, 0, it);
       if (timed) tstop(h);
       if (timed) tstart(h, "cg_bap");
-      hipLaunchKernelGGL(crt_cg_bap_kernel, dim3((N + 255) / 256), dim3(256), 0, h->stream, G);
+      hipLaunchKernelGGL(crt_cg_bap_kernel, dim3((N + 255) / 256, G.nslice), dim3(256), 0, h->stream, G);
+      hipLaunchKernelGGL(crt_cg_bap_sum_kernel, dim3((N + 255) / 256), dim3(256), 0, h->stream, G);
       if (timed) tstop(h);
       if (timed) tstart(h, "cg_bb");
       hipLaunchKernelGGL(crt_cg_bb_kernel, dim3((unsigned)(((size_t)M * 64 + 255) / 256)), dim3(256), 0,

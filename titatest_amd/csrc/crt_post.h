@@ -78,6 +78,8 @@ struct CgArgs {
   double2* dpar;
   double* cgres;          // ncgmax + 2
   double* scal;           // 8 device scalars of the recurrence
+  double2* part;          // [nslice][nanz] partial sums of J (cgfac o p)
+  int nslice, chunk;      // column slices of that product and their width
 };
 
 __global__ void crt_precal_area_kernel(PrecalArgs P);
@@ -103,6 +105,7 @@ __global__ void crt_jhwr_kernel(const double2* J, int nanz, int manz, const doub
                                 double2* g);
 __global__ void crt_cg_rhs_kernel(CgArgs G, const double2* dat, const double2* sigmaa, const double2* rm);
 __global__ void crt_cg_bap_kernel(CgArgs G);
+__global__ void crt_cg_bap_sum_kernel(CgArgs G);
 __global__ void crt_cg_bb_kernel(CgArgs G);
 __global__ void crt_cg_scalar_kernel(CgArgs G, int stage, int iter);
 

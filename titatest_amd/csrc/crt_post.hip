@@ -782,15 +782,32 @@ __global__ __launch_bounds__(256) void crt_cg_rhs_kernel(CgArgs G, const double2
 
 // ap_i = sum_j p_j J_ij cgfac_j (bap, src/cg_mod.f90:520-545).  Thread per
 // row i, coalesced over i for each column.
+// in two stages: slice q of the columns (chunk consecutive j, ascending) for
+// rows i in stage 1, the slices added in order per row in stage 2, so that a
+// small nanz still fills the device
 __global__ void crt_cg_bap_kernel(CgArgs G) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int q = blockIdx.y;
   if (i >= G.nanz) return;
+  const int j0 = q * G.chunk, j1 = min(G.manz, j0 + G.chunk);
   double sr = 0.0, si = 0.0;
-  for (int j = 0; j < G.manz; ++j) {
+  for (int j = j0; j < j1; ++j) {
     const cplx p = rmul(G.cgfac[j], ldc(G.pvec + j));
     const cplx s = ldc(G.J + (size_t)j * G.nanz + i);
     sr += p.re * s.re - p.im * s.im;
     si += p.re * s.im + p.im * s.re;
+  }
+  stc(G.part + (size_t)q * G.nanz + i, mkc(sr, si));
+}
+
+__global__ void crt_cg_bap_sum_kernel(CgArgs G) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= G.nanz) return;
+  double sr = 0.0, si = 0.0;
+  for (int q = 0; q < G.nslice; ++q) {
+    const cplx v = ldc(G.part + (size_t)q * G.nanz + i);
+    sr += v.re;
+    si += v.im;
   }
   stc(G.ap + i, mkc(sr, si));
 }
