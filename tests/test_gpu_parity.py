@@ -262,8 +262,30 @@ def test_large_mesh_systems_match_oracle(crtmod, oracle_lib):
     check_volt(v, vo)
 
 
+def test_batching_does_not_change_the_potentials(crtmod):
+    """The 240x80 workload's forward in its default form (the full rounds of
+    the persistent factor kernel, then the last round with the first
+    launch's back substitutions on a second stream) and under a device budget
+    that splits it into batches of single launches: the same potentials, bit
+    for bit (every system is factored and substituted by the same code)."""
+    w = meshgen.large_grf()
+    say("large_grf forward: default vs 6 GB device budget")
+    out = []
+    for budget in (None, 6 << 30):  # a handle keeps the buffers it has: one each
+        g = crtmod.CRT(w.mesh)
+        if budget:
+            g.set_device_budget(budget)
+        g.rwaven(w.strnr, w.vnr)
+        g.precal()
+        (k,) = g.forward(w.sigma, want_kpot=True)
+        out.append(k)
+        del g
+    np.testing.assert_array_equal(out[0], out[1])
+
+
 def test_triangles_large_band_match_oracle(crtmod, oracle_lib):
-    """Triangular mesh with topography and mb = 101 (two trailing tiles per thread)."""
+    """Triangular mesh with topography and mb = 101 (the factor kernel's
+    512-thread form: one workgroup's LDS fills the CU)."""
     w = meshgen.topo_triangles(nx_nodes=40, nz_nodes=100, eanz=24)
     say("triangles mb=", w.mesh.mb, "n=", w.mesh.sanz)
     g = crtmod.CRT(w.mesh, lsytop=True)
