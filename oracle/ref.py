@@ -311,3 +311,25 @@ def rdati(path, eanz, ldc=False, model=(5.0, 1e-3, 0.0, 0.0, 0.0, 0.0), nmax=100
     n = int(info[0])
     return e, dict(strnr=st[:n], vnr=vn[:n], dat=dat[:n], wmatdr=wr[:n], wmatdp=wp[:n], wmatd_cri=wc[:n],
                    wdfak=wd[:n], lindiv=bool(info[1]))
+
+
+def update(sens, w, dat, sigmaa, par, smatm, lam, nx, nz, ncgmax, eps, step=1.0):
+    """The reference update (llam = .false., ltri = 0): (dpar, cgfac, cgres,
+    par_new, bdpar); dpar is scaled by cgfac as update leaves it."""
+    L = lib()
+    L.ref_update.argtypes = [C.c_int, C.c_int, _z, _d, _z, _z, _z, _d, C.c_double, C.c_int, C.c_int, C.c_int,
+                             C.c_double, C.c_double, _z, _d, _d, C.POINTER(C.c_double)]
+    S = np.asfortranarray(sens, np.complex128)
+    N, M = S.shape
+    p = np.array(par, np.complex128)
+    dpar = np.zeros(M, np.complex128)
+    cgfac = np.zeros(M)
+    cgres = np.zeros(ncgmax + 1)
+    bd = C.c_double()
+    e = L.ref_update(N, M, np.ascontiguousarray(S.ravel(order="F")), np.ascontiguousarray(w, np.float64),
+                     np.ascontiguousarray(dat, np.complex128), np.ascontiguousarray(sigmaa, np.complex128), p,
+                     np.ascontiguousarray(np.asfortranarray(smatm, np.float64).ravel(order="F")), float(lam),
+                     int(nx), int(nz), int(ncgmax), float(eps), float(step), dpar, cgfac, cgres, C.byref(bd))
+    if e:
+        raise RefError("update", e)
+    return dpar, cgfac, cgres, p, bd.value

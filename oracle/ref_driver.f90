@@ -691,4 +691,61 @@ contains
     c_wdfak(1:nanz) = wdfak
   end function ref_rdati
 
+  !> Reference update (src/update.f90, not the lambda-search branch: llam =
+  !> .false.) with the rectangular smoothness operator (ltri = 0) on the
+  !> caller's J (nanz x manz, column major), weights w (wdfak = 1), data dat,
+  !> model response sigmaa and model par (in/out: par + step dpar).  Runs the
+  !> reference's cgfac, right-hand side, cjg -> cjggra and model update.
+  !> Outputs dpar (scaled by cgfac, as update leaves it), cgfac, cgres
+  !> (ncgmax+1: iterations, then the residuals) and bdpar.
+  integer(c_int) function ref_update(c_nanz, c_manz, c_sens, c_w, c_dat, c_sigmaa, c_par, c_smatm, &
+       c_lam, c_nx, c_nz, c_ncgmax, c_eps, c_step, c_dpar, c_cgfac, c_cgres, c_bdpar) bind(C, name='ref_update')
+    use invmod, only: dat, wmatd, wdfak, par, dpar, dpar2, lfpi
+    use cjgmod, only: cgres, cgres2, bvec, cgfac, eps, ncgmax, con_cjgmod, des_cjgmod
+    use konvmod, only: ltri, lam, nx, nz, lverb, lw_ref, llam, step, lphi0, lprior, ldiff, bdpar
+    use femmod, only: ldc
+    use modelmod, only: w_ref_re, w_ref_im
+    integer(c_int), value :: c_nanz, c_manz, c_nx, c_nz, c_ncgmax
+    complex(c_double_complex), intent(in) :: c_sens(c_nanz, c_manz), c_dat(c_nanz), c_sigmaa(c_nanz)
+    complex(c_double_complex), intent(inout) :: c_par(c_manz)
+    real(c_double), intent(in) :: c_w(c_nanz), c_smatm(c_manz, 3)
+    real(c_double), value :: c_lam, c_eps, c_step
+    complex(c_double_complex), intent(out) :: c_dpar(c_manz)
+    real(c_double), intent(out) :: c_cgfac(c_manz), c_cgres(c_ncgmax + 1), c_bdpar
+    nanz = c_nanz; manz = c_manz
+    if (allocated(sens)) deallocate(sens)
+    if (allocated(smatm)) deallocate(smatm)
+    if (allocated(wmatd)) deallocate(wmatd)
+    if (allocated(wdfak)) deallocate(wdfak)
+    if (allocated(dat)) deallocate(dat)
+    if (allocated(sigmaa)) deallocate(sigmaa)
+    if (allocated(par)) deallocate(par)
+    if (allocated(dpar)) deallocate(dpar)
+    if (allocated(dpar2)) deallocate(dpar2)
+    if (allocated(w_ref_re)) deallocate(w_ref_re)
+    if (allocated(w_ref_im)) deallocate(w_ref_im)
+    allocate(sens(nanz, manz), smatm(manz, 3), wmatd(nanz), wdfak(nanz), dat(nanz), sigmaa(nanz), &
+         par(manz), dpar(manz), dpar2(manz), w_ref_re(max(manz, nanz + 1)), w_ref_im(max(manz, nanz + 1)))
+    sens = c_sens; smatm = c_smatm; wmatd = c_w; wdfak = 1; dat = c_dat; sigmaa = c_sigmaa
+    par = c_par; dpar = dcmplx(0d0); dpar2 = dcmplx(0d0); w_ref_re = 0d0; w_ref_im = 0d0
+    ltri = 0; lam = c_lam; nx = c_nx; nz = c_nz; ncgmax = c_ncgmax; eps = c_eps; step = c_step
+    llam = .false.; lfpi = .false.; ldc = .false.; ldiff = .false.; lprior = .false.
+    lw_ref = .false.; lphi0 = .false.; lverb = .false.
+    if (allocated(cgres)) call des_cjgmod(1, fetxt, errnr)
+    call con_cjgmod(1, fetxt, errnr)
+    if (errnr /= 0) then
+       ref_update = errnr
+       return
+    end if
+    cgres = 0d0; cgres2 = 0d0
+    errnr = 0
+    call update()
+    c_dpar = dpar
+    c_cgfac = cgfac
+    c_cgres = cgres(1:ncgmax + 1)
+    c_par = par
+    c_bdpar = bdpar
+    ref_update = errnr
+  end function ref_update
+
 end module crt_ref_driver

@@ -260,7 +260,36 @@ def rdati_cases():
     print("wrote ref_rdati.npz", {n: int(out[f"{n}__errnr"]) for n, *_ in RDATI_FILES})
 
 
-CASES = {"inversion": inversion_steps, "blam0": blam0_case, "rdati": rdati_cases}
+def update_case():
+    """The reference update (src/update.f90, llam = .false., ltri = 0: cgfac,
+    right-hand side J^H W (d - f) - lam R m, cjg -> cjggra, dpar scaled by
+    cgfac, par + dpar) on the small block mesh: J and the smoothness matrix
+    from the oracle (both pinned to the reference elsewhere), seeded data, m =
+    ln sigma, lam = 10, three (ncgmax, eps) settings."""
+    from oracle import cpu, ref
+    w = meshgen.small_block()
+    o = cpu.Oracle(w.mesh)
+    o.rwaven(w.strnr, w.vnr)
+    o.precal()
+    o.forward(w.sigma)
+    v, sa = o.volt(w.strnr, w.vnr, 1)
+    J = o.sens(1)
+    rng = np.random.default_rng(5)
+    N, M = J.shape
+    dat = sa + 0.01 * (rng.standard_normal(N) + 1j * rng.standard_normal(N))
+    wt = 1.0 / (0.02 + 1e-4 / np.abs(v)) ** 2
+    sm = cpu.bsmatmreg(o, 40, 20, 1.0, 1.0)
+    par = np.log(w.sigma)
+    out = dict(dat=dat, w=wt, lam=10.0, nx=40, nz=20)
+    for name, ncg, eps in (("a", 10, 0.0), ("b", 20, 1e-4), ("c", 30, 1e-6)):
+        dpar, cgfac, cgres, pnew, bd = ref.update(J, wt, dat, sa, par, sm, 10.0, 40, 20, ncg, eps, 1.0)
+        out.update({f"{name}_ncg": ncg, f"{name}_eps": eps, f"{name}_dpar": dpar, f"{name}_cgfac": cgfac,
+                    f"{name}_cgres": cgres, f"{name}_par": pnew, f"{name}_bdpar": bd})
+    np.savez_compressed(OUT / "ref_update.npz", **out)
+    print("wrote ref_update.npz")
+
+
+CASES = {"inversion": inversion_steps, "blam0": blam0_case, "rdati": rdati_cases, "update": update_case}
 
 
 def main():
@@ -284,6 +313,7 @@ def main():
     inversion_steps()
     blam0_case()
     rdati_cases()
+    update_case()
 
 
 if __name__ == "__main__":

@@ -147,3 +147,32 @@ def test_blam0_matches_reference(crtmod):
         assert abs(lam - d["lammax"][ref_mode]) <= 1e-8 * d["lammax"][ref_mode], (mode, lam)
     with pytest.raises(crtmod.CRTError):
         g.blam0(d["w"], 2)
+
+
+def test_cg_update_matches_reference_update(crtmod, oracle_lib):
+    """One model update on the device -- crt_cg_solve (cgfac, right-hand side,
+    10 CG iterations) and crt_update_step -- against the reference's update
+    (src/update.f90 -> cjg -> cjggra, golden vectors ref_update.npz) at the
+    sensitivity tolerance: the device's J and modelled data agree with the
+    reference's to ~1e-12, which 10 iterations amplify only mildly."""
+    from oracle import cpu
+    w = meshgen.small_block()
+    d = golden("update")
+    say("cg update vs reference update, small_block")
+    g = crtmod.CRT(w.mesh)
+    g.rwaven(w.strnr, w.vnr)
+    g.precal()
+    g.forward(w.sigma)
+    v, sa = g.bvolti(w.strnr, w.vnr)
+    g.sensitivities(1, to_host=False)
+    nx, nz = int(d["nx"]), int(d["nz"])
+    sm = g.smoothness(0, 1.0, 1.0, nx=nx, nz=nz)
+    par = np.log(w.sigma)
+    rm = cpu.rm_reg(sm, nx, par)
+    dpar, cgfac, cgres = g.cg_solve(d["w"], d["dat"], sa, float(d["lam"]), 0, sm, sm[:, 0], int(d["a_ncg"]),
+                                    eps=float(d["a_eps"]), nx=nx, rm=rm)
+    assert rel_err(cgfac, d["a_cgfac"]) < 1e-12
+    assert int(cgres[0]) == int(d["a_cgres"][0])
+    assert rel_err(dpar, d["a_dpar"]) < 1e-8
+    p, bd, _ = g.update_step(par, dpar, 1.0)
+    assert rel_err(p, d["a_par"]) < 1e-8 and abs(bd - d["a_bdpar"]) < 1e-8 * d["a_bdpar"]
