@@ -293,11 +293,17 @@ def blam0(sens, w, mode=0, alfx=1.0, alfz=1.0):
     return out.value
 
 
-def rdati(path, eanz, ldc=False, model=(5.0, 1e-3, 0.0, 0.0, 0.0, 0.0), nmax=100000):
-    """The reference rdati on a data file: (errnr, dict of arrays)."""
+def rdati(path, eanz, ldc=False, model=(5.0, 1e-3, 0.0, 0.0, 0.0, 0.0), nmax=100000, noise=None):
+    """The reference rdati on a data file: (errnr, dict of arrays).  noise =
+    (iseed, nstabw0, nstabm0, nstabpA1, nstabpB, nstabpA2, nstabp0) switches
+    on its synthetic noise; its listing files go to a temporary directory."""
+    import os
+    import tempfile
     L = lib()
-    L.ref_rdati.argtypes = [C.c_char_p, C.c_int, C.c_int, C.c_int, _d, C.c_int, _i, _i, _i, _z, _d, _d, _d, _i]
-    p = str(path).encode()
+    L.ref_rdati.argtypes = [C.c_char_p, C.c_int, C.c_int, C.c_int, _d, C.c_int, _i, _i, _i, _z, _d, _d, _d, _i,
+                            C.c_int, _d]
+    p = str(Path(path).resolve()).encode()
+    nz = np.asarray(noise if noise is not None else (1, 0, 0, 0, 0, 0, 0), np.float64)
     info = np.zeros(2, np.int32)
     st = np.zeros(nmax, np.int32)
     vn = np.zeros(nmax, np.int32)
@@ -306,8 +312,14 @@ def rdati(path, eanz, ldc=False, model=(5.0, 1e-3, 0.0, 0.0, 0.0, 0.0), nmax=100
     wr = np.zeros(nmax)
     wp = np.zeros(nmax)
     wc = np.zeros(nmax)
-    e = L.ref_rdati(p, len(p), int(eanz), int(bool(ldc)), np.asarray(model, np.float64), nmax, info, st, vn, dat,
-                    wr, wp, wc, wd)
+    cwd = os.getcwd()
+    with tempfile.TemporaryDirectory(dir="/tmp", prefix="crtn") as td:
+        os.chdir(td)
+        try:
+            e = L.ref_rdati(p, len(p), int(eanz), int(bool(ldc)), np.asarray(model, np.float64), nmax, info, st, vn,
+                            dat, wr, wp, wc, wd, int(noise is not None), nz)
+        finally:
+            os.chdir(cwd)
     n = int(info[0])
     return e, dict(strnr=st[:n], vnr=vn[:n], dat=dat[:n], wmatdr=wr[:n], wmatdp=wp[:n], wmatd_cri=wc[:n],
                    wdfak=wd[:n], lindiv=bool(info[1]))

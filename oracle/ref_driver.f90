@@ -641,13 +641,14 @@ contains
   !> error model stabw0, stabm0, stabpA1, stabpB, stabpA2, stabp0.  Outputs are
   !> sized nanz_max; c_info returns nanz, lindiv.
   integer(c_int) function ref_rdati(c_path, c_plen, c_eanz, c_ldc, c_model, c_nmax, c_info, c_strnr, &
-       c_vnr, c_dat, c_wmatdr, c_wmatdp, c_wmatdcri, c_wdfak) bind(C, name='ref_rdati')
+       c_vnr, c_dat, c_wmatdr, c_wmatdp, c_wmatdcri, c_wdfak, c_lnse, c_noise) bind(C, name='ref_rdati')
     use invmod, only: dat, wmatd, wdfak, wmatd2
     use konvmod, only: lverb
     use femmod, only: ldc
-    integer(c_int), value :: c_plen, c_eanz, c_ldc, c_nmax
+    use alloci, only: rnd_r, rnd_p
+    integer(c_int), value :: c_plen, c_eanz, c_ldc, c_nmax, c_lnse
     character(kind=c_char), intent(in) :: c_path(c_plen)
-    real(c_double), intent(in) :: c_model(6)
+    real(c_double), intent(in) :: c_model(6), c_noise(7)
     integer(c_int), intent(out) :: c_info(2), c_strnr(c_nmax), c_vnr(c_nmax), c_wdfak(c_nmax)
     complex(c_double_complex), intent(out) :: c_dat(c_nmax)
     real(c_double), intent(out) :: c_wmatdr(c_nmax), c_wmatdp(c_nmax), c_wmatdcri(c_nmax)
@@ -659,7 +660,14 @@ contains
     end do
     eanz = c_eanz
     ldc = c_ldc /= 0
-    lnse = .false.; lverb = .false.
+    ! synthetic noise (c_noise: iseed, nstabw0, nstabm0, nstabpA1, nstabpB,
+    ! nstabpA2, nstabp0); rdati writes its listing files to the working
+    ! directory
+    lnse = c_lnse /= 0; lverb = .false.
+    iseed = nint(c_noise(1)); nstabw0 = c_noise(2); nstabm0 = c_noise(3); nstabpA1 = c_noise(4)
+    nstabpB = c_noise(5); nstabpA2 = c_noise(6); nstabp0 = c_noise(7)
+    if (allocated(rnd_r)) deallocate(rnd_r)
+    if (allocated(rnd_p)) deallocate(rnd_p)
     stabw0 = c_model(1); stabm0 = c_model(2); stabpA1 = c_model(3); stabpB = c_model(4)
     stabpA2 = c_model(5); stabp0 = c_model(6)
     if (allocated(strnr)) deallocate(strnr)

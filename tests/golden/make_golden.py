@@ -241,6 +241,10 @@ RDATI_FILES = [
 ]
 
 
+# synthetic noise models: iseed, nstabw0, nstabm0, nstabpA1, nstabpB, nstabpA2, nstabp0
+RDATI_NOISE = [(1, 2.0, 1e-3, 0.3, 0.5, 1.5, 0.2), (12345, 5.0, 0.0, 0.0, 0.0, 0.0, 0.0)]
+
+
 def rdati_cases():
     """rdati on the data files above (written to tests/golden/rdati/), with
     the error model RDATI_MODEL; error numbers or the arrays rdati fills."""
@@ -256,6 +260,14 @@ def rdati_cases():
         if e == 0:
             for k, v in arr.items():
                 out[f"{name}__{k}"] = np.asarray(v)
+    for k, (name, eanz, ldc, _) in enumerate([c for c in RDATI_FILES if c[0] in ("crt_model", "abmn_model",
+                                                                                  "dc_model")]):
+        for j, nz in enumerate(RDATI_NOISE):
+            e, arr = ref.rdati(d / f"{name}.dat", eanz, ldc, RDATI_MODEL, noise=nz)
+            key = f"{name}__noise{j}"
+            out[f"{key}__errnr"] = np.int32(e)
+            for kk, v in arr.items():
+                out[f"{key}__{kk}"] = np.asarray(v)
     np.savez_compressed(OUT / "ref_rdati.npz", **out)
     print("wrote ref_rdati.npz", {n: int(out[f"{n}__errnr"]) for n, *_ in RDATI_FILES})
 

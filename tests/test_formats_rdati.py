@@ -48,3 +48,28 @@ def test_read_data_error_model_closed_form(tmp_path):
     np.testing.assert_allclose(d.wmatdp, 1 / sp ** 2, rtol=1e-15)
     np.testing.assert_allclose(d.wmatd_cri, 1 / (sr ** 2 + sp ** 2), rtol=1e-15)
     np.testing.assert_allclose(d.dat, -np.log(R) - 1j * phi / 1e3, rtol=1e-15)
+
+
+NOISE = [(1, 2.0, 1e-3, 0.3, 0.5, 1.5, 0.2), (12345, 5.0, 0.0, 0.0, 0.0, 0.0, 0.0)]
+
+
+@pytest.mark.parametrize("name,eanz,ldc", [("crt_model", 8, False), ("abmn_model", 8, False), ("dc_model", 6, True)])
+@pytest.mark.parametrize("j", [0, 1])
+def test_read_data_synthetic_noise_matches_rdati(name, eanz, ldc, j):
+    """rdati's synthetic noise: the reference's generator (make_noise) seeded
+    with iseed and -iseed, bit for bit."""
+    g = golden("rdati")
+    nz = NOISE[j]
+    key = f"{name}__noise{j}"
+    assert int(g[f"{key}__errnr"]) == 0
+    d = F.read_data(HERE / "rdati" / f"{name}.dat", eanz, ldc, **MODEL, noise=F.NoiseModel(int(nz[0]), *nz[1:]))
+    for k in ("dat", "wmatdr", "wmatdp", "wmatd_cri"):
+        np.testing.assert_array_equal(getattr(d, k), g[f"{key}__{k}"], err_msg=k)
+    assert not np.array_equal(d.dat, F.read_data(HERE / "rdati" / f"{name}.dat", eanz, ldc, **MODEL).dat)
+
+
+def test_ref_random_gauss_moments():
+    # the generator's deviates: mean 0, variance 1 (50,000 draws)
+    g = F.RefRandom(7)
+    x = np.array([g.gauss() for _ in range(50000)])
+    assert abs(x.mean()) < 0.02 and abs(x.var() - 1.0) < 0.03

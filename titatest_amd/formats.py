@@ -352,10 +352,80 @@ class DataSet:
     lindiv: bool           # individual errors in the file
 
 
+class RefRandom:
+    """The reference's pseudo-random generator (src/make_noise.f90:171-261):
+    three linear congruential sequences, the first two combined into 97
+    shuffled uniforms and the third choosing among them (the classic
+    three-generator shuffle), and the polar method for Gaussian deviates.  The
+    scale factors 1/M1 and 1/M2 are the reference's single-precision
+    constants."""
+
+    M1, IA1, IC1 = 259200, 7141, 54773
+    M2, IA2, IC2 = 134456, 8121, 28441
+    M3, IA3, IC3 = 243000, 4561, 51349
+    N_RND = 97
+    RM1 = float(np.float32(1.0) / np.float32(259200))
+    RM2 = float(np.float32(1.0) / np.float32(134456))
+
+    @staticmethod
+    def _mod(a: int, p: int) -> int:  # Fortran MOD: the sign of a
+        return int(math.fmod(a, p))
+
+    def __init__(self, iseed: int = 1):
+        m = self._mod
+        self.ix1 = m(self.IC1 + iseed, self.M1)
+        self.ix1 = m(self.IA1 * self.ix1 + self.IC1, self.M1)
+        self.ix2 = m(self.ix1, self.M2)
+        self.ix1 = m(self.IA1 * self.ix1 + self.IC1, self.M1)
+        self.ix3 = m(self.ix1, self.M3)
+        self.rnd = [0.0] * self.N_RND
+        for i in range(self.N_RND):
+            self.ix1 = m(self.IA1 * self.ix1 + self.IC1, self.M1)
+            self.ix2 = m(self.IA2 * self.ix2 + self.IC2, self.M2)
+            self.rnd[i] = (float(self.ix1) + float(self.ix2) * self.RM2) * self.RM1
+
+    def draw(self) -> float:
+        m = self._mod
+        while True:
+            self.ix1 = m(self.IA1 * self.ix1 + self.IC1, self.M1)
+            self.ix2 = m(self.IA2 * self.ix2 + self.IC2, self.M2)
+            self.ix3 = m(self.IA3 * self.ix3 + self.IC3, self.M3)
+            a = self.N_RND * self.ix3  # Fortran integer division truncates toward zero
+            i = 1 + (a // self.M3 if a >= 0 else -((-a) // self.M3))
+            if i > self.N_RND or i < 1:
+                continue
+            v = self.rnd[i - 1]
+            self.rnd[i - 1] = (float(self.ix1) + float(self.ix2) * self.RM2) * self.RM1
+            return v
+
+    def gauss(self) -> float:
+        while True:
+            x = 2.0 * self.draw() - 1.0
+            y = 2.0 * self.draw() - 1.0
+            z = x * x + y * y
+            if z <= 1.0:
+                break
+        return x * math.sqrt(-2.0 * math.log(z) / z)
+
+
+@dataclasses.dataclass
+class NoiseModel:
+    """rdati's synthetic noise (lnse; src/rall.f90:480-510): seed and the
+    noise model's relative and absolute magnitude errors, phase parameters."""
+    iseed: int = 1
+    nstabw0: float = 0.0
+    nstabm0: float = 0.0
+    nstabpA1: float = 0.0
+    nstabpB: float = 0.0
+    nstabpA2: float = 0.0
+    nstabp0: float = 0.0
+
+
 def read_data(path, eanz: int, ldc: bool = False, stabw0: float = 5.0, stabm0: float = 1e-3,
-              stabpA1: float = 0.0, stabpB: float = 0.0, stabpA2: float = 0.0, stabp0: float = 0.0) -> DataSet:
+              stabpA1: float = 0.0, stabpB: float = 0.0, stabpA2: float = 0.0, stabp0: float = 0.0,
+              noise: NoiseModel | None = None) -> DataSet:
     """Read a measured data file (volt.dat layout) with the semantics of
-    src/rdati.f90:84-363 (its synthetic-noise option excepted).
+    src/rdati.f90:84-363.
 
     * First record ``nanz [lindiv]``; without a logical second value the
       error model applies (:84-88).  The first datum's first value tells the
@@ -369,6 +439,12 @@ def read_data(path, eanz: int, ldc: bool = False, stabw0: float = 5.0, stabm0: f
       only, as there), 46 electrode out of range, 88 non-positive error, 94
       non-positive |R|, 95 |phase| > 1000 pi mrad, 2 premature end of file,
       3 unreadable value.
+    * ``noise`` adds the reference's synthetic noise (lnse, :110-135 and
+      :256-292): Gaussian deviates from the reference's generator seeded with
+      iseed (magnitudes) and -iseed (phases), scaled by eps_R = nstabw0/100 |R| +
+      nstabm0 and eps_phi = nstabpA1 eps_R^nstabpB + nstabpA2/100 |phase| +
+      nstabp0, added to the data of error-model files after their weights
+      are formed.  The reference's listing files are not written.
     """
     pi = math.acos(-1.0)
     rec = _Records(Path(path).read_text())
@@ -393,6 +469,12 @@ def read_data(path, eanz: int, ldc: bool = False, stabw0: float = 5.0, stabm0: f
         wcri = np.zeros(nanz)
         ncol = (2 if crtf else 4) + (1 if ldc else 2) + ((1 if ldc else 2) if lindiv else 0)
         pha = 0.0
+        if noise is not None:
+            g = RefRandom(noise.iseed)
+            rnd_r = [g.gauss() for _ in range(nanz)]
+            if not ldc:
+                g = RefRandom(-noise.iseed)
+                rnd_p = [g.gauss() for _ in range(nanz)]
         for i in range(nanz):
             v = rec.read(ncol)
             if crtf:
@@ -430,6 +512,13 @@ def read_data(path, eanz: int, ldc: bool = False, stabw0: float = 5.0, stabm0: f
                 if bet <= 0.0:
                     raise FormatError(94, f"non-positive resistance of datum {i + 1}")
                 stabw = 1e-2 * stabw0 + stabm0 / bet
+                if noise is not None:
+                    eps_r = 1e-2 * noise.nstabw0 * bet + noise.nstabm0
+                    if not ldc:
+                        eps_p = (noise.nstabpA1 * eps_r ** noise.nstabpB + 1e-2 * noise.nstabpA2 * abs(pha)
+                                 + noise.nstabp0)
+                        pha = pha + rnd_p[i] * eps_p
+                    bet = bet + rnd_r[i] * eps_r
             if bet <= 0.0:
                 raise FormatError(94, f"non-positive resistance of datum {i + 1}")
             if ldc:
