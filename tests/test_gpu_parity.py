@@ -302,6 +302,41 @@ def test_triangles_large_band_match_oracle(crtmod, oracle_lib):
         assert rel_err(kpot[:, l, k], o.kpot[:, l, k]) < VTOL, (k, l)
 
 
+def test_widest_band_matches_oracle(crtmod, oracle_lib):
+    """mb = 124, the widest band the factor kernel takes: the 512-thread form
+    with three panel rows per lane (W = 132), on a 33x122-quadrilateral mesh.
+    A sample of (k, l) systems against the oracle, k = 0, l = 0 against the
+    long-double solution, and mb = 125 refused with 1003."""
+    w = meshgen.large_grf(nx_side=4, n_uniform=24, nz=122, eanz=24)
+    say("widest band mb=", w.mesh.mb, "n=", w.mesh.sanz)
+    assert w.mesh.mb == 124
+    g = crtmod.CRT(w.mesh)
+    g.rwaven(w.strnr, w.vnr)
+    g.precal()
+    (kpot,) = g.forward(w.sigma, want_kpot=True)
+    o = oracle_lib.Oracle(w.mesh)
+    o.rwaven(w.strnr, w.vnr)
+    o.precal()
+    E = w.mesh.eanz
+    for s in (0, 5 * E + 13, (g.K - 1) * E + E - 1):
+        o.forward(w.sigma, sys_range=(s, s + 1))
+        k, l = divmod(s, E)
+        assert rel_err(kpot[:, l, k], o.kpot[:, l, k]) < PTOL, (k, l)
+    kl = o.solve_ld(w.sigma, 0, 0)
+    e_hip, e_orc = rel_err(kpot[:, 0, 0], kl), rel_err(o.kpot[:, 0, 0], kl)
+    say("k=0 l=0 distance from the long-double solution: hip", e_hip, "oracle", e_orc)
+    assert e_hip < 3 * e_orc, (e_hip, e_orc)
+    del g
+    w2 = meshgen.large_grf(nx_side=4, n_uniform=24, nz=123, eanz=24)
+    assert w2.mesh.mb == 125
+    g2 = crtmod.CRT(w2.mesh)
+    g2.rwaven(w2.strnr, w2.vnr)
+    g2.precal()
+    with pytest.raises(crtmod.CRTError) as e:
+        g2.forward(w2.sigma)
+    assert e.value.code == 1003
+
+
 def test_fortran_host_crmod_fixture(crtmod, tmp_path):
     """The Fortran host (iso_c_binding over the C-ABI) on the reference's
     tests/minimal tomodir writes the reference's voltages."""
