@@ -115,6 +115,36 @@ def test_sensitivities_both_modes(small_pair):
     assert rel_err(s1, o.sens(1)) < STOL
 
 
+@pytest.mark.parametrize("nmeas", [900, 2200])
+def test_sensitivities_many_distinct_pairs(crtmod, oracle_lib, nmeas):
+    """Random quadrupoles on 48 electrodes.  900 measurements hold 738 distinct
+    current and 728 potential pairs: the pair tables take 96,896 bytes of LDS,
+    one workgroup per CU, the pair kernel's 512-thread form.  2,200 hold 1,437
+    and 1,397 (184,448 bytes, above the CU's LDS): the per-measurement
+    kernel.  Both modes against the oracle."""
+    w = meshgen.large_grf(nx_side=4, n_uniform=47, nz=10, eanz=48)
+    E = w.mesh.eanz
+    rng = np.random.default_rng(nmeas)
+    q = np.array([rng.choice(E, 4, replace=False) + 1 for _ in range(nmeas)])
+    st, vn = q[:, 0] * 10000 + q[:, 1], q[:, 2] * 10000 + q[:, 3]
+    say("sens many pairs: measurements", nmeas)
+    g = crtmod.CRT(w.mesh)
+    g.rwaven(st, vn)
+    g.precal()
+    g.forward(w.sigma)
+    o = oracle_lib.Oracle(w.mesh)
+    o.rwaven(st, vn)
+    o.precal()
+    o.forward(w.sigma)
+    v, _ = g.bvolt(st, vn)
+    vo, _ = o.volt(st, vn, 0)
+    check_volt(v, vo)
+    assert rel_err(g.bsens(), o.sens(0)) < STOL
+    g.bvolti(st, vn)
+    o.volt(st, vn, 1)
+    assert rel_err(g.bsensi(), o.sens(1)) < STOL
+
+
 def test_adjoint_gradient_equals_jhwr(small_pair, oracle_lib):
     w, g, o = small_pair
     say("adjoint small_block")
