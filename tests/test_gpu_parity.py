@@ -205,6 +205,26 @@ def test_dirichlet_factor_reuse(crtmod, oracle_lib):
     check_volt(v, ge["volt"])
 
 
+def test_dirichlet_factor_reuse_widest_band(crtmod, oracle_lib):
+    """The shared-factor forward at mb = 124: the column-band copy, forward
+    sweeps and back substitutions in their 4-row group form."""
+    w = meshgen.large_grf(nx_side=4, n_uniform=24, nz=122, eanz=24, side_type=13)
+    say("dirichlet widest band mb=", w.mesh.mb)
+    assert w.mesh.mb == 124
+    g = crtmod.CRT(w.mesh)
+    g.rwaven(w.strnr, w.vnr)
+    assert g.precal() == (0, 1)
+    (kpot,) = g.forward(w.sigma, want_kpot=True)
+    o = oracle_lib.Oracle(w.mesh)
+    o.rwaven(w.strnr, w.vnr)
+    o.precal()
+    o.forward(w.sigma)
+    assert rel_err(kpot, o.kpot) < PTOL
+    v, _ = g.bvolt(w.strnr, w.vnr)
+    vo, _ = o.volt(w.strnr, w.vnr, 0)
+    check_volt(v, vo)
+
+
 @pytest.mark.parametrize("case", FIXTURE_CASES, ids=[c[0] for c in FIXTURE_CASES])
 def test_reference_fixtures(crtmod, case):
     name, swrtr, lsink, nsink = case
