@@ -162,6 +162,33 @@ def test_adjoint_gradient_equals_jhwr(small_pair, oracle_lib):
         assert rel_err(ga, go) < STOL
 
 
+def test_adjoint_gradient_more_electrodes_than_the_mfma_gemm(crtmod, oracle_lib):
+    """160 electrodes (the matrix-core GEMM of the adjoint takes up to 128):
+    the CUDA-core GEMM with its 80 KB coefficient block in LDS, against
+    J^H (w o r); J itself against the oracle."""
+    w = meshgen.large_grf(nx_side=4, n_uniform=160, nz=8, eanz=160)
+    say("adjoint E=160 n=", w.mesh.sanz, "N=", len(w.strnr))
+    g = crtmod.CRT(w.mesh)
+    g.rwaven(w.strnr, w.vnr)
+    g.precal()
+    g.forward(w.sigma)
+    o = oracle_lib.Oracle(w.mesh)
+    o.rwaven(w.strnr, w.vnr)
+    o.precal()
+    o.forward(w.sigma)
+    N = len(w.strnr)
+    rng = np.random.default_rng(11)
+    wgt = rng.uniform(0.5, 2.0, N)
+    r = rng.standard_normal(N) + 1j * rng.standard_normal(N)
+    for mode in (0, 1):
+        g.voltages(w.strnr, w.vnr, mode)
+        J = g.sensitivities(mode)
+        o.volt(w.strnr, w.vnr, mode)
+        assert rel_err(J, o.sens(mode)) < STOL
+        ga = g.adjoint_grad(wgt, r, mode=mode)
+        assert rel_err(ga, J.conj().T @ (wgt * r)) < STOL
+
+
 def test_cg_solve_matches_oracle(small_pair, oracle_lib):
     w, g, o = small_pair
     say("cg small_block")

@@ -7,6 +7,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <numeric>
 #include <string>
 #include <vector>
@@ -55,7 +56,6 @@ struct crt_handle {
   std::vector<int> pairs_key;
   std::vector<int2> sens_cp, sens_pp, sens_mp;
   std::vector<int> sens_ord;
-  size_t sens_lds_set = 0;  // dynamic LDS limit set for the pair kernel
   // element-to-parameter map (modelmod::mnr) for bpar/bsigma; empty = identity
   int manz_map = 0;
   std::vector<int> mnr_host;
@@ -73,6 +73,7 @@ struct crt_handle {
   hipEvent_t ev_fac = nullptr, ev_join = nullptr;
   int ncu = 0;
   size_t lds_max = 0;
+  int dev = 0;  // the device current at crt_create
   hipError_t hip_err = hipSuccess;
   std::map<std::string, void*> buf;
   std::map<std::string, size_t> bufsz;
@@ -517,6 +518,7 @@ int crt_create(crt_handle** out, const crt_mesh* m) {
     delete h;
     return kErrHip;
   }
+  h->dev = dev;
   h->ncu = prop.multiProcessorCount;
   h->lds_max = prop.sharedMemPerBlock;
   if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
@@ -785,6 +787,28 @@ int crt_get_precal(crt_handle* h, double* elbg, double* relbg, double* kg) {
   return 0;
 }
 
+// The dynamic-LDS limit of a kernel (hipFuncAttributeMaxDynamicSharedMemorySize)
+// belongs to the kernel on its device, not to a handle: handles in flight on
+// several threads would lower it under one another's launches if each set its
+// own size.  It is raised once per device and kernel to all a workgroup can
+// take (the CU's LDS less the kernel's static LDS) and left there.
+// *dyn (optional) receives that dynamic limit.
+static int open_lds(crt_handle* h, const void* kern, size_t* dyn = nullptr) {
+  static std::mutex mu;
+  static std::map<std::pair<int, const void*>, size_t> open;
+  std::lock_guard<std::mutex> lock(mu);
+  auto it = open.find({h->dev, kern});
+  if (it == open.end()) {
+    hipFuncAttributes fa;
+    HIPCHK(hipFuncGetAttributes(&fa, kern));
+    const size_t lim = h->lds_max - std::min(h->lds_max, (size_t)fa.sharedSizeBytes);
+    HIPCHK(hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lim));
+    it = open.emplace(std::make_pair(h->dev, kern), lim).first;
+  }
+  if (dyn) *dyn = it->second;
+  return 0;
+}
+
 static int run_forward(crt_handle* h) {
   const int n = h->n, mb = h->mb, E = h->E, K = h->K;
   const size_t lds = crt_factor_lds_bytes(mb);
@@ -811,7 +835,7 @@ static int run_forward(crt_handle* h) {
     else if (rows == 3 && maxt <= 2) kern = crt_factor_kernel<3, 2, 512>;
   }
   if (!kern) return kErrBand;
-  hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  if (int rc = open_lds(h, reinterpret_cast<const void*>(kern))) return rc;
   int per_cu = 0;
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, nt, lds) != hipSuccess || per_cu < 1) per_cu = 1;
   const int nsys = K * E;
@@ -1256,23 +1280,27 @@ static int run_sens(crt_handle* h, int32_t mode, const double2* sigma) {
   Q.ncp = (int)h->sens_cp.size();
   Q.npp = (int)h->sens_pp.size();
   const size_t lds_pairs = (size_t)(4 * h->E + 4 * Q.ncp + 4 * Q.npp) * 16;
+  // the pair form: 256 threads while its LDS (dynamic and static) leaves room
+  // for two workgroups per CU, else 512 in one; past the CU's LDS the
+  // per-measurement kernel, which holds one row of potentials per element
+  // node and electrode
+  size_t dyn_pairs = 0, dyn = 0;
+  if (int rc = open_lds(h, reinterpret_cast<const void*>(crt_sens_pairs_kernel), &dyn_pairs)) return rc;
+  const bool pairs = lds_pairs <= dyn_pairs;
+  const size_t lds = pairs ? lds_pairs : (size_t)4 * h->E * 16;
+  if (!pairs) {
+    if (int rc = open_lds(h, reinterpret_cast<const void*>(crt_sens_kernel), &dyn)) return rc;
+    if (lds > dyn) return kErrArg;
+  }
   tstart(h, "sens");
-  // the pair form: 256 threads while its LDS leaves room for two workgroups
-  // per CU, else 512 in one
-  if (lds_pairs <= h->lds_max) {
-    const int nt = 2 * lds_pairs <= h->lds_max ? 256 : 512;
+  if (pairs) {
+    const int nt = 2 * (lds + h->lds_max - dyn_pairs) <= h->lds_max ? 256 : 512;
     Q.cp = h->at<int2>("sens_cp");
     Q.pp = h->at<int2>("sens_pp");
     Q.mp = h->at<int2>("sens_mp");
     Q.ord = h->at<int>("sens_ord");
-    if (lds_pairs > h->sens_lds_set) {
-      HIPCHK(hipFuncSetAttribute(reinterpret_cast<const void*>(crt_sens_pairs_kernel),
-                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_pairs));
-      h->sens_lds_set = lds_pairs;
-    }
-    hipLaunchKernelGGL(crt_sens_pairs_kernel, dim3(h->elanz), dim3(nt), lds_pairs, h->stream, S, Q);
+    hipLaunchKernelGGL(crt_sens_pairs_kernel, dim3(h->elanz), dim3(nt), lds, h->stream, S, Q);
   } else {
-    const size_t lds = (size_t)4 * h->E * 16;
     hipLaunchKernelGGL(crt_sens_kernel, dim3(h->elanz), dim3(256), lds, h->stream, S);
   }
   tstop(h);
@@ -1304,6 +1332,14 @@ int crt_sensitivities_dc(crt_handle* h, double* sensdc) {
 
 static int run_adjoint(crt_handle* h, int32_t mode, const double* w_dev, const double2* r_dev, double2* g_dev) {
   const int n = h->n, E = h->E, K = h->K, N = h->nanz;
+  // more electrodes than the matrix-core GEMM takes: E x kAdjColsHost entries
+  // of the coefficient matrix in LDS
+  const size_t lds = (size_t)E * kAdjColsHost * 16;
+  if (E > 16 * kAdjTilesHost) {
+    size_t dyn = 0;
+    if (int rc = open_lds(h, reinterpret_cast<const void*>(crt_adj_gemm_kernel), &dyn)) return rc;
+    if (lds > dyn) return kErrArg;
+  }
   double2* gamma = h->get<double2>("adj_gamma", N);
   double2* C = h->get<double2>("adj_C", (size_t)E * E);
   double2* Y = h->get<double2>("adj_Y", (size_t)K * n * E);
@@ -1339,7 +1375,6 @@ static int run_adjoint(crt_handle* h, int32_t mode, const double* w_dev, const d
                      (const int*)h->at<int>("strnr"), (const int*)h->at<int>("vnr"), (const int*)dqp,
                      (const int*)dql, (const double2*)gamma, C);
   tstop(h);
-  const size_t lds = (size_t)E * kAdjColsHost * 16;
   tstart(h, "adj_gemm");
   if (E <= 16 * kAdjTilesHost)
     hipLaunchKernelGGL(crt_adj_gemm_mfma_kernel, dim3((n + 63) / 64, K), dim3(256), 0, h->stream,
