@@ -189,6 +189,38 @@ def test_adjoint_gradient_more_electrodes_than_the_mfma_gemm(crtmod, oracle_lib)
         assert rel_err(ga, J.conj().T @ (wgt * r)) < STOL
 
 
+def test_zero_conductivity_around_a_node_is_error_27(crtmod, oracle_lib):
+    """scalab's zero diagonal (src/scalab.f90:56-63): conductivity 0 on the four
+    elements around one interior node leaves |a_ii| = 0 in every (k, l)
+    system.  Code 27 from the device as from the oracle, a (k, l) in range from
+    crt_error_info, and the handle recovers with a valid forward."""
+    w = meshgen.small_block()
+    say("error 27: zero conductivity around one node")
+    m = w.mesh
+    nrel = np.asarray(m.nrel)
+    node = nrel[m.elanz // 2, 2]
+    around = [e for e in range(m.elanz) if node in nrel[e, :4]]
+    sig = w.sigma.copy()
+    sig[around] = 0
+    g = crtmod.CRT(m)
+    g.rwaven(w.strnr, w.vnr)
+    g.precal()
+    o = oracle_lib.Oracle(m)
+    o.rwaven(w.strnr, w.vnr)
+    o.precal()
+    with pytest.raises(oracle_lib.OracleError) as eo:
+        o.forward(sig)
+    assert eo.value.code == 27
+    with pytest.raises(crtmod.CRTError) as ex:
+        g.forward(sig)
+    assert ex.value.code == 27
+    code, k, l = g.error_info()
+    assert code == 27 and 1 <= k <= g.K and 1 <= l <= m.eanz
+    (kpot,) = g.forward(w.sigma, want_kpot=True)
+    o.forward(w.sigma)
+    assert rel_err(kpot, o.kpot) < VTOL
+
+
 def test_cg_solve_matches_oracle(small_pair, oracle_lib):
     w, g, o = small_pair
     say("cg small_block")
