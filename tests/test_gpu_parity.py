@@ -371,6 +371,25 @@ def test_large_mesh_systems_match_oracle(crtmod, oracle_lib):
     check_volt(v, vo)
 
 
+def test_full_size_dirichlet_reciprocity_and_euler_rule(crtmod):
+    """Size-independent properties at the full 240x80 size (19,521 nodes,
+    3,090 quadrupoles, 19,200 cells), Dirichlet sides and bottom: A is
+    symmetric and independent of the source, so swapping the current and
+    potential dipoles leaves every voltage unchanged; and A is homogeneous of
+    degree 1 in sigma, so sum_e sigma_e dV/dsigma_e = -V for every row of J."""
+    w = meshgen.large_grf(side_type=13)
+    say("full-size dirichlet: reciprocity and Euler rule")
+    g = crtmod.CRT(w.mesh)
+    g.rwaven(w.strnr, w.vnr)
+    g.precal()
+    g.forward(w.sigma)
+    v1, _ = g.bvolt(w.vnr, w.strnr)
+    v, _ = g.bvolt(w.strnr, w.vnr)
+    assert rel_err(v1, v) < 1e-10
+    J = g.bsens()
+    assert rel_err(J @ w.sigma, -v) < STOL
+
+
 def test_batching_does_not_change_the_potentials(crtmod):
     """The 240x80 workload's forward in its default form (the full rounds of
     the persistent factor kernel, then the last round with the first
