@@ -390,6 +390,25 @@ def test_full_size_dirichlet_reciprocity_and_euler_rule(crtmod):
     assert rel_err(J @ w.sigma, -v) < STOL
 
 
+def test_full_size_mixed_scaling_is_exact(crtmod):
+    """The headline 240x80 workload (mixed sides and bottom, 1,344 factored
+    systems): A is linear in sigma, boundary terms included, so 4 sigma gives
+    4 A, scalab's factors f / 2 (sqrt(4x) = 2 sqrt(x) exactly in IEEE
+    arithmetic), the same scaled matrix and factor bit for bit, and potentials
+    and voltages exactly a quarter of sigma's."""
+    w = meshgen.large_grf()
+    say("full-size mixed: sigma and 4 sigma")
+    g = crtmod.CRT(w.mesh)
+    g.rwaven(w.strnr, w.vnr)
+    g.precal()
+    (k1,) = g.forward(w.sigma, want_kpot=True)
+    v1, _ = g.bvolt(w.strnr, w.vnr)
+    (k4,) = g.forward(4.0 * w.sigma, want_kpot=True)
+    v4, _ = g.bvolt(w.strnr, w.vnr)
+    np.testing.assert_array_equal(4.0 * k4, k1)
+    np.testing.assert_array_equal(4.0 * v4, v1)
+
+
 def test_batching_does_not_change_the_potentials(crtmod):
     """The 240x80 workload's forward in its default form (the full rounds of
     the persistent factor kernel, then the last round with the first
