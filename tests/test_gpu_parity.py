@@ -419,6 +419,26 @@ def test_full_size_mixed_scaling_is_exact(crtmod, workload):
     np.testing.assert_array_equal(4.0 * v4, v1)
 
 
+def test_full_size_adjoint_gradient_equals_jhwr(crtmod):
+    """The headline workload's adjoint gradient (no J formed) against J^H (w o r)
+    from the explicit 3,090 x 19,200 Jacobian, both modes."""
+    w = meshgen.large_grf()
+    say("full-size adjoint gradient vs explicit J")
+    g = crtmod.CRT(w.mesh)
+    g.rwaven(w.strnr, w.vnr)
+    g.precal()
+    g.forward(w.sigma)
+    N = len(w.strnr)
+    rng = np.random.default_rng(5)
+    wgt = rng.uniform(0.5, 2.0, N)
+    r = rng.standard_normal(N) + 1j * rng.standard_normal(N)
+    for mode in (0, 1):
+        g.voltages(w.strnr, w.vnr, mode)
+        J = g.sensitivities(mode)
+        ga = g.adjoint_grad(wgt, r, mode=mode)
+        assert rel_err(ga, J.conj().T @ (wgt * r)) < STOL
+
+
 def test_batching_does_not_change_the_potentials(crtmod):
     """The 240x80 workload's forward in its default form (the full rounds of
     the persistent factor kernel, then the last round with the first
