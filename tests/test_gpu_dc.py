@@ -112,3 +112,21 @@ def test_dc_state_guards(crtmod):
     with pytest.raises(crtmod.CRTError) as ex:
         g.bsensi()
     assert ex.value.code == 1002
+
+
+def test_dc_full_size_scaling_is_exact(crtmod):
+    """The DC twin on the full 240x80 mesh (mixed sides and bottom): 4 sigma
+    gives scaldc's factors f / 2 exactly, the same scaled matrix and factor,
+    and potentials exactly a quarter (below 1e-250, where sums of underflowing
+    products lose bits, to 1e-12 relative)."""
+    w = meshgen.large_grf()
+    say("DC full size: sigma and 4 sigma")
+    g = crtmod.CRT(w.mesh)
+    g.rwaven(w.strnr, w.vnr)
+    g.precal()
+    (k1,) = g.forward_dc(w.sigma, want_kpot=True)
+    (k4,) = g.forward_dc(4.0 * w.sigma, want_kpot=True)
+    a, b = (4.0 * k4).ravel(), k1.ravel()
+    big = np.abs(b) >= 1e-250
+    np.testing.assert_array_equal(a[big], b[big])
+    assert np.all(np.abs(a - b)[~big] <= 1e-12 * np.abs(b[~big]) + 1e-300)
