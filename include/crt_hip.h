@@ -132,12 +132,20 @@ int crt_jhw_r(crt_handle *h, const double *w, const double *resid, double *grad)
 /* The regularisation's smoothness operator, built on the host once per
  * inversion (bsmatm, src/bsmatm_mod.f90): ltri = 0 the rectangular stencil of
  * bsmatmreg (:190-362; elements numbered row by row, nx x nz; smatm manz x 3),
- * ltri = 1 or 2 the triangular operator of bsmatmtri (:364-461, no decoupled
- * edges; smatm manz x (maxnodes+1)) with the edge neighbours of bnachbar
+ * ltri = 1 or 2 the triangular operator of bsmatmtri (:364-461, with the
+ * decoupled edges of crt_set_decouplings; smatm manz x (maxnodes+1)) with the edge neighbours of bnachbar
  * (src/bnachbar.f90) in nachbar (manz x (maxnodes+1), may be NULL).  Column
  * major, the reference's layout; the input crt_cg_solve takes. */
 int crt_smoothness(crt_handle *h, int32_t ltri, double alfx, double alfz, int32_t nx, int32_t nz,
                    double *smatm, int32_t *nachbar);
+
+/* Decoupled edges for the triangular operator (ltri 1/2) of the next
+ * crt_smoothness calls: edecoup decanz x 2 column major (1-based element
+ * pairs), edecstr decanz weight factors, as rall reads decouplings.dat
+ * (src/rall.f90:648-665) and bsmatmtri applies them (src/bsmatm_mod.f90:434-445:
+ * the weight of the edge between the pair, either order, times the factor).
+ * decanz = 0 clears them.  Replaces the decoupling part of those two sites. */
+int crt_set_decouplings(crt_handle *h, int32_t decanz, const int32_t *edecoup, const double *edecstr);
 
 /* One model update's linear solve, src/update.f90:179-307 with cjggra
  * (src/cg_mod.f90:426-518): preconditioner cgfac, scaled RHS, then Fletcher-Reeves

@@ -67,7 +67,7 @@ def lib():
         L.orc_rm_reg.argtypes = [C.c_int, C.c_int, _d, _z, _z]
         L.orc_bnachbar.argtypes = [P, _i]
         L.orc_bnachbar.restype = None
-        L.orc_bsmatmtri.argtypes = [P, C.c_double, C.c_double, _i, _d]
+        L.orc_bsmatmtri.argtypes = [P, C.c_double, C.c_double, _i, C.c_int, _i, _d, _d]
         L.orc_bsmatmtri.restype = None
         L.orc_rm_reg.restype = None
         L.orc_assemble_dc.argtypes = [P, _d, _d, _d, C.c_int, _z, C.c_int, C.c_int, _d, _d, _d]
@@ -311,12 +311,20 @@ def bnachbar(orc: Oracle):
     return nb.reshape((orc.elanz, mx + 1), order="F")
 
 
-def bsmatmtri(orc: Oracle, alfx, alfz, nachbar):
-    """Triangular smoothness operator smatm (manz, maxnodes+1)."""
+def bsmatmtri(orc: Oracle, alfx, alfz, nachbar, decouplings=None):
+    """Triangular smoothness operator smatm (manz, maxnodes+1).
+    decouplings: (edecoup (ndec, 2) 1-based element pairs, edecstr (ndec,))."""
     mx = orc.mesh.maxnodes
     sm = np.zeros(orc.elanz * (mx + 1))
+    if decouplings is None or len(decouplings[1]) == 0:
+        ed, es = np.zeros(2, np.int32), np.zeros(1)
+        nd = 0
+    else:
+        ed = np.asfortranarray(decouplings[0], np.int32).ravel(order="F").copy()
+        es = np.ascontiguousarray(decouplings[1], np.float64)
+        nd = len(es)
     lib().orc_bsmatmtri(C.byref(orc.cm), alfx, alfz,
-                        np.asfortranarray(nachbar, np.int32).ravel(order="F"), sm)
+                        np.asfortranarray(nachbar, np.int32).ravel(order="F"), nd, ed, es, sm)
     return sm.reshape((orc.elanz, mx + 1), order="F")
 
 

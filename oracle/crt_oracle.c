@@ -1312,7 +1312,8 @@ void orc_bnachbar(const orc_mesh *m, int *nachbar) {
  * sqrt((alfx cos a)^2 + (alfz sin a)^2), a the angle of the centroid
  * difference; smatm(i,k) = -w, smatm(i,maxnodes+1) += w.  Centroids as relem
  * sets them (src/relem.f90:134-143).  smatm: manz x (maxnodes+1). */
-void orc_bsmatmtri(const orc_mesh *m, double alfx, double alfz, const int *nachbar, double *smatm) {
+void orc_bsmatmtri(const orc_mesh *m, double alfx, double alfz, const int *nachbar, int ndec,
+                   const int *edecoup, const double *edecstr, double *smatm) {
   int elanz = m_elanz(m), nel = elanz + m_relanz(m), mx = m_maxnodes(m);
   double *ex = calloc((size_t)elanz, sizeof(double)), *ey = calloc((size_t)elanz, sizeof(double));
   int iel = 0;
@@ -1341,6 +1342,12 @@ void orc_bsmatmtri(const orc_mesh *m, double alfx, double alfz, const int *nachb
         double cx = alfx * cos(ang), cz = alfz * sin(ang);
         double alfgeo = sqrt(cx * cx + cz * cz);
         double w = edglen / dist * alfgeo;
+        /* decoupled edges, src/bsmatm_mod.f90:434-445: every listed pair
+         * (either orientation) scales the weight, in list order */
+        for (int d = 0; d < ndec; d++)
+          if ((i + 1 == edecoup[d] && nb == edecoup[d + ndec]) ||
+              (nb == edecoup[d] && i + 1 == edecoup[d + ndec]))
+            w = w * edecstr[d];
         smatm[i + (size_t)elanz * k] = -w;
         smatm[i + (size_t)elanz * mx] = smatm[i + (size_t)elanz * mx] + w;
       }

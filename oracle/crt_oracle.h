@@ -53,7 +53,8 @@ int orc_cjggra(int nanz, int manz, const orc_c *sens, const double *wmatd, const
 void orc_bsmatmreg(const orc_mesh *m, int nx, int nz, double alfx, double alfz,
                    double *smatm);
 void orc_bnachbar(const orc_mesh *m, int *nachbar);
-void orc_bsmatmtri(const orc_mesh *m, double alfx, double alfz, const int *nachbar, double *smatm);
+void orc_bsmatmtri(const orc_mesh *m, double alfx, double alfz, const int *nachbar, int ndec,
+                   const int *edecoup, const double *edecstr, double *smatm);
 void orc_rm_reg(int manz, int nx, const double *smatm, const orc_c *par, orc_c *out);
 int orc_assemble_dc(const orc_mesh *m, const double *elbg, const double *relbg, const double *kg,
                     int lrandb, const orc_c *sigma, int l, int k, double *a, double *b, double *fak);

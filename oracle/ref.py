@@ -45,7 +45,8 @@ def lib():
         L.ref_sens.argtypes = [C.c_int, _z]
         L.ref_forward_dc.argtypes = [_z, _d, _d, C.c_int]
         L.ref_set_randb2.argtypes = [C.c_int, _i, _z]
-        L.ref_bsmatmtri.argtypes = [C.c_double, C.c_double, _d, _i]
+        L.ref_bsmatmtri.argtypes = [C.c_double, C.c_double, C.c_int, _i, _d, _d, _i]
+        L.ref_bsmatmreg.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, _d]
         L.ref_assemble_dc.argtypes = [C.c_int, C.c_int, _z, _d, _d, _d]
         L.ref_choldc.argtypes = [_d]
         L.ref_sens_dc.argtypes = [_d]
@@ -194,15 +195,30 @@ class Reference:
             raise RefError("choldc", e)
         return a
 
-    def bsmatmtri(self, alfx, alfz):
-        """bnachbar + bsmatm(ltri=1): smatm and nachbar, (manz, maxnodes+1)."""
+    def bsmatmtri(self, alfx, alfz, decouplings=None):
+        """bnachbar + bsmatm(ltri=1): smatm and nachbar, (manz, maxnodes+1).
+        decouplings: (edecoup (ndec, 2) 1-based element pairs, edecstr (ndec,))."""
         mx = self.mesh.maxnodes
         sm = np.zeros(self.elanz * (mx + 1))
         nb = np.zeros(self.elanz * (mx + 1), np.int32)
-        e = lib().ref_bsmatmtri(alfx, alfz, sm, nb)
+        if decouplings is None or len(decouplings[1]) == 0:
+            ed, es, nd = np.zeros(2, np.int32), np.zeros(1), 0
+        else:
+            ed = np.asfortranarray(decouplings[0], np.int32).ravel(order="F").copy()
+            es = np.ascontiguousarray(decouplings[1], np.float64)
+            nd = len(es)
+        e = lib().ref_bsmatmtri(alfx, alfz, nd, ed, es, sm, nb)
         if e:
             raise RefError("bsmatmtri", e)
         return sm.reshape((self.elanz, mx + 1), order="F"), nb.reshape((self.elanz, mx + 1), order="F")
+
+    def bsmatmreg(self, nx, nz, alfx, alfz):
+        """bsmatm(ltri=0), i.e. bsmatmreg: smatm (manz, 3)."""
+        sm = np.zeros(self.elanz * 3)
+        e = lib().ref_bsmatmreg(int(nx), int(nz), float(alfx), float(alfz), sm)
+        if e:
+            raise RefError("bsmatmreg", e)
+        return sm.reshape((self.elanz, 3), order="F")
 
     def sens_dc(self):
         out = np.zeros(self.nanz * self.elanz)

@@ -377,12 +377,18 @@ contains
   !> Neighbours and the triangular smoothness operator: bnachbar
   !> (src/bnachbar.f90) and bsmatm with ltri = 1, i.e. bsmatmtri
   !> (src/bsmatm_mod.f90:364-461), with the element centroids as relem sets
-  !> them (src/relem.f90:134-143) and no decoupled edges.
+  !> them (src/relem.f90:134-143) and c_ndec decoupled edges: element pairs
+  !> c_edecoup (c_ndec x 2, column major, 1-based) with strength factors
+  !> c_edecstr, the arrays rall fills from decouplings.dat (src/rall.f90:648-665).
   !> smatm_out and nachbar_out: manz x (maxnodes+1), column major.
-  integer(c_int) function ref_bsmatmtri(c_alfx, c_alfz, c_smatm, c_nachbar) bind(C, name='ref_bsmatmtri')
+  integer(c_int) function ref_bsmatmtri(c_alfx, c_alfz, c_ndec, c_edecoup, c_edecstr, &
+       c_smatm, c_nachbar) bind(C, name='ref_bsmatmtri')
     use bsmatm_mod, only: bsmatm
     use konvmod, only: ltri, alfx, alfz
     real(c_double), value :: c_alfx, c_alfz
+    integer(c_int), value :: c_ndec
+    integer(c_int), intent(in) :: c_edecoup(*)
+    real(c_double), intent(in) :: c_edecstr(*)
     real(c_double), intent(out) :: c_smatm(*)
     integer(c_int), intent(out) :: c_nachbar(*)
     logical :: lb
@@ -408,7 +414,15 @@ contains
     end do
     if (allocated(nachbar)) deallocate(nachbar)
     call bnachbar()
-    decanz = 0
+    if (allocated(edecoup)) deallocate(edecoup)
+    if (allocated(edecstr)) deallocate(edecstr)
+    decanz = c_ndec
+    allocate(edecoup(max(decanz, 1), 2), edecstr(max(decanz, 1)))
+    do k = 1, decanz
+       edecoup(k, 1) = c_edecoup(k)
+       edecoup(k, 2) = c_edecoup(k + decanz)
+       edecstr(k) = c_edecstr(k)
+    end do
     ltri = 1
     alfx = c_alfx
     alfz = c_alfz
@@ -419,6 +433,28 @@ contains
     c_nachbar(1:manz*ncol) = reshape(nachbar, (/manz*ncol/))
     ref_bsmatmtri = errnr
   end function ref_bsmatmtri
+
+  !> The rectangular smoothness operator: bsmatm with ltri = 0, i.e.
+  !> bsmatmreg (src/bsmatm_mod.f90:190-362), on an nx x nz grid of
+  !> quadrilateral elements numbered row by row (element (i, j) = (i-1)*nx + j).
+  !> smatm_out: manz x 3, column major (diagonal, right neighbour, lower neighbour).
+  integer(c_int) function ref_bsmatmreg(c_nx, c_nz, c_alfx, c_alfz, c_smatm) bind(C, name='ref_bsmatmreg')
+    use bsmatm_mod, only: bsmatm
+    use konvmod, only: ltri, alfx, alfz, nx, nz
+    integer(c_int), value :: c_nx, c_nz
+    real(c_double), value :: c_alfx, c_alfz
+    real(c_double), intent(out) :: c_smatm(*)
+    logical :: lb
+    ltri = 0
+    nx = c_nx
+    nz = c_nz
+    alfx = c_alfx
+    alfz = c_alfz
+    if (allocated(smatm)) deallocate(smatm)
+    call bsmatm(1, lb)
+    c_smatm(1:manz*3) = reshape(smatm(:, 1:3), (/manz*3/))
+    ref_bsmatmreg = errnr
+  end function ref_bsmatmreg
 
   !> A bounded sample of the forward loop for timing: systems s = (k-1)*eanz + l
   !> in [c_first, c_last] (1-based), each assembled, scaled, factored and

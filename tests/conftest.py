@@ -40,6 +40,7 @@ FIXTURE_CASES = [
     ("mixed_surf", 1, False, 0),
     ("decoupling", 0, True, 373),
     ("rect_complex", 1, False, 0),
+    ("non_ordered", 1, False, 0),
 ]
 
 

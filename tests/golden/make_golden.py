@@ -36,7 +36,7 @@ def electrode_slice(kpot, enr):
     return kpot[np.asarray(enr) - 1, :, :]
 
 
-def synthetic(w, lsytop=False):
+def synthetic(w, lsytop=False, nelbg=64):
     m = w.mesh
     r = Reference(m, lsytop=lsytop)
     r.load_data(w.strnr, w.vnr)
@@ -57,8 +57,8 @@ def synthetic(w, lsytop=False):
     mixed = btyp == 11
     np.savez_compressed(
         OUT / f"ref_{w.name}.npz",
-        kwn=kwn, kwnwi=kwnwi, lbeta=r.lbeta, lrandb=r.lrandb,
-        elbg=elbg[:64], relbg=relbg[mixed][:, :3], kg=kg[mixed][:64],
+        kwn=kwn, kwnwi=kwnwi, lbeta=r.lbeta, lrandb=r.lrandb, sytop=r.sytop,
+        elbg=elbg[:nelbg], relbg=relbg[mixed][:, :3], kg=kg[mixed][:64],
         volt=v0, sigmaa0=sa0, sigmaa1=sa1,
         hpot_elec=hpot[np.asarray(m.enr) - 1, :],
         kpot_elec=electrode_slice(kpot, m.enr),
@@ -125,6 +125,89 @@ def triangular_smoothness():
     sm, nb = r.bsmatmtri(1.0, 0.5)
     np.savez_compressed(OUT / "ref_tri_small_smooth.npz", smatm=sm, nachbar=nb)
     print(f"wrote ref_tri_small_smooth.npz  manz={sm.shape[0]}")
+
+
+def topo_small():
+    """A small terrain-following triangular mesh (elem3 on sheared triangles,
+    30 m of the 2 sin(2 pi x / 40) surface so the no-flow mean is not zero)
+    with lsytop: the mixed-boundary image sources mirrored about bsytop's
+    height (src/bsytop.f90:35-69, src/beta.f90:78-90).  Every element matrix
+    is stored."""
+    w = meshgen.topo_triangles(nx_nodes=31, nz_nodes=12, eanz=16, length=30.0, depth=12.0, seed=1013)
+    w.name = "topo_tri_small"
+    synthetic(w, lsytop=True, nelbg=w.mesh.elanz)
+
+
+# (l, k) systems whose full kpot column the triangular fixture stores
+TRI_FIXTURE_SYSTEMS = ((0, 0), (107, 5), (214, 10))
+
+
+def fixture_triangular(name="modelling_triangular"):
+    """The reference's own triangular modelling case (tests/modelling_triangular:
+    1,489 nodes, band width 199, 215 electrodes, 90 measurements from its
+    config.dat), run through the reference routines: voltages both modes,
+    electrode rows of hpot, three full kpot columns, the first sensitivity
+    rows."""
+    d = FIX / name
+    m = F.read_elem(d / "elem.dat")
+    m.enr = F.read_elec(d / "elec.dat")
+    sig = F.read_rho(d / "rho.dat", m.elanz)
+    st, vn = F.read_config(d / "config.dat", m.eanz)
+    r = Reference(m)
+    r.load_data(st, vn)
+    kwn, kwnwi = r.rwaven()
+    r.precal()
+    kpot, hpot = r.forward(sig, nthreads=8)
+    v0, sa0 = r.volt(0)
+    v1, sa1 = r.volt(1)
+    s1 = r.sens(1)
+    sysl = np.array([s for s in TRI_FIXTURE_SYSTEMS if s[1] < len(kwn)], np.int32)
+    np.savez_compressed(
+        OUT / f"ref_{name}.npz", kwn=kwn, kwnwi=kwnwi, volt=v0, sigmaa0=sa0, sigmaa1=sa1,
+        hpot_elec=hpot[np.asarray(m.enr) - 1, :], systems=sysl,
+        kpot_sys=np.stack([kpot[:, l, k] for l, k in sysl]), sens1=s1[:4])
+    print(f"wrote ref_{name}.npz  K={len(kwn)} N={len(v0)} mb={m.mb}")
+
+
+def smoothness_cases():
+    """bsmatmreg (ltri = 0) on a graded quadrilateral mesh (geometric growth
+    sideways and with depth, so every weight has its own cell geometry), and
+    bsmatmtri with decoupled edges: the reference's own decouplings.dat files
+    on their grid (tests/GridDecoupling/regular_grids), and seeded pairs on a
+    terrain-following triangular mesh (one pair listed twice, one reversed)."""
+    w = meshgen.large_grf(nx_side=5, n_uniform=14, nz=9, eanz=12, seed=1021)
+    nx = 14 + 2 * 5
+    r = Reference(w.mesh)
+    out = dict(reg_nx=nx, reg_nz=9)
+    out["reg_smatm"] = r.bsmatmreg(nx, 9, 1.0, 0.5)
+    out["reg_smatm_b"] = r.bsmatmreg(nx, 9, 3.0, 0.25)
+    d = FIX / "decoupling"
+    m = F.read_elem(d / "elem.dat")
+    m.enr = F.read_elec(d / "elec.dat")
+    r = Reference(m)
+    for tag in ("line", "full"):
+        ed, es = F.read_decouplings(d / f"decouplings_{tag}.dat")
+        sm, nb = r.bsmatmtri(1.0, 1.0, (ed, es))
+        out[f"dec_{tag}_smatm"] = sm
+    out["dec_nachbar"] = nb
+    wt = meshgen.topo_triangles(nx_nodes=25, nz_nodes=10, eanz=12)
+    r = Reference(wt.mesh)
+    sm0, nb = r.bsmatmtri(1.0, 0.5)
+    rng = np.random.default_rng(1022)
+    el = rng.choice(wt.mesh.elanz, 24, replace=False)
+    pairs = []
+    for e in el:
+        nbs = [x for x in nb[e, :3] if x > 0]
+        pairs.append((e + 1, int(rng.choice(nbs))))
+    pairs.append(pairs[0])                       # the same edge twice: two factors
+    pairs.append((pairs[1][1], pairs[1][0]))     # reversed orientation
+    ed = np.array(pairs, np.int32)
+    es = rng.uniform(0.0, 1.0, len(pairs))
+    es[3] = 0.0
+    sm, _ = r.bsmatmtri(1.0, 0.5, (ed, es))
+    out.update(tri_edecoup=ed, tri_edecstr=es, tri_smatm=sm)
+    np.savez_compressed(OUT / "ref_smoothness.npz", **out)
+    print("wrote ref_smoothness.npz")
 
 
 def fixture(name, swrtr, lsink=False, nsink=0):
@@ -301,7 +384,9 @@ def update_case():
     print("wrote ref_update.npz")
 
 
-CASES = {"inversion": inversion_steps, "blam0": blam0_case, "rdati": rdati_cases, "update": update_case}
+CASES = {"inversion": inversion_steps, "blam0": blam0_case, "rdati": rdati_cases, "update": update_case,
+         "topo_small": topo_small, "triangular": fixture_triangular, "smoothness": smoothness_cases,
+         "non_ordered": lambda: fixture("non_ordered", 1)}
 
 
 def main():
@@ -322,6 +407,10 @@ def main():
     fixture("mixed_surf", 1)
     fixture("decoupling", 0, lsink=True, nsink=373)
     fixture("rect_complex", 1)
+    fixture("non_ordered", 1)
+    topo_small()
+    fixture_triangular()
+    smoothness_cases()
     inversion_steps()
     blam0_case()
     rdati_cases()

@@ -45,6 +45,7 @@ def test_null_arguments_are_rejected_without_a_device():
     assert lib.crt_sensitivities_dc(None, None) == 1000
     assert lib.crt_set_boundary_values(None, 0, None, None) == 1000
     assert lib.crt_smoothness(None, 0, 1.0, 1.0, 1, 1, None, None) == 1000
+    assert lib.crt_set_decouplings(None, 0, None, None) == 1000
     assert lib.crt_set_graph(None, 1) == 1000
     assert lib.crt_set_device_budget(None, 1 << 30) == 1000
     assert lib.crt_dmisft(None, 4, None, None, None, None, None, None, None, 0, 1.0, None, None) == 1000

@@ -36,7 +36,7 @@ EXPORTS = (
     "crt_get_precal", "crt_forward", "crt_forward_dev", "crt_voltages", "crt_voltages_dev",
     "crt_sensitivities", "crt_adjoint_gradient", "crt_adjoint_gradient_dev", "crt_jhw_r",
     "crt_cg_solve", "crt_forward_dc", "crt_sensitivities_dc", "crt_set_boundary_values",
-    "crt_smoothness", "crt_set_graph", "crt_set_device_budget",
+    "crt_smoothness", "crt_set_decouplings", "crt_set_graph", "crt_set_device_budget",
     "crt_error_info", "crt_set_timing", "crt_stage_times", "crt_factor_phase_ticks",
     "crt_kpot_device", "crt_dmisft", "crt_dmisft_dev", "crt_set_model_map", "crt_bpar", "crt_bpar_dev",
     "crt_bsigma", "crt_bsigma_dev", "crt_update_step", "crt_update_step_dev", "crt_blam0",
@@ -96,6 +96,7 @@ def load_library(path: Path = LIB_PATH):
     L.crt_forward_dc.argtypes = [vp, vp, vp, vp]
     L.crt_set_boundary_values.argtypes = [vp, i32, vp, vp]
     L.crt_smoothness.argtypes = [vp, i32, f64, f64, i32, i32, vp, vp]
+    L.crt_set_decouplings.argtypes = [vp, i32, vp, vp]
     L.crt_set_graph.argtypes = [vp, i32]
     L.crt_set_device_budget.argtypes = [vp, C.c_uint64]
     L.crt_sensitivities_dc.argtypes = [vp, vp]
@@ -310,6 +311,13 @@ class CRT:
         return g
 
     # A16/A17 ----------------------------------------------------------------
+    def set_decouplings(self, edecoup, edecstr):
+        """Decoupled edges of the triangular operator (decouplings.dat):
+        edecoup (decanz, 2) 1-based element pairs, edecstr (decanz,) factors."""
+        ed = np.asfortranarray(edecoup, np.int32).reshape(-1, 2).ravel(order="F").copy()
+        es = np.ascontiguousarray(edecstr, np.float64)
+        self._check("set_decouplings", self.lib.crt_set_decouplings(self.h, len(es), _ptr(ed), _ptr(es)))
+
     def smoothness(self, ltri, alfx, alfz, nx=0, nz=0):
         """bsmatm's smoothness operator: smatm (manz, 3) for ltri 0, or
         (smatm, nachbar), both (manz, maxnodes+1), for ltri 1/2."""

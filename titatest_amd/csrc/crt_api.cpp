@@ -56,6 +56,10 @@ struct crt_handle {
   std::vector<int> pairs_key;
   std::vector<int2> sens_cp, sens_pp, sens_mp;
   std::vector<int> sens_ord;
+  // decoupled edges of the triangular smoothness operator (decouplings.dat):
+  // element pairs (1-based) and their weight factors
+  std::vector<int2> dec_pairs;
+  std::vector<double> dec_str;
   // element-to-parameter map (modelmod::mnr) for bpar/bsigma; empty = identity
   int manz_map = 0;
   std::vector<int> mnr_host;
@@ -1650,10 +1654,12 @@ static void neighbours(crt_handle* h, std::vector<int>& nb) {
     }
 }
 
-// ltri = 1, 2: bsmatmtri (src/bsmatm_mod.f90:364-461) without decoupled
-// edges.  Across edge k of element i with neighbour n: w = edge length /
-// centroid distance * sqrt((alfx cos a)^2 + (alfz sin a)^2), a the direction
-// of the centroid difference; smatm(i,k) = -w, smatm(i,maxnodes+1) += w.
+// ltri = 1, 2: bsmatmtri (src/bsmatm_mod.f90:364-461).  Across edge k of
+// element i with neighbour n: w = edge length / centroid distance *
+// sqrt((alfx cos a)^2 + (alfz sin a)^2), a the direction of the centroid
+// difference, times the factor of every decoupling record that names the pair
+// (i, n) in either order (:434-445, applied in record order);
+// smatm(i,k) = -w, smatm(i,maxnodes+1) += w.
 static void smooth_tri(crt_handle* h, double alfx, double alfz, const std::vector<int>& nb, double* sm) {
   const int manz = h->elanz, mx = h->maxnodes;
   std::vector<double> ex(manz, 0.0), ey(manz, 0.0);
@@ -1681,7 +1687,11 @@ static void smooth_tri(crt_handle* h, double alfx, double alfz, const std::vecto
       const double dist = std::sqrt(px * px + py * py);
       const double ang = std::atan2(py, px);
       const double ax = alfx * std::cos(ang), az = alfz * std::sin(ang);
-      const double w = edglen / dist * std::sqrt(ax * ax + az * az);
+      double w = edglen / dist * std::sqrt(ax * ax + az * az);
+      for (size_t d = 0; d < h->dec_pairs.size(); ++d) {
+        const int2 p = h->dec_pairs[d];
+        if ((p.x == i + 1 && p.y == n) || (p.x == n && p.y == i + 1)) w = w * h->dec_str[d];
+      }
       sm[i + (size_t)manz * c] = -w;
       sm[i + (size_t)manz * mx] = sm[i + (size_t)manz * mx] + w;
     }
@@ -1696,6 +1706,19 @@ int crt_smoothness(crt_handle* h, int32_t ltri, double alfx, double alfz, int32_
   neighbours(h, nb);
   smooth_tri(h, alfx, alfz, nb, smatm);
   if (nachbar) std::copy(nb.begin(), nb.end(), nachbar);
+  return 0;
+}
+
+int crt_set_decouplings(crt_handle* h, int32_t decanz, const int32_t* edecoup, const double* edecstr) {
+  if (!h || decanz < 0 || (decanz > 0 && (!edecoup || !edecstr))) return kErrArg;
+  std::vector<int2> pairs((size_t)decanz);
+  for (int d = 0; d < decanz; ++d) {
+    const int a = edecoup[d], b = edecoup[d + (size_t)decanz];
+    if (a < 1 || a > h->elanz || b < 1 || b > h->elanz) return kErrArg;
+    pairs[d] = make_int2(a, b);
+  }
+  h->dec_pairs = std::move(pairs);
+  h->dec_str.assign(edecstr, edecstr + decanz);
   return 0;
 }
 

@@ -226,6 +226,26 @@ def read_config(path, eanz: int | None = None):
     return strnr, vnr
 
 
+def read_decouplings(path):
+    """decouplings.dat -> (edecoup (decanz, 2) int32 1-based element pairs,
+    edecstr (decanz,) float64 weight factors), as rall reads the file
+    (src/rall.f90:648-665): the count, then one ``elem1 elem2 factor`` record
+    per line.  The smoothness operator multiplies the weight of the edge
+    between the two elements by the factor, in either orientation
+    (src/bsmatm_mod.f90:434-445)."""
+    lines = [ln.split() for ln in Path(path).read_text().splitlines() if ln.strip()]
+    decanz = int(lines[0][0])
+    if len(lines) < 1 + decanz:
+        raise ValueError(f"decouplings.dat lists {decanz} records, has {len(lines) - 1}")
+    ed = np.empty((decanz, 2), np.int32)
+    es = np.empty(decanz, np.float64)
+    for j in range(decanz):
+        row = lines[1 + j]
+        ed[j, 0], ed[j, 1] = int(row[0]), int(row[1])
+        es[j] = _f_real(row[2])
+    return ed, es
+
+
 def decode(strnr, vnr):
     """Electrode numbers e1..e4 as decoded by bvolt (src/bvolt.f90:44-49)."""
     strnr = np.asarray(strnr, np.int64)

@@ -77,6 +77,16 @@ module crt_hip
        type(c_ptr), value :: nachbar
      end function crt_smoothness
 
+     ! decoupled edges (decouplings.dat): edecoup(decanz, 2), edecstr(decanz)
+     integer(c_int) function crt_set_decouplings(h, decanz, edecoup, edecstr) &
+          bind(C, name='crt_set_decouplings')
+       import :: c_int, c_ptr, c_int32_t, c_double
+       type(c_ptr), value :: h
+       integer(c_int32_t), value :: decanz
+       integer(c_int32_t), intent(in) :: edecoup(*)
+       real(c_double), intent(in) :: edecstr(*)
+     end function crt_set_decouplings
+
      ! DC twins (ldc): kpotdc/hpotdc real, or c_null_ptr
      integer(c_int) function crt_forward_dc(h, sigma, kpotdc, hpotdc) bind(C, name='crt_forward_dc')
        import :: c_int, c_ptr, c_double_complex
