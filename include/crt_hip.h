@@ -223,6 +223,20 @@ int crt_dmisft_dev(crt_handle *h, int32_t nanz, const double *dat_dev, const dou
                    double *wmatd_dev, const double *wmatdr_dev, const double *wmatdp_dev, const double *d0_dev,
                    const double *fm0_dev, int32_t flags, double l1min, double *stats_dev, int32_t *wdlok_dev);
 
+/* No counterpart in the reference: crt_dmisft_dev for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  Same arguments, results and stream; the
+ * misfit kernel's per-datum scratch lives in a static device array of
+ * 262,144 doubles (2 MiB per device, resolved by crt_create), so the call
+ * allocates nothing.  nanz above 262,144 returns 1004.  Calls that overlap on
+ * one device share that array and race: serialise them (one stream, or
+ * events between streams).  h must come from crt_create on the current
+ * device. */
+int crt_dmisft_dev_noalloc(crt_handle *h, int32_t nanz, const double *dat_dev, const double *sigmaa_dev,
+                           double *wmatd_dev, const double *wmatdr_dev, const double *wmatdp_dev,
+                           const double *d0_dev, const double *fm0_dev, int32_t flags, double l1min,
+                           double *stats_dev, int32_t *wdlok_dev);
+
 /* Element-to-parameter map of modelmod (mnr, 1-based, elanz entries; NULL =
  * identity with manz = elanz), used by crt_bpar and crt_bsigma.  Replaces the
  * set-up of modelmod::mnr and manz (src/rall.f90:686-692, :719-728). */

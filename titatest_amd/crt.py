@@ -38,7 +38,7 @@ EXPORTS = (
     "crt_cg_solve", "crt_forward_dc", "crt_sensitivities_dc", "crt_set_boundary_values",
     "crt_smoothness", "crt_set_decouplings", "crt_set_graph", "crt_set_device_budget",
     "crt_error_info", "crt_set_timing", "crt_stage_times", "crt_factor_phase_ticks",
-    "crt_kpot_device", "crt_dmisft", "crt_dmisft_dev", "crt_set_model_map", "crt_bpar", "crt_bpar_dev",
+    "crt_kpot_device", "crt_dmisft", "crt_dmisft_dev", "crt_dmisft_dev_noalloc", "crt_set_model_map", "crt_bpar", "crt_bpar_dev",
     "crt_bsigma", "crt_bsigma_dev", "crt_update_step", "crt_update_step_dev", "crt_blam0",
 )
 
@@ -55,6 +55,7 @@ ERRORS = {
     111: "boundary element not linked to an area element",
     1000: "invalid argument", 1001: "HIP runtime failure", 1002: "stage called out of order",
     1003: "band width above the supported limit",
+    1004: "size above the no-allocation entry point's limit",
 }
 
 
@@ -115,6 +116,7 @@ def load_library(path: Path = LIB_PATH):
     L.crt_factor_phase_ticks.argtypes = [vp, i32, vp]
     L.crt_dmisft.argtypes = [vp, i32, vp, vp, vp, vp, vp, vp, vp, i32, f64, vp, vp]
     L.crt_dmisft_dev.argtypes = [vp, i32, vp, vp, vp, vp, vp, vp, vp, i32, f64, vp, vp]
+    L.crt_dmisft_dev_noalloc.argtypes = [vp, i32, vp, vp, vp, vp, vp, vp, vp, i32, f64, vp, vp]
     L.crt_set_model_map.argtypes = [vp, i32, vp]
     L.crt_bpar.argtypes = [vp, vp, vp]
     L.crt_bpar_dev.argtypes = [vp, vp, vp]

@@ -23,6 +23,29 @@ constexpr int kErrArg = 1000;
 constexpr int kErrHip = 1001;
 constexpr int kErrOrder = 1002;
 constexpr int kErrBand = 1003;
+constexpr int kErrNoAllocSize = 1004;
+
+// Scratch of crt_dmisft_dev_noalloc: the per-datum misfit values psi, for at
+// most kNoAllocMaxData data (2 MiB per device, held until the process ends).
+// The host never passes `scratch` itself to a kernel: crt_create resolves its
+// address on each device once (hipGetSymbolAddress, the device current) into
+// noalloc_ptr[device], and the entry point passes that pointer.
+constexpr int kNoAllocMaxData = 1 << 18;
+constexpr int kMaxDevices = 64;
+static __device__ __align__(256) double scratch[kNoAllocMaxData];
+double* noalloc_ptr[kMaxDevices] = {};
+std::mutex noalloc_mu;
+
+// Resolve `scratch` on device dev (current); 0 or kErrHip.
+int noalloc_setup(int dev) {
+  if (dev < 0 || dev >= kMaxDevices) return kErrHip;
+  std::lock_guard<std::mutex> lock(noalloc_mu);
+  if (noalloc_ptr[dev]) return 0;
+  void* ptr = nullptr;
+  if (hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)) != hipSuccess || !ptr) return kErrHip;
+  noalloc_ptr[dev] = static_cast<double*>(ptr);
+  return 0;
+}
 
 struct Timer {
   std::string name;
@@ -523,6 +546,10 @@ int crt_create(crt_handle** out, const crt_mesh* m) {
     return kErrHip;
   }
   h->dev = dev;
+  if (noalloc_setup(dev)) {
+    delete h;
+    return kErrHip;
+  }
   h->ncu = prop.multiProcessorCount;
   h->lds_max = prop.sharedMemPerBlock;
   if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
@@ -1811,6 +1838,37 @@ int crt_dmisft_dev(crt_handle* h, int32_t nanz, const double* dat_dev, const dou
   tstart(h, "misfit");
   hipLaunchKernelGGL(crt_misfit_kernel, dim3(1), dim3(kInvThreads), 0, h->stream, A);
   tstop(h);
+  return hipGetLastError() == hipSuccess ? 0 : kErrHip;
+}
+
+int crt_dmisft_dev_noalloc(crt_handle* h, int32_t nanz, const double* dat_dev, const double* sigmaa_dev,
+                           double* wmatd_dev, const double* wmatdr_dev, const double* wmatdp_dev,
+                           const double* d0_dev, const double* fm0_dev, int32_t flags, double l1min,
+                           double* stats_dev, int32_t* wdlok_dev) {
+  if (!h || nanz <= 0 || !dat_dev || !sigmaa_dev || !wmatd_dev || !wmatdr_dev || !wmatdp_dev || !stats_dev ||
+      !wdlok_dev || (flags & ~15))
+    return kErrArg;
+  if ((flags & kMisfitLdiff) && (!d0_dev || !fm0_dev)) return kErrArg;
+  if (nanz > kNoAllocMaxData) return kErrNoAllocSize;
+  double* psi = (h->dev >= 0 && h->dev < kMaxDevices) ? noalloc_ptr[h->dev] : nullptr;
+  if (!psi) return kErrHip;
+  MisfitArgs A;
+  A.nanz = nanz;
+  A.flags = flags;
+  A.l1min = l1min;
+  A.dat = reinterpret_cast<const double2*>(dat_dev);
+  A.sigmaa = reinterpret_cast<const double2*>(sigmaa_dev);
+  A.wmatd = wmatd_dev;
+  A.wmatdr = wmatdr_dev;
+  A.wmatdp = wmatdp_dev;
+  A.d0 = reinterpret_cast<const double2*>(d0_dev);
+  A.fm0 = reinterpret_cast<const double2*>(fm0_dev);
+  // every psi[i] is written before it is read, by the same thread: nothing
+  // of the previous call's contents is used
+  A.psi = psi;
+  A.wdlok = wdlok_dev;
+  A.stats = stats_dev;
+  hipLaunchKernelGGL(crt_misfit_kernel, dim3(1), dim3(kInvThreads), 0, h->stream, A);
   return hipGetLastError() == hipSuccess ? 0 : kErrHip;
 }
 
