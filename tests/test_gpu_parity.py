@@ -480,10 +480,11 @@ def test_triangles_large_band_match_oracle(crtmod, oracle_lib):
 
 
 def test_widest_band_matches_oracle(crtmod, oracle_lib):
-    """mb = 124, the widest band the factor kernel takes: the 512-thread form
-    with three panel rows per lane (W = 132), on a 33x122-quadrilateral mesh.
-    A sample of (k, l) systems against the oracle, k = 0, l = 0 against the
-    long-double solution, and mb = 125 refused with 1003."""
+    """mb = 124, the widest band whose trailing window fits in LDS: the
+    512-thread form with three panel rows per lane (W = 132), on a
+    33x122-quadrilateral mesh.  A sample of (k, l) systems against the oracle,
+    k = 0, l = 0 against the long-double solution, and mb = 125 (the first
+    band of the wide form, window in global memory) likewise."""
     w = meshgen.large_grf(nx_side=4, n_uniform=24, nz=122, eanz=24)
     say("widest band mb=", w.mesh.mb, "n=", w.mesh.sanz)
     assert w.mesh.mb == 124
@@ -506,12 +507,18 @@ def test_widest_band_matches_oracle(crtmod, oracle_lib):
     del g
     w2 = meshgen.large_grf(nx_side=4, n_uniform=24, nz=123, eanz=24)
     assert w2.mesh.mb == 125
+    say("first wide band mb=", w2.mesh.mb)
     g2 = crtmod.CRT(w2.mesh)
     g2.rwaven(w2.strnr, w2.vnr)
     g2.precal()
-    with pytest.raises(crtmod.CRTError) as e:
-        g2.forward(w2.sigma)
-    assert e.value.code == 1003
+    (kpot2,) = g2.forward(w2.sigma, want_kpot=True)
+    o2 = oracle_lib.Oracle(w2.mesh)
+    o2.rwaven(w2.strnr, w2.vnr)
+    o2.precal()
+    for s in (0, 5 * E + 13):
+        o2.forward(w2.sigma, sys_range=(s, s + 1))
+        k, l = divmod(s, E)
+        assert rel_err(kpot2[:, l, k], o2.kpot[:, l, k]) < PTOL, (k, l)
 
 
 def test_fortran_host_crmod_fixture(crtmod, tmp_path):

@@ -840,30 +840,57 @@ static int open_lds(crt_handle* h, const void* kern, size_t* dyn = nullptr) {
   return 0;
 }
 
+// The substitutions keep a window of 64 NS rows in NS register slots per lane
+// and refill every G rows: G < 64 NS + 1 - mb keeps a refill ahead of its
+// first update.  NS = 2 up to mb = 124, NS = 4 up to kMaxBand.
+static void launch_backsolve(int mb, int nrhs, hipStream_t st, const BacksolveArgs& B) {
+  if (mb <= 120)
+    hipLaunchKernelGGL((crt_backsolve_kernel<8, 8, 2>), dim3(nrhs), dim3(64), 0, st, B);
+  else if (mb <= 124)
+    hipLaunchKernelGGL((crt_backsolve_kernel<8, 4, 2>), dim3(nrhs), dim3(64), 0, st, B);
+  else
+    hipLaunchKernelGGL((crt_backsolve_kernel<8, 8, 4>), dim3(nrhs), dim3(64), 0, st, B);
+}
+static void launch_fwdsweep(int mb, int nrhs, hipStream_t st, const FwdArgs& F) {
+  if (mb <= 120)
+    hipLaunchKernelGGL((crt_fwdsweep_kernel<8, 8, 2>), dim3(nrhs), dim3(64), 0, st, F);
+  else if (mb <= 124)
+    hipLaunchKernelGGL((crt_fwdsweep_kernel<8, 4, 2>), dim3(nrhs), dim3(64), 0, st, F);
+  else
+    hipLaunchKernelGGL((crt_fwdsweep_kernel<8, 8, 4>), dim3(nrhs), dim3(64), 0, st, F);
+}
+
 static int run_forward(crt_handle* h) {
   const int n = h->n, mb = h->mb, E = h->E, K = h->K;
-  const size_t lds = crt_factor_lds_bytes(mb);
   // the substitutions address one factor with 32-bit byte offsets
-  if (lds > h->lds_max || mb > 124 || (size_t)n * (mb + 1) * 16 >= (size_t(1) << 31)) return kErrBand;
+  if (mb > kMaxBand || (size_t)n * (mb + 1) * 16 >= (size_t(1) << 31)) return kErrBand;
+  // The trailing-window ring lives in LDS up to mb = 124 (when it fits), in a
+  // per-workgroup global scratch above (the wide-band form).
+  const bool gring = mb > 124 || crt_factor_lds_bytes(mb) > h->lds_max;
+  const size_t lds = gring ? crt_factor_lds_bytes_gr(mb) : crt_factor_lds_bytes(mb);
+  if (lds > h->lds_max) return kErrBand;
   // wave 0 holds the W = mb + kNB panel rows, ROWS per lane; the other waves
   // own the trailing window's 4x4 tiles beyond the look-ahead columns, MAXT
   // each.  When one workgroup's LDS leaves no room for a second on the CU, the
-  // workgroup takes 512 threads instead of 256.
+  // workgroup takes 512 threads instead of 256 (the wide-band form always).
   const int W = mb + kNB, tw = (mb + 3) / 4;
   const int nrest = tw > 2 ? ((tw - 2) * (tw - 1)) / 2 : 0;
-  const int nt = 2 * lds > h->lds_max ? 512 : 256;
+  const int nt = gring || 2 * lds > h->lds_max ? 512 : 256;
   const int rows = (W + 63) / 64, maxt = std::max(1, (nrest + nt - 65) / (nt - 64));
   void (*kern)(FactorArgs) = nullptr;
-  if (nt == 256) {
-    if (rows == 1 && maxt == 1) kern = crt_factor_kernel<1, 1, 256>;
-    else if (rows == 2 && maxt == 1) kern = crt_factor_kernel<2, 1, 256>;
-    else if (rows == 2 && maxt == 2) kern = crt_factor_kernel<2, 2, 256>;
-    else if (rows == 2 && maxt == 3) kern = crt_factor_kernel<2, 3, 256>;
-    else if (rows == 3 && maxt <= 3) kern = crt_factor_kernel<3, 3, 256>;
+  if (gring) {
+    if (rows <= 3 && maxt <= 5) kern = crt_factor_kernel<3, 5, 512, true>;
+    else if (rows == 4 && maxt <= 5) kern = crt_factor_kernel<4, 5, 512, true>;
+  } else if (nt == 256) {
+    if (rows == 1 && maxt == 1) kern = crt_factor_kernel<1, 1, 256, false>;
+    else if (rows == 2 && maxt == 1) kern = crt_factor_kernel<2, 1, 256, false>;
+    else if (rows == 2 && maxt == 2) kern = crt_factor_kernel<2, 2, 256, false>;
+    else if (rows == 2 && maxt == 3) kern = crt_factor_kernel<2, 3, 256, false>;
+    else if (rows == 3 && maxt <= 3) kern = crt_factor_kernel<3, 3, 256, false>;
   } else {
-    if (rows <= 2 && maxt == 1) kern = crt_factor_kernel<2, 1, 512>;
-    else if (rows <= 2 && maxt == 2) kern = crt_factor_kernel<2, 2, 512>;
-    else if (rows == 3 && maxt <= 2) kern = crt_factor_kernel<3, 2, 512>;
+    if (rows <= 2 && maxt == 1) kern = crt_factor_kernel<2, 1, 512, false>;
+    else if (rows <= 2 && maxt == 2) kern = crt_factor_kernel<2, 2, 512, false>;
+    else if (rows == 3 && maxt <= 2) kern = crt_factor_kernel<3, 2, 512, false>;
   }
   if (!kern) return kErrBand;
   if (int rc = open_lds(h, reinterpret_cast<const void*>(kern))) return rc;
@@ -898,6 +925,7 @@ static int run_forward(crt_handle* h) {
   A.lsink = h->lsink;
   A.nsink = h->nsink - 1;
   A.dc = h->dc ? 1 : 0;
+  A.ring = nullptr;
   double2* av = h->get<double2>("av", (size_t)K * n * h->R);
   double2* kpot = h->get<double2>("kpot", (size_t)K * E * n);
   NEED(av);
@@ -1004,6 +1032,10 @@ static int run_forward(crt_handle* h) {
     if (rb2) hipLaunchKernelGGL(crt_rhs_kernel, dim3(rgrid), dim3(256), 0, h->stream, A);
     tstop(h);
     h->slots = std::min(cnt, h->ncu * per_cu);
+    if (gring) {
+      A.ring = h->get<double2>("ring", (size_t)h->slots * ((mb + 1) / 2) * (mb + 1));
+      NEED(A.ring);
+    }
     const int nrhs = cnt * rpf;
     // Own factors per system: the systems of the persistent kernel's full
     // rounds are factored first, then the partial last round, during which
@@ -1038,10 +1070,7 @@ static int run_forward(crt_handle* h) {
           HIPCHK(hipStreamWaitEvent(h->stream2, h->ev_fac, 0));
         }
         const int slot = tstart_on(h, "backsolve", bs);
-        if (mb <= 120)
-          hipLaunchKernelGGL((crt_backsolve_kernel<8, 8>), dim3(pc), dim3(64), 0, bs, Bp);
-        else
-          hipLaunchKernelGGL((crt_backsolve_kernel<8, 4>), dim3(pc), dim3(64), 0, bs, Bp);
+        launch_backsolve(mb, pc, bs, Bp);
         tstop_on(h, slot, bs);
       }
       HIPCHK(hipEventRecord(h->ev_join, h->stream2));
@@ -1060,20 +1089,13 @@ static int run_forward(crt_handle* h) {
                          (const double2*)A.L, Lc, n, mb, cnt);
       tstop(h);
       tstart(h, "forward_sweep");
-      // row groups G < 129 - mb (the refill stays ahead of its first update)
-      if (mb <= 120)
-        hipLaunchKernelGGL((crt_fwdsweep_kernel<8, 8>), dim3(nrhs), dim3(64), 0, h->stream, F);
-      else
-        hipLaunchKernelGGL((crt_fwdsweep_kernel<8, 4>), dim3(nrhs), dim3(64), 0, h->stream, F);
+      launch_fwdsweep(mb, nrhs, h->stream, F);
       tstop(h);
     }
     B.nrhs = nrhs;
     B.fac_first = first;
     tstart(h, "backsolve");
-    if (mb <= 120)
-      hipLaunchKernelGGL((crt_backsolve_kernel<8, 8>), dim3(nrhs), dim3(64), 0, h->stream, B);
-    else
-      hipLaunchKernelGGL((crt_backsolve_kernel<8, 4>), dim3(nrhs), dim3(64), 0, h->stream, B);
+    launch_backsolve(mb, nrhs, h->stream, B);
     tstop(h);
   }
   // electrode slice of the back-transform, used by the voltage stage

@@ -346,20 +346,38 @@ __global__ void crt_rhs_kernel(FactorArgs A) {
   stc(A.rhs + (size_t)s * A.n + r, rmul(A.fak[(size_t)s * A.n + r], b));
 }
 
+// Stage barrier of the factor kernel: LDS only when the trailing window lives
+// in LDS; with the window in global memory (GR) the workgroup-scope fences
+// also order the ring's global loads and stores between the waves.
+template <bool GR>
+__device__ __forceinline__ void stage_barrier() {
+  if constexpr (GR) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+  } else {
+    lds_barrier();
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Lookahead band factorisation.  ROWS: panel rows per lane of wave 0
 // (W = mb + kNB <= 64 ROWS); MAXT: trailing tiles per thread of the other
 // waves; NT threads per workgroup: 256 (two workgroups per CU) or, when the
 // LDS of one workgroup leaves no room for a second (mb above ~98), 512, so
 // that the CU still holds 8 wavefronts (7 for the look-ahead and trailing
-// update).
-template <int ROWS, int MAXT, int NT>
+// update).  GR: the trailing-window ring lives in a global scratch of half x
+// (mb + 1) entries per workgroup (A.ring) instead of LDS, for bands whose
+// ring exceeds the CU's LDS (mb above 124); the panels and staged rows stay
+// in LDS.
+template <int ROWS, int MAXT, int NT, bool GR>
 __global__ __launch_bounds__(NT, 512 / NT) void crt_factor_kernel(FactorArgs A) {
   extern __shared__ double2 smem[];
   const int n = A.n, mb = A.mb, R = A.R;
   const int W = mb + kNB, M1 = mb + 1, half = (mb + 1) / 2;
-  cplx* Tt = reinterpret_cast<cplx*>(smem);  // half x M1 ring slots
-  cplx* P0 = Tt + half * M1;                  // 2 panels of W x kNB
+  // half x M1 ring slots
+  cplx* Tt = GR ? reinterpret_cast<cplx*>(A.ring) + (size_t)blockIdx.x * half * M1 : reinterpret_cast<cplx*>(smem);
+  cplx* P0 = GR ? reinterpret_cast<cplx*>(smem) : Tt + half * M1;  // 2 panels of W x kNB
   cplx* sv = P0 + 2 * W * kNB;                // kStg batches of kNB x kMaxR values
   cplx* xb = sv + kStg * kNB * kMaxR;         // kNB: column broadcast of the panel
   cplx* xd = xb + kNB;                        // kNB: the panel's pivots d_c
@@ -407,7 +425,7 @@ __global__ __launch_bounds__(NT, 512 / NT) void crt_factor_kernel(FactorArgs A) 
     // ---- prologue: ring zeroed, rows [0, mb - kNB) in place, batches
     // [mb - kNB, mb) and [mb, mb + kNB) staged, wave 0's right-hand side
     for (int i = tid; i < half * M1; i += NT) Tt[i] = mkc(0.0, 0.0);
-    lds_barrier();
+    stage_barrier<GR>();
     {
       Ring ring0{mb, M1, half, 0};
       for (int r = tid; r < min(mb - kNB, n); r += NT)
@@ -449,7 +467,7 @@ __global__ __launch_bounds__(NT, 512 / NT) void crt_factor_kernel(FactorArgs A) 
       const int rw = lane + 64 * q;
       yv[q] = csel(wave == 0 && rw < W && rw < n, rhs(rw), mkc(0.0, 0.0));
     }
-    lds_barrier();
+    stage_barrier<GR>();
     clk.mark(0);
 
     for (int J = 0; J < nst; ++J) {
@@ -503,7 +521,7 @@ __global__ __launch_bounds__(NT, 512 / NT) void crt_factor_kernel(FactorArgs A) 
       }
       __builtin_amdgcn_s_setprio(0);
       clk.mark(0);
-      lds_barrier();
+      stage_barrier<GR>();
       clk.mark(1);
 
       // ---- phase B
@@ -652,44 +670,49 @@ __global__ __launch_bounds__(NT, 512 / NT) void crt_factor_kernel(FactorArgs A) 
         }
         clk.mark(2);
       }
-      lds_barrier();
+      stage_barrier<GR>();
       clk.mark(3);
     }
   }
   clk.flush();
 }
 
-template __global__ void crt_factor_kernel<1, 1, 256>(FactorArgs);
-template __global__ void crt_factor_kernel<2, 1, 256>(FactorArgs);
-template __global__ void crt_factor_kernel<2, 2, 256>(FactorArgs);
-template __global__ void crt_factor_kernel<2, 3, 256>(FactorArgs);
-template __global__ void crt_factor_kernel<3, 3, 256>(FactorArgs);
-template __global__ void crt_factor_kernel<2, 1, 512>(FactorArgs);
-template __global__ void crt_factor_kernel<2, 2, 512>(FactorArgs);
-template __global__ void crt_factor_kernel<3, 2, 512>(FactorArgs);
+template __global__ void crt_factor_kernel<1, 1, 256, false>(FactorArgs);
+template __global__ void crt_factor_kernel<2, 1, 256, false>(FactorArgs);
+template __global__ void crt_factor_kernel<2, 2, 256, false>(FactorArgs);
+template __global__ void crt_factor_kernel<2, 3, 256, false>(FactorArgs);
+template __global__ void crt_factor_kernel<3, 3, 256, false>(FactorArgs);
+template __global__ void crt_factor_kernel<2, 1, 512, false>(FactorArgs);
+template __global__ void crt_factor_kernel<2, 2, 512, false>(FactorArgs);
+template __global__ void crt_factor_kernel<3, 2, 512, false>(FactorArgs);
+template __global__ void crt_factor_kernel<3, 5, 512, true>(FactorArgs);
+template __global__ void crt_factor_kernel<4, 5, 512, true>(FactorArgs);
 
 // ---------------------------------------------------------------------------
 // Back substitution, one wavefront per right-hand side.  Lane q (q < 64) keeps
-// the working values of the rows j = q and j = q + 64 (mod 128) of the band
-// window in two slots (128 > mb): slot value y_j + sum_{i>j} L_ij x_i.  Row
+// the working values of the rows j = q + 64 s (mod 64 NS), s < NS, of the band
+// window in NS slots (64 NS > mb): slot value y_j + sum_{i>j} L_ij x_i.  Row
 // i's pivot lane forms x_i = -(its value) / L_ii, the value is broadcast and
 // every lane adds L_ij x_i to its rows j in [i-mb, i) -- for each j the
 // updates arrive in descending i, the reference's order.  Rows are taken in
-// 64-row blocks.  Slot A of a lane holds the class of the block's own rows
-// (lane + 64 (b & 1)), slot B the other; the two swap between blocks.  Row i's
-// pivot is slot A of lane i mod 64, which holds -1/L_ii of its block row
+// 64-row blocks.  Slot s of a lane holds the class of block b - s (rows
+// lane + 64 ((b - s) mod NS)); after a block the slots rotate by one.  Row
+// i's pivot is slot 0 of lane i mod 64, which holds -1/L_ii of its block row
 // (formed once per block from a block-wide load, with f_i and the refill
-// values y_{i-128}).  The pivot lane's slot A takes no update at its own step
-// (the load of L_ii is masked), so it keeps row i's final value; every G rows
-// the group's lanes keep x (kpot is stored per block, one row per lane) and
-// refill slot A with y_{i-128}, which row i-128+mb first updates -- G < 129 -
-// mb keeps the refill ahead of it.  Rows of L are fetched DEPTH rows ahead
-// with buffer loads whose offsets for out-of-band, above-n and below-0 entries
-// lie past the end of L, so they read zero: no lane-divergent branch and no
-// per-value mask in the loop.
-template <int DEPTH, int G>
+// values y_{i-64 NS}).  The pivot lane's slot 0 takes no update at its own
+// step (the load of L_ii is masked), so it keeps row i's final value; every G
+// rows the group's lanes keep x (kpot is stored per block, one row per lane)
+// and refill slot 0 with y_{i-64 NS}, which row i-64 NS+mb first updates --
+// G < 64 NS + 1 - mb keeps the refill ahead of it.  Rows of L are fetched
+// DEPTH rows ahead with buffer loads whose offsets for out-of-band, above-n and
+// below-0 entries lie past the end of L, so they read zero: no lane-divergent
+// branch and no per-value mask in the loop.
+template <int DEPTH, int G, int NS>
 __global__ __launch_bounds__(64) void crt_backsolve_kernel(BacksolveArgs B) {
   static_assert(DEPTH % G == 0 && 64 % DEPTH == 0, "row groups tile the block");
+  static_assert(NS == 2 || NS == 4, "slot classes");
+  constexpr int WIN = 64 * NS;                // rows of the slot window
+  constexpr int RMASK = (WIN - 1) * 16;       // (x mod WIN) * 16 on 16-byte units
   const int lane = threadIdx.x;
   const int r = blockIdx.x;
   if (r >= B.nrhs) return;
@@ -704,29 +727,30 @@ __global__ __launch_bounds__(64) void crt_backsolve_kernel(BacksolveArgs B) {
   const double* fak = B.fak + (size_t)f * n;
   double2* out = B.kpot + ((size_t)k * B.eanz + l) * n;
 
-  // Entries of row t of L for both slots: the slot's row is j = t - rf,
-  // rf = (t - q) mod 128, at band offset mb - rf, in range for 1 <= rf <=
-  // min(mb, t).  For slot A (q = lane + (t & 64)) rf = (t mod 64 - lane) mod
-  // 128; slot B has rf ^ 64.  rf in 16-byte units.
+  // Entries of row t of L for every slot: slot s's row is j = t - rf, rf =
+  // (t mod 64 - lane + 64 s) mod WIN, at band offset mb - rf, in range for 1
+  // <= rf <= min(mb, t).  rf in 16-byte units.
   const int m16 = m1 * 16;
   auto fetch = [&](int t, cplx* dst) __attribute__((always_inline)) {
     const int base = t * m16 + mb * 16;
     const unsigned lim = (unsigned)max(min(mb, t), 0) * 16u;
-    const int ra = (((t & 63) - lane) * 16) & 0x7F0;
-    const int rb = ra ^ 0x400;
-    dst[0] = bldc(Lr, (unsigned)(ra - 16) < lim ? (unsigned)(base - ra) : kBufOut);
-    dst[1] = bldc(Lr, (unsigned)(rb - 16) < lim ? (unsigned)(base - rb) : kBufOut);
+    const int ra = (((t & 63) - lane) * 16) & RMASK;
+#pragma unroll
+    for (int q = 0; q < NS; ++q) {
+      const int rq = (ra + q * 0x400) & RMASK;
+      dst[q] = bldc(Lr, (unsigned)(rq - 16) < lim ? (unsigned)(base - rq) : kBufOut);
+    }
   };
-  // f_i, L_ii and y_{i-128} of block b's row i = 64 b + lane (raw loads;
+  // f_i, L_ii and y_{i-WIN} of block b's row i = 64 b + lane (raw loads;
   // masked when the block becomes current)
   auto load_blk = [&](int b, double& fb, cplx& db, cplx& yb) __attribute__((always_inline)) {
     const int i = 64 * b + lane;
     fb = fak[min(max(i, 0), n - 1)];
     db = bldc(Lr, (unsigned)(i * m16 + mb * 16));
-    yb = ldc(yg + min(max(i - 128, 0), n - 1));
+    yb = ldc(yg + min(max(i - WIN, 0), n - 1));
   };
-  cplx ya, yb;  // slots A and B
-  cplx lr[DEPTH][2];
+  cplx ys[NS];  // slot s: the class of block b - s
+  cplx lr[DEPTH][NS];
   // Rows are taken in 64-row blocks from the top; the top block is padded
   // above n with rows that change nothing (x = 0: their -1/L_ii is zero).
   const int btop = (n - 1) >> 6;
@@ -735,20 +759,19 @@ __global__ __launch_bounds__(64) void crt_backsolve_kernel(BacksolveArgs B) {
   auto enter_blk = [&](int b) __attribute__((always_inline)) {
     const int i = 64 * b + lane;
     fcur = i < n ? fcur : 0.0;
-    ycur = csel(i - 128 >= 0 && i - 128 < n, ycur, mkc(0.0, 0.0));
+    ycur = csel(i - WIN >= 0 && i - WIN < n, ycur, mkc(0.0, 0.0));
     const cplx rc = crecip(dcur);
     nl = csel(i < n, mkc(-rc.re, -rc.im), mkc(0.0, 0.0));
   };
   load_blk(btop, fcur, dcur, ycur);
   load_blk(btop - 1, fnxt, dnxt, ynxt);
   const int itop = 64 * btop + 63;
-  {
-    // slot A's class at the top block is lane + 64 (btop & 1)
-    const int qa = lane + (itop & 64), qb = qa ^ 64;
-    const int ja = itop - (((itop - qa) % 128) + 128) % 128;
-    const int jb = itop - (((itop - qb) % 128) + 128) % 128;
-    ya = csel(ja >= 0 && ja < n, ldc(yg + min(max(ja, 0), n - 1)), mkc(0.0, 0.0));
-    yb = csel(jb >= 0 && jb < n, ldc(yg + min(max(jb, 0), n - 1)), mkc(0.0, 0.0));
+#pragma unroll
+  for (int q = 0; q < NS; ++q) {
+    // slot q's class at the top block: lane + 64 ((btop - q) mod NS)
+    const int cls = lane + 64 * (((btop - q) % NS + NS) % NS);
+    const int j = itop - (((itop - cls) % WIN) + WIN) % WIN;
+    ys[q] = csel(j >= 0 && j < n, ldc(yg + min(max(j, 0), n - 1)), mkc(0.0, 0.0));
   }
   // in row order, as the loop refills them: the loop's wait for row i's
   // entries then counts the same loads behind them on entry as on the back edge
@@ -772,34 +795,41 @@ __global__ __launch_bounds__(64) void crt_backsolve_kernel(BacksolveArgs B) {
 #pragma unroll
       for (int d = 0; d < DEPTH; ++d) {
         const int i = 64 * b + 63 - c - d;
-        const cplx xb = rlc(cmul(ya, nl), i & 63);
-        ya = cmadd(ya, lr[d][0], xb);
-        yb = cmadd(yb, lr[d][1], xb);
-        // the updates are formed before lr[d] is refilled (otherwise they
-        // may sink below the refill, which then needs a copy and a wait)
-        asm volatile("" : "+v"(ya.re), "+v"(ya.im), "+v"(yb.re), "+v"(yb.im));
+        const cplx xb = rlc(cmul(ys[0], nl), i & 63);
+#pragma unroll
+        for (int q = 0; q < NS; ++q) {
+          ys[q] = cmadd(ys[q], lr[d][q], xb);
+          // the updates are formed before lr[d] is refilled (otherwise they
+          // may sink below the refill, which then needs a copy and a wait)
+          asm volatile("" : "+v"(ys[q].re), "+v"(ys[q].im));
+        }
         fetch(i - DEPTH, lr[d]);
         // the fetch stays here, DEPTH rows ahead of its use (the scheduler
         // would otherwise sink it next to the use)
         __builtin_amdgcn_sched_barrier(0);
         if ((d + 1) % G == 0) {
-          // rows i+G-1 .. i done: their lanes keep x and refill slot A
+          // rows i+G-1 .. i done: their lanes keep x and refill slot 0
           const bool g = (unsigned)(lane - (i & 63)) < (unsigned)G;
-          xo = csel(g, cmul(ya, nl), xo);
-          ya = csel(g, ycur, ya);
+          xo = csel(g, cmul(ys[0], nl), xo);
+          ys[0] = csel(g, ycur, ys[0]);
         }
       }
     }
     const int io = 64 * b + lane;
     if (io < n) out[io] = make_double2(fcur * xo.re, fcur * xo.im);
-    const cplx t = ya;
-    ya = yb;
-    yb = t;
+    // the next block's class is in slot 1; slot 0 (refilled with the class
+    // WIN rows down) goes last
+    const cplx t = ys[0];
+#pragma unroll
+    for (int q = 0; q + 1 < NS; ++q) ys[q] = ys[q + 1];
+    ys[NS - 1] = t;
   }
 }
 
-template __global__ void crt_backsolve_kernel<8, 8>(BacksolveArgs);
-template __global__ void crt_backsolve_kernel<8, 4>(BacksolveArgs);
+template __global__ void crt_backsolve_kernel<8, 8, 2>(BacksolveArgs);
+template __global__ void crt_backsolve_kernel<8, 4, 2>(BacksolveArgs);
+template __global__ void crt_backsolve_kernel<8, 8, 4>(BacksolveArgs);
+template __global__ void crt_backsolve_kernel<8, 4, 4>(BacksolveArgs);
 
 // ---------------------------------------------------------------------------
 // Column-band copy of shared factors for the forward sweep: row j of Lc holds
@@ -828,27 +858,30 @@ __global__ void crt_lcol_kernel(const double2* __restrict__ L, double2* __restri
 // rows (the source electrode, the sink), so y is zero up to the first of them
 // and the sweep starts at that row's 64-row block.  The mirror image of the
 // back substitution above, on the column-band copy Lc: lane q (q < 64) keeps
-// the partial sums b_i - sum_{j'<j} L_ij' y_j' of the rows i = q and q + 64
-// (mod 128) ahead of step j in two slots.  Row j's pivot lane forms w = -y_j =
+// the partial sums b_i - sum_{j'<j} L_ij' y_j' of the rows i = q + 64 s (mod
+// 64 NS) ahead of step j in NS slots.  Row j's pivot lane forms w = -y_j =
 // -(its sum) / L_jj, w is broadcast and every lane adds L_ij w to its rows i
 // in (j, j+mb] -- for each i the terms arrive in ascending j, the order of
-// vre's row sums (src/vre.f90).  Slot A of a lane holds the current block's
-// row 64 b + lane, slot B the next block's; the two swap between blocks.  The
-// pivot lane's slot A takes no update at its own step (the load of its entry
-// is masked), so it keeps row j's final sum; every G rows the group's lanes
-// keep y and refill slot A with b_{j+128}, which column j+128-mb first
-// updates -- G < 129 - mb keeps the refill ahead of it.  Columns of Lc are
-// fetched DEPTH steps ahead with buffer loads whose offsets for entries past
-// the band or past row n-1 lie beyond the end of Lc, so they read zero.
-template <int DEPTH, int G>
+// vre's row sums (src/vre.f90).  Slot s of a lane holds block b + s's row
+// 64 (b + s) + lane; the slots rotate by one between blocks.  The pivot lane's
+// slot 0 takes no update at its own step (the load of its entry is masked), so
+// it keeps row j's final sum; every G rows the group's lanes keep y and refill
+// slot 0 with b_{j+64 NS}, which column j+64 NS-mb first updates -- G < 64 NS
+// + 1 - mb keeps the refill ahead of it.  Columns of Lc are fetched DEPTH
+// steps ahead with buffer loads whose offsets for entries past the band or
+// past row n-1 lie beyond the end of Lc, so they read zero.
+template <int DEPTH, int G, int NS>
 __global__ __launch_bounds__(64) void crt_fwdsweep_kernel(FwdArgs F) {
   static_assert(DEPTH % G == 0 && 64 % DEPTH == 0, "row groups tile the block");
+  static_assert(NS == 2 || NS == 4, "slot classes");
+  constexpr int WIN = 64 * NS;
+  constexpr int RMASK = (WIN - 1) * 16;
   const int lane = threadIdx.x;
   const int r = blockIdx.x;
   if (r >= F.nrhs) return;
   const int f = r / F.eanz, l = r % F.eanz;
   const int n = F.n, mb = F.mb, m1 = mb + 1;
-  // n m1 16 < 2^31 (checked by the host): offsets of rows in [0, n + 128) fit
+  // n m1 16 < 2^31 (checked by the host): offsets of rows in [0, n + WIN) fit
   const brsrc Lr = make_rsrc(F.Lc + (size_t)f * n * m1, (unsigned)(n * m1 * 16));
   double2* yg = F.y + (size_t)r * n;
   // b = -f at the source electrode, +f at the sink (the sink wins a tie).
@@ -878,23 +911,25 @@ __global__ __launch_bounds__(64) void crt_fwdsweep_kernel(FwdArgs F) {
   for (int i = lane; i < 64 * b0; i += 64) yg[i] = make_double2(0.0, 0.0);
 
   const int m16 = m1 * 16;
-  // Entries of column j for both slots: slot A's row is j + rf, rf = (lane -
-  // j mod 64) mod 128, slot B's j + (rf ^ 64); in range for 1 <= rf <=
-  // min(mb, n-1-j).  rf in 16-byte units.
+  // Entries of column j for every slot: slot s's row is j + rf, rf = (lane +
+  // 64 s - j mod 64) mod WIN; in range for 1 <= rf <= min(mb, n-1-j).  rf in
+  // 16-byte units.
   auto fetch = [&](int j, cplx* dst) __attribute__((always_inline)) {
     const unsigned base = (unsigned)j * (unsigned)m16;
     const unsigned lim = (unsigned)max(min(mb, n - 1 - j), 0) * 16u;
-    const int ra = ((lane - (j & 63)) * 16) & 0x7F0;
-    const int rb = ra ^ 0x400;
-    dst[0] = bldc(Lr, (unsigned)(ra - 16) < lim ? base + (unsigned)ra : kBufOut);
-    dst[1] = bldc(Lr, (unsigned)(rb - 16) < lim ? base + (unsigned)rb : kBufOut);
+    const int ra = ((lane - (j & 63)) * 16) & RMASK;
+#pragma unroll
+    for (int q = 0; q < NS; ++q) {
+      const int rq = (ra + q * 0x400) & RMASK;
+      dst[q] = bldc(Lr, (unsigned)(rq - 16) < lim ? base + (unsigned)rq : kBufOut);
+    }
   };
-  // L_jj and b_{j+128} of block b's row j = 64 b + lane (L_jj reads zero for
+  // L_jj and b_{j+WIN} of block b's row j = 64 b + lane (L_jj reads zero for
   // rows past n-1; masked when the block becomes current)
   auto load_blk = [&](int b, cplx& db, cplx& bb) __attribute__((always_inline)) {
     const int j = 64 * b + lane;
     db = bldc(Lr, (unsigned)j * (unsigned)m16);
-    bb = bload(j + 128);
+    bb = bload(j + WIN);
   };
   cplx dcur, dnxt, bcur, bnxt, nl, yo = mkc(0.0, 0.0);
   double ecur;
@@ -902,14 +937,17 @@ __global__ __launch_bounds__(64) void crt_fwdsweep_kernel(FwdArgs F) {
   auto enter_blk = [&](int b) __attribute__((always_inline)) {
     const cplx rc = crecip(dcur);
     nl = csel(64 * b + lane < n, mkc(-rc.re, -rc.im), mkc(0.0, 0.0));
-    ecur = bterm(64 * b + lane + 128);
+    ecur = bterm(64 * b + lane + WIN);
   };
   load_blk(b0, dcur, bcur);
   load_blk(b0 + 1, dnxt, bnxt);
-  cplx ya = bload(64 * b0 + lane), yb = bload(64 * b0 + 64 + lane);  // slots A and B
-  ya.re += bterm(64 * b0 + lane);
-  yb.re += bterm(64 * b0 + 64 + lane);
-  cplx lr[DEPTH][2];
+  cplx ys[NS];  // slot s: block b + s
+#pragma unroll
+  for (int q = 0; q < NS; ++q) {
+    ys[q] = bload(64 * (b0 + q) + lane);
+    ys[q].re += bterm(64 * (b0 + q) + lane);
+  }
+  cplx lr[DEPTH][NS];
   // in step order, as the loop refills them: the loop's wait for step j's
   // entries then counts the same loads behind them on entry as on the back edge
 #pragma unroll
@@ -936,31 +974,36 @@ __global__ __launch_bounds__(64) void crt_fwdsweep_kernel(FwdArgs F) {
 #pragma unroll
       for (int d = 0; d < DEPTH; ++d) {
         const int j = 64 * b + c + d;
-        const cplx w = rlc(cmul(ya, nl), j & 63);
-        ya = cmadd(ya, lr[d][0], w);
-        yb = cmadd(yb, lr[d][1], w);
-        // the updates are formed before lr[d] is refilled (otherwise they
-        // may sink below the refill, which then needs a copy and a wait)
-        asm volatile("" : "+v"(ya.re), "+v"(ya.im), "+v"(yb.re), "+v"(yb.im));
+        const cplx w = rlc(cmul(ys[0], nl), j & 63);
+#pragma unroll
+        for (int q = 0; q < NS; ++q) {
+          ys[q] = cmadd(ys[q], lr[d][q], w);
+          // the updates are formed before lr[d] is refilled (otherwise they
+          // may sink below the refill, which then needs a copy and a wait)
+          asm volatile("" : "+v"(ys[q].re), "+v"(ys[q].im));
+        }
         fetch(j + DEPTH, lr[d]);
         // the fetch stays here, DEPTH steps ahead of its use (the scheduler
         // would otherwise sink it next to the use)
         __builtin_amdgcn_sched_barrier(0);
         if ((d + 1) % G == 0) {
-          // rows j-G+1 .. j done: their lanes keep y and refill slot A
+          // rows j-G+1 .. j done: their lanes keep y and refill slot 0
           const bool g = (unsigned)(lane - ((j & 63) - (G - 1))) < (unsigned)G;
-          yo = csel(g, cmul(ya, nl), yo);
-          ya = csel(g, mkc(bcur.re + ecur, bcur.im), ya);
+          yo = csel(g, cmul(ys[0], nl), yo);
+          ys[0] = csel(g, mkc(bcur.re + ecur, bcur.im), ys[0]);
         }
       }
     }
     const int jo = 64 * b + lane;
     if (jo < n) yg[jo] = make_double2(-yo.re, -yo.im);
-    const cplx t = ya;
-    ya = yb;
-    yb = t;
+    const cplx t = ys[0];
+#pragma unroll
+    for (int q = 0; q + 1 < NS; ++q) ys[q] = ys[q + 1];
+    ys[NS - 1] = t;
   }
 }
 
-template __global__ void crt_fwdsweep_kernel<8, 8>(FwdArgs);
-template __global__ void crt_fwdsweep_kernel<8, 4>(FwdArgs);
+template __global__ void crt_fwdsweep_kernel<8, 8, 2>(FwdArgs);
+template __global__ void crt_fwdsweep_kernel<8, 4, 2>(FwdArgs);
+template __global__ void crt_fwdsweep_kernel<8, 8, 4>(FwdArgs);
+template __global__ void crt_fwdsweep_kernel<8, 4, 4>(FwdArgs);

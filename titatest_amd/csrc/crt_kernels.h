@@ -57,6 +57,7 @@ struct FactorArgs {
   const int* rb_last;       // n: last file position of a prescribed node, -1 otherwise
   const double2* rwd;       // file values
   double2* rhs;             // [nsys][n] scaled right-hand sides (crt_rhs_kernel), or null
+  double2* ring;            // [grid][half][mb+1] trailing-window rings in global memory (wide bands), or null
   int* err;                 // first error code (0 = ok) and the failing system
   unsigned long long* prof; // optional: 8 phase-time accumulators (wall clock ticks)
 };
@@ -88,12 +89,12 @@ __global__ void crt_area_values_kernel(FactorArgs A, int K, double2* av);
 __global__ void crt_scale_kernel(FactorArgs A);
 __global__ void crt_rows_kernel(FactorArgs A);
 __global__ void crt_rhs_kernel(FactorArgs A);
-template <int ROWS, int MAXT, int NT>
+template <int ROWS, int MAXT, int NT, bool GR>
 __global__ void crt_factor_kernel(FactorArgs A);
-template <int DEPTH, int G>
+template <int DEPTH, int G, int NS>
 __global__ void crt_backsolve_kernel(BacksolveArgs B);
 __global__ void crt_lcol_kernel(const double2* L, double2* Lc, int n, int mb, int nfac);
-template <int DEPTH, int G>
+template <int DEPTH, int G, int NS>
 __global__ void crt_fwdsweep_kernel(FwdArgs F);
 
 // LDS bytes of the factor kernel for bandwidth mb: the trailing-window rings,
@@ -102,3 +103,13 @@ inline size_t crt_factor_lds_bytes(int mb) {
   const size_t M1 = (size_t)mb + 1, half = M1 / 2, W = (size_t)mb + kNB;
   return (half * M1 + 2 * W * kNB + 3 * kNB * kMaxR + 4 * kNB) * 16 + 3 * kNB * kMaxR * 2;
 }
+// The same with the rings in global memory (the wide-band form): panels,
+// staged batches and broadcast slots only.
+inline size_t crt_factor_lds_bytes_gr(int mb) {
+  const size_t W = (size_t)mb + kNB;
+  return (2 * W * kNB + 3 * kNB * kMaxR + 4 * kNB) * 16 + 3 * kNB * kMaxR * 2;
+}
+// Widest band the factorisation and substitutions take: the wide-band factor
+// form holds the W = mb + kNB panel rows in four rows per lane, and the
+// substitutions' 256-row slot window needs 8-row groups below 257 - mb.
+constexpr int kMaxBand = 248;
