@@ -215,6 +215,32 @@ class Oracle:
         self.sigma = np.ascontiguousarray(sigma, np.complex128)
         return self.kpot, self.hpot
 
+    def forward_systems(self, sigma, systems, workers=None):
+        """kpot columns of the listed (l, k) systems only (one factorisation
+        each, as forward computes them), in parallel threads; the rest of
+        self.kpot stays zero.  Returns self.kpot (sanz, eanz, K)."""
+        import os
+        from concurrent.futures import ThreadPoolExecutor
+        n, E, K = self.mesh.sanz, self.mesh.eanz, self.K
+        kpot = np.zeros(n * E * K, np.complex128)
+        hpot = np.zeros(n * E, np.complex128)
+        sig = np.ascontiguousarray(sigma, np.complex128)
+        workers = workers or min(16, os.cpu_count() or 1)
+
+        def one(lk):
+            s = lk[1] * E + lk[0]
+            return lib().orc_forward(C.byref(self.cm), K, self.kwnwi, self.swrtr, self.elbg, self.relbg,
+                                     self.kg, self.lbeta, self.lrandb, sig, kpot, hpot, 1, s, s + 1)
+
+        with ThreadPoolExecutor(max_workers=workers) as ex:
+            errs = list(ex.map(one, list(systems)))
+        for e in errs:
+            if e:
+                raise OracleError("forward", e)
+        self.kpot = kpot.reshape((n, E, K), order="F")
+        self.sigma = sig
+        return self.kpot
+
     # --- A19: DC twins --------------------------------------------------------
     def assemble_dc(self, sigma, l, k):
         """kompadc + randdc + scaldc: the scaled real band matrix, rhs and fak."""

@@ -26,7 +26,7 @@ import numpy as np
 from . import crt as _crt
 from .meshgen import Workload, centroids
 
-__all__ = ["GNModel", "prepare", "gn_step", "gn_iterate", "run_batch", "rm_rect"]
+__all__ = ["GNModel", "prepare", "prepare_tri", "gn_step", "gn_iterate", "run_batch", "rm_rect", "rm_tri"]
 
 
 def rm_rect(smatm: np.ndarray, nx: int, m: np.ndarray) -> np.ndarray:
@@ -42,6 +42,20 @@ def rm_rect(smatm: np.ndarray, nx: int, m: np.ndarray) -> np.ndarray:
     return out
 
 
+def rm_tri(smatm: np.ndarray, nachbar: np.ndarray, m: np.ndarray) -> np.ndarray:
+    """R m for the triangular operator (ltri 1/2, src/update.f90:99-117): per
+    element the neighbour terms smatm(i,k) m(nachbar(i,k)) in edge order, then
+    the diagonal smatm(i, maxnodes+1) m(i).  nachbar (manz, maxnodes), 1-based,
+    0 = no neighbour."""
+    M, mx = nachbar.shape
+    out = np.zeros(M, np.complex128)
+    for k in range(mx):
+        nb = nachbar[:, k]
+        has = nb > 0
+        out[has] = out[has] + smatm[has, k] * m[nb[has] - 1]
+    return out + smatm[:, mx] * m
+
+
 @dataclass
 class GNModel:
     """One model of a batch, prepared once: handle, observed data, weights."""
@@ -51,8 +65,16 @@ class GNModel:
     nz: int
     dobs: np.ndarray       # observed -ln V
     wgt: np.ndarray        # wmatd * wdfak
-    smatm: np.ndarray      # bsmatmreg, manz x 3
+    smatm: np.ndarray      # bsmatmreg, manz x 3 (ltri 0), or bsmatmtri, manz x (maxnodes+1)
     rm: np.ndarray         # R ln(sigma)
+    ltri: int = 0          # 0: rectangular stencil, 1: triangular operator with nachbar
+    nachbar: np.ndarray = None  # (manz, maxnodes) edge neighbours for ltri 1
+
+    def rm_of(self, m: np.ndarray) -> np.ndarray:
+        return rm_rect(self.smatm, self.nx, m) if self.ltri == 0 else rm_tri(self.smatm, self.nachbar, m)
+
+    def rdiag(self) -> np.ndarray:
+        return self.smatm[:, 0] if self.ltri == 0 else self.smatm[:, -1]
 
 
 def prepare(w: Workload, nx: int, nz: int, seed: int = 0, alfx: float = 1.0, alfz: float = 1.0,
@@ -67,20 +89,41 @@ def prepare(w: Workload, nx: int, nz: int, seed: int = 0, alfx: float = 1.0, alf
     g.rwaven(w.strnr, w.vnr)
     g.precal()
     smatm = g.smoothness(0, alfx, alfz, nx=nx, nz=nz)
+    dobs, wgt = _observed(g, w, w.sigma, np.random.default_rng(seed), contrast)
+    rm = rm_rect(smatm, nx, np.log(w.sigma))
+    return GNModel(w, g, nx, nz, dobs, wgt, smatm, rm)
+
+
+def _observed(g, w, sigma, rng, contrast):
+    """Synthetic observed -ln V from a model perturbed by ``contrast`` in a
+    disc, with 2 % + 1e-4 Ohm Gaussian noise, and CRTomo's error-model weights
+    (src/rdati.f90:254,315-320)."""
     cx, cz = centroids(w.mesh)
-    rng = np.random.default_rng(seed)
     x0 = rng.uniform(cx.min(), cx.max())
     z0 = rng.uniform(cz.min() * 0.5, 0.0)
     blob = (cx - x0) ** 2 + (cz - z0) ** 2 < (0.2 * (cx.max() - cx.min())) ** 2
-    sig_true = w.sigma * np.where(blob, contrast, 1.0)
-    g.forward(sig_true)
+    g.forward(sigma * np.where(blob, contrast, 1.0))
     v, _ = g.bvolti(w.strnr, w.vnr)
     r = np.abs(v) * (1.0 + 0.02 * rng.standard_normal(len(v))) + 1e-4 * rng.standard_normal(len(v))
-    ph = np.angle(v)
-    dobs = -np.log(np.abs(r)) - 1j * ph
+    dobs = -np.log(np.abs(r)) - 1j * np.angle(v)
     wgt = 1.0 / (0.02 + 1e-4 / np.abs(v)) ** 2
-    rm = rm_rect(smatm, nx, np.log(w.sigma))
-    return GNModel(w, g, nx, nz, dobs, wgt, smatm, rm)
+    return dobs, wgt
+
+
+def prepare_tri(w: Workload, seed: int = 0, alfx: float = 1.0, alfz: float = 1.0, lsytop: bool = True,
+                contrast: float = 1.2) -> GNModel:
+    """As ``prepare`` for a triangular (or any unstructured) mesh: the
+    triangular smoothness operator of bsmatmtri with bnachbar's neighbours
+    (ltri = 1)."""
+    g = _crt.CRT(w.mesh, lsytop=lsytop)
+    g.rwaven(w.strnr, w.vnr)
+    g.precal()
+    smatm, nb = g.smoothness(1, alfx, alfz)
+    mx = w.mesh.maxnodes
+    nachbar = np.ascontiguousarray(nb[:, :mx])
+    dobs, wgt = _observed(g, w, w.sigma, np.random.default_rng(seed), contrast)
+    rm = rm_tri(smatm, nachbar, np.log(w.sigma))
+    return GNModel(w, g, 0, 0, dobs, wgt, smatm, rm, ltri=1, nachbar=nachbar)
 
 
 def gn_step(mod: GNModel, lam: float = 10.0, ncg: int = 20, use_graph: bool = False):
@@ -125,9 +168,9 @@ def gn_iterate(mod: GNModel, iters: int = 3, lam: float = 10.0, ncg: int = 20, s
         if it == iters:
             break
         g.sensitivities(1, to_host=False)
-        rm = rm_rect(mod.smatm, mod.nx, np.log(sigma))
-        dpar, _, _ = g.cg_solve(mod.wgt, mod.dobs, sa, lam, 0, mod.smatm, mod.smatm[:, 0], ncg, eps=eps,
-                                nx=mod.nx, rm=rm)
+        rm = mod.rm_of(np.log(sigma))
+        dpar, _, _ = g.cg_solve(mod.wgt, mod.dobs, sa, lam, mod.ltri, mod.smatm, mod.rdiag(), ncg, eps=eps,
+                                nx=mod.nx, nachbar=mod.nachbar, rm=rm)
         par = g.bpar(sigma)
         par, _, _ = g.update_step(par, dpar, step, lphi0)
         sigma = g.bsigma(par)

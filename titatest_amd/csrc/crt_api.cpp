@@ -5,6 +5,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <mutex>
@@ -140,11 +141,20 @@ struct crt_handle {
 
 namespace {
 
+// With CRT_VERBOSE_ERRORS set in the environment, a failing HIP call is
+// reported on stderr (the call, HIP's error string, the source line) before
+// the entry point returns 1001.
+void report_hip(const char* what, hipError_t e, int line) {
+  static const bool on = std::getenv("CRT_VERBOSE_ERRORS") != nullptr;
+  if (on) std::fprintf(stderr, "crt: %s failed: %s (crt_api.cpp:%d)\n", what, hipGetErrorString(e), line);
+}
+
 #define HIPCHK(x)                          \
   do {                                     \
     hipError_t e_ = (x);                   \
     if (e_ != hipSuccess) {                \
       h->hip_err = e_;                     \
+      report_hip(#x, e_, __LINE__);        \
       return kErrHip;                      \
     }                                      \
   } while (0)
@@ -200,14 +210,26 @@ int finish(crt_handle* h) {
   tcollect(h);
   if (e != hipSuccess) {
     h->hip_err = e;
+    report_hip("hipStreamSynchronize", e, __LINE__);
     return kErrHip;
   }
   e = hipGetLastError();
   if (e != hipSuccess) {
     h->hip_err = e;
+    report_hip("hipGetLastError", e, __LINE__);
     return kErrHip;
   }
   return 0;
+}
+
+// Device-to-host copy on the handle's stream, waited for.  A synchronous
+// hipMemcpy goes through the legacy default stream, which HIP refuses while
+// another thread captures a graph on its handle's stream -- and the refusal
+// invalidates that capture -- so the entry points never use one.
+hipError_t d2h(crt_handle* h, void* dst, const void* src, size_t bytes) {
+  hipError_t e = hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, h->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+  return e;
 }
 
 int read_err(crt_handle* h) {
@@ -800,20 +822,20 @@ int crt_get_precal(crt_handle* h, double* elbg, double* relbg, double* kg) {
   const int K = h->K, el = h->elanz, rl = h->relanz, nimn = h->nimn, E = h->E;
   if (elbg) {
     std::vector<double> t((size_t)K * el * nimn);
-    HIPCHK(hipMemcpy(t.data(), h->at<double>("elbg"), t.size() * 8, hipMemcpyDeviceToHost));
+    HIPCHK(d2h(h, t.data(), h->at<double>("elbg"), t.size() * 8));
     for (int k = 0; k < K; ++k)
       for (int e = 0; e < el; ++e)
         for (int q = 0; q < nimn; ++q) elbg[e + (size_t)el * (q + (size_t)nimn * k)] = t[((size_t)k * el + e) * nimn + q];
   }
   if (relbg && rl > 0) {
     std::vector<double> t((size_t)rl * nimn);
-    HIPCHK(hipMemcpy(t.data(), h->at<double>("relbg"), t.size() * 8, hipMemcpyDeviceToHost));
+    HIPCHK(d2h(h, t.data(), h->at<double>("relbg"), t.size() * 8));
     for (int r = 0; r < rl; ++r)
       for (int q = 0; q < nimn; ++q) relbg[r + (size_t)rl * q] = t[(size_t)r * nimn + q];
   }
   if (kg && rl > 0) {
     // device layout [K][E][relanz] is the reference's kg(relanz,eanz,kwnanz)
-    HIPCHK(hipMemcpy(kg, h->at<double>("kg"), (size_t)K * E * rl * 8, hipMemcpyDeviceToHost));
+    HIPCHK(d2h(h, kg, h->at<double>("kg"), (size_t)K * E * rl * 8));
   }
   return 0;
 }
@@ -1112,7 +1134,7 @@ static int run_forward(crt_handle* h) {
   if (e) return e;
   if (h->kprof) {
     unsigned long long t[8];
-    HIPCHK(hipMemcpy(t, A.prof, sizeof(t), hipMemcpyDeviceToHost));
+    HIPCHK(d2h(h, t, A.prof, sizeof(t)));
     for (int i = 0; i < 8; ++i) h->kprof_ticks[i] += t[i];
   }
   h->forward_done = true;
@@ -1130,7 +1152,7 @@ int crt_forward(crt_handle* h, const double* sigma, double* kpot, double* hpot) 
   int rc = run_forward(h);
   if (rc) return rc;
   const int n = h->n, E = h->E, K = h->K;
-  if (kpot) HIPCHK(hipMemcpy(kpot, h->at<double2>("kpot"), (size_t)K * E * n * 16, hipMemcpyDeviceToHost));
+  if (kpot) HIPCHK(d2h(h, kpot, h->at<double2>("kpot"), (size_t)K * E * n * 16));
   if (hpot) {
     double2* dh = h->get<double2>("hpot", (size_t)E * n);
     NEED(dh);
@@ -1141,7 +1163,7 @@ int crt_forward(crt_handle* h, const double* sigma, double* kpot, double* hpot) 
     tstop(h);
     rc = finish(h);
     if (rc) return rc;
-    HIPCHK(hipMemcpy(hpot, dh, (size_t)E * n * 16, hipMemcpyDeviceToHost));
+    HIPCHK(d2h(h, hpot, dh, (size_t)E * n * 16));
   }
   return 0;
 }
@@ -1164,7 +1186,7 @@ static int copy_real(crt_handle* h, const double2* src, size_t n, double* host) 
   hipLaunchKernelGGL(crt_real_part_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, h->stream, src, n, d);
   int rc = finish(h);
   if (rc) return rc;
-  HIPCHK(hipMemcpy(host, d, n * 8, hipMemcpyDeviceToHost));
+  HIPCHK(d2h(h, host, d, n * 8));
   return 0;
 }
 
@@ -1241,8 +1263,8 @@ int crt_voltages(crt_handle* h, int32_t nanz, const int32_t* strnr, const int32_
   if (!h) return kErrArg;
   int rc = run_voltages(h, nanz, strnr, vnr, mode, kfak);
   if (rc) return rc;
-  if (volt) HIPCHK(hipMemcpy(volt, h->at<double2>("volt"), (size_t)nanz * 16, hipMemcpyDeviceToHost));
-  if (sigmaa) HIPCHK(hipMemcpy(sigmaa, h->at<double2>("sigmaa"), (size_t)nanz * 16, hipMemcpyDeviceToHost));
+  if (volt) HIPCHK(d2h(h, volt, h->at<double2>("volt"), (size_t)nanz * 16));
+  if (sigmaa) HIPCHK(d2h(h, sigmaa, h->at<double2>("sigmaa"), (size_t)nanz * 16));
   return 0;
 }
 
@@ -1367,7 +1389,7 @@ int crt_sensitivities(crt_handle* h, int32_t mode, double* sens) {
   if (rc) return rc;
   h->sens_done = true;
   h->sens_mode = mode;
-  if (sens) HIPCHK(hipMemcpy(sens, h->at<double2>("sens"), (size_t)h->nanz * h->elanz * 16, hipMemcpyDeviceToHost));
+  if (sens) HIPCHK(d2h(h, sens, h->at<double2>("sens"), (size_t)h->nanz * h->elanz * 16));
   return 0;
 }
 
@@ -1473,7 +1495,7 @@ int crt_adjoint_gradient(crt_handle* h, int32_t mode, const double* w, const dou
   NEED(dg);
   rc = run_adjoint(h, mode, dw, dr, dg);
   if (rc) return rc;
-  HIPCHK(hipMemcpy(grad, dg, (size_t)h->elanz * 16, hipMemcpyDeviceToHost));
+  HIPCHK(d2h(h, grad, dg, (size_t)h->elanz * 16));
   return 0;
 }
 
@@ -1502,7 +1524,7 @@ int crt_jhw_r(crt_handle* h, const double* w, const double* resid, double* grad)
   tstop(h);
   rc = finish(h);
   if (rc) return rc;
-  HIPCHK(hipMemcpy(grad, dg, (size_t)h->elanz * 16, hipMemcpyDeviceToHost));
+  HIPCHK(d2h(h, grad, dg, (size_t)h->elanz * 16));
   return 0;
 }
 
@@ -1623,11 +1645,11 @@ int crt_cg_solve(crt_handle* h, const double* w, const double* dat, const double
   if (rc) return rc;
   std::vector<double> cf(M), sc(8);
   std::vector<double> dp(2 * (size_t)M);
-  HIPCHK(hipMemcpy(cf.data(), G.cgfac, M * 8, hipMemcpyDeviceToHost));
-  HIPCHK(hipMemcpy(dp.data(), G.dpar, (size_t)M * 16, hipMemcpyDeviceToHost));
-  HIPCHK(hipMemcpy(sc.data(), G.scal, 64, hipMemcpyDeviceToHost));
+  HIPCHK(d2h(h, cf.data(), G.cgfac, M * 8));
+  HIPCHK(d2h(h, dp.data(), G.dpar, (size_t)M * 16));
+  HIPCHK(d2h(h, sc.data(), G.scal, 64));
   if (cgres) {
-    HIPCHK(hipMemcpy(cgres, G.cgres, (ncgmax + 2) * sizeof(double), hipMemcpyDeviceToHost));
+    HIPCHK(d2h(h, cgres, G.cgres, (ncgmax + 2) * sizeof(double)));
     cgres[0] = sc[6];
   }
   // dpar = dpar * cgfac (src/update.f90:307)
@@ -1924,11 +1946,11 @@ int crt_dmisft(crt_handle* h, int32_t nanz, const double* dat, const double* sig
   rc = finish(h);
   if (rc) return rc;
   double st[6];
-  HIPCHK(hipMemcpy(st, dst, sizeof(st), hipMemcpyDeviceToHost));
+  HIPCHK(d2h(h, st, dst, sizeof(st)));
   if (st[5] == (double)nanz) return 99;  // dmisft :125-130: every datum excluded
   std::memcpy(stats, st, sizeof(st));
-  HIPCHK(hipMemcpy(wmatd, dw, (size_t)nanz * 8, hipMemcpyDeviceToHost));
-  if (wdlok) HIPCHK(hipMemcpy(wdlok, dwd, (size_t)nanz * 4, hipMemcpyDeviceToHost));
+  HIPCHK(d2h(h, wmatd, dw, (size_t)nanz * 8));
+  if (wdlok) HIPCHK(d2h(h, wdlok, dwd, (size_t)nanz * 4));
   return 0;
 }
 
@@ -1986,7 +2008,7 @@ int crt_bpar(crt_handle* h, const double* sigma, double* par) {
   if (rc) return rc;
   rc = finish(h);
   if (rc) return rc;
-  HIPCHK(hipMemcpy(par, dp, (size_t)manz * 16, hipMemcpyDeviceToHost));
+  HIPCHK(d2h(h, par, dp, (size_t)manz * 16));
   return 0;
 }
 
@@ -2001,7 +2023,7 @@ int crt_bsigma(crt_handle* h, const double* par, double* sigma) {
   if (rc) return rc;
   rc = finish(h);
   if (rc) return rc;
-  HIPCHK(hipMemcpy(sigma, ds, (size_t)h->elanz * 16, hipMemcpyDeviceToHost));
+  HIPCHK(d2h(h, sigma, ds, (size_t)h->elanz * 16));
   return 0;
 }
 
@@ -2028,8 +2050,8 @@ int crt_update_step(crt_handle* h, int32_t manz, double* par, const double* dpar
   if (rc) return rc;
   rc = finish(h);
   if (rc) return rc;
-  HIPCHK(hipMemcpy(par, dp, (size_t)manz * 16, hipMemcpyDeviceToHost));
-  HIPCHK(hipMemcpy(out, dout, 16, hipMemcpyDeviceToHost));
+  HIPCHK(d2h(h, par, dp, (size_t)manz * 16));
+  HIPCHK(d2h(h, out, dout, 16));
   return 0;
 }
 
@@ -2061,7 +2083,7 @@ int crt_blam0(crt_handle* h, const double* w, int32_t mode, double alfx, double 
   int rc = finish(h);
   if (rc) return rc;
   std::vector<double2> th(M);
-  HIPCHK(hipMemcpy(th.data(), t, (size_t)M * 16, hipMemcpyDeviceToHost));
+  HIPCHK(d2h(h, th.data(), t, (size_t)M * 16));
   double sum = 0.0;
   for (int j = 0; j < M; ++j) sum += mode ? std::fabs(th[j].x) : std::hypot(th[j].x, th[j].y);
   *lammax = sum / (double)M * 2.0 / (alfx + alfz) * 5.0;
