@@ -85,3 +85,76 @@ def test_large_workload_shape():
     lr = np.log10(w.mag)
     assert 1.0 < lr.mean() < 3.0
     assert w.pha_mrad.min() >= -30.0 - 1e-9 and w.pha_mrad.max() <= -2.0 + 1e-9
+
+
+REFERENCE_VOLT_FILES = ["minimal", "decoupling", "rect_complex", "non_ordered"]
+
+
+@pytest.mark.parametrize("name", REFERENCE_VOLT_FILES)
+def test_list_directed_writer_reproduces_reference_files(name):
+    """The shipped volt.dat files (written by the reference's wdatm through
+    gfortran's list-directed output) re-emitted record by record from their
+    own values: byte for byte.  (tests/2.5D_mixed_surf's file comes from
+    another compiler's list-directed layout and is left out.)"""
+    from conftest import FIXTURES
+    lines = (FIXTURES / name / "volt.dat").read_text().splitlines()
+    assert lines[0] == F.list_int(int(lines[0]))
+    for ln in lines[1:]:
+        t = ln.split()
+        assert ln == F.list_int(int(t[0])) + F.list_int(int(t[1])) + F.list_real4(float(t[2])) + \
+            F.list_real4(float(t[3])), ln
+
+
+def test_write_volt_reference_layout(tmp_path):
+    """write_volt(reference=True): the FP32 values of |R| and the phase in
+    mrad in the list-directed layout, the polarity swap applied, readable."""
+    rng = np.random.default_rng(4)
+    n = 50
+    st = (rng.integers(1, 50, n) * 10000 + rng.integers(1, 50, n)).astype(np.int32)
+    vn = (rng.integers(1, 50, n) * 10000 + rng.integers(1, 50, n)).astype(np.int32)
+    sa = 10.0 ** rng.uniform(-4, 2, n) * np.exp(1j * rng.uniform(-3.1, 3.1, n))
+    F.write_volt(tmp_path / "v.dat", st, vn, sa, reference=True)
+    lines = (tmp_path / "v.dat").read_text().splitlines()
+    assert lines[0] == f"{n:12d}" and all(len(ln) == 58 for ln in lines[1:])
+    bet, pha, vout = F.volt_fields(st, vn, sa)
+    s2, v2, m2, p2 = F.read_volt(tmp_path / "v.dat")
+    np.testing.assert_array_equal(s2, st)
+    np.testing.assert_array_equal(v2, vout)
+    # nine significant digits identify each FP32 value
+    np.testing.assert_array_equal(m2.astype(np.float32), bet.astype(np.float32))
+    np.testing.assert_array_equal(p2.astype(np.float32), pha.astype(np.float32))
+
+
+def test_fortran_host_formats_like_the_reference(tmp_path):
+    """The Fortran host's list-directed REAL(4) formatter (crmod_hip
+    --format-test, no device involved) against formats.list_real4 on
+    decade boundaries, zero, the range ends and 3,000 seeded values."""
+    import subprocess
+    from titatest_amd import crt
+    exe = crt.LIB_PATH.parent / "crmod_hip"
+    if not exe.exists():
+        pytest.skip("no Fortran compiler in this environment")
+    rng = np.random.default_rng(3)
+    vals = [0.0, 0.1, -0.1, 0.09999999, 0.099999994, 1e9, 999999999.0, 999999940.0, 1e-38, 3e38, 1.0,
+            -1.0, 0.5, 123456789.0, 12345678.9, 9.99999999, 99.9999999]
+    vals += list(10.0 ** rng.uniform(-12, 12, 3000) * rng.choice([-1, 1], 3000))
+    vals = [float(np.float32(v)) for v in vals]
+    (tmp_path / "in.txt").write_text("".join(repr(v) + "\n" for v in vals))
+    r = subprocess.run([str(exe), "--format-test", str(tmp_path / "in.txt"), str(tmp_path / "out.txt")],
+                       capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0, r.stdout + r.stderr
+    out = (tmp_path / "out.txt").read_text().splitlines()
+    assert len(out) == len(vals)
+    for v, got in zip(vals, out):
+        assert got == F.list_real4(v), (v, got)
+
+
+@pytest.mark.parametrize("name", ["minimal", "decoupling", "rect_complex"])
+def test_reference_values_in_reference_layout_are_the_shipped_file(name, tmp_path):
+    """The full-precision reference (golden vectors of the reference's routines
+    compiled from source) written in wdatm's layout is the shipped volt.dat,
+    byte for byte: the drop-in output a byte-level diff can compare."""
+    from conftest import golden
+    st, vn, _, _ = F.read_volt(FIXTURES / name / "volt.dat")
+    F.write_volt(tmp_path / "v.dat", st, vn, golden(name)["sigmaa0"], reference=True)
+    assert (tmp_path / "v.dat").read_text() == (FIXTURES / name / "volt.dat").read_text()

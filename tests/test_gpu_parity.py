@@ -535,14 +535,23 @@ def test_fortran_host_crmod_fixture(crtmod, tmp_path):
     d = FIXTURES / "minimal"
     F.write_config(tmp_path / "config.dat", st, vn)
     say("fortran host minimal")
-    r = subprocess.run([str(exe), str(d / "elem.dat"), str(d / "elec.dat"), str(d / "rho.dat"),
-                        str(tmp_path / "config.dat"), str(tmp_path / "volt.dat")],
-                       capture_output=True, text=True, timeout=120, cwd=tmp_path)
+    args = [str(exe), str(d / "elem.dat"), str(d / "elec.dat"), str(d / "rho.dat"),
+            str(tmp_path / "config.dat"), str(tmp_path / "volt.dat")]
+    # the reference's own bytes: FP32 list-directed, as wdatm writes them
+    r = subprocess.run(args, capture_output=True, text=True, timeout=120, cwd=tmp_path)
+    assert r.returncode == 0, r.stdout + r.stderr
+    ge = golden("minimal")
+    F.write_volt(tmp_path / "ref.dat", st, vn, ge["sigmaa0"], reference=True)
+    assert (tmp_path / "volt.dat").read_text() == (tmp_path / "ref.dat").read_text()
+    assert (tmp_path / "volt.dat").read_text() == (d / "volt.dat").read_text()
+    # full precision on request
+    import os
+    env = dict(os.environ, CRMOD_HIP_FULL_PRECISION="1")
+    r = subprocess.run(args, capture_output=True, text=True, timeout=120, cwd=tmp_path, env=env)
     assert r.returncode == 0, r.stdout + r.stderr
     s2, v2, mag2, pha2 = F.read_volt(tmp_path / "volt.dat")
-    ge = golden("minimal")["volt"]
     np.testing.assert_array_equal(s2, st)
-    assert np.max(np.abs(mag2 - np.abs(ge)) / np.abs(ge)) < VTOL
+    assert np.max(np.abs(mag2 - np.abs(ge["volt"])) / np.abs(ge["volt"])) < VTOL
     assert np.max(np.abs(mag2 - mag) / mag) < 2e-7
 
 

@@ -298,8 +298,40 @@ def volt_fields(strnr, vnr, sigmaa):
     return bet, 1e3 * pha, vout
 
 
-def write_volt(path, strnr, vnr, sigmaa, full_precision=True) -> None:
+def list_int(i: int) -> str:
+    """An integer as the reference's list-directed WRITE puts it in its files
+    (gfortran: a 12-character field, right-justified)."""
+    return f"{int(i):12d}"
+
+
+def list_real4(x) -> str:
+    """A REAL(4) value as the reference's list-directed WRITE puts it in its
+    files (gfortran, src/wdatm.f90:76): a blank, then G editing with 9
+    significant digits in 16 characters -- F form with four trailing blanks
+    when the value, rounded to 9 digits, lies in [0.1, 1e9) (zero included),
+    otherwise 1PE16.8E2."""
+    v = float(np.float32(x))
+    if v == 0.0:
+        return " " + f"{v:.8f}".rjust(12) + "    "
+    ex = int(f"{abs(v):.8e}".split("e")[1])
+    if -1 <= ex < 9:
+        # F editing keeps the decimal point when no digit follows it
+        txt = f"{v:.{8 - ex}f}" + ("." if ex == 8 else "")
+        return " " + txt.rjust(12) + "    "
+    return " " + f"{v:.8E}".rjust(16)
+
+
+def write_volt(path, strnr, vnr, sigmaa, full_precision=True, reference=False) -> None:
+    """volt.dat: nanz, then strnr vnr |R| phase[mrad] per measurement (with
+    wdatm's polarity swap).  reference=True writes the reference's own bytes
+    (FP32 values in list-directed form, src/wdatm.f90:55-79); otherwise
+    full_precision chooses 17 or 9 significant digits in E form."""
     bet, pha_mrad, vout = volt_fields(strnr, vnr, sigmaa)
+    if reference:
+        body = "".join(list_int(a) + list_int(b) + list_real4(m) + list_real4(p) + "\n"
+                       for a, b, m, p in zip(strnr, vout, bet, pha_mrad))
+        Path(path).write_text(list_int(len(strnr)) + "\n" + body)
+        return
     fmt = "{:d} {:d} {:.17e} {:.17e}\n" if full_precision else "{:d} {:d} {:.8e} {:.8e}\n"
     if not full_precision:
         bet = bet.astype(np.float32).astype(np.float64)

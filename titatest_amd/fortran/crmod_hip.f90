@@ -1,9 +1,12 @@
 !> CRMod-style forward modelling on the MI355X: reads a tomodir's elem.dat,
 !> elec.dat, rho.dat and config.dat, runs the forward path through the C-ABI
-!> and writes volt.dat the way the reference's wdatm does (src/wdatm.f90:55-79),
-!> at full precision.
+!> and writes volt.dat the way the reference's wdatm does (src/wdatm.f90:55-79):
+!> FP32 values in the list-directed layout of the reference's own files, or
+!> 17 significant digits when CRMOD_HIP_FULL_PRECISION is set.
 !>
 !>   crmod_hip elem.dat elec.dat rho.dat config.dat volt.dat [swrtr [nsink [sens.dat]]]
+!>   crmod_hip --format-test in.txt out.txt   (host only: format the REAL(4)
+!>                                             values of in.txt, one per line)
 !>
 !> swrtr = 1 (default) is the 2.5D forward; 0 is 2-D (one wavenumber 0).
 !> nsink > 0 adds the fictitious sink at that node (crmod.cfg lsink/nsink).
@@ -32,7 +35,17 @@ program crmod_hip
   character(len=512) :: f_elem, f_elec, f_rho, f_cfg, f_volt, f_sens, arg
   integer :: i, j, k, idum, e1, e2, ios, rc, ie1, ie2, ic, jk1, jk2, l
   real(c_double) :: rb, rp
+  logical :: full
 
+  if (command_argument_count() == 3) then
+     call get_command_argument(1, arg)
+     if (trim(arg) == '--format-test') then
+        call get_command_argument(2, f_elem)
+        call get_command_argument(3, f_volt)
+        call format_test(trim(f_elem), trim(f_volt))
+        stop
+     end if
+  end if
   if (command_argument_count() < 5) then
      write (*, '(a)') 'usage: crmod_hip elem.dat elec.dat rho.dat config.dat volt.dat [swrtr [nsink [sens.dat]]]'
      stop 1
@@ -181,8 +194,14 @@ program crmod_hip
   rc = crt_destroy(h)
 
   ! ---- volt.dat as wdatm writes it (polarity swap when the phase is near pi)
+  call get_environment_variable('CRMOD_HIP_FULL_PRECISION', arg, status=ios)
+  full = ios == 0
   open (14, file=trim(f_volt), status='replace', action='write')
-  write (14, *) nanz
+  if (full) then
+     write (14, *) nanz
+  else
+     write (14, '(a)') list_int(nanz)
+  end if
   do i = 1, nanz
      bet = abs(sigmaa(i))
      pha = atan2(dimag(sigmaa(i)), dble(sigmaa(i)))
@@ -193,12 +212,74 @@ program crmod_hip
         ie2 = (vnr(i) - ie1) / 10000
         vnr(i) = ie1 * 10000 + ie2
      end if
-     write (14, '(2I12,2ES26.17)') strnr(i), vnr(i), bet, 1d3 * pha
+     if (full) then
+        write (14, '(2I12,2ES26.17)') strnr(i), vnr(i), bet, 1d3 * pha
+     else
+        write (14, '(a)') list_int(strnr(i)) // list_int(vnr(i)) // list_real4(real(bet)) // &
+             list_real4(real(1d3 * pha))
+     end if
   end do
   close (14)
   ic = 0
 
 contains
+
+  ! An integer as the reference's list-directed WRITE puts it in its files
+  ! (gfortran: a 12-character field, right-justified).
+  function list_int(i) result(out)
+    integer(c_int32_t), intent(in) :: i
+    character(len=12) :: out
+    write (out, '(I12)') i
+  end function list_int
+
+  ! A REAL(4) value as the reference's list-directed WRITE puts it in its
+  ! files (src/wdatm.f90:76 built with gfortran): a blank, then G editing
+  ! with 9 significant digits in 16 characters -- F form with four trailing
+  ! blanks when the value, rounded to 9 digits, lies in [0.1, 1e9) (zero
+  ! included), otherwise 1PE16.8E2.
+  function list_real4(x) result(out)
+    real(4), intent(in) :: x
+    character(len=17) :: out
+    character(len=20) :: es
+    character(len=16) :: fb, fmt
+    integer :: ex, lead
+    if (x == 0.0) then
+       ex = 0
+    else
+       write (es, '(ES20.8E3)') x
+       read (es(len_trim(es) - 3:len_trim(es)), *) ex
+    end if
+    if (ex >= -1 .and. ex < 9) then
+       write (fmt, '(a,i0,a)') '(F12.', 8 - ex, ')'
+       write (fb, fmt) x
+       ! F editing may drop the zero before the decimal point of |x| < 1
+       lead = index(fb(1:12), '.')
+       if (lead >= 2) then
+          if (fb(lead - 1:lead - 1) == ' ' .or. fb(lead - 1:lead - 1) == '-') then
+             fb(1:lead - 1) = fb(2:lead - 1) // '0'
+          end if
+       end if
+       out = ' ' // fb(1:12) // '    '
+    else
+       write (fb, '(1PE16.8E2)') x
+       out = ' ' // fb(1:16)
+    end if
+  end function list_real4
+
+  subroutine format_test(fin, fout)
+    character(len=*), intent(in) :: fin, fout
+    real(4) :: v
+    integer :: st
+    open (21, file=fin, status='old', action='read')
+    open (22, file=fout, status='replace', action='write')
+    do
+       read (21, *, iostat=st) v
+       if (st /= 0) exit
+       write (22, '(a)') list_real4(v)
+    end do
+    close (21)
+    close (22)
+  end subroutine format_test
 
   ! wsens (src/wsens.f90:40-98): one file per measurement, the centroids as
   ! relem sets them (src/relem.f90:134-143)
