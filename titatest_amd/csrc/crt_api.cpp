@@ -899,20 +899,37 @@ static int run_forward(crt_handle* h) {
   const int nrest = tw > 2 ? ((tw - 2) * (tw - 1)) / 2 : 0;
   const int nt = gring || 2 * lds > h->lds_max ? 512 : 256;
   const int rows = (W + 63) / 64, maxt = std::max(1, (nrest + nt - 65) / (nt - 64));
+  // The look-ahead and trailing updates run on the vector ALU (4 x 4 and
+  // 2 x 2 tiles, one product at a time).  CRT_FACTOR_MFMA selects the form
+  // with 16 x 16 tiles on the matrix cores instead: as accurate, but slower on
+  // gfx950, whose FP64 matrix rate is below its FP64 vector rate.
+  static const bool mfma = std::getenv("CRT_FACTOR_MFMA") != nullptr;
   void (*kern)(FactorArgs) = nullptr;
-  if (gring) {
-    if (rows <= 3 && maxt <= 5) kern = crt_factor_kernel<3, 5, 512, true>;
-    else if (rows == 4 && maxt <= 5) kern = crt_factor_kernel<4, 5, 512, true>;
+  if (mfma) {
+    if (gring) {
+      if (rows <= 3) kern = crt_factor_kernel<3, 1, 512, true, true>;
+      else if (rows == 4) kern = crt_factor_kernel<4, 1, 512, true, true>;
+    } else if (nt == 256) {
+      if (rows == 1) kern = crt_factor_kernel<1, 1, 256, false, true>;
+      else if (rows == 2) kern = crt_factor_kernel<2, 1, 256, false, true>;
+      else if (rows == 3) kern = crt_factor_kernel<3, 1, 256, false, true>;
+    } else {
+      if (rows <= 2) kern = crt_factor_kernel<2, 1, 512, false, true>;
+      else if (rows == 3) kern = crt_factor_kernel<3, 1, 512, false, true>;
+    }
+  } else if (gring) {
+    if (rows <= 3 && maxt <= 5) kern = crt_factor_kernel<3, 5, 512, true, false>;
+    else if (rows == 4 && maxt <= 5) kern = crt_factor_kernel<4, 5, 512, true, false>;
   } else if (nt == 256) {
-    if (rows == 1 && maxt == 1) kern = crt_factor_kernel<1, 1, 256, false>;
-    else if (rows == 2 && maxt == 1) kern = crt_factor_kernel<2, 1, 256, false>;
-    else if (rows == 2 && maxt == 2) kern = crt_factor_kernel<2, 2, 256, false>;
-    else if (rows == 2 && maxt == 3) kern = crt_factor_kernel<2, 3, 256, false>;
-    else if (rows == 3 && maxt <= 3) kern = crt_factor_kernel<3, 3, 256, false>;
+    if (rows == 1 && maxt == 1) kern = crt_factor_kernel<1, 1, 256, false, false>;
+    else if (rows == 2 && maxt == 1) kern = crt_factor_kernel<2, 1, 256, false, false>;
+    else if (rows == 2 && maxt == 2) kern = crt_factor_kernel<2, 2, 256, false, false>;
+    else if (rows == 2 && maxt == 3) kern = crt_factor_kernel<2, 3, 256, false, false>;
+    else if (rows == 3 && maxt <= 3) kern = crt_factor_kernel<3, 3, 256, false, false>;
   } else {
-    if (rows <= 2 && maxt == 1) kern = crt_factor_kernel<2, 1, 512, false>;
-    else if (rows <= 2 && maxt == 2) kern = crt_factor_kernel<2, 2, 512, false>;
-    else if (rows == 3 && maxt <= 2) kern = crt_factor_kernel<3, 2, 512, false>;
+    if (rows <= 2 && maxt == 1) kern = crt_factor_kernel<2, 1, 512, false, false>;
+    else if (rows <= 2 && maxt == 2) kern = crt_factor_kernel<2, 2, 512, false, false>;
+    else if (rows == 3 && maxt <= 2) kern = crt_factor_kernel<3, 2, 512, false, false>;
   }
   if (!kern) return kErrBand;
   if (int rc = open_lds(h, reinterpret_cast<const void*>(kern))) return rc;

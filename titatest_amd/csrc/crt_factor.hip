@@ -220,6 +220,53 @@ __device__ __forceinline__ void tile_update(const Stage& C, const cplx* Tt, int 
   }
 }
 
+typedef double v4d __attribute__((ext_vector_type(4)));
+
+// One 16 x 16 tile of the stage window on the matrix cores: window rows
+// 16 TI + (lane >> 4) + 4 v (v < 4) and column 16 TJ + (lane & 15) -- the
+// v_mfma_f64_16x16x4f64 accumulator layout.  out[v] = S - P with S the
+// current value (ring, or the staged batch for the window's last kNB rows; 0
+// outside the window's lower triangle) and P = sum_k L_rk L_ck over panel
+// J-1's kNB columns, formed from zero: the real part as the L_re L_re
+// products over k, then the -L_im L_im ones, the imaginary part as L_re L_im
+// then L_im L_re, each chain accumulated by the matrix core, and subtracted
+// from S once.  (A product block formed first and subtracted once is as
+// close to the exact solution as the reference's one-product-at-a-time
+// update on the worst-conditioned systems; accumulating the products into S
+// itself is not.)  Every lane takes part (the matrix cores need the whole
+// wavefront); the caller masks the stores.
+__device__ __forceinline__ void mfma_tile(const Stage& C, const cplx* Tt, int TI, int TJ, int lane, cplx out[4]) {
+  const int col = lane & 15, q = lane >> 4;
+  const int cw = 16 * TJ + col;
+#pragma unroll
+  for (int v = 0; v < 4; ++v) {
+    const int rw = 16 * TI + q + 4 * v;
+    cplx sv = mkc(0.0, 0.0);
+    if (tile_ok(C, rw, cw)) {
+      const int ib = rw - (C.mb - kNB);
+      sv = ib >= 0 ? staged(C.sv_old, C.so_old, ib, cw - rw + C.mb) : Tt[C.ring.pos(rw, cw)];
+    }
+    out[v] = sv;
+  }
+  if (!C.apply) return;
+  // operand rows of panel J-1 (stage rows + kNB), clamped for the padding
+  // rows of the last tile row (their results are not stored)
+  const int ra = min(16 * TI + col + kNB, C.W - 1), ca = min(cw + kNB, C.W - 1);
+  const cplx a0 = C.Pp[pix(ra, q)], a1 = C.Pp[pix(ra, q + 4)];
+  const cplx b0 = C.Pp[pix(ca, q)], b1 = C.Pp[pix(ca, q + 4)];
+  v4d pr = {0.0, 0.0, 0.0, 0.0}, pi = {0.0, 0.0, 0.0, 0.0};
+  pr = __builtin_amdgcn_mfma_f64_16x16x4f64(a0.re, b0.re, pr, 0, 0, 0);
+  pr = __builtin_amdgcn_mfma_f64_16x16x4f64(a1.re, b1.re, pr, 0, 0, 0);
+  pr = __builtin_amdgcn_mfma_f64_16x16x4f64(-a0.im, b0.im, pr, 0, 0, 0);
+  pr = __builtin_amdgcn_mfma_f64_16x16x4f64(-a1.im, b1.im, pr, 0, 0, 0);
+  pi = __builtin_amdgcn_mfma_f64_16x16x4f64(a0.re, b0.im, pi, 0, 0, 0);
+  pi = __builtin_amdgcn_mfma_f64_16x16x4f64(a1.re, b1.im, pi, 0, 0, 0);
+  pi = __builtin_amdgcn_mfma_f64_16x16x4f64(a0.im, b0.re, pi, 0, 0, 0);
+  pi = __builtin_amdgcn_mfma_f64_16x16x4f64(a1.im, b1.re, pi, 0, 0, 0);
+#pragma unroll
+  for (int v = 0; v < 4; ++v) out[v] = mkc(out[v].re - pr[v], out[v].im - pi[v]);
+}
+
 // lower-triangle tile index x -> (ti, tj), x = ti (ti + 1) / 2 + tj
 __device__ __forceinline__ void tri_tile(int x, int& ti, int& tj) {
   int i = (int)((sqrt(8.0 * x + 1.0) - 1.0) * 0.5);
@@ -369,8 +416,11 @@ __device__ __forceinline__ void stage_barrier() {
 // update).  GR: the trailing-window ring lives in a global scratch of half x
 // (mb + 1) entries per workgroup (A.ring) instead of LDS, for bands whose
 // ring exceeds the CU's LDS (mb above 124); the panels and staged rows stay
-// in LDS.
-template <int ROWS, int MAXT, int NT, bool GR>
+// in LDS.  MF: the look-ahead and trailing updates run on the matrix cores in
+// 16 x 16 tiles (mfma_tile): in phase A the tiles of window columns [0, 16)
+// (panel J's columns to the panel buffer, columns [kNB, 16) back to the
+// ring), in phase B the rest; MAXT is unused.
+template <int ROWS, int MAXT, int NT, bool GR, bool MF>
 __global__ __launch_bounds__(NT, 512 / NT) void crt_factor_kernel(FactorArgs A) {
   extern __shared__ double2 smem[];
   const int n = A.n, mb = A.mb, R = A.R;
@@ -494,7 +544,23 @@ __global__ __launch_bounds__(NT, 512 / NT) void crt_factor_kernel(FactorArgs A) 
 
       // ---- phase A: waves 1-3 apply panel J-1 to panel J's columns; wave 0
       // copies the entering batch J into panel J's last kNB rows
-      if (wave > 0) {
+      if (MF && wave > 0) {
+        __builtin_amdgcn_s_setprio(2);
+        const int T = (mb + 15) / 16;
+        for (int TI = wave - 1; TI < T; TI += NT / 64 - 1) {
+          cplx out[4];
+          mfma_tile(C, Tt, TI, 0, lane, out);
+          const int cw = lane & 15;
+#pragma unroll
+          for (int v = 0; v < 4; ++v) {
+            const int rw = 16 * TI + (lane >> 4) + 4 * v;
+            if (tile_ok(C, rw, cw)) {
+              if (cw < kNB) Pc[pix(rw, cw)] = out[v];
+              else Tt[C.ring.pos(rw, cw)] = out[v];
+            }
+          }
+        }
+      } else if (wave > 0) {
         // the look-ahead precedes the panel on the critical path as well
         __builtin_amdgcn_s_setprio(2);
         for (int x = x3; x < nla; x += NT - 64) {
@@ -652,9 +718,25 @@ __global__ __launch_bounds__(NT, 512 / NT) void crt_factor_kernel(FactorArgs A) 
         int nt = -1;
         if (x3 < kNB * kMaxR) row_slot(A, Sg, o + kNB + mb + x3 / kMaxR, x3 % kMaxR, nv, nt);
         // the rest of the window: panel J-1's update in place
+        if constexpr (MF) {
+          const int T = (mb + 15) / 16;
+          int cnt = 0;
+          for (int TJ = 1; TJ < T; ++TJ)
+            for (int TI = TJ; TI < T; ++TI, ++cnt) {
+              if (cnt % (NT / 64 - 1) != wave - 1) continue;
+              cplx out[4];
+              mfma_tile(C, Tt, TI, TJ, lane, out);
+              const int cw = 16 * TJ + (lane & 15);
+#pragma unroll
+              for (int v = 0; v < 4; ++v) {
+                const int rw = 16 * TI + (lane >> 4) + 4 * v;
+                if (tile_ok(C, rw, cw)) Tt[C.ring.pos(rw, cw)] = out[v];
+              }
+            }
+        }
 #pragma unroll
         for (int q = 0; q < MAXT; ++q) {
-          if (rs_r0[q] >= 0) {
+          if (!MF && rs_r0[q] >= 0) {
             cplx acc[4][4];
             tile_update<4, 4>(C, Tt, rs_r0[q], rs_c0[q], acc);
 #pragma unroll
@@ -677,16 +759,23 @@ __global__ __launch_bounds__(NT, 512 / NT) void crt_factor_kernel(FactorArgs A) 
   clk.flush();
 }
 
-template __global__ void crt_factor_kernel<1, 1, 256, false>(FactorArgs);
-template __global__ void crt_factor_kernel<2, 1, 256, false>(FactorArgs);
-template __global__ void crt_factor_kernel<2, 2, 256, false>(FactorArgs);
-template __global__ void crt_factor_kernel<2, 3, 256, false>(FactorArgs);
-template __global__ void crt_factor_kernel<3, 3, 256, false>(FactorArgs);
-template __global__ void crt_factor_kernel<2, 1, 512, false>(FactorArgs);
-template __global__ void crt_factor_kernel<2, 2, 512, false>(FactorArgs);
-template __global__ void crt_factor_kernel<3, 2, 512, false>(FactorArgs);
-template __global__ void crt_factor_kernel<3, 5, 512, true>(FactorArgs);
-template __global__ void crt_factor_kernel<4, 5, 512, true>(FactorArgs);
+template __global__ void crt_factor_kernel<1, 1, 256, false, false>(FactorArgs);
+template __global__ void crt_factor_kernel<2, 1, 256, false, false>(FactorArgs);
+template __global__ void crt_factor_kernel<2, 2, 256, false, false>(FactorArgs);
+template __global__ void crt_factor_kernel<2, 3, 256, false, false>(FactorArgs);
+template __global__ void crt_factor_kernel<3, 3, 256, false, false>(FactorArgs);
+template __global__ void crt_factor_kernel<2, 1, 512, false, false>(FactorArgs);
+template __global__ void crt_factor_kernel<2, 2, 512, false, false>(FactorArgs);
+template __global__ void crt_factor_kernel<3, 2, 512, false, false>(FactorArgs);
+template __global__ void crt_factor_kernel<3, 5, 512, true, false>(FactorArgs);
+template __global__ void crt_factor_kernel<4, 5, 512, true, false>(FactorArgs);
+template __global__ void crt_factor_kernel<1, 1, 256, false, true>(FactorArgs);
+template __global__ void crt_factor_kernel<2, 1, 256, false, true>(FactorArgs);
+template __global__ void crt_factor_kernel<3, 1, 256, false, true>(FactorArgs);
+template __global__ void crt_factor_kernel<2, 1, 512, false, true>(FactorArgs);
+template __global__ void crt_factor_kernel<3, 1, 512, false, true>(FactorArgs);
+template __global__ void crt_factor_kernel<3, 1, 512, true, true>(FactorArgs);
+template __global__ void crt_factor_kernel<4, 1, 512, true, true>(FactorArgs);
 
 // ---------------------------------------------------------------------------
 // Back substitution, one wavefront per right-hand side.  Lane q (q < 64) keeps

@@ -89,7 +89,7 @@ __global__ void crt_area_values_kernel(FactorArgs A, int K, double2* av);
 __global__ void crt_scale_kernel(FactorArgs A);
 __global__ void crt_rows_kernel(FactorArgs A);
 __global__ void crt_rhs_kernel(FactorArgs A);
-template <int ROWS, int MAXT, int NT, bool GR>
+template <int ROWS, int MAXT, int NT, bool GR, bool MF>
 __global__ void crt_factor_kernel(FactorArgs A);
 template <int DEPTH, int G, int NS>
 __global__ void crt_backsolve_kernel(BacksolveArgs B);
