@@ -167,6 +167,28 @@ int crt_cg_solve(crt_handle *h, const double *wmatd_wdfak, const double *dat,
  * factor batches.  For callers keeping many handles alive at once. */
 int crt_set_device_budget(crt_handle *h, uint64_t bytes);
 
+/* Launch choices for the handle's forwards, 0 = the built-in rule: threads
+ * per factor workgroup (256 or 512; bands whose LDS window fits twice per CU
+ * take 256 by default) and the back substitution's prefetch depth in rows (8
+ * or 16).  The results are the same bits either way; an autotuner
+ * (titatest_amd/autotune.py) times them.  No counterpart in the reference. */
+int crt_set_tuning(crt_handle *h, int32_t factor_threads, int32_t backsolve_depth);
+
+/* The forward of several models in one pass (the reference runs one model per
+ * program run; this is the loop of src/fem.f90:328-397 over many handles).
+ * hs: nmodels distinct handles on one device, after crt_precal; sigma_dev:
+ * each model's conductivities on the device (elanz complex).  Each model's
+ * assembly runs on its own stream; then one persistent factor launch per
+ * factor-kernel form takes the (k, l) systems of all its models from a work
+ * queue, costliest models first, and one back-substitution launch per form
+ * takes all their right-hand sides.  Every system's factors stay resident
+ * (the device budget does not apply).  Models without mixed boundaries (one
+ * factor per wavenumber) run their own forward.  Afterwards each handle is as
+ * after crt_forward_dev: crt_voltages_dev, the sensitivities and the adjoint
+ * gradient follow per model.  Returns the first error (the handle's
+ * crt_error_info names the system). */
+int crt_forward_many(int32_t nmodels, crt_handle *const *hs, const double *const *sigma_dev);
+
 /* No reference counterpart.  enable != 0: crt_cg_solve runs its iterations as
  * one HIP graph, captured on the first call and replayed while the problem's
  * sizes, buffers and parameters stay the same (for callers running many short

@@ -131,3 +131,47 @@ def test_gn_iterations_reduce_the_misfit(crtmod, oracle_lib):
     so, _, _ = cpu.dmisft(m.dobs, sa, m.wgt, m.wgt, m.wgt)
     assert abs(nr[3] - so[0]) <= 1e-13 * so[0]
     assert abs(hist[3][1] - so[1]) <= 1e-13 * so[1] and abs(hist[3][2] - so[2]) <= 1e-13 * so[2]
+
+
+def test_forward_many_equals_per_model_forwards(crtmod):
+    """crt_forward_many over models of different bands and mesh types -- four
+    uneven quadrilateral models, a terrain-following triangular mesh with
+    lsytop, a band of 132 (the wide form: window in global memory) and a
+    Dirichlet model (one factor per wavenumber, its own forward) -- gives
+    each model the voltages (both modes) and Jacobian of its own
+    crt_forward, bit for bit: the persistent launches run the same per-system
+    code."""
+    from titatest_amd import batch
+    ws = sorted(meshgen.uneven_batch(count=8, seed=1006), key=lambda w: w.mesh.elanz)[:4]
+    tri = meshgen.topo_triangles(nx_nodes=41, nz_nodes=20, eanz=16)
+    ws += [tri, meshgen.large_grf(nx_side=2, n_uniform=12, nz=130, eanz=12),
+           meshgen.large_grf(nx_side=4, n_uniform=20, nz=12, eanz=16, side_type=13)]
+    lsy = [w is tri for w in ws]
+    say("forward_many over", [(w.mesh.sanz, w.mesh.mb) for w in ws])
+    gm = [crtmod.CRT(w.mesh, lsytop=t) for w, t in zip(ws, lsy)]
+    gr = [crtmod.CRT(w.mesh, lsytop=t) for w, t in zip(ws, lsy)]
+    for g, w in zip(gm + gr, ws + ws):
+        g.rwaven(w.strnr, w.vnr)
+        g.precal()
+    batch.forward_many(gm, [w.sigma for w in ws])
+    for g1, g2, w in zip(gm, gr, ws):
+        g2.forward(w.sigma)
+        for mode in (0, 1):
+            v1, s1 = g1.voltages(w.strnr, w.vnr, mode)
+            v2, s2 = g2.voltages(w.strnr, w.vnr, mode)
+            np.testing.assert_array_equal(v1, v2)
+            np.testing.assert_array_equal(s1, s2)
+        np.testing.assert_array_equal(g1.sensitivities(1), g2.sensitivities(1))
+
+
+def test_persistent_batch_equals_per_model_steps(crtmod):
+    from titatest_amd import batch
+    cases = _small_models(3)
+    say("persistent batch", [(w.mesh.sanz, w.mesh.eanz) for w, _, _ in cases])
+    mods = [batch.prepare(w, nx, nz, seed=i) for i, (w, nx, nz) in enumerate(cases)]
+    seq = [batch.gn_step(m, use_graph=False) for m in mods]
+    per, _ = batch.run_batch_persistent(mods, threads=3, use_graph=True)
+    for (d1, g1, c1), (d2, g2, c2) in zip(seq, per):
+        np.testing.assert_array_equal(d1, d2)
+        np.testing.assert_array_equal(g1, g2)
+        np.testing.assert_array_equal(c1, c2)

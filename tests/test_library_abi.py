@@ -48,6 +48,8 @@ def test_null_arguments_are_rejected_without_a_device():
     assert lib.crt_set_decouplings(None, 0, None, None) == 1000
     assert lib.crt_set_graph(None, 1) == 1000
     assert lib.crt_set_device_budget(None, 1 << 30) == 1000
+    assert lib.crt_set_tuning(None, 0, 0) == 1000
+    assert lib.crt_forward_many(0, None, None) == 1000
     assert lib.crt_dmisft(None, 4, None, None, None, None, None, None, None, 0, 1.0, None, None) == 1000
     assert lib.crt_dmisft_dev(None, 4, None, None, None, None, None, None, None, 0, 1.0, None, None) == 1000
     assert lib.crt_set_model_map(None, 0, None) == 1000

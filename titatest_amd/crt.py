@@ -36,7 +36,8 @@ EXPORTS = (
     "crt_get_precal", "crt_forward", "crt_forward_dev", "crt_voltages", "crt_voltages_dev",
     "crt_sensitivities", "crt_adjoint_gradient", "crt_adjoint_gradient_dev", "crt_jhw_r",
     "crt_cg_solve", "crt_forward_dc", "crt_sensitivities_dc", "crt_set_boundary_values",
-    "crt_smoothness", "crt_set_decouplings", "crt_set_graph", "crt_set_device_budget",
+    "crt_smoothness", "crt_set_decouplings", "crt_set_graph", "crt_set_device_budget", "crt_set_tuning",
+    "crt_forward_many",
     "crt_error_info", "crt_set_timing", "crt_stage_times", "crt_factor_phase_ticks",
     "crt_kpot_device", "crt_dmisft", "crt_dmisft_dev", "crt_dmisft_dev_noalloc", "crt_set_model_map", "crt_bpar", "crt_bpar_dev",
     "crt_bsigma", "crt_bsigma_dev", "crt_update_step", "crt_update_step_dev", "crt_blam0",
@@ -100,6 +101,8 @@ def load_library(path: Path = LIB_PATH):
     L.crt_set_decouplings.argtypes = [vp, i32, vp, vp]
     L.crt_set_graph.argtypes = [vp, i32]
     L.crt_set_device_budget.argtypes = [vp, C.c_uint64]
+    L.crt_set_tuning.argtypes = [vp, i32, i32]
+    L.crt_forward_many.argtypes = [i32, vp, vp]
     L.crt_sensitivities_dc.argtypes = [vp, vp]
     L.crt_voltages.argtypes = [vp, i32, vp, vp, i32, vp, vp, vp]
     L.crt_voltages_dev.argtypes = [vp, i32, vp, vp, i32, vp, vp]
@@ -131,6 +134,18 @@ def load_library(path: Path = LIB_PATH):
 
 def _ptr(a):
     return None if a is None else a.ctypes.data
+
+
+def forward_many(handles, sigma_dev_ptrs):
+    """crt_forward_many over CRT objects, with each model's conductivities
+    given as a device pointer (e.g. a torch tensor's data_ptr())."""
+    L = load_library()
+    n = len(handles)
+    hs = (C.c_void_p * n)(*[g.h.value for g in handles])
+    ps = (C.c_void_p * n)(*[int(p) for p in sigma_dev_ptrs])
+    rc = L.crt_forward_many(n, hs, ps)
+    if rc:
+        raise CRTError("forward_many", rc)
 
 
 def _c128(a):
@@ -331,6 +346,11 @@ class CRT:
                                                           nx, nz, _ptr(sm), _ptr(nb)))
         sm = sm.reshape((M, ncol), order="F")
         return sm if ltri == 0 else (sm, nb.reshape((M, mx + 1), order="F"))
+
+    def set_tuning(self, factor_threads=0, backsolve_depth=0):
+        """Factor workgroup threads (0, 256, 512) and back-substitution
+        prefetch depth (0, 8, 16) for this handle's forwards."""
+        self._check("set_tuning", self.lib.crt_set_tuning(self.h, int(factor_threads), int(backsolve_depth)))
 
     def set_device_budget(self, nbytes):
         """Cap the device memory a forward keeps for factors in flight."""

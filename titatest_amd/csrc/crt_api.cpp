@@ -108,6 +108,9 @@ struct crt_handle {
   int slots = 0;
   int R = 1;
   size_t batch_bytes_max = (size_t)64 << 30;  // device bytes for factors in flight
+  // launch choices set by crt_set_tuning (0 = the built-in rule): factor
+  // workgroup threads (256 or 512) and back-substitution prefetch depth (8, 16)
+  int tune_nt = 0, tune_depth = 8;
   int last_err = 0, last_k = 0, last_l = 0;
   // factor-kernel phase clock (wall-clock ticks summed over workgroups)
   bool kprof = false;
@@ -865,8 +868,12 @@ static int open_lds(crt_handle* h, const void* kern, size_t* dyn = nullptr) {
 // The substitutions keep a window of 64 NS rows in NS register slots per lane
 // and refill every G rows: G < 64 NS + 1 - mb keeps a refill ahead of its
 // first update.  NS = 2 up to mb = 124, NS = 4 up to kMaxBand.
-static void launch_backsolve(int mb, int nrhs, hipStream_t st, const BacksolveArgs& B) {
-  if (mb <= 120)
+static void launch_backsolve(int mb, int nrhs, hipStream_t st, const BacksolveArgs& B, int depth = 8) {
+  if (depth == 16 && mb <= 120)
+    hipLaunchKernelGGL((crt_backsolve_kernel<16, 8, 2>), dim3(nrhs), dim3(64), 0, st, B);
+  else if (depth == 16 && mb > 124)
+    hipLaunchKernelGGL((crt_backsolve_kernel<16, 8, 4>), dim3(nrhs), dim3(64), 0, st, B);
+  else if (mb <= 120)
     hipLaunchKernelGGL((crt_backsolve_kernel<8, 8, 2>), dim3(nrhs), dim3(64), 0, st, B);
   else if (mb <= 124)
     hipLaunchKernelGGL((crt_backsolve_kernel<8, 4, 2>), dim3(nrhs), dim3(64), 0, st, B);
@@ -882,8 +889,19 @@ static void launch_fwdsweep(int mb, int nrhs, hipStream_t st, const FwdArgs& F) 
     hipLaunchKernelGGL((crt_fwdsweep_kernel<8, 8, 4>), dim3(nrhs), dim3(64), 0, st, F);
 }
 
-static int run_forward(crt_handle* h) {
-  const int n = h->n, mb = h->mb, E = h->E, K = h->K;
+// The factor kernel's form for a handle's band: the single-model kernel, its
+// multi-model twin (null for forms without one), threads per workgroup and
+// dynamic LDS.  Tuning (crt_set_tuning) may ask for 256 or 512 threads.
+struct FactorForm {
+  void (*kern)(FactorArgs) = nullptr;
+  void (*many)(const FactorArgs*, const int2*, int, int*) = nullptr;
+  int nt = 256;
+  size_t lds = 0;
+  bool gring = false;
+};
+
+static int choose_factor(const crt_handle* h, FactorForm& F) {
+  const int n = h->n, mb = h->mb;
   // the substitutions address one factor with 32-bit byte offsets
   if (mb > kMaxBand || (size_t)n * (mb + 1) * 16 >= (size_t(1) << 31)) return kErrBand;
   // The trailing-window ring lives in LDS up to mb = 124 (when it fits), in a
@@ -897,49 +915,56 @@ static int run_forward(crt_handle* h) {
   // workgroup takes 512 threads instead of 256 (the wide-band form always).
   const int W = mb + kNB, tw = (mb + 3) / 4;
   const int nrest = tw > 2 ? ((tw - 2) * (tw - 1)) / 2 : 0;
-  const int nt = gring || 2 * lds > h->lds_max ? 512 : 256;
+  int nt = gring || 2 * lds > h->lds_max ? 512 : 256;
+  if (!gring && (h->tune_nt == 256 || h->tune_nt == 512)) nt = h->tune_nt;
   const int rows = (W + 63) / 64, maxt = std::max(1, (nrest + nt - 65) / (nt - 64));
   // The look-ahead and trailing updates run on the vector ALU (4 x 4 and
   // 2 x 2 tiles, one product at a time).  CRT_FACTOR_MFMA selects the form
   // with 16 x 16 tiles on the matrix cores instead: as accurate, but slower on
   // gfx950, whose FP64 matrix rate is below its FP64 vector rate.
   static const bool mfma = std::getenv("CRT_FACTOR_MFMA") != nullptr;
-  void (*kern)(FactorArgs) = nullptr;
+#define FORM(R, T, N, G) (F.kern = crt_factor_kernel<R, T, N, G, false>, F.many = crt_factor_many_kernel<R, T, N, G, false>)
   if (mfma) {
     if (gring) {
-      if (rows <= 3) kern = crt_factor_kernel<3, 1, 512, true, true>;
-      else if (rows == 4) kern = crt_factor_kernel<4, 1, 512, true, true>;
+      if (rows <= 3) F.kern = crt_factor_kernel<3, 1, 512, true, true>;
+      else if (rows == 4) F.kern = crt_factor_kernel<4, 1, 512, true, true>;
     } else if (nt == 256) {
-      if (rows == 1) kern = crt_factor_kernel<1, 1, 256, false, true>;
-      else if (rows == 2) kern = crt_factor_kernel<2, 1, 256, false, true>;
-      else if (rows == 3) kern = crt_factor_kernel<3, 1, 256, false, true>;
+      if (rows == 1) F.kern = crt_factor_kernel<1, 1, 256, false, true>;
+      else if (rows == 2) F.kern = crt_factor_kernel<2, 1, 256, false, true>;
+      else if (rows == 3) F.kern = crt_factor_kernel<3, 1, 256, false, true>;
     } else {
-      if (rows <= 2) kern = crt_factor_kernel<2, 1, 512, false, true>;
-      else if (rows == 3) kern = crt_factor_kernel<3, 1, 512, false, true>;
+      if (rows <= 2) F.kern = crt_factor_kernel<2, 1, 512, false, true>;
+      else if (rows == 3) F.kern = crt_factor_kernel<3, 1, 512, false, true>;
     }
   } else if (gring) {
-    if (rows <= 3 && maxt <= 5) kern = crt_factor_kernel<3, 5, 512, true, false>;
-    else if (rows == 4 && maxt <= 5) kern = crt_factor_kernel<4, 5, 512, true, false>;
+    if (rows <= 3 && maxt <= 5) FORM(3, 5, 512, true);
+    else if (rows == 4 && maxt <= 5) FORM(4, 5, 512, true);
   } else if (nt == 256) {
-    if (rows == 1 && maxt == 1) kern = crt_factor_kernel<1, 1, 256, false, false>;
-    else if (rows == 2 && maxt == 1) kern = crt_factor_kernel<2, 1, 256, false, false>;
-    else if (rows == 2 && maxt == 2) kern = crt_factor_kernel<2, 2, 256, false, false>;
-    else if (rows == 2 && maxt == 3) kern = crt_factor_kernel<2, 3, 256, false, false>;
-    else if (rows == 3 && maxt <= 3) kern = crt_factor_kernel<3, 3, 256, false, false>;
+    if (rows == 1 && maxt == 1) FORM(1, 1, 256, false);
+    else if (rows == 2 && maxt == 1) FORM(2, 1, 256, false);
+    else if (rows == 2 && maxt == 2) FORM(2, 2, 256, false);
+    else if (rows == 2 && maxt == 3) FORM(2, 3, 256, false);
+    else if (rows == 3 && maxt <= 3) FORM(3, 3, 256, false);
   } else {
-    if (rows <= 2 && maxt == 1) kern = crt_factor_kernel<2, 1, 512, false, false>;
-    else if (rows <= 2 && maxt == 2) kern = crt_factor_kernel<2, 2, 512, false, false>;
-    else if (rows == 3 && maxt <= 2) kern = crt_factor_kernel<3, 2, 512, false, false>;
+    if (rows <= 2 && maxt == 1) FORM(2, 1, 512, false);
+    else if (rows <= 2 && maxt == 2) FORM(2, 2, 512, false);
+    else if (rows == 3 && maxt <= 2) FORM(3, 2, 512, false);
   }
-  if (!kern) return kErrBand;
-  if (int rc = open_lds(h, reinterpret_cast<const void*>(kern))) return rc;
-  int per_cu = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, nt, lds) != hipSuccess || per_cu < 1) per_cu = 1;
-  const int nsys = K * E;
-  FactorArgs A;
-  A.n = n;
-  A.mb = mb;
-  A.eanz = E;
+#undef FORM
+  if (!F.kern) return kErrBand;
+  F.nt = nt;
+  F.lds = lds;
+  F.gring = gring;
+  return 0;
+}
+
+// The model-level fields of the factor and assembly kernels' arguments (the
+// per-batch buffers are set by the caller).
+static void factor_args(crt_handle* h, FactorArgs& A) {
+  std::memset(&A, 0, sizeof(A));
+  A.n = h->n;
+  A.mb = h->mb;
+  A.eanz = h->E;
   A.elanz = h->elanz;
   A.relanz = h->relanz;
   A.nimn = h->nimn;
@@ -965,6 +990,24 @@ static int run_forward(crt_handle* h) {
   A.nsink = h->nsink - 1;
   A.dc = h->dc ? 1 : 0;
   A.ring = nullptr;
+  A.ring_stride = 0;
+  A.sys_stride = 1;
+}
+
+static int run_forward(crt_handle* h) {
+  const int n = h->n, mb = h->mb, E = h->E, K = h->K;
+  FactorForm form;
+  if (int rc = choose_factor(h, form)) return rc;
+  void (*kern)(FactorArgs) = form.kern;
+  const int nt = form.nt;
+  const size_t lds = form.lds;
+  const bool gring = form.gring;
+  if (int rc = open_lds(h, reinterpret_cast<const void*>(kern))) return rc;
+  int per_cu = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, nt, lds) != hipSuccess || per_cu < 1) per_cu = 1;
+  const int nsys = K * E;
+  FactorArgs A;
+  factor_args(h, A);
   double2* av = h->get<double2>("av", (size_t)K * n * h->R);
   double2* kpot = h->get<double2>("kpot", (size_t)K * E * n);
   NEED(av);
@@ -1072,7 +1115,8 @@ static int run_forward(crt_handle* h) {
     tstop(h);
     h->slots = std::min(cnt, h->ncu * per_cu);
     if (gring) {
-      A.ring = h->get<double2>("ring", (size_t)h->slots * ((mb + 1) / 2) * (mb + 1));
+      A.ring_stride = (size_t)((mb + 1) / 2) * (mb + 1);
+      A.ring = h->get<double2>("ring", (size_t)h->slots * A.ring_stride);
       NEED(A.ring);
     }
     const int nrhs = cnt * rpf;
@@ -1109,7 +1153,7 @@ static int run_forward(crt_handle* h) {
           HIPCHK(hipStreamWaitEvent(h->stream2, h->ev_fac, 0));
         }
         const int slot = tstart_on(h, "backsolve", bs);
-        launch_backsolve(mb, pc, bs, Bp);
+        launch_backsolve(mb, pc, bs, Bp, h->tune_depth);
         tstop_on(h, slot, bs);
       }
       HIPCHK(hipEventRecord(h->ev_join, h->stream2));
@@ -1134,7 +1178,7 @@ static int run_forward(crt_handle* h) {
     B.nrhs = nrhs;
     B.fac_first = first;
     tstart(h, "backsolve");
-    launch_backsolve(mb, nrhs, h->stream, B);
+    launch_backsolve(mb, nrhs, h->stream, B, h->tune_depth);
     tstop(h);
   }
   // electrode slice of the back-transform, used by the voltage stage
@@ -1157,6 +1201,221 @@ static int run_forward(crt_handle* h) {
   h->forward_done = true;
   h->volt_done = h->sens_done = false;
   h->kpt_valid = false;
+  return 0;
+}
+
+// Back-substitution form for a band (the rule of launch_backsolve).
+static void (*backsolve_many_form(int mb, int depth))(const BacksolveArgs*, const int2*, int) {
+  if (depth == 16 && mb <= 120) return crt_backsolve_many_kernel<16, 8, 2>;
+  if (depth == 16 && mb > 124) return crt_backsolve_many_kernel<16, 8, 4>;
+  if (mb <= 120) return crt_backsolve_many_kernel<8, 8, 2>;
+  if (mb <= 124) return crt_backsolve_many_kernel<8, 4, 2>;
+  return crt_backsolve_many_kernel<8, 8, 4>;
+}
+
+int crt_forward_many(int32_t nmodels, crt_handle* const* hs, const double* const* sigma_dev) {
+  if (nmodels <= 0 || !hs || !sigma_dev) return kErrArg;
+  for (int m = 0; m < nmodels; ++m) {
+    if (!hs[m] || !sigma_dev[m]) return kErrArg;
+    if (hs[m]->dev != hs[0]->dev) return kErrArg;
+    for (int j = 0; j < m; ++j)
+      if (hs[j] == hs[m]) return kErrArg;
+    if (!hs[m]->precal_done) return kErrOrder;
+  }
+  crt_handle* h0 = hs[0];
+  hipStream_t st = h0->stream;
+  struct Model {
+    crt_handle* h;
+    FactorForm form;
+    FactorArgs A;
+    long long cost;
+  };
+  std::vector<Model> ms;
+  // ---- per model: conductivities, area sums, scaled rows (and right-hand
+  // sides) of every system on its own stream; models without mixed
+  // boundaries (one factor per wavenumber) run the single-model forward
+  for (int m = 0; m < nmodels; ++m) {
+    crt_handle* h = hs[m];
+    double2* ds = h->get<double2>("sigma", h->elanz);
+    NEED(ds);
+    if (hipMemcpyAsync(ds, sigma_dev[m], (size_t)h->elanz * 16, hipMemcpyDeviceToDevice, h->stream) != hipSuccess)
+      return kErrHip;
+    h->dc = false;
+    Model M;
+    M.h = h;
+    if (int rc = choose_factor(h, M.form)) return rc;
+    if (!h->has11 || !M.form.many) {
+      if (int rc = run_forward(h)) return rc;
+      continue;
+    }
+    const int n = h->n, mb = h->mb, E = h->E, K = h->K, nsys = K * E;
+    FactorArgs& A = M.A;
+    factor_args(h, A);
+    A.nsys = nsys;
+    A.sys_first = 0;
+    double2* av = h->get<double2>("av", (size_t)K * n * h->R);
+    NEED(av);
+    NEED(h->get<double2>("kpot", (size_t)K * E * n));
+    A.av = av;
+    if (clear_err(h)) return kErrHip;
+    A.err = h->at<int>("err");
+    A.L = h->get<double2>("L", (size_t)nsys * n * (mb + 1));
+    A.S = h->get<double2>("S", (size_t)nsys * n * h->R);
+    A.y = h->get<double2>("y", (size_t)nsys * n);
+    A.fak = h->get<double>("fak", (size_t)nsys * n);
+    NEED(A.L);
+    NEED(A.S);
+    NEED(A.y);
+    NEED(A.fak);
+    A.mv_ptr = A.mv_ir = A.mv_e = A.rb_last = nullptr;
+    A.mv_imn = nullptr;
+    A.rwd = nullptr;
+    A.rhs = nullptr;
+    const bool rb2 = !h->rwdnr.empty();
+    if (rb2) {
+      A.mv_ptr = h->at<int>("mv_ptr");
+      A.mv_ir = h->at<int>("mv_ir");
+      A.mv_e = h->at<int>("mv_e");
+      A.mv_imn = h->at<int8_t>("mv_imn");
+      A.rb_last = h->at<int>("rb_last");
+      A.rwd = h->at<double2>("rwd");
+      A.rhs = h->get<double2>("rhs", (size_t)nsys * n);
+      NEED(A.rhs);
+    }
+    hipLaunchKernelGGL(crt_area_values_kernel, dim3((unsigned)(((size_t)K * n + 255) / 256)), dim3(256), 0,
+                       h->stream, A, K, av);
+    const unsigned rgrid = (unsigned)(((size_t)nsys * n + 255) / 256);
+    hipLaunchKernelGGL(crt_scale_kernel, dim3(rgrid), dim3(256), 0, h->stream, A);
+    hipLaunchKernelGGL(crt_rows_kernel, dim3((unsigned)(((size_t)nsys * n * h->R + 255) / 256)), dim3(256), 0,
+                       h->stream, A);
+    if (rb2) hipLaunchKernelGGL(crt_rhs_kernel, dim3(rgrid), dim3(256), 0, h->stream, A);
+    HIPCHK(hipEventRecord(h->ev_fac, h->stream));
+    HIPCHK(hipStreamWaitEvent(st, h->ev_fac, 0));
+    M.cost = (long long)n * mb * mb;
+    ms.push_back(M);
+  }
+  if (ms.empty()) return 0;
+  crt_handle* h = h0;  // HIPCHK records errors on the first handle
+  // ---- one persistent factor launch per kernel form over all its models'
+  // systems, the costliest models' systems first
+  std::vector<std::vector<FactorArgs>> keep_args;
+  std::vector<std::vector<int2>> keep_items;
+  std::map<const void*, std::vector<int>> bins;
+  for (int i = 0; i < (int)ms.size(); ++i) bins[reinterpret_cast<const void*>(ms[i].form.many)].push_back(i);
+  int ib = 0;
+  const int slot_t = tstart_on(h0, "factor_solve_many", st);
+  for (auto& kv : bins) {
+    std::vector<int>& idx = kv.second;
+    std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) { return ms[a].cost > ms[b].cost; });
+    const FactorForm& F = ms[idx[0]].form;
+    size_t lds = 0, stride = 0;
+    for (int i : idx) {
+      lds = std::max(lds, ms[i].form.lds);
+      stride = std::max(stride, (size_t)((ms[i].h->mb + 1) / 2) * (ms[i].h->mb + 1));
+    }
+    if (int rc = open_lds(h0, reinterpret_cast<const void*>(F.many))) return rc;
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, F.many, F.nt, lds) != hipSuccess || per_cu < 1)
+      per_cu = 1;
+    keep_args.emplace_back();
+    keep_items.emplace_back();
+    std::vector<FactorArgs>& args = keep_args.back();
+    std::vector<int2>& items = keep_items.back();
+    for (int j = 0; j < (int)idx.size(); ++j) {
+      args.push_back(ms[idx[j]].A);
+      for (int sys = 0; sys < ms[idx[j]].A.nsys; ++sys) items.push_back(make_int2(j, sys));
+    }
+    const int nitems = (int)items.size();
+    const int grid = std::min(nitems, h0->ncu * per_cu);
+    if (F.gring) {
+      double2* ring = h0->get<double2>("many_ring" + std::to_string(ib), (size_t)grid * stride);
+      NEED(ring);
+      for (auto& a : args) {
+        a.ring = ring;
+        a.ring_stride = stride;
+      }
+    }
+    FactorArgs* dargs = h0->get<FactorArgs>("many_fargs" + std::to_string(ib), args.size());
+    int2* ditems = h0->get<int2>("many_fitems" + std::to_string(ib), items.size());
+    int* counter = h0->get<int>("many_counter" + std::to_string(ib), 1);
+    NEED(dargs);
+    NEED(ditems);
+    NEED(counter);
+    HIPCHK(hipMemcpyAsync(dargs, args.data(), args.size() * sizeof(FactorArgs), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(ditems, items.data(), items.size() * sizeof(int2), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemsetAsync(counter, 0, sizeof(int), st));
+    hipLaunchKernelGGL(F.many, dim3(grid), dim3(F.nt), lds, st, (const FactorArgs*)dargs, (const int2*)ditems,
+                       nitems, counter);
+    ++ib;
+  }
+  tstop_on(h0, slot_t, st);
+  // ---- back substitutions, one launch per form over every right-hand side
+  std::map<const void*, std::vector<int>> bbins;
+  for (int i = 0; i < (int)ms.size(); ++i)
+    bbins[reinterpret_cast<const void*>(backsolve_many_form(ms[i].h->mb, ms[i].h->tune_depth))].push_back(i);
+  std::vector<std::vector<BacksolveArgs>> keep_bargs;
+  const int slot_b = tstart_on(h0, "backsolve_many", st);
+  ib = 0;
+  for (auto& kv : bbins) {
+    keep_bargs.emplace_back();
+    keep_items.emplace_back();
+    std::vector<BacksolveArgs>& bargs = keep_bargs.back();
+    std::vector<int2>& items = keep_items.back();
+    for (int j = 0; j < (int)kv.second.size(); ++j) {
+      const Model& M = ms[kv.second[j]];
+      BacksolveArgs B;
+      B.n = M.h->n;
+      B.mb = M.h->mb;
+      B.eanz = M.h->E;
+      B.nrhs = M.A.nsys;
+      B.rhs_per_fac = 1;
+      B.fac_first = 0;
+      B.fstride = 1;
+      B.L = M.A.L;
+      B.y = M.A.y;
+      B.fak = M.A.fak;
+      B.kpot = M.h->at<double2>("kpot");
+      bargs.push_back(B);
+      for (int r = 0; r < B.nrhs; ++r) items.push_back(make_int2(j, r));
+    }
+    BacksolveArgs* dargs = h0->get<BacksolveArgs>("many_bargs" + std::to_string(ib), bargs.size());
+    int2* ditems = h0->get<int2>("many_bitems" + std::to_string(ib), items.size());
+    NEED(dargs);
+    NEED(ditems);
+    HIPCHK(hipMemcpyAsync(dargs, bargs.data(), bargs.size() * sizeof(BacksolveArgs), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(ditems, items.data(), items.size() * sizeof(int2), hipMemcpyHostToDevice, st));
+    auto kern = reinterpret_cast<void (*)(const BacksolveArgs*, const int2*, int)>(const_cast<void*>(kv.first));
+    hipLaunchKernelGGL(kern, dim3((unsigned)items.size()), dim3(64), 0, st, (const BacksolveArgs*)dargs,
+                       (const int2*)ditems, (int)items.size());
+    ++ib;
+  }
+  tstop_on(h0, slot_b, st);
+  // ---- electrode rows of the back-transform, per model
+  for (const Model& M : ms) {
+    crt_handle* hm = M.h;
+    double2* he = hm->get<double2>("he", (size_t)hm->E * hm->E);
+    NEED(he);
+    hipLaunchKernelGGL(crt_rtrafo_elec_kernel, dim3((hm->E * hm->E + 255) / 256), dim3(256), 0, st,
+                       (const double2*)hm->at<double2>("kpot"), (const double*)hm->at<double>("kwnwi"),
+                       (const int*)M.A.enr, hm->n, hm->E, hm->K, hm->swrtr, he);
+  }
+  if (int rc = finish(h0)) return rc;
+  for (const Model& M : ms) {
+    const int e = read_err(M.h);
+    if (e) return e;
+    M.h->forward_done = true;
+    M.h->volt_done = M.h->sens_done = false;
+    M.h->kpt_valid = false;
+  }
+  return 0;
+}
+
+int crt_set_tuning(crt_handle* h, int32_t factor_threads, int32_t backsolve_depth) {
+  if (!h) return kErrArg;
+  if (factor_threads != 0 && factor_threads != 256 && factor_threads != 512) return kErrArg;
+  if (backsolve_depth != 0 && backsolve_depth != 8 && backsolve_depth != 16) return kErrArg;
+  h->tune_nt = factor_threads;
+  h->tune_depth = backsolve_depth == 0 ? 8 : backsolve_depth;
   return 0;
 }
 

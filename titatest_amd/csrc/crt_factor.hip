@@ -420,6 +420,349 @@ __device__ __forceinline__ void stage_barrier() {
 // 16 x 16 tiles (mfma_tile): in phase A the tiles of window columns [0, 16)
 // (panel J's columns to the panel buffer, columns [kNB, 16) back to the
 // ring), in phase B the rest; MAXT is unused.
+// The trailing 4x4 tiles of this thread for band mb (window rows and columns
+// from 8 on; tile x of the lower triangle goes to thread 64 + x mod (NT - 64)),
+// -1 where the thread has none.
+template <int MAXT, int NT>
+__device__ __forceinline__ void trailing_tiles(int mb, int (&rs_r0)[MAXT], int (&rs_c0)[MAXT]) {
+  const int tid = threadIdx.x, wave = tid >> 6, x3 = tid - 64;
+  const int tw = (mb + 3) / 4;
+  const int nrest = tw > 2 ? ((tw - 2) * (tw - 1)) / 2 : 0;  // 4x4 tiles, tile columns 2..
+#pragma unroll
+  for (int q = 0; q < MAXT; ++q) {
+    rs_r0[q] = -1;
+    rs_c0[q] = 0;
+    const int x = x3 + (NT - 64) * q;
+    if (wave > 0 && x < nrest) {
+      int ti, tj;
+      tri_tile(x, ti, tj);
+      rs_r0[q] = 4 * (ti + 2);
+      rs_c0[q] = 4 * (tj + 2);
+    }
+  }
+}
+
+template <int ROWS, int MAXT, int NT, bool GR, bool MF>
+__device__ __forceinline__ void factor_system(const FactorArgs& A, int s, double2* smem, PhaseClock& clk) {
+  const int n = A.n, mb = A.mb, R = A.R;
+  const int W = mb + kNB, M1 = mb + 1, half = (mb + 1) / 2;
+  // half x M1 ring slots
+  cplx* Tt = GR ? reinterpret_cast<cplx*>(A.ring) + (size_t)blockIdx.x * A.ring_stride : reinterpret_cast<cplx*>(smem);
+  cplx* P0 = GR ? reinterpret_cast<cplx*>(smem) : Tt + half * M1;  // 2 panels of W x kNB
+  cplx* sv = P0 + 2 * W * kNB;                // kStg batches of kNB x kMaxR values
+  cplx* xb = sv + kStg * kNB * kMaxR;         // kNB: column broadcast of the panel
+  cplx* xd = xb + kNB;                        // kNB: the panel's pivots d_c
+  cplx* xi = xd + kNB;                        // kNB: 1/d_c
+  cplx* xl = xi + kNB;                        // kNB: L_cc = sqrt(d_c)
+  int16_t* so = reinterpret_cast<int16_t*>(xl + kNB);  // kStg x kNB x kMaxR offsets
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int x3 = tid - 64;                    // thread index within waves 1-3
+  const int nst = (n + kNB - 1) / kNB;
+  const int tw = (mb + 3) / 4;                          // 4x4 tiles per window side
+  // look-ahead: 2x2 tiles over window rows [0, mb) x columns [0, kNB), column
+  // by column of tiles; tile column j holds the tile rows i >= j
+  const int t2 = (mb + 1) / 2;
+  int nla = 0;
+#pragma unroll
+  for (int j = 0; j < kNB / 2; ++j) nla += max(t2 - j, 0);
+  // trailing 4x4 tiles of this thread
+  int rs_r0[MAXT], rs_c0[MAXT];
+  trailing_tiles<MAXT, NT>(mb, rs_r0, rs_c0);
+
+  {
+    const int sys = (A.sys_first + s) * A.sys_stride;
+    const int l = sys % A.eanz;
+    double2* Lg = A.L + (size_t)s * n * M1;
+    double2* yg = A.y + (size_t)s * n;
+    const double2* Sg = A.S + (size_t)s * n * R;
+    const double* fk = A.fak + (size_t)s * n;
+
+    // ---- prologue: ring zeroed, rows [0, mb - kNB) in place, batches
+    // [mb - kNB, mb) and [mb, mb + kNB) staged, wave 0's right-hand side
+    for (int i = tid; i < half * M1; i += NT) Tt[i] = mkc(0.0, 0.0);
+    stage_barrier<GR>();
+    {
+      Ring ring0{mb, M1, half, 0};
+      for (int r = tid; r < min(mb - kNB, n); r += NT)
+        for (int q = 0; q < R; ++q) {
+          cplx v;
+          int t;
+          row_slot(A, Sg, r, q, v, t);
+          if (t >= 0) Tt[ring0.pos(r, r - mb + t)] = v;
+        }
+      for (int x = tid; x < 2 * kNB * kMaxR; x += NT) {
+        const int b = x / (kNB * kMaxR), i = (x / kMaxR) % kNB, q = x % kMaxR;
+        const int r = (b == 0 ? mb - kNB : mb) + i;
+        const int slot = (b == 0 ? kStg - 1 : 0) * kNB * kMaxR + i * kMaxR + q;
+        cplx v;
+        int t;
+        row_slot(A, Sg, r, q, v, t);
+        sv[slot] = v;
+        so[slot] = (int16_t)t;
+      }
+    }
+    // right-hand side b(enr(l)) = -1, b(nsink) = +1 (kompab :162,:183), zero on
+    // Dirichlet rows (randb :49), scaled by f (scalab :65)
+    const int rel = A.enr[l];
+    const int rsink = A.lsink ? A.nsink : -1;
+    double be = 0.0, bs = 0.0;
+    if (wave == 0) {
+      if (!(A.isdir && A.isdir[rel])) be = -fk[rel];
+      if (rsink >= 0 && !(A.isdir && A.isdir[rsink])) bs = fk[rsink];
+    }
+    // prescribed boundary values: the right-hand side comes from crt_rhs_kernel
+    const double2* Bs = A.rhs ? A.rhs + (size_t)s * n : nullptr;
+    auto rhs = [&](int r) -> cplx {
+      if (Bs) return ldc(Bs + min(max(r, 0), n - 1));
+      return mkc(r == rsink ? bs : (r == rel ? be : 0.0), 0.0);
+    };
+    cplx yv[ROWS];
+#pragma unroll
+    for (int q = 0; q < ROWS; ++q) {
+      const int rw = lane + 64 * q;
+      yv[q] = csel(wave == 0 && rw < W && rw < n, rhs(rw), mkc(0.0, 0.0));
+    }
+    stage_barrier<GR>();
+    clk.mark(0);
+
+    for (int J = 0; J < nst; ++J) {
+      const int o = J * kNB;
+      const int nbc = min(kNB, n - o);
+      Stage C;
+      C.n = n;
+      C.mb = mb;
+      C.W = W;
+      C.o = o;
+      C.ring = Ring{mb, M1, half, o % M1};
+      C.apply = J > 0;
+      C.Pp = P0 + ((J + 1) & 1) * W * kNB;
+      const int bo = (J + kStg - 1) % kStg, bc = J % kStg, bn = (J + 1) % kStg;
+      C.sv_old = sv + bo * kNB * kMaxR;
+      C.so_old = so + bo * kNB * kMaxR;
+      cplx* Pc = P0 + (J & 1) * W * kNB;
+
+      // The tile coordinates are loop invariant; re-declaring them as changed
+      // each stage keeps the position arithmetic of 16 entries per tile from
+      // being hoisted out of the stage loop (it would hold ~100 registers).
+#pragma unroll
+      for (int q = 0; q < MAXT; ++q) asm volatile("" : "+v"(rs_r0[q]), "+v"(rs_c0[q]));
+
+      // ---- phase A: waves 1-3 apply panel J-1 to panel J's columns; wave 0
+      // copies the entering batch J into panel J's last kNB rows
+      if (MF && wave > 0) {
+        __builtin_amdgcn_s_setprio(2);
+        const int T = (mb + 15) / 16;
+        for (int TI = wave - 1; TI < T; TI += NT / 64 - 1) {
+          cplx out[4];
+          mfma_tile(C, Tt, TI, 0, lane, out);
+          const int cw = lane & 15;
+#pragma unroll
+          for (int v = 0; v < 4; ++v) {
+            const int rw = 16 * TI + (lane >> 4) + 4 * v;
+            if (tile_ok(C, rw, cw)) {
+              if (cw < kNB) Pc[pix(rw, cw)] = out[v];
+              else Tt[C.ring.pos(rw, cw)] = out[v];
+            }
+          }
+        }
+      } else if (wave > 0) {
+        // the look-ahead precedes the panel on the critical path as well
+        __builtin_amdgcn_s_setprio(2);
+        for (int x = x3; x < nla; x += NT - 64) {
+          int j = 0, i = x;
+#pragma unroll
+          for (int jj = 0; jj < kNB / 2 - 1; ++jj)
+            if (i >= t2 - j) {
+              i -= t2 - j;
+              ++j;
+            }
+          const int rw0 = 2 * (i + j), cw0 = 2 * j;
+          cplx acc[2][2];
+          tile_update<2, 2>(C, Tt, rw0, cw0, acc);
+#pragma unroll
+          for (int a = 0; a < 2; ++a)
+#pragma unroll
+            for (int b = 0; b < 2; ++b)
+              if (tile_ok(C, rw0 + a, cw0 + b)) Pc[pix(rw0 + a, cw0 + b)] = acc[a][b];
+        }
+      } else {
+        const int i = lane / kNB, cw = lane % kNB, rw = mb + i;
+        const bool ok = o + rw < n && cw >= i;
+        Pc[pix(rw, cw)] = csel(ok, staged(sv + bc * kNB * kMaxR, so + bc * kNB * kMaxR, i, cw - i), mkc(0.0, 0.0));
+      }
+      __builtin_amdgcn_s_setprio(0);
+      clk.mark(0);
+      stage_barrier<GR>();
+      clk.mark(1);
+
+      // ---- phase B
+      if (wave == 0) {
+        // the panel is the stage's critical path: its wave takes issue
+        // precedence over the other workgroup's trailing wave on its SIMD
+        __builtin_amdgcn_s_setprio(3);
+        // panel J in registers: lane holds panel rows lane + 64 q
+        const int rows = min(W, n - o);
+        cplx v[ROWS][kNB];
+#pragma unroll
+        for (int q = 0; q < ROWS; ++q) {
+          const int rw = lane + 64 * q;
+#pragma unroll
+          for (int k = 0; k < kNB; ++k) {
+            const bool ok = rw < rows && k < nbc && k <= rw && rw - k <= mb;
+            v[q][k] = csel(ok, Pc[pix(min(rw, W - 1), k)], mkc(0.0, 0.0));
+          }
+        }
+        // Column c, in the L D L^T form of the same elimination: with d the
+        // pivot and s_rc the column's updated entries, l_rc = s_rc / d and the
+        // later columns take -l_rc s_c2c (= -L_rc L_c2c).  The next pivot,
+        // d_{c+1} = s_{c+1,c+1} - l_{c+1,c} s_{c+1,c}, needs no square root, so
+        // the column-to-column dependency is a reciprocal and two products;
+        // L_cc = sqrt(d), L_rc = l_rc L_cc and y_c = y'_c / L_cc follow off it.
+        // lrc is 0 for rows at or above c, so every update is unconditional
+        // (a - 0 * x leaves a unchanged).
+        // Columns past the matrix end (the last stage only) are skipped; their
+        // pivot slots hold 1 so the square roots below stay finite.  The loop leaves
+        // l_rc in place (0 at and above the diagonal); the square roots of all
+        // kNB pivots are taken at once afterwards (lane c) and turn l into L
+        // and y' into y.  Broadcasts go through LDS (one wavefront: its LDS
+        // operations complete in issue order), except the next pivot and y'_c.
+        cplx piv = rlc(v[0][0], 0);
+        bool zero_pivot = false;
+#pragma unroll
+        for (int c = 0; c < kNB; ++c) {
+          if (c < nbc) {
+            // chol :66 -- |s| <= 0 is an exact zero (a NaN pivot passes, as
+            // there); choldc :66 -- s <= 0 for the real DC pivots
+            zero_pivot |= A.dc ? piv.re <= 0.0 : (piv.re == 0.0 && piv.im == 0.0);
+            const cplx inv = crecip(piv);
+            xd[c] = piv;  // every lane stores the same value
+            xi[c] = inv;
+            cplx lrc[ROWS];
+#pragma unroll
+            for (int q = 0; q < ROWS; ++q) lrc[q] = csel(lane + 64 * q > c, cmul(v[q][c], inv), mkc(0.0, 0.0));
+            const int cn = c + 1 < kNB ? c + 1 : kNB - 1;
+            cplx pivn = mkc(1.0, 0.0);
+            if (c + 1 < nbc) pivn = rlc(cmsub(v[0][cn], lrc[0], v[0][c]), cn);
+            const cplx yp = rlc(yv[0], c);  // y'_c, final
+            // s_{c2,c} of the panel's diagonal block from lanes c+1..kNB-1
+            if (lane > c && lane < kNB) xb[lane] = v[0][c];
+#pragma unroll
+            for (int c2 = c + 1; c2 < kNB; ++c2) {
+              if (c2 < nbc) {
+                const cplx s2 = xb[c2];
+#pragma unroll
+                for (int q = 0; q < ROWS; ++q) v[q][c2] = cmsub(v[q][c2], lrc[q], s2);
+              }
+            }
+#pragma unroll
+            for (int q = 0; q < ROWS; ++q) {
+              yv[q] = cmsub(yv[q], lrc[q], yp);
+              v[q][c] = lrc[q];
+            }
+            piv = pivn;
+          } else {
+            xd[c] = mkc(1.0, 0.0);
+            xi[c] = mkc(1.0, 0.0);
+          }
+        }
+        const int lk = min(lane, kNB - 1);
+        const cplx dd = xd[lk], invd = xi[lk];
+        const cplx lcc = csqrt_(dd);
+        if (lane < kNB) xl[lane] = lcc;
+#pragma unroll
+        for (int c = 0; c < kNB; ++c) {
+          const cplx lc = xl[c];
+#pragma unroll
+          for (int q = 0; q < ROWS; ++q) v[q][c] = cmul(v[q][c], lc);
+          v[0][c] = csel(lane == c, lc, v[0][c]);
+        }
+        yv[0] = csel(lane < kNB, cmul(cmul(yv[0], lcc), invd), yv[0]);
+        if (zero_pivot && lane == 0 && atomicCAS(A.err, 0, 28) == 0) A.err[1] = sys;
+        // L of panel J: back to the panel buffer (stage J+1's operand) and out
+        // to the band rows; y of its columns is final
+#pragma unroll
+        for (int q = 0; q < ROWS; ++q) {
+          const int rw = lane + 64 * q;
+          if (rw < W) {
+#pragma unroll
+            for (int k = 0; k < kNB; ++k) {
+              const bool ok = rw < rows && k < nbc && k <= rw && rw - k <= mb;
+              Pc[pix(rw, k)] = csel(ok, v[q][k], mkc(0.0, 0.0));
+              if (ok) Lg[(size_t)(o + rw) * M1 + (k - rw + mb)] = make_double2(v[q][k].re, v[q][k].im);
+            }
+          }
+        }
+        if (lane < nbc) yg[o + lane] = make_double2(yv[0].re, yv[0].im);
+        // y of rows [o + kNB, o + W) moves up kNB rows; entering rows get b
+        const int src = (lane + kNB) & 63;
+        cplx ny[ROWS];
+#pragma unroll
+        for (int q = 0; q < ROWS; ++q) {
+          cplx a, b = mkc(0.0, 0.0);
+          a.re = __shfl(yv[q].re, src, 64);
+          a.im = __shfl(yv[q].im, src, 64);
+          if (q + 1 < ROWS) {
+            b.re = __shfl(yv[q + 1].re, src, 64);
+            b.im = __shfl(yv[q + 1].im, src, 64);
+          }
+          ny[q] = csel(lane + kNB < 64, a, b);
+          const int rwn = lane + 64 * q;
+          if (rwn >= mb) {
+            const int r = o + kNB + rwn;
+            ny[q] = csel(rwn < W && r < n, rhs(r), mkc(0.0, 0.0));
+          }
+        }
+#pragma unroll
+        for (int q = 0; q < ROWS; ++q) yv[q] = ny[q];
+        __builtin_amdgcn_s_setprio(0);
+        clk.mark(2);
+      } else {
+        // batch J+1 = rows [o + kNB + mb, o + 2 kNB + mb): loads issued first,
+        // stored to LDS after the tiles
+        cplx nv = mkc(0.0, 0.0);
+        int nt = -1;
+        if (x3 < kNB * kMaxR) row_slot(A, Sg, o + kNB + mb + x3 / kMaxR, x3 % kMaxR, nv, nt);
+        // the rest of the window: panel J-1's update in place
+        if constexpr (MF) {
+          const int T = (mb + 15) / 16;
+          int cnt = 0;
+          for (int TJ = 1; TJ < T; ++TJ)
+            for (int TI = TJ; TI < T; ++TI, ++cnt) {
+              if (cnt % (NT / 64 - 1) != wave - 1) continue;
+              cplx out[4];
+              mfma_tile(C, Tt, TI, TJ, lane, out);
+              const int cw = 16 * TJ + (lane & 15);
+#pragma unroll
+              for (int v = 0; v < 4; ++v) {
+                const int rw = 16 * TI + (lane >> 4) + 4 * v;
+                if (tile_ok(C, rw, cw)) Tt[C.ring.pos(rw, cw)] = out[v];
+              }
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < MAXT; ++q) {
+          if (!MF && rs_r0[q] >= 0) {
+            cplx acc[4][4];
+            tile_update<4, 4>(C, Tt, rs_r0[q], rs_c0[q], acc);
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+              for (int j = 0; j < 4; ++j)
+                if (tile_ok(C, rs_r0[q] + i, rs_c0[q] + j)) Tt[C.ring.pos(rs_r0[q] + i, rs_c0[q] + j)] = acc[i][j];
+          }
+        }
+        if (x3 < kNB * kMaxR) {
+          sv[bn * kNB * kMaxR + x3] = nv;
+          so[bn * kNB * kMaxR + x3] = (int16_t)nt;
+        }
+        clk.mark(2);
+      }
+      stage_barrier<GR>();
+      clk.mark(3);
+    }
+  }
+}
+
 template <int ROWS, int MAXT, int NT, bool GR, bool MF>
 __global__ __launch_bounds__(NT, 512 / NT) void crt_factor_kernel(FactorArgs A) {
   extern __shared__ double2 smem[];
@@ -759,6 +1102,31 @@ __global__ __launch_bounds__(NT, 512 / NT) void crt_factor_kernel(FactorArgs A) 
   clk.flush();
 }
 
+// The same over the systems of several models whose bands share this form:
+// work item w is system items[w].y of model items[w].x (its arguments
+// args[items[w].x]); persistent workgroups take the items in list order from
+// an atomic counter (zeroed before the launch) until the list is exhausted.
+template <int ROWS, int MAXT, int NT, bool GR, bool MF>
+__global__ __launch_bounds__(NT, 512 / NT) void crt_factor_many_kernel(const FactorArgs* __restrict__ args,
+                                                                      const int2* __restrict__ items, int nitems,
+                                                                      int* counter) {
+  extern __shared__ double2 smem[];
+  __shared__ int next;
+  PhaseClock clk;
+  clk.start(nullptr, 0);
+  for (;;) {
+    if (threadIdx.x == 0) next = atomicAdd(counter, 1);
+    __syncthreads();
+    const int w = next;
+    __syncthreads();
+    if (w >= nitems) break;
+    const int2 it = items[w];
+    const FactorArgs A = args[it.x];
+    factor_system<ROWS, MAXT, NT, GR, MF>(A, it.y, smem, clk);
+  }
+}
+
+
 template __global__ void crt_factor_kernel<1, 1, 256, false, false>(FactorArgs);
 template __global__ void crt_factor_kernel<2, 1, 256, false, false>(FactorArgs);
 template __global__ void crt_factor_kernel<2, 2, 256, false, false>(FactorArgs);
@@ -776,6 +1144,16 @@ template __global__ void crt_factor_kernel<2, 1, 512, false, true>(FactorArgs);
 template __global__ void crt_factor_kernel<3, 1, 512, false, true>(FactorArgs);
 template __global__ void crt_factor_kernel<3, 1, 512, true, true>(FactorArgs);
 template __global__ void crt_factor_kernel<4, 1, 512, true, true>(FactorArgs);
+template __global__ void crt_factor_many_kernel<1, 1, 256, false, false>(const FactorArgs*, const int2*, int, int*);
+template __global__ void crt_factor_many_kernel<2, 1, 256, false, false>(const FactorArgs*, const int2*, int, int*);
+template __global__ void crt_factor_many_kernel<2, 2, 256, false, false>(const FactorArgs*, const int2*, int, int*);
+template __global__ void crt_factor_many_kernel<2, 3, 256, false, false>(const FactorArgs*, const int2*, int, int*);
+template __global__ void crt_factor_many_kernel<3, 3, 256, false, false>(const FactorArgs*, const int2*, int, int*);
+template __global__ void crt_factor_many_kernel<2, 1, 512, false, false>(const FactorArgs*, const int2*, int, int*);
+template __global__ void crt_factor_many_kernel<2, 2, 512, false, false>(const FactorArgs*, const int2*, int, int*);
+template __global__ void crt_factor_many_kernel<3, 2, 512, false, false>(const FactorArgs*, const int2*, int, int*);
+template __global__ void crt_factor_many_kernel<3, 5, 512, true, false>(const FactorArgs*, const int2*, int, int*);
+template __global__ void crt_factor_many_kernel<4, 5, 512, true, false>(const FactorArgs*, const int2*, int, int*);
 
 // ---------------------------------------------------------------------------
 // Back substitution, one wavefront per right-hand side.  Lane q (q < 64) keeps
@@ -797,14 +1175,12 @@ template __global__ void crt_factor_kernel<4, 1, 512, true, true>(FactorArgs);
 // below-0 entries lie past the end of L, so they read zero: no lane-divergent
 // branch and no per-value mask in the loop.
 template <int DEPTH, int G, int NS>
-__global__ __launch_bounds__(64) void crt_backsolve_kernel(BacksolveArgs B) {
+__device__ __forceinline__ void backsolve_rhs(const BacksolveArgs& B, int r) {
   static_assert(DEPTH % G == 0 && 64 % DEPTH == 0, "row groups tile the block");
   static_assert(NS == 2 || NS == 4, "slot classes");
   constexpr int WIN = 64 * NS;                // rows of the slot window
   constexpr int RMASK = (WIN - 1) * 16;       // (x mod WIN) * 16 on 16-byte units
   const int lane = threadIdx.x;
-  const int r = blockIdx.x;
-  if (r >= B.nrhs) return;
   const int f = r / B.rhs_per_fac;
   const int sys = (B.fac_first + f) * B.fstride + r % B.rhs_per_fac;
   const int k = sys / B.eanz, l = sys % B.eanz;
@@ -915,10 +1291,33 @@ __global__ __launch_bounds__(64) void crt_backsolve_kernel(BacksolveArgs B) {
   }
 }
 
+template <int DEPTH, int G, int NS>
+__global__ __launch_bounds__(64) void crt_backsolve_kernel(BacksolveArgs B) {
+  if ((int)blockIdx.x < B.nrhs) backsolve_rhs<DEPTH, G, NS>(B, blockIdx.x);
+}
+
+// The same over the right-hand sides of several models: block w solves rhs
+// items[w].y with the arguments of model items[w].x.
+template <int DEPTH, int G, int NS>
+__global__ __launch_bounds__(64) void crt_backsolve_many_kernel(const BacksolveArgs* __restrict__ args,
+                                                               const int2* __restrict__ items, int nitems) {
+  if ((int)blockIdx.x >= nitems) return;
+  const int2 it = items[blockIdx.x];
+  const BacksolveArgs B = args[it.x];
+  backsolve_rhs<DEPTH, G, NS>(B, it.y);
+}
+
 template __global__ void crt_backsolve_kernel<8, 8, 2>(BacksolveArgs);
 template __global__ void crt_backsolve_kernel<8, 4, 2>(BacksolveArgs);
 template __global__ void crt_backsolve_kernel<8, 8, 4>(BacksolveArgs);
 template __global__ void crt_backsolve_kernel<8, 4, 4>(BacksolveArgs);
+template __global__ void crt_backsolve_many_kernel<8, 8, 2>(const BacksolveArgs*, const int2*, int);
+template __global__ void crt_backsolve_many_kernel<8, 4, 2>(const BacksolveArgs*, const int2*, int);
+template __global__ void crt_backsolve_many_kernel<8, 8, 4>(const BacksolveArgs*, const int2*, int);
+template __global__ void crt_backsolve_kernel<16, 8, 2>(BacksolveArgs);
+template __global__ void crt_backsolve_kernel<16, 8, 4>(BacksolveArgs);
+template __global__ void crt_backsolve_many_kernel<16, 8, 2>(const BacksolveArgs*, const int2*, int);
+template __global__ void crt_backsolve_many_kernel<16, 8, 4>(const BacksolveArgs*, const int2*, int);
 
 // ---------------------------------------------------------------------------
 // Column-band copy of shared factors for the forward sweep: row j of Lc holds

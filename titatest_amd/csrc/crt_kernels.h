@@ -57,7 +57,8 @@ struct FactorArgs {
   const int* rb_last;       // n: last file position of a prescribed node, -1 otherwise
   const double2* rwd;       // file values
   double2* rhs;             // [nsys][n] scaled right-hand sides (crt_rhs_kernel), or null
-  double2* ring;            // [grid][half][mb+1] trailing-window rings in global memory (wide bands), or null
+  double2* ring;            // [grid][ring_stride] trailing-window rings in global memory (wide bands), or null
+  size_t ring_stride;       // entries per workgroup's ring: at least half x (mb + 1)
   int* err;                 // first error code (0 = ok) and the failing system
   unsigned long long* prof; // optional: 8 phase-time accumulators (wall clock ticks)
 };
@@ -91,8 +92,12 @@ __global__ void crt_rows_kernel(FactorArgs A);
 __global__ void crt_rhs_kernel(FactorArgs A);
 template <int ROWS, int MAXT, int NT, bool GR, bool MF>
 __global__ void crt_factor_kernel(FactorArgs A);
+template <int ROWS, int MAXT, int NT, bool GR, bool MF>
+__global__ void crt_factor_many_kernel(const FactorArgs* args, const int2* items, int nitems, int* counter);
 template <int DEPTH, int G, int NS>
 __global__ void crt_backsolve_kernel(BacksolveArgs B);
+template <int DEPTH, int G, int NS>
+__global__ void crt_backsolve_many_kernel(const BacksolveArgs* args, const int2* items, int nitems);
 __global__ void crt_lcol_kernel(const double2* L, double2* Lc, int n, int mb, int nfac);
 template <int DEPTH, int G, int NS>
 __global__ void crt_fwdsweep_kernel(FwdArgs F);
