@@ -68,10 +68,27 @@ def cpu_baseline(w, nsys, K, sample=64):
         "value": w.nanz / t_full,
         "unit": "quadrupoles/s",
         "cores": cores,
+        "nproc": os.cpu_count(),
         "kind": kind,
         "sample": f"{sample} of {nsys} (k,l) systems (assemble+scale+chol+vre), "
-                  f"{dt:.2f} s wall, extrapolated linearly to the full forward",
+                  f"{dt:.2f} s wall on {cores} OpenMP threads (the reference uses at most one per "
+                  f"wavenumber) of {os.cpu_count()} host cores, extrapolated linearly to the full forward",
     }
+
+
+def k20_rule(g):
+    """20 wavenumbers built the way rwaven builds its set (src/rwaven.f90:
+    129-151) -- Gauss-Legendre on [0, 1] mapped to k0 x^2 with weights 2 k0 w
+    x, then 4 Gauss-Laguerre points mapped to k0 (x + 1) with weights k0 w e^x,
+    k0 = 1 / (2 amin) -- with 16 Legendre points instead of the rule's count."""
+    import numpy as np
+    k0 = float(g.kwn[-4] / (np.polynomial.laguerre.laggauss(4)[0][0] + 1.0))
+    x, wt = np.polynomial.legendre.leggauss(16)
+    xs, ws = 0.5 * (x + 1.0), 0.5 * wt
+    xl, wl = np.polynomial.laguerre.laggauss(4)
+    kwn = np.concatenate([k0 * xs ** 2, k0 * (xl + 1.0)])
+    kwnwi = np.concatenate([2.0 * k0 * ws * xs, k0 * wl * np.exp(xl)])
+    return kwn, kwnwi
 
 
 def uneven_batch(args):
@@ -144,6 +161,13 @@ def main():
     ap.add_argument("--tri", action="store_true",
                     help="the terrain-following triangular mesh instead: 257x100 nodes (25,700), 50,688 "
                          "triangles, mb 101, 128 electrodes, 7,074 quadrupoles, mixed sides and bottom")
+    ap.add_argument("--k20", action="store_true",
+                    help="the 240x80 workload with 20 wavenumbers (the reference's quadrature construction "
+                         "with 16 Gauss-Legendre points forced instead of its rule's 10): 1,920 systems, "
+                         "a stress variant without a parity counterpart")
+    ap.add_argument("--autotune", action="store_true",
+                    help="choose the factor workgroup size and back-substitution depth by timing them first "
+                         "(titatest_amd/autotune.py)")
     ap.add_argument("--dirichlet", action="store_true",
                     help="the same 240x80 model with Dirichlet sides and bottom: one shared factor per "
                          "wavenumber, then a forward and a backward sweep per source (kompb)")
@@ -182,7 +206,16 @@ def main():
     m = w.mesh
     g = crt.CRT(m)
     g.rwaven(w.strnr, w.vnr)
+    if args.k20:
+        g.set_wavenumbers(*k20_rule(g))
     g.precal()
+    tuned = None
+    if args.autotune:
+        from titatest_amd import autotune
+        choice, times = autotune.tune(g, w.sigma, force=True)
+        tuned = {"factor_threads": choice[0], "backsolve_depth": choice[1],
+                 "ms": {f"{t}x{d}": v for (t, d), v in times.items()}}
+        log(f"autotune: {tuned}")
     n, mb, E, K, N = m.sanz, m.mb, m.eanz, g.K, w.nanz
     nsys = K * E
     nfac = K if args.dirichlet else nsys   # factorisations per forward
@@ -260,16 +293,20 @@ def main():
                                 "240x80 quads, 19521 nodes, mb 82, 96 electrodes, "
                                 "3090 dipole-dipole quadrupoles, 14 wavenumbers x 96 sources"
                                 + (", Dirichlet sides and bottom (one factor per wavenumber)"
-                                   if args.dirichlet else ", mixed sides and bottom")),
+                                   if args.dirichlet else ", mixed sides and bottom")
+                                + (", 20-wavenumber stress variant (no parity counterpart)" if args.k20 else "")),
                    "systems": nsys, "factorisations": nfac, "nodes": n, "bandwidth": mb, "electrodes": E,
                    "quadrupoles": N, "wavenumbers": K},
-        "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP64_TFLOPS,
+        "roofline": {"bound": "mfma", "pipe": "FP64 vector ALU (v_fma_f64); MI355X's FP64 vector and "
+                     "matrix peaks are both 78.6 TFLOP/s", "achieved": achieved, "peak": PEAK_FP64_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_TFLOPS, "traffic": traffic,
                      "kernel": "crt_factor_kernel", "avg_launch_ms": fac_ms / fac_n,
                      "launches_per_step": launches,
                      "algorithmic_flops": flops, "algorithmic_bytes": lbytes},
         "stage_ms_per_step": {k: v[0] / args.steps for k, v in times.items()},
     }
+    if tuned is not None:
+        result["config"]["tuning"] = tuned
 
     if not args.skip_adjoint:
         # config with data: the adjoint gradient of the weighted misfit vs the
@@ -305,7 +342,7 @@ def main():
         }
         log(f"adjoint {adj_ms:.3f} ms, explicit J {exp_ms:.3f} ms")
 
-    if not args.no_cpu_baseline and not (args.dirichlet or args.tri):
+    if not args.no_cpu_baseline and not (args.dirichlet or args.tri or args.k20):
         log("cpu baseline sample")
         result["cpu_baseline"] = cpu_baseline(w, nsys, K, args.cpu_sample)
     print(json.dumps(result), flush=True)

@@ -179,3 +179,34 @@ def test_persistent_batch_equals_per_model_steps(crtmod):
         assert rel_err(g1, g2) < 1e-10
         assert rel_err(d1, d2) < STOL
         assert int(c1[0]) == int(c2[0])
+
+
+def test_autotuner_choice_and_cache(crtmod, tmp_path):
+    """The autotuner times the four launch choices on the model, applies the
+    fastest, records it by model shape, and a second handle of that shape
+    takes it from the cache; every choice gives the default voltages to
+    rounding."""
+    from titatest_amd import autotune
+    w = meshgen.small_block()
+    g = crtmod.CRT(w.mesh)
+    g.rwaven(w.strnr, w.vnr)
+    g.precal()
+    g.forward(w.sigma)
+    v0, _ = g.bvolt(w.strnr, w.vnr)
+    cache = tmp_path / "tune.json"
+    choice, times = autotune.tune(g, w.sigma, cache=cache, reps=1)
+    say("autotune choice", choice, times)
+    assert set(times) == {(t, d) for t in (256, 512) for d in (8, 16)}
+    assert choice == min(times, key=times.get)
+    for t, d in times:
+        g.set_tuning(t, d)
+        g.forward(w.sigma)
+        v, _ = g.bvolt(w.strnr, w.vnr)
+        assert rel_err(v, v0) < 1e-12
+    g2 = crtmod.CRT(w.mesh)
+    g2.rwaven(w.strnr, w.vnr)
+    g2.precal()
+    choice2, times2 = autotune.tune(g2, w.sigma, cache=cache)
+    assert choice2 == choice and times2 == {}
+    with pytest.raises(crtmod.CRTError):
+        g2.set_tuning(384, 8)
