@@ -152,6 +152,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample", type=int, default=256)
     ap.add_argument("--skip-adjoint", action="store_true")
+    ap.add_argument("--skip-dc", action="store_true", help="leave out the DC-twin forward comparison")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (calibrates HBM byte counters)")
     ap.add_argument("--uneven-batch", type=int, default=0, metavar="COUNT",
@@ -341,6 +342,29 @@ def main():
                            **{k: v[0] / v[1] for k, v in t_exp.items()}},
         }
         log(f"adjoint {adj_ms:.3f} ms, explicit J {exp_ms:.3f} ms")
+
+    if not args.skip_dc:
+        # the DC twin (real-entry scaled rows, factor and substitutions) on the
+        # same model against the complex forward: wall time per forward (host
+        # sigma in, nothing copied out) and HIP-event stage times
+        def per_forward(fn, reps=3):
+            fn(w.sigma)
+            torch.cuda.synchronize()
+            g.set_timing(True)
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                fn(w.sigma)
+            torch.cuda.synchronize()
+            dt = (time.perf_counter() - t0) * 1e3 / reps
+            stages = {k: v[0] / reps for k, v in g.stage_times().items()}
+            g.set_timing(False)
+            return dt, stages
+
+        dc_ms, dc_st = per_forward(g.forward_dc)
+        cx_ms, cx_st = per_forward(g.forward)
+        result["dc"] = {"forward_ms": dc_ms, "complex_forward_ms": cx_ms, "dc_over_complex": dc_ms / cx_ms,
+                        "stage_ms": dc_st, "complex_stage_ms": cx_st}
+        log(f"DC forward {dc_ms:.2f} ms, complex forward {cx_ms:.2f} ms ({dc_ms / cx_ms:.3f})")
 
     if not args.no_cpu_baseline and not (args.dirichlet or args.tri or args.k20):
         log("cpu baseline sample")

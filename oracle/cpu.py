@@ -285,6 +285,35 @@ class Oracle:
         self.sigma = np.ascontiguousarray(sigma, np.complex128)
         return self.kpotdc, self.hpotdc
 
+    def forward_dc_systems(self, sigma, systems, workers=None):
+        """kpotdc columns (sanz,) of the listed (l, k) systems, as
+        orc_forward_dc computes them (crt_oracle.c, the DC loop of
+        src/inv.f90:343-400): kompadc + randdc + scaldc, choldc, vredc,
+        pot * fak; without mixed boundaries the sources l > 0 take kompbdc's
+        vector with the factor of source 0.  Returns {(l, k): column}."""
+        import os
+        from concurrent.futures import ThreadPoolExecutor
+        m = self.mesh
+        sig = np.ascontiguousarray(sigma, np.complex128)
+        workers = workers or min(16, os.cpu_count() or 1)
+
+        def one(lk):
+            l, k = lk
+            a, b, fak = self.assemble_dc(sig, l if self.lbeta else 0, k)
+            a = self.choldc(a)
+            if not self.lbeta and l > 0:
+                enr = self._keep["enr"]
+                b = np.zeros(m.sanz)
+                b[enr[l] - 1] = -fak[enr[l] - 1]
+                if self.cm.lsink:
+                    b[self.cm.nsink - 1] = fak[self.cm.nsink - 1]
+            return self.vredc(a, b) * fak
+
+        systems = list(systems)
+        with ThreadPoolExecutor(max_workers=workers) as ex:
+            cols = list(ex.map(one, systems))
+        return dict(zip(systems, cols))
+
     def sens_dc(self, nthreads=0):
         """bsendc: real sensdc (nanz, elanz)."""
         N = len(self.strnr)

@@ -1,5 +1,6 @@
 """DC twins on the device (crt_forward_dc, crt_sensitivities_dc) against the
-oracle's DC restatement and the reference's DC golden vectors.
+oracle's DC restatement and the reference's DC golden vectors.  The DC forward
+runs the real-entry kernels (8-byte scaled rows, factor and substitutions).
 
 * kpotdc / hpotdc at 1e-10 relative, the DC voltages at 1e-10 in magnitude
   and 1e-9 rad in phase, sensdc at 1e-8 relative -- on a mesh with mixed
@@ -8,7 +9,15 @@ oracle's DC restatement and the reference's DC golden vectors.
 * the DC error rules from the host side: -sigma makes every diagonal
   negative (scaldc: 27) and a striped sigma of +1 / -0.5 by element column
   keeps every diagonal positive while the matrix is indefinite (choldc: 28);
-  the complex path runs both without error.
+  the complex path runs both without error;
+* every real kernel form (one to four panel rows per lane, 256 and 512
+  threads, the wide-band form with the ring in global memory, the shared
+  factor with its column copy and forward sweep, back-substitution depths 8
+  and 16) on selected systems against the oracle's DC systems at 1e-10;
+* on the full 240x80 mesh, the DC twin equals the complex forward of the
+  real conductivities (scalab, chol and vre on zero imaginary parts are
+  scaldc, choldc and vredc) within 1e-9 on every system, and on the worst
+  system it is as close to the long-double solution as the oracle.
 """
 import numpy as np
 import pytest
@@ -130,3 +139,69 @@ def test_dc_full_size_scaling_is_exact(crtmod):
     big = np.abs(b) >= 1e-250
     np.testing.assert_array_equal(a[big], b[big])
     assert np.all(np.abs(a - b)[~big] <= 1e-12 * np.abs(b[~big]) + 1e-300)
+
+
+DC_FORMS = {
+    # name: (workload, tuning or None); the real kernel form it selects
+    "mb22_rows1": (lambda: meshgen.small_block(), None),
+    "mb82_rows2": (lambda: meshgen.large_grf(nx_side=4, n_uniform=24, nz=80, eanz=24), None),
+    "mb82_rows2_512_depth16": (lambda: meshgen.large_grf(nx_side=4, n_uniform=24, nz=80, eanz=24), (512, 16)),
+    "mb102_rows2_maxt2": (lambda: meshgen.large_grf(nx_side=4, n_uniform=24, nz=100, eanz=24), None),
+    "mb124_rows3_maxt3": (lambda: meshgen.large_grf(nx_side=4, n_uniform=24, nz=122, eanz=24), None),
+    "mb124_rows3_512": (lambda: meshgen.large_grf(nx_side=4, n_uniform=24, nz=122, eanz=24), (512, 8)),
+    "mb142_wide_shared": (lambda: meshgen.large_grf(nx_side=4, n_uniform=24, nz=140, eanz=24, side_type=13), None),
+    "mb248_wide_rows4": (lambda: meshgen.topo_triangles(nx_nodes=24, nz_nodes=247, eanz=16, length=30.0,
+                                                        depth=40.0, seed=1031), None),
+}
+
+
+@pytest.mark.parametrize("name", list(DC_FORMS))
+def test_dc_real_forms(crtmod, oracle_lib, name):
+    make, tuning = DC_FORMS[name]
+    w = make()
+    lsytop = name.startswith("mb248")
+    m = w.mesh
+    say(name, "DC forward, mb=", m.mb, "n=", m.sanz, "E=", m.eanz)
+    g = crtmod.CRT(m, lsytop=lsytop)
+    g.rwaven(w.strnr, w.vnr)
+    g.precal()
+    if tuning:
+        g.set_tuning(*tuning)
+    (kp,) = g.forward_dc(w.sigma, want_kpot=True)
+    assert kp.dtype == np.float64
+    o = oracle_lib.Oracle(m, lsytop=lsytop)
+    o.rwaven(w.strnr, w.vnr)
+    o.precal()
+    E, K = m.eanz, g.K
+    systems = [(0, 0), (E - 1, K // 2), (E // 2, K - 1), (1, 0)]
+    ref = o.forward_dc_systems(w.sigma, systems)
+    for (l, k), col in ref.items():
+        assert rel_err(kp[:, l, k], col) < TOL_POT, (l, k, rel_err(kp[:, l, k], col))
+
+
+def test_dc_equals_complex_at_real_sigma_full_size(crtmod, oracle_lib):
+    """Every system of the 240x80 model within 1e-9 (the full-size potential
+    tolerance: these systems' conditioning), and on the worst one the DC twin
+    is as close to the long-double solution as the oracle's choldc/vredc."""
+    w = meshgen.large_grf()
+    say("DC twin vs complex forward of real sigma, 240x80 mesh, every system")
+    g = crtmod.CRT(w.mesh)
+    g.rwaven(w.strnr, w.vnr)
+    g.precal()
+    sr = np.real(w.sigma).astype(np.complex128)
+    (kd,) = g.forward_dc(w.sigma, want_kpot=True)
+    (kc,) = g.forward(sr, want_kpot=True)
+    assert np.max(np.abs(kc.imag)) == 0.0
+    n, E, K = kd.shape
+    errs = {(l, k): rel_err(kd[:, l, k], kc[:, l, k].real) for l in range(E) for k in range(K)}
+    (l, k), worst = max(errs.items(), key=lambda kv: kv[1])
+    say("worst system", (l, k), "relative difference", worst)
+    assert worst < 1e-9
+    o = oracle_lib.Oracle(w.mesh)
+    o.rwaven(w.strnr, w.vnr)
+    o.precal()
+    kl = o.solve_ld(sr, l, k).real
+    ko = o.forward_dc_systems(w.sigma, [(l, k)])[(l, k)]
+    e_dc, e_cx, e_orc = rel_err(kd[:, l, k], kl), rel_err(kc[:, l, k].real, kl), rel_err(ko, kl)
+    say("distance from the long-double solution: DC", e_dc, "complex", e_cx, "oracle DC", e_orc)
+    assert e_dc < max(3 * e_orc, 1e-13), (e_dc, e_orc)

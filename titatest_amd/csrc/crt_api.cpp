@@ -867,31 +867,34 @@ static int open_lds(crt_handle* h, const void* kern, size_t* dyn = nullptr) {
 
 // The substitutions keep a window of 64 NS rows in NS register slots per lane
 // and refill every G rows: G < 64 NS + 1 - mb keeps a refill ahead of its
-// first update.  NS = 2 up to mb = 124, NS = 4 up to kMaxBand.
-static void launch_backsolve(int mb, int nrhs, hipStream_t st, const BacksolveArgs& B, int depth = 8) {
-  if (depth == 16 && mb <= 120)
-    hipLaunchKernelGGL((crt_backsolve_kernel<16, 8, 2>), dim3(nrhs), dim3(64), 0, st, B);
-  else if (depth == 16 && mb > 124)
-    hipLaunchKernelGGL((crt_backsolve_kernel<16, 8, 4>), dim3(nrhs), dim3(64), 0, st, B);
-  else if (mb <= 120)
-    hipLaunchKernelGGL((crt_backsolve_kernel<8, 8, 2>), dim3(nrhs), dim3(64), 0, st, B);
-  else if (mb <= 124)
-    hipLaunchKernelGGL((crt_backsolve_kernel<8, 4, 2>), dim3(nrhs), dim3(64), 0, st, B);
-  else
-    hipLaunchKernelGGL((crt_backsolve_kernel<8, 8, 4>), dim3(nrhs), dim3(64), 0, st, B);
+// first update.  NS = 2 up to mb = 124, NS = 4 up to kMaxBand.  T = double
+// for the DC twin's real factors.
+static void launch_backsolve(int mb, int nrhs, hipStream_t st, const BacksolveArgs& B, int depth, bool real) {
+#define BS(D, G, NS) (real ? crt_backsolve_kernel<D, G, NS, double> : crt_backsolve_kernel<D, G, NS, cplx>)
+  void (*k)(BacksolveArgs);
+  if (depth == 16 && mb <= 120) k = BS(16, 8, 2);
+  else if (depth == 16 && mb > 124) k = BS(16, 8, 4);
+  else if (mb <= 120) k = BS(8, 8, 2);
+  else if (mb <= 124) k = BS(8, 4, 2);
+  else k = BS(8, 8, 4);
+#undef BS
+  hipLaunchKernelGGL(k, dim3(nrhs), dim3(64), 0, st, B);
 }
-static void launch_fwdsweep(int mb, int nrhs, hipStream_t st, const FwdArgs& F) {
-  if (mb <= 120)
-    hipLaunchKernelGGL((crt_fwdsweep_kernel<8, 8, 2>), dim3(nrhs), dim3(64), 0, st, F);
-  else if (mb <= 124)
-    hipLaunchKernelGGL((crt_fwdsweep_kernel<8, 4, 2>), dim3(nrhs), dim3(64), 0, st, F);
-  else
-    hipLaunchKernelGGL((crt_fwdsweep_kernel<8, 8, 4>), dim3(nrhs), dim3(64), 0, st, F);
+static void launch_fwdsweep(int mb, int nrhs, hipStream_t st, const FwdArgs& F, bool real) {
+#define FS(D, G, NS) (real ? crt_fwdsweep_kernel<D, G, NS, double> : crt_fwdsweep_kernel<D, G, NS, cplx>)
+  void (*k)(FwdArgs);
+  if (mb <= 120) k = FS(8, 8, 2);
+  else if (mb <= 124) k = FS(8, 4, 2);
+  else k = FS(8, 8, 4);
+#undef FS
+  hipLaunchKernelGGL(k, dim3(nrhs), dim3(64), 0, st, F);
 }
 
 // The factor kernel's form for a handle's band: the single-model kernel, its
 // multi-model twin (null for forms without one), threads per workgroup and
-// dynamic LDS.  Tuning (crt_set_tuning) may ask for 256 or 512 threads.
+// dynamic LDS.  Tuning (crt_set_tuning) may ask for 256 or 512 threads.  The
+// DC twin (h->dc) takes the real-entry kernel of the same shape: half the
+// LDS per entry, so narrower bands keep two workgroups per CU.
 struct FactorForm {
   void (*kern)(FactorArgs) = nullptr;
   void (*many)(const FactorArgs*, const int2*, int, int*) = nullptr;
@@ -905,9 +908,11 @@ static int choose_factor(const crt_handle* h, FactorForm& F) {
   // the substitutions address one factor with 32-bit byte offsets
   if (mb > kMaxBand || (size_t)n * (mb + 1) * 16 >= (size_t(1) << 31)) return kErrBand;
   // The trailing-window ring lives in LDS up to mb = 124 (when it fits), in a
-  // per-workgroup global scratch above (the wide-band form).
-  const bool gring = mb > 124 || crt_factor_lds_bytes(mb) > h->lds_max;
-  const size_t lds = gring ? crt_factor_lds_bytes_gr(mb) : crt_factor_lds_bytes(mb);
+  // per-workgroup global scratch above (the wide-band form).  The DC twin's
+  // real entries take half the LDS per entry.
+  const size_t es = h->dc ? 8 : 16;
+  const bool gring = mb > 124 || crt_factor_lds_bytes(mb, es, false) > h->lds_max;
+  const size_t lds = crt_factor_lds_bytes(mb, es, gring);
   if (lds > h->lds_max) return kErrBand;
   // wave 0 holds the W = mb + kNB panel rows, ROWS per lane; the other waves
   // own the trailing window's 4x4 tiles beyond the look-ahead columns, MAXT
@@ -918,25 +923,10 @@ static int choose_factor(const crt_handle* h, FactorForm& F) {
   int nt = gring || 2 * lds > h->lds_max ? 512 : 256;
   if (!gring && (h->tune_nt == 256 || h->tune_nt == 512)) nt = h->tune_nt;
   const int rows = (W + 63) / 64, maxt = std::max(1, (nrest + nt - 65) / (nt - 64));
-  // The look-ahead and trailing updates run on the vector ALU (4 x 4 and
-  // 2 x 2 tiles, one product at a time).  CRT_FACTOR_MFMA selects the form
-  // with 16 x 16 tiles on the matrix cores instead: as accurate, but slower on
-  // gfx950, whose FP64 matrix rate is below its FP64 vector rate.
-  static const bool mfma = std::getenv("CRT_FACTOR_MFMA") != nullptr;
-#define FORM(R, T, N, G) (F.kern = crt_factor_kernel<R, T, N, G, false>, F.many = crt_factor_many_kernel<R, T, N, G, false>)
-  if (mfma) {
-    if (gring) {
-      if (rows <= 3) F.kern = crt_factor_kernel<3, 1, 512, true, true>;
-      else if (rows == 4) F.kern = crt_factor_kernel<4, 1, 512, true, true>;
-    } else if (nt == 256) {
-      if (rows == 1) F.kern = crt_factor_kernel<1, 1, 256, false, true>;
-      else if (rows == 2) F.kern = crt_factor_kernel<2, 1, 256, false, true>;
-      else if (rows == 3) F.kern = crt_factor_kernel<3, 1, 256, false, true>;
-    } else {
-      if (rows <= 2) F.kern = crt_factor_kernel<2, 1, 512, false, true>;
-      else if (rows == 3) F.kern = crt_factor_kernel<3, 1, 512, false, true>;
-    }
-  } else if (gring) {
+#define FORM(R, T, N, G)                                                   \
+  (h->dc ? (F.kern = crt_factor_kernel<R, T, N, G, double>, F.many = nullptr) \
+         : (F.kern = crt_factor_kernel<R, T, N, G, cplx>, F.many = crt_factor_many_kernel<R, T, N, G>))
+  if (gring) {
     if (rows <= 3 && maxt <= 5) FORM(3, 5, 512, true);
     else if (rows == 4 && maxt <= 5) FORM(4, 5, 512, true);
   } else if (nt == 256) {
@@ -1109,7 +1099,8 @@ static int run_forward(crt_handle* h) {
     const unsigned rgrid = (unsigned)(((size_t)cnt * n + 255) / 256);
     tstart(h, "assemble_scale");
     hipLaunchKernelGGL(crt_scale_kernel, dim3(rgrid), dim3(256), 0, h->stream, A);
-    hipLaunchKernelGGL(crt_rows_kernel, dim3((unsigned)(((size_t)cnt * n * h->R + 255) / 256)), dim3(256), 0,
+    hipLaunchKernelGGL(h->dc ? crt_rows_kernel<double> : crt_rows_kernel<cplx>,
+                       dim3((unsigned)(((size_t)cnt * n * h->R + 255) / 256)), dim3(256), 0,
                        h->stream, A);
     if (rb2) hipLaunchKernelGGL(crt_rhs_kernel, dim3(rgrid), dim3(256), 0, h->stream, A);
     tstop(h);
@@ -1133,9 +1124,15 @@ static int run_forward(crt_handle* h) {
         FactorArgs Ap = A;
         Ap.sys_first = first + p0;
         Ap.nsys = pc;
-        Ap.L = A.L + (size_t)p0 * n * (mb + 1);
-        Ap.S = A.S + (size_t)p0 * n * h->R;
-        Ap.y = A.y + (size_t)p0 * n;
+        // system p0's scaled rows, L and y: entries of 16 bytes, or 8 for
+        // the DC twin's real kernels
+        const size_t es = h->dc ? 8 : 16;
+        auto at_entry = [es](double2* p, size_t entries) {
+          return reinterpret_cast<double2*>(reinterpret_cast<char*>(p) + entries * es);
+        };
+        Ap.L = at_entry(A.L, (size_t)p0 * n * (mb + 1));
+        Ap.S = at_entry(A.S, (size_t)p0 * n * h->R);
+        Ap.y = at_entry(A.y, (size_t)p0 * n);
         Ap.fak = A.fak + (size_t)p0 * n;
         if (A.rhs) Ap.rhs = A.rhs + (size_t)p0 * n;
         tstart(h, "factor_solve");
@@ -1153,7 +1150,7 @@ static int run_forward(crt_handle* h) {
           HIPCHK(hipStreamWaitEvent(h->stream2, h->ev_fac, 0));
         }
         const int slot = tstart_on(h, "backsolve", bs);
-        launch_backsolve(mb, pc, bs, Bp, h->tune_depth);
+        launch_backsolve(mb, pc, bs, Bp, h->tune_depth, h->dc);
         tstop_on(h, slot, bs);
       }
       HIPCHK(hipEventRecord(h->ev_join, h->stream2));
@@ -1168,17 +1165,18 @@ static int run_forward(crt_handle* h) {
       F.fac_first = first;
       tstart(h, "column_copy");
       const size_t ncopy = (size_t)cnt * n * (mb + 1);
-      hipLaunchKernelGGL(crt_lcol_kernel, dim3((unsigned)((ncopy + 255) / 256)), dim3(256), 0, h->stream,
+      hipLaunchKernelGGL(h->dc ? crt_lcol_kernel<double> : crt_lcol_kernel<cplx>,
+                         dim3((unsigned)((ncopy + 255) / 256)), dim3(256), 0, h->stream,
                          (const double2*)A.L, Lc, n, mb, cnt);
       tstop(h);
       tstart(h, "forward_sweep");
-      launch_fwdsweep(mb, nrhs, h->stream, F);
+      launch_fwdsweep(mb, nrhs, h->stream, F, h->dc);
       tstop(h);
     }
     B.nrhs = nrhs;
     B.fac_first = first;
     tstart(h, "backsolve");
-    launch_backsolve(mb, nrhs, h->stream, B, h->tune_depth);
+    launch_backsolve(mb, nrhs, h->stream, B, h->tune_depth, h->dc);
     tstop(h);
   }
   // electrode slice of the back-transform, used by the voltage stage
@@ -1286,7 +1284,7 @@ int crt_forward_many(int32_t nmodels, crt_handle* const* hs, const double* const
                        h->stream, A, K, av);
     const unsigned rgrid = (unsigned)(((size_t)nsys * n + 255) / 256);
     hipLaunchKernelGGL(crt_scale_kernel, dim3(rgrid), dim3(256), 0, h->stream, A);
-    hipLaunchKernelGGL(crt_rows_kernel, dim3((unsigned)(((size_t)nsys * n * h->R + 255) / 256)), dim3(256), 0,
+    hipLaunchKernelGGL(crt_rows_kernel<cplx>, dim3((unsigned)(((size_t)nsys * n * h->R + 255) / 256)), dim3(256), 0,
                        h->stream, A);
     if (rb2) hipLaunchKernelGGL(crt_rhs_kernel, dim3(rgrid), dim3(256), 0, h->stream, A);
     HIPCHK(hipEventRecord(h->ev_fac, h->stream));

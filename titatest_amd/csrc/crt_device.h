@@ -10,6 +10,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 struct cplx {
   double re, im;
 };
@@ -99,6 +101,58 @@ __device__ __forceinline__ cplx csqrt_(cplx z) {
 
 __device__ __forceinline__ cplx clog_(cplx z) { return mkc(log(cabs_(z)), atan2(z.im, z.re)); }
 
+// Entry-generic forms for the kernels instantiated on complex entries (cplx,
+// stored in global memory as double2) and on the real entries of the DC twin
+// (double, stored as double).  The complex forms are the functions above.
+template <class T>
+using store_t = std::conditional_t<std::is_same<T, cplx>::value, double2, double>;
+template <class T>
+__device__ __forceinline__ T sreal(double x);  // x + 0i
+template <>
+__device__ __forceinline__ cplx sreal<cplx>(double x) { return mkc(x, 0.0); }
+template <>
+__device__ __forceinline__ double sreal<double>(double x) { return x; }
+template <class T>
+__device__ __forceinline__ T szero() { return sreal<T>(0.0); }
+template <class T>
+__device__ __forceinline__ T sfrom(cplx z);  // the entry of a complex value (DC: its real part)
+template <>
+__device__ __forceinline__ cplx sfrom<cplx>(cplx z) { return z; }
+template <>
+__device__ __forceinline__ double sfrom<double>(cplx z) { return z.re; }
+__device__ __forceinline__ double msub(double a, double b, double c) { return a - b * c; }
+__device__ __forceinline__ cplx msub(cplx a, cplx b, cplx c) { return cmsub(a, b, c); }
+__device__ __forceinline__ double madd(double a, double b, double c) { return a + b * c; }
+__device__ __forceinline__ cplx madd(cplx a, cplx b, cplx c) { return cmadd(a, b, c); }
+__device__ __forceinline__ double smul(double a, double b) { return a * b; }
+__device__ __forceinline__ cplx smul(cplx a, cplx b) { return cmul(a, b); }
+__device__ __forceinline__ double sneg(double a) { return -a; }
+__device__ __forceinline__ cplx sneg(cplx a) { return mkc(-a.re, -a.im); }
+__device__ __forceinline__ double sadd_re(double a, double x) { return a + x; }
+__device__ __forceinline__ cplx sadd_re(cplx a, double x) { return mkc(a.re + x, a.im); }
+__device__ __forceinline__ double sel(bool c, double a, double b) { return c ? a : b; }
+__device__ __forceinline__ cplx sel(bool c, cplx a, cplx b) { return csel(c, a, b); }
+// 1/x as for crecip: x scaled by 2^-il into [1, 2) (exact), v_rcp_f64
+// refined by two Newton steps, rescaled.
+__device__ __forceinline__ double srecip(double x) {
+  const double m = fabs(x);
+  const int il = (m != 0.0 && isfinite(m)) ? __builtin_amdgcn_frexp_exp(m) - 1 : 0;
+  const double xs = __builtin_amdgcn_ldexp(x, -il);
+  double r = __builtin_amdgcn_rcp(xs);
+  double e = fma(-xs, r, 1.0);
+  r = fma(r, e, r);
+  e = fma(-xs, r, 1.0);
+  r = fma(r, e, r);
+  return __builtin_amdgcn_ldexp(r, -il);
+}
+__device__ __forceinline__ cplx srecip(cplx z) { return crecip(z); }
+__device__ __forceinline__ double ssqrt(double x) { return sqrt(x); }
+__device__ __forceinline__ cplx ssqrt(cplx z) { return csqrt_(z); }
+// Values the compiler must treat as changed here (keeps an update from being
+// moved across the point).
+__device__ __forceinline__ void spin(double& x) { asm volatile("" : "+v"(x)); }
+__device__ __forceinline__ void spin(cplx& z) { asm volatile("" : "+v"(z.re), "+v"(z.im)); }
+
 // Workgroup barrier ordering LDS only.  __syncthreads() also waits for every
 // outstanding global store of the calling wave; kernels whose threads only
 // communicate through LDS use this instead so stores stream out behind it.
@@ -129,3 +183,25 @@ __device__ __forceinline__ cplx ldc(const double2* p) {
   return mkc(v.x, v.y);
 }
 __device__ __forceinline__ void stc(double2* p, cplx z) { *p = make_double2(z.re, z.im); }
+
+// Entry loads and stores for the entry-generic kernels.
+__device__ __forceinline__ cplx ld(const double2* p) { return ldc(p); }
+__device__ __forceinline__ double ld(const double* p) { return *p; }
+__device__ __forceinline__ void st(double2* p, cplx z) { stc(p, z); }
+__device__ __forceinline__ void st(double* p, double x) { *p = x; }
+// An 8-byte buffer load (the range check of bldc).
+__device__ __forceinline__ double bldd(brsrc r, unsigned off) {
+  return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, (int)off, 0, 0));
+}
+__device__ __forceinline__ void bld(brsrc r, unsigned off, cplx& z) { z = bldc(r, off); }
+__device__ __forceinline__ void bld(brsrc r, unsigned off, double& x) { x = bldd(r, off); }
+// Buffer stores: a store at a byte offset at or past the buffer's size is
+// dropped by the memory pipeline (the store-side range check).
+__device__ __forceinline__ void bst(brsrc r, unsigned off, cplx z) {
+  const dbl2v v = {z.re, z.im};
+  __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u32x4, v), r, (int)off, 0, 0);
+}
+__device__ __forceinline__ void bst(brsrc r, unsigned off, double x) {
+  typedef unsigned u32x2 __attribute__((ext_vector_type(2)));
+  __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, x), r, (int)off, 0, 0);
+}

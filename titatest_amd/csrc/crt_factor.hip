@@ -1,6 +1,7 @@
 // Batched complex-symmetric band factorisation with fused forward sweep
 // (gfx950), the assembly/scaling kernels that feed it, and the
-// wavefront-per-system back substitution.
+// wavefront-per-system substitutions; complex entries, or the real entries of
+// the DC twin (scaldc, choldc, vredc).
 //
 // crt_area_values_kernel: per wavenumber k and row r, the sums of the area
 //   element contributions to each stored band entry of row r, in element order
@@ -14,16 +15,17 @@
 //   a_rr/|a_rr|, a_rc f_r f_c (scalab, src/scalab.f90:52-76).  One record of R
 //   values per row (the row's distinct band offsets).
 //
-// crt_factor_kernel: band L L^T without conjugation (chol, src/chol.f90:44-78)
-//   plus the forward substitution L y = b (vre, src/vre.f90:45-56), one
-//   workgroup per system, persistent over systems.  The matrix is processed in
-//   stages of kNB columns.  Stage J: waves 1-3 first apply panel J-1 to the
-//   kNB columns of panel J (the look-ahead), then, while wave 0 factors panel J
-//   in registers without any workgroup barrier, they apply panel J-1 to the
-//   rest of the trailing window.  Two barriers per stage.  Every entry receives
-//   -L_ik L_jk in ascending k and is divided by L_jj once (the reference's
-//   arithmetic per entry).  Rows enter from the scaled-row records one batch of
-//   kNB rows per stage, staged in LDS a stage ahead.
+// crt_factor_kernel: band L L^T without conjugation (chol, src/chol.f90:44-78;
+//   choldc for DC) plus the forward substitution L y = b (vre, src/vre.f90:
+//   45-56), one workgroup per system, persistent over systems (factor_system
+//   below).  The matrix is processed in stages of kNB columns.  Stage J: the
+//   update waves first apply panel J-1 to the kNB columns of panel J (the
+//   look-ahead), then, while wave 0 factors panel J in registers without any
+//   workgroup barrier, they apply panel J-1 to the rest of the trailing
+//   window.  Two barriers per stage.  Every entry receives -L_ik L_jk in
+//   ascending k and is divided by L_jj once (the reference's arithmetic per
+//   entry).  The window lives in a ring that rows enter from the scaled-row
+//   records one batch of kNB rows per stage.
 //
 // crt_backsolve_kernel: x = -L^{-T} y (vre, src/vre.f90:58-69) and
 //   kpot = x * f (src/fem.f90:385), one wavefront per system.
@@ -33,8 +35,6 @@
 #include "crt_kernels.h"
 
 namespace {
-
-constexpr int kStg = 3;  // staged row batches: the stage's previous, current and next
 
 // Optional per-phase timing (constant-rate clock), kept by thread 0 (wave 0:
 // accumulators 0-3) and thread 64 (waves 1-3: accumulators 4-7).
@@ -96,11 +96,12 @@ __device__ __forceinline__ int pix(int rw, int k) { return rw * kNB + (k ^ ((rw 
 
 // Value at band offset t of row i of a staged batch (0 when t is not one of
 // the row's stored offsets).
-__device__ __forceinline__ cplx staged(const cplx* sv, const int16_t* so, int i, int t) {
+template <class T>
+__device__ __forceinline__ T staged(const T* sv, const int16_t* so, int i, int t) {
   int qf = -1;
 #pragma unroll
   for (int q = 0; q < kMaxR; ++q) qf = (so[i * kMaxR + q] == t) ? q : qf;
-  return csel(qf >= 0, sv[i * kMaxR + max(qf, 0)], mkc(0.0, 0.0));
+  return sel(qf >= 0, sv[i * kMaxR + max(qf, 0)], szero<T>());
 }
 
 __device__ __forceinline__ double rl(double x, int l) {
@@ -110,161 +111,35 @@ __device__ __forceinline__ double rl(double x, int l) {
   return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
 }
 __device__ __forceinline__ cplx rlc(cplx z, int l) { return mkc(rl(z.re, l), rl(z.im, l)); }
+__device__ __forceinline__ double srl(double x, int l) { return rl(x, l); }
+__device__ __forceinline__ cplx srl(cplx z, int l) { return rlc(z, l); }
+__device__ __forceinline__ double sshfl(double x, int src) { return __shfl(x, src, 64); }
+__device__ __forceinline__ cplx sshfl(cplx z, int src) { return mkc(__shfl(z.re, src, 64), __shfl(z.im, src, 64)); }
 
 // Scaled-row record slot q of row r (value and band offset), or (0, -1).
-__device__ __forceinline__ void row_slot(const FactorArgs& A, const double2* Sg, int r, int q, cplx& v, int& t) {
-  v = mkc(0.0, 0.0);
+template <class E, class T>
+__device__ __forceinline__ void row_slot(const FactorArgs& A, const E* Sg, int r, int q, T& v, int& t) {
+  v = szero<T>();
   t = -1;
   if (r >= 0 && r < A.n && q < A.R) {
     t = A.pat_t[(size_t)r * A.R + q];
-    if (t >= 0) v = ldc(Sg + (size_t)r * A.R + q);
+    if (t >= 0) v = ld(Sg + (size_t)r * A.R + q);
   }
 }
 
+template <class T>
 struct Stage {
   int n, mb, W, o;
   Ring ring;
   bool apply;              // panel J-1 exists
-  const cplx* Pp;          // panel J-1 (its window coordinates: stage rows + kNB)
-  const cplx* sv_old;      // batch J-1 = stage rows [mb - kNB, mb)
-  const int16_t* so_old;
+  const T* Pp;             // panel J-1 (its window coordinates: stage rows + kNB)
 };
 
 // Entry (rw, cw) of a tile exists: lower triangle, inside the window and the
 // matrix.
-__device__ __forceinline__ bool tile_ok(const Stage& C, int rw, int cw) {
+template <class T>
+__device__ __forceinline__ bool tile_ok(const Stage<T>& C, int rw, int cw) {
   return cw <= rw && rw < C.mb && C.o + rw < C.n;
-}
-
-// TR x TC tile of the stage window at (rw0, cw0): the current values (ring, or
-// the staged batch for the window's last kNB rows) minus panel J-1's rank-kNB
-// product, accumulated in ascending k.  For the 2x2 look-ahead tiles the
-// current values are fetched without branches, in two rounds of LDS loads
-// whose latencies overlap: the ring entries and each tile row's offset record
-// of the staged batch (kMaxR int16 offsets, one 16-byte load), then the
-// staged values at the matching slots; every index is clamped in range and the
-// choice is a select.
-template <int TR, int TC>
-__device__ __forceinline__ void tile_update(const Stage& C, const cplx* Tt, int rw0, int cw0, cplx acc[TR][TC]) {
-  if constexpr (TR * TC <= 4) {
-    // look-ahead tiles (on the critical path): branch-free two-round fetch
-    int slot[TR][TC];
-#pragma unroll
-    for (int i = 0; i < TR; ++i) {
-      const int rw = rw0 + i;
-      const int ibc = min(max(rw - (C.mb - kNB), 0), kNB - 1);
-      const u32x4 rec = *reinterpret_cast<const u32x4*>(C.so_old + ibc * kMaxR);
-      const int rwc = min(max(rw, 0), C.mb - 1);
-#pragma unroll
-      for (int j = 0; j < TC; ++j) {
-        const int cw = cw0 + j;
-        acc[i][j] = Tt[C.ring.pos(rwc, min(max(cw, 0), rwc))];
-        const int t = cw - rw + C.mb;
-        int q = -1;
-#pragma unroll
-        for (int sl = 0; sl < kMaxR; ++sl) {
-          const int off = (int)(short)(unsigned short)(rec[sl >> 1] >> (16 * (sl & 1)));
-          q = off == t ? sl : q;
-        }
-        slot[i][j] = q;
-      }
-    }
-#pragma unroll
-    for (int i = 0; i < TR; ++i) {
-      const int rw = rw0 + i;
-      const int ib = rw - (C.mb - kNB);
-      const int ibc = min(max(ib, 0), kNB - 1);
-#pragma unroll
-      for (int j = 0; j < TC; ++j) {
-        const cplx sv = C.sv_old[ibc * kMaxR + max(slot[i][j], 0)];
-        const cplx v = ib >= 0 ? csel(slot[i][j] >= 0, sv, mkc(0.0, 0.0)) : acc[i][j];
-        acc[i][j] = csel(tile_ok(C, rw, cw0 + j), v, mkc(0.0, 0.0));
-      }
-    }
-  } else {
-    // trailing tiles: only the few on the window's last kNB rows take the
-    // staged batch's slot search
-#pragma unroll
-    for (int i = 0; i < TR; ++i)
-#pragma unroll
-      for (int j = 0; j < TC; ++j) {
-        const int rw = rw0 + i, cw = cw0 + j;
-        cplx v = mkc(0.0, 0.0);
-        if (tile_ok(C, rw, cw)) {
-          const int ib = rw - (C.mb - kNB);
-          v = ib >= 0 ? staged(C.sv_old, C.so_old, ib, cw - rw + C.mb) : Tt[C.ring.pos(rw, cw)];
-        }
-        acc[i][j] = v;
-      }
-  }
-  if (!C.apply) return;
-  // operand rows of panel J-1 (stage rows + kNB)
-  int ra[TR], ca[TC];
-#pragma unroll
-  for (int i = 0; i < TR; ++i) ra[i] = min(rw0 + i + kNB, C.W - 1);
-#pragma unroll
-  for (int j = 0; j < TC; ++j) ca[j] = min(cw0 + j + kNB, C.W - 1);
-  // the 2x2 look-ahead tiles issue all their operand loads at once; the
-  // 4x4 trailing tiles keep one k in registers at a time
-#pragma unroll (TR * TC <= 4 ? kNB : 1)
-  for (int k = 0; k < kNB; ++k) {
-    cplx a[TR], b[TC];
-#pragma unroll
-    for (int i = 0; i < TR; ++i) a[i] = C.Pp[pix(ra[i], k)];
-#pragma unroll
-    for (int j = 0; j < TC; ++j) b[j] = C.Pp[pix(ca[j], k)];
-#pragma unroll
-    for (int i = 0; i < TR; ++i)
-#pragma unroll
-      for (int j = 0; j < TC; ++j) acc[i][j] = cmsub(acc[i][j], a[i], b[j]);
-  }
-}
-
-typedef double v4d __attribute__((ext_vector_type(4)));
-
-// One 16 x 16 tile of the stage window on the matrix cores: window rows
-// 16 TI + (lane >> 4) + 4 v (v < 4) and column 16 TJ + (lane & 15) -- the
-// v_mfma_f64_16x16x4f64 accumulator layout.  out[v] = S - P with S the
-// current value (ring, or the staged batch for the window's last kNB rows; 0
-// outside the window's lower triangle) and P = sum_k L_rk L_ck over panel
-// J-1's kNB columns, formed from zero: the real part as the L_re L_re
-// products over k, then the -L_im L_im ones, the imaginary part as L_re L_im
-// then L_im L_re, each chain accumulated by the matrix core, and subtracted
-// from S once.  (A product block formed first and subtracted once is as
-// close to the exact solution as the reference's one-product-at-a-time
-// update on the worst-conditioned systems; accumulating the products into S
-// itself is not.)  Every lane takes part (the matrix cores need the whole
-// wavefront); the caller masks the stores.
-__device__ __forceinline__ void mfma_tile(const Stage& C, const cplx* Tt, int TI, int TJ, int lane, cplx out[4]) {
-  const int col = lane & 15, q = lane >> 4;
-  const int cw = 16 * TJ + col;
-#pragma unroll
-  for (int v = 0; v < 4; ++v) {
-    const int rw = 16 * TI + q + 4 * v;
-    cplx sv = mkc(0.0, 0.0);
-    if (tile_ok(C, rw, cw)) {
-      const int ib = rw - (C.mb - kNB);
-      sv = ib >= 0 ? staged(C.sv_old, C.so_old, ib, cw - rw + C.mb) : Tt[C.ring.pos(rw, cw)];
-    }
-    out[v] = sv;
-  }
-  if (!C.apply) return;
-  // operand rows of panel J-1 (stage rows + kNB), clamped for the padding
-  // rows of the last tile row (their results are not stored)
-  const int ra = min(16 * TI + col + kNB, C.W - 1), ca = min(cw + kNB, C.W - 1);
-  const cplx a0 = C.Pp[pix(ra, q)], a1 = C.Pp[pix(ra, q + 4)];
-  const cplx b0 = C.Pp[pix(ca, q)], b1 = C.Pp[pix(ca, q + 4)];
-  v4d pr = {0.0, 0.0, 0.0, 0.0}, pi = {0.0, 0.0, 0.0, 0.0};
-  pr = __builtin_amdgcn_mfma_f64_16x16x4f64(a0.re, b0.re, pr, 0, 0, 0);
-  pr = __builtin_amdgcn_mfma_f64_16x16x4f64(a1.re, b1.re, pr, 0, 0, 0);
-  pr = __builtin_amdgcn_mfma_f64_16x16x4f64(-a0.im, b0.im, pr, 0, 0, 0);
-  pr = __builtin_amdgcn_mfma_f64_16x16x4f64(-a1.im, b1.im, pr, 0, 0, 0);
-  pi = __builtin_amdgcn_mfma_f64_16x16x4f64(a0.re, b0.im, pi, 0, 0, 0);
-  pi = __builtin_amdgcn_mfma_f64_16x16x4f64(a1.re, b1.im, pi, 0, 0, 0);
-  pi = __builtin_amdgcn_mfma_f64_16x16x4f64(a0.im, b0.re, pi, 0, 0, 0);
-  pi = __builtin_amdgcn_mfma_f64_16x16x4f64(a1.im, b1.re, pi, 0, 0, 0);
-#pragma unroll
-  for (int v = 0; v < 4; ++v) out[v] = mkc(out[v].re - pr[v], out[v].im - pi[v]);
 }
 
 // lower-triangle tile index x -> (ti, tj), x = ti (ti + 1) / 2 + tj
@@ -330,7 +205,8 @@ __global__ void crt_scale_kernel(FactorArgs A) {
 // Scaled band entries of every row of every system of the batch: a_rr/|a_rr|
 // on the diagonal, (f_r f_c) a_rc elsewhere (scalab :58-73).  One thread per
 // entry (system, row, slot), so that consecutive threads read av and write S
-// at consecutive addresses.
+// at consecutive addresses.  T = double (the DC twin) stores the real parts.
+template <class T>
 __global__ void crt_rows_kernel(FactorArgs A) {
   const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= (long)A.nsys * A.n * A.R) return;
@@ -350,8 +226,10 @@ __global__ void crt_rows_kernel(FactorArgs A) {
     if (t == A.mb) v = A.dc ? mkc(1.0, 0.0) : cdiv(v, mkc(cabs_(v), 0.0));  // scaldc :60
     else v = rmul(fk[r] * fk[r - A.mb + t], v);
   }
-  A.S[idx] = make_double2(v.re, v.im);
+  st(reinterpret_cast<store_t<T>*>(A.S) + idx, sfrom<T>(v));
 }
+template __global__ void crt_rows_kernel<cplx>(FactorArgs);
+template __global__ void crt_rows_kernel<double>(FactorArgs);
 
 // Right-hand side of every system of the batch when boundary values are
 // prescribed: kompab's source and sink (:162, :183; the sink wins a tie), zero
@@ -408,18 +286,6 @@ __device__ __forceinline__ void stage_barrier() {
 }
 
 // ---------------------------------------------------------------------------
-// Lookahead band factorisation.  ROWS: panel rows per lane of wave 0
-// (W = mb + kNB <= 64 ROWS); MAXT: trailing tiles per thread of the other
-// waves; NT threads per workgroup: 256 (two workgroups per CU) or, when the
-// LDS of one workgroup leaves no room for a second (mb above ~98), 512, so
-// that the CU still holds 8 wavefronts (7 for the look-ahead and trailing
-// update).  GR: the trailing-window ring lives in a global scratch of half x
-// (mb + 1) entries per workgroup (A.ring) instead of LDS, for bands whose
-// ring exceeds the CU's LDS (mb above 124); the panels and staged rows stay
-// in LDS.  MF: the look-ahead and trailing updates run on the matrix cores in
-// 16 x 16 tiles (mfma_tile): in phase A the tiles of window columns [0, 16)
-// (panel J's columns to the panel buffer, columns [kNB, 16) back to the
-// ring), in phase B the rest; MAXT is unused.
 // The trailing 4x4 tiles of this thread for band mb (window rows and columns
 // from 8 on; tile x of the lower triangle goes to thread 64 + x mod (NT - 64)),
 // -1 where the thread has none.
@@ -442,663 +308,351 @@ __device__ __forceinline__ void trailing_tiles(int mb, int (&rs_r0)[MAXT], int (
   }
 }
 
-template <int ROWS, int MAXT, int NT, bool GR, bool MF>
-__device__ __forceinline__ void factor_system(const FactorArgs& A, int s, double2* smem, PhaseClock& clk) {
-  const int n = A.n, mb = A.mb, R = A.R;
-  const int W = mb + kNB, M1 = mb + 1, half = (mb + 1) / 2;
-  // half x M1 ring slots
-  cplx* Tt = GR ? reinterpret_cast<cplx*>(A.ring) + (size_t)blockIdx.x * A.ring_stride : reinterpret_cast<cplx*>(smem);
-  cplx* P0 = GR ? reinterpret_cast<cplx*>(smem) : Tt + half * M1;  // 2 panels of W x kNB
-  cplx* sv = P0 + 2 * W * kNB;                // kStg batches of kNB x kMaxR values
-  cplx* xb = sv + kStg * kNB * kMaxR;         // kNB: column broadcast of the panel
-  cplx* xd = xb + kNB;                        // kNB: the panel's pivots d_c
-  cplx* xi = xd + kNB;                        // kNB: 1/d_c
-  cplx* xl = xi + kNB;                        // kNB: L_cc = sqrt(d_c)
-  int16_t* so = reinterpret_cast<int16_t*>(xl + kNB);  // kStg x kNB x kMaxR offsets
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int x3 = tid - 64;                    // thread index within waves 1-3
-  const int nst = (n + kNB - 1) / kNB;
-  const int tw = (mb + 3) / 4;                          // 4x4 tiles per window side
-  // look-ahead: 2x2 tiles over window rows [0, mb) x columns [0, kNB), column
-  // by column of tiles; tile column j holds the tile rows i >= j
-  const int t2 = (mb + 1) / 2;
-  int nla = 0;
+// TR x TC tile of the stage window at (rw0, cw0) in the ring-resident form:
+// the current values from the ring (every window entry lives there; 0
+// outside the window's lower triangle and the matrix, branch-free with the
+// positions clamped in range) minus panel J-1's rank-kNB product, in
+// ascending k.
+template <int TR, int TC, class T>
+__device__ __forceinline__ void ring_tile(const Stage<T>& C, const T* Tt, int rw0, int cw0, T acc[TR][TC]) {
 #pragma unroll
-  for (int j = 0; j < kNB / 2; ++j) nla += max(t2 - j, 0);
-  // trailing 4x4 tiles of this thread
-  int rs_r0[MAXT], rs_c0[MAXT];
-  trailing_tiles<MAXT, NT>(mb, rs_r0, rs_c0);
-
-  {
-    const int sys = (A.sys_first + s) * A.sys_stride;
-    const int l = sys % A.eanz;
-    double2* Lg = A.L + (size_t)s * n * M1;
-    double2* yg = A.y + (size_t)s * n;
-    const double2* Sg = A.S + (size_t)s * n * R;
-    const double* fk = A.fak + (size_t)s * n;
-
-    // ---- prologue: ring zeroed, rows [0, mb - kNB) in place, batches
-    // [mb - kNB, mb) and [mb, mb + kNB) staged, wave 0's right-hand side
-    for (int i = tid; i < half * M1; i += NT) Tt[i] = mkc(0.0, 0.0);
-    stage_barrier<GR>();
-    {
-      Ring ring0{mb, M1, half, 0};
-      for (int r = tid; r < min(mb - kNB, n); r += NT)
-        for (int q = 0; q < R; ++q) {
-          cplx v;
-          int t;
-          row_slot(A, Sg, r, q, v, t);
-          if (t >= 0) Tt[ring0.pos(r, r - mb + t)] = v;
-        }
-      for (int x = tid; x < 2 * kNB * kMaxR; x += NT) {
-        const int b = x / (kNB * kMaxR), i = (x / kMaxR) % kNB, q = x % kMaxR;
-        const int r = (b == 0 ? mb - kNB : mb) + i;
-        const int slot = (b == 0 ? kStg - 1 : 0) * kNB * kMaxR + i * kMaxR + q;
-        cplx v;
-        int t;
-        row_slot(A, Sg, r, q, v, t);
-        sv[slot] = v;
-        so[slot] = (int16_t)t;
-      }
+  for (int i = 0; i < TR; ++i) {
+    const int rw = rw0 + i;
+    const int rwc = min(max(rw, 0), C.mb - 1);
+#pragma unroll
+    for (int j = 0; j < TC; ++j) {
+      const int cw = cw0 + j;
+      acc[i][j] = sel(tile_ok(C, rw, cw), Tt[C.ring.pos(rwc, min(max(cw, 0), rwc))], szero<T>());
     }
-    // right-hand side b(enr(l)) = -1, b(nsink) = +1 (kompab :162,:183), zero on
-    // Dirichlet rows (randb :49), scaled by f (scalab :65)
-    const int rel = A.enr[l];
-    const int rsink = A.lsink ? A.nsink : -1;
-    double be = 0.0, bs = 0.0;
-    if (wave == 0) {
-      if (!(A.isdir && A.isdir[rel])) be = -fk[rel];
-      if (rsink >= 0 && !(A.isdir && A.isdir[rsink])) bs = fk[rsink];
-    }
-    // prescribed boundary values: the right-hand side comes from crt_rhs_kernel
-    const double2* Bs = A.rhs ? A.rhs + (size_t)s * n : nullptr;
-    auto rhs = [&](int r) -> cplx {
-      if (Bs) return ldc(Bs + min(max(r, 0), n - 1));
-      return mkc(r == rsink ? bs : (r == rel ? be : 0.0), 0.0);
-    };
-    cplx yv[ROWS];
+  }
+  if (!C.apply) return;
+  int ra[TR], ca[TC];
 #pragma unroll
-    for (int q = 0; q < ROWS; ++q) {
-      const int rw = lane + 64 * q;
-      yv[q] = csel(wave == 0 && rw < W && rw < n, rhs(rw), mkc(0.0, 0.0));
-    }
-    stage_barrier<GR>();
-    clk.mark(0);
-
-    for (int J = 0; J < nst; ++J) {
-      const int o = J * kNB;
-      const int nbc = min(kNB, n - o);
-      Stage C;
-      C.n = n;
-      C.mb = mb;
-      C.W = W;
-      C.o = o;
-      C.ring = Ring{mb, M1, half, o % M1};
-      C.apply = J > 0;
-      C.Pp = P0 + ((J + 1) & 1) * W * kNB;
-      const int bo = (J + kStg - 1) % kStg, bc = J % kStg, bn = (J + 1) % kStg;
-      C.sv_old = sv + bo * kNB * kMaxR;
-      C.so_old = so + bo * kNB * kMaxR;
-      cplx* Pc = P0 + (J & 1) * W * kNB;
-
-      // The tile coordinates are loop invariant; re-declaring them as changed
-      // each stage keeps the position arithmetic of 16 entries per tile from
-      // being hoisted out of the stage loop (it would hold ~100 registers).
+  for (int i = 0; i < TR; ++i) ra[i] = min(rw0 + i + kNB, C.W - 1);
 #pragma unroll
-      for (int q = 0; q < MAXT; ++q) asm volatile("" : "+v"(rs_r0[q]), "+v"(rs_c0[q]));
-
-      // ---- phase A: waves 1-3 apply panel J-1 to panel J's columns; wave 0
-      // copies the entering batch J into panel J's last kNB rows
-      if (MF && wave > 0) {
-        __builtin_amdgcn_s_setprio(2);
-        const int T = (mb + 15) / 16;
-        for (int TI = wave - 1; TI < T; TI += NT / 64 - 1) {
-          cplx out[4];
-          mfma_tile(C, Tt, TI, 0, lane, out);
-          const int cw = lane & 15;
+  for (int j = 0; j < TC; ++j) ca[j] = min(cw0 + j + kNB, C.W - 1);
+#pragma unroll (TR * TC <= 4 ? kNB : 1)
+  for (int k = 0; k < kNB; ++k) {
+    T a[TR], b[TC];
 #pragma unroll
-          for (int v = 0; v < 4; ++v) {
-            const int rw = 16 * TI + (lane >> 4) + 4 * v;
-            if (tile_ok(C, rw, cw)) {
-              if (cw < kNB) Pc[pix(rw, cw)] = out[v];
-              else Tt[C.ring.pos(rw, cw)] = out[v];
-            }
-          }
-        }
-      } else if (wave > 0) {
-        // the look-ahead precedes the panel on the critical path as well
-        __builtin_amdgcn_s_setprio(2);
-        for (int x = x3; x < nla; x += NT - 64) {
-          int j = 0, i = x;
+    for (int i = 0; i < TR; ++i) a[i] = C.Pp[pix(ra[i], k)];
 #pragma unroll
-          for (int jj = 0; jj < kNB / 2 - 1; ++jj)
-            if (i >= t2 - j) {
-              i -= t2 - j;
-              ++j;
-            }
-          const int rw0 = 2 * (i + j), cw0 = 2 * j;
-          cplx acc[2][2];
-          tile_update<2, 2>(C, Tt, rw0, cw0, acc);
+    for (int j = 0; j < TC; ++j) b[j] = C.Pp[pix(ca[j], k)];
 #pragma unroll
-          for (int a = 0; a < 2; ++a)
+    for (int i = 0; i < TR; ++i)
 #pragma unroll
-            for (int b = 0; b < 2; ++b)
-              if (tile_ok(C, rw0 + a, cw0 + b)) Pc[pix(rw0 + a, cw0 + b)] = acc[a][b];
-        }
-      } else {
-        const int i = lane / kNB, cw = lane % kNB, rw = mb + i;
-        const bool ok = o + rw < n && cw >= i;
-        Pc[pix(rw, cw)] = csel(ok, staged(sv + bc * kNB * kMaxR, so + bc * kNB * kMaxR, i, cw - i), mkc(0.0, 0.0));
-      }
-      __builtin_amdgcn_s_setprio(0);
-      clk.mark(0);
-      stage_barrier<GR>();
-      clk.mark(1);
-
-      // ---- phase B
-      if (wave == 0) {
-        // the panel is the stage's critical path: its wave takes issue
-        // precedence over the other workgroup's trailing wave on its SIMD
-        __builtin_amdgcn_s_setprio(3);
-        // panel J in registers: lane holds panel rows lane + 64 q
-        const int rows = min(W, n - o);
-        cplx v[ROWS][kNB];
-#pragma unroll
-        for (int q = 0; q < ROWS; ++q) {
-          const int rw = lane + 64 * q;
-#pragma unroll
-          for (int k = 0; k < kNB; ++k) {
-            const bool ok = rw < rows && k < nbc && k <= rw && rw - k <= mb;
-            v[q][k] = csel(ok, Pc[pix(min(rw, W - 1), k)], mkc(0.0, 0.0));
-          }
-        }
-        // Column c, in the L D L^T form of the same elimination: with d the
-        // pivot and s_rc the column's updated entries, l_rc = s_rc / d and the
-        // later columns take -l_rc s_c2c (= -L_rc L_c2c).  The next pivot,
-        // d_{c+1} = s_{c+1,c+1} - l_{c+1,c} s_{c+1,c}, needs no square root, so
-        // the column-to-column dependency is a reciprocal and two products;
-        // L_cc = sqrt(d), L_rc = l_rc L_cc and y_c = y'_c / L_cc follow off it.
-        // lrc is 0 for rows at or above c, so every update is unconditional
-        // (a - 0 * x leaves a unchanged).
-        // Columns past the matrix end (the last stage only) are skipped; their
-        // pivot slots hold 1 so the square roots below stay finite.  The loop leaves
-        // l_rc in place (0 at and above the diagonal); the square roots of all
-        // kNB pivots are taken at once afterwards (lane c) and turn l into L
-        // and y' into y.  Broadcasts go through LDS (one wavefront: its LDS
-        // operations complete in issue order), except the next pivot and y'_c.
-        cplx piv = rlc(v[0][0], 0);
-        bool zero_pivot = false;
-#pragma unroll
-        for (int c = 0; c < kNB; ++c) {
-          if (c < nbc) {
-            // chol :66 -- |s| <= 0 is an exact zero (a NaN pivot passes, as
-            // there); choldc :66 -- s <= 0 for the real DC pivots
-            zero_pivot |= A.dc ? piv.re <= 0.0 : (piv.re == 0.0 && piv.im == 0.0);
-            const cplx inv = crecip(piv);
-            xd[c] = piv;  // every lane stores the same value
-            xi[c] = inv;
-            cplx lrc[ROWS];
-#pragma unroll
-            for (int q = 0; q < ROWS; ++q) lrc[q] = csel(lane + 64 * q > c, cmul(v[q][c], inv), mkc(0.0, 0.0));
-            const int cn = c + 1 < kNB ? c + 1 : kNB - 1;
-            cplx pivn = mkc(1.0, 0.0);
-            if (c + 1 < nbc) pivn = rlc(cmsub(v[0][cn], lrc[0], v[0][c]), cn);
-            const cplx yp = rlc(yv[0], c);  // y'_c, final
-            // s_{c2,c} of the panel's diagonal block from lanes c+1..kNB-1
-            if (lane > c && lane < kNB) xb[lane] = v[0][c];
-#pragma unroll
-            for (int c2 = c + 1; c2 < kNB; ++c2) {
-              if (c2 < nbc) {
-                const cplx s2 = xb[c2];
-#pragma unroll
-                for (int q = 0; q < ROWS; ++q) v[q][c2] = cmsub(v[q][c2], lrc[q], s2);
-              }
-            }
-#pragma unroll
-            for (int q = 0; q < ROWS; ++q) {
-              yv[q] = cmsub(yv[q], lrc[q], yp);
-              v[q][c] = lrc[q];
-            }
-            piv = pivn;
-          } else {
-            xd[c] = mkc(1.0, 0.0);
-            xi[c] = mkc(1.0, 0.0);
-          }
-        }
-        const int lk = min(lane, kNB - 1);
-        const cplx dd = xd[lk], invd = xi[lk];
-        const cplx lcc = csqrt_(dd);
-        if (lane < kNB) xl[lane] = lcc;
-#pragma unroll
-        for (int c = 0; c < kNB; ++c) {
-          const cplx lc = xl[c];
-#pragma unroll
-          for (int q = 0; q < ROWS; ++q) v[q][c] = cmul(v[q][c], lc);
-          v[0][c] = csel(lane == c, lc, v[0][c]);
-        }
-        yv[0] = csel(lane < kNB, cmul(cmul(yv[0], lcc), invd), yv[0]);
-        if (zero_pivot && lane == 0 && atomicCAS(A.err, 0, 28) == 0) A.err[1] = sys;
-        // L of panel J: back to the panel buffer (stage J+1's operand) and out
-        // to the band rows; y of its columns is final
-#pragma unroll
-        for (int q = 0; q < ROWS; ++q) {
-          const int rw = lane + 64 * q;
-          if (rw < W) {
-#pragma unroll
-            for (int k = 0; k < kNB; ++k) {
-              const bool ok = rw < rows && k < nbc && k <= rw && rw - k <= mb;
-              Pc[pix(rw, k)] = csel(ok, v[q][k], mkc(0.0, 0.0));
-              if (ok) Lg[(size_t)(o + rw) * M1 + (k - rw + mb)] = make_double2(v[q][k].re, v[q][k].im);
-            }
-          }
-        }
-        if (lane < nbc) yg[o + lane] = make_double2(yv[0].re, yv[0].im);
-        // y of rows [o + kNB, o + W) moves up kNB rows; entering rows get b
-        const int src = (lane + kNB) & 63;
-        cplx ny[ROWS];
-#pragma unroll
-        for (int q = 0; q < ROWS; ++q) {
-          cplx a, b = mkc(0.0, 0.0);
-          a.re = __shfl(yv[q].re, src, 64);
-          a.im = __shfl(yv[q].im, src, 64);
-          if (q + 1 < ROWS) {
-            b.re = __shfl(yv[q + 1].re, src, 64);
-            b.im = __shfl(yv[q + 1].im, src, 64);
-          }
-          ny[q] = csel(lane + kNB < 64, a, b);
-          const int rwn = lane + 64 * q;
-          if (rwn >= mb) {
-            const int r = o + kNB + rwn;
-            ny[q] = csel(rwn < W && r < n, rhs(r), mkc(0.0, 0.0));
-          }
-        }
-#pragma unroll
-        for (int q = 0; q < ROWS; ++q) yv[q] = ny[q];
-        __builtin_amdgcn_s_setprio(0);
-        clk.mark(2);
-      } else {
-        // batch J+1 = rows [o + kNB + mb, o + 2 kNB + mb): loads issued first,
-        // stored to LDS after the tiles
-        cplx nv = mkc(0.0, 0.0);
-        int nt = -1;
-        if (x3 < kNB * kMaxR) row_slot(A, Sg, o + kNB + mb + x3 / kMaxR, x3 % kMaxR, nv, nt);
-        // the rest of the window: panel J-1's update in place
-        if constexpr (MF) {
-          const int T = (mb + 15) / 16;
-          int cnt = 0;
-          for (int TJ = 1; TJ < T; ++TJ)
-            for (int TI = TJ; TI < T; ++TI, ++cnt) {
-              if (cnt % (NT / 64 - 1) != wave - 1) continue;
-              cplx out[4];
-              mfma_tile(C, Tt, TI, TJ, lane, out);
-              const int cw = 16 * TJ + (lane & 15);
-#pragma unroll
-              for (int v = 0; v < 4; ++v) {
-                const int rw = 16 * TI + (lane >> 4) + 4 * v;
-                if (tile_ok(C, rw, cw)) Tt[C.ring.pos(rw, cw)] = out[v];
-              }
-            }
-        }
-#pragma unroll
-        for (int q = 0; q < MAXT; ++q) {
-          if (!MF && rs_r0[q] >= 0) {
-            cplx acc[4][4];
-            tile_update<4, 4>(C, Tt, rs_r0[q], rs_c0[q], acc);
-#pragma unroll
-            for (int i = 0; i < 4; ++i)
-#pragma unroll
-              for (int j = 0; j < 4; ++j)
-                if (tile_ok(C, rs_r0[q] + i, rs_c0[q] + j)) Tt[C.ring.pos(rs_r0[q] + i, rs_c0[q] + j)] = acc[i][j];
-          }
-        }
-        if (x3 < kNB * kMaxR) {
-          sv[bn * kNB * kMaxR + x3] = nv;
-          so[bn * kNB * kMaxR + x3] = (int16_t)nt;
-        }
-        clk.mark(2);
-      }
-      stage_barrier<GR>();
-      clk.mark(3);
-    }
+      for (int j = 0; j < TC; ++j) acc[i][j] = msub(acc[i][j], a[i], b[j]);
   }
 }
 
-template <int ROWS, int MAXT, int NT, bool GR, bool MF>
-__global__ __launch_bounds__(NT, 512 / NT) void crt_factor_kernel(FactorArgs A) {
-  extern __shared__ double2 smem[];
+// ---------------------------------------------------------------------------
+// The factorisation of one system, on complex (T = cplx) or real (T = double,
+// the DC twin) entries.  Phase A: the update waves apply panel J-1 to panel
+// J's columns (2x2 tiles) while wave 0 copies the entering batch's entries in
+// those columns into the panel's last kNB rows; phase B: wave 0 factors panel
+// J in registers while the update waves apply panel J-1 to the rest of the
+// window (4x4 tiles).
+// * Every window entry is read from the ring.  The look-ahead moves panel
+//   J's column entries out of the ring into the panel buffer and zeroes
+//   their slots; in phase B the update waves scatter batch J's stored entries
+//   right of panel J into the ring in window J+1's coordinates.  A departing
+//   column's slots are the entering rows' slots (Ring), so the slots are free
+//   and zero when the scatter writes them, and an entering row's unstored
+//   band positions read 0.  Batches are staged in LDS for wave 0's copy and
+//   the scatter only (two buffers, the current and the next).
+// * The panel needs no column bounds: every panel position is written each
+//   stage (0 outside the band, the matrix and the lower triangle), the
+//   columns past the matrix end (last stage) are identity columns, and L goes
+//   out through range-checked buffer stores.  Broadcasts within the panel
+//   wave are lane reads into scalar registers, not LDS round trips.
+// ROWS: panel rows per lane of wave 0 (W = mb + kNB <= 64 ROWS); MAXT:
+// trailing tiles per thread of the update waves; NT threads per workgroup:
+// 256 (one panel and three update waves; two workgroups per CU, three for
+// the real entries) or, when one workgroup's LDS leaves no room for a second
+// (complex, mb above ~98), 512 (seven update waves); GR: the ring lives in a
+// global scratch of half x (mb + 1) entries per workgroup (A.ring) instead of
+// LDS, for bands above 124; the panels and staged rows stay in LDS.
+template <int ROWS, int MAXT, int NT, bool GR, class T>
+__device__ __forceinline__ void factor_system(const FactorArgs& A, int s, double2* smem, PhaseClock& clk) {
+  using E = store_t<T>;
+  constexpr int ES = sizeof(E);
+  constexpr int NB = kNB * kMaxR;             // records per staged batch
   const int n = A.n, mb = A.mb, R = A.R;
   const int W = mb + kNB, M1 = mb + 1, half = (mb + 1) / 2;
-  // half x M1 ring slots
-  cplx* Tt = GR ? reinterpret_cast<cplx*>(A.ring) + (size_t)blockIdx.x * half * M1 : reinterpret_cast<cplx*>(smem);
-  cplx* P0 = GR ? reinterpret_cast<cplx*>(smem) : Tt + half * M1;  // 2 panels of W x kNB
-  cplx* sv = P0 + 2 * W * kNB;                // kStg batches of kNB x kMaxR values
-  cplx* xb = sv + kStg * kNB * kMaxR;         // kNB: column broadcast of the panel
-  cplx* xd = xb + kNB;                        // kNB: the panel's pivots d_c
-  cplx* xi = xd + kNB;                        // kNB: 1/d_c
-  cplx* xl = xi + kNB;                        // kNB: L_cc = sqrt(d_c)
-  int16_t* so = reinterpret_cast<int16_t*>(xl + kNB);  // kStg x kNB x kMaxR offsets
+  T* Tt = GR ? reinterpret_cast<T*>(A.ring) + (size_t)blockIdx.x * A.ring_stride : reinterpret_cast<T*>(smem);
+  T* P0 = GR ? reinterpret_cast<T*>(smem) : Tt + crt_ring_lds_entries(mb, ES);  // 2 panels of W x kNB
+  T* sv = P0 + 2 * W * kNB;                   // 2 batches of kNB x kMaxR values
+  int16_t* so = reinterpret_cast<int16_t*>(sv + 2 * NB);  // their band offsets
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int x3 = tid - 64;                    // thread index within waves 1-3
+  const int x3 = tid - 64;                    // thread index within waves 1..
   const int nst = (n + kNB - 1) / kNB;
-  const int tw = (mb + 3) / 4;                          // 4x4 tiles per window side
   // look-ahead: 2x2 tiles over window rows [0, mb) x columns [0, kNB), column
   // by column of tiles; tile column j holds the tile rows i >= j
   const int t2 = (mb + 1) / 2;
   int nla = 0;
 #pragma unroll
   for (int j = 0; j < kNB / 2; ++j) nla += max(t2 - j, 0);
-  const int nrest = tw > 2 ? ((tw - 2) * (tw - 1)) / 2 : 0;  // 4x4 tiles, tile columns 2..
-
-  // trailing 4x4 tiles of this thread (fixed for the kernel)
   int rs_r0[MAXT], rs_c0[MAXT];
-#pragma unroll
-  for (int q = 0; q < MAXT; ++q) {
-    rs_r0[q] = -1;
-    rs_c0[q] = 0;
-    const int x = x3 + (NT - 64) * q;
-    if (wave > 0 && x < nrest) {
-      int ti, tj;
-      tri_tile(x, ti, tj);
-      rs_r0[q] = 4 * (ti + 2);
-      rs_c0[q] = 4 * (tj + 2);
-    }
-  }
+  trailing_tiles<MAXT, NT>(mb, rs_r0, rs_c0);
 
-  PhaseClock clk;
-  clk.start(tid == 0 ? A.prof : (tid == 64 ? A.prof : nullptr), tid == 0 ? 0 : 4);
+  const int sys = (A.sys_first + s) * A.sys_stride;
+  const int l = sys % A.eanz;
+  E* Lg = reinterpret_cast<E*>(A.L) + (size_t)s * n * M1;
+  const brsrc Lr = make_rsrc(Lg, (unsigned)(n * M1 * ES));
+  E* yg = reinterpret_cast<E*>(A.y) + (size_t)s * n;
+  const E* Sg = reinterpret_cast<const E*>(A.S) + (size_t)s * n * R;
+  const double* fk = A.fak + (size_t)s * n;
 
-  for (int s = blockIdx.x; s < A.nsys; s += gridDim.x) {
-    const int sys = (A.sys_first + s) * A.sys_stride;
-    const int l = sys % A.eanz;
-    double2* Lg = A.L + (size_t)s * n * M1;
-    double2* yg = A.y + (size_t)s * n;
-    const double2* Sg = A.S + (size_t)s * n * R;
-    const double* fk = A.fak + (size_t)s * n;
-
-    // ---- prologue: ring zeroed, rows [0, mb - kNB) in place, batches
-    // [mb - kNB, mb) and [mb, mb + kNB) staged, wave 0's right-hand side
-    for (int i = tid; i < half * M1; i += NT) Tt[i] = mkc(0.0, 0.0);
-    stage_barrier<GR>();
-    {
-      Ring ring0{mb, M1, half, 0};
-      for (int r = tid; r < min(mb - kNB, n); r += NT)
-        for (int q = 0; q < R; ++q) {
-          cplx v;
-          int t;
-          row_slot(A, Sg, r, q, v, t);
-          if (t >= 0) Tt[ring0.pos(r, r - mb + t)] = v;
-        }
-      for (int x = tid; x < 2 * kNB * kMaxR; x += NT) {
-        const int b = x / (kNB * kMaxR), i = (x / kMaxR) % kNB, q = x % kMaxR;
-        const int r = (b == 0 ? mb - kNB : mb) + i;
-        const int slot = (b == 0 ? kStg - 1 : 0) * kNB * kMaxR + i * kMaxR + q;
-        cplx v;
+  // ---- prologue: ring zeroed, window rows [0, mb) scattered into it, batch
+  // 0 = rows [mb, mb + kNB) staged, wave 0's right-hand side
+  for (int i = tid; i < half * M1; i += NT) Tt[i] = szero<T>();
+  stage_barrier<GR>();
+  {
+    Ring ring0{mb, M1, half, 0};
+    for (int r = tid; r < min(mb, n); r += NT)
+      for (int q = 0; q < R; ++q) {
+        T v;
         int t;
         row_slot(A, Sg, r, q, v, t);
-        sv[slot] = v;
-        so[slot] = (int16_t)t;
+        if (t >= 0) Tt[ring0.pos(r, r - mb + t)] = v;
       }
-    }
-    // right-hand side b(enr(l)) = -1, b(nsink) = +1 (kompab :162,:183), zero on
-    // Dirichlet rows (randb :49), scaled by f (scalab :65)
-    const int rel = A.enr[l];
-    const int rsink = A.lsink ? A.nsink : -1;
-    double be = 0.0, bs = 0.0;
-    if (wave == 0) {
-      if (!(A.isdir && A.isdir[rel])) be = -fk[rel];
-      if (rsink >= 0 && !(A.isdir && A.isdir[rsink])) bs = fk[rsink];
-    }
-    // prescribed boundary values: the right-hand side comes from crt_rhs_kernel
-    const double2* Bs = A.rhs ? A.rhs + (size_t)s * n : nullptr;
-    auto rhs = [&](int r) -> cplx {
-      if (Bs) return ldc(Bs + min(max(r, 0), n - 1));
-      return mkc(r == rsink ? bs : (r == rel ? be : 0.0), 0.0);
-    };
-    cplx yv[ROWS];
-#pragma unroll
-    for (int q = 0; q < ROWS; ++q) {
-      const int rw = lane + 64 * q;
-      yv[q] = csel(wave == 0 && rw < W && rw < n, rhs(rw), mkc(0.0, 0.0));
-    }
-    stage_barrier<GR>();
-    clk.mark(0);
-
-    for (int J = 0; J < nst; ++J) {
-      const int o = J * kNB;
-      const int nbc = min(kNB, n - o);
-      Stage C;
-      C.n = n;
-      C.mb = mb;
-      C.W = W;
-      C.o = o;
-      C.ring = Ring{mb, M1, half, o % M1};
-      C.apply = J > 0;
-      C.Pp = P0 + ((J + 1) & 1) * W * kNB;
-      const int bo = (J + kStg - 1) % kStg, bc = J % kStg, bn = (J + 1) % kStg;
-      C.sv_old = sv + bo * kNB * kMaxR;
-      C.so_old = so + bo * kNB * kMaxR;
-      cplx* Pc = P0 + (J & 1) * W * kNB;
-
-      // The tile coordinates are loop invariant; re-declaring them as changed
-      // each stage keeps the position arithmetic of 16 entries per tile from
-      // being hoisted out of the stage loop (it would hold ~100 registers).
-#pragma unroll
-      for (int q = 0; q < MAXT; ++q) asm volatile("" : "+v"(rs_r0[q]), "+v"(rs_c0[q]));
-
-      // ---- phase A: waves 1-3 apply panel J-1 to panel J's columns; wave 0
-      // copies the entering batch J into panel J's last kNB rows
-      if (MF && wave > 0) {
-        __builtin_amdgcn_s_setprio(2);
-        const int T = (mb + 15) / 16;
-        for (int TI = wave - 1; TI < T; TI += NT / 64 - 1) {
-          cplx out[4];
-          mfma_tile(C, Tt, TI, 0, lane, out);
-          const int cw = lane & 15;
-#pragma unroll
-          for (int v = 0; v < 4; ++v) {
-            const int rw = 16 * TI + (lane >> 4) + 4 * v;
-            if (tile_ok(C, rw, cw)) {
-              if (cw < kNB) Pc[pix(rw, cw)] = out[v];
-              else Tt[C.ring.pos(rw, cw)] = out[v];
-            }
-          }
-        }
-      } else if (wave > 0) {
-        // the look-ahead precedes the panel on the critical path as well
-        __builtin_amdgcn_s_setprio(2);
-        for (int x = x3; x < nla; x += NT - 64) {
-          int j = 0, i = x;
-#pragma unroll
-          for (int jj = 0; jj < kNB / 2 - 1; ++jj)
-            if (i >= t2 - j) {
-              i -= t2 - j;
-              ++j;
-            }
-          const int rw0 = 2 * (i + j), cw0 = 2 * j;
-          cplx acc[2][2];
-          tile_update<2, 2>(C, Tt, rw0, cw0, acc);
-#pragma unroll
-          for (int a = 0; a < 2; ++a)
-#pragma unroll
-            for (int b = 0; b < 2; ++b)
-              if (tile_ok(C, rw0 + a, cw0 + b)) Pc[pix(rw0 + a, cw0 + b)] = acc[a][b];
-        }
-      } else {
-        const int i = lane / kNB, cw = lane % kNB, rw = mb + i;
-        const bool ok = o + rw < n && cw >= i;
-        Pc[pix(rw, cw)] = csel(ok, staged(sv + bc * kNB * kMaxR, so + bc * kNB * kMaxR, i, cw - i), mkc(0.0, 0.0));
-      }
-      __builtin_amdgcn_s_setprio(0);
-      clk.mark(0);
-      stage_barrier<GR>();
-      clk.mark(1);
-
-      // ---- phase B
-      if (wave == 0) {
-        // the panel is the stage's critical path: its wave takes issue
-        // precedence over the other workgroup's trailing wave on its SIMD
-        __builtin_amdgcn_s_setprio(3);
-        // panel J in registers: lane holds panel rows lane + 64 q
-        const int rows = min(W, n - o);
-        cplx v[ROWS][kNB];
-#pragma unroll
-        for (int q = 0; q < ROWS; ++q) {
-          const int rw = lane + 64 * q;
-#pragma unroll
-          for (int k = 0; k < kNB; ++k) {
-            const bool ok = rw < rows && k < nbc && k <= rw && rw - k <= mb;
-            v[q][k] = csel(ok, Pc[pix(min(rw, W - 1), k)], mkc(0.0, 0.0));
-          }
-        }
-        // Column c, in the L D L^T form of the same elimination: with d the
-        // pivot and s_rc the column's updated entries, l_rc = s_rc / d and the
-        // later columns take -l_rc s_c2c (= -L_rc L_c2c).  The next pivot,
-        // d_{c+1} = s_{c+1,c+1} - l_{c+1,c} s_{c+1,c}, needs no square root, so
-        // the column-to-column dependency is a reciprocal and two products;
-        // L_cc = sqrt(d), L_rc = l_rc L_cc and y_c = y'_c / L_cc follow off it.
-        // lrc is 0 for rows at or above c, so every update is unconditional
-        // (a - 0 * x leaves a unchanged).
-        // Columns past the matrix end (the last stage only) are skipped; their
-        // pivot slots hold 1 so the square roots below stay finite.  The loop leaves
-        // l_rc in place (0 at and above the diagonal); the square roots of all
-        // kNB pivots are taken at once afterwards (lane c) and turn l into L
-        // and y' into y.  Broadcasts go through LDS (one wavefront: its LDS
-        // operations complete in issue order), except the next pivot and y'_c.
-        cplx piv = rlc(v[0][0], 0);
-        bool zero_pivot = false;
-#pragma unroll
-        for (int c = 0; c < kNB; ++c) {
-          if (c < nbc) {
-            // chol :66 -- |s| <= 0 is an exact zero (a NaN pivot passes, as
-            // there); choldc :66 -- s <= 0 for the real DC pivots
-            zero_pivot |= A.dc ? piv.re <= 0.0 : (piv.re == 0.0 && piv.im == 0.0);
-            const cplx inv = crecip(piv);
-            xd[c] = piv;  // every lane stores the same value
-            xi[c] = inv;
-            cplx lrc[ROWS];
-#pragma unroll
-            for (int q = 0; q < ROWS; ++q) lrc[q] = csel(lane + 64 * q > c, cmul(v[q][c], inv), mkc(0.0, 0.0));
-            const int cn = c + 1 < kNB ? c + 1 : kNB - 1;
-            cplx pivn = mkc(1.0, 0.0);
-            if (c + 1 < nbc) pivn = rlc(cmsub(v[0][cn], lrc[0], v[0][c]), cn);
-            const cplx yp = rlc(yv[0], c);  // y'_c, final
-            // s_{c2,c} of the panel's diagonal block from lanes c+1..kNB-1
-            if (lane > c && lane < kNB) xb[lane] = v[0][c];
-#pragma unroll
-            for (int c2 = c + 1; c2 < kNB; ++c2) {
-              if (c2 < nbc) {
-                const cplx s2 = xb[c2];
-#pragma unroll
-                for (int q = 0; q < ROWS; ++q) v[q][c2] = cmsub(v[q][c2], lrc[q], s2);
-              }
-            }
-#pragma unroll
-            for (int q = 0; q < ROWS; ++q) {
-              yv[q] = cmsub(yv[q], lrc[q], yp);
-              v[q][c] = lrc[q];
-            }
-            piv = pivn;
-          } else {
-            xd[c] = mkc(1.0, 0.0);
-            xi[c] = mkc(1.0, 0.0);
-          }
-        }
-        const int lk = min(lane, kNB - 1);
-        const cplx dd = xd[lk], invd = xi[lk];
-        const cplx lcc = csqrt_(dd);
-        if (lane < kNB) xl[lane] = lcc;
-#pragma unroll
-        for (int c = 0; c < kNB; ++c) {
-          const cplx lc = xl[c];
-#pragma unroll
-          for (int q = 0; q < ROWS; ++q) v[q][c] = cmul(v[q][c], lc);
-          v[0][c] = csel(lane == c, lc, v[0][c]);
-        }
-        yv[0] = csel(lane < kNB, cmul(cmul(yv[0], lcc), invd), yv[0]);
-        if (zero_pivot && lane == 0 && atomicCAS(A.err, 0, 28) == 0) A.err[1] = sys;
-        // L of panel J: back to the panel buffer (stage J+1's operand) and out
-        // to the band rows; y of its columns is final
-#pragma unroll
-        for (int q = 0; q < ROWS; ++q) {
-          const int rw = lane + 64 * q;
-          if (rw < W) {
-#pragma unroll
-            for (int k = 0; k < kNB; ++k) {
-              const bool ok = rw < rows && k < nbc && k <= rw && rw - k <= mb;
-              Pc[pix(rw, k)] = csel(ok, v[q][k], mkc(0.0, 0.0));
-              if (ok) Lg[(size_t)(o + rw) * M1 + (k - rw + mb)] = make_double2(v[q][k].re, v[q][k].im);
-            }
-          }
-        }
-        if (lane < nbc) yg[o + lane] = make_double2(yv[0].re, yv[0].im);
-        // y of rows [o + kNB, o + W) moves up kNB rows; entering rows get b
-        const int src = (lane + kNB) & 63;
-        cplx ny[ROWS];
-#pragma unroll
-        for (int q = 0; q < ROWS; ++q) {
-          cplx a, b = mkc(0.0, 0.0);
-          a.re = __shfl(yv[q].re, src, 64);
-          a.im = __shfl(yv[q].im, src, 64);
-          if (q + 1 < ROWS) {
-            b.re = __shfl(yv[q + 1].re, src, 64);
-            b.im = __shfl(yv[q + 1].im, src, 64);
-          }
-          ny[q] = csel(lane + kNB < 64, a, b);
-          const int rwn = lane + 64 * q;
-          if (rwn >= mb) {
-            const int r = o + kNB + rwn;
-            ny[q] = csel(rwn < W && r < n, rhs(r), mkc(0.0, 0.0));
-          }
-        }
-#pragma unroll
-        for (int q = 0; q < ROWS; ++q) yv[q] = ny[q];
-        __builtin_amdgcn_s_setprio(0);
-        clk.mark(2);
-      } else {
-        // batch J+1 = rows [o + kNB + mb, o + 2 kNB + mb): loads issued first,
-        // stored to LDS after the tiles
-        cplx nv = mkc(0.0, 0.0);
-        int nt = -1;
-        if (x3 < kNB * kMaxR) row_slot(A, Sg, o + kNB + mb + x3 / kMaxR, x3 % kMaxR, nv, nt);
-        // the rest of the window: panel J-1's update in place
-        if constexpr (MF) {
-          const int T = (mb + 15) / 16;
-          int cnt = 0;
-          for (int TJ = 1; TJ < T; ++TJ)
-            for (int TI = TJ; TI < T; ++TI, ++cnt) {
-              if (cnt % (NT / 64 - 1) != wave - 1) continue;
-              cplx out[4];
-              mfma_tile(C, Tt, TI, TJ, lane, out);
-              const int cw = 16 * TJ + (lane & 15);
-#pragma unroll
-              for (int v = 0; v < 4; ++v) {
-                const int rw = 16 * TI + (lane >> 4) + 4 * v;
-                if (tile_ok(C, rw, cw)) Tt[C.ring.pos(rw, cw)] = out[v];
-              }
-            }
-        }
-#pragma unroll
-        for (int q = 0; q < MAXT; ++q) {
-          if (!MF && rs_r0[q] >= 0) {
-            cplx acc[4][4];
-            tile_update<4, 4>(C, Tt, rs_r0[q], rs_c0[q], acc);
-#pragma unroll
-            for (int i = 0; i < 4; ++i)
-#pragma unroll
-              for (int j = 0; j < 4; ++j)
-                if (tile_ok(C, rs_r0[q] + i, rs_c0[q] + j)) Tt[C.ring.pos(rs_r0[q] + i, rs_c0[q] + j)] = acc[i][j];
-          }
-        }
-        if (x3 < kNB * kMaxR) {
-          sv[bn * kNB * kMaxR + x3] = nv;
-          so[bn * kNB * kMaxR + x3] = (int16_t)nt;
-        }
-        clk.mark(2);
-      }
-      stage_barrier<GR>();
-      clk.mark(3);
+    for (int x = tid; x < NB; x += NT) {
+      T v;
+      int t;
+      row_slot(A, Sg, mb + x / kMaxR, x % kMaxR, v, t);
+      sv[x] = v;
+      so[x] = (int16_t)t;
     }
   }
+  // right-hand side b(enr(l)) = -1, b(nsink) = +1 (kompab :162,:183), zero on
+  // Dirichlet rows (randb :49), scaled by f (scalab :65)
+  const int rel = A.enr[l];
+  const int rsink = A.lsink ? A.nsink : -1;
+  double be = 0.0, bs = 0.0;
+  if (wave == 0) {
+    if (!(A.isdir && A.isdir[rel])) be = -fk[rel];
+    if (rsink >= 0 && !(A.isdir && A.isdir[rsink])) bs = fk[rsink];
+  }
+  // prescribed boundary values: the right-hand side comes from crt_rhs_kernel
+  const double2* Bs = A.rhs ? A.rhs + (size_t)s * n : nullptr;
+  auto rhs = [&](int r) -> T {
+    if (Bs) return sfrom<T>(ldc(Bs + min(max(r, 0), n - 1)));
+    return sreal<T>(r == rsink ? bs : (r == rel ? be : 0.0));
+  };
+  T yv[ROWS];
+#pragma unroll
+  for (int q = 0; q < ROWS; ++q) {
+    const int rw = lane + 64 * q;
+    yv[q] = sel(wave == 0 && rw < W && rw < n, rhs(rw), szero<T>());
+  }
+  stage_barrier<GR>();
+  clk.mark(0);
+
+  for (int J = 0; J < nst; ++J) {
+    const int o = J * kNB;
+    const int nbc = min(kNB, n - o);
+    Stage<T> C;
+    C.n = n;
+    C.mb = mb;
+    C.W = W;
+    C.o = o;
+    C.ring = Ring{mb, M1, half, o % M1};
+    C.apply = J > 0;
+    C.Pp = P0 + ((J + 1) & 1) * W * kNB;
+    T* Pc = P0 + (J & 1) * W * kNB;
+    T* svc = sv + (J & 1) * NB;                 // batch J: rows [o + mb, o + mb + kNB)
+    int16_t* soc = so + (J & 1) * NB;
+    T* svn = sv + ((J + 1) & 1) * NB;           // batch J+1
+    int16_t* son = so + ((J + 1) & 1) * NB;
+    // the tile coordinates are loop invariant; re-declaring them as changed
+    // each stage keeps the position arithmetic of 16 entries per tile from
+    // being hoisted out of the stage loop (it would hold ~100 registers)
+#pragma unroll
+    for (int q = 0; q < MAXT; ++q) asm volatile("" : "+v"(rs_r0[q]), "+v"(rs_c0[q]));
+
+    // ---- phase A: waves 1.. apply panel J-1 to panel J's columns (out of the
+    // ring into the panel buffer, their slots zeroed); wave 0 copies batch J's
+    // entries in panel J's columns into the panel's last kNB rows
+    if (wave > 0) {
+      __builtin_amdgcn_s_setprio(2);
+      for (int x = x3; x < nla; x += NT - 64) {
+        int j = 0, i = x;
+#pragma unroll
+        for (int jj = 0; jj < kNB / 2 - 1; ++jj)
+          if (i >= t2 - j) {
+            i -= t2 - j;
+            ++j;
+          }
+        const int rw0 = 2 * (i + j), cw0 = 2 * j;
+        T acc[2][2];
+        ring_tile<2, 2>(C, Tt, rw0, cw0, acc);
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+          for (int b = 0; b < 2; ++b) {
+            const int rw = rw0 + a, cw = cw0 + b;
+            if (rw < mb) Pc[pix(rw, cw)] = acc[a][b];
+            if (tile_ok(C, rw, cw)) Tt[C.ring.pos(rw, cw)] = szero<T>();
+          }
+      }
+    } else {
+      const int i = lane / kNB, cw = lane % kNB, rw = mb + i;
+      const bool ok = o + rw < n && cw >= i;
+      Pc[pix(rw, cw)] = sel(ok, staged(svc, soc, i, cw - i), szero<T>());
+    }
+    __builtin_amdgcn_s_setprio(0);
+    clk.mark(0);
+    stage_barrier<GR>();
+    clk.mark(1);
+
+    // ---- phase B
+    if (wave == 0) {
+      __builtin_amdgcn_s_setprio(3);
+      const int rows = min(W, n - o);
+      T v[ROWS][kNB];
+#pragma unroll
+      for (int q = 0; q < ROWS; ++q) {
+        const int rw = lane + 64 * q;
+#pragma unroll
+        for (int k = 0; k < kNB; ++k) v[q][k] = sel(rw < W, Pc[pix(min(rw, W - 1), k)], szero<T>());
+      }
+      // identity columns past the matrix end
+#pragma unroll
+      for (int k = 0; k < kNB; ++k) v[0][k] = sel(lane == k && k >= nbc, sreal<T>(1.0), v[0][k]);
+      // Column c in the L D L^T form of the same elimination: with d the pivot
+      // and s_rc the column's updated entries, l_rc = s_rc / d and the later
+      // columns take -l_rc s_c2c (= -L_rc L_c2c).  The next pivot d_{c+1} =
+      // s_{c+1,c+1} - l_{c+1,c} s_{c+1,c} needs no square root, so the
+      // column-to-column dependency is a reciprocal and two products; L_cc =
+      // sqrt(d), L_rc = l_rc L_cc and y_c = y'_c / L_cc follow off it.  lrc is
+      // 0 for rows at or above c, so every update is unconditional (a - 0 * x
+      // leaves a unchanged).  Broadcasts are lane reads into scalar registers.
+      T piv = srl(v[0][0], 0);
+      T dd = sreal<T>(1.0), invd = sreal<T>(1.0);  // lane c: d_c and 1/d_c
+      bool zero_pivot = false;
+#pragma unroll
+      for (int c = 0; c < kNB; ++c) {
+        // chol :66 -- |s| <= 0 is an exact zero (a NaN pivot passes, as
+        // there); choldc :66 -- s <= 0 for the real DC pivots
+        if constexpr (std::is_same<T, cplx>::value)
+          zero_pivot |= A.dc ? piv.re <= 0.0 : (piv.re == 0.0 && piv.im == 0.0);
+        else
+          zero_pivot |= piv <= 0.0;
+        const T inv = srecip(piv);
+        dd = sel(lane == c, piv, dd);
+        invd = sel(lane == c, inv, invd);
+        T lrc[ROWS];
+#pragma unroll
+        for (int q = 0; q < ROWS; ++q) lrc[q] = sel(lane + 64 * q > c, smul(v[q][c], inv), szero<T>());
+        T pivn = piv;
+        if (c + 1 < kNB) pivn = srl(msub(v[0][c + 1], lrc[0], v[0][c]), c + 1);
+        const T yp = srl(yv[0], c);  // y'_c, final
+#pragma unroll
+        for (int c2 = c + 1; c2 < kNB; ++c2) {
+          const T s2 = srl(v[0][c], c2);  // s_{c2,c} of the panel's diagonal block
+#pragma unroll
+          for (int q = 0; q < ROWS; ++q) v[q][c2] = msub(v[q][c2], lrc[q], s2);
+        }
+#pragma unroll
+        for (int q = 0; q < ROWS; ++q) {
+          yv[q] = msub(yv[q], lrc[q], yp);
+          v[q][c] = lrc[q];
+        }
+        piv = pivn;
+      }
+      const T lcc = ssqrt(dd);
+#pragma unroll
+      for (int c = 0; c < kNB; ++c) {
+        const T lc = srl(lcc, c);
+#pragma unroll
+        for (int q = 0; q < ROWS; ++q) v[q][c] = smul(v[q][c], lc);
+        v[0][c] = sel(lane == c, lc, v[0][c]);
+      }
+      yv[0] = sel(lane < kNB, smul(smul(yv[0], lcc), invd), yv[0]);
+      if (zero_pivot && lane == 0 && atomicCAS(A.err, 0, 28) == 0) A.err[1] = sys;
+      // L of panel J: to the panel buffer (stage J+1's operand; 0 where it is
+      // not an entry) and to the band rows
+#pragma unroll
+      for (int q = 0; q < ROWS; ++q) {
+        const int rw = lane + 64 * q;
+        if (rw < W) {
+#pragma unroll
+          for (int k = 0; k < kNB; ++k) Pc[pix(rw, k)] = v[q][k];
+        }
+        const int klo = max(rw - mb, 0), khi = min(rw, nbc - 1);
+#pragma unroll
+        for (int k = 0; k < kNB; ++k) {
+          const bool ok = rw < rows && k >= klo && k <= khi;
+          bst(Lr, ok ? (unsigned)(((o + rw) * M1 + (k - rw + mb)) * ES) : kBufOut, v[q][k]);
+        }
+      }
+      if (lane < nbc) st(yg + o + lane, yv[0]);
+      // y of rows [o + kNB, o + W) moves up kNB rows; entering rows get b
+      const int src = (lane + kNB) & 63;
+      T ny[ROWS];
+#pragma unroll
+      for (int q = 0; q < ROWS; ++q) {
+        const T a = sshfl(yv[q], src);
+        T b = szero<T>();
+        if (q + 1 < ROWS) b = sshfl(yv[q + 1], src);
+        ny[q] = sel(lane + kNB < 64, a, b);
+        const int rwn = lane + 64 * q;
+        if (rwn >= mb) {
+          const int r = o + kNB + rwn;
+          ny[q] = sel(rwn < W && r < n, rhs(r), szero<T>());
+        }
+      }
+#pragma unroll
+      for (int q = 0; q < ROWS; ++q) yv[q] = ny[q];
+      __builtin_amdgcn_s_setprio(0);
+      clk.mark(2);
+    } else {
+      // batch J+1 = rows [o + kNB + mb, o + 2 kNB + mb): loads issued first,
+      // staged after the tiles
+      T nv = szero<T>();
+      int nt = -1;
+      if (x3 < NB) row_slot(A, Sg, o + kNB + mb + x3 / kMaxR, x3 % kMaxR, nv, nt);
+      // the rest of the window: panel J-1's update in place
+#pragma unroll
+      for (int q = 0; q < MAXT; ++q) {
+        if (rs_r0[q] >= 0) {
+          T acc[4][4];
+          ring_tile<4, 4>(C, Tt, rs_r0[q], rs_c0[q], acc);
+#pragma unroll
+          for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+              if (tile_ok(C, rs_r0[q] + i, rs_c0[q] + j)) Tt[C.ring.pos(rs_r0[q] + i, rs_c0[q] + j)] = acc[i][j];
+        }
+      }
+      if (x3 < NB) {
+        // batch J's entries right of panel J into the ring (window J+1: row
+        // mb - kNB + i, column i + t - kNB)
+        const int i = x3 / kMaxR, t = soc[x3], cw1 = i + t - kNB;
+        if (t >= 0 && cw1 >= 0) {
+          const Ring ring1{mb, M1, half, (o + kNB) % M1};
+          Tt[ring1.pos(mb - kNB + i, cw1)] = svc[x3];
+        }
+        svn[x3] = nv;
+        son[x3] = (int16_t)nt;
+      }
+      clk.mark(2);
+    }
+    stage_barrier<GR>();
+    clk.mark(3);
+  }
+}
+
+// Persistent over the batch's systems (workgroup b takes systems b, b + grid,
+// ...).  T = double is the DC twin; its halved registers and LDS leave room
+// for three workgroups per CU at the narrower bands.
+template <int ROWS, int MAXT, int NT, bool GR, class T>
+__global__ __launch_bounds__(NT, (std::is_same<T, double>::value ? 768 : 512) / NT) void crt_factor_kernel(
+    FactorArgs A) {
+  extern __shared__ double2 smem[];
+  PhaseClock clk;
+  clk.start(threadIdx.x == 0 ? A.prof : (threadIdx.x == 64 ? A.prof : nullptr), threadIdx.x == 0 ? 0 : 4);
+  for (int s = blockIdx.x; s < A.nsys; s += gridDim.x) factor_system<ROWS, MAXT, NT, GR, T>(A, s, smem, clk);
   clk.flush();
 }
 
@@ -1106,7 +660,7 @@ __global__ __launch_bounds__(NT, 512 / NT) void crt_factor_kernel(FactorArgs A) 
 // work item w is system items[w].y of model items[w].x (its arguments
 // args[items[w].x]); persistent workgroups take the items in list order from
 // an atomic counter (zeroed before the launch) until the list is exhausted.
-template <int ROWS, int MAXT, int NT, bool GR, bool MF>
+template <int ROWS, int MAXT, int NT, bool GR>
 __global__ __launch_bounds__(NT, 512 / NT) void crt_factor_many_kernel(const FactorArgs* __restrict__ args,
                                                                       const int2* __restrict__ items, int nitems,
                                                                       int* counter) {
@@ -1122,38 +676,40 @@ __global__ __launch_bounds__(NT, 512 / NT) void crt_factor_many_kernel(const Fac
     if (w >= nitems) break;
     const int2 it = items[w];
     const FactorArgs A = args[it.x];
-    factor_system<ROWS, MAXT, NT, GR, MF>(A, it.y, smem, clk);
+    factor_system<ROWS, MAXT, NT, GR, cplx>(A, it.y, smem, clk);
   }
 }
 
-
-template __global__ void crt_factor_kernel<1, 1, 256, false, false>(FactorArgs);
-template __global__ void crt_factor_kernel<2, 1, 256, false, false>(FactorArgs);
-template __global__ void crt_factor_kernel<2, 2, 256, false, false>(FactorArgs);
-template __global__ void crt_factor_kernel<2, 3, 256, false, false>(FactorArgs);
-template __global__ void crt_factor_kernel<3, 3, 256, false, false>(FactorArgs);
-template __global__ void crt_factor_kernel<2, 1, 512, false, false>(FactorArgs);
-template __global__ void crt_factor_kernel<2, 2, 512, false, false>(FactorArgs);
-template __global__ void crt_factor_kernel<3, 2, 512, false, false>(FactorArgs);
-template __global__ void crt_factor_kernel<3, 5, 512, true, false>(FactorArgs);
-template __global__ void crt_factor_kernel<4, 5, 512, true, false>(FactorArgs);
-template __global__ void crt_factor_kernel<1, 1, 256, false, true>(FactorArgs);
-template __global__ void crt_factor_kernel<2, 1, 256, false, true>(FactorArgs);
-template __global__ void crt_factor_kernel<3, 1, 256, false, true>(FactorArgs);
-template __global__ void crt_factor_kernel<2, 1, 512, false, true>(FactorArgs);
-template __global__ void crt_factor_kernel<3, 1, 512, false, true>(FactorArgs);
-template __global__ void crt_factor_kernel<3, 1, 512, true, true>(FactorArgs);
-template __global__ void crt_factor_kernel<4, 1, 512, true, true>(FactorArgs);
-template __global__ void crt_factor_many_kernel<1, 1, 256, false, false>(const FactorArgs*, const int2*, int, int*);
-template __global__ void crt_factor_many_kernel<2, 1, 256, false, false>(const FactorArgs*, const int2*, int, int*);
-template __global__ void crt_factor_many_kernel<2, 2, 256, false, false>(const FactorArgs*, const int2*, int, int*);
-template __global__ void crt_factor_many_kernel<2, 3, 256, false, false>(const FactorArgs*, const int2*, int, int*);
-template __global__ void crt_factor_many_kernel<3, 3, 256, false, false>(const FactorArgs*, const int2*, int, int*);
-template __global__ void crt_factor_many_kernel<2, 1, 512, false, false>(const FactorArgs*, const int2*, int, int*);
-template __global__ void crt_factor_many_kernel<2, 2, 512, false, false>(const FactorArgs*, const int2*, int, int*);
-template __global__ void crt_factor_many_kernel<3, 2, 512, false, false>(const FactorArgs*, const int2*, int, int*);
-template __global__ void crt_factor_many_kernel<3, 5, 512, true, false>(const FactorArgs*, const int2*, int, int*);
-template __global__ void crt_factor_many_kernel<4, 5, 512, true, false>(const FactorArgs*, const int2*, int, int*);
+template __global__ void crt_factor_kernel<1, 1, 256, false, cplx>(FactorArgs);
+template __global__ void crt_factor_kernel<2, 1, 256, false, cplx>(FactorArgs);
+template __global__ void crt_factor_kernel<2, 2, 256, false, cplx>(FactorArgs);
+template __global__ void crt_factor_kernel<2, 3, 256, false, cplx>(FactorArgs);
+template __global__ void crt_factor_kernel<3, 3, 256, false, cplx>(FactorArgs);
+template __global__ void crt_factor_kernel<2, 1, 512, false, cplx>(FactorArgs);
+template __global__ void crt_factor_kernel<2, 2, 512, false, cplx>(FactorArgs);
+template __global__ void crt_factor_kernel<3, 2, 512, false, cplx>(FactorArgs);
+template __global__ void crt_factor_kernel<3, 5, 512, true, cplx>(FactorArgs);
+template __global__ void crt_factor_kernel<4, 5, 512, true, cplx>(FactorArgs);
+template __global__ void crt_factor_kernel<1, 1, 256, false, double>(FactorArgs);
+template __global__ void crt_factor_kernel<2, 1, 256, false, double>(FactorArgs);
+template __global__ void crt_factor_kernel<2, 2, 256, false, double>(FactorArgs);
+template __global__ void crt_factor_kernel<2, 3, 256, false, double>(FactorArgs);
+template __global__ void crt_factor_kernel<3, 3, 256, false, double>(FactorArgs);
+template __global__ void crt_factor_kernel<2, 1, 512, false, double>(FactorArgs);
+template __global__ void crt_factor_kernel<2, 2, 512, false, double>(FactorArgs);
+template __global__ void crt_factor_kernel<3, 2, 512, false, double>(FactorArgs);
+template __global__ void crt_factor_kernel<3, 5, 512, true, double>(FactorArgs);
+template __global__ void crt_factor_kernel<4, 5, 512, true, double>(FactorArgs);
+template __global__ void crt_factor_many_kernel<1, 1, 256, false>(const FactorArgs*, const int2*, int, int*);
+template __global__ void crt_factor_many_kernel<2, 1, 256, false>(const FactorArgs*, const int2*, int, int*);
+template __global__ void crt_factor_many_kernel<2, 2, 256, false>(const FactorArgs*, const int2*, int, int*);
+template __global__ void crt_factor_many_kernel<2, 3, 256, false>(const FactorArgs*, const int2*, int, int*);
+template __global__ void crt_factor_many_kernel<3, 3, 256, false>(const FactorArgs*, const int2*, int, int*);
+template __global__ void crt_factor_many_kernel<2, 1, 512, false>(const FactorArgs*, const int2*, int, int*);
+template __global__ void crt_factor_many_kernel<2, 2, 512, false>(const FactorArgs*, const int2*, int, int*);
+template __global__ void crt_factor_many_kernel<3, 2, 512, false>(const FactorArgs*, const int2*, int, int*);
+template __global__ void crt_factor_many_kernel<3, 5, 512, true>(const FactorArgs*, const int2*, int, int*);
+template __global__ void crt_factor_many_kernel<4, 5, 512, true>(const FactorArgs*, const int2*, int, int*);
 
 // ---------------------------------------------------------------------------
 // Back substitution, one wavefront per right-hand side.  Lane q (q < 64) keeps
@@ -1174,59 +730,64 @@ template __global__ void crt_factor_many_kernel<4, 5, 512, true, false>(const Fa
 // DEPTH rows ahead with buffer loads whose offsets for out-of-band, above-n and
 // below-0 entries lie past the end of L, so they read zero: no lane-divergent
 // branch and no per-value mask in the loop.
-template <int DEPTH, int G, int NS>
+// kpot = x f (complex; the DC twin's imaginary part is 0)
+__device__ __forceinline__ double2 kpot_of(double f, cplx x) { return make_double2(f * x.re, f * x.im); }
+__device__ __forceinline__ double2 kpot_of(double f, double x) { return make_double2(f * x, 0.0); }
+
+template <int DEPTH, int G, int NS, class T>
 __device__ __forceinline__ void backsolve_rhs(const BacksolveArgs& B, int r) {
   static_assert(DEPTH % G == 0 && 64 % DEPTH == 0, "row groups tile the block");
   static_assert(NS == 2 || NS == 4, "slot classes");
+  using E = store_t<T>;
+  constexpr int ES = sizeof(E);               // entry bytes: 16 complex, 8 real
   constexpr int WIN = 64 * NS;                // rows of the slot window
-  constexpr int RMASK = (WIN - 1) * 16;       // (x mod WIN) * 16 on 16-byte units
+  constexpr int RMASK = (WIN - 1) * ES;       // (x mod WIN) * ES on entry units
   const int lane = threadIdx.x;
   const int f = r / B.rhs_per_fac;
   const int sys = (B.fac_first + f) * B.fstride + r % B.rhs_per_fac;
   const int k = sys / B.eanz, l = sys % B.eanz;
   const int n = B.n, mb = B.mb, m1 = mb + 1;
-  // n m1 16 < 2^31 (checked by the host): offsets of rows in [0, n + 64) fit,
+  // n m1 ES < 2^31 (checked by the host): offsets of rows in [0, n + 64) fit,
   // offsets of rows below 0 wrap past the end
-  const brsrc Lr = make_rsrc(B.L + (size_t)f * n * m1, (unsigned)(n * m1 * 16));
-  const double2* yg = B.y + (size_t)r * n;
+  const brsrc Lr = make_rsrc(reinterpret_cast<const E*>(B.L) + (size_t)f * n * m1, (unsigned)(n * m1 * ES));
+  const E* yg = reinterpret_cast<const E*>(B.y) + (size_t)r * n;
   const double* fak = B.fak + (size_t)f * n;
   double2* out = B.kpot + ((size_t)k * B.eanz + l) * n;
 
   // Entries of row t of L for every slot: slot s's row is j = t - rf, rf =
   // (t mod 64 - lane + 64 s) mod WIN, at band offset mb - rf, in range for 1
-  // <= rf <= min(mb, t).  rf in 16-byte units.
-  const int m16 = m1 * 16;
-  auto fetch = [&](int t, cplx* dst) __attribute__((always_inline)) {
-    const int base = t * m16 + mb * 16;
-    const unsigned lim = (unsigned)max(min(mb, t), 0) * 16u;
-    const int ra = (((t & 63) - lane) * 16) & RMASK;
+  // <= rf <= min(mb, t).  rf in entry units (ES bytes).
+  const int mES = m1 * ES;
+  auto fetch = [&](int t, T* dst) __attribute__((always_inline)) {
+    const int base = t * mES + mb * ES;
+    const unsigned lim = (unsigned)max(min(mb, t), 0) * (unsigned)ES;
+    const int ra = (((t & 63) - lane) * ES) & RMASK;
 #pragma unroll
     for (int q = 0; q < NS; ++q) {
-      const int rq = (ra + q * 0x400) & RMASK;
-      dst[q] = bldc(Lr, (unsigned)(rq - 16) < lim ? (unsigned)(base - rq) : kBufOut);
+      const int rq = (ra + q * 64 * ES) & RMASK;
+      bld(Lr, (unsigned)(rq - ES) < lim ? (unsigned)(base - rq) : kBufOut, dst[q]);
     }
   };
   // f_i, L_ii and y_{i-WIN} of block b's row i = 64 b + lane (raw loads;
   // masked when the block becomes current)
-  auto load_blk = [&](int b, double& fb, cplx& db, cplx& yb) __attribute__((always_inline)) {
+  auto load_blk = [&](int b, double& fb, T& db, T& yb) __attribute__((always_inline)) {
     const int i = 64 * b + lane;
     fb = fak[min(max(i, 0), n - 1)];
-    db = bldc(Lr, (unsigned)(i * m16 + mb * 16));
-    yb = ldc(yg + min(max(i - WIN, 0), n - 1));
+    bld(Lr, (unsigned)(i * mES + mb * ES), db);
+    yb = ld(yg + min(max(i - WIN, 0), n - 1));
   };
-  cplx ys[NS];  // slot s: the class of block b - s
-  cplx lr[DEPTH][NS];
+  T ys[NS];  // slot s: the class of block b - s
+  T lr[DEPTH][NS];
   // Rows are taken in 64-row blocks from the top; the top block is padded
   // above n with rows that change nothing (x = 0: their -1/L_ii is zero).
   const int btop = (n - 1) >> 6;
   double fcur, fnxt;
-  cplx dcur, dnxt, ycur, ynxt, nl, xo = mkc(0.0, 0.0);
+  T dcur, dnxt, ycur, ynxt, nl, xo = szero<T>();
   auto enter_blk = [&](int b) __attribute__((always_inline)) {
     const int i = 64 * b + lane;
     fcur = i < n ? fcur : 0.0;
-    ycur = csel(i - WIN >= 0 && i - WIN < n, ycur, mkc(0.0, 0.0));
-    const cplx rc = crecip(dcur);
-    nl = csel(i < n, mkc(-rc.re, -rc.im), mkc(0.0, 0.0));
+    ycur = sel(i - WIN >= 0 && i - WIN < n, ycur, szero<T>());
+    nl = sel(i < n, sneg(srecip(dcur)), szero<T>());
   };
   load_blk(btop, fcur, dcur, ycur);
   load_blk(btop - 1, fnxt, dnxt, ynxt);
@@ -1236,7 +797,7 @@ __device__ __forceinline__ void backsolve_rhs(const BacksolveArgs& B, int r) {
     // slot q's class at the top block: lane + 64 ((btop - q) mod NS)
     const int cls = lane + 64 * (((btop - q) % NS + NS) % NS);
     const int j = itop - (((itop - cls) % WIN) + WIN) % WIN;
-    ys[q] = csel(j >= 0 && j < n, ldc(yg + min(max(j, 0), n - 1)), mkc(0.0, 0.0));
+    ys[q] = sel(j >= 0 && j < n, ld(yg + min(max(j, 0), n - 1)), szero<T>());
   }
   // in row order, as the loop refills them: the loop's wait for row i's
   // entries then counts the same loads behind them on entry as on the back edge
@@ -1260,13 +821,13 @@ __device__ __forceinline__ void backsolve_rhs(const BacksolveArgs& B, int r) {
 #pragma unroll
       for (int d = 0; d < DEPTH; ++d) {
         const int i = 64 * b + 63 - c - d;
-        const cplx xb = rlc(cmul(ys[0], nl), i & 63);
+        const T xb = srl(smul(ys[0], nl), i & 63);
 #pragma unroll
         for (int q = 0; q < NS; ++q) {
-          ys[q] = cmadd(ys[q], lr[d][q], xb);
+          ys[q] = madd(ys[q], lr[d][q], xb);
           // the updates are formed before lr[d] is refilled (otherwise they
           // may sink below the refill, which then needs a copy and a wait)
-          asm volatile("" : "+v"(ys[q].re), "+v"(ys[q].im));
+          spin(ys[q]);
         }
         fetch(i - DEPTH, lr[d]);
         // the fetch stays here, DEPTH rows ahead of its use (the scheduler
@@ -1275,25 +836,25 @@ __device__ __forceinline__ void backsolve_rhs(const BacksolveArgs& B, int r) {
         if ((d + 1) % G == 0) {
           // rows i+G-1 .. i done: their lanes keep x and refill slot 0
           const bool g = (unsigned)(lane - (i & 63)) < (unsigned)G;
-          xo = csel(g, cmul(ys[0], nl), xo);
-          ys[0] = csel(g, ycur, ys[0]);
+          xo = sel(g, smul(ys[0], nl), xo);
+          ys[0] = sel(g, ycur, ys[0]);
         }
       }
     }
     const int io = 64 * b + lane;
-    if (io < n) out[io] = make_double2(fcur * xo.re, fcur * xo.im);
+    if (io < n) out[io] = kpot_of(fcur, xo);
     // the next block's class is in slot 1; slot 0 (refilled with the class
     // WIN rows down) goes last
-    const cplx t = ys[0];
+    const T t = ys[0];
 #pragma unroll
     for (int q = 0; q + 1 < NS; ++q) ys[q] = ys[q + 1];
     ys[NS - 1] = t;
   }
 }
 
-template <int DEPTH, int G, int NS>
+template <int DEPTH, int G, int NS, class T>
 __global__ __launch_bounds__(64) void crt_backsolve_kernel(BacksolveArgs B) {
-  if ((int)blockIdx.x < B.nrhs) backsolve_rhs<DEPTH, G, NS>(B, blockIdx.x);
+  if ((int)blockIdx.x < B.nrhs) backsolve_rhs<DEPTH, G, NS, T>(B, blockIdx.x);
 }
 
 // The same over the right-hand sides of several models: block w solves rhs
@@ -1304,7 +865,7 @@ __global__ __launch_bounds__(64) void crt_backsolve_many_kernel(const BacksolveA
   if ((int)blockIdx.x >= nitems) return;
   const int2 it = items[blockIdx.x];
   const BacksolveArgs B = args[it.x];
-  backsolve_rhs<DEPTH, G, NS>(B, it.y);
+  backsolve_rhs<DEPTH, G, NS, cplx>(B, it.y);
 }
 
 template __global__ void crt_backsolve_kernel<8, 8, 2>(BacksolveArgs);
@@ -1318,6 +879,11 @@ template __global__ void crt_backsolve_kernel<16, 8, 2>(BacksolveArgs);
 template __global__ void crt_backsolve_kernel<16, 8, 4>(BacksolveArgs);
 template __global__ void crt_backsolve_many_kernel<16, 8, 2>(const BacksolveArgs*, const int2*, int);
 template __global__ void crt_backsolve_many_kernel<16, 8, 4>(const BacksolveArgs*, const int2*, int);
+template __global__ void crt_backsolve_kernel<8, 8, 2, double>(BacksolveArgs);
+template __global__ void crt_backsolve_kernel<8, 4, 2, double>(BacksolveArgs);
+template __global__ void crt_backsolve_kernel<8, 8, 4, double>(BacksolveArgs);
+template __global__ void crt_backsolve_kernel<16, 8, 2, double>(BacksolveArgs);
+template __global__ void crt_backsolve_kernel<16, 8, 4, double>(BacksolveArgs);
 
 // ---------------------------------------------------------------------------
 // Column-band copy of shared factors for the forward sweep: row j of Lc holds
@@ -1325,8 +891,11 @@ template __global__ void crt_backsolve_many_kernel<16, 8, 4>(const BacksolveArgs
 // (zero past row n-1).  One thread per entry of Lc, consecutive threads along
 // a row of Lc (coalesced stores); the reads walk an anti-diagonal of L whose
 // lines neighbouring rows of Lc share in cache.
-__global__ void crt_lcol_kernel(const double2* __restrict__ L, double2* __restrict__ Lc, int n, int mb,
-                                int nfac) {
+template <class T>
+__global__ void crt_lcol_kernel(const void* __restrict__ Lv, void* __restrict__ Lcv, int n, int mb, int nfac) {
+  using E = store_t<T>;
+  const E* L = reinterpret_cast<const E*>(Lv);
+  E* Lc = reinterpret_cast<E*>(Lcv);
   const int m1 = mb + 1;
   const size_t per = (size_t)n * m1;
   const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1334,10 +903,12 @@ __global__ void crt_lcol_kernel(const double2* __restrict__ L, double2* __restri
   const size_t f = idx / per;
   const int rem = (int)(idx - f * per);
   const int j = rem / m1, s = rem - j * m1;
-  double2 v = make_double2(0.0, 0.0);
-  if (j + s < n) v = L[f * per + (size_t)(j + s) * m1 + (mb - s)];
-  Lc[idx] = v;
+  T v = szero<T>();
+  if (j + s < n) v = ld(L + f * per + (size_t)(j + s) * m1 + (mb - s));
+  st(Lc + idx, v);
 }
+template __global__ void crt_lcol_kernel<cplx>(const void*, void*, int, int, int);
+template __global__ void crt_lcol_kernel<double>(const void*, void*, int, int, int);
 
 // ---------------------------------------------------------------------------
 // Forward sweep L y = b, one wavefront per right-hand side, for sources that
@@ -1358,20 +929,22 @@ __global__ void crt_lcol_kernel(const double2* __restrict__ L, double2* __restri
 // + 1 - mb keeps the refill ahead of it.  Columns of Lc are fetched DEPTH
 // steps ahead with buffer loads whose offsets for entries past the band or
 // past row n-1 lie beyond the end of Lc, so they read zero.
-template <int DEPTH, int G, int NS>
+template <int DEPTH, int G, int NS, class T>
 __global__ __launch_bounds__(64) void crt_fwdsweep_kernel(FwdArgs F) {
   static_assert(DEPTH % G == 0 && 64 % DEPTH == 0, "row groups tile the block");
   static_assert(NS == 2 || NS == 4, "slot classes");
+  using E = store_t<T>;
+  constexpr int ES = sizeof(E);
   constexpr int WIN = 64 * NS;
-  constexpr int RMASK = (WIN - 1) * 16;
+  constexpr int RMASK = (WIN - 1) * ES;
   const int lane = threadIdx.x;
   const int r = blockIdx.x;
   if (r >= F.nrhs) return;
   const int f = r / F.eanz, l = r % F.eanz;
   const int n = F.n, mb = F.mb, m1 = mb + 1;
-  // n m1 16 < 2^31 (checked by the host): offsets of rows in [0, n + WIN) fit
-  const brsrc Lr = make_rsrc(F.Lc + (size_t)f * n * m1, (unsigned)(n * m1 * 16));
-  double2* yg = F.y + (size_t)r * n;
+  // n m1 ES < 2^31 (checked by the host): offsets of rows in [0, n + WIN) fit
+  const brsrc Lr = make_rsrc(reinterpret_cast<const E*>(F.Lc) + (size_t)f * n * m1, (unsigned)(n * m1 * ES));
+  E* yg = reinterpret_cast<E*>(F.y) + (size_t)r * n;
   // b = -f at the source electrode, +f at the sink (the sink wins a tie).
   // Source 0 is assembled with the matrix (kompab, randb: zero on Dirichlet
   // rows; with prescribed values the whole vector comes from crt_rhs_kernel);
@@ -1388,54 +961,50 @@ __global__ __launch_bounds__(64) void crt_fwdsweep_kernel(FwdArgs F) {
   // vector) plus the electrode term, no branch.  Split in two so that the
   // refill values are combined only at the refill (a use next to the load
   // would wait for every fetch in flight).
-  auto bload = [&](int i) __attribute__((always_inline)) -> cplx {
-    return bldc(Br, (unsigned)i < (unsigned)n ? (unsigned)i * 16u : kBufOut);
+  auto bload = [&](int i) __attribute__((always_inline)) -> T {
+    return sfrom<T>(bldc(Br, (unsigned)i < (unsigned)n ? (unsigned)i * 16u : kBufOut));
   };
   auto bterm = [&](int i) __attribute__((always_inline)) -> double {
     return i == rs ? brs : (i == re ? bre : 0.0);
   };
   const int j0 = B0 ? 0 : (rs >= 0 ? min(re, rs) : re);
   const int b0 = j0 >> 6, btop = (n - 1) >> 6;
-  for (int i = lane; i < 64 * b0; i += 64) yg[i] = make_double2(0.0, 0.0);
+  for (int i = lane; i < 64 * b0; i += 64) st(yg + i, szero<T>());
 
-  const int m16 = m1 * 16;
+  const int mES = m1 * ES;
   // Entries of column j for every slot: slot s's row is j + rf, rf = (lane +
   // 64 s - j mod 64) mod WIN; in range for 1 <= rf <= min(mb, n-1-j).  rf in
-  // 16-byte units.
-  auto fetch = [&](int j, cplx* dst) __attribute__((always_inline)) {
-    const unsigned base = (unsigned)j * (unsigned)m16;
-    const unsigned lim = (unsigned)max(min(mb, n - 1 - j), 0) * 16u;
-    const int ra = ((lane - (j & 63)) * 16) & RMASK;
+  // entry units (ES bytes).
+  auto fetch = [&](int j, T* dst) __attribute__((always_inline)) {
+    const unsigned base = (unsigned)j * (unsigned)mES;
+    const unsigned lim = (unsigned)max(min(mb, n - 1 - j), 0) * (unsigned)ES;
+    const int ra = ((lane - (j & 63)) * ES) & RMASK;
 #pragma unroll
     for (int q = 0; q < NS; ++q) {
-      const int rq = (ra + q * 0x400) & RMASK;
-      dst[q] = bldc(Lr, (unsigned)(rq - 16) < lim ? base + (unsigned)rq : kBufOut);
+      const int rq = (ra + q * 64 * ES) & RMASK;
+      bld(Lr, (unsigned)(rq - ES) < lim ? base + (unsigned)rq : kBufOut, dst[q]);
     }
   };
   // L_jj and b_{j+WIN} of block b's row j = 64 b + lane (L_jj reads zero for
   // rows past n-1; masked when the block becomes current)
-  auto load_blk = [&](int b, cplx& db, cplx& bb) __attribute__((always_inline)) {
+  auto load_blk = [&](int b, T& db, T& bb) __attribute__((always_inline)) {
     const int j = 64 * b + lane;
-    db = bldc(Lr, (unsigned)j * (unsigned)m16);
+    bld(Lr, (unsigned)j * (unsigned)mES, db);
     bb = bload(j + WIN);
   };
-  cplx dcur, dnxt, bcur, bnxt, nl, yo = mkc(0.0, 0.0);
+  T dcur, dnxt, bcur, bnxt, nl, yo = szero<T>();
   double ecur;
   // rows past n-1 (padding of the last block) change nothing: w = 0
   auto enter_blk = [&](int b) __attribute__((always_inline)) {
-    const cplx rc = crecip(dcur);
-    nl = csel(64 * b + lane < n, mkc(-rc.re, -rc.im), mkc(0.0, 0.0));
+    nl = sel(64 * b + lane < n, sneg(srecip(dcur)), szero<T>());
     ecur = bterm(64 * b + lane + WIN);
   };
   load_blk(b0, dcur, bcur);
   load_blk(b0 + 1, dnxt, bnxt);
-  cplx ys[NS];  // slot s: block b + s
+  T ys[NS];  // slot s: block b + s
 #pragma unroll
-  for (int q = 0; q < NS; ++q) {
-    ys[q] = bload(64 * (b0 + q) + lane);
-    ys[q].re += bterm(64 * (b0 + q) + lane);
-  }
-  cplx lr[DEPTH][NS];
+  for (int q = 0; q < NS; ++q) ys[q] = sadd_re(bload(64 * (b0 + q) + lane), bterm(64 * (b0 + q) + lane));
+  T lr[DEPTH][NS];
   // in step order, as the loop refills them: the loop's wait for step j's
   // entries then counts the same loads behind them on entry as on the back edge
 #pragma unroll
@@ -1453,7 +1022,7 @@ __global__ __launch_bounds__(64) void crt_fwdsweep_kernel(FwdArgs F) {
       // the copy out of bnxt comes first, so the load can write its registers
       // directly (a load into a temporary needs a copy after it, and the copy
       // a wait for every fetch in flight)
-      asm volatile("" : "+v"(bcur.re), "+v"(bcur.im));
+      spin(bcur);
       __builtin_amdgcn_sched_barrier(0);
       load_blk(b + 1, dnxt, bnxt);
     }
@@ -1462,13 +1031,13 @@ __global__ __launch_bounds__(64) void crt_fwdsweep_kernel(FwdArgs F) {
 #pragma unroll
       for (int d = 0; d < DEPTH; ++d) {
         const int j = 64 * b + c + d;
-        const cplx w = rlc(cmul(ys[0], nl), j & 63);
+        const T w = srl(smul(ys[0], nl), j & 63);
 #pragma unroll
         for (int q = 0; q < NS; ++q) {
-          ys[q] = cmadd(ys[q], lr[d][q], w);
+          ys[q] = madd(ys[q], lr[d][q], w);
           // the updates are formed before lr[d] is refilled (otherwise they
           // may sink below the refill, which then needs a copy and a wait)
-          asm volatile("" : "+v"(ys[q].re), "+v"(ys[q].im));
+          spin(ys[q]);
         }
         fetch(j + DEPTH, lr[d]);
         // the fetch stays here, DEPTH steps ahead of its use (the scheduler
@@ -1477,14 +1046,14 @@ __global__ __launch_bounds__(64) void crt_fwdsweep_kernel(FwdArgs F) {
         if ((d + 1) % G == 0) {
           // rows j-G+1 .. j done: their lanes keep y and refill slot 0
           const bool g = (unsigned)(lane - ((j & 63) - (G - 1))) < (unsigned)G;
-          yo = csel(g, cmul(ys[0], nl), yo);
-          ys[0] = csel(g, mkc(bcur.re + ecur, bcur.im), ys[0]);
+          yo = sel(g, smul(ys[0], nl), yo);
+          ys[0] = sel(g, sadd_re(bcur, ecur), ys[0]);
         }
       }
     }
     const int jo = 64 * b + lane;
-    if (jo < n) yg[jo] = make_double2(-yo.re, -yo.im);
-    const cplx t = ys[0];
+    if (jo < n) st(yg + jo, sneg(yo));
+    const T t = ys[0];
 #pragma unroll
     for (int q = 0; q + 1 < NS; ++q) ys[q] = ys[q + 1];
     ys[NS - 1] = t;
@@ -1495,3 +1064,6 @@ template __global__ void crt_fwdsweep_kernel<8, 8, 2>(FwdArgs);
 template __global__ void crt_fwdsweep_kernel<8, 4, 2>(FwdArgs);
 template __global__ void crt_fwdsweep_kernel<8, 8, 4>(FwdArgs);
 template __global__ void crt_fwdsweep_kernel<8, 4, 4>(FwdArgs);
+template __global__ void crt_fwdsweep_kernel<8, 8, 2, double>(FwdArgs);
+template __global__ void crt_fwdsweep_kernel<8, 4, 2, double>(FwdArgs);
+template __global__ void crt_fwdsweep_kernel<8, 8, 4, double>(FwdArgs);

@@ -88,31 +88,40 @@ struct FwdArgs {
 
 __global__ void crt_area_values_kernel(FactorArgs A, int K, double2* av);
 __global__ void crt_scale_kernel(FactorArgs A);
+template <class T = cplx>
 __global__ void crt_rows_kernel(FactorArgs A);
 __global__ void crt_rhs_kernel(FactorArgs A);
-template <int ROWS, int MAXT, int NT, bool GR, bool MF>
+// The factor kernel on complex (T = cplx) or the DC twin's real (T = double)
+// entries (choldc, src/choldc.f90, with vredc's forward sweep): the scaled
+// rows, L and y hold entries of T's size.
+template <int ROWS, int MAXT, int NT, bool GR, class T>
 __global__ void crt_factor_kernel(FactorArgs A);
-template <int ROWS, int MAXT, int NT, bool GR, bool MF>
+template <int ROWS, int MAXT, int NT, bool GR>
 __global__ void crt_factor_many_kernel(const FactorArgs* args, const int2* items, int nitems, int* counter);
-template <int DEPTH, int G, int NS>
+// Substitutions and the column-band copy on complex (T = cplx) or real (T =
+// double, the DC twin) factors.  kpot stays complex (imaginary part 0 for DC).
+template <int DEPTH, int G, int NS, class T = cplx>
 __global__ void crt_backsolve_kernel(BacksolveArgs B);
 template <int DEPTH, int G, int NS>
 __global__ void crt_backsolve_many_kernel(const BacksolveArgs* args, const int2* items, int nitems);
-__global__ void crt_lcol_kernel(const double2* L, double2* Lc, int n, int mb, int nfac);
-template <int DEPTH, int G, int NS>
+template <class T = cplx>
+__global__ void crt_lcol_kernel(const void* L, void* Lc, int n, int mb, int nfac);
+template <int DEPTH, int G, int NS, class T = cplx>
 __global__ void crt_fwdsweep_kernel(FwdArgs F);
 
-// LDS bytes of the factor kernel for bandwidth mb: the trailing-window rings,
-// two panels, three staged row batches and the panel's broadcast slots.
-inline size_t crt_factor_lds_bytes(int mb) {
-  const size_t M1 = (size_t)mb + 1, half = M1 / 2, W = (size_t)mb + kNB;
-  return (half * M1 + 2 * W * kNB + 3 * kNB * kMaxR + 4 * kNB) * 16 + 3 * kNB * kMaxR * 2;
+// Entries of the trailing-window rings in LDS for bandwidth mb and entry size
+// es bytes (16 complex, 8 real): half x (mb + 1), rounded up to an even count
+// for 8-byte entries so that what follows stays 16-byte aligned.
+__host__ __device__ inline int crt_ring_lds_entries(int mb, size_t es) {
+  const int M1 = mb + 1, n = (M1 / 2) * M1;
+  return es == 16 ? n : (n + 1) & ~1;
 }
-// The same with the rings in global memory (the wide-band form): panels,
-// staged batches and broadcast slots only.
-inline size_t crt_factor_lds_bytes_gr(int mb) {
-  const size_t W = (size_t)mb + kNB;
-  return (2 * W * kNB + 3 * kNB * kMaxR + 4 * kNB) * 16 + 3 * kNB * kMaxR * 2;
+// LDS bytes of the factor kernel for bandwidth mb, entry size es (16 complex,
+// 8 real) and the ring in LDS (gr false) or global memory: the ring, two
+// panels and two staged row batches with their band offsets.
+inline size_t crt_factor_lds_bytes(int mb, size_t es, bool gr) {
+  const size_t W = (size_t)mb + kNB, ring = gr ? 0 : (size_t)crt_ring_lds_entries(mb, es);
+  return (ring + 2 * W * kNB + 2 * kNB * kMaxR) * es + 2 * kNB * kMaxR * 2;
 }
 // Widest band the factorisation and substitutions take: the wide-band factor
 // form holds the W = mb + kNB panel rows in four rows per lane, and the
