@@ -92,7 +92,8 @@ struct Ring {
 // Panel buffer: W rows of kNB columns, column index swizzled by the row's
 // 4-row group so that tiles reading one column from different rows spread
 // over the LDS banks.
-__device__ __forceinline__ int pix(int rw, int k) { return rw * kNB + (k ^ ((rw >> 2) & (kNB - 1))); }
+__device__ __forceinline__ int pkey(int rw) { return (rw >> 2) & (kNB - 1); }
+__device__ __forceinline__ int pix(int rw, int k) { return rw * kNB + (k ^ pkey(rw)); }
 
 // Value at band offset t of row i of a staged batch (0 when t is not one of
 // the row's stored offsets).
@@ -308,36 +309,40 @@ __device__ __forceinline__ void trailing_tiles(int mb, int (&rs_r0)[MAXT], int (
   }
 }
 
-// TR x TC tile of the stage window at (rw0, cw0) in the ring-resident form:
-// the current values from the ring (every window entry lives there; 0
-// outside the window's lower triangle and the matrix, branch-free with the
-// positions clamped in range) minus panel J-1's rank-kNB product, in
-// ascending k.
+// TR x TC tile of the stage window at (rw0, cw0): the current values from the
+// ring (every window entry lives there; 0 outside the window's lower triangle
+// and the matrix) minus panel J-1's rank-kNB product, in ascending k.  slot
+// receives each entry's ring position (0 where the entry does not exist), for
+// the caller's store.  rw0 and cw0 are multiples of TR = TC (2 or 4), so the
+// tile's operand rows of panel J-1 (stage rows + kNB) lie in one 4-row group:
+// they share one swizzle key and sit kNB entries apart, and the panel buffer's
+// 4 pad rows take the groups that reach past its W rows (their products fall
+// on entries that are not stored).
 template <int TR, int TC, class T>
-__device__ __forceinline__ void ring_tile(const Stage<T>& C, const T* Tt, int rw0, int cw0, T acc[TR][TC]) {
+__device__ __forceinline__ void ring_tile(const Stage<T>& C, const T* Tt, int rw0, int cw0, T acc[TR][TC],
+                                          int slot[TR][TC]) {
 #pragma unroll
-  for (int i = 0; i < TR; ++i) {
-    const int rw = rw0 + i;
-    const int rwc = min(max(rw, 0), C.mb - 1);
+  for (int i = 0; i < TR; ++i)
 #pragma unroll
     for (int j = 0; j < TC; ++j) {
-      const int cw = cw0 + j;
-      acc[i][j] = sel(tile_ok(C, rw, cw), Tt[C.ring.pos(rwc, min(max(cw, 0), rwc))], szero<T>());
+      const int rw = rw0 + i, cw = cw0 + j;
+      const bool ok = tile_ok(C, rw, cw);
+      const int rwc = min(max(rw, 0), C.mb - 1);
+      slot[i][j] = ok ? C.ring.pos(rwc, min(max(cw, 0), rwc)) : 0;
+      acc[i][j] = sel(ok, Tt[slot[i][j]], szero<T>());
     }
-  }
   if (!C.apply) return;
-  int ra[TR], ca[TC];
-#pragma unroll
-  for (int i = 0; i < TR; ++i) ra[i] = min(rw0 + i + kNB, C.W - 1);
-#pragma unroll
-  for (int j = 0; j < TC; ++j) ca[j] = min(cw0 + j + kNB, C.W - 1);
+  const int ra0 = rw0 + kNB, ca0 = cw0 + kNB;
+  const T* pa = C.Pp + ra0 * kNB;
+  const T* pb = C.Pp + ca0 * kNB;
+  const int ka = pkey(ra0), kb = pkey(ca0);
 #pragma unroll (TR * TC <= 4 ? kNB : 1)
   for (int k = 0; k < kNB; ++k) {
     T a[TR], b[TC];
 #pragma unroll
-    for (int i = 0; i < TR; ++i) a[i] = C.Pp[pix(ra[i], k)];
+    for (int i = 0; i < TR; ++i) a[i] = pa[i * kNB + (k ^ ka)];
 #pragma unroll
-    for (int j = 0; j < TC; ++j) b[j] = C.Pp[pix(ca[j], k)];
+    for (int j = 0; j < TC; ++j) b[j] = pb[j * kNB + (k ^ kb)];
 #pragma unroll
     for (int i = 0; i < TR; ++i)
 #pragma unroll
@@ -380,8 +385,9 @@ __device__ __forceinline__ void factor_system(const FactorArgs& A, int s, double
   const int n = A.n, mb = A.mb, R = A.R;
   const int W = mb + kNB, M1 = mb + 1, half = (mb + 1) / 2;
   T* Tt = GR ? reinterpret_cast<T*>(A.ring) + (size_t)blockIdx.x * A.ring_stride : reinterpret_cast<T*>(smem);
-  T* P0 = GR ? reinterpret_cast<T*>(smem) : Tt + crt_ring_lds_entries(mb, ES);  // 2 panels of W x kNB
-  T* sv = P0 + 2 * W * kNB;                   // 2 batches of kNB x kMaxR values
+  const int WP = W + 4;                       // panel rows with the 4 pad rows
+  T* P0 = GR ? reinterpret_cast<T*>(smem) : Tt + crt_ring_lds_entries(mb, ES);  // 2 panels of WP x kNB
+  T* sv = P0 + 2 * WP * kNB;                  // 2 batches of kNB x kMaxR values
   int16_t* so = reinterpret_cast<int16_t*>(sv + 2 * NB);  // their band offsets
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int x3 = tid - 64;                    // thread index within waves 1..
@@ -458,8 +464,8 @@ __device__ __forceinline__ void factor_system(const FactorArgs& A, int s, double
     C.o = o;
     C.ring = Ring{mb, M1, half, o % M1};
     C.apply = J > 0;
-    C.Pp = P0 + ((J + 1) & 1) * W * kNB;
-    T* Pc = P0 + (J & 1) * W * kNB;
+    C.Pp = P0 + ((J + 1) & 1) * WP * kNB;
+    T* Pc = P0 + (J & 1) * WP * kNB;
     T* svc = sv + (J & 1) * NB;                 // batch J: rows [o + mb, o + mb + kNB)
     int16_t* soc = so + (J & 1) * NB;
     T* svn = sv + ((J + 1) & 1) * NB;           // batch J+1
@@ -485,14 +491,15 @@ __device__ __forceinline__ void factor_system(const FactorArgs& A, int s, double
           }
         const int rw0 = 2 * (i + j), cw0 = 2 * j;
         T acc[2][2];
-        ring_tile<2, 2>(C, Tt, rw0, cw0, acc);
+        int slot[2][2];
+        ring_tile<2, 2>(C, Tt, rw0, cw0, acc, slot);
 #pragma unroll
         for (int a = 0; a < 2; ++a)
 #pragma unroll
           for (int b = 0; b < 2; ++b) {
             const int rw = rw0 + a, cw = cw0 + b;
             if (rw < mb) Pc[pix(rw, cw)] = acc[a][b];
-            if (tile_ok(C, rw, cw)) Tt[C.ring.pos(rw, cw)] = szero<T>();
+            if (tile_ok(C, rw, cw)) Tt[slot[a][b]] = szero<T>();
           }
       }
     } else {
@@ -617,12 +624,13 @@ __device__ __forceinline__ void factor_system(const FactorArgs& A, int s, double
       for (int q = 0; q < MAXT; ++q) {
         if (rs_r0[q] >= 0) {
           T acc[4][4];
-          ring_tile<4, 4>(C, Tt, rs_r0[q], rs_c0[q], acc);
+          int slot[4][4];
+          ring_tile<4, 4>(C, Tt, rs_r0[q], rs_c0[q], acc, slot);
 #pragma unroll
           for (int i = 0; i < 4; ++i)
 #pragma unroll
             for (int j = 0; j < 4; ++j)
-              if (tile_ok(C, rs_r0[q] + i, rs_c0[q] + j)) Tt[C.ring.pos(rs_r0[q] + i, rs_c0[q] + j)] = acc[i][j];
+              if (tile_ok(C, rs_r0[q] + i, rs_c0[q] + j)) Tt[slot[i][j]] = acc[i][j];
         }
       }
       if (x3 < NB) {

@@ -118,10 +118,11 @@ __host__ __device__ inline int crt_ring_lds_entries(int mb, size_t es) {
 }
 // LDS bytes of the factor kernel for bandwidth mb, entry size es (16 complex,
 // 8 real) and the ring in LDS (gr false) or global memory: the ring, two
-// panels and two staged row batches with their band offsets.
+// panels of mb + kNB rows and 4 pad rows, and two staged row batches with
+// their band offsets.
 inline size_t crt_factor_lds_bytes(int mb, size_t es, bool gr) {
-  const size_t W = (size_t)mb + kNB, ring = gr ? 0 : (size_t)crt_ring_lds_entries(mb, es);
-  return (ring + 2 * W * kNB + 2 * kNB * kMaxR) * es + 2 * kNB * kMaxR * 2;
+  const size_t WP = (size_t)mb + kNB + 4, ring = gr ? 0 : (size_t)crt_ring_lds_entries(mb, es);
+  return (ring + 2 * WP * kNB + 2 * kNB * kMaxR) * es + 2 * kNB * kMaxR * 2;
 }
 // Widest band the factorisation and substitutions take: the wide-band factor
 // form holds the W = mb + kNB panel rows in four rows per lane, and the
