@@ -124,6 +124,14 @@ __device__ __forceinline__ double msub(double a, double b, double c) { return a 
 __device__ __forceinline__ cplx msub(cplx a, cplx b, cplx c) { return cmsub(a, b, c); }
 __device__ __forceinline__ double madd(double a, double b, double c) { return a + b * c; }
 __device__ __forceinline__ cplx madd(cplx a, cplx b, cplx c) { return cmadd(a, b, c); }
+// acc + b*c as fused multiply-adds in a fixed order (terms of a block
+// product accumulated from zero)
+__device__ __forceinline__ double fmacc(double acc, double b, double c) { return fma(b, c, acc); }
+__device__ __forceinline__ cplx fmacc(cplx acc, cplx b, cplx c) {
+  return mkc(fma(-b.im, c.im, fma(b.re, c.re, acc.re)), fma(b.im, c.re, fma(b.re, c.im, acc.im)));
+}
+__device__ __forceinline__ double ssub(double a, double b) { return a - b; }
+__device__ __forceinline__ cplx ssub(cplx a, cplx b) { return csub(a, b); }
 __device__ __forceinline__ double smul(double a, double b) { return a * b; }
 __device__ __forceinline__ cplx smul(cplx a, cplx b) { return cmul(a, b); }
 __device__ __forceinline__ double sneg(double a) { return -a; }

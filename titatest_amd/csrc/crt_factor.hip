@@ -95,6 +95,16 @@ struct Ring {
 __device__ __forceinline__ int pkey(int rw) { return (rw >> 2) & (kNB - 1); }
 __device__ __forceinline__ int pix(int rw, int k) { return rw * kNB + (k ^ pkey(rw)); }
 
+// Panel rows of wave 0's lanes.  Slot 0: lanes 0-7 of every 16-lane row hold
+// the panel's diagonal rows 0-7 (four copies, so that a DPP row broadcast
+// reads diagonal row c from lane c), lanes 8-15 rows 8 + 8 g + (lane - 8) of
+// 16-lane row g (rows 8-39); slot q >= 1: row 40 + 64 (q - 1) + lane.  Slots
+// needed for the W = mb + kNB panel rows: 1 + ceil((W - 40) / 64).
+__device__ __forceinline__ int prow(int lane, int q) {
+  const int j = lane & 15;
+  return q == 0 ? (j < kNB ? j : kNB + kNB * (lane >> 4) + (j - kNB)) : 40 + 64 * (q - 1) + lane;
+}
+
 // Value at band offset t of row i of a staged batch (0 when t is not one of
 // the row's stored offsets).
 template <class T>
@@ -114,6 +124,31 @@ __device__ __forceinline__ double rl(double x, int l) {
 __device__ __forceinline__ cplx rlc(cplx z, int l) { return mkc(rl(z.re, l), rl(z.im, l)); }
 __device__ __forceinline__ double srl(double x, int l) { return rl(x, l); }
 __device__ __forceinline__ cplx srl(cplx z, int l) { return rlc(z, l); }
+// Lane L of every 16-lane row to the whole row (DPP row_newbcast, two 32-bit
+// moves per double); rbcast takes L from a fully unrolled loop, where the
+// switch folds to one case.
+template <int L>
+__device__ __forceinline__ double rbc(double x) {
+  const long long b = __double_as_longlong(x);
+  const int lo = __builtin_amdgcn_mov_dpp((int)(unsigned)(b & 0xffffffffLL), 0x150 + L, 0xF, 0xF, true);
+  const int hi = __builtin_amdgcn_mov_dpp((int)(b >> 32), 0x150 + L, 0xF, 0xF, true);
+  return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+template <int L>
+__device__ __forceinline__ cplx rbc(cplx z) { return mkc(rbc<L>(z.re), rbc<L>(z.im)); }
+template <class T>
+__device__ __forceinline__ T rbcast(T x, int l) {
+  switch (l) {
+    case 0: return rbc<0>(x);
+    case 1: return rbc<1>(x);
+    case 2: return rbc<2>(x);
+    case 3: return rbc<3>(x);
+    case 4: return rbc<4>(x);
+    case 5: return rbc<5>(x);
+    case 6: return rbc<6>(x);
+    default: return rbc<7>(x);
+  }
+}
 __device__ __forceinline__ double sshfl(double x, int src) { return __shfl(x, src, 64); }
 __device__ __forceinline__ cplx sshfl(cplx z, int src) { return mkc(__shfl(z.re, src, 64), __shfl(z.im, src, 64)); }
 
@@ -321,33 +356,55 @@ __device__ __forceinline__ void trailing_tiles(int mb, int (&rs_r0)[MAXT], int (
 template <int TR, int TC, class T>
 __device__ __forceinline__ void ring_tile(const Stage<T>& C, const T* Tt, int rw0, int cw0, T acc[TR][TC],
                                           int slot[TR][TC]) {
-#pragma unroll
-  for (int i = 0; i < TR; ++i)
-#pragma unroll
-    for (int j = 0; j < TC; ++j) {
-      const int rw = rw0 + i, cw = cw0 + j;
-      const bool ok = tile_ok(C, rw, cw);
-      const int rwc = min(max(rw, 0), C.mb - 1);
-      slot[i][j] = ok ? C.ring.pos(rwc, min(max(cw, 0), rwc)) : 0;
-      acc[i][j] = sel(ok, Tt[slot[i][j]], szero<T>());
-    }
-  if (!C.apply) return;
-  const int ra0 = rw0 + kNB, ca0 = cw0 + kNB;
-  const T* pa = C.Pp + ra0 * kNB;
-  const T* pb = C.Pp + ca0 * kNB;
-  const int ka = pkey(ra0), kb = pkey(ca0);
-#pragma unroll (TR * TC <= 4 ? kNB : 1)
-  for (int k = 0; k < kNB; ++k) {
-    T a[TR], b[TC];
-#pragma unroll
-    for (int i = 0; i < TR; ++i) a[i] = pa[i * kNB + (k ^ ka)];
-#pragma unroll
-    for (int j = 0; j < TC; ++j) b[j] = pb[j * kNB + (k ^ kb)];
+  // The rank-kNB block product P = sum_k L_ik L_jk from zero (fused
+  // multiply-adds, ascending k), subtracted from each entry once: as accurate
+  // as the reference's entry-by-entry s - x*y (it rounds the products alone,
+  // not the running entry), with 4 instead of 6 operations per complex term
+  // and independent accumulator chains.  Small tiles read the window's
+  // entries before the product (their LDS latency hides under it); 4x4 tiles
+  // read them after it, which keeps the registers below the kernel's limit.
+  constexpr bool early = TR * TC <= 4;
+  auto window = [&]() __attribute__((always_inline)) {
 #pragma unroll
     for (int i = 0; i < TR; ++i)
 #pragma unroll
-      for (int j = 0; j < TC; ++j) acc[i][j] = msub(acc[i][j], a[i], b[j]);
+      for (int j = 0; j < TC; ++j) {
+        const int rw = rw0 + i, cw = cw0 + j;
+        const bool ok = tile_ok(C, rw, cw);
+        const int rwc = min(max(rw, 0), C.mb - 1);
+        slot[i][j] = ok ? C.ring.pos(rwc, min(max(cw, 0), rwc)) : 0;
+        acc[i][j] = sel(ok, Tt[slot[i][j]], szero<T>());
+      }
+  };
+  if (early) window();
+  T P[TR][TC];
+#pragma unroll
+  for (int i = 0; i < TR; ++i)
+#pragma unroll
+    for (int j = 0; j < TC; ++j) P[i][j] = szero<T>();
+  if (C.apply) {
+    const int ra0 = rw0 + kNB, ca0 = cw0 + kNB;
+    const T* pa = C.Pp + ra0 * kNB;
+    const T* pb = C.Pp + ca0 * kNB;
+    const int ka = pkey(ra0), kb = pkey(ca0);
+#pragma unroll (early ? kNB : 1)
+    for (int k = 0; k < kNB; ++k) {
+      T a[TR], b[TC];
+#pragma unroll
+      for (int i = 0; i < TR; ++i) a[i] = pa[i * kNB + (k ^ ka)];
+#pragma unroll
+      for (int j = 0; j < TC; ++j) b[j] = pb[j * kNB + (k ^ kb)];
+#pragma unroll
+      for (int i = 0; i < TR; ++i)
+#pragma unroll
+        for (int j = 0; j < TC; ++j) P[i][j] = fmacc(P[i][j], a[i], b[j]);
+    }
   }
+  if (!early) window();
+#pragma unroll
+  for (int i = 0; i < TR; ++i)
+#pragma unroll
+    for (int j = 0; j < TC; ++j) acc[i][j] = ssub(acc[i][j], P[i][j]);
 }
 
 // ---------------------------------------------------------------------------
@@ -369,8 +426,10 @@ __device__ __forceinline__ void ring_tile(const Stage<T>& C, const T* Tt, int rw
 //   stage (0 outside the band, the matrix and the lower triangle), the
 //   columns past the matrix end (last stage) are identity columns, and L goes
 //   out through range-checked buffer stores.  Broadcasts within the panel
-//   wave are lane reads into scalar registers, not LDS round trips.
-// ROWS: panel rows per lane of wave 0 (W = mb + kNB <= 64 ROWS); MAXT:
+//   wave are DPP row broadcasts from the diagonal rows, which lanes 0-7 of
+//   every 16-lane row hold.
+// ROWS: panel row slots per lane of wave 0 (prow; W <= 40 + 64 (ROWS - 1));
+// MAXT:
 // trailing tiles per thread of the update waves; NT threads per workgroup:
 // 256 (one panel and three update waves; two workgroups per CU, three for
 // the real entries) or, when one workgroup's LDS leaves no room for a second
@@ -445,10 +504,11 @@ __device__ __forceinline__ void factor_system(const FactorArgs& A, int s, double
     if (Bs) return sfrom<T>(ldc(Bs + min(max(r, 0), n - 1)));
     return sreal<T>(r == rsink ? bs : (r == rel ? be : 0.0));
   };
+  // wave 0's forward-sweep values of its panel rows
   T yv[ROWS];
 #pragma unroll
   for (int q = 0; q < ROWS; ++q) {
-    const int rw = lane + 64 * q;
+    const int rw = prow(lane, q);
     yv[q] = sel(wave == 0 && rw < W && rw < n, rhs(rw), szero<T>());
   }
   stage_barrier<GR>();
@@ -475,13 +535,16 @@ __device__ __forceinline__ void factor_system(const FactorArgs& A, int s, double
     // being hoisted out of the stage loop (it would hold ~100 registers)
 #pragma unroll
     for (int q = 0; q < MAXT; ++q) asm volatile("" : "+v"(rs_r0[q]), "+v"(rs_c0[q]));
+    // the same for the lane-derived row and record indices below
+    int ln = lane, lx = x3;
+    asm volatile("" : "+v"(ln), "+v"(lx));
 
     // ---- phase A: waves 1.. apply panel J-1 to panel J's columns (out of the
     // ring into the panel buffer, their slots zeroed); wave 0 copies batch J's
     // entries in panel J's columns into the panel's last kNB rows
     if (wave > 0) {
       __builtin_amdgcn_s_setprio(2);
-      for (int x = x3; x < nla; x += NT - 64) {
+      for (int x = lx; x < nla; x += NT - 64) {
         int j = 0, i = x;
 #pragma unroll
         for (int jj = 0; jj < kNB / 2 - 1; ++jj)
@@ -503,7 +566,7 @@ __device__ __forceinline__ void factor_system(const FactorArgs& A, int s, double
           }
       }
     } else {
-      const int i = lane / kNB, cw = lane % kNB, rw = mb + i;
+      const int i = ln / kNB, cw = ln % kNB, rw = mb + i;
       const bool ok = o + rw < n && cw >= i;
       Pc[pix(rw, cw)] = sel(ok, staged(svc, soc, i, cw - i), szero<T>());
     }
@@ -516,47 +579,53 @@ __device__ __forceinline__ void factor_system(const FactorArgs& A, int s, double
     if (wave == 0) {
       __builtin_amdgcn_s_setprio(3);
       const int rows = min(W, n - o);
+      const int lj = ln & 15;
+      const bool dg = lj < kNB;  // the lane's slot-0 row is a diagonal row
+      int rwq[ROWS];
+#pragma unroll
+      for (int q = 0; q < ROWS; ++q) rwq[q] = prow(ln, q);
       T v[ROWS][kNB];
 #pragma unroll
       for (int q = 0; q < ROWS; ++q) {
-        const int rw = lane + 64 * q;
+        const int rw = min(rwq[q], W + 3);  // rows past W read the pad rows; never stored
 #pragma unroll
-        for (int k = 0; k < kNB; ++k) v[q][k] = sel(rw < W, Pc[pix(min(rw, W - 1), k)], szero<T>());
+        for (int k = 0; k < kNB; ++k) v[q][k] = Pc[pix(rw, k)];
       }
       // identity columns past the matrix end
+      if (nbc < kNB) {
 #pragma unroll
-      for (int k = 0; k < kNB; ++k) v[0][k] = sel(lane == k && k >= nbc, sreal<T>(1.0), v[0][k]);
+        for (int k = 0; k < kNB; ++k) v[0][k] = sel(dg && lj == k && k >= nbc, sreal<T>(1.0), v[0][k]);
+      }
       // Column c in the L D L^T form of the same elimination: with d the pivot
       // and s_rc the column's updated entries, l_rc = s_rc / d and the later
-      // columns take -l_rc s_c2c (= -L_rc L_c2c).  The next pivot d_{c+1} =
-      // s_{c+1,c+1} - l_{c+1,c} s_{c+1,c} needs no square root, so the
-      // column-to-column dependency is a reciprocal and two products; L_cc =
-      // sqrt(d), L_rc = l_rc L_cc and y_c = y'_c / L_cc follow off it.  lrc is
-      // 0 for rows at or above c, so every update is unconditional (a - 0 * x
-      // leaves a unchanged).  Broadcasts are lane reads into scalar registers.
-      T piv = srl(v[0][0], 0);
-      T dd = sreal<T>(1.0), invd = sreal<T>(1.0);  // lane c: d_c and 1/d_c
+      // columns take -l_rc s_c2c (= -L_rc L_c2c).  The next pivot needs no
+      // square root, so the column-to-column dependency is a reciprocal and
+      // two products; L_cc = sqrt(d), L_rc = l_rc L_cc and y_c = y'_c / L_cc
+      // follow off it.  lrc is 0 for the diagonal rows at or above c, so every
+      // update is unconditional.  Broadcasts are DPP row broadcasts from the
+      // diagonal rows.
+      const bool dcz = A.dc != 0;
+      T dd = sreal<T>(1.0);  // d of the lane's diagonal row
       bool zero_pivot = false;
 #pragma unroll
       for (int c = 0; c < kNB; ++c) {
+        const T piv = rbcast(v[0][c], c);
         // chol :66 -- |s| <= 0 is an exact zero (a NaN pivot passes, as
         // there); choldc :66 -- s <= 0 for the real DC pivots
         if constexpr (std::is_same<T, cplx>::value)
-          zero_pivot |= A.dc ? piv.re <= 0.0 : (piv.re == 0.0 && piv.im == 0.0);
+          zero_pivot |= (dcz & (piv.re <= 0.0)) | (!dcz & (piv.re == 0.0) & (piv.im == 0.0));
         else
           zero_pivot |= piv <= 0.0;
         const T inv = srecip(piv);
-        dd = sel(lane == c, piv, dd);
-        invd = sel(lane == c, inv, invd);
+        dd = sel(lj == c, piv, dd);
         T lrc[ROWS];
+        lrc[0] = sel(!dg || lj > c, smul(v[0][c], inv), szero<T>());
 #pragma unroll
-        for (int q = 0; q < ROWS; ++q) lrc[q] = sel(lane + 64 * q > c, smul(v[q][c], inv), szero<T>());
-        T pivn = piv;
-        if (c + 1 < kNB) pivn = srl(msub(v[0][c + 1], lrc[0], v[0][c]), c + 1);
-        const T yp = srl(yv[0], c);  // y'_c, final
+        for (int q = 1; q < ROWS; ++q) lrc[q] = smul(v[q][c], inv);
+        const T yp = rbcast(yv[0], c);  // y'_c, final
 #pragma unroll
         for (int c2 = c + 1; c2 < kNB; ++c2) {
-          const T s2 = srl(v[0][c], c2);  // s_{c2,c} of the panel's diagonal block
+          const T s2 = rbcast(v[0][c], c2);  // s_{c2,c} of the diagonal block
 #pragma unroll
           for (int q = 0; q < ROWS; ++q) v[q][c2] = msub(v[q][c2], lrc[q], s2);
         }
@@ -565,45 +634,57 @@ __device__ __forceinline__ void factor_system(const FactorArgs& A, int s, double
           yv[q] = msub(yv[q], lrc[q], yp);
           v[q][c] = lrc[q];
         }
-        piv = pivn;
       }
       const T lcc = ssqrt(dd);
 #pragma unroll
       for (int c = 0; c < kNB; ++c) {
-        const T lc = srl(lcc, c);
+        const T lc = rbcast(lcc, c);
 #pragma unroll
         for (int q = 0; q < ROWS; ++q) v[q][c] = smul(v[q][c], lc);
-        v[0][c] = sel(lane == c, lc, v[0][c]);
+        v[0][c] = sel(dg && lj == c, lc, v[0][c]);
       }
-      yv[0] = sel(lane < kNB, smul(smul(yv[0], lcc), invd), yv[0]);
-      if (zero_pivot && lane == 0 && atomicCAS(A.err, 0, 28) == 0) A.err[1] = sys;
-      // L of panel J: to the panel buffer (stage J+1's operand; 0 where it is
-      // not an entry) and to the band rows
+      yv[0] = sel(dg, smul(yv[0], srecip(lcc)), yv[0]);
+      if (zero_pivot && ln == 0 && atomicCAS(A.err, 0, 28) == 0) A.err[1] = sys;
+      // L of panel J: rows kNB.. to the panel buffer (stage J+1's operands;
+      // the diagonal rows are not read there) and every row to the band rows
+      // (the diagonal rows from lanes 0-7)
 #pragma unroll
       for (int q = 0; q < ROWS; ++q) {
-        const int rw = lane + 64 * q;
-        if (rw < W) {
+        const int rw = rwq[q];
+        if (rw >= kNB && rw < W) {
 #pragma unroll
           for (int k = 0; k < kNB; ++k) Pc[pix(rw, k)] = v[q][k];
         }
+        const bool mine = q > 0 || !dg || ln < 16;
         const int klo = max(rw - mb, 0), khi = min(rw, nbc - 1);
 #pragma unroll
         for (int k = 0; k < kNB; ++k) {
-          const bool ok = rw < rows && k >= klo && k <= khi;
+          const bool ok = mine && rw < rows && k >= klo && k <= khi;
           bst(Lr, ok ? (unsigned)(((o + rw) * M1 + (k - rw + mb)) * ES) : kBufOut, v[q][k]);
         }
       }
-      if (lane < nbc) st(yg + o + lane, yv[0]);
-      // y of rows [o + kNB, o + W) moves up kNB rows; entering rows get b
-      const int src = (lane + kNB) & 63;
+      if (ln < nbc) st(yg + o + ln, yv[0]);
+      // y of rows [o + kNB, o + W) moves up kNB rows (row r of the next
+      // window is row r + kNB of this one: slot q or q + 1 of another lane);
+      // entering rows get b
       T ny[ROWS];
 #pragma unroll
       for (int q = 0; q < ROWS; ++q) {
-        const T a = sshfl(yv[q], src);
+        const int r1 = rwq[q] + kNB;  // this window's row
+        int sl;
+        bool hi;
+        if (r1 < 40) {  // slot 0
+          sl = 16 * ((r1 - kNB) >> 3) + kNB + ((r1 - kNB) & 7);
+          hi = false;
+        } else {
+          sl = (r1 - 40) & 63;
+          hi = 1 + (r1 - 40) / 64 > q;
+        }
+        const T a = sshfl(yv[q], sl);
         T b = szero<T>();
-        if (q + 1 < ROWS) b = sshfl(yv[q + 1], src);
-        ny[q] = sel(lane + kNB < 64, a, b);
-        const int rwn = lane + 64 * q;
+        if (q + 1 < ROWS) b = sshfl(yv[q + 1], sl);
+        ny[q] = sel(hi, b, a);
+        const int rwn = rwq[q];
         if (rwn >= mb) {
           const int r = o + kNB + rwn;
           ny[q] = sel(rwn < W && r < n, rhs(r), szero<T>());
@@ -618,7 +699,7 @@ __device__ __forceinline__ void factor_system(const FactorArgs& A, int s, double
       // staged after the tiles
       T nv = szero<T>();
       int nt = -1;
-      if (x3 < NB) row_slot(A, Sg, o + kNB + mb + x3 / kMaxR, x3 % kMaxR, nv, nt);
+      if (lx < NB) row_slot(A, Sg, o + kNB + mb + lx / kMaxR, lx % kMaxR, nv, nt);
       // the rest of the window: panel J-1's update in place
 #pragma unroll
       for (int q = 0; q < MAXT; ++q) {
@@ -633,16 +714,16 @@ __device__ __forceinline__ void factor_system(const FactorArgs& A, int s, double
               if (tile_ok(C, rs_r0[q] + i, rs_c0[q] + j)) Tt[slot[i][j]] = acc[i][j];
         }
       }
-      if (x3 < NB) {
+      if (lx < NB) {
         // batch J's entries right of panel J into the ring (window J+1: row
         // mb - kNB + i, column i + t - kNB)
-        const int i = x3 / kMaxR, t = soc[x3], cw1 = i + t - kNB;
+        const int i = lx / kMaxR, t = soc[lx], cw1 = i + t - kNB;
         if (t >= 0 && cw1 >= 0) {
           const Ring ring1{mb, M1, half, (o + kNB) % M1};
-          Tt[ring1.pos(mb - kNB + i, cw1)] = svc[x3];
+          Tt[ring1.pos(mb - kNB + i, cw1)] = svc[lx];
         }
-        svn[x3] = nv;
-        son[x3] = (int16_t)nt;
+        svn[lx] = nv;
+        son[lx] = (int16_t)nt;
       }
       clk.mark(2);
     }
@@ -698,6 +779,7 @@ template __global__ void crt_factor_kernel<2, 2, 512, false, cplx>(FactorArgs);
 template __global__ void crt_factor_kernel<3, 2, 512, false, cplx>(FactorArgs);
 template __global__ void crt_factor_kernel<3, 5, 512, true, cplx>(FactorArgs);
 template __global__ void crt_factor_kernel<4, 5, 512, true, cplx>(FactorArgs);
+template __global__ void crt_factor_kernel<5, 5, 512, true, cplx>(FactorArgs);
 template __global__ void crt_factor_kernel<1, 1, 256, false, double>(FactorArgs);
 template __global__ void crt_factor_kernel<2, 1, 256, false, double>(FactorArgs);
 template __global__ void crt_factor_kernel<2, 2, 256, false, double>(FactorArgs);
@@ -708,6 +790,7 @@ template __global__ void crt_factor_kernel<2, 2, 512, false, double>(FactorArgs)
 template __global__ void crt_factor_kernel<3, 2, 512, false, double>(FactorArgs);
 template __global__ void crt_factor_kernel<3, 5, 512, true, double>(FactorArgs);
 template __global__ void crt_factor_kernel<4, 5, 512, true, double>(FactorArgs);
+template __global__ void crt_factor_kernel<5, 5, 512, true, double>(FactorArgs);
 template __global__ void crt_factor_many_kernel<1, 1, 256, false>(const FactorArgs*, const int2*, int, int*);
 template __global__ void crt_factor_many_kernel<2, 1, 256, false>(const FactorArgs*, const int2*, int, int*);
 template __global__ void crt_factor_many_kernel<2, 2, 256, false>(const FactorArgs*, const int2*, int, int*);
@@ -718,6 +801,7 @@ template __global__ void crt_factor_many_kernel<2, 2, 512, false>(const FactorAr
 template __global__ void crt_factor_many_kernel<3, 2, 512, false>(const FactorArgs*, const int2*, int, int*);
 template __global__ void crt_factor_many_kernel<3, 5, 512, true>(const FactorArgs*, const int2*, int, int*);
 template __global__ void crt_factor_many_kernel<4, 5, 512, true>(const FactorArgs*, const int2*, int, int*);
+template __global__ void crt_factor_many_kernel<5, 5, 512, true>(const FactorArgs*, const int2*, int, int*);
 
 // ---------------------------------------------------------------------------
 // Back substitution, one wavefront per right-hand side.  Lane q (q < 64) keeps

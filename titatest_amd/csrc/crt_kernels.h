@@ -125,6 +125,7 @@ inline size_t crt_factor_lds_bytes(int mb, size_t es, bool gr) {
   return (ring + 2 * WP * kNB + 2 * kNB * kMaxR) * es + 2 * kNB * kMaxR * 2;
 }
 // Widest band the factorisation and substitutions take: the wide-band factor
-// form holds the W = mb + kNB panel rows in four rows per lane, and the
+// form holds the mb panel rows below the diagonal block in four rows per
+// lane, and the
 // substitutions' 256-row slot window needs 8-row groups below 257 - mb.
 constexpr int kMaxBand = 248;
