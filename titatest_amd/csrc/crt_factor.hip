@@ -427,7 +427,7 @@ __device__ __forceinline__ void ring_tile(const Stage<T>& C, const T* Tt, int rw
 //   columns past the matrix end (last stage) are identity columns, and L goes
 //   out through range-checked buffer stores.  Broadcasts within the panel
 //   wave are DPP row broadcasts from the diagonal rows, which lanes 0-7 of
-//   every 16-lane row hold.
+//   every 16-lane row hold, or lane reads.
 // ROWS: panel row slots per lane of wave 0 (prow; W <= 40 + 64 (ROWS - 1));
 // MAXT:
 // trailing tiles per thread of the update waves; NT threads per workgroup:
@@ -602,8 +602,10 @@ __device__ __forceinline__ void factor_system(const FactorArgs& A, int s, double
       // square root, so the column-to-column dependency is a reciprocal and
       // two products; L_cc = sqrt(d), L_rc = l_rc L_cc and y_c = y'_c / L_cc
       // follow off it.  lrc is 0 for the diagonal rows at or above c, so every
-      // update is unconditional.  Broadcasts are DPP row broadcasts from the
-      // diagonal rows.
+      // update is unconditional.  The pivot and y'_c, on the column-to-column
+      // chain, come by DPP row broadcast from the diagonal rows; the other
+      // entries of the diagonal block by lane reads into scalar registers,
+      // which keeps them out of the vector registers.
       const bool dcz = A.dc != 0;
       T dd = sreal<T>(1.0);  // d of the lane's diagonal row
       bool zero_pivot = false;
@@ -625,7 +627,7 @@ __device__ __forceinline__ void factor_system(const FactorArgs& A, int s, double
         const T yp = rbcast(yv[0], c);  // y'_c, final
 #pragma unroll
         for (int c2 = c + 1; c2 < kNB; ++c2) {
-          const T s2 = rbcast(v[0][c], c2);  // s_{c2,c} of the diagonal block
+          const T s2 = srl(v[0][c], c2);  // s_{c2,c} of the diagonal block
 #pragma unroll
           for (int q = 0; q < ROWS; ++q) v[q][c2] = msub(v[q][c2], lrc[q], s2);
         }
