@@ -450,8 +450,7 @@ class CRT:
             out[nm] = (float(ms[i]), int(cnt[i]))
         return out
 
-    # wave 0 (panel wave) then waves 1-3 (trailing waves), per stage phase
-    # wave 0 (panel wave) then waves 1-3 (trailing waves), per stage phase
+    # wave 0 (panel wave) then waves 1.. (trailing waves), per stage phase
     PHASES = ("w0_prologue", "w0_wait_lookahead", "w0_panel", "w0_wait_trailing",
               "w1_prologue_lookahead", "w1_wait_lookahead", "w1_trailing", "w1_wait_panel")
 
