@@ -101,6 +101,9 @@ def test_device_variants_equal_host_variants(handle, crtmod):
     wm, wr, wp = t(d["wmatd"], torch.float64), t(d["wmatdr"], torch.float64), t(d["wmatdp"], torch.float64)
     st = torch.zeros(6, dtype=torch.float64, device=dev)
     wd = torch.zeros(N, dtype=torch.int32, device=dev)
+    # the inputs come from torch's stream; the entry points queue on the
+    # handle's own stream without waiting for it
+    torch.cuda.synchronize()
     rc = g.lib.crt_dmisft_dev(g.h, N, dat.data_ptr(), sig.data_ptr(), wm.data_ptr(), wr.data_ptr(), wp.data_ptr(),
                               None, None, flags, 1.0, st.data_ptr(), wd.data_ptr())
     assert rc == 0
@@ -118,6 +121,7 @@ def test_device_variants_equal_host_variants(handle, crtmod):
     dpar = t(dp_h, torch.complex128)
     out = torch.zeros(2, dtype=torch.float64, device=dev)
     sg2 = torch.zeros_like(sg)
+    torch.cuda.synchronize()
     assert g.lib.crt_bpar_dev(g.h, sg.data_ptr(), par.data_ptr()) == 0
     assert g.lib.crt_update_step_dev(g.h, M, par.data_ptr(), dpar.data_ptr(), 0.5, 1, out.data_ptr()) == 0
     assert g.lib.crt_bsigma_dev(g.h, par.data_ptr(), sg2.data_ptr()) == 0

@@ -38,6 +38,9 @@ def _run(lib, g, fn, N, arrays, flags, l1min):
     twp = torch.tensor(wp, dtype=torch.float64, device=dev)
     st = torch.zeros(6, dtype=torch.float64, device=dev)
     wd = torch.zeros(N, dtype=torch.int32, device=dev)
+    # the inputs come from torch's stream; the entry point queues on the
+    # handle's own stream without waiting for it
+    torch.cuda.synchronize()
     rc = fn(g.h, N, td.data_ptr(), ts.data_ptr(), tw.data_ptr(), twr.data_ptr(), twp.data_ptr(), None, None,
             flags, l1min, st.data_ptr(), wd.data_ptr())
     assert rc == 0, rc

@@ -1727,10 +1727,17 @@ static int run_adjoint(crt_handle* h, int32_t mode, const double* w_dev, const d
                      (const int*)dql, (const double2*)gamma, C);
   tstop(h);
   tstart(h, "adj_gemm");
-  if (E <= 16 * kAdjTilesHost)
-    hipLaunchKernelGGL(crt_adj_gemm_mfma_kernel, dim3((n + 63) / 64, K), dim3(256), 0, h->stream,
+  if (E <= 16 * kAdjTilesHost) {
+    // C stationary in the wavefronts' registers, 16-node tiles of the
+    // potentials through LDS; about four workgroups per CU over all k
+    const int ns = E <= 64 ? 16 : (E <= 96 ? 24 : 32);
+    void (*kern)(const double2*, const double2*, int, int, double2*) =
+        ns == 16 ? crt_adj_gemm_cs_kernel<16> : (ns == 24 ? crt_adj_gemm_cs_kernel<24> : crt_adj_gemm_cs_kernel<32>);
+    const int ntiles = (n + 15) / 16;
+    const int grid = std::max(1, std::min(ntiles, (1024 + K - 1) / K));
+    hipLaunchKernelGGL(kern, dim3(grid, K), dim3(64 * ((E + 15) / 16)), (size_t)2 * 64 * ns * 16, h->stream,
                        (const double2*)h->at<double2>("kpot"), (const double2*)C, n, E, Y);
-  else
+  } else
     hipLaunchKernelGGL(crt_adj_gemm_kernel, dim3((n + 255) / 256, (E + kAdjColsHost - 1) / kAdjColsHost, K),
                        dim3(256), lds, h->stream, (const double2*)h->at<double2>("kpot"), (const double2*)C, n, E, Y);
   tstop(h);
@@ -1753,8 +1760,9 @@ static int run_adjoint(crt_handle* h, int32_t mode, const double* w_dev, const d
   A.sigma = h->at<double2>("sigma");
   A.grad = g_dev;
   tstart(h, "adj_elem");
-  hipLaunchKernelGGL(crt_adj_elem_kernel, dim3((unsigned)(((size_t)h->elanz * 64 + 255) / 256)), dim3(256), 0,
-                     h->stream, A);
+  // 8 elements per block, the grid a multiple of 8 blocks (XCD-major mapping)
+  const unsigned eblocks = (unsigned)(8 * (((h->elanz + 7) / 8 + 7) / 8));
+  hipLaunchKernelGGL(crt_adj_elem_kernel, dim3(eblocks), dim3(256), 0, h->stream, A);
   tstop(h);
   return finish(h);
 }

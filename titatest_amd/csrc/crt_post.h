@@ -112,3 +112,7 @@ __global__ void crt_cg_scalar_kernel(CgArgs G, int stage, int iter);
 constexpr int kAdjColsHost = 32;
 constexpr int kAdjTilesHost = 8;  // electrodes up to 128 take the matrix-core GEMM
 __global__ void crt_adj_gemm_mfma_kernel(const double2* kpot, const double2* C, int n, int eanz, double2* Y);
+// C-stationary form: NS 4-electrode steps (eanz <= 4 NS), 64 ceil(eanz / 16)
+// threads, dynamic LDS 2 x 64 NS complex values
+template <int NS>
+__global__ void crt_adj_gemm_cs_kernel(const double2* kpot, const double2* C, int n, int eanz, double2* Y);

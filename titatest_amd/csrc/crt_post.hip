@@ -417,7 +417,8 @@ __global__ __launch_bounds__(256) void crt_sens_kernel(SensArgs S) {
 // potential pair are formed once in LDS (the same operations, in the same
 // order, as the per-measurement form of crt_sens_kernel); each measurement then
 // takes h_k = sum_a d34_a u_a from two LDS rows.  The measurements are taken in
-// (current pair, potential pair) order and written to their own rows.  LDS:
+// (current pair, potential pair) order, a contiguous run per thread, and
+// written to their own rows.  LDS:
 // (4 E + 4 ncp + 4 npp) complex values.
 __global__ __launch_bounds__(512) void crt_sens_pairs_kernel(SensArgs S, SensPairs Q) {
   extern __shared__ double2 sm[];
@@ -433,12 +434,18 @@ __global__ __launch_bounds__(512) void crt_sens_pairs_kernel(SensArgs S, SensPai
   const cplx sig = ldc(S.sigma + e);
   auto pick = [&](int m, int el) { return csel(el > 0, P[m * E + max(el - 1, 0)], mkc(0.0, 0.0)); };
   for (int i0 = 0; i0 < S.nanz; i0 += kSensPairQ * blockDim.x) {
+    // this pass's measurements in (current pair, potential pair) order, a
+    // contiguous run of rp per thread: the run's measurements mostly share
+    // their current pair, whose u is read once per change
+    const int npass = min(S.nanz - i0, kSensPairQ * (int)blockDim.x);
+    const int rp = (npass + (int)blockDim.x - 1) / (int)blockDim.x;
+    const int jb = i0 + (int)threadIdx.x * rp, je = i0 + npass;
     cplx acc[kSensPairQ];
     int ci[kSensPairQ], pi[kSensPairQ];
 #pragma unroll
     for (int q = 0; q < kSensPairQ; ++q) {
       acc[q] = mkc(0.0, 0.0);
-      const int i = Q.ord[min(i0 + (int)threadIdx.x + q * (int)blockDim.x, S.nanz - 1)];
+      const int i = Q.ord[min(jb + q, S.nanz - 1)];
       const int2 m = Q.mp[i];
       ci[q] = m.x;
       pi[q] = m.y;
@@ -497,18 +504,27 @@ __global__ __launch_bounds__(512) void crt_sens_pairs_kernel(SensArgs S, SensPai
       }
       __syncthreads();
       const double wk = S.kwnwi[k];
+      int cur = -1;
+      cplx u[4];
 #pragma unroll
       for (int q = 0; q < kSensPairQ; ++q) {
-        cplx h = mkc(0.0, 0.0);
+        if (q < rp && jb + q < je) {
+          if (ci[q] != cur) {
+            cur = ci[q];
 #pragma unroll
-        for (int a = 0; a < 4; ++a) h = cadd(h, cmul(D[a * npp + pi[q]], U[a * ncp + ci[q]]));
-        acc[q] = S.swrtr == 0 ? h : cadd(acc[q], rmul(wk, h));
+            for (int a = 0; a < 4; ++a) u[a] = U[a * ncp + cur];
+          }
+          cplx h = mkc(0.0, 0.0);
+#pragma unroll
+          for (int a = 0; a < 4; ++a) h = cadd(h, cmul(D[a * npp + pi[q]], u[a]));
+          acc[q] = S.swrtr == 0 ? h : cadd(acc[q], rmul(wk, h));
+        }
       }
     }
 #pragma unroll
     for (int q = 0; q < kSensPairQ; ++q) {
-      const int j = i0 + threadIdx.x + q * blockDim.x;
-      if (j >= S.nanz) continue;
+      const int j = jb + q;
+      if (q >= rp || j >= je) continue;
       const int i = Q.ord[j];
       cplx out;
       if (S.mode == 0) {
@@ -629,6 +645,83 @@ __global__ __launch_bounds__(256) void crt_adj_gemm_mfma_kernel(const double2* _
   }
 }
 
+// The same product with C stationary: wavefront w of the workgroup keeps the
+// 16 columns [16 w, 16 w + 16) of C in registers for all NS 4-electrode steps
+// (operand B), the workgroup stages 16-node tiles of P[k] (every electrode)
+// in LDS, double buffered, and each wavefront forms its 16 x 16 block of Y
+// per node tile (operand A from LDS, four real products per complex step).
+// C is read once per workgroup and P once per column tile set; the next
+// tile's global loads are in flight during the products.
+template <int NS>
+__global__ __launch_bounds__(512) void crt_adj_gemm_cs_kernel(const double2* __restrict__ kpot,
+                                                            const double2* __restrict__ C, int n, int eanz,
+                                                            double2* __restrict__ Y) {
+  extern __shared__ double2 ptile[];  // 2 x [4 NS electrodes][16 nodes]
+  constexpr int NA = 4 * NS, TILE = NA * 16, MAXSTG = 8;
+  const int k = blockIdx.y;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int col = lane & 15, kk = lane >> 4;
+  const int b = 16 * w + col;
+  double cr[NS], ci[NS];
+#pragma unroll
+  for (int s = 0; s < NS; ++s) {
+    const int a = 4 * s + kk;
+    const bool ok = a < eanz && b < eanz;
+    const double2 c = C[(size_t)min(a, eanz - 1) * eanz + min(b, eanz - 1)];
+    cr[s] = ok ? c.x : 0.0;
+    ci[s] = ok ? c.y : 0.0;
+  }
+  const double2* Pk = kpot + (size_t)k * eanz * n;
+  const int ntiles = (n + 15) / 16;
+  double2 st[MAXSTG];
+  auto fetch = [&](int t) __attribute__((always_inline)) {
+    const int j0 = 16 * t;
+#pragma unroll
+    for (int r = 0; r < MAXSTG; ++r) {
+      const int x = threadIdx.x + r * blockDim.x, a = x >> 4, j = j0 + (x & 15);
+      st[r] = (x < TILE && a < eanz && j < n) ? Pk[(size_t)a * n + j] : make_double2(0.0, 0.0);
+    }
+  };
+  auto put = [&](double2* buf) __attribute__((always_inline)) {
+#pragma unroll
+    for (int r = 0; r < MAXSTG; ++r) {
+      const int x = threadIdx.x + r * blockDim.x;
+      if (x < TILE) buf[x] = st[r];
+    }
+  };
+  int t = blockIdx.x;
+  if (t < ntiles) {
+    fetch(t);
+    put(ptile);
+  }
+  __syncthreads();
+  for (int it = 0; t < ntiles; t += gridDim.x, ++it) {
+    const double2* cur = ptile + (it & 1) * TILE;
+    double2* nxt = ptile + ((it + 1) & 1) * TILE;
+    const int tn = t + gridDim.x;
+    if (tn < ntiles) fetch(tn);
+    v4d dr = v4d{0.0, 0.0, 0.0, 0.0}, di = v4d{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int s = 0; s < NS; ++s) {
+      const double2 p = cur[(4 * s + kk) * 16 + col];
+      dr = __builtin_amdgcn_mfma_f64_16x16x4f64(p.x, cr[s], dr, 0, 0, 0);
+      dr = __builtin_amdgcn_mfma_f64_16x16x4f64(p.y, -ci[s], dr, 0, 0, 0);
+      di = __builtin_amdgcn_mfma_f64_16x16x4f64(p.x, ci[s], di, 0, 0, 0);
+      di = __builtin_amdgcn_mfma_f64_16x16x4f64(p.y, cr[s], di, 0, 0, 0);
+    }
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int j = 16 * t + kk + 4 * v;
+      if (j < n && b < eanz) Y[((size_t)k * n + j) * eanz + b] = make_double2(dr[v], di[v]);
+    }
+    if (tn < ntiles) put(nxt);
+    __syncthreads();
+  }
+}
+template __global__ void crt_adj_gemm_cs_kernel<16>(const double2*, const double2*, int, int, double2*);
+template __global__ void crt_adj_gemm_cs_kernel<24>(const double2*, const double2*, int, int, double2*);
+template __global__ void crt_adj_gemm_cs_kernel<32>(const double2*, const double2*, int, int, double2*);
+
 __global__ __launch_bounds__(256) void crt_adj_gemm_kernel(const double2* __restrict__ kpot,
                                                          const double2* __restrict__ C, int n, int eanz,
                                                          double2* __restrict__ Y) {
@@ -660,14 +753,20 @@ __global__ __launch_bounds__(256) void crt_adj_gemm_kernel(const double2* __rest
   for (int bb = 0; bb < nb; ++bb) out[bb] = make_double2(ar[bb], ai[bb]);
 }
 
-// Element contraction: one wavefront per element, lanes over source
-// electrodes b; each lane accumulates sum_k w_k sum_{mi,mj} S Y P for its b,
-// then a wave reduction.
+// Element contraction: half a wavefront (32 lanes) per element, lanes over
+// source electrodes b; each lane accumulates sum_k w_k sum_{mi,mj} S Y P for
+// its b, then a reduction over the 32 lanes.  Blocks take elements
+// XCD-major: the blocks the dispatcher places on one XCD (block index mod 8)
+// take one contiguous element range, so neighbouring elements, which share
+// nodes, read them through the same L2.  The grid is a multiple of 8 blocks.
+constexpr int kAdjElemPerBlock = 8;
 __global__ __launch_bounds__(256) void crt_adj_elem_kernel(AdjArgs A) {
-  const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-  const int lane = threadIdx.x & 63;
-  if (wave >= A.elanz) return;
-  const int e = wave;
+  constexpr int kXcd = 8;
+  const int per = gridDim.x / kXcd;
+  const int blk = (blockIdx.x % kXcd) * per + blockIdx.x / kXcd;
+  const int hl = threadIdx.x & 31;
+  const int e0 = blk * kAdjElemPerBlock + (threadIdx.x >> 5);
+  const int e = min(e0, A.elanz - 1);
   const int nk = A.el_nk[e];
   int nodes[4];
 #pragma unroll
@@ -689,7 +788,7 @@ __global__ __launch_bounds__(256) void crt_adj_elem_kernel(AdjArgs A) {
     // reads a contiguous run of electrodes
     const double2* Yk = A.Y + (size_t)k * A.n * E;
     const double2* Pk = A.kpt + (size_t)k * A.n * E;
-    for (int b = lane; b < E; b += 64) {
+    for (int b = hl; b < E; b += 32) {
       cplx yv[4], pv[4];
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
@@ -709,11 +808,11 @@ __global__ __launch_bounds__(256) void crt_adj_elem_kernel(AdjArgs A) {
       acci += wk * t.im;
     }
   }
-  for (int off = 32; off > 0; off >>= 1) {
-    accr += __shfl_xor(accr, off, 64);
-    acci += __shfl_xor(acci, off, 64);
+  for (int off = 16; off > 0; off >>= 1) {
+    accr += __shfl_xor(accr, off, 32);
+    acci += __shfl_xor(acci, off, 32);
   }
-  if (lane == 0) {
+  if (hl == 0 && e0 < A.elanz) {
     cplx s = mkc(accr, acci);
     if (A.swrtr != 0) s = rmul(1.0 / M_PI, s);
     const cplx beta = (A.mode == 0) ? mkc(1.0, 0.0) : ldc(A.sigma + e);
