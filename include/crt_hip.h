@@ -174,6 +174,21 @@ int crt_set_device_budget(crt_handle *h, uint64_t bytes);
  * (titatest_amd/autotune.py) times them.  No counterpart in the reference. */
 int crt_set_tuning(crt_handle *h, int32_t factor_threads, int32_t backsolve_depth);
 
+/* Per-wavenumber recomputation (checkpointing) of the potentials for
+ * problems whose K x eanz x sanz potentials should not stay in device memory.
+ * wavenumbers_per_pass 0 (the default) stores every wavenumber's potentials.
+ * k > 0: a forward solves the systems in passes of k wavenumbers and keeps,
+ * from each pass, only its contribution to the electrode rows of the
+ * back-transform (the voltages are the stored mode's bit for bit); the
+ * adjoint gradient (crt_adjoint_gradient[_dev]) re-solves each pass's
+ * systems and adds its contribution, so at most k wavenumbers' potentials are
+ * held at once.  In this mode crt_forward and crt_forward_dc take no kpot or
+ * hpot output (1000), crt_sensitivities, crt_sensitivities_dc and
+ * crt_kpot_device need the stored potentials (1002), and crt_forward_many
+ * does not take the handle (1000).  The reference stores kpot(sanz, eanz,
+ * kwnanz) (src/femmod.f90); no counterpart. */
+int crt_set_recompute(crt_handle *h, int32_t wavenumbers_per_pass);
+
 /* The forward of several models in one pass (the reference runs one model per
  * program run; this is the loop of src/fem.f90:328-397 over many handles).
  * hs: nmodels distinct handles on one device, after crt_precal; sigma_dev:

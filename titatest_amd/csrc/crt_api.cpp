@@ -73,6 +73,11 @@ struct crt_handle {
   bool precal_done = false, forward_done = false, volt_done = false, sens_done = false;
   bool dc = false;  // the last forward was the DC twin (crt_forward_dc)
   bool kpt_valid = false;  // "kpt" holds the last forward's kpot transposed
+  // wavenumbers per pass of the recomputing mode (crt_set_recompute; 0 =
+  // every wavenumber's potentials stored), and whether the last forward
+  // stored them all
+  int recompute = 0;
+  bool kpot_full = false;
   // CG iterations as one captured HIP graph (crt_set_graph), with the kernel
   // arguments and iteration count it was captured for
   bool use_graph = false;
@@ -986,7 +991,12 @@ static void factor_args(crt_handle* h, FactorArgs& A) {
   A.sys_stride = 1;
 }
 
-static int run_forward(crt_handle* h) {
+// The (k, l) systems of wavenumbers k0 .. k0 + kc - 1: assembly, scaling,
+// factorisation and substitutions, kpot of those wavenumbers into kpot_dst
+// ([kc][E][n]).  `area` forms the per-wavenumber area sums of every
+// wavenumber first (once per conductivity model; a recomputing pass reuses
+// them).  Queued on the handle's stream.
+static int run_forward_range(crt_handle* h, int k0, int kc, double2* kpot_dst, bool area) {
   const int n = h->n, mb = h->mb, E = h->E, K = h->K;
   FactorForm form;
   if (int rc = choose_factor(h, form)) return rc;
@@ -997,33 +1007,34 @@ static int run_forward(crt_handle* h) {
   if (int rc = open_lds(h, reinterpret_cast<const void*>(kern))) return rc;
   int per_cu = 0;
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, nt, lds) != hipSuccess || per_cu < 1) per_cu = 1;
-  const int nsys = K * E;
+  const int nsys = kc * E;
   FactorArgs A;
   factor_args(h, A);
   double2* av = h->get<double2>("av", (size_t)K * n * h->R);
-  double2* kpot = h->get<double2>("kpot", (size_t)K * E * n);
+  double2* kpot = kpot_dst;
   NEED(av);
   NEED(kpot);
   A.av = av;
-  if (clear_err(h)) return kErrHip;
   A.err = h->at<int>("err");
   A.prof = nullptr;
   if (h->kprof) {
-    A.prof = h->get<unsigned long long>("kprof", 8);
+    A.prof = h->get<unsigned long long>("kprof", 8);  // zeroed by run_forward
     NEED(A.prof);
-    HIPCHK(hipMemsetAsync(A.prof, 0, 8 * sizeof(unsigned long long), h->stream));
   }
   // per-wavenumber area sums
-  tstart(h, "area_values");
-  hipLaunchKernelGGL(crt_area_values_kernel, dim3((unsigned)(((size_t)K * n + 255) / 256)), dim3(256), 0, h->stream,
-                     A, K, av);
-  tstop(h);
+  if (area) {
+    tstart(h, "area_values");
+    hipLaunchKernelGGL(crt_area_values_kernel, dim3((unsigned)(((size_t)K * n + 255) / 256)), dim3(256), 0,
+                       h->stream, A, K, av);
+    tstop(h);
+  }
   // Without mixed boundaries the matrix does not depend on the source: one
   // factor per wavenumber serves all eanz sources (src/fem.f90:372-376,
   // kompb), each source then takes both sweeps.  Otherwise every (k, l) system
   // is factored with its own forward sweep fused in.
   const bool reuse = !h->has11;
-  const int nfac = reuse ? K : nsys;
+  const int nfac = reuse ? kc : nsys;
+  const int fac0 = reuse ? k0 : k0 * E;  // first factor (system, or wavenumber when shared)
   const int rpf = reuse ? E : 1;  // right-hand sides per factor
   A.sys_stride = reuse ? E : 1;
   // factors per batch: the scaled rows, L, y and the scaling factors of every
@@ -1067,6 +1078,7 @@ static int run_forward(crt_handle* h) {
   B.y = reuse ? yr : A.y;
   B.fak = A.fak;
   B.kpot = kpot;
+  B.kpot_k0 = k0;
   FwdArgs F;
   F.n = n;
   F.mb = mb;
@@ -1094,8 +1106,8 @@ static int run_forward(crt_handle* h) {
     NEED(A.rhs);
   }
   F.rhs = A.rhs;
-  for (int first = 0; first < nfac; first += batch) {
-    const int cnt = std::min(batch, nfac - first);
+  for (int first = fac0; first < fac0 + nfac; first += batch) {
+    const int cnt = std::min(batch, fac0 + nfac - first);
     A.sys_first = first;
     A.nsys = cnt;
     const unsigned rgrid = (unsigned)(((size_t)cnt * n + 255) / 256);
@@ -1181,24 +1193,66 @@ static int run_forward(crt_handle* h) {
     launch_backsolve(mb, nrhs, h->stream, B, h->tune_depth, h->dc);
     tstop(h);
   }
-  // electrode slice of the back-transform, used by the voltage stage
+  return 0;
+}
+
+// The electrode slice of the back-transform for the wavenumbers in kpot
+// (k0 .. k0 + kc - 1), carried between passes in "he_acc".
+static int rtrafo_elec(crt_handle* h, const double2* kpot, int k0, int kc) {
+  const int E = h->E;
   double2* he = h->get<double2>("he", (size_t)E * E);
+  double2* acc = h->get<double2>("he_acc", (size_t)E * E);
   NEED(he);
+  NEED(acc);
   tstart(h, "rtrafo_elec");
-  hipLaunchKernelGGL(crt_rtrafo_elec_kernel, dim3((E * E + 255) / 256), dim3(256), 0, h->stream,
-                     (const double2*)kpot, (const double*)h->at<double>("kwnwi"), (const int*)A.enr, n, E, K,
-                     h->swrtr, he);
+  hipLaunchKernelGGL(crt_rtrafo_elec_kernel, dim3((E * E + 255) / 256), dim3(256), 0, h->stream, kpot,
+                     (const double*)h->at<double>("kwnwi"), (const int*)h->at<int>("enr"), h->n, E, k0, kc,
+                     h->swrtr, k0 == 0 ? 1 : 0, k0 + kc == h->K ? 1 : 0, acc, he);
   tstop(h);
+  return 0;
+}
+
+// Wavenumbers per pass of the recomputing mode (crt_set_recompute), 0 when
+// every wavenumber's potentials are stored.
+static int recompute_pass(const crt_handle* h) { return h->recompute > 0 ? std::min(h->recompute, h->K) : 0; }
+
+// The forward of every (k, l) system: kpot of all wavenumbers stored, or, in
+// the recomputing mode, passes of recompute_pass() wavenumbers of which only
+// the electrode slice of the back-transform is kept.
+static int run_forward(crt_handle* h) {
+  const int n = h->n, E = h->E, K = h->K;
+  if (clear_err(h)) return kErrHip;
+  const int kp = recompute_pass(h);
+  if (h->kprof) {
+    unsigned long long* prof = h->get<unsigned long long>("kprof", 8);
+    NEED(prof);
+    HIPCHK(hipMemsetAsync(prof, 0, 8 * sizeof(unsigned long long), h->stream));
+  }
+  if (!kp) {
+    double2* kpot = h->get<double2>("kpot", (size_t)K * E * n);
+    NEED(kpot);
+    if (int rc = run_forward_range(h, 0, K, kpot, true)) return rc;
+    if (int rc = rtrafo_elec(h, kpot, 0, K)) return rc;
+  } else {
+    double2* kpot = h->get<double2>("kpot_pass", (size_t)kp * E * n);
+    NEED(kpot);
+    for (int k0 = 0; k0 < K; k0 += kp) {
+      const int kc = std::min(kp, K - k0);
+      if (int rc = run_forward_range(h, k0, kc, kpot, k0 == 0)) return rc;
+      if (int rc = rtrafo_elec(h, kpot, k0, kc)) return rc;
+    }
+  }
   int rc = finish(h);
   if (rc) return rc;
   int e = read_err(h);
   if (e) return e;
   if (h->kprof) {
     unsigned long long t[8];
-    HIPCHK(d2h(h, t, A.prof, sizeof(t)));
+    HIPCHK(d2h(h, t, h->at<unsigned long long>("kprof"), sizeof(t)));
     for (int i = 0; i < 8; ++i) h->kprof_ticks[i] += t[i];
   }
   h->forward_done = true;
+  h->kpot_full = kp == 0;
   h->volt_done = h->sens_done = false;
   h->kpt_valid = false;
   return 0;
@@ -1221,6 +1275,7 @@ int crt_forward_many(int32_t nmodels, crt_handle* const* hs, const double* const
     for (int j = 0; j < m; ++j)
       if (hs[j] == hs[m]) return kErrArg;
     if (!hs[m]->precal_done) return kErrOrder;
+    if (hs[m]->recompute) return kErrArg;  // the batched forward stores every model's potentials
   }
   crt_handle* h0 = hs[0];
   hipStream_t st = h0->stream;
@@ -1375,6 +1430,7 @@ int crt_forward_many(int32_t nmodels, crt_handle* const* hs, const double* const
       B.y = M.A.y;
       B.fak = M.A.fak;
       B.kpot = M.h->at<double2>("kpot");
+      B.kpot_k0 = 0;
       bargs.push_back(B);
       for (int r = 0; r < B.nrhs; ++r) items.push_back(make_int2(j, r));
     }
@@ -1397,13 +1453,14 @@ int crt_forward_many(int32_t nmodels, crt_handle* const* hs, const double* const
     NEED(he);
     hipLaunchKernelGGL(crt_rtrafo_elec_kernel, dim3((hm->E * hm->E + 255) / 256), dim3(256), 0, st,
                        (const double2*)hm->at<double2>("kpot"), (const double*)hm->at<double>("kwnwi"),
-                       (const int*)M.A.enr, hm->n, hm->E, hm->K, hm->swrtr, he);
+                       (const int*)M.A.enr, hm->n, hm->E, 0, hm->K, hm->swrtr, 1, 1, he, he);
   }
   if (int rc = finish(h0)) return rc;
   for (const Model& M : ms) {
     const int e = read_err(M.h);
     if (e) return e;
     M.h->forward_done = true;
+    M.h->kpot_full = true;
     M.h->volt_done = M.h->sens_done = false;
     M.h->kpt_valid = false;
   }
@@ -1419,9 +1476,16 @@ int crt_set_tuning(crt_handle* h, int32_t factor_threads, int32_t backsolve_dept
   return 0;
 }
 
+int crt_set_recompute(crt_handle* h, int32_t wavenumbers_per_pass) {
+  if (!h || wavenumbers_per_pass < 0) return kErrArg;
+  h->recompute = wavenumbers_per_pass;
+  return 0;
+}
+
 int crt_forward(crt_handle* h, const double* sigma, double* kpot, double* hpot) {
   if (!h || !sigma) return kErrArg;
   if (!h->precal_done) return kErrOrder;
+  if (h->recompute && (kpot || hpot)) return kErrArg;  // the potentials are not stored
   double2* ds;
   if (upload(h, "sigma", reinterpret_cast<const double2*>(sigma), h->elanz, &ds)) return kErrHip;
   h->dc = false;
@@ -1469,6 +1533,7 @@ static int copy_real(crt_handle* h, const double2* src, size_t n, double* host) 
 int crt_forward_dc(crt_handle* h, const double* sigma, double* kpotdc, double* hpotdc) {
   if (!h || !sigma) return kErrArg;
   if (!h->precal_done) return kErrOrder;
+  if (h->recompute && (kpotdc || hpotdc)) return kErrArg;
   // the caller's sigma stays for bsendc's sigma/V; the assembly takes its real part
   double2* din;
   if (upload(h, "sigma_dc", reinterpret_cast<const double2*>(sigma), h->elanz, &din)) return kErrHip;
@@ -1571,6 +1636,7 @@ static double2* transposed_kpot(crt_handle* h) {
 }
 
 static int run_sens(crt_handle* h, int32_t mode, const double2* sigma) {
+  if (!h->kpot_full) return kErrOrder;  // the explicit Jacobian reads every wavenumber's potentials
   SensArgs S;
   S.n = h->n;
   S.elanz = h->elanz;
@@ -1693,7 +1759,9 @@ static int run_adjoint(crt_handle* h, int32_t mode, const double* w_dev, const d
   }
   double2* gamma = h->get<double2>("adj_gamma", N);
   double2* C = h->get<double2>("adj_C", (size_t)E * E);
-  double2* Y = h->get<double2>("adj_Y", (size_t)K * n * E);
+  // potentials of every wavenumber, or of one pass of the recomputing mode
+  const int kp = h->kpot_full ? K : std::min(std::max(h->recompute, 1), K);
+  double2* Y = h->get<double2>("adj_Y", (size_t)kp * n * E);
   NEED(gamma);
   NEED(C);
   NEED(Y);
@@ -1726,21 +1794,6 @@ static int run_adjoint(crt_handle* h, int32_t mode, const double* w_dev, const d
                      (const int*)h->at<int>("strnr"), (const int*)h->at<int>("vnr"), (const int*)dqp,
                      (const int*)dql, (const double2*)gamma, C);
   tstop(h);
-  tstart(h, "adj_gemm");
-  if (E <= 16 * kAdjTilesHost) {
-    // C stationary in the wavefronts' registers, 16-node tiles of the
-    // potentials through LDS; about four workgroups per CU over all k
-    const int ns = E <= 64 ? 16 : (E <= 96 ? 24 : 32);
-    void (*kern)(const double2*, const double2*, int, int, double2*) =
-        ns == 16 ? crt_adj_gemm_cs_kernel<16> : (ns == 24 ? crt_adj_gemm_cs_kernel<24> : crt_adj_gemm_cs_kernel<32>);
-    const int ntiles = (n + 15) / 16;
-    const int grid = std::max(1, std::min(ntiles, (1024 + K - 1) / K));
-    hipLaunchKernelGGL(kern, dim3(grid, K), dim3(64 * ((E + 15) / 16)), (size_t)2 * 64 * ns * 16, h->stream,
-                       (const double2*)h->at<double2>("kpot"), (const double2*)C, n, E, Y);
-  } else
-    hipLaunchKernelGGL(crt_adj_gemm_kernel, dim3((n + 255) / 256, (E + kAdjColsHost - 1) / kAdjColsHost, K),
-                       dim3(256), lds, h->stream, (const double2*)h->at<double2>("kpot"), (const double2*)C, n, E, Y);
-  tstop(h);
   AdjArgs A;
   A.n = n;
   A.elanz = h->elanz;
@@ -1753,18 +1806,65 @@ static int run_adjoint(crt_handle* h, int32_t mode, const double* w_dev, const d
   A.el_nodes = h->at<int>("el_nodes");
   A.elbg = h->at<double>("elbg");
   A.kwnwi = h->at<double>("kwnwi");
-  A.kpot = h->at<double2>("kpot");
-  A.kpt = transposed_kpot(h);
-  NEED(A.kpt);
   A.Y = Y;
   A.sigma = h->at<double2>("sigma");
   A.grad = g_dev;
-  tstart(h, "adj_elem");
-  // 8 elements per block, the grid a multiple of 8 blocks (XCD-major mapping)
-  const unsigned eblocks = (unsigned)(8 * (((h->elanz + 7) / 8 + 7) / 8));
-  hipLaunchKernelGGL(crt_adj_elem_kernel, dim3(eblocks), dim3(256), 0, h->stream, A);
-  tstop(h);
-  return finish(h);
+  A.acc = h->get<double2>("adj_acc", h->elanz);
+  NEED(A.acc);
+  // Y = P^T C and the element contraction, over all wavenumbers at once from
+  // the stored potentials, or pass by pass with the potentials of each pass
+  // recomputed (the same systems as the forward: assembly from the stored
+  // area sums, factorisation, substitutions)
+  for (int k0 = 0; k0 < K; k0 += kp) {
+    const int kc = std::min(kp, K - k0);
+    const double2* P;
+    if (h->kpot_full) {
+      P = h->at<double2>("kpot");
+      A.kpt = transposed_kpot(h);
+    } else {
+      double2* kpass = h->get<double2>("kpot_pass", (size_t)kp * E * n);
+      double2* kpt = h->get<double2>("kpt_pass", (size_t)kp * n * E);
+      NEED(kpass);
+      NEED(kpt);
+      if (int rc = run_forward_range(h, k0, kc, kpass, false)) return rc;
+      tstart(h, "kpot_transpose");
+      hipLaunchKernelGGL(crt_kpot_transpose_kernel, dim3((n + 63) / 64, (E + 15) / 16, kc), dim3(256), 0,
+                         h->stream, (const double2*)kpass, n, E, kpt);
+      tstop(h);
+      P = kpass;
+      A.kpt = kpt;
+    }
+    NEED(A.kpt);
+    A.kpot = P;
+    tstart(h, "adj_gemm");
+    if (E <= 16 * kAdjTilesHost) {
+      // C stationary in the wavefronts' registers, 16-node tiles of the
+      // potentials through LDS; about four workgroups per CU over all k
+      const int ns = E <= 64 ? 16 : (E <= 96 ? 24 : 32);
+      void (*kern)(const double2*, const double2*, int, int, double2*) =
+          ns == 16 ? crt_adj_gemm_cs_kernel<16> : (ns == 24 ? crt_adj_gemm_cs_kernel<24> : crt_adj_gemm_cs_kernel<32>);
+      const int ntiles = (n + 15) / 16;
+      const int grid = std::max(1, std::min(ntiles, (1024 + kc - 1) / kc));
+      hipLaunchKernelGGL(kern, dim3(grid, kc), dim3(64 * ((E + 15) / 16)), (size_t)2 * 64 * ns * 16, h->stream, P,
+                         (const double2*)C, n, E, Y);
+    } else {
+      hipLaunchKernelGGL(crt_adj_gemm_kernel, dim3((n + 255) / 256, (E + kAdjColsHost - 1) / kAdjColsHost, kc),
+                         dim3(256), lds, h->stream, P, (const double2*)C, n, E, Y);
+    }
+    tstop(h);
+    A.k0 = k0;
+    A.kc = kc;
+    A.first = k0 == 0 ? 1 : 0;
+    A.last = k0 + kc == K ? 1 : 0;
+    tstart(h, "adj_elem");
+    // 8 elements per block, the grid a multiple of 8 blocks (XCD-major mapping)
+    const unsigned eblocks = (unsigned)(8 * (((h->elanz + 7) / 8 + 7) / 8));
+    hipLaunchKernelGGL(crt_adj_elem_kernel, dim3(eblocks), dim3(256), 0, h->stream, A);
+    tstop(h);
+  }
+  int rc = finish(h);
+  if (rc) return rc;
+  return h->kpot_full ? 0 : read_err(h);
 }
 
 int crt_adjoint_gradient(crt_handle* h, int32_t mode, const double* w, const double* resid, double* grad) {
@@ -2131,7 +2231,7 @@ int crt_factor_phase_ticks(crt_handle* h, int32_t enable, uint64_t* ticks8) {
 
 int crt_kpot_device(crt_handle* h, const double** kpot_dev) {
   if (!h || !kpot_dev) return kErrArg;
-  if (!h->forward_done) return kErrOrder;
+  if (!h->forward_done || !h->kpot_full) return kErrOrder;
   *kpot_dev = reinterpret_cast<const double*>(h->at<double2>("kpot"));
   return 0;
 }

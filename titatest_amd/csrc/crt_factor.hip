@@ -846,7 +846,7 @@ __device__ __forceinline__ void backsolve_rhs(const BacksolveArgs& B, int r) {
   const brsrc Lr = make_rsrc(reinterpret_cast<const E*>(B.L) + (size_t)f * n * m1, (unsigned)(n * m1 * ES));
   const E* yg = reinterpret_cast<const E*>(B.y) + (size_t)r * n;
   const double* fak = B.fak + (size_t)f * n;
-  double2* out = B.kpot + ((size_t)k * B.eanz + l) * n;
+  double2* out = B.kpot + ((size_t)(k - B.kpot_k0) * B.eanz + l) * n;
 
   // Entries of row t of L for every slot: slot s's row is j = t - rf, rf =
   // (t mod 64 - lane + 64 s) mod WIN, at band offset mb - rf, in range for 1

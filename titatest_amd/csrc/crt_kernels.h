@@ -71,7 +71,8 @@ struct BacksolveArgs {
   const double2* L;         // [factor][n][mb+1]
   const double2* y;         // [rhs][n]
   const double* fak;        // [factor][n]
-  double2* kpot;            // [K][eanz][n]
+  double2* kpot;            // [K'][eanz][n]: wavenumbers kpot_k0 .. kpot_k0 + K' - 1
+  int kpot_k0;
 };
 
 // Forward sweep L y = b for every source of a shared factor (vre :45-56).

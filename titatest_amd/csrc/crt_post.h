@@ -60,6 +60,10 @@ struct AdjArgs {
   const double2* Y;       // [K][n][eanz]
   const double2* sigma;
   double2* grad;          // elanz
+  // wavenumbers k0 .. k0 + kc - 1 in kpt and Y (a pass of the recomputing
+  // gradient; 0 .. K - 1 otherwise); acc carries the sums between passes
+  int k0, kc, first, last;
+  double2* acc;           // elanz
 };
 
 struct CgArgs {
@@ -88,8 +92,8 @@ __global__ void crt_rtrafo_kernel(const double2* kpot, const double* kwnwi, int 
                                   int swrtr, double2* hpot);
 __global__ void crt_real_sigma_kernel(const double2* in, int n, double2* out);
 __global__ void crt_real_part_kernel(const double2* in, size_t n, double* out);
-__global__ void crt_rtrafo_elec_kernel(const double2* kpot, const double* kwnwi, const int* enr, int n,
-                                       int eanz, int K, int swrtr, double2* he);
+__global__ void crt_rtrafo_elec_kernel(const double2* kpot, const double* kwnwi, const int* enr, int n, int eanz,
+                                       int k0, int kc, int swrtr, int first, int last, double2* acc, double2* he);
 __global__ void crt_volt_kernel(const double2* he, int eanz, int nanz, const int* strnr, const int* vnr,
                                 int mode, const double* kfak, double2* volt, double2* sigmaa, int* err);
 __global__ void crt_sens_kernel(SensArgs S);

@@ -231,19 +231,24 @@ __global__ void crt_real_part_kernel(const double2* __restrict__ in, size_t n, d
   if (i < n) out[i] = in[i].x;
 }
 
-// Electrode slice of the back-transform: he[l][m] = hpot(enr(m), l).
+// Electrode slice of the back-transform: he[l][m] = hpot(enr(m), l), the
+// sum over wavenumbers in ascending k.  kpot holds wavenumbers k0 .. k0 + kc
+// - 1; a forward that holds them one pass at a time carries the partial sums
+// in acc between passes (first: start from zero; last: finish into he), with
+// the same operations in the same order as one pass over all K.
 __global__ void crt_rtrafo_elec_kernel(const double2* __restrict__ kpot, const double* __restrict__ kwnwi,
-                                       const int* __restrict__ enr, int n, int eanz, int K, int swrtr,
-                                       double2* __restrict__ he) {
+                                       const int* __restrict__ enr, int n, int eanz, int k0, int kc, int swrtr,
+                                       int first, int last, double2* __restrict__ acc, double2* __restrict__ he) {
   const int idx = blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= eanz * eanz) return;
   const int m = idx % eanz, l = idx / eanz;
   const size_t base = (size_t)l * n + enr[m];
   if (swrtr == 0) { he[idx] = kpot[base]; return; }
-  cplx s = mkc(0.0, 0.0);
+  cplx s = first ? mkc(0.0, 0.0) : ldc(acc + idx);
   const size_t stride = (size_t)n * eanz;
-  for (int k = 0; k < K; ++k) s = cadd(s, rmul(kwnwi[k], ldc(kpot + base + k * stride)));
-  stc(he + idx, cdiv(s, mkc(M_PI, 0.0)));
+  for (int k = 0; k < kc; ++k) s = cadd(s, rmul(kwnwi[k0 + k], ldc(kpot + base + k * stride)));
+  if (last) stc(he + idx, cdiv(s, mkc(M_PI, 0.0)));
+  else stc(acc + idx, s);
 }
 
 // Voltage superposition (src/bvolt.f90:54-63) from the electrode slice;
@@ -646,22 +651,25 @@ __global__ __launch_bounds__(256) void crt_adj_gemm_mfma_kernel(const double2* _
 }
 
 // The same product with C stationary: wavefront w of the workgroup keeps the
-// 16 columns [16 w, 16 w + 16) of C in registers for all NS 4-electrode steps
-// (operand B), the workgroup stages 16-node tiles of P[k] (every electrode)
-// in LDS, double buffered, and each wavefront forms its 16 x 16 block of Y
-// per node tile (operand A from LDS, four real products per complex step).
-// C is read once per workgroup and P once per column tile set; the next
-// tile's global loads are in flight during the products.
+// 16 columns [16 w, 16 w + 16) of C in registers for the ceil(eanz / 4)
+// 4-electrode steps (operand B), the workgroup stages 16-node tiles of P[k]
+// (every electrode) in LDS, double buffered, and each wavefront forms its
+// 16 x 16 block of Y per node tile (operand A from LDS).  The four real
+// products of a complex step go to four accumulators (Pr Cr, Pi Ci, Pr Ci,
+// Pi Cr; Y = (Pr Cr - Pi Ci) + i (Pr Ci + Pi Cr) at the end), so four
+// independent chains share the matrix core.  C is read once per workgroup;
+// the next tile's global loads are in flight during the products.
 template <int NS>
 __global__ __launch_bounds__(512) void crt_adj_gemm_cs_kernel(const double2* __restrict__ kpot,
                                                             const double2* __restrict__ C, int n, int eanz,
                                                             double2* __restrict__ Y) {
   extern __shared__ double2 ptile[];  // 2 x [4 NS electrodes][16 nodes]
-  constexpr int NA = 4 * NS, TILE = NA * 16, MAXSTG = 8;
+  constexpr int NA = 4 * NS, TILE = NA * 16, MAXSTG = 4;
   const int k = blockIdx.y;
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int col = lane & 15, kk = lane >> 4;
   const int b = 16 * w + col;
+  const int nsr = (eanz + 3) / 4, tile = 64 * nsr;  // steps and staged values in use
   double cr[NS], ci[NS];
 #pragma unroll
   for (int s = 0; s < NS; ++s) {
@@ -679,14 +687,14 @@ __global__ __launch_bounds__(512) void crt_adj_gemm_cs_kernel(const double2* __r
 #pragma unroll
     for (int r = 0; r < MAXSTG; ++r) {
       const int x = threadIdx.x + r * blockDim.x, a = x >> 4, j = j0 + (x & 15);
-      st[r] = (x < TILE && a < eanz && j < n) ? Pk[(size_t)a * n + j] : make_double2(0.0, 0.0);
+      st[r] = (x < tile && a < eanz && j < n) ? Pk[(size_t)a * n + j] : make_double2(0.0, 0.0);
     }
   };
   auto put = [&](double2* buf) __attribute__((always_inline)) {
 #pragma unroll
     for (int r = 0; r < MAXSTG; ++r) {
       const int x = threadIdx.x + r * blockDim.x;
-      if (x < TILE) buf[x] = st[r];
+      if (x < tile) buf[x] = st[r];
     }
   };
   int t = blockIdx.x;
@@ -700,19 +708,21 @@ __global__ __launch_bounds__(512) void crt_adj_gemm_cs_kernel(const double2* __r
     double2* nxt = ptile + ((it + 1) & 1) * TILE;
     const int tn = t + gridDim.x;
     if (tn < ntiles) fetch(tn);
-    v4d dr = v4d{0.0, 0.0, 0.0, 0.0}, di = v4d{0.0, 0.0, 0.0, 0.0};
+    v4d rr = v4d{0.0, 0.0, 0.0, 0.0}, ii = rr, ri = rr, ir = rr;
 #pragma unroll
     for (int s = 0; s < NS; ++s) {
-      const double2 p = cur[(4 * s + kk) * 16 + col];
-      dr = __builtin_amdgcn_mfma_f64_16x16x4f64(p.x, cr[s], dr, 0, 0, 0);
-      dr = __builtin_amdgcn_mfma_f64_16x16x4f64(p.y, -ci[s], dr, 0, 0, 0);
-      di = __builtin_amdgcn_mfma_f64_16x16x4f64(p.x, ci[s], di, 0, 0, 0);
-      di = __builtin_amdgcn_mfma_f64_16x16x4f64(p.y, cr[s], di, 0, 0, 0);
+      if (s < nsr) {
+        const double2 p = cur[(4 * s + kk) * 16 + col];
+        rr = __builtin_amdgcn_mfma_f64_16x16x4f64(p.x, cr[s], rr, 0, 0, 0);
+        ii = __builtin_amdgcn_mfma_f64_16x16x4f64(p.y, ci[s], ii, 0, 0, 0);
+        ri = __builtin_amdgcn_mfma_f64_16x16x4f64(p.x, ci[s], ri, 0, 0, 0);
+        ir = __builtin_amdgcn_mfma_f64_16x16x4f64(p.y, cr[s], ir, 0, 0, 0);
+      }
     }
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
       const int j = 16 * t + kk + 4 * v;
-      if (j < n && b < eanz) Y[((size_t)k * n + j) * eanz + b] = make_double2(dr[v], di[v]);
+      if (j < n && b < eanz) Y[((size_t)k * n + j) * eanz + b] = make_double2(rr[v] - ii[v], ri[v] + ir[v]);
     }
     if (tn < ntiles) put(nxt);
     __syncthreads();
@@ -759,6 +769,8 @@ __global__ __launch_bounds__(256) void crt_adj_gemm_kernel(const double2* __rest
 // XCD-major: the blocks the dispatcher places on one XCD (block index mod 8)
 // take one contiguous element range, so neighbouring elements, which share
 // nodes, read them through the same L2.  The grid is a multiple of 8 blocks.
+// The wavenumbers of a pass (A.k0, A.kc) add to the sums carried in A.acc;
+// the last pass forms the gradient.
 constexpr int kAdjElemPerBlock = 8;
 __global__ __launch_bounds__(256) void crt_adj_elem_kernel(AdjArgs A) {
   constexpr int kXcd = 8;
@@ -773,7 +785,8 @@ __global__ __launch_bounds__(256) void crt_adj_elem_kernel(AdjArgs A) {
   for (int q = 0; q < 4; ++q) nodes[q] = A.el_nodes[e * 4 + min(q, nk - 1)];
   const int E = A.eanz;
   double accr = 0.0, acci = 0.0;
-  for (int k = 0; k < A.K; ++k) {
+  for (int kr = 0; kr < A.kc; ++kr) {
+    const int k = A.k0 + kr;  // wavenumber; kr its index in kpt and Y
     const double* sel = A.elbg + ((size_t)k * A.elanz + e) * A.nimn;
     double sm[4][4];
 #pragma unroll
@@ -786,8 +799,8 @@ __global__ __launch_bounds__(256) void crt_adj_elem_kernel(AdjArgs A) {
     const double wk = (A.swrtr == 0) ? 1.0 : A.kwnwi[k];
     // Y and the transposed potentials are both [K][n][E]: every load below
     // reads a contiguous run of electrodes
-    const double2* Yk = A.Y + (size_t)k * A.n * E;
-    const double2* Pk = A.kpt + (size_t)k * A.n * E;
+    const double2* Yk = A.Y + (size_t)kr * A.n * E;
+    const double2* Pk = A.kpt + (size_t)kr * A.n * E;
     for (int b = hl; b < E; b += 32) {
       cplx yv[4], pv[4];
 #pragma unroll
@@ -814,6 +827,11 @@ __global__ __launch_bounds__(256) void crt_adj_elem_kernel(AdjArgs A) {
   }
   if (hl == 0 && e0 < A.elanz) {
     cplx s = mkc(accr, acci);
+    if (!A.first) s = cadd(ldc(A.acc + e), s);
+    if (!A.last) {
+      stc(A.acc + e, s);
+      return;
+    }
     if (A.swrtr != 0) s = rmul(1.0 / M_PI, s);
     const cplx beta = (A.mode == 0) ? mkc(1.0, 0.0) : ldc(A.sigma + e);
     stc(A.grad + e, cconj(cmul(beta, s)));
