@@ -172,6 +172,11 @@ def main():
     ap.add_argument("--dirichlet", action="store_true",
                     help="the same 240x80 model with Dirichlet sides and bottom: one shared factor per "
                          "wavenumber, then a forward and a backward sweep per source (kompb)")
+    ap.add_argument("--recompute-passes", type=int, default=7, metavar="K",
+                    help="also time forward + adjoint gradient on a second handle that holds K wavenumbers' "
+                         "potentials at a time (crt_set_recompute; 0 leaves it out)")
+    ap.add_argument("--recompute-budget", type=float, default=20.0, metavar="GIB",
+                    help="device budget for factors in flight on the recomputing handle (crt_set_device_budget)")
     args = ap.parse_args()
     if args.copy_probe:
         import torch
@@ -205,6 +210,7 @@ def main():
     else:
         w = meshgen.large_grf(side_type=13 if args.dirichlet else 11)
     m = w.mesh
+    free_start = torch.cuda.mem_get_info()[0]
     g = crt.CRT(m)
     g.rwaven(w.strnr, w.vnr)
     if args.k20:
@@ -342,6 +348,61 @@ def main():
                            **{k: v[0] / v[1] for k, v in t_exp.items()}},
         }
         log(f"adjoint {adj_ms:.3f} ms, explicit J {exp_ms:.3f} ms")
+
+        if args.recompute_passes > 0:
+            # the same forward + adjoint gradient holding a pass of
+            # wavenumbers' potentials at a time, against the stored mode
+            # above: wall time and device bytes the handle holds (free device
+            # memory before the handle and after its adjoint; the stored
+            # handle also holds its explicit J)
+            torch.cuda.synchronize()
+            stored_bytes = free_start - torch.cuda.mem_get_info()[0]
+            free0 = torch.cuda.mem_get_info()[0]
+            gr = crt.CRT(m)
+            gr.rwaven(w.strnr, w.vnr)
+            if args.k20:
+                gr.set_wavenumbers(*k20_rule(gr))
+            gr.precal()
+            gr.set_recompute(args.recompute_passes)
+            gr.set_device_budget(int(args.recompute_budget * (1 << 30)))
+
+            def fwd_adj(h):
+                def fwd():
+                    rc = h.lib.crt_forward_dev(h.h, sigma_dev.data_ptr())
+                    if rc:
+                        raise crt.CRTError("forward_dev", rc)
+                    rc = h.lib.crt_voltages_dev(h.h, N, st.ctypes.data, vn.ctypes.data, 0, volt.data_ptr(),
+                                                sigmaa.data_ptr())
+                    if rc:
+                        raise crt.CRTError("voltages_dev", rc)
+                fwd()
+                h.adjoint_grad(wgt, r, mode=1)  # the pass buffers exist before the timed calls
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()
+                fwd()
+                torch.cuda.synchronize()
+                t1 = time.perf_counter()
+                gg = h.adjoint_grad(wgt, r, mode=1)
+                t2 = time.perf_counter()
+                return (t1 - t0) * 1e3, (t2 - t1) * 1e3, gg
+
+            rf_ms, ra_ms, gr_grad = fwd_adj(gr)
+            torch.cuda.synchronize()
+            rec_bytes = free0 - torch.cuda.mem_get_info()[0]
+            sf_ms, sa_ms, _ = fwd_adj(g)
+            result["recompute"] = {
+                "wavenumbers_per_pass": args.recompute_passes,
+                "factor_budget_bytes": int(args.recompute_budget * (1 << 30)),
+                "forward_ms": rf_ms, "adjoint_gradient_ms": ra_ms,
+                "stored_forward_ms": sf_ms, "stored_adjoint_gradient_ms": sa_ms,
+                "device_bytes": rec_bytes, "stored_device_bytes": stored_bytes,
+                "potential_bytes_per_copy": 16 * min(args.recompute_passes, K) * E * n,
+                "stored_potential_bytes_per_copy": 16 * K * E * n,
+                "rel_diff": float(np.max(np.abs(gr_grad - ga)) / np.max(np.abs(ga))),
+            }
+            log(f"recompute passes of {args.recompute_passes}: forward {rf_ms:.2f} ms, adjoint {ra_ms:.2f} ms, "
+                f"{rec_bytes / 1e9:.2f} GB held (stored {stored_bytes / 1e9:.2f} GB)")
+            gr.close()
 
     if not args.skip_dc:
         # the DC twin (real-entry scaled rows, factor and substitutions) on the

@@ -22,10 +22,11 @@
 //   update waves first apply panel J-1 to the kNB columns of panel J (the
 //   look-ahead), then, while wave 0 factors panel J in registers without any
 //   workgroup barrier, they apply panel J-1 to the rest of the trailing
-//   window.  Two barriers per stage.  Every entry receives -L_ik L_jk in
-//   ascending k and is divided by L_jj once (the reference's arithmetic per
-//   entry).  The window lives in a ring that rows enter from the scaled-row
-//   records one batch of kNB rows per stage.
+//   window.  Two barriers per stage.  Every entry receives each earlier
+//   panel's product sum_k L_ik L_jk (ascending k, formed from zero) as one
+//   subtraction, and is divided by its pivot once.  The window lives in a
+//   ring that rows enter from the scaled-row records one batch of kNB rows
+//   per stage.
 //
 // crt_backsolve_kernel: x = -L^{-T} y (vre, src/vre.f90:58-69) and
 //   kpot = x * f (src/fem.f90:385), one wavefront per system.
