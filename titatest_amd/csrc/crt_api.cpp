@@ -1120,7 +1120,7 @@ static int run_forward_range(crt_handle* h, int k0, int kc, double2* kpot_dst, b
     tstop(h);
     h->slots = std::min(cnt, h->ncu * per_cu);
     if (gring) {
-      A.ring_stride = (size_t)((mb + 1) / 2) * (mb + 1);
+      A.ring_stride = (size_t)crt_ring_entries(mb);
       A.ring = h->get<double2>("ring", (size_t)h->slots * A.ring_stride);
       NEED(A.ring);
     }
@@ -1366,7 +1366,7 @@ int crt_forward_many(int32_t nmodels, crt_handle* const* hs, const double* const
     size_t lds = 0, stride = 0;
     for (int i : idx) {
       lds = std::max(lds, ms[i].form.lds);
-      stride = std::max(stride, (size_t)((ms[i].h->mb + 1) / 2) * (ms[i].h->mb + 1));
+      stride = std::max(stride, (size_t)crt_ring_entries(ms[i].h->mb));
     }
     if (int rc = open_lds(h0, reinterpret_cast<const void*>(F.many))) return rc;
     int per_cu = 0;

@@ -90,11 +90,11 @@ struct Ring {
   }
 };
 
-// Panel buffer: W rows of kNB columns, column index swizzled by the row's
-// 4-row group so that tiles reading one column from different rows spread
-// over the LDS banks.
-__device__ __forceinline__ int pkey(int rw) { return (rw >> 2) & (kNB - 1); }
-__device__ __forceinline__ int pix(int rw, int k) { return rw * kNB + (k ^ pkey(rw)); }
+// Panel buffer: rows of kNB columns, one pad entry after every 4-row group,
+// so that tiles reading one column from rows of different groups spread over
+// the LDS banks, while the columns of a row stay at constant offsets from it
+// (immediate offsets of the LDS loads).
+__device__ __forceinline__ int pix(int rw, int k) { return rw * kNB + (rw >> 2) + k; }
 
 // Panel rows of wave 0's lanes.  Slot 0: lanes 0-7 of every 16-lane row hold
 // the panel's diagonal rows 0-7 (four copies, so that a DPP row broadcast
@@ -348,8 +348,9 @@ __device__ __forceinline__ void trailing_tiles(int mb, int (&rs_r0)[MAXT], int (
 // TR x TC tile of the stage window at (rw0, cw0): the current values from the
 // ring (every window entry lives there; 0 outside the window's lower triangle
 // and the matrix) minus panel J-1's rank-kNB product, in ascending k.  slot
-// receives each entry's ring position (0 where the entry does not exist), for
-// the caller's store.  rw0 and cw0 are multiples of TR = TC (2 or 4), so the
+// receives each entry's ring position for the caller's store: the ring's
+// discard slot where the entry does not exist, so that every store is
+// unconditional.  rw0 and cw0 are multiples of TR = TC (2 or 4), so the
 // tile's operand rows of panel J-1 (stage rows + kNB) lie in one 4-row group:
 // they share one swizzle key and sit kNB entries apart, and the panel buffer's
 // 4 pad rows take the groups that reach past its W rows (their products fall
@@ -366,15 +367,36 @@ __device__ __forceinline__ void ring_tile(const Stage<T>& C, const T* Tt, int rw
   // read them after it, which keeps the registers below the kernel's limit.
   constexpr bool early = TR * TC <= 4;
   auto window = [&]() __attribute__((always_inline)) {
+    // Ring positions (Ring::pos) by parts: the tile's diagonals t = dt + i - j
+    // take row t (below half) or mb - 1 - t of the ring; the slot within the
+    // row is (c + om) mod (mb + 1) below half, (r + 1 + om) mod (mb + 1) for
+    // the partners, one wrap per tile column and per tile row.  Absent
+    // entries read the ring's zero slot and store to its discard slot.
+    const int M1 = C.ring.M1, om = C.ring.om, half = C.ring.half;
+    const int zs = half * M1;  // zero slot; zs + 1 discards
+    const int dt = rw0 - cw0, t0 = dt * M1, q0 = (C.mb - 1 - dt) * M1;
+    int cm[TC], rm[TR];
+    bool okr[TR];
+#pragma unroll
+    for (int j = 0; j < TC; ++j) {
+      const int p = cw0 + j + om;
+      cm[j] = p >= M1 ? p - M1 : p;
+    }
+#pragma unroll
+    for (int i = 0; i < TR; ++i) {
+      const int p = rw0 + i + 1 + om;
+      rm[i] = p >= M1 ? p - M1 : p;
+      okr[i] = rw0 + i < C.mb && C.o + rw0 + i < C.n;
+    }
 #pragma unroll
     for (int i = 0; i < TR; ++i)
 #pragma unroll
       for (int j = 0; j < TC; ++j) {
-        const int rw = rw0 + i, cw = cw0 + j;
-        const bool ok = tile_ok(C, rw, cw);
-        const int rwc = min(max(rw, 0), C.mb - 1);
-        slot[i][j] = ok ? C.ring.pos(rwc, min(max(cw, 0), rwc)) : 0;
-        acc[i][j] = sel(ok, Tt[slot[i][j]], szero<T>());
+        const int d = i - j;
+        const bool ok = okr[i] && dt + d >= 0;
+        const int pos = dt + d < half ? t0 + d * M1 + cm[j] : q0 - d * M1 + rm[i];
+        slot[i][j] = ok ? pos : zs + 1;
+        acc[i][j] = Tt[ok ? pos : zs];
       }
   };
   if (early) window();
@@ -384,17 +406,15 @@ __device__ __forceinline__ void ring_tile(const Stage<T>& C, const T* Tt, int rw
 #pragma unroll
     for (int j = 0; j < TC; ++j) P[i][j] = szero<T>();
   if (C.apply) {
-    const int ra0 = rw0 + kNB, ca0 = cw0 + kNB;
-    const T* pa = C.Pp + ra0 * kNB;
-    const T* pb = C.Pp + ca0 * kNB;
-    const int ka = pkey(ra0), kb = pkey(ca0);
+    const T* pa = C.Pp + pix(rw0 + kNB, 0);
+    const T* pb = C.Pp + pix(cw0 + kNB, 0);
 #pragma unroll (early ? kNB : 1)
     for (int k = 0; k < kNB; ++k) {
       T a[TR], b[TC];
 #pragma unroll
-      for (int i = 0; i < TR; ++i) a[i] = pa[i * kNB + (k ^ ka)];
+      for (int i = 0; i < TR; ++i) a[i] = pa[i * kNB + k];
 #pragma unroll
-      for (int j = 0; j < TC; ++j) b[j] = pb[j * kNB + (k ^ kb)];
+      for (int j = 0; j < TC; ++j) b[j] = pb[j * kNB + k];
 #pragma unroll
       for (int i = 0; i < TR; ++i)
 #pragma unroll
@@ -445,9 +465,9 @@ __device__ __forceinline__ void factor_system(const FactorArgs& A, int s, double
   const int n = A.n, mb = A.mb, R = A.R;
   const int W = mb + kNB, M1 = mb + 1, half = (mb + 1) / 2;
   T* Tt = GR ? reinterpret_cast<T*>(A.ring) + (size_t)blockIdx.x * A.ring_stride : reinterpret_cast<T*>(smem);
-  const int WP = W + 4;                       // panel rows with the 4 pad rows
-  T* P0 = GR ? reinterpret_cast<T*>(smem) : Tt + crt_ring_lds_entries(mb, ES);  // 2 panels of WP x kNB
-  T* sv = P0 + 2 * WP * kNB;                  // 2 batches of kNB x kMaxR values
+  const int PS = crt_panel_entries(mb);      // W rows, 4 pad rows, group pads
+  T* P0 = GR ? reinterpret_cast<T*>(smem) : Tt + crt_ring_lds_entries(mb, ES);  // 2 panels
+  T* sv = P0 + 2 * PS;                        // 2 batches of kNB x kMaxR values
   int16_t* so = reinterpret_cast<int16_t*>(sv + 2 * NB);  // their band offsets
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int x3 = tid - 64;                    // thread index within waves 1..
@@ -471,7 +491,7 @@ __device__ __forceinline__ void factor_system(const FactorArgs& A, int s, double
 
   // ---- prologue: ring zeroed, window rows [0, mb) scattered into it, batch
   // 0 = rows [mb, mb + kNB) staged, wave 0's right-hand side
-  for (int i = tid; i < half * M1; i += NT) Tt[i] = szero<T>();
+  for (int i = tid; i < crt_ring_entries(mb); i += NT) Tt[i] = szero<T>();
   stage_barrier<GR>();
   {
     Ring ring0{mb, M1, half, 0};
@@ -525,8 +545,8 @@ __device__ __forceinline__ void factor_system(const FactorArgs& A, int s, double
     C.o = o;
     C.ring = Ring{mb, M1, half, o % M1};
     C.apply = J > 0;
-    C.Pp = P0 + ((J + 1) & 1) * WP * kNB;
-    T* Pc = P0 + (J & 1) * WP * kNB;
+    C.Pp = P0 + ((J + 1) & 1) * PS;
+    T* Pc = P0 + (J & 1) * PS;
     T* svc = sv + (J & 1) * NB;                 // batch J: rows [o + mb, o + mb + kNB)
     int16_t* soc = so + (J & 1) * NB;
     T* svn = sv + ((J + 1) & 1) * NB;           // batch J+1
@@ -544,7 +564,9 @@ __device__ __forceinline__ void factor_system(const FactorArgs& A, int s, double
     // ring into the panel buffer, their slots zeroed); wave 0 copies batch J's
     // entries in panel J's columns into the panel's last kNB rows
     if (wave > 0) {
-      __builtin_amdgcn_s_setprio(2);
+      // the look-ahead is on the chain from panel to panel: it issues ahead
+      // of the other workgroup's trailing update, as the panel does
+      __builtin_amdgcn_s_setprio(3);
       for (int x = lx; x < nla; x += NT - 64) {
         int j = 0, i = x;
 #pragma unroll
@@ -563,7 +585,7 @@ __device__ __forceinline__ void factor_system(const FactorArgs& A, int s, double
           for (int b = 0; b < 2; ++b) {
             const int rw = rw0 + a, cw = cw0 + b;
             if (rw < mb) Pc[pix(rw, cw)] = acc[a][b];
-            if (tile_ok(C, rw, cw)) Tt[slot[a][b]] = szero<T>();
+            Tt[slot[a][b]] = szero<T>();
           }
       }
     } else {
@@ -714,7 +736,7 @@ __device__ __forceinline__ void factor_system(const FactorArgs& A, int s, double
           for (int i = 0; i < 4; ++i)
 #pragma unroll
             for (int j = 0; j < 4; ++j)
-              if (tile_ok(C, rs_r0[q] + i, rs_c0[q] + j)) Tt[slot[i][j]] = acc[i][j];
+              Tt[slot[i][j]] = acc[i][j];
         }
       }
       if (lx < NB) {

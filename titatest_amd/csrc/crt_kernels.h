@@ -110,20 +110,29 @@ __global__ void crt_lcol_kernel(const void* L, void* Lc, int n, int mb, int nfac
 template <int DEPTH, int G, int NS, class T = cplx>
 __global__ void crt_fwdsweep_kernel(FwdArgs F);
 
-// Entries of the trailing-window rings in LDS for bandwidth mb and entry size
-// es bytes (16 complex, 8 real): half x (mb + 1), rounded up to an even count
-// for 8-byte entries so that what follows stays 16-byte aligned.
+// Entries of one workgroup's trailing-window ring: (mb + 1) / 2 rows of mb + 1
+// slots, then a slot that stays zero (read for entries outside the window)
+// and a discard slot (their stores).
+__host__ __device__ inline int crt_ring_entries(int mb) { return ((mb + 1) / 2) * (mb + 1) + 2; }
+// Entries of the ring in LDS for bandwidth mb and entry size es bytes (16
+// complex, 8 real), rounded up to an even count for 8-byte entries so that
+// what follows stays 16-byte aligned.
 __host__ __device__ inline int crt_ring_lds_entries(int mb, size_t es) {
-  const int M1 = mb + 1, n = (M1 / 2) * M1;
+  const int n = crt_ring_entries(mb);
   return es == 16 ? n : (n + 1) & ~1;
+}
+// Entries of one panel buffer of the factor kernel: mb + kNB rows and 4 pad
+// rows of kNB columns, with one pad entry after every 4-row group.
+__host__ __device__ inline int crt_panel_entries(int mb) {
+  const int WP = mb + kNB + 4;
+  return WP * kNB + (WP + 3) / 4;
 }
 // LDS bytes of the factor kernel for bandwidth mb, entry size es (16 complex,
 // 8 real) and the ring in LDS (gr false) or global memory: the ring, two
-// panels of mb + kNB rows and 4 pad rows, and two staged row batches with
-// their band offsets.
+// panel buffers, and two staged row batches with their band offsets.
 inline size_t crt_factor_lds_bytes(int mb, size_t es, bool gr) {
-  const size_t WP = (size_t)mb + kNB + 4, ring = gr ? 0 : (size_t)crt_ring_lds_entries(mb, es);
-  return (ring + 2 * WP * kNB + 2 * kNB * kMaxR) * es + 2 * kNB * kMaxR * 2;
+  const size_t ring = gr ? 0 : (size_t)crt_ring_lds_entries(mb, es);
+  return (ring + 2 * (size_t)crt_panel_entries(mb) + 2 * kNB * kMaxR) * es + 2 * kNB * kMaxR * 2;
 }
 // Widest band the factorisation and substitutions take: the wide-band factor
 // form holds the mb panel rows below the diagonal block in four rows per
