@@ -48,6 +48,7 @@ struct SensPairs {
                    // wavefront then mostly share u and read neighbouring d34
 };
 constexpr int kSensPairQ = 16;  // measurements per thread per pass
+constexpr int kSensRow = 5;     // LDS entries per pair row of the pair form (4 used)
 
 struct AdjArgs {
   int n, elanz, eanz, K, nimn, swrtr, mode;

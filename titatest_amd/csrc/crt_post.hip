@@ -422,38 +422,42 @@ __global__ __launch_bounds__(256) void crt_sens_kernel(SensArgs S) {
 // potential pair are formed once in LDS (the same operations, in the same
 // order, as the per-measurement form of crt_sens_kernel); each measurement then
 // takes h_k = sum_a d34_a u_a from two LDS rows.  The measurements are taken in
-// (current pair, potential pair) order, a contiguous run per thread, and
-// written to their own rows.  LDS:
-// (4 E + 4 ncp + 4 npp) complex values.
+// (current pair, potential pair) order, interleaved over the threads, and
+// written to their own rows.  LDS layout, complex values: P [4][E + 1] (column
+// E stays zero: the potential of an absent electrode), U [ncp][kSensRow] and D
+// [npp][kSensRow].  A pair's four values sit at constant offsets from its row
+// (immediate offsets of the LDS loads), and the row of kSensRow = 5 entries
+// puts 16 consecutive rows on distinct banks.
 __global__ __launch_bounds__(512) void crt_sens_pairs_kernel(SensArgs S, SensPairs Q) {
   extern __shared__ double2 sm[];
   const int e = blockIdx.x;
   const int nk = S.el_nk[e];
-  const int E = S.eanz, ncp = Q.ncp, npp = Q.npp;
-  cplx* P = reinterpret_cast<cplx*>(sm);  // [4][E]
-  cplx* U = P + 4 * E;                    // [4][ncp]
-  cplx* D = U + 4 * ncp;                  // [4][npp]
+  const int E = S.eanz, E1 = E + 1, ncp = Q.ncp, npp = Q.npp;
+  cplx* P = reinterpret_cast<cplx*>(sm);  // [4][E + 1]
+  cplx* U = P + 4 * E1;                   // [ncp][kSensRow]
+  cplx* D = U + kSensRow * ncp;           // [npp][kSensRow]
   __shared__ double sel[4][4];
   __shared__ int nodes[4];
   if (threadIdx.x < 4) nodes[threadIdx.x] = S.el_nodes[e * 4 + threadIdx.x];
   const cplx sig = ldc(S.sigma + e);
-  auto pick = [&](int m, int el) { return csel(el > 0, P[m * E + max(el - 1, 0)], mkc(0.0, 0.0)); };
+  // column of electrode el (1-based, 0 = none) in P
+  auto col = [&](int el) { return el > 0 ? el - 1 : E; };
   for (int i0 = 0; i0 < S.nanz; i0 += kSensPairQ * blockDim.x) {
-    // this pass's measurements in (current pair, potential pair) order, a
-    // contiguous run of rp per thread: the run's measurements mostly share
-    // their current pair, whose u is read once per change
-    const int npass = min(S.nanz - i0, kSensPairQ * (int)blockDim.x);
-    const int rp = (npass + (int)blockDim.x - 1) / (int)blockDim.x;
-    const int jb = i0 + (int)threadIdx.x * rp, je = i0 + npass;
+    // this pass's measurements in (current pair, potential pair) order,
+    // interleaved over the threads: the lanes of a wavefront take consecutive
+    // measurements, so their potential pairs' rows of D are mostly
+    // consecutive, their current pairs' rows of U mostly one broadcast, and
+    // their stores one run of the sensitivity row
+    const int nq = min(kSensPairQ, (S.nanz - i0 - (int)threadIdx.x + (int)blockDim.x - 1) / (int)blockDim.x);
     cplx acc[kSensPairQ];
-    int ci[kSensPairQ], pi[kSensPairQ];
+    int uo[kSensPairQ], dq[kSensPairQ];
 #pragma unroll
     for (int q = 0; q < kSensPairQ; ++q) {
       acc[q] = mkc(0.0, 0.0);
-      const int i = Q.ord[min(jb + q, S.nanz - 1)];
+      const int i = Q.ord[min(i0 + (int)threadIdx.x + q * (int)blockDim.x, S.nanz - 1)];
       const int2 m = Q.mp[i];
-      ci[q] = m.x;
-      pi[q] = m.y;
+      uo[q] = kSensRow * m.x;
+      dq[q] = kSensRow * m.y;
     }
     // the element's potentials of wavenumber k + 1 are loaded into registers
     // while k is evaluated (kSensPre per thread covers 4 E for E <= 128 at
@@ -470,16 +474,18 @@ __global__ __launch_bounds__(512) void crt_sens_pairs_kernel(SensArgs S, SensPai
     };
     __syncthreads();  // nodes[]
     fetch(0);
+    if (threadIdx.x < 4) P[threadIdx.x * E1 + E] = mkc(0.0, 0.0);
     for (int k = 0; k < S.K; ++k) {
       __syncthreads();
 #pragma unroll
       for (int r = 0; r < kSensPre; ++r) {
         const int t = threadIdx.x + r * blockDim.x;
-        if (t < 4 * E) P[t] = pre[r];
+        const int q = t / E, a = t - q * E;
+        if (t < 4 * E) P[q * E1 + a] = pre[r];
       }
       for (int t = threadIdx.x + kSensPre * blockDim.x; t < 4 * E; t += blockDim.x) {
         const int q = t / E, a = t - q * E;
-        P[t] = q < nk ? ldc(S.kpt + ((size_t)k * S.n + nodes[q]) * E + a) : mkc(0.0, 0.0);
+        P[q * E1 + a] = q < nk ? ldc(S.kpt + ((size_t)k * S.n + nodes[q]) * E + a) : mkc(0.0, 0.0);
       }
       if (k + 1 < S.K) fetch(k + 1);
       if (threadIdx.x < 16) {
@@ -491,45 +497,50 @@ __global__ __launch_bounds__(512) void crt_sens_pairs_kernel(SensArgs S, SensPai
       __syncthreads();
       for (int c = threadIdx.x; c < ncp; c += blockDim.x) {
         const int2 pr = Q.cp[c];
+        const int c1 = col(pr.x), c2 = col(pr.y);
         cplx d12[4];
 #pragma unroll
-        for (int m = 0; m < 4; ++m) d12[m] = csub(pick(m, pr.y), pick(m, pr.x));
+        for (int m = 0; m < 4; ++m) d12[m] = csub(P[m * E1 + c2], P[m * E1 + c1]);
 #pragma unroll
         for (int a = 0; a < 4; ++a) {
           cplx u = mkc(0.0, 0.0);
 #pragma unroll
           for (int b = 0; b < 4; ++b) u = cadd(u, rmul(sel[a][b], d12[b]));
-          U[a * ncp + c] = u;
+          U[c * kSensRow + a] = u;
         }
       }
       for (int p = threadIdx.x; p < npp; p += blockDim.x) {
         const int2 pr = Q.pp[p];
+        const int c1 = col(pr.x), c2 = col(pr.y);
 #pragma unroll
-        for (int m = 0; m < 4; ++m) D[m * npp + p] = csub(pick(m, pr.y), pick(m, pr.x));
+        for (int m = 0; m < 4; ++m) D[p * kSensRow + m] = csub(P[m * E1 + c2], P[m * E1 + c1]);
       }
       __syncthreads();
+      // h of every measurement; without the wavenumber sum (swrtr = 0) the
+      // sum is h itself (one branch for the whole loop, uniform)
       const double wk = S.kwnwi[k];
-      int cur = -1;
-      cplx u[4];
+      auto eval = [&](auto accumulate) __attribute__((always_inline)) {
 #pragma unroll
-      for (int q = 0; q < kSensPairQ; ++q) {
-        if (q < rp && jb + q < je) {
-          if (ci[q] != cur) {
-            cur = ci[q];
+        for (int q = 0; q < kSensPairQ; ++q) {
+          if (q < nq) {
+            const cplx* dr = D + dq[q];
+            const cplx* ur = U + uo[q];
+            cplx h = mkc(0.0, 0.0);
 #pragma unroll
-            for (int a = 0; a < 4; ++a) u[a] = U[a * ncp + cur];
+            for (int a = 0; a < 4; ++a) h = fmacc(h, dr[a], ur[a]);
+            acc[q] = accumulate(acc[q], h);
           }
-          cplx h = mkc(0.0, 0.0);
-#pragma unroll
-          for (int a = 0; a < 4; ++a) h = cadd(h, cmul(D[a * npp + pi[q]], u[a]));
-          acc[q] = S.swrtr == 0 ? h : cadd(acc[q], rmul(wk, h));
         }
-      }
+      };
+      if (S.swrtr == 0)
+        eval([](cplx, cplx h) { return h; });
+      else
+        eval([wk](cplx a, cplx h) { return mkc(fma(wk, h.re, a.re), fma(wk, h.im, a.im)); });
     }
 #pragma unroll
     for (int q = 0; q < kSensPairQ; ++q) {
-      const int j = jb + q;
-      if (q >= rp || j >= je) continue;
+      const int j = i0 + threadIdx.x + q * blockDim.x;
+      if (q >= nq) continue;
       const int i = Q.ord[j];
       cplx out;
       if (S.mode == 0) {
@@ -654,11 +665,12 @@ __global__ __launch_bounds__(256) void crt_adj_gemm_mfma_kernel(const double2* _
 // 16 columns [16 w, 16 w + 16) of C in registers for the ceil(eanz / 4)
 // 4-electrode steps (operand B), the workgroup stages 16-node tiles of P[k]
 // (every electrode) in LDS, double buffered, and each wavefront forms its
-// 16 x 16 block of Y per node tile (operand A from LDS).  The four real
-// products of a complex step go to four accumulators (Pr Cr, Pi Ci, Pr Ci,
-// Pi Cr; Y = (Pr Cr - Pi Ci) + i (Pr Ci + Pi Cr) at the end), so four
-// independent chains share the matrix core.  C is read once per workgroup;
-// the next tile's global loads are in flight during the products.
+// 16 x 16 block of Y per node tile (operand A from LDS).  A complex step
+// takes three real products (Gauss): Pr Cr, Pi Ci and (Pr + Pi)(Cr + Ci) in
+// three accumulators, Y = (Pr Cr - Pi Ci) + i ((Pr + Pi)(Cr + Ci) - Pr Cr -
+// Pi Ci) at the end; three independent chains share the matrix core.  C is
+// read once per workgroup; the next tile's global loads are in flight during
+// the products.
 template <int NS>
 __global__ __launch_bounds__(512) void crt_adj_gemm_cs_kernel(const double2* __restrict__ kpot,
                                                             const double2* __restrict__ C, int n, int eanz,
@@ -708,21 +720,20 @@ __global__ __launch_bounds__(512) void crt_adj_gemm_cs_kernel(const double2* __r
     double2* nxt = ptile + ((it + 1) & 1) * TILE;
     const int tn = t + gridDim.x;
     if (tn < ntiles) fetch(tn);
-    v4d rr = v4d{0.0, 0.0, 0.0, 0.0}, ii = rr, ri = rr, ir = rr;
+    v4d rr = v4d{0.0, 0.0, 0.0, 0.0}, ii = rr, ss = rr;
 #pragma unroll
     for (int s = 0; s < NS; ++s) {
       if (s < nsr) {
         const double2 p = cur[(4 * s + kk) * 16 + col];
         rr = __builtin_amdgcn_mfma_f64_16x16x4f64(p.x, cr[s], rr, 0, 0, 0);
         ii = __builtin_amdgcn_mfma_f64_16x16x4f64(p.y, ci[s], ii, 0, 0, 0);
-        ri = __builtin_amdgcn_mfma_f64_16x16x4f64(p.x, ci[s], ri, 0, 0, 0);
-        ir = __builtin_amdgcn_mfma_f64_16x16x4f64(p.y, cr[s], ir, 0, 0, 0);
+        ss = __builtin_amdgcn_mfma_f64_16x16x4f64(p.x + p.y, cr[s] + ci[s], ss, 0, 0, 0);
       }
     }
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
       const int j = 16 * t + kk + 4 * v;
-      if (j < n && b < eanz) Y[((size_t)k * n + j) * eanz + b] = make_double2(rr[v] - ii[v], ri[v] + ir[v]);
+      if (j < n && b < eanz) Y[((size_t)k * n + j) * eanz + b] = make_double2(rr[v] - ii[v], ss[v] - rr[v] - ii[v]);
     }
     if (tn < ntiles) put(nxt);
     __syncthreads();
