@@ -452,7 +452,7 @@ __device__ __forceinline__ void ring_tile(const Stage<T>& C, const T* Tt, int rw
 // ROWS: panel row slots per lane of wave 0 (prow; W <= 40 + 64 (ROWS - 1));
 // MAXT:
 // trailing tiles per thread of the update waves; NT threads per workgroup:
-// 256 (one panel and three update waves; two workgroups per CU, three for
+// 256 (one panel and three update waves; two workgroups per CU, four for
 // the real entries) or, when one workgroup's LDS leaves no room for a second
 // (complex, mb above ~98), 512 (seven update waves); GR: the ring lives in a
 // global scratch of half x (mb + 1) entries per workgroup (A.ring) instead of
@@ -758,10 +758,11 @@ __device__ __forceinline__ void factor_system(const FactorArgs& A, int s, double
 }
 
 // Persistent over the batch's systems (workgroup b takes systems b, b + grid,
-// ...).  T = double is the DC twin; its halved registers and LDS leave room
-// for three workgroups per CU at the narrower bands.
+// ...).  T = double is the DC twin; its halved LDS (41 KB at mb = 82) and
+// registers (128, a few values spilled outside the stage loop) leave room
+// for four workgroups per CU at the narrower bands.
 template <int ROWS, int MAXT, int NT, bool GR, class T>
-__global__ __launch_bounds__(NT, (std::is_same<T, double>::value ? 768 : 512) / NT) void crt_factor_kernel(
+__global__ __launch_bounds__(NT, (std::is_same<T, double>::value ? 1024 : 512) / NT) void crt_factor_kernel(
     FactorArgs A) {
   extern __shared__ double2 smem[];
   PhaseClock clk;
