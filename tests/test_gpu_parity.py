@@ -47,6 +47,8 @@ def small_pair(crtmod, oracle_lib):
     g = crtmod.CRT(w.mesh)
     g.rwaven(w.strnr, w.vnr)
     g.precal()
+    # every test of this module finds a forward done, whichever of them runs
+    g.forward(w.sigma)
     o = oracle_lib.Oracle(w.mesh)
     o.rwaven(w.strnr, w.vnr)
     o.precal()

@@ -1697,14 +1697,19 @@ static int run_sens(crt_handle* h, int32_t mode, const double2* sigma) {
   Q.ncp = (int)h->sens_cp.size();
   Q.npp = (int)h->sens_pp.size();
   const size_t lds_pairs = (size_t)(4 * (h->E + 1) + kSensRow * (Q.ncp + Q.npp)) * 16;
-  // the pair form: 256 threads while its LDS (dynamic and static) leaves room
-  // for two workgroups per CU, else 512 in one (512 threads with 8
-  // measurements per thread and two workgroups per CU measured slower: the
-  // 128-register bound spills); past the CU's LDS the per-measurement
-  // kernel, which holds one row of potentials per element node and electrode
+  // the pair form: 256 threads, each with a contiguous run of measurements,
+  // while its LDS (dynamic and static) leaves room for two workgroups per CU,
+  // else 512 in one with the measurements interleaved (each measured the
+  // faster in its case; 512 threads with 8 measurements each and two
+  // workgroups per CU measured slower: the 128-register bound spills); past
+  // the CU's LDS the per-measurement kernel, which holds one row of
+  // potentials per element node and electrode
   size_t dyn_pairs = 0, dyn = 0;
-  if (int rc = open_lds(h, reinterpret_cast<const void*>(crt_sens_pairs_kernel), &dyn_pairs)) return rc;
+  if (int rc = open_lds(h, reinterpret_cast<const void*>(crt_sens_pairs_kernel<false>), &dyn_pairs)) return rc;
   const bool pairs = lds_pairs <= dyn_pairs;
+  const bool two = 2 * (lds_pairs + h->lds_max - dyn_pairs) <= h->lds_max;
+  if (two)
+    if (int rc = open_lds(h, reinterpret_cast<const void*>(crt_sens_pairs_kernel<true>), &dyn_pairs)) return rc;
   const size_t lds = pairs ? lds_pairs : (size_t)4 * h->E * 16;
   if (!pairs) {
     if (int rc = open_lds(h, reinterpret_cast<const void*>(crt_sens_kernel), &dyn)) return rc;
@@ -1716,8 +1721,10 @@ static int run_sens(crt_handle* h, int32_t mode, const double2* sigma) {
     Q.pp = h->at<int2>("sens_pp");
     Q.mp = h->at<int2>("sens_mp");
     Q.ord = h->at<int>("sens_ord");
-    const int nt = 2 * (lds + h->lds_max - dyn_pairs) <= h->lds_max ? 256 : 512;
-    hipLaunchKernelGGL(crt_sens_pairs_kernel, dim3(h->elanz), dim3(nt), lds, h->stream, S, Q);
+    if (two)
+      hipLaunchKernelGGL(crt_sens_pairs_kernel<true>, dim3(h->elanz), dim3(256), lds, h->stream, S, Q);
+    else
+      hipLaunchKernelGGL(crt_sens_pairs_kernel<false>, dim3(h->elanz), dim3(512), lds, h->stream, S, Q);
   } else {
     hipLaunchKernelGGL(crt_sens_kernel, dim3(h->elanz), dim3(256), lds, h->stream, S);
   }

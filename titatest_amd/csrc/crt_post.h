@@ -98,6 +98,8 @@ __global__ void crt_rtrafo_elec_kernel(const double2* kpot, const double* kwnwi,
 __global__ void crt_volt_kernel(const double2* he, int eanz, int nanz, const int* strnr, const int* vnr,
                                 int mode, const double* kfak, double2* volt, double2* sigmaa, int* err);
 __global__ void crt_sens_kernel(SensArgs S);
+// RUNS: each thread takes a contiguous run of measurements (else interleaved)
+template <bool RUNS>
 __global__ void crt_sens_pairs_kernel(SensArgs S, SensPairs Q);
 __global__ void crt_kpot_transpose_kernel(const double2* kpot, int n, int E, double2* kpt);
 __global__ void crt_adj_gamma_kernel(int nanz, int mode, const double* w, const double2* resid,

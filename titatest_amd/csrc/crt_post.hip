@@ -422,12 +422,13 @@ __global__ __launch_bounds__(256) void crt_sens_kernel(SensArgs S) {
 // potential pair are formed once in LDS (the same operations, in the same
 // order, as the per-measurement form of crt_sens_kernel); each measurement then
 // takes h_k = sum_a d34_a u_a from two LDS rows.  The measurements are taken in
-// (current pair, potential pair) order, interleaved over the threads, and
-// written to their own rows.  LDS layout, complex values: P [4][E + 1] (column
+// (current pair, potential pair) order, interleaved over the threads or in
+// contiguous runs per thread (RUNS), and written to their own rows.  LDS layout, complex values: P [4][E + 1] (column
 // E stays zero: the potential of an absent electrode), U [ncp][kSensRow] and D
 // [npp][kSensRow].  A pair's four values sit at constant offsets from its row
 // (immediate offsets of the LDS loads), and the row of kSensRow = 5 entries
 // puts 16 consecutive rows on distinct banks.
+template <bool RUNS>
 __global__ __launch_bounds__(512) void crt_sens_pairs_kernel(SensArgs S, SensPairs Q) {
   extern __shared__ double2 sm[];
   const int e = blockIdx.x;
@@ -447,14 +448,20 @@ __global__ __launch_bounds__(512) void crt_sens_pairs_kernel(SensArgs S, SensPai
     // interleaved over the threads: the lanes of a wavefront take consecutive
     // measurements, so their potential pairs' rows of D are mostly
     // consecutive, their current pairs' rows of U mostly one broadcast, and
-    // their stores one run of the sensitivity row
-    const int nq = min(kSensPairQ, (S.nanz - i0 - (int)threadIdx.x + (int)blockDim.x - 1) / (int)blockDim.x);
+    // their stores one run of the sensitivity row.
+    // RUNS: a contiguous run of measurements per thread instead, which mostly
+    // share their current pair, so that u is read once per change
+    const int npass = min(S.nanz - i0, kSensPairQ * (int)blockDim.x);
+    const int rp = (npass + (int)blockDim.x - 1) / (int)blockDim.x;
+    const int jb = RUNS ? i0 + (int)threadIdx.x * rp : i0 + (int)threadIdx.x;
+    const int js = RUNS ? 1 : (int)blockDim.x;  // step between a thread's measurements
+    const int nq = RUNS ? max(0, min(rp, i0 + npass - jb)) : (i0 + npass - jb + js - 1) / js;
     cplx acc[kSensPairQ];
     int uo[kSensPairQ], dq[kSensPairQ];
 #pragma unroll
     for (int q = 0; q < kSensPairQ; ++q) {
       acc[q] = mkc(0.0, 0.0);
-      const int i = Q.ord[min(i0 + (int)threadIdx.x + q * (int)blockDim.x, S.nanz - 1)];
+      const int i = Q.ord[min(jb + q * js, S.nanz - 1)];
       const int2 m = Q.mp[i];
       uo[q] = kSensRow * m.x;
       dq[q] = kSensRow * m.y;
@@ -520,14 +527,20 @@ __global__ __launch_bounds__(512) void crt_sens_pairs_kernel(SensArgs S, SensPai
       // sum is h itself (one branch for the whole loop, uniform)
       const double wk = S.kwnwi[k];
       auto eval = [&](auto accumulate) __attribute__((always_inline)) {
+        int cur = -1;
+        cplx u[4];
 #pragma unroll
         for (int q = 0; q < kSensPairQ; ++q) {
           if (q < nq) {
+            if (!RUNS || uo[q] != cur) {
+              cur = uo[q];
+#pragma unroll
+              for (int a = 0; a < 4; ++a) u[a] = U[cur + a];
+            }
             const cplx* dr = D + dq[q];
-            const cplx* ur = U + uo[q];
             cplx h = mkc(0.0, 0.0);
 #pragma unroll
-            for (int a = 0; a < 4; ++a) h = fmacc(h, dr[a], ur[a]);
+            for (int a = 0; a < 4; ++a) h = fmacc(h, dr[a], u[a]);
             acc[q] = accumulate(acc[q], h);
           }
         }
@@ -539,7 +552,7 @@ __global__ __launch_bounds__(512) void crt_sens_pairs_kernel(SensArgs S, SensPai
     }
 #pragma unroll
     for (int q = 0; q < kSensPairQ; ++q) {
-      const int j = i0 + threadIdx.x + q * blockDim.x;
+      const int j = jb + q * js;
       if (q >= nq) continue;
       const int i = Q.ord[j];
       cplx out;
@@ -554,6 +567,9 @@ __global__ __launch_bounds__(512) void crt_sens_pairs_kernel(SensArgs S, SensPai
     }
   }
 }
+
+template __global__ void crt_sens_pairs_kernel<false>(SensArgs, SensPairs);
+template __global__ void crt_sens_pairs_kernel<true>(SensArgs, SensPairs);
 
 // ------------------------------------------------------- adjoint gradient --
 // g_e = sum_i conj(J_ie) w_i r_i without forming J.  With J_ie = alpha_i beta_e
