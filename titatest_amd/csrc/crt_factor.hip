@@ -720,6 +720,9 @@ __device__ __forceinline__ void factor_system(const FactorArgs& A, int s, double
       __builtin_amdgcn_s_setprio(0);
       clk.mark(2);
     } else {
+      // real entries (four workgroups per CU): the trailing update at the
+      // panel's priority as well (measured faster there, slower for complex)
+      if constexpr (std::is_same<T, double>::value) __builtin_amdgcn_s_setprio(3);
       // batch J+1 = rows [o + kNB + mb, o + 2 kNB + mb): loads issued first,
       // staged after the tiles
       T nv = szero<T>();
