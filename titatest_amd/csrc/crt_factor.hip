@@ -648,11 +648,21 @@ __device__ __forceinline__ void factor_system(const FactorArgs& A, int s, double
 #pragma unroll
         for (int q = 1; q < ROWS; ++q) lrc[q] = smul(v[q][c], inv);
         const T yp = rbcast(yv[0], c);  // y'_c, final
+        // With three or more row slots per lane each lane read is used
+        // before the next one is issued (the updates pinned in order): the
+        // compiler otherwise issues all of a column's reads at once and
+        // spills them from the scalar registers (with two slots the
+        // unpinned form measured faster).
 #pragma unroll
         for (int c2 = c + 1; c2 < kNB; ++c2) {
-          const T s2 = srl(v[0][c], c2);  // s_{c2,c} of the diagonal block
+          T src = v[0][c];
+          if constexpr (ROWS >= 3) spin(src);
+          const T s2 = srl(src, c2);  // s_{c2,c} of the diagonal block
 #pragma unroll
-          for (int q = 0; q < ROWS; ++q) v[q][c2] = msub(v[q][c2], lrc[q], s2);
+          for (int q = 0; q < ROWS; ++q) {
+            v[q][c2] = msub(v[q][c2], lrc[q], s2);
+            if constexpr (ROWS >= 3) spin(v[q][c2]);
+          }
         }
 #pragma unroll
         for (int q = 0; q < ROWS; ++q) {
