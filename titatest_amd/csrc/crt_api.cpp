@@ -919,16 +919,19 @@ static int choose_factor(const crt_handle* h, FactorForm& F) {
   const bool gring = mb > 124 || crt_factor_lds_bytes(mb, es, false) > h->lds_max;
   const size_t lds = crt_factor_lds_bytes(mb, es, gring);
   if (lds > h->lds_max) return kErrBand;
-  // wave 0 holds the W = mb + kNB panel rows in ROWS slots per lane (slot 0
-  // holds 32 rows and four copies of the kNB diagonal rows, every further
-  // slot 64 rows); the other waves own the trailing window's 4x4 tiles beyond
-  // the look-ahead columns, MAXT each.  When one workgroup's LDS leaves no room for a second on the CU, the
-  // workgroup takes 512 threads instead of 256 (the wide-band form always).
+  // wave 0 holds the W = mb + kNB panel rows in ROWS slots per lane (256
+  // threads: slot 0 holds 32 rows and four copies of the kNB diagonal rows,
+  // every further slot 64 rows; 512 threads: 64 rows per slot); the other
+  // waves own the trailing window's 4x4 tiles beyond the look-ahead columns,
+  // MAXT each.  When one workgroup's LDS leaves no room for a second on the
+  // CU, the workgroup takes 512 threads instead of 256 (the wide-band form
+  // always).
   const int W = mb + kNB, tw = (mb + 3) / 4;
   const int nrest = tw > 2 ? ((tw - 2) * (tw - 1)) / 2 : 0;
   int nt = gring || 2 * lds > h->lds_max ? 512 : 256;
   if (!gring && (h->tune_nt == 256 || h->tune_nt == 512)) nt = h->tune_nt;
-  const int rows = W <= 40 ? 1 : 1 + (W - 40 + 63) / 64, maxt = std::max(1, (nrest + nt - 65) / (nt - 64));
+  const int rows = nt == 256 ? (W <= 40 ? 1 : 1 + (W - 40 + 63) / 64) : (W + 63) / 64;
+  const int maxt = std::max(1, (nrest + nt - 65) / (nt - 64));
 #define FORM(R, T, N, G)                                                   \
   (h->dc ? (F.kern = crt_factor_kernel<R, T, N, G, double>, F.many = nullptr) \
          : (F.kern = crt_factor_kernel<R, T, N, G, cplx>, F.many = crt_factor_many_kernel<R, T, N, G>))

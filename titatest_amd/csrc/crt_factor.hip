@@ -96,12 +96,16 @@ struct Ring {
 // (immediate offsets of the LDS loads).
 __device__ __forceinline__ int pix(int rw, int k) { return rw * kNB + (rw >> 2) + k; }
 
-// Panel rows of wave 0's lanes.  Slot 0: lanes 0-7 of every 16-lane row hold
-// the panel's diagonal rows 0-7 (four copies, so that a DPP row broadcast
-// reads diagonal row c from lane c), lanes 8-15 rows 8 + 8 g + (lane - 8) of
-// 16-lane row g (rows 8-39); slot q >= 1: row 40 + 64 (q - 1) + lane.  Slots
-// needed for the W = mb + kNB panel rows: 1 + ceil((W - 40) / 64).
+// Panel rows of wave 0's lanes.  Replicated layout (REP): slot 0: lanes 0-7
+// of every 16-lane row hold the panel's diagonal rows 0-7 (four copies, so
+// that a DPP row broadcast reads diagonal row c from lane c), lanes 8-15 rows
+// 8 + 8 g + (lane - 8) of 16-lane row g (rows 8-39); slot q >= 1: row 40 +
+// 64 (q - 1) + lane; slots needed for the W = mb + kNB panel rows: 1 +
+// ceil((W - 40) / 64).  Plain layout: row 64 q + lane, ceil(W / 64) slots, and
+// the diagonal rows in lanes 0-7 of slot 0 are read by lane reads.
+template <bool REP>
 __device__ __forceinline__ int prow(int lane, int q) {
+  if constexpr (!REP) return 64 * q + lane;
   const int j = lane & 15;
   return q == 0 ? (j < kNB ? j : kNB + kNB * (lane >> 4) + (j - kNB)) : 40 + 64 * (q - 1) + lane;
 }
@@ -449,8 +453,9 @@ __device__ __forceinline__ void ring_tile(const Stage<T>& C, const T* Tt, int rw
 //   out through range-checked buffer stores.  Broadcasts within the panel
 //   wave are DPP row broadcasts from the diagonal rows, which lanes 0-7 of
 //   every 16-lane row hold, or lane reads.
-// ROWS: panel row slots per lane of wave 0 (prow; W <= 40 + 64 (ROWS - 1));
-// MAXT:
+// ROWS: panel row slots per lane of wave 0 (prow; W <= 40 + 64 (ROWS - 1)
+// in the replicated layout of the 256-thread forms, W <= 64 ROWS in the plain
+// layout of the 512-thread forms, which hold one workgroup per CU); MAXT:
 // trailing tiles per thread of the update waves; NT threads per workgroup:
 // 256 (one panel and three update waves; two workgroups per CU, four for
 // the real entries) or, when one workgroup's LDS leaves no room for a second
@@ -471,6 +476,10 @@ __device__ __forceinline__ void factor_system(const FactorArgs& A, int s, double
   int16_t* so = reinterpret_cast<int16_t*>(sv + 2 * NB);  // their band offsets
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int x3 = tid - 64;                    // thread index within waves 1..
+  // panel layout: replicated diagonal rows with DPP broadcasts for the
+  // two-workgroup forms, plain rows with lane reads for the 512-thread forms
+  // (one workgroup per CU, where the panel's slot count sets the pace)
+  constexpr bool REP = NT == 256;
   const int nst = (n + kNB - 1) / kNB;
   // look-ahead: 2x2 tiles over window rows [0, mb) x columns [0, kNB), column
   // by column of tiles; tile column j holds the tile rows i >= j
@@ -529,7 +538,7 @@ __device__ __forceinline__ void factor_system(const FactorArgs& A, int s, double
   T yv[ROWS];
 #pragma unroll
   for (int q = 0; q < ROWS; ++q) {
-    const int rw = prow(lane, q);
+    const int rw = prow<REP>(lane, q);
     yv[q] = sel(wave == 0 && rw < W && rw < n, rhs(rw), szero<T>());
   }
   stage_barrier<GR>();
@@ -602,11 +611,16 @@ __device__ __forceinline__ void factor_system(const FactorArgs& A, int s, double
     if (wave == 0) {
       __builtin_amdgcn_s_setprio(3);
       const int rows = min(W, n - o);
-      const int lj = ln & 15;
+      const int lj = REP ? (ln & 15) : ln;
       const bool dg = lj < kNB;  // the lane's slot-0 row is a diagonal row
+      // lane c's value of a diagonal row to the whole wave
+      auto bc = [&](T x, int c) __attribute__((always_inline)) {
+        if constexpr (REP) return rbcast(x, c);
+        else return srl(x, c);
+      };
       int rwq[ROWS];
 #pragma unroll
-      for (int q = 0; q < ROWS; ++q) rwq[q] = prow(ln, q);
+      for (int q = 0; q < ROWS; ++q) rwq[q] = prow<REP>(ln, q);
       T v[ROWS][kNB];
 #pragma unroll
       for (int q = 0; q < ROWS; ++q) {
@@ -634,7 +648,7 @@ __device__ __forceinline__ void factor_system(const FactorArgs& A, int s, double
       bool zero_pivot = false;
 #pragma unroll
       for (int c = 0; c < kNB; ++c) {
-        const T piv = rbcast(v[0][c], c);
+        const T piv = bc(v[0][c], c);
         // chol :66 -- |s| <= 0 is an exact zero (a NaN pivot passes, as
         // there); choldc :66 -- s <= 0 for the real DC pivots
         if constexpr (std::is_same<T, cplx>::value)
@@ -647,7 +661,7 @@ __device__ __forceinline__ void factor_system(const FactorArgs& A, int s, double
         lrc[0] = sel(!dg || lj > c, smul(v[0][c], inv), szero<T>());
 #pragma unroll
         for (int q = 1; q < ROWS; ++q) lrc[q] = smul(v[q][c], inv);
-        const T yp = rbcast(yv[0], c);  // y'_c, final
+        const T yp = bc(yv[0], c);  // y'_c, final
         // With three or more row slots per lane each lane read is used
         // before the next one is issued (the updates pinned in order): the
         // compiler otherwise issues all of a column's reads at once and
@@ -656,12 +670,12 @@ __device__ __forceinline__ void factor_system(const FactorArgs& A, int s, double
 #pragma unroll
         for (int c2 = c + 1; c2 < kNB; ++c2) {
           T src = v[0][c];
-          if constexpr (ROWS >= 3) spin(src);
+          if constexpr (ROWS >= 3 || !REP) spin(src);
           const T s2 = srl(src, c2);  // s_{c2,c} of the diagonal block
 #pragma unroll
           for (int q = 0; q < ROWS; ++q) {
             v[q][c2] = msub(v[q][c2], lrc[q], s2);
-            if constexpr (ROWS >= 3) spin(v[q][c2]);
+            if constexpr (ROWS >= 3 || !REP) spin(v[q][c2]);
           }
         }
 #pragma unroll
@@ -673,7 +687,7 @@ __device__ __forceinline__ void factor_system(const FactorArgs& A, int s, double
       const T lcc = ssqrt(dd);
 #pragma unroll
       for (int c = 0; c < kNB; ++c) {
-        const T lc = rbcast(lcc, c);
+        const T lc = bc(lcc, c);
 #pragma unroll
         for (int q = 0; q < ROWS; ++q) v[q][c] = smul(v[q][c], lc);
         v[0][c] = sel(dg && lj == c, lc, v[0][c]);
@@ -690,7 +704,7 @@ __device__ __forceinline__ void factor_system(const FactorArgs& A, int s, double
 #pragma unroll
           for (int k = 0; k < kNB; ++k) Pc[pix(rw, k)] = v[q][k];
         }
-        const bool mine = q > 0 || !dg || ln < 16;
+        const bool mine = !REP || q > 0 || !dg || ln < 16;  // one copy of each row
         const int klo = max(rw - mb, 0), khi = min(rw, nbc - 1);
 #pragma unroll
         for (int k = 0; k < kNB; ++k) {
@@ -708,7 +722,10 @@ __device__ __forceinline__ void factor_system(const FactorArgs& A, int s, double
         const int r1 = rwq[q] + kNB;  // this window's row
         int sl;
         bool hi;
-        if (r1 < 40) {  // slot 0
+        if (!REP) {
+          sl = r1 & 63;
+          hi = (r1 >> 6) > q;
+        } else if (r1 < 40) {  // slot 0
           sl = 16 * ((r1 - kNB) >> 3) + kNB + ((r1 - kNB) & 7);
           hi = false;
         } else {
