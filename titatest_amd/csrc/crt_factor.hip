@@ -354,11 +354,12 @@ __device__ __forceinline__ void trailing_tiles(int mb, int (&rs_r0)[MAXT], int (
 // and the matrix) minus panel J-1's rank-kNB product, in ascending k.  slot
 // receives each entry's ring position for the caller's store: the ring's
 // discard slot where the entry does not exist, so that every store is
-// unconditional.  rw0 and cw0 are multiples of TR = TC (2 or 4), so the
-// tile's operand rows of panel J-1 (stage rows + kNB) lie in one 4-row group:
-// they share one swizzle key and sit kNB entries apart, and the panel buffer's
-// 4 pad rows take the groups that reach past its W rows (their products fall
-// on entries that are not stored).
+// unconditional.  rw0 and cw0 are multiples of TR = TC (2 or 4), so each of
+// the tile's two sets of operand rows of panel J-1 (stage rows + kNB) lies in
+// one 4-row group, its rows kNB entries apart (the panel buffer pads only
+// between groups), and the panel buffer's 4 pad rows take the groups that
+// reach past its W rows (their products fall on entries that are not
+// stored).
 template <int TR, int TC, class T>
 __device__ __forceinline__ void ring_tile(const Stage<T>& C, const T* Tt, int rw0, int cw0, T acc[TR][TC],
                                           int slot[TR][TC]) {
