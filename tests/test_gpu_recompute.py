@@ -11,7 +11,9 @@ back-transform, the adjoint gradient re-solves each pass's systems.
 * the entry points that need every wavenumber's potentials refuse with 1002
   (1000 for the forward's kpot and hpot outputs);
 * the headline 240x80 workload in passes of 6 wavenumbers: voltages bit for
-  bit, gradient to 1e-12.
+  bit, gradient to 1e-12;
+* the DC twin in passes: its voltages equal the stored DC forward's bit for
+  bit, and bsendc (which needs every potential) refuses with 1002.
 """
 import numpy as np
 import pytest
@@ -76,3 +78,28 @@ def test_recompute_headline_workload(hip_lib):
     np.testing.assert_array_equal(vr, v)
     assert rel_err(g1r, g1) < 1e-12
     assert rel_err(g0r, g0) < 1e-12
+
+
+@pytest.mark.parametrize("mesh", ["small_block", "small_dirichlet"])
+def test_recompute_dc_twin(hip_lib, mesh):
+    from titatest_amd import crt
+    w = getattr(meshgen, mesh)()
+    say("recompute DC", mesh)
+    g = crt.CRT(w.mesh)
+    g.rwaven(w.strnr, w.vnr)
+    g.precal()
+    g.forward_dc(w.sigma)
+    v, _ = g.bvolti(w.strnr, w.vnr)
+    for passes in (1, 4):
+        gr = crt.CRT(w.mesh)
+        gr.rwaven(w.strnr, w.vnr)
+        gr.precal()
+        gr.set_recompute(passes)
+        gr.forward_dc(w.sigma)
+        vr, _ = gr.bvolti(w.strnr, w.vnr)
+        np.testing.assert_array_equal(vr, v)
+        out = np.zeros(gr.nanz * gr.elanz)
+        assert gr.lib.crt_sensitivities_dc(gr.h, crt._ptr(out)) == 1002
+        with pytest.raises(crt.CRTError) as e:
+            gr.forward_dc(w.sigma, want_kpot=True)
+        assert e.value.code == 1000
