@@ -174,6 +174,18 @@ int crt_set_device_budget(crt_handle *h, uint64_t bytes);
  * (titatest_amd/autotune.py) times them.  No counterpart in the reference. */
 int crt_set_tuning(crt_handle *h, int32_t factor_threads, int32_t backsolve_depth);
 
+/* Kernel form of crt_sensitivities / crt_sensitivities_dc (bsens, bsensi,
+ * bsendc): 0 = the built-in rule, 1 = the pair form (per element and
+ * wavenumber, each distinct electrode pair's node differences, then one
+ * bilinear term per measurement), 2 = the Gram form (per element, the E x E
+ * matrix of every source pair's bilinear term summed over the wavenumbers on
+ * the FP64 matrix cores, then one double difference per measurement; E <=
+ * 128, else 1000).  The built-in rule takes the Gram form when E <= 128 and
+ * its lower triangle of 16 x 16 tiles holds at most twice as many entries as
+ * there are measurements.  Both forms agree to rounding.  No counterpart in
+ * the reference. */
+int crt_set_sens_form(crt_handle *h, int32_t form);
+
 /* Per-wavenumber recomputation (checkpointing) of the potentials for
  * problems whose K x eanz x sanz potentials should not stay in device memory.
  * wavenumbers_per_pass 0 (the default) stores every wavenumber's potentials.

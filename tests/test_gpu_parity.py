@@ -123,7 +123,8 @@ def test_sensitivities_many_distinct_pairs(crtmod, oracle_lib, nmeas):
     current and 728 potential pairs: the pair tables take 96,896 bytes of LDS,
     one workgroup per CU, the pair kernel's 512-thread form.  2,200 hold 1,437
     and 1,397 (184,448 bytes, above the CU's LDS): the per-measurement
-    kernel.  Both modes against the oracle."""
+    kernel (the pair form forced: the built-in rule takes the Gram form
+    here).  Both modes against the oracle."""
     w = meshgen.large_grf(nx_side=4, n_uniform=47, nz=10, eanz=48)
     E = w.mesh.eanz
     rng = np.random.default_rng(nmeas)
@@ -134,6 +135,7 @@ def test_sensitivities_many_distinct_pairs(crtmod, oracle_lib, nmeas):
     g.rwaven(st, vn)
     g.precal()
     g.forward(w.sigma)
+    g.set_sens_form(1)
     o = oracle_lib.Oracle(w.mesh)
     o.rwaven(st, vn)
     o.precal()

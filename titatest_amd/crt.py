@@ -37,7 +37,7 @@ EXPORTS = (
     "crt_sensitivities", "crt_adjoint_gradient", "crt_adjoint_gradient_dev", "crt_jhw_r",
     "crt_cg_solve", "crt_forward_dc", "crt_sensitivities_dc", "crt_set_boundary_values",
     "crt_smoothness", "crt_set_decouplings", "crt_set_graph", "crt_set_device_budget", "crt_set_tuning",
-    "crt_set_recompute",
+    "crt_set_recompute", "crt_set_sens_form",
     "crt_forward_many",
     "crt_error_info", "crt_set_timing", "crt_stage_times", "crt_factor_phase_ticks",
     "crt_kpot_device", "crt_dmisft", "crt_dmisft_dev", "crt_dmisft_dev_noalloc", "crt_set_model_map", "crt_bpar", "crt_bpar_dev",
@@ -104,6 +104,7 @@ def load_library(path: Path = LIB_PATH):
     L.crt_set_device_budget.argtypes = [vp, C.c_uint64]
     L.crt_set_tuning.argtypes = [vp, i32, i32]
     L.crt_set_recompute.argtypes = [vp, i32]
+    L.crt_set_sens_form.argtypes = [vp, i32]
     L.crt_forward_many.argtypes = [i32, vp, vp]
     L.crt_sensitivities_dc.argtypes = [vp, vp]
     L.crt_voltages.argtypes = [vp, i32, vp, vp, i32, vp, vp, vp]
@@ -353,6 +354,11 @@ class CRT:
         """Factor workgroup threads (0, 256, 512) and back-substitution
         prefetch depth (0, 8, 16) for this handle's forwards."""
         self._check("set_tuning", self.lib.crt_set_tuning(self.h, int(factor_threads), int(backsolve_depth)))
+
+    def set_sens_form(self, form):
+        """Sensitivity kernel form: 0 built-in rule, 1 pair form, 2 Gram
+        form on the matrix cores (crt_set_sens_form)."""
+        self._check("set_sens_form", self.lib.crt_set_sens_form(self.h, int(form)))
 
     def set_recompute(self, wavenumbers_per_pass):
         """Hold at most this many wavenumbers' potentials at once (0 = store

@@ -85,6 +85,8 @@ struct crt_handle {
   std::vector<int> pairs_key;
   std::vector<int2> sens_cp, sens_pp, sens_mp;
   std::vector<int> sens_ord;
+  std::vector<int4> sens_quad;  // (c1, c2, p1, p2) 0-based, -1 = none (Gram form)
+  int sens_form = 0;            // crt_set_sens_form
   // decoupled edges of the triangular smoothness operator (decouplings.dat):
   // element pairs (1-based) and their weight factors
   std::vector<int2> dec_pairs;
@@ -1479,6 +1481,13 @@ int crt_set_tuning(crt_handle* h, int32_t factor_threads, int32_t backsolve_dept
   return 0;
 }
 
+int crt_set_sens_form(crt_handle* h, int32_t form) {
+  if (!h || form < 0 || form > 2) return kErrArg;
+  if (form == 2 && h->E > kGramMaxE) return kErrArg;
+  h->sens_form = form;
+  return 0;
+}
+
 int crt_set_recompute(crt_handle* h, int32_t wavenumbers_per_pass) {
   if (!h || wavenumbers_per_pass < 0) return kErrArg;
   h->recompute = wavenumbers_per_pass;
@@ -1694,7 +1703,34 @@ static int run_sens(crt_handle* h, int32_t mode, const double2* sigma) {
         upload(h, "sens_mp", h->sens_mp.data(), h->sens_mp.size(), &dm) ||
         upload(h, "sens_ord", h->sens_ord.data(), h->sens_ord.size(), &dord))
       return kErrHip;
+    h->sens_quad.resize(h->nanz);
+    for (int i = 0; i < h->nanz; ++i) {
+      const int s1 = h->strnr[i], v1 = h->vnr[i];
+      h->sens_quad[i] = make_int4(s1 % 10000 - 1, s1 / 10000 - 1, v1 % 10000 - 1, v1 / 10000 - 1);
+    }
+    int4* dq;
+    if (upload(h, "sens_quad", h->sens_quad.data(), h->sens_quad.size(), &dq)) return kErrHip;
     h->pairs_key.swap(key);
+  }
+  // The Gram form (one E x E product per element and wavenumber on the
+  // matrix cores, E <= 128) when its tiles hold at most twice as many entries
+  // as there are measurements, else the pair form (crt_set_sens_form
+  // overrides)
+  const int gtiles = crt_gram_tiles(h->E);
+  const bool gram_fits = h->E <= kGramMaxE;
+  const bool gram = gram_fits && (h->sens_form == 2 || (h->sens_form == 0 && gtiles * 256 <= 2 * h->nanz));
+  if (gram) {
+    const int mt = gtiles <= 12 ? 3 : gtiles <= 24 ? 6 : 9;
+    void (*k)(SensArgs, const int4*) =
+        mt == 3 ? crt_sens_gram_kernel<3> : mt == 6 ? crt_sens_gram_kernel<6> : crt_sens_gram_kernel<9>;
+    size_t dyn = 0;
+    if (int rc = open_lds(h, reinterpret_cast<const void*>(k), &dyn)) return rc;
+    const size_t lds = crt_gram_lds_bytes(h->E);
+    if (lds > dyn) return kErrArg;
+    tstart(h, "sens");
+    hipLaunchKernelGGL(k, dim3(h->elanz), dim3(kGramNT), lds, h->stream, S, h->at<int4>("sens_quad"));
+    tstop(h);
+    return finish(h);
   }
   SensPairs Q;
   Q.ncp = (int)h->sens_cp.size();

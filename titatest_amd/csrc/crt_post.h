@@ -101,6 +101,22 @@ __global__ void crt_sens_kernel(SensArgs S);
 // RUNS: each thread takes a contiguous run of measurements (else interleaved)
 template <bool RUNS>
 __global__ void crt_sens_pairs_kernel(SensArgs S, SensPairs Q);
+// Gram form (FP64 matrix cores): 256 threads, MT tiles of 16 x 16 per
+// wavefront, quad[i] = (c1, c2, p1, p2) 0-based electrodes of measurement i
+// (-1 = none); dynamic LDS crt_gram_lds_bytes(eanz)
+constexpr int kGramMaxE = 128;
+constexpr int kGramNT = 256;
+template <int MT>
+__global__ void crt_sens_gram_kernel(SensArgs S, const int4* quad);
+inline int crt_gram_tiles(int eanz) {
+  const int tb = (eanz + 15) / 16;
+  return tb * (tb + 1) / 2;
+}
+inline size_t crt_gram_lds_bytes(int eanz) {
+  const size_t ep = 16 * (size_t)((eanz + 15) / 16);
+  const size_t stage = 2 * 8 * ep * 16, tiles = (size_t)crt_gram_tiles(eanz) * 256 * 8;
+  return stage > tiles ? stage : tiles;
+}
 __global__ void crt_kpot_transpose_kernel(const double2* kpot, int n, int E, double2* kpt);
 __global__ void crt_adj_gamma_kernel(int nanz, int mode, const double* w, const double2* resid,
                                      const double2* volt, double2* gamma);

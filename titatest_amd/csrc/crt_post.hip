@@ -759,6 +759,146 @@ template __global__ void crt_adj_gemm_cs_kernel<16>(const double2*, const double
 template __global__ void crt_adj_gemm_cs_kernel<24>(const double2*, const double2*, int, int, double2*);
 template __global__ void crt_adj_gemm_cs_kernel<32>(const double2*, const double2*, int, int, double2*);
 
+// Gram form of the Jacobian (bsens/bsensi, src/bsens.f90:137-187): for the
+// element e and wavenumber k, with P_k the potentials of every source
+// electrode at the element's nodes (4 x E) and S_k the element matrix
+// (elbg), every measurement's term is a double difference of one E x E
+// matrix, h = G[p2][c2] - G[p2][c1] - G[p1][c2] + G[p1][c1] with G = P_k^T
+// S_k P_k (the reference's sum over node pairs of elbg (sup2 - sup1)
+// (sup4 - sup3), expanded; an absent electrode's terms are 0).  G is
+// symmetric, so its lower triangle of 16 x 16 tiles holds it; the tiles
+// accumulate sum_k w_k G_k as rank-4 products on the FP64 matrix cores
+// (A = w_k S_k P_k, B = P_k: one v_mfma_f64_16x16x4f64 per real product,
+// four per complex tile and wavenumber), one wavefront per tile set.  Each
+// wavenumber's P_k and w_k S_k P_k are staged in LDS by one thread per
+// electrode (double-buffered, one barrier per wavenumber); at the end the
+// tiles go to LDS, real then imaginary part, and every thread reads the four
+// entries of its measurements.  MT: tiles per wavefront (ceil(TB (TB + 1) /
+// 8) for TB = ceil(E / 16) tile rows, E <= 16 TB <= 128).
+template <int MT>
+__global__ __launch_bounds__(kGramNT, MT > 6 ? 2 : 3) void crt_sens_gram_kernel(SensArgs S,
+                                                                           const int4* __restrict__ quad) {
+  extern __shared__ double2 sm[];
+  const int e = blockIdx.x;
+  const int E = S.eanz, TB = (E + 15) / 16, EP = 16 * TB, ntl = TB * (TB + 1) / 2;
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, col = lane & 15, kk = lane >> 4;
+  const int nk = S.el_nk[e];
+  // this wavefront's tiles x = w + 4 q of the lower triangle (row ti, column
+  // tj, x = ti (ti + 1) / 2 + tj): operand offsets in a staging buffer of
+  // P rows [4][EP] followed by the A rows [4][EP]
+  int oa[MT], ob[MT];
+  const int nq = ntl > w ? (ntl - w + 3) / 4 : 0;
+#pragma unroll
+  for (int q = 0; q < MT; ++q) {
+    const int x = w + 4 * q;
+    int ti = 0;
+    while ((ti + 1) * (ti + 2) / 2 <= x) ++ti;
+    const int tj = x - ti * (ti + 1) / 2;
+    oa[q] = 4 * EP + kk * EP + 16 * ti + col;
+    ob[q] = kk * EP + 16 * tj + col;
+  }
+  v4d gr[MT], gi[MT];
+#pragma unroll
+  for (int q = 0; q < MT; ++q) gr[q] = gi[q] = v4d{0.0, 0.0, 0.0, 0.0};
+  // without the wavenumber sum (swrtr = 0) the reference keeps one
+  // wavenumber's h (the last one, as the pair form)
+  const int k0 = S.swrtr == 0 ? S.K - 1 : 0;
+  int node[4];
+#pragma unroll
+  for (int b = 0; b < 4; ++b) node[b] = S.el_nodes[e * 4 + b];
+  cplx pc[4];
+  auto fetch = [&](int k) __attribute__((always_inline)) {
+#pragma unroll
+    for (int b = 0; b < 4; ++b)
+      pc[b] = (tid < E && b < nk) ? ldc(S.kpt + ((size_t)k * S.n + node[b]) * E + tid) : mkc(0.0, 0.0);
+  };
+  if (tid < EP) fetch(k0);
+  for (int k = k0; k < S.K; ++k) {
+    double2* buf = sm + (k & 1) * 8 * EP;
+    if (tid < EP) {
+      const double wk = S.swrtr == 0 ? 1.0 : S.kwnwi[k];
+      const double* el = S.elbg + ((size_t)k * S.elanz + e) * S.nimn;
+#pragma unroll
+      for (int a = 0; a < 4; ++a) {
+        cplx s = mkc(0.0, 0.0);
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+          const int imax = a > b ? a : b, imin = a > b ? b : a;
+          const double sab = (a < nk && b < nk) ? el[(imax * (imax + 1)) / 2 + imin] : 0.0;
+          s = mkc(fma(sab, pc[b].re, s.re), fma(sab, pc[b].im, s.im));
+        }
+        buf[a * EP + tid] = make_double2(pc[a].re, pc[a].im);
+        buf[4 * EP + a * EP + tid] = make_double2(wk * s.re, wk * s.im);
+      }
+      if (k + 1 < S.K) fetch(k + 1);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < MT; ++q) {
+      if (q < nq) {
+        const double2 a = buf[oa[q]], b = buf[ob[q]];
+        gr[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(a.x, b.x, gr[q], 0, 0, 0);
+        gr[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(-a.y, b.y, gr[q], 0, 0, 0);
+        gi[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(a.x, b.y, gi[q], 0, 0, 0);
+        gi[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(a.y, b.x, gi[q], 0, 0, 0);
+      }
+    }
+  }
+  // every wavefront's last staging reads are done before the tiles overwrite them
+  __syncthreads();
+  double* G = reinterpret_cast<double*>(sm);  // [ntl][16][16]
+  auto put = [&](const v4d (&t)[MT]) __attribute__((always_inline)) {
+#pragma unroll
+    for (int q = 0; q < MT; ++q)
+      if (q < nq) {
+        double* g = G + (w + 4 * q) * 256 + kk * 16 + col;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) g[64 * v] = t[q][v];
+      }
+  };
+  // G[a][b] for 0-based electrodes, 0 where either is absent (-1)
+  auto at = [&](int a, int b) __attribute__((always_inline)) {
+    const bool none = a < 0 || b < 0;
+    a = max(a, 0);
+    b = max(b, 0);
+    // the upper tile of (a, b) is the lower tile of (b, a); diagonal tiles
+    // hold both orders
+    const bool sw = (a >> 4) < (b >> 4);
+    const int r = sw ? b : a, c = sw ? a : b, ti = r >> 4, tj = c >> 4;
+    const double v = G[(ti * (ti + 1) / 2 + tj) * 256 + (r & 15) * 16 + (c & 15)];
+    return none ? 0.0 : v;
+  };
+  auto hsum = [&](int4 m) __attribute__((always_inline)) {
+    // m = (c1, c2, p1, p2)
+    return (at(m.w, m.y) - at(m.w, m.x)) - (at(m.z, m.y) - at(m.z, m.x));
+  };
+  // real parts first, parked in the real half of this element's output
+  // (read back by the same thread), then the imaginary parts
+  double2* out = S.sens + (size_t)e * S.nanz;
+  put(gr);
+  __syncthreads();
+  for (int i = tid; i < S.nanz; i += kGramNT) out[i].x = hsum(quad[i]);
+  __syncthreads();
+  put(gi);
+  __syncthreads();
+  const cplx sig = ldc(S.sigma + e);
+  for (int i = tid; i < S.nanz; i += kGramNT) {
+    const cplx acc = mkc(out[i].x, hsum(quad[i]));
+    cplx o;
+    if (S.mode == 0) {
+      cplx dum = (S.swrtr == 0) ? acc : rmul(1.0 / M_PI, acc);
+      o = mkc(0.0 - dum.re, 0.0 - dum.im);
+    } else {
+      cplx dum = (S.swrtr == 0) ? acc : cdiv(acc, mkc(M_PI, 0.0));
+      o = cdiv(cmul(dum, sig), ldc(S.volt + i));
+    }
+    stc(out + i, o);
+  }
+}
+template __global__ void crt_sens_gram_kernel<3>(SensArgs, const int4*);
+template __global__ void crt_sens_gram_kernel<6>(SensArgs, const int4*);
+template __global__ void crt_sens_gram_kernel<9>(SensArgs, const int4*);
+
 __global__ __launch_bounds__(256) void crt_adj_gemm_kernel(const double2* __restrict__ kpot,
                                                          const double2* __restrict__ C, int n, int eanz,
                                                          double2* __restrict__ Y) {
