@@ -85,7 +85,7 @@ struct crt_handle {
   std::vector<int> pairs_key;
   std::vector<int2> sens_cp, sens_pp, sens_mp;
   std::vector<int> sens_ord;
-  std::vector<int4> sens_quad;  // (c1, c2, p1, p2) 0-based, -1 = none (Gram form)
+  std::vector<uint2> sens_goff;  // Gram form: tile-buffer offsets of each measurement's terms
   int sens_form = 0;            // crt_set_sens_form
   // decoupled edges of the triangular smoothness operator (decouplings.dat):
   // element pairs (1-based) and their weight factors
@@ -1703,13 +1703,18 @@ static int run_sens(crt_handle* h, int32_t mode, const double2* sigma) {
         upload(h, "sens_mp", h->sens_mp.data(), h->sens_mp.size(), &dm) ||
         upload(h, "sens_ord", h->sens_ord.data(), h->sens_ord.size(), &dord))
       return kErrHip;
-    h->sens_quad.resize(h->nanz);
+    // the Gram form's h = G[p2][c2] - G[p2][c1] - (G[p1][c2] - G[p1][c1])
+    // (0-based electrodes, -1 = none) as four tile-buffer offsets
+    h->sens_goff.resize(h->nanz);
     for (int i = 0; i < h->nanz; ++i) {
-      const int s1 = h->strnr[i], v1 = h->vnr[i];
-      h->sens_quad[i] = make_int4(s1 % 10000 - 1, s1 / 10000 - 1, v1 % 10000 - 1, v1 / 10000 - 1);
+      const int c1 = h->strnr[i] % 10000 - 1, c2 = h->strnr[i] / 10000 - 1;
+      const int p1 = h->vnr[i] % 10000 - 1, p2 = h->vnr[i] / 10000 - 1;
+      const int E = h->E;
+      h->sens_goff[i] = make_uint2((unsigned)crt_gram_offset(E, p2, c2) | (unsigned)crt_gram_offset(E, p2, c1) << 16,
+                                   (unsigned)crt_gram_offset(E, p1, c2) | (unsigned)crt_gram_offset(E, p1, c1) << 16);
     }
-    int4* dq;
-    if (upload(h, "sens_quad", h->sens_quad.data(), h->sens_quad.size(), &dq)) return kErrHip;
+    uint2* dq;
+    if (upload(h, "sens_goff", h->sens_goff.data(), h->sens_goff.size(), &dq)) return kErrHip;
     h->pairs_key.swap(key);
   }
   // The Gram form (one E x E product per element and wavenumber on the
@@ -1720,15 +1725,16 @@ static int run_sens(crt_handle* h, int32_t mode, const double2* sigma) {
   const bool gram_fits = h->E <= kGramMaxE;
   const bool gram = gram_fits && (h->sens_form == 2 || (h->sens_form == 0 && gtiles * 256 <= 2 * h->nanz));
   if (gram) {
-    const int mt = gtiles <= 12 ? 3 : gtiles <= 24 ? 6 : 9;
-    void (*k)(SensArgs, const int4*) =
-        mt == 3 ? crt_sens_gram_kernel<3> : mt == 6 ? crt_sens_gram_kernel<6> : crt_sens_gram_kernel<9>;
+    // tiles per wavefront: ceil(gtiles / 4)
+    void (*k)(SensArgs, const uint2*) = gtiles <= 12   ? crt_sens_gram_kernel<3>
+                                        : gtiles <= 24 ? crt_sens_gram_kernel<6>
+                                                       : crt_sens_gram_kernel<9>;
     size_t dyn = 0;
     if (int rc = open_lds(h, reinterpret_cast<const void*>(k), &dyn)) return rc;
     const size_t lds = crt_gram_lds_bytes(h->E);
     if (lds > dyn) return kErrArg;
     tstart(h, "sens");
-    hipLaunchKernelGGL(k, dim3(h->elanz), dim3(kGramNT), lds, h->stream, S, h->at<int4>("sens_quad"));
+    hipLaunchKernelGGL(k, dim3(h->elanz), dim3(kGramNT), lds, h->stream, S, h->at<uint2>("sens_goff"));
     tstop(h);
     return finish(h);
   }

@@ -101,20 +101,36 @@ __global__ void crt_sens_kernel(SensArgs S);
 // RUNS: each thread takes a contiguous run of measurements (else interleaved)
 template <bool RUNS>
 __global__ void crt_sens_pairs_kernel(SensArgs S, SensPairs Q);
-// Gram form (FP64 matrix cores): 256 threads, MT tiles of 16 x 16 per
-// wavefront, quad[i] = (c1, c2, p1, p2) 0-based electrodes of measurement i
-// (-1 = none); dynamic LDS crt_gram_lds_bytes(eanz)
+// Gram form (FP64 matrix cores): kGramNT threads, MT tiles of 16 x 16 per
+// wavefront; off[i] = the tile-buffer offsets of measurement i's four terms
+// (crt_gram_offset), two 16-bit offsets per word; dynamic LDS
+// crt_gram_lds_bytes(eanz)
 constexpr int kGramMaxE = 128;
 constexpr int kGramNT = 256;
+constexpr int kGramQC = 4;  // measurements per thread per extraction pass
 template <int MT>
-__global__ void crt_sens_gram_kernel(SensArgs S, const int4* quad);
+__global__ void crt_sens_gram_kernel(SensArgs S, const uint2* off);
 inline int crt_gram_tiles(int eanz) {
   const int tb = (eanz + 15) / 16;
   return tb * (tb + 1) / 2;
 }
+// Offset (in doubles) of G[a][b] in the kernel's buffer of lower-triangle
+// tiles (tile x = ti (ti + 1) / 2 + tj row-major, 256 entries each) for
+// 0-based electrodes a, b; the zero slot after the tiles where either is
+// absent (-1).  G is symmetric: (a, b) in an upper tile reads (b, a).
+inline int crt_gram_offset(int eanz, int a, int b) {
+  if (a < 0 || b < 0) return crt_gram_tiles(eanz) * 256;
+  if ((a >> 4) < (b >> 4)) {
+    const int t = a;
+    a = b;
+    b = t;
+  }
+  const int ti = a >> 4, tj = b >> 4;
+  return (ti * (ti + 1) / 2 + tj) * 256 + (a & 15) * 16 + (b & 15);
+}
 inline size_t crt_gram_lds_bytes(int eanz) {
   const size_t ep = 16 * (size_t)((eanz + 15) / 16);
-  const size_t stage = 2 * 8 * ep * 16, tiles = (size_t)crt_gram_tiles(eanz) * 256 * 8;
+  const size_t stage = 2 * 8 * ep * 16, tiles = (size_t)crt_gram_tiles(eanz) * 256 * 8 + 16;
   return stage > tiles ? stage : tiles;
 }
 __global__ void crt_kpot_transpose_kernel(const double2* kpot, int n, int E, double2* kpt);

@@ -771,26 +771,28 @@ template __global__ void crt_adj_gemm_cs_kernel<32>(const double2*, const double
 // (A = w_k S_k P_k, B = P_k: one v_mfma_f64_16x16x4f64 per real product,
 // four per complex tile and wavenumber), one wavefront per tile set.  Each
 // wavenumber's P_k and w_k S_k P_k are staged in LDS by one thread per
-// electrode (double-buffered, one barrier per wavenumber); at the end the
-// tiles go to LDS, real then imaginary part, and every thread reads the four
-// entries of its measurements.  MT: tiles per wavefront (ceil(TB (TB + 1) /
+// electrode (double-buffered, one barrier per wavenumber); at the end, per
+// pass of kGramQC measurements per thread, the tiles go to LDS, real then
+// imaginary part, and every thread reads its measurements' four terms at
+// offsets the host computed (a zero slot for absent electrodes).  MT: tiles per wavefront (ceil(TB (TB + 1) /
 // 8) for TB = ceil(E / 16) tile rows, E <= 16 TB <= 128).
 template <int MT>
 __global__ __launch_bounds__(kGramNT, MT > 6 ? 2 : 3) void crt_sens_gram_kernel(SensArgs S,
-                                                                           const int4* __restrict__ quad) {
+                                                                           const uint2* __restrict__ off) {
+  constexpr int NT = kGramNT, NW = NT / 64;
   extern __shared__ double2 sm[];
   const int e = blockIdx.x;
   const int E = S.eanz, TB = (E + 15) / 16, EP = 16 * TB, ntl = TB * (TB + 1) / 2;
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, col = lane & 15, kk = lane >> 4;
   const int nk = S.el_nk[e];
-  // this wavefront's tiles x = w + 4 q of the lower triangle (row ti, column
+  // this wavefront's tiles x = w + NW q of the lower triangle (row ti, column
   // tj, x = ti (ti + 1) / 2 + tj): operand offsets in a staging buffer of
   // P rows [4][EP] followed by the A rows [4][EP]
   int oa[MT], ob[MT];
-  const int nq = ntl > w ? (ntl - w + 3) / 4 : 0;
+  const int nq = ntl > w ? (ntl - w + NW - 1) / NW : 0;
 #pragma unroll
   for (int q = 0; q < MT; ++q) {
-    const int x = w + 4 * q;
+    const int x = w + NW * q;
     int ti = 0;
     while ((ti + 1) * (ti + 2) / 2 <= x) ++ti;
     const int tj = x - ti * (ti + 1) / 2;
@@ -846,58 +848,55 @@ __global__ __launch_bounds__(kGramNT, MT > 6 ? 2 : 3) void crt_sens_gram_kernel(
   }
   // every wavefront's last staging reads are done before the tiles overwrite them
   __syncthreads();
-  double* G = reinterpret_cast<double*>(sm);  // [ntl][16][16]
+  double* G = reinterpret_cast<double*>(sm);  // [ntl][16][16], then the zero slot
+  if (tid == 0) G[ntl * 256] = 0.0;
   auto put = [&](const v4d (&t)[MT]) __attribute__((always_inline)) {
 #pragma unroll
     for (int q = 0; q < MT; ++q)
       if (q < nq) {
-        double* g = G + (w + 4 * q) * 256 + kk * 16 + col;
+        double* g = G + (w + NW * q) * 256 + kk * 16 + col;
 #pragma unroll
         for (int v = 0; v < 4; ++v) g[64 * v] = t[q][v];
       }
   };
-  // G[a][b] for 0-based electrodes, 0 where either is absent (-1)
-  auto at = [&](int a, int b) __attribute__((always_inline)) {
-    const bool none = a < 0 || b < 0;
-    a = max(a, 0);
-    b = max(b, 0);
-    // the upper tile of (a, b) is the lower tile of (b, a); diagonal tiles
-    // hold both orders
-    const bool sw = (a >> 4) < (b >> 4);
-    const int r = sw ? b : a, c = sw ? a : b, ti = r >> 4, tj = c >> 4;
-    const double v = G[(ti * (ti + 1) / 2 + tj) * 256 + (r & 15) * 16 + (c & 15)];
-    return none ? 0.0 : v;
+  // h from its four terms' offsets (crt_gram_offset, two 16-bit offsets per word)
+  auto hsum = [&](uint2 o) __attribute__((always_inline)) {
+    return (G[o.x & 0xffff] - G[o.x >> 16]) - (G[o.y & 0xffff] - G[o.y >> 16]);
   };
-  auto hsum = [&](int4 m) __attribute__((always_inline)) {
-    // m = (c1, c2, p1, p2)
-    return (at(m.w, m.y) - at(m.w, m.x)) - (at(m.z, m.y) - at(m.z, m.x));
-  };
-  // real parts first, parked in the real half of this element's output
-  // (read back by the same thread), then the imaginary parts
-  double2* out = S.sens + (size_t)e * S.nanz;
-  put(gr);
-  __syncthreads();
-  for (int i = tid; i < S.nanz; i += kGramNT) out[i].x = hsum(quad[i]);
-  __syncthreads();
-  put(gi);
-  __syncthreads();
   const cplx sig = ldc(S.sigma + e);
-  for (int i = tid; i < S.nanz; i += kGramNT) {
-    const cplx acc = mkc(out[i].x, hsum(quad[i]));
-    cplx o;
-    if (S.mode == 0) {
-      cplx dum = (S.swrtr == 0) ? acc : rmul(1.0 / M_PI, acc);
-      o = mkc(0.0 - dum.re, 0.0 - dum.im);
-    } else {
-      cplx dum = (S.swrtr == 0) ? acc : cdiv(acc, mkc(M_PI, 0.0));
-      o = cdiv(cmul(dum, sig), ldc(S.volt + i));
+  double2* out = S.sens + (size_t)e * S.nanz;
+  // kGramQC measurements per thread per pass: the real parts of the pass,
+  // then the imaginary parts (offsets read again: fewer registers held)
+  for (int i0 = 0; i0 < S.nanz; i0 += kGramQC * NT) {
+    double hr[kGramQC];
+    put(gr);
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < kGramQC; ++j) hr[j] = hsum(off[min(i0 + j * NT + tid, S.nanz - 1)]);
+    __syncthreads();
+    put(gi);
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < kGramQC; ++j) {
+      const int i = i0 + j * NT + tid;
+      if (i >= S.nanz) continue;
+      const cplx acc = mkc(hr[j], hsum(off[i]));
+      cplx r;
+      if (S.mode == 0) {
+        cplx dum = (S.swrtr == 0) ? acc : rmul(1.0 / M_PI, acc);
+        r = mkc(0.0 - dum.re, 0.0 - dum.im);
+      } else {
+        cplx dum = (S.swrtr == 0) ? acc : cdiv(acc, mkc(M_PI, 0.0));
+        r = cdiv(cmul(dum, sig), ldc(S.volt + i));
+      }
+      stc(out + i, r);
     }
-    stc(out + i, o);
+    __syncthreads();
   }
 }
-template __global__ void crt_sens_gram_kernel<3>(SensArgs, const int4*);
-template __global__ void crt_sens_gram_kernel<6>(SensArgs, const int4*);
-template __global__ void crt_sens_gram_kernel<9>(SensArgs, const int4*);
+template __global__ void crt_sens_gram_kernel<3>(SensArgs, const uint2*);
+template __global__ void crt_sens_gram_kernel<6>(SensArgs, const uint2*);
+template __global__ void crt_sens_gram_kernel<9>(SensArgs, const uint2*);
 
 __global__ __launch_bounds__(256) void crt_adj_gemm_kernel(const double2* __restrict__ kpot,
                                                          const double2* __restrict__ C, int n, int eanz,
