@@ -86,6 +86,7 @@ struct crt_handle {
   std::vector<int2> sens_cp, sens_pp, sens_mp;
   std::vector<int> sens_ord;
   std::vector<uint2> sens_goff;  // Gram form: tile-buffer offsets of each measurement's terms
+  std::vector<int> sens_gtiles;  // Gram form: the tiles they read
   int sens_form = 0;            // crt_set_sens_form
   // decoupled edges of the triangular smoothness operator (decouplings.dat):
   // element pairs (1-based) and their weight factors
@@ -1704,37 +1705,49 @@ static int run_sens(crt_handle* h, int32_t mode, const double2* sigma) {
         upload(h, "sens_ord", h->sens_ord.data(), h->sens_ord.size(), &dord))
       return kErrHip;
     // the Gram form's h = G[p2][c2] - G[p2][c1] - (G[p1][c2] - G[p1][c1])
-    // (0-based electrodes, -1 = none) as four tile-buffer offsets
-    h->sens_goff.resize(h->nanz);
-    for (int i = 0; i < h->nanz; ++i) {
+    // (0-based electrodes, -1 = none) as four tile-buffer offsets (below
+    // 2^16 for E <= kGramMaxE; larger E does not take the Gram form)
+    h->sens_goff.assign(h->E <= kGramMaxE ? h->nanz : 1, make_uint2(0, 0));
+    for (int i = 0; i < h->nanz && h->E <= kGramMaxE; ++i) {
       const int c1 = h->strnr[i] % 10000 - 1, c2 = h->strnr[i] / 10000 - 1;
       const int p1 = h->vnr[i] % 10000 - 1, p2 = h->vnr[i] / 10000 - 1;
       const int E = h->E;
       h->sens_goff[i] = make_uint2((unsigned)crt_gram_offset(E, p2, c2) | (unsigned)crt_gram_offset(E, p2, c1) << 16,
                                    (unsigned)crt_gram_offset(E, p1, c2) | (unsigned)crt_gram_offset(E, p1, c1) << 16);
     }
+    // the tiles those terms read (the rest of G is never formed)
+    std::vector<char> used(crt_gram_tiles(std::min(h->E, kGramMaxE)) + 1, 0);
+    for (const uint2& o : h->sens_goff)
+      for (unsigned v : {o.x & 0xffffu, o.x >> 16, o.y & 0xffffu, o.y >> 16}) used[v / 256] = 1;
+    h->sens_gtiles.clear();
+    for (int x = 0; x + 1 < (int)used.size(); ++x)
+      if (used[x]) h->sens_gtiles.push_back(x);
+    if (h->sens_gtiles.empty()) h->sens_gtiles.push_back(0);
     uint2* dq;
-    if (upload(h, "sens_goff", h->sens_goff.data(), h->sens_goff.size(), &dq)) return kErrHip;
+    int* dt;
+    if (upload(h, "sens_goff", h->sens_goff.data(), h->sens_goff.size(), &dq) ||
+        upload(h, "sens_gtiles", h->sens_gtiles.data(), h->sens_gtiles.size(), &dt))
+      return kErrHip;
     h->pairs_key.swap(key);
   }
   // The Gram form (one E x E product per element and wavenumber on the
-  // matrix cores, E <= 128) when its tiles hold at most twice as many entries
-  // as there are measurements, else the pair form (crt_set_sens_form
-  // overrides)
-  const int gtiles = crt_gram_tiles(h->E);
+  // matrix cores, E <= 128) when the tiles its measurements read hold at most
+  // twice as many entries as there are measurements, else the pair form
+  // (crt_set_sens_form overrides)
+  const int gtiles = (int)h->sens_gtiles.size();
   const bool gram_fits = h->E <= kGramMaxE;
   const bool gram = gram_fits && (h->sens_form == 2 || (h->sens_form == 0 && gtiles * 256 <= 2 * h->nanz));
   if (gram) {
     // tiles per wavefront: ceil(gtiles / 4)
-    void (*k)(SensArgs, const uint2*) = gtiles <= 12   ? crt_sens_gram_kernel<3>
-                                        : gtiles <= 24 ? crt_sens_gram_kernel<6>
-                                                       : crt_sens_gram_kernel<9>;
+    void (*k)(SensArgs, const uint2*, const int*, int) =
+        gtiles <= 12 ? crt_sens_gram_kernel<3> : gtiles <= 24 ? crt_sens_gram_kernel<6> : crt_sens_gram_kernel<9>;
     size_t dyn = 0;
     if (int rc = open_lds(h, reinterpret_cast<const void*>(k), &dyn)) return rc;
     const size_t lds = crt_gram_lds_bytes(h->E);
     if (lds > dyn) return kErrArg;
     tstart(h, "sens");
-    hipLaunchKernelGGL(k, dim3(h->elanz), dim3(kGramNT), lds, h->stream, S, h->at<uint2>("sens_goff"));
+    hipLaunchKernelGGL(k, dim3(h->elanz), dim3(kGramNT), lds, h->stream, S, h->at<uint2>("sens_goff"),
+                       h->at<int>("sens_gtiles"), gtiles);
     tstop(h);
     return finish(h);
   }

@@ -103,13 +103,13 @@ template <bool RUNS>
 __global__ void crt_sens_pairs_kernel(SensArgs S, SensPairs Q);
 // Gram form (FP64 matrix cores): kGramNT threads, MT tiles of 16 x 16 per
 // wavefront; off[i] = the tile-buffer offsets of measurement i's four terms
-// (crt_gram_offset), two 16-bit offsets per word; dynamic LDS
-// crt_gram_lds_bytes(eanz)
+// (crt_gram_offset), two 16-bit offsets per word; tiles = the nt tiles those
+// offsets fall in, ascending; dynamic LDS crt_gram_lds_bytes(eanz)
 constexpr int kGramMaxE = 128;
 constexpr int kGramNT = 256;
 constexpr int kGramQC = 4;  // measurements per thread per extraction pass
 template <int MT>
-__global__ void crt_sens_gram_kernel(SensArgs S, const uint2* off);
+__global__ void crt_sens_gram_kernel(SensArgs S, const uint2* off, const int* tiles, int nt);
 inline int crt_gram_tiles(int eanz) {
   const int tb = (eanz + 15) / 16;
   return tb * (tb + 1) / 2;

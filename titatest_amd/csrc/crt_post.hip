@@ -778,21 +778,25 @@ template __global__ void crt_adj_gemm_cs_kernel<32>(const double2*, const double
 // 8) for TB = ceil(E / 16) tile rows, E <= 16 TB <= 128).
 template <int MT>
 __global__ __launch_bounds__(kGramNT, MT > 6 ? 2 : 3) void crt_sens_gram_kernel(SensArgs S,
-                                                                           const uint2* __restrict__ off) {
+                                                                           const uint2* __restrict__ off,
+                                                                           const int* __restrict__ tiles,
+                                                                           int nt) {
   constexpr int NT = kGramNT, NW = NT / 64;
   extern __shared__ double2 sm[];
   const int e = blockIdx.x;
   const int E = S.eanz, TB = (E + 15) / 16, EP = 16 * TB, ntl = TB * (TB + 1) / 2;
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, col = lane & 15, kk = lane >> 4;
   const int nk = S.el_nk[e];
-  // this wavefront's tiles x = w + NW q of the lower triangle (row ti, column
-  // tj, x = ti (ti + 1) / 2 + tj): operand offsets in a staging buffer of
-  // P rows [4][EP] followed by the A rows [4][EP]
-  int oa[MT], ob[MT];
-  const int nq = ntl > w ? (ntl - w + NW - 1) / NW : 0;
+  // this wavefront's tiles tiles[w + NW q] of the lower triangle (row ti,
+  // column tj, x = ti (ti + 1) / 2 + tj; only the tiles that some
+  // measurement reads): operand offsets in a staging buffer of P rows
+  // [4][EP] followed by the A rows [4][EP]
+  int oa[MT], ob[MT], xq[MT];
+  const int nq = nt > w ? (nt - w + NW - 1) / NW : 0;
 #pragma unroll
   for (int q = 0; q < MT; ++q) {
-    const int x = w + NW * q;
+    const int x = tiles[min(w + NW * q, nt - 1)];
+    xq[q] = x;
     int ti = 0;
     while ((ti + 1) * (ti + 2) / 2 <= x) ++ti;
     const int tj = x - ti * (ti + 1) / 2;
@@ -854,7 +858,7 @@ __global__ __launch_bounds__(kGramNT, MT > 6 ? 2 : 3) void crt_sens_gram_kernel(
 #pragma unroll
     for (int q = 0; q < MT; ++q)
       if (q < nq) {
-        double* g = G + (w + NW * q) * 256 + kk * 16 + col;
+        double* g = G + xq[q] * 256 + kk * 16 + col;
 #pragma unroll
         for (int v = 0; v < 4; ++v) g[64 * v] = t[q][v];
       }
@@ -894,9 +898,9 @@ __global__ __launch_bounds__(kGramNT, MT > 6 ? 2 : 3) void crt_sens_gram_kernel(
     __syncthreads();
   }
 }
-template __global__ void crt_sens_gram_kernel<3>(SensArgs, const uint2*);
-template __global__ void crt_sens_gram_kernel<6>(SensArgs, const uint2*);
-template __global__ void crt_sens_gram_kernel<9>(SensArgs, const uint2*);
+template __global__ void crt_sens_gram_kernel<3>(SensArgs, const uint2*, const int*, int);
+template __global__ void crt_sens_gram_kernel<6>(SensArgs, const uint2*, const int*, int);
+template __global__ void crt_sens_gram_kernel<9>(SensArgs, const uint2*, const int*, int);
 
 __global__ __launch_bounds__(256) void crt_adj_gemm_kernel(const double2* __restrict__ kpot,
                                                          const double2* __restrict__ C, int n, int eanz,
