@@ -7,12 +7,15 @@ reference's golden rows at 1e-8 relative, and against the pair form
 
 * the 861-node block (20 electrodes, one tile row), both modes, golden rows;
 * pole configurations (electrode 0 in a current or potential pair);
-* random quadrupoles on 37 electrodes (a partly filled third tile row) and on
-  128 (eight tile rows, nine tiles per wavefront), both modes;
+* random quadrupoles on 37 electrodes (a partly filled third tile row, three
+  tiles per wavefront, Gauss's three products), 96 (all 21 tiles, six per
+  wavefront, four products) and 128 (36 tiles, nine per wavefront), both
+  modes;
 * no wavenumber sum (swrtr 0, one wavenumber);
 * the DC twin (bsendc);
-* the 240 x 80 workload (96 electrodes, 3,090 quadrupoles), where the
-  built-in rule takes the Gram form: equal to the pair form.
+* the 240 x 80 workload (96 electrodes, 3,090 quadrupoles, 18 of the 21
+  tiles formed, five per wavefront), where the built-in rule takes the Gram
+  form: equal to the pair form.
 """
 import numpy as np
 import pytest
@@ -85,7 +88,7 @@ def test_gram_pole_configurations(crtmod, oracle_lib):
         assert rel_err(jg, jp) < AGREE
 
 
-@pytest.mark.parametrize("E,nmeas", [(37, 700), (128, 1500)])
+@pytest.mark.parametrize("E,nmeas", [(37, 700), (96, 1500), (128, 1500)])
 def test_gram_random_quadrupoles(crtmod, oracle_lib, E, nmeas):
     w = meshgen.large_grf(nx_side=4, n_uniform=E, nz=8, eanz=E)
     rng = np.random.default_rng(E)

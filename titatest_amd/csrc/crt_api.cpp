@@ -1739,8 +1739,10 @@ static int run_sens(crt_handle* h, int32_t mode, const double2* sigma) {
   const bool gram = gram_fits && (h->sens_form == 2 || (h->sens_form == 0 && gtiles * 256 <= 2 * h->nanz));
   if (gram) {
     // tiles per wavefront: ceil(gtiles / 4)
-    void (*k)(SensArgs, const uint2*, const int*, int) =
-        gtiles <= 12 ? crt_sens_gram_kernel<3> : gtiles <= 24 ? crt_sens_gram_kernel<6> : crt_sens_gram_kernel<9>;
+    void (*k)(SensArgs, const uint2*, const int*, int) = gtiles <= 12   ? crt_sens_gram_kernel<3>
+                                                         : gtiles <= 20 ? crt_sens_gram_kernel<5>
+                                                         : gtiles <= 24 ? crt_sens_gram_kernel<6>
+                                                                        : crt_sens_gram_kernel<9>;
     size_t dyn = 0;
     if (int rc = open_lds(h, reinterpret_cast<const void*>(k), &dyn)) return rc;
     const size_t lds = crt_gram_lds_bytes(h->E);

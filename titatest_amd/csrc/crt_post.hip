@@ -769,13 +769,14 @@ template __global__ void crt_adj_gemm_cs_kernel<32>(const double2*, const double
 // symmetric, so its lower triangle of 16 x 16 tiles holds it; the tiles
 // accumulate sum_k w_k G_k as rank-4 products on the FP64 matrix cores
 // (A = w_k S_k P_k, B = P_k: one v_mfma_f64_16x16x4f64 per real product,
-// four per complex tile and wavenumber), one wavefront per tile set.  Each
+// three or four per complex tile and wavenumber), one wavefront per tile set.  Each
 // wavenumber's P_k and w_k S_k P_k are staged in LDS by one thread per
 // electrode (double-buffered, one barrier per wavenumber); at the end, per
 // pass of kGramQC measurements per thread, the tiles go to LDS, real then
 // imaginary part, and every thread reads its measurements' four terms at
-// offsets the host computed (a zero slot for absent electrodes).  MT: tiles per wavefront (ceil(TB (TB + 1) /
-// 8) for TB = ceil(E / 16) tile rows, E <= 16 TB <= 128).
+// offsets the host computed (a zero slot for absent electrodes).  MT: tiles
+// per wavefront, at least a quarter of the formed tiles (at most TB (TB + 1)
+// / 2 for TB = ceil(E / 16) tile rows, E <= 16 TB <= 128).
 template <int MT>
 __global__ __launch_bounds__(kGramNT, MT > 6 ? 2 : 3) void crt_sens_gram_kernel(SensArgs S,
                                                                            const uint2* __restrict__ off,
@@ -803,9 +804,14 @@ __global__ __launch_bounds__(kGramNT, MT > 6 ? 2 : 3) void crt_sens_gram_kernel(
     oa[q] = 4 * EP + kk * EP + 16 * ti + col;
     ob[q] = kk * EP + 16 * tj + col;
   }
-  v4d gr[MT], gi[MT];
+  // complex tile products: up to five tiles per wavefront Gauss's three real
+  // products (Pr Qr, Pi Qi, (Pr + Pi)(Qr + Qi)) in three accumulators,
+  // which the registers hold at three workgroups per CU; more tiles the four
+  // products of the plain form in two
+  constexpr bool kGauss = MT <= 5;
+  v4d gr[MT], gi[MT], gs[kGauss ? MT : 1];
 #pragma unroll
-  for (int q = 0; q < MT; ++q) gr[q] = gi[q] = v4d{0.0, 0.0, 0.0, 0.0};
+  for (int q = 0; q < MT; ++q) gr[q] = gi[q] = gs[kGauss ? q : 0] = v4d{0.0, 0.0, 0.0, 0.0};
   // without the wavenumber sum (swrtr = 0) the reference keeps one
   // wavenumber's h (the last one, as the pair form)
   const int k0 = S.swrtr == 0 ? S.K - 1 : 0;
@@ -843,15 +849,30 @@ __global__ __launch_bounds__(kGramNT, MT > 6 ? 2 : 3) void crt_sens_gram_kernel(
     for (int q = 0; q < MT; ++q) {
       if (q < nq) {
         const double2 a = buf[oa[q]], b = buf[ob[q]];
-        gr[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(a.x, b.x, gr[q], 0, 0, 0);
-        gr[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(-a.y, b.y, gr[q], 0, 0, 0);
-        gi[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(a.x, b.y, gi[q], 0, 0, 0);
-        gi[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(a.y, b.x, gi[q], 0, 0, 0);
+        if constexpr (kGauss) {
+          gr[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(a.x, b.x, gr[q], 0, 0, 0);
+          gi[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(a.y, b.y, gi[q], 0, 0, 0);
+          gs[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(a.x + a.y, b.x + b.y, gs[q], 0, 0, 0);
+        } else {
+          gr[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(a.x, b.x, gr[q], 0, 0, 0);
+          gr[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(-a.y, b.y, gr[q], 0, 0, 0);
+          gi[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(a.x, b.y, gi[q], 0, 0, 0);
+          gi[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(a.y, b.x, gi[q], 0, 0, 0);
+        }
       }
     }
   }
   // every wavefront's last staging reads are done before the tiles overwrite them
   __syncthreads();
+  if constexpr (kGauss) {
+    // re = Pr Qr - Pi Qi, im = (Pr + Pi)(Qr + Qi) - Pr Qr - Pi Qi
+#pragma unroll
+    for (int q = 0; q < MT; ++q) {
+      gs[q] = gs[q] - gr[q] - gi[q];
+      gr[q] = gr[q] - gi[q];
+      gi[q] = gs[q];
+    }
+  }
   double* G = reinterpret_cast<double*>(sm);  // [ntl][16][16], then the zero slot
   if (tid == 0) G[ntl * 256] = 0.0;
   auto put = [&](const v4d (&t)[MT]) __attribute__((always_inline)) {
@@ -899,6 +920,7 @@ __global__ __launch_bounds__(kGramNT, MT > 6 ? 2 : 3) void crt_sens_gram_kernel(
   }
 }
 template __global__ void crt_sens_gram_kernel<3>(SensArgs, const uint2*, const int*, int);
+template __global__ void crt_sens_gram_kernel<5>(SensArgs, const uint2*, const int*, int);
 template __global__ void crt_sens_gram_kernel<6>(SensArgs, const uint2*, const int*, int);
 template __global__ void crt_sens_gram_kernel<9>(SensArgs, const uint2*, const int*, int);
 
