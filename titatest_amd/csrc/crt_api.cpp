@@ -1748,7 +1748,7 @@ static int run_sens(crt_handle* h, int32_t mode, const double2* sigma) {
     const size_t lds = crt_gram_lds_bytes(h->E);
     if (lds > dyn) return kErrArg;
     tstart(h, "sens");
-    hipLaunchKernelGGL(k, dim3(h->elanz), dim3(kGramNT), lds, h->stream, S, h->at<uint2>("sens_goff"),
+    hipLaunchKernelGGL(k, dim3((h->elanz + 7) / 8 * 8), dim3(kGramNT), lds, h->stream, S, h->at<uint2>("sens_goff"),
                        h->at<int>("sens_gtiles"), gtiles);
     tstop(h);
     return finish(h);

@@ -784,7 +784,12 @@ __global__ __launch_bounds__(kGramNT, MT > 6 ? 2 : 3) void crt_sens_gram_kernel(
                                                                            int nt) {
   constexpr int NT = kGramNT, NW = NT / 64;
   extern __shared__ double2 sm[];
-  const int e = blockIdx.x;
+  // XCD-major element order: the dispatcher deals blocks to the 8 XCDs in
+  // turn, so block b takes element (b mod 8) grid / 8 + b / 8 and each XCD
+  // works through one contiguous element range, whose elements share nodes
+  // and read them through the same L2 (grid: elanz rounded up to 8)
+  const int e = (blockIdx.x % 8) * (gridDim.x / 8) + blockIdx.x / 8;
+  if (e >= S.elanz) return;
   const int E = S.eanz, TB = (E + 15) / 16, EP = 16 * TB, ntl = TB * (TB + 1) / 2;
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, col = lane & 15, kk = lane >> 4;
   const int nk = S.el_nk[e];
