@@ -1648,7 +1648,7 @@ static double2* transposed_kpot(crt_handle* h) {
   return kpt;
 }
 
-static int run_sens(crt_handle* h, int32_t mode, const double2* sigma) {
+static int run_sens(crt_handle* h, int32_t mode, const double2* sigma, bool real = false) {
   if (!h->kpot_full) return kErrOrder;  // the explicit Jacobian reads every wavenumber's potentials
   SensArgs S;
   S.n = h->n;
@@ -1739,10 +1739,15 @@ static int run_sens(crt_handle* h, int32_t mode, const double2* sigma) {
   const bool gram = gram_fits && (h->sens_form == 2 || (h->sens_form == 0 && gtiles * 256 <= 2 * h->nanz));
   if (gram) {
     // tiles per wavefront: ceil(gtiles / 4)
-    void (*k)(SensArgs, const uint2*, const int*, int) = gtiles <= 12   ? crt_sens_gram_kernel<3>
-                                                         : gtiles <= 20 ? crt_sens_gram_kernel<5>
-                                                         : gtiles <= 24 ? crt_sens_gram_kernel<6>
-                                                                        : crt_sens_gram_kernel<9>;
+    // (real potentials: the DC twin's, one product per tile)
+    void (*k)(SensArgs, const uint2*, const int*, int);
+    if (real)
+      k = gtiles <= 12 ? crt_sens_gram_kernel<3, true>
+          : gtiles <= 24 ? crt_sens_gram_kernel<6, true> : crt_sens_gram_kernel<9, true>;
+    else
+      k = gtiles <= 12 ? crt_sens_gram_kernel<3, false>
+          : gtiles <= 20 ? crt_sens_gram_kernel<5, false>
+          : gtiles <= 24 ? crt_sens_gram_kernel<6, false> : crt_sens_gram_kernel<9, false>;
     size_t dyn = 0;
     if (int rc = open_lds(h, reinterpret_cast<const void*>(k), &dyn)) return rc;
     const size_t lds = crt_gram_lds_bytes(h->E);
@@ -1808,7 +1813,7 @@ int crt_sensitivities(crt_handle* h, int32_t mode, double* sens) {
 int crt_sensitivities_dc(crt_handle* h, double* sensdc) {
   if (!h || !sensdc) return kErrArg;
   if (!h->volt_done || !h->dc) return kErrOrder;
-  int rc = run_sens(h, 1, h->at<double2>("sigma_dc"));
+  int rc = run_sens(h, 1, h->at<double2>("sigma_dc"), true);
   if (rc) return rc;
   // the complex J of this call is not the J the CG stage expects
   h->sens_done = false;

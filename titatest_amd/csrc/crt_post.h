@@ -108,7 +108,8 @@ __global__ void crt_sens_pairs_kernel(SensArgs S, SensPairs Q);
 constexpr int kGramMaxE = 128;
 constexpr int kGramNT = 256;
 constexpr int kGramQC = 4;  // measurements per thread per extraction pass
-template <int MT>
+// REAL: the potentials are real (the DC twin): one real product per tile
+template <int MT, bool REAL>
 __global__ void crt_sens_gram_kernel(SensArgs S, const uint2* off, const int* tiles, int nt);
 inline int crt_gram_tiles(int eanz) {
   const int tb = (eanz + 15) / 16;

@@ -777,7 +777,7 @@ template __global__ void crt_adj_gemm_cs_kernel<32>(const double2*, const double
 // offsets the host computed (a zero slot for absent electrodes).  MT: tiles
 // per wavefront, at least a quarter of the formed tiles (at most TB (TB + 1)
 // / 2 for TB = ceil(E / 16) tile rows, E <= 16 TB <= 128).
-template <int MT>
+template <int MT, bool REAL>
 __global__ __launch_bounds__(kGramNT, MT > 6 ? 2 : 3) void crt_sens_gram_kernel(SensArgs S,
                                                                            const uint2* __restrict__ off,
                                                                            const int* __restrict__ tiles,
@@ -813,7 +813,7 @@ __global__ __launch_bounds__(kGramNT, MT > 6 ? 2 : 3) void crt_sens_gram_kernel(
   // products (Pr Qr, Pi Qi, (Pr + Pi)(Qr + Qi)) in three accumulators,
   // which the registers hold at three workgroups per CU; more tiles the four
   // products of the plain form in two
-  constexpr bool kGauss = MT <= 5;
+  constexpr bool kGauss = MT <= 5 && !REAL;
   v4d gr[MT], gi[MT], gs[kGauss ? MT : 1];
 #pragma unroll
   for (int q = 0; q < MT; ++q) gr[q] = gi[q] = gs[kGauss ? q : 0] = v4d{0.0, 0.0, 0.0, 0.0};
@@ -854,7 +854,10 @@ __global__ __launch_bounds__(kGramNT, MT > 6 ? 2 : 3) void crt_sens_gram_kernel(
     for (int q = 0; q < MT; ++q) {
       if (q < nq) {
         const double2 a = buf[oa[q]], b = buf[ob[q]];
-        if constexpr (kGauss) {
+        if constexpr (REAL) {
+          // the DC twin's potentials are real: G is real
+          gr[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(a.x, b.x, gr[q], 0, 0, 0);
+        } else if constexpr (kGauss) {
           gr[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(a.x, b.x, gr[q], 0, 0, 0);
           gi[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(a.y, b.y, gi[q], 0, 0, 0);
           gs[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(a.x + a.y, b.x + b.y, gs[q], 0, 0, 0);
@@ -903,14 +906,16 @@ __global__ __launch_bounds__(kGramNT, MT > 6 ? 2 : 3) void crt_sens_gram_kernel(
     __syncthreads();
 #pragma unroll
     for (int j = 0; j < kGramQC; ++j) hr[j] = hsum(off[min(i0 + j * NT + tid, S.nanz - 1)]);
-    __syncthreads();
-    put(gi);
-    __syncthreads();
+    if constexpr (!REAL) {
+      __syncthreads();
+      put(gi);
+      __syncthreads();
+    }
 #pragma unroll
     for (int j = 0; j < kGramQC; ++j) {
       const int i = i0 + j * NT + tid;
       if (i >= S.nanz) continue;
-      const cplx acc = mkc(hr[j], hsum(off[i]));
+      const cplx acc = mkc(hr[j], REAL ? 0.0 : hsum(off[i]));
       cplx r;
       if (S.mode == 0) {
         cplx dum = (S.swrtr == 0) ? acc : rmul(1.0 / M_PI, acc);
@@ -924,10 +929,15 @@ __global__ __launch_bounds__(kGramNT, MT > 6 ? 2 : 3) void crt_sens_gram_kernel(
     __syncthreads();
   }
 }
-template __global__ void crt_sens_gram_kernel<3>(SensArgs, const uint2*, const int*, int);
-template __global__ void crt_sens_gram_kernel<5>(SensArgs, const uint2*, const int*, int);
-template __global__ void crt_sens_gram_kernel<6>(SensArgs, const uint2*, const int*, int);
-template __global__ void crt_sens_gram_kernel<9>(SensArgs, const uint2*, const int*, int);
+#define GRAM(MT, R) template __global__ void crt_sens_gram_kernel<MT, R>(SensArgs, const uint2*, const int*, int);
+GRAM(3, false)
+GRAM(5, false)
+GRAM(6, false)
+GRAM(9, false)
+GRAM(3, true)
+GRAM(6, true)
+GRAM(9, true)
+#undef GRAM
 
 __global__ __launch_bounds__(256) void crt_adj_gemm_kernel(const double2* __restrict__ kpot,
                                                          const double2* __restrict__ C, int n, int eanz,
